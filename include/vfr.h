@@ -1,0 +1,225 @@
+/*
+ * vfr.h -- C ABI of the MI355X-native VF-Ray renderer (libvfr.so).
+ *
+ * VF-Ray renders unstructured tetrahedral meshes by ray casting: visible
+ * external faces seed one ray per pixel, each ray walks tet-to-tet through the
+ * face adjacency and composites a partially pre-integrated segment per tet.
+ *
+ * Conventions
+ *   - Every function returns an int status: VFR_OK (0) or a negative VFR_E*
+ *     code; vfr_last_error() returns the text of the last failure on the
+ *     calling thread.
+ *   - Host buffers are caller-owned; device buffers are library-owned.
+ *   - Ids in and out of the library are the caller's ids (the order of the
+ *     arrays given to vfr_mesh_create); any internal reordering is hidden.
+ *   - One vfr_ctx may be used by one host thread at a time.
+ *
+ * Reference interface each group replaces (file:line in the VF-Ray sources):
+ *   mesh / connectivity : readOff+normalizeVertices+buildIncid+buildCon+
+ *                         buildExtF+buildLmt  (vfRayPreComp.h:97-279,
+ *                         vfRayPreComp.cc:29-145, 219-270, 424-501, 710-737,
+ *                         808-853)
+ *   transfer function   : readTF/buildTF (vfRayPreComp.cc:551-622)
+ *   psi table           : psiGammaTable512.h:10-20 (float[512][512])
+ *   render              : findVisibleFaces + host compaction +
+ *                         projectVisibleFaces + vfRay, driven by
+ *                         executeKernel1..3 (vfRayKernelCPU.h:61-80,
+ *                         main.cc:74-224, vfRay.cu:319-568)
+ */
+#ifndef VFR_H
+#define VFR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define VFR_OK 0
+#define VFR_EINVAL (-1)   /* bad argument (null pointer, size, id)          */
+#define VFR_ENOMEM (-2)   /* host or device allocation failed               */
+#define VFR_EHIP (-3)     /* HIP runtime error                              */
+#define VFR_EIO (-4)      /* file missing, unreadable or malformed          */
+#define VFR_ESTATE (-5)   /* object not in a state that allows the call     */
+#define VFR_EBUDGET (-6)  /* request exceeds the configured HBM budget      */
+#define VFR_ENODEV (-7)   /* no usable gfx950 device                        */
+#define VFR_EMESH (-8)    /* mesh violates an assumption (see message)      */
+
+const char* vfr_last_error(void);
+/* Library version string, e.g. "vfr 0.1 gfx950". */
+const char* vfr_version(void);
+
+/* --------------------------------------------------------------- context */
+typedef struct vfr_ctx vfr_ctx;
+typedef struct vfr_mesh vfr_mesh;
+typedef struct vfr_plan vfr_plan;
+typedef struct vfr_server vfr_server;
+
+/* Opens `device` (HIP ordinal), uploads the closed-form psi table. */
+int vfr_ctx_create(int device, vfr_ctx** out);
+void vfr_ctx_destroy(vfr_ctx* ctx);
+
+/* Replaces the context's 512x512 psi table (row = gamma_front index,
+ * column = gamma_back index, as psiGammaTable512.h:12-20).  NULL restores the
+ * table generated from the closed form. */
+int vfr_psi_load(vfr_ctx* ctx, const float* table512x512);
+/* Copies the context's psi table to host (512*512 floats). */
+int vfr_psi_read(vfr_ctx* ctx, float* out512x512);
+/* The closed-form table on host, no device needed (512*512 floats). */
+int vfr_psi_generate(float* out512x512);
+
+/* Registers a 256-entry RGBA transfer function (1024 floats, r g b a per
+ * entry, as the .tf file, vfRayPreComp.cc:551-579).  Returns its id. */
+int vfr_tf_create(vfr_ctx* ctx, const float rgba[1024], int* tf_id);
+
+/* ------------------------------------------------------------------ mesh */
+#define VFR_MESH_NORMALIZE 0x1u  /* apply normalizeVertices on upload      */
+#define VFR_MESH_MORTON 0x2u     /* reorder vertices/tets along a Z-curve  */
+
+typedef struct vfr_mesh_info {
+  uint32_t n_verts, n_tets, n_ext_faces;
+  float max_edge_length, max_z, min_z; /* the .lmt triple                 */
+  uint64_t device_bytes;               /* resident footprint              */
+  double build_ms;                     /* device preprocessing time       */
+} vfr_mesh_info;
+
+/* Uploads a mesh: xyzs = nV*(x,y,z,s) floats, tets = nT*4 vertex ids.
+ * con (nT*4, neighbour per face or the tet itself on external faces, the
+ * .con body) and extf (n_ext*2, (tet, face) pairs, the .extf body) may be
+ * given (read from cache files) or NULL, in which case they are built on the
+ * device by sort-and-match over packed face keys.  lmt (3 floats) may be
+ * given or NULL (built).  Flags: VFR_MESH_*. */
+int vfr_mesh_create(vfr_ctx* ctx, const float* xyzs, uint32_t n_verts,
+                    const uint32_t* tets, uint32_t n_tets,
+                    const uint32_t* con, const uint32_t* extf, uint32_t n_ext,
+                    const float* lmt, uint32_t flags, vfr_mesh** out);
+void vfr_mesh_destroy(vfr_mesh* mesh);
+int vfr_mesh_get_info(const vfr_mesh* mesh, vfr_mesh_info* out);
+/* Copies connectivity back in caller ids: con_out nT*4 (may be NULL),
+ * extf_out n_ext*2 (may be NULL), lmt_out 3 floats (may be NULL). */
+int vfr_mesh_connectivity(const vfr_mesh* mesh, uint32_t* con_out,
+                          uint32_t* extf_out, float* lmt_out);
+/* Copies the (possibly normalized) vertices back, caller order, nV*4. */
+int vfr_mesh_vertices(const vfr_mesh* mesh, float* xyzs_out);
+
+/* ---------------------------------------------------------------- render */
+#define VFR_PROJ_ORTHO 0u       /* X,Y,Z = A*(x,y,z)+b, rays march -Z      */
+#define VFR_PROJ_PERSPECTIVE 1u /* pinhole camera                          */
+
+/* Compositing modes.  REF reproduces vfRayKernelCPU.cc:448-482 (segment put
+ * in front of the accumulation, no early termination) and the reference's
+ * per-face pixel loop semantics (last visible face in order wins a pixel,
+ * a column stops at an entry scalar outside [0,1]).  FTB composites front to
+ * back from the eye with early termination and re-enters through concave
+ * boundaries. */
+#define VFR_MODE_REF 0u
+#define VFR_MODE_FTB 1u
+
+typedef struct vfr_view {
+  uint32_t width, height;
+  uint32_t projection;  /* VFR_PROJ_*                                      */
+  int32_t tf_id;        /* transfer function id                            */
+  float ortho[12];      /* row-major 3x4 affine for VFR_PROJ_ORTHO         */
+  float eye[3];         /* VFR_PROJ_PERSPECTIVE                            */
+  float target[3];
+  float up[3];
+  float fov_y_deg;
+} vfr_view;
+
+/* Schedules. */
+#define VFR_SCHED_PER_VIEW 0u  /* one K1/K2/K3 launch set per view, frame sync */
+#define VFR_SCHED_BATCHED 1u   /* all views in one persistent work-queue launch */
+
+typedef struct vfr_render_opts {
+  uint32_t mode;        /* VFR_MODE_*                                      */
+  uint32_t schedule;    /* VFR_SCHED_*                                     */
+  float term_alpha;     /* FTB early termination (0.99 default if <= 0)    */
+  uint32_t waves_per_cu;/* persistent kernel occupancy, 0 = tuned          */
+  uint32_t refill;      /* dead lanes that trigger a wave refill, 0 = tuned */
+  uint32_t use_graph;   /* capture the batched launch set in a hipGraph    */
+  uint32_t max_entries; /* FTB entry-list slots per pixel, 0 = 8           */
+  uint32_t reserved;
+} vfr_render_opts;
+
+typedef struct vfr_frame_out {
+  float* rgba;          /* W*H*4, may be NULL                              */
+  uint8_t* rgb8;        /* W*H*3 PPM body (row = pix.y), may be NULL       */
+  uint32_t* entry_ext;  /* W*H, ext-face index or 0xFFFFFFFF, may be NULL  */
+  uint32_t* entry_tet;  /* W*H, caller tet id or 0xFFFFFFFF, may be NULL   */
+  uint32_t* steps;      /* W*H, tets visited per ray, may be NULL          */
+} vfr_frame_out;
+
+typedef struct vfr_stats {
+  uint64_t rays;          /* sum over views of W*H                          */
+  uint64_t active_rays;   /* rays with an entry face                        */
+  uint64_t steps;         /* tets visited (sum over rays)                   */
+  uint64_t anomalies;     /* rays stopped with no exit face found           */
+  double ms_total;        /* HIP-event time of the whole render             */
+  double ms_visibility;   /* K1+K2 (visibility, projection, entry lists)    */
+  double ms_traversal;    /* K3                                             */
+  double ms_frame_p50;    /* per-view frame latency (PER_VIEW schedule)     */
+  double ms_frame_p99;
+} vfr_stats;
+
+/* A plan owns every device buffer a render needs; executing it allocates
+ * nothing.  Views are copied. */
+int vfr_plan_create(vfr_ctx* ctx, const vfr_mesh* mesh, const vfr_view* views,
+                    int n_views, const vfr_render_opts* opts, vfr_plan** out);
+int vfr_plan_execute(vfr_plan* plan, vfr_stats* stats);
+int vfr_plan_read(vfr_plan* plan, int view, vfr_frame_out* out);
+void vfr_plan_destroy(vfr_plan* plan);
+
+/* create + execute + read + destroy. outs has n_views entries. */
+int vfr_render(vfr_ctx* ctx, const vfr_mesh* mesh, const vfr_view* views,
+               int n_views, const vfr_render_opts* opts, vfr_frame_out* outs,
+               vfr_stats* stats);
+
+/* --------------------------------------------------------------- formats */
+/* Text formats of vfRayPreComp.cc, byte-identical output.  Readers allocate
+ * with malloc; free with vfr_free. */
+void vfr_free(void* p);
+int vfr_read_off(const char* path, float** xyzs, uint32_t* n_verts,
+                 uint32_t** tets, uint32_t* n_tets);           /* :29-80   */
+int vfr_write_off(const char* path, const float* xyzs, uint32_t n_verts,
+                  const uint32_t* tets, uint32_t n_tets);
+void vfr_normalize_vertices(float* xyzs, uint32_t n_verts);    /* :84-145  */
+int vfr_read_con(const char* path, uint32_t n_tets, uint32_t** con,
+                 uint32_t* n_ext);                             /* :375-419 */
+int vfr_write_con(const char* path, const uint32_t* con, uint32_t n_tets,
+                  uint32_t n_ext);                             /* :507-543 */
+int vfr_read_extf(const char* path, uint32_t n_ext, uint32_t** extf); /* :668-705 */
+int vfr_write_extf(const char* path, const uint32_t* extf, uint32_t n_ext); /* :743-774 */
+int vfr_read_tf(const char* path, float rgba_out[1024]);       /* :551-590 */
+int vfr_write_tf(const char* path, const float rgba[1024]);    /* :628-660 */
+void vfr_default_tf(float rgba_out[1024]);                     /* :595-622 */
+int vfr_read_lmt(const char* path, float lmt_out[3]);          /* :782-803 */
+int vfr_write_lmt(const char* path, const float lmt[3]);       /* :859-880 */
+void vfr_build_lmt(const float* xyzs, uint32_t n_verts, const uint32_t* tets,
+                   uint32_t n_tets, float lmt_out[3]);         /* :808-853 */
+/* buildIncid + writeIncid (vfRayPreComp.cc:219-332): the .incid cache the
+ * reference writes on its way to the .con file. */
+int vfr_write_incid(const char* path, const uint32_t* tets, uint32_t n_tets,
+                    uint32_t n_verts);
+int vfr_write_ppm(const char* path, const uint8_t* rgb, uint32_t w,
+                  uint32_t h);                                 /* :915-934 */
+
+/* ------------------------------------------------------------- synthetic */
+/* Seeded meshes of the benchmark recipes (hex grid of n^3 cells on [0,1]^3,
+ * 6 Freudenthal tets per cell, jitter, Gaussian scalar field).
+ * variant 0: plain grid; variant 1: concave (graded radial map + voids).
+ * The arrays are malloc'ed; free them with vfr_free. */
+int vfr_synth_grid(uint32_t n, uint64_t seed, uint32_t variant, float** xyzs,
+                   uint32_t* n_verts, uint32_t** tets, uint32_t* n_tets);
+/* 256-entry RGBA TF, piecewise linear through 6 seeded control points,
+ * quantized through the 6-significant-digit .tf text. */
+void vfr_synth_tf(uint64_t seed, float rgba_out[1024]);
+/* Rotation that maps unit direction d to -z, applied about (0.5,0.5,0.5):
+ * the reference renders along -z, so this bakes a view into the .off. */
+void vfr_synth_rotate(float* xyzs, uint32_t n_verts, const float d[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VFR_H */

@@ -1,0 +1,805 @@
+/*
+ * vfr_oracle.cpp -- CPU oracle for the VF-Ray hot path.  TEST INFRASTRUCTURE.
+ *
+ * Only the test suite, the smoke check and the benchmark's CPU-baseline leg
+ * load this library, and only as the checker: the product path (libvfr.so)
+ * never links, loads or calls it.
+ *
+ * What it restates (file:line in the VF-Ray sources):
+ *   orc_normalize        normalizeVertices        vfRayPreComp.cc:84-145
+ *   orc_connectivity     buildIncid + buildCon    vfRayPreComp.cc:219-270, 424-501
+ *   orc_extf             buildExtF                vfRayPreComp.cc:710-737
+ *   orc_lmt              buildLmt                 vfRayPreComp.cc:808-853
+ *   orc_render_ref       main.cc executeKernel1..3 (main.cc:74-224) over
+ *                        findVisibleFaces         vfRayKernelCPU.cc:51-132
+ *                        projectVisibleFaces      vfRayKernelCPU.cc:135-169
+ *                        vfRay                    vfRayKernelCPU.cc:172-566
+ *   orc_render_ftb       the renderer's own front-to-back mode (perspective,
+ *                        early termination, re-entry): no reference
+ *                        counterpart; it reuses the reference's exit-face test
+ *                        (vfRayKernelCPU.cc:312-350) and PPI segment formula
+ *                        (vfRayKernelCPU.cc:448-482) with front = entry point.
+ *   FP64 re-integration  (both render functions) the same tet sequence
+ *                        integrated in double with the exact psi integral.
+ *
+ * Arithmetic: strict IEEE binary32 in the reference's operation order, with
+ * the reference's double-promoted sub-expressions made explicit (the
+ * reference declares real = float, RDELTA as the double 1e-12,
+ * vfRayKernelCPU.h:28-33).  Build with -ffp-contract=off and without
+ * -ffast-math.
+ *
+ * Pinning (what checks each function; the tests hold the checks):
+ *   normalize/connectivity/extf/lmt : byte-equal against the compiled
+ *     reference (oracle/_ref) on synthetic meshes; connectivity also against
+ *     an O(T^2) brute force on tiny meshes.
+ *   orc_render_ref : per-pixel last-writer face ids and uchar image against
+ *     the compiled reference built with strict IEEE flags; closed-form
+ *     single-tet and constant-TF cases.
+ *   orc_render_ftb : entry faces against brute-force ray/triangle tests in
+ *     double; colours against the FP64 re-integration and closed forms.
+ *   FP64 re-integration : closed forms (constant TF, single tet), psi
+ *     integral against scipy quadrature.
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+static const double RDELTA = 1e-12; /* vfRayKernelCPU.h:28 */
+
+/* A walk longer than this stops as an anomaly (the reference has no bound; a
+ * ray cycling around a degenerate edge would never end). Same bound as the
+ * renderer. */
+static const u32 MAX_STEPS = 65536u;
+
+struct F4 { float x, y, z, w; };
+
+static inline F4 ld4(const float* v, u32 i) {
+  F4 r; r.x = v[i * 4 + 0]; r.y = v[i * 4 + 1]; r.z = v[i * 4 + 2]; r.w = v[i * 4 + 3];
+  return r;
+}
+
+/* ------------------------------------------------------------ preprocessing */
+
+/* normalizeVertices, vfRayPreComp.cc:84-145. */
+extern "C" void orc_normalize(float* v, u32 nV) {
+  float mn[4], mx[4], center[3];
+  for (int i = 0; i < 4; ++i) { mn[i] = v[i]; mx[i] = v[i]; }
+  for (u32 i = 1; i < nV; ++i)
+    for (int j = 0; j < 4; ++j) {
+      float value = v[i * 4 + j];
+      if (value < mn[j]) mn[j] = value;
+      if (value > mx[j]) mx[j] = value;
+    }
+  for (int i = 0; i < 3; ++i) {
+    center[i] = (float)((double)(mn[i] + mx[i]) / 2.0);
+    mx[i] -= center[i];
+    mn[i] -= center[i];
+  }
+  float maxCoord = (mx[1] > mx[2]) ? mx[1] : mx[2];
+  maxCoord = (mx[0] > maxCoord) ? mx[0] : maxCoord;
+  float scaleCoord = (float)(1.0 / (double)maxCoord);
+  float scaleScalar = (float)(1.0 / (double)(mx[3] - mn[3]));
+  for (u32 i = 0; i < nV; ++i) {
+    for (int j = 0; j < 3; ++j) v[i * 4 + j] = (v[i * 4 + j] - center[j]) * scaleCoord;
+    v[i * 4 + 3] = (v[i * 4 + 3] - mn[3]) * scaleScalar;
+  }
+}
+
+/* buildIncid's per-vertex ascending tet lists (vfRayPreComp.cc:231-266, a tet
+ * is listed once per vertex it contains) followed by buildCon's search
+ * (vfRayPreComp.cc:438-497): the neighbour across face f of tet i is the
+ * smallest tet id != i that appears in the incidence lists of all three face
+ * vertices MOD4(k,f), k=0..2; none -> i itself (external). Returns numExtFaces
+ * or -1 when a vertex has no incident tet (buildIncid's anomaly, :260-264). */
+extern "C" long orc_connectivity(const u32* tets, u32 nT, u32 nV, u32* con) {
+  std::vector<u32> cnt(nV + 1, 0);
+  for (u32 j = 0; j < nT; ++j) {
+    /* one entry per (vertex, tet): the reference breaks at the first slot k
+     * that matches vertex i, so a repeated vertex id lists the tet once */
+    for (int k = 0; k < 4; ++k) {
+      u32 vi = tets[j * 4 + k];
+      bool dup = false;
+      for (int q = 0; q < k; ++q) dup |= (tets[j * 4 + q] == vi);
+      if (!dup) cnt[vi + 1]++;
+    }
+  }
+  for (u32 i = 0; i < nV; ++i) {
+    if (cnt[i + 1] == 0) return -1;
+    cnt[i + 1] += cnt[i];
+  }
+  std::vector<u32> lst(cnt[nV]);
+  std::vector<u32> pos(cnt.begin(), cnt.end() - 1);
+  for (u32 j = 0; j < nT; ++j)
+    for (int k = 0; k < 4; ++k) {
+      u32 vi = tets[j * 4 + k];
+      bool dup = false;
+      for (int q = 0; q < k; ++q) dup |= (tets[j * 4 + q] == vi);
+      if (!dup) lst[pos[vi]++] = j;
+    }
+  long numExt = 0;
+  for (u32 i = 0; i < nT; ++i) {
+    for (u32 f = 0; f < 4; ++f) {
+      u32 cv[3];
+      for (u32 k = 0; k < 3; ++k) cv[k] = tets[i * 4 + ((k + f) % 4)];
+      con[i * 4 + f] = i;
+      const u32* l0 = &lst[cnt[cv[0]]]; u32 n0 = cnt[cv[0] + 1] - cnt[cv[0]];
+      const u32* l1 = &lst[cnt[cv[1]]]; u32 n1 = cnt[cv[1] + 1] - cnt[cv[1]];
+      const u32* l2 = &lst[cnt[cv[2]]]; u32 n2 = cnt[cv[2] + 1] - cnt[cv[2]];
+      for (u32 j = 0; j < n0; ++j) {
+        u32 cur = l0[j];
+        if (i == cur) continue;
+        bool found = false;
+        for (u32 k = 0; k < n1; ++k) {
+          if (l1[k] > cur) break;
+          if (l1[k] == cur) {
+            for (u32 l = 0; l < n2; ++l) {
+              if (l2[l] > cur) break;
+              if (l2[l] == cur) { found = true; break; }
+            }
+            if (found) break;
+          }
+        }
+        if (found) { con[i * 4 + f] = cur; break; }
+      }
+      if (con[i * 4 + f] == i) numExt++;
+    }
+  }
+  return numExt;
+}
+
+/* buildExtF, vfRayPreComp.cc:710-737. */
+extern "C" u32 orc_extf(const u32* con, u32 nT, u32* extf) {
+  u32 n = 0;
+  for (u32 i = 0; i < nT; ++i)
+    for (u32 f = 0; f < 4; ++f)
+      if (con[i * 4 + f] == i) {
+        if (extf) { extf[n * 2 + 0] = i; extf[n * 2 + 1] = f; }
+        n++;
+      }
+  return n;
+}
+
+/* buildLmt, vfRayPreComp.cc:808-853. */
+extern "C" void orc_lmt(const float* v, u32 nV, const u32* t, u32 nT, float out[3]) {
+  static const int E[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+  float maxEdge = 0.0f;
+  for (u32 i = 0; i < nT; ++i)
+    for (int j = 0; j < 6; ++j) {
+      float e[3];
+      for (int k = 0; k < 3; ++k)
+        e[k] = v[t[i * 4 + E[j][0]] * 4 + k] - v[t[i * 4 + E[j][1]] * 4 + k];
+      float len = sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+      if (len > maxEdge) maxEdge = len;
+    }
+  float maxZ = -1.0f, minZ = 1.0f;
+  for (u32 i = 0; i < nV; ++i) {
+    if (v[i * 4 + 2] > maxZ) maxZ = v[i * 4 + 2];
+    if (v[i * 4 + 2] < minZ) minZ = v[i * 4 + 2];
+  }
+  out[0] = maxEdge; out[1] = maxZ; out[2] = minZ;
+}
+
+/* ----------------------------------------------------------- shared maths */
+
+/* |x| < RDELTA with x promoted to double (vfRayKernelCPU.cc:85, 206, 323). */
+static inline bool tiny(float x) { return (double)fabsf(x) < RDELTA; }
+
+/* The Cramer solve of vfRayKernelCPU.cc:73-122 (same as :380-429): plane
+ * parameters p with [x y 1].p = target over the three face vertices. */
+struct Planes { float z[3], s[3]; };
+
+static inline void cramer(const F4& a, const F4& b, const F4& c, float tx, float ty, float tz,
+                          float det, float out[3]) {
+  float c0x = a.x, c0y = b.x, c0z = c.x; /* fmCol0 */
+  float c1x = a.y, c1y = b.y, c1z = c.y; /* fmCol1 */
+  float tmpx = ty - tz;
+  float tmpy = (ty * c1z) - (c1y * tz);
+  float tmpz = (c0y * tz) - (ty * c0z);
+  out[0] = ((tx * (c1y - c1z)) - (c1x * tmpx) + tmpy) / det;
+  out[1] = ((c0x * tmpx) - tx * (c0y - c0z) + tmpz) / det;
+  out[2] = (c0x * (-tmpy) - (c1x * tmpz) + tx * (c0y * c1z - c1y * c0z)) / det;
+}
+
+static inline Planes face_planes(const F4& a, const F4& b, const F4& c) {
+  float c0x = a.x, c0y = b.x, c0z = c.x;
+  float c1x = a.y, c1y = b.y, c1z = c.y;
+  float det = c0x * (c1y - c1z) - c1x * (c0y - c0z) + ((c0y * c1z) - (c1y * c0z));
+  det = tiny(det) ? (float)RDELTA : det;
+  Planes p;
+  cramer(a, b, c, a.z, b.z, c.z, det, p.z);
+  cramer(a, b, c, a.w, b.w, c.w, det, p.s);
+  return p;
+}
+
+static inline float plane_eval(const float p[3], float ox, float oy) {
+  return ox * p[0] + oy * p[1] + p[2];
+}
+
+/* obj = (pix*2)/(real)(res-1) - 1.0, vfRayKernelCPU.cc:250, 254. */
+static inline float pix_to_obj(u32 pix, u32 res) {
+  float q = (float)(pix * 2) / (float)(res - 1);
+  return (float)((double)q - 1.0);
+}
+
+/* The exit-face test of vfRayKernelCPU.cc:312-350 over the rotated vertex
+ * array Q (Q[k] = tet slot (entry+k)&3); candidate i = 1..3 is face
+ * (entry+i)&3 with vertices Q[i], Q[i+1], Q[i+2] (mod 4).  Returns i or 0.
+ * Also returns the barycentric pieces of the passing face. */
+struct ExitHit { int i; float cy, cz, D; bool swapped; };
+
+static inline bool tri_test(const F4& a, const F4& b, const F4& c, float ox, float oy,
+                            float slack_is_rdelta, ExitHit* h) {
+  (void)slack_is_rdelta;
+  float v1x = b.x - a.x, v1y = b.y - a.y;
+  float v2x = c.x - a.x, v2y = c.y - a.y;
+  float cx = (v1x * v2y) - (v1y * v2x);
+  if (tiny(cx)) return false;
+  bool sw = false;
+  if (cx < 0.0f) {
+    float tx = v1x, ty = v1y;
+    v1x = v2x; v1y = v2y; v2x = tx; v2y = ty;
+    cx = -cx; sw = true;
+  }
+  float v0x = ox - a.x, v0y = oy - a.y;
+  float cy = (v1x * v0y) - (v1y * v0x);
+  float cz = (v0x * v2y) - (v0y * v2x);
+  if ((cy >= 0.0f) && (cz >= 0.0f) && ((double)(cy + cz) <= (double)cx + RDELTA)) {
+    h->cy = cy; h->cz = cz; h->D = cx; h->swapped = sw;
+    return true;
+  }
+  return false;
+}
+
+static inline ExitHit find_exit(const F4 Q[4], float ox, float oy) {
+  ExitHit h; h.i = 0; h.cy = h.cz = h.D = 0.0f; h.swapped = false;
+  for (int i = 1; i < 4; ++i) {
+    if (tri_test(Q[i], Q[(i + 1) & 3], Q[(i + 2) & 3], ox, oy, 1.0f, &h)) { h.i = i; return h; }
+  }
+  return h;
+}
+
+/* Barycentric interpolation over the triangle (a,b,c) from the inside-test
+ * crosses: with e1 = b-a, e2 = c-a (swapped when their cross is negative),
+ * cy = e1 x p, cz = p x e2, D = e1 x e2 > 0.  Weight of b is cz/D unswapped,
+ * cy/D swapped; weight of c the other one. */
+static inline float bary(float A, float B, float C, const ExitHit& h) {
+  float wb = h.swapped ? h.cy : h.cz;
+  float wc = h.swapped ? h.cz : h.cy;
+  return A + ((wb * (B - A) + wc * (C - A)) / h.D);
+}
+
+/* ------------------------------------------------------------- psi (FP64) */
+
+/* Psi(a, b) = int_0^1 exp(-a s - (b - a) s^2 / 2) ds with a = tau_front,
+ * b = tau_back (the integral psiGammaTable512.h tabulates in gamma = tau/(1+tau)).
+ * Composite 8-point Gauss-Legendre, panels scaled to the decay rate. */
+static const double GLX[8] = {-0.9602898564975363, -0.7966664774136267, -0.5255324099163290,
+                              -0.1834346424956498, 0.1834346424956498, 0.5255324099163290,
+                              0.7966664774136267, 0.9602898564975363};
+static const double GLW[8] = {0.1012285362903763, 0.2223810344533745, 0.3137066458778873,
+                              0.3626837833783620, 0.3626837833783620, 0.3137066458778873,
+                              0.2223810344533745, 0.1012285362903763};
+
+extern "C" double orc_psi_exact(double a, double b) {
+  double m = a > b ? a : b;
+  int panels = 1 + (int)(m * 0.5);
+  if (panels > 4096) panels = 4096;
+  double h = 1.0 / panels, sum = 0.0;
+  for (int p = 0; p < panels; ++p) {
+    double lo = p * h;
+    for (int k = 0; k < 8; ++k) {
+      double s = lo + 0.5 * h * (GLX[k] + 1.0);
+      sum += GLW[k] * exp(-a * s - (b - a) * s * s * 0.5);
+    }
+  }
+  return sum * 0.5 * h;
+}
+
+/* ---------------------------------------------------------- REF rendering */
+
+struct RefOut {
+  uint8_t* rgb8; float* rgba; double* rgba64; u32* entry_ext; u32* steps;
+};
+
+/* One visible face of the reference's K3 (vfRayKernelCPU.cc:172-566), with
+ * the face-plane memo (:234-245, :369-435) omitted: it is keyed by (tet, exit
+ * face) and returns the values the same Cramer solve produces.  Returns 1 if
+ * the face ended with the reference's face-level `return` (:365). */
+static int ref_face(const float* V, const u32* T, const u32* con, const float* tf,
+                    const float* psi, const u32* extf, u32 extIdx, const Planes& cf,
+                    const u32 bb[4], u32 W, u32 H, float maxEdge, RefOut* o) {
+  u32 t0 = extf[extIdx * 2 + 0], f0 = extf[extIdx * 2 + 1];
+  F4 fv0 = ld4(V, T[t0 * 4 + ((f0 + 0) & 3)]);
+  F4 fv1 = ld4(V, T[t0 * 4 + ((f0 + 1) & 3)]);
+  F4 fv2 = ld4(V, T[t0 * 4 + ((f0 + 2) & 3)]);
+  float v01x = fv1.x - fv0.x, v01y = fv1.y - fv0.y;
+  float v02x = fv2.x - fv0.x, v02y = fv2.y - fv0.y;
+  float cx = (v01x * v02y) - (v01y * v02x);
+  cx = tiny(cx) ? (float)RDELTA : cx;
+  if (cx < 0.0f) {
+    float tx = v01x, ty = v01y;
+    v01x = v02x; v01y = v02y; v02x = tx; v02y = ty;
+    cx = -cx;
+  }
+  for (u32 px = bb[0]; px <= bb[2]; ++px) {
+    float ox = pix_to_obj(px, W);
+    for (u32 py = bb[1]; py <= bb[3]; ++py) {
+      float oy = pix_to_obj(py, H);
+      float v0px = ox - fv0.x, v0py = oy - fv0.y;
+      float cy = (v01x * v0py) - (v01y * v0px);
+      float cz = (v0px * v02y) - (v0py * v02x);
+      if (!((cy >= 0.0f) && (cz >= 0.0f) && ((cy + cz) <= cx))) continue;
+      float zPrev = plane_eval(cf.z, ox, oy);
+      float sPrev = plane_eval(cf.s, ox, oy);
+      if ((sPrev < 0.0f) || (sPrev > 1.0f)) break; /* :274 leaves the column */
+      F4 cB = ld4(tf, (u32)(255.0f * sPrev));
+      float ax = 0.0f, ay = 0.0f, az = 0.0f, Tr = 1.0f;
+      double T64 = 1.0;
+      double dB[4] = {cB.x, cB.y, cB.z, cB.w};
+      double A64[3] = {0, 0, 0};
+      u32 tcur = t0, fent = f0, steps = 0;
+      while (1) {
+        if (++steps > MAX_STEPS) break;
+        F4 fv[4];
+        for (int k = 0; k < 4; ++k) fv[k] = ld4(V, T[tcur * 4 + k]);
+        F4 Q[4];
+        for (int k = 0; k < 4; ++k) Q[k] = fv[(fent + k) & 3];
+        ExitHit h = find_exit(Q, ox, oy);
+        if (h.i == 0) break; /* :353 */
+        u32 fex = (fent + h.i) & 3;
+        u32 tnext = con[tcur * 4 + fex];
+        if (tnext != tcur) {
+          u32 k;
+          for (k = 0; k < 4; ++k) if (con[tnext * 4 + k] == tcur) break;
+          if (k == 4) return 1; /* :365 */
+          fent = k;
+        }
+        Planes nf = face_planes(fv[(fex + 0) & 3], fv[(fex + 1) & 3], fv[(fex + 2) & 3]);
+        float zNext = plane_eval(nf.z, ox, oy);
+        float sNext = plane_eval(nf.s, ox, oy);
+        float dz = fabsf(zPrev - zNext);
+        dz = ((double)dz < RDELTA) ? 0.0f : dz;
+        if ((sNext < 0.0f) || (sNext > 1.0f)) break; /* :446 */
+        dz /= maxEdge;
+        F4 cF = ld4(tf, (u32)(255.0f * sNext));
+        float taux = cF.w * dz, tauy = cB.w * dz;
+        float zeta = (float)exp(-((double)taux * 0.5 + (double)tauy * 0.5));
+        float gx = (float)((double)taux / (1.0 + (double)taux));
+        float gy = (float)((double)tauy / (1.0 + (double)tauy));
+        float tcx = (float)((double)gx + (0.5 / 512));
+        float tcy = (float)((double)gy + (0.5 / 512));
+        if ((tcx < 0.0f) || (tcx > 1.0f) || (tcy < 0.0f) || (tcy > 1.0f)) break; /* :468 */
+        u32 ix = (u32)(tcx * 512.0f), iy = (u32)(tcy * 512.0f);
+        if (ix > 511) ix = 511; /* texCoord == 1.0 would index row 512 */
+        if (iy > 511) iy = 511;
+        float ps = psi[ix * 512 + iy];
+        float d = ps - zeta; /* float - float, :473 */
+        float fx = (float)((double)cF.x * (1.0 - (double)ps) + (double)(cB.x * d));
+        float fy = (float)((double)cF.y * (1.0 - (double)ps) + (double)(cB.y * d));
+        float fz = (float)((double)cF.z * (1.0 - (double)ps) + (double)(cB.z * d));
+        float fw = (float)(1.0 - (double)zeta);
+        ax = (float)((double)fx + (1.0 - (double)fw) * (double)ax);
+        ay = (float)((double)fy + (1.0 - (double)fw) * (double)ay);
+        az = (float)((double)fz + (1.0 - (double)fw) * (double)az);
+        Tr *= zeta;
+        if (o->rgba64) {
+          /* FP64 re-integration of the same segment: plane values in double
+           * from the same face vertices, exact psi. */
+          double dzz = fabs((double)zPrev - (double)zNext) / (double)maxEdge;
+          double tf64 = (double)cF.w * dzz, tb64 = dB[3] * dzz;
+          double z64 = exp(-(tf64 + tb64) * 0.5);
+          double p64 = orc_psi_exact(tf64, tb64);
+          double F[3] = {cF.x * (1.0 - p64) + dB[0] * (p64 - z64),
+                         cF.y * (1.0 - p64) + dB[1] * (p64 - z64),
+                         cF.z * (1.0 - p64) + dB[2] * (p64 - z64)};
+          for (int c = 0; c < 3; ++c) A64[c] = F[c] + z64 * A64[c];
+          T64 *= z64;
+          dB[0] = cF.x; dB[1] = cF.y; dB[2] = cF.z; dB[3] = cF.w;
+        }
+        cB = cF;
+        if (tnext == tcur) break; /* :535 */
+        tcur = tnext;
+        zPrev = zNext;
+        sPrev = sNext;
+      }
+      if (ax > 1.0f) ax = 1.0f;
+      if (ay > 1.0f) ay = 1.0f;
+      if (az > 1.0f) az = 1.0f;
+      u32 p = py * W + px;
+      if (o->rgb8) {
+        o->rgb8[p * 3 + 0] = (uint8_t)(int)(255.0f * ax);
+        o->rgb8[p * 3 + 1] = (uint8_t)(int)(255.0f * ay);
+        o->rgb8[p * 3 + 2] = (uint8_t)(int)(255.0f * az);
+      }
+      if (o->rgba) {
+        o->rgba[p * 4 + 0] = ax; o->rgba[p * 4 + 1] = ay; o->rgba[p * 4 + 2] = az;
+        o->rgba[p * 4 + 3] = 1.0f - Tr;
+      }
+      if (o->rgba64) {
+        for (int c = 0; c < 3; ++c) o->rgba64[p * 4 + c] = A64[c] > 1.0 ? 1.0 : A64[c];
+        o->rgba64[p * 4 + 3] = 1.0 - T64;
+      }
+      if (o->entry_ext) o->entry_ext[p] = extIdx;
+      if (o->steps) o->steps[p] = steps > MAX_STEPS ? MAX_STEPS : steps;
+    }
+  }
+  return 0;
+}
+
+/* The reference's projectVisibleFaces bbox (vfRayKernelCPU.cc:141-167):
+ * img = (uint)((res - 1.0) * (x + 1.0) / 2.0) in double. */
+static inline u32 img_coord(float x, u32 res) {
+  return (u32)(((double)res - 1.0) * ((double)x + 1.0) / 2.0);
+}
+
+/* main.cc:74-224 (executeKernel1, compaction with the duplicated last face,
+ * executeKernel2, executeKernel3 in visible-face order).  V must be
+ * normalized.  Returns the number of faces that ended with the face-level
+ * `return` (vfRayKernelCPU.cc:365), or -1 on bad input.  out_nvis receives
+ * numVisFaces including the duplicate. */
+extern "C" long orc_render_ref(const float* V, u32 nV, const u32* T, u32 nT, const u32* con,
+                               const u32* extf, u32 nExt, const float* tf, const float* psi,
+                               const float* lmt, u32 W, u32 H, uint8_t* rgb8, float* rgba,
+                               double* rgba64, u32* entry_ext, u32* steps, u32* out_nvis) {
+  (void)nV; (void)nT;
+  if (W < 2 || H < 2) return -1;
+  std::vector<Planes> pl(nExt);
+  std::vector<u32> vis;
+  for (u32 i = 0; i < nExt; ++i) {
+    u32 t = extf[i * 2], f = extf[i * 2 + 1];
+    F4 fv0 = ld4(V, T[t * 4 + ((f + 0) & 3)]);
+    F4 fv1 = ld4(V, T[t * 4 + ((f + 1) & 3)]);
+    F4 fv2 = ld4(V, T[t * 4 + ((f + 2) & 3)]);
+    F4 op = ld4(V, T[t * 4 + ((f + 3) & 3)]);
+    pl[i] = face_planes(fv0, fv1, fv2);
+    float faceZ = op.x * pl[i].z[0] + op.y * pl[i].z[1] + pl[i].z[2];
+    if (faceZ < op.z) pl[i].s[0] = -1.0f;            /* vfRayKernelCPU.cc:128 */
+    if (pl[i].s[0] != -1.0f) vis.push_back(i);        /* main.cc:97, 112 */
+  }
+  if (vis.empty()) { if (out_nvis) *out_nvis = 0; return 0; }
+  vis.push_back(vis.back());                          /* main.cc:124-126 */
+  if (out_nvis) *out_nvis = (u32)vis.size();
+  size_t np = (size_t)W * H;
+  if (rgb8) memset(rgb8, 0, np * 3);
+  if (rgba) memset(rgba, 0, np * 4 * sizeof(float));
+  if (rgba64) memset(rgba64, 0, np * 4 * sizeof(double));
+  if (entry_ext) for (size_t p = 0; p < np; ++p) entry_ext[p] = 0xFFFFFFFFu;
+  if (steps) memset(steps, 0, np * sizeof(u32));
+  RefOut o = {rgb8, rgba, rgba64, entry_ext, steps};
+  long returns = 0;
+  for (size_t k = 0; k < vis.size(); ++k) {
+    u32 i = vis[k];
+    u32 t = extf[i * 2], f = extf[i * 2 + 1];
+    u32 bb[4] = {W - 1, H - 1, 0, 0};
+    for (u32 j = 0; j < 3; ++j) {
+      F4 fv = ld4(V, T[t * 4 + ((f + j) & 3)]);
+      u32 ix = img_coord(fv.x, W), iy = img_coord(fv.y, H);
+      if (ix < bb[0]) bb[0] = ix;
+      if (iy < bb[1]) bb[1] = iy;
+      if (ix > bb[2]) bb[2] = ix;
+      if (iy > bb[3]) bb[3] = iy;
+    }
+    returns += ref_face(V, T, con, tf, psi, extf, i, pl[i], bb, W, H, lmt[0], &o);
+  }
+  return returns;
+}
+
+/* ---------------------------------------------------------- FTB rendering */
+
+/* View in the renderer's projective space P = (X, Y, Z, S):
+ *   ortho:       X,Y,Z = rows of M applied to (x,y,z,1); S = s.
+ *   perspective: xc,yc,dc = rows of M applied to (x,y,z,1);
+ *                Z = 1/dc, X = xc*Z, Y = yc*Z, S = s*Z.
+ * Planes map to planes, so rays through pixels are the lines X = ox, Y = oy
+ * and march towards decreasing Z, as the reference's rays march -z.
+ * Segment length in world units: perspective |1/Z1 - 1/Z0| *
+ * sqrt(1 + (ox^2 + oy^2) k^2) with k = tan(fov/2); ortho |Z1 - Z0| * k with
+ * k = world length of a unit step along -Z.  k is passed as `view_k`. */
+struct OView { u32 proj; float M[12]; float k; };
+
+static inline F4 xform(const OView& v, const F4& p) {
+  const float* M = v.M;
+  float a = ((M[0] * p.x + M[1] * p.y) + M[2] * p.z) + M[3];
+  float b = ((M[4] * p.x + M[5] * p.y) + M[6] * p.z) + M[7];
+  float c = ((M[8] * p.x + M[9] * p.y) + M[10] * p.z) + M[11];
+  F4 r;
+  if (v.proj == 0) { r.x = a; r.y = b; r.z = c; r.w = p.w; }
+  else { r.z = 1.0f / c; r.x = a * r.z; r.y = b * r.z; r.w = p.w * r.z; }
+  return r;
+}
+
+struct D4 { double x, y, z, w; };
+static inline D4 xform64(const OView& v, const F4& p) {
+  const float* M = v.M;
+  double a = (double)M[0] * p.x + (double)M[1] * p.y + (double)M[2] * p.z + (double)M[3];
+  double b = (double)M[4] * p.x + (double)M[5] * p.y + (double)M[6] * p.z + (double)M[7];
+  double c = (double)M[8] * p.x + (double)M[9] * p.y + (double)M[10] * p.z + (double)M[11];
+  D4 r;
+  if (v.proj == 0) { r.x = a; r.y = b; r.z = c; r.w = p.w; }
+  else { r.x = a / c; r.y = b / c; r.z = 1.0 / c; r.w = p.w * r.z; }
+  return r;
+}
+
+/* Order-preserving map of a float to u32 (larger float -> larger key). */
+static inline u32 fkey(float f) {
+  u32 u; memcpy(&u, &f, 4);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+static inline float clamp01(float s) { return s < 0.0f ? 0.0f : (s > 1.0f ? 1.0f : s); }
+
+/* Barycentric value of a triangle at (ox,oy) in double. */
+static inline double bary64(const D4& a, const D4& b, const D4& c, double ox, double oy,
+                            double D4::*m) {
+  double e1x = b.x - a.x, e1y = b.y - a.y, e2x = c.x - a.x, e2y = c.y - a.y;
+  double px = ox - a.x, py = oy - a.y;
+  double D = e1x * e2y - e1y * e2x;
+  double u = (px * e2y - py * e2x) / D;   /* weight of b */
+  double w = (e1x * py - e1y * px) / D;   /* weight of c */
+  return a.*m + u * (b.*m - a.*m) + w * (c.*m - a.*m);
+}
+
+/* exp(x) for x <= 0 in binary32: Cody-Waite reduction x = n ln2 + r with a
+ * two-part ln2, degree-7 Taylor polynomial on |r| <= ln2/2, scaled by 2^n.
+ * The renderer's kernels evaluate the same operations, so the two agree bit
+ * for bit; accuracy against libm is checked by the tests. */
+extern "C" float orc_expf(float x) {
+  if (x < -87.0f) return 0.0f;
+  float n = rintf(x * 1.44269504f);
+  float r = (x - n * 0.693145751953125f) - n * 1.42860682030941723e-6f;
+  float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.166666672f + r * (0.0416666679f +
+            r * (0.00833333377f + r * (0.00138888892f + r * 0.000198412701f))))));
+  return ldexpf(p, (int)n);
+}
+
+static inline u32 psi_index(float g) {
+  u32 i = (u32)(g * 512.0f + 0.5f);
+  return i > 511 ? 511 : i;
+}
+
+/* Front-to-back render of one view.  Entry candidates are every visible
+ * external face whose projection holds the pixel (the reference's inclusive
+ * inside test), keyed by (Z at the pixel, ext index) and walked front to
+ * back; after an exit at depth Z_exit the walk resumes at the first entry with
+ * Z < Z_exit.  Outputs rgba (W*H*4), rgba64 (FP64 re-integration, may be
+ * NULL), entry_ext (front-most entry), steps.  Returns the number of
+ * anomalous rays (no exit face found), or -1 on bad input. */
+extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, const u32* con,
+                               const u32* extf, u32 nExt, const float* tf, const float* psi,
+                               const float* lmt, u32 proj, const float* M, float view_k,
+                               u32 W, u32 H, float term, float* rgba, double* rgba64,
+                               u32* entry_ext, u32* steps_out, u32* n_entries_out) {
+  (void)nT;
+  if (W < 2 || H < 2) return -1;
+  OView view; view.proj = proj; memcpy(view.M, M, sizeof(view.M)); view.k = view_k;
+  std::vector<F4> P(nV);
+  for (u32 i = 0; i < nV; ++i) P[i] = xform(view, ld4(V, i));
+  /* visibility: the reference's K1 test in P space */
+  std::vector<u32> vis;
+  for (u32 i = 0; i < nExt; ++i) {
+    u32 t = extf[i * 2], f = extf[i * 2 + 1];
+    const F4& a = P[T[t * 4 + ((f + 0) & 3)]];
+    const F4& b = P[T[t * 4 + ((f + 1) & 3)]];
+    const F4& c = P[T[t * 4 + ((f + 2) & 3)]];
+    const F4& op = P[T[t * 4 + ((f + 3) & 3)]];
+    Planes pl = face_planes(a, b, c);
+    float faceZ = op.x * pl.z[0] + op.y * pl.z[1] + pl.z[2];
+    if (!(faceZ < op.z)) vis.push_back(i);
+  }
+  size_t np = (size_t)W * H;
+  std::vector<u32> cnt(np + 1, 0);
+  std::vector<u64> keys;
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<u32> fill;
+    if (pass == 1) {
+      for (size_t p = 0; p < np; ++p) cnt[p + 1] += cnt[p];
+      keys.assign(cnt[np], 0);
+      fill.assign(cnt.begin(), cnt.end() - 1);
+    }
+    for (u32 i : vis) {
+      u32 t = extf[i * 2], f = extf[i * 2 + 1];
+      const F4& a = P[T[t * 4 + ((f + 0) & 3)]];
+      const F4& b = P[T[t * 4 + ((f + 1) & 3)]];
+      const F4& c = P[T[t * 4 + ((f + 2) & 3)]];
+      double lo[2] = {1e30, 1e30}, hi[2] = {-1e30, -1e30};
+      const F4* vv[3] = {&a, &b, &c};
+      for (int j = 0; j < 3; ++j) {
+        double fx = ((double)W - 1.0) * ((double)vv[j]->x + 1.0) / 2.0;
+        double fy = ((double)H - 1.0) * ((double)vv[j]->y + 1.0) / 2.0;
+        lo[0] = std::min(lo[0], fx); hi[0] = std::max(hi[0], fx);
+        lo[1] = std::min(lo[1], fy); hi[1] = std::max(hi[1], fy);
+      }
+      if (hi[0] < 0.0 || hi[1] < 0.0 || lo[0] > W - 1.0 || lo[1] > H - 1.0) continue;
+      u32 bx0 = lo[0] < 0.0 ? 0 : (u32)lo[0], by0 = lo[1] < 0.0 ? 0 : (u32)lo[1];
+      u32 bx1 = hi[0] > W - 1.0 ? W - 1 : (u32)hi[0], by1 = hi[1] > H - 1.0 ? H - 1 : (u32)hi[1];
+      float v01x = b.x - a.x, v01y = b.y - a.y, v02x = c.x - a.x, v02y = c.y - a.y;
+      float cx = (v01x * v02y) - (v01y * v02x);
+      cx = tiny(cx) ? (float)RDELTA : cx;
+      bool sw = false;
+      if (cx < 0.0f) {
+        float tx = v01x, ty = v01y;
+        v01x = v02x; v01y = v02y; v02x = tx; v02y = ty;
+        cx = -cx; sw = true;
+      }
+      for (u32 px = bx0; px <= bx1; ++px) {
+        float ox = pix_to_obj(px, W);
+        for (u32 py = by0; py <= by1; ++py) {
+          float oy = pix_to_obj(py, H);
+          float v0px = ox - a.x, v0py = oy - a.y;
+          float cy = (v01x * v0py) - (v01y * v0px);
+          float cz = (v0px * v02y) - (v0py * v02x);
+          if (!((cy >= 0.0f) && (cz >= 0.0f) && ((cy + cz) <= cx))) continue;
+          size_t p = (size_t)py * W + px;
+          if (pass == 0) { cnt[p + 1]++; continue; }
+          ExitHit h; h.cy = cy; h.cz = cz; h.D = cx; h.swapped = sw; h.i = 0;
+          float Zp = bary(a.z, b.z, c.z, h);
+          keys[fill[p]++] = ((u64)fkey(Zp) << 32) | i;
+        }
+      }
+    }
+  }
+  const float maxEdge = lmt[0];
+  const float inv_me = 1.0f / maxEdge;
+  long anomalies = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : anomalies)
+  for (long pl = 0; pl < (long)np; ++pl) {
+    size_t p = (size_t)pl;
+    u32 px = (u32)(p % W), py = (u32)(p / W);
+    u64* K = keys.data() + cnt[p];
+    u32 nk = cnt[p + 1] - cnt[p];
+    std::sort(K, K + nk, [](u64 x, u64 y) { return x > y; });
+    float out[4] = {0, 0, 0, 0};
+    double out64[4] = {0, 0, 0, 0};
+    u32 steps = 0;
+    if (entry_ext) entry_ext[p] = nk ? (u32)(K[0] & 0xFFFFFFFFu) : 0xFFFFFFFFu;
+    if (n_entries_out) n_entries_out[p] = nk;
+    if (nk) {
+      float ox = pix_to_obj(px, W), oy = pix_to_obj(py, H);
+      float rayf = 1.0f;
+      double rayf64 = 1.0;
+      if (proj == 1) {
+        rayf = sqrtf(1.0f + (ox * ox + oy * oy) * (view_k * view_k));
+        rayf64 = sqrt(1.0 + ((double)ox * ox + (double)oy * oy) * ((double)view_k * view_k));
+      }
+      float ar = 0, ag = 0, ab = 0, Tr = 1.0f;
+      double ar64 = 0, ag64 = 0, ab64 = 0, T64 = 1.0;
+      bool live32 = true, live64 = (rgba64 != nullptr);
+      u32 cur = 0;
+      while (cur < nk && (live32 || live64)) {
+        u32 ei = (u32)(K[cur] & 0xFFFFFFFFu);
+        u32 tcur = extf[ei * 2], fent = extf[ei * 2 + 1];
+        /* entry values */
+        const F4& a = P[T[tcur * 4 + ((fent + 0) & 3)]];
+        const F4& b = P[T[tcur * 4 + ((fent + 1) & 3)]];
+        const F4& c = P[T[tcur * 4 + ((fent + 2) & 3)]];
+        float v01x = b.x - a.x, v01y = b.y - a.y, v02x = c.x - a.x, v02y = c.y - a.y;
+        float cx = (v01x * v02y) - (v01y * v02x);
+        cx = tiny(cx) ? (float)RDELTA : cx;
+        bool sw = false;
+        if (cx < 0.0f) {
+          float tx = v01x, ty = v01y;
+          v01x = v02x; v01y = v02y; v02x = tx; v02y = ty;
+          cx = -cx; sw = true;
+        }
+        float v0px = ox - a.x, v0py = oy - a.y;
+        ExitHit he; he.i = 0; he.swapped = sw; he.D = cx;
+        he.cy = (v01x * v0py) - (v01y * v0px);
+        he.cz = (v0px * v02y) - (v0py * v02x);
+        float Zprev = bary(a.z, b.z, c.z, he);
+        float Sprev = bary(a.w, b.w, c.w, he);
+        float dprev = 0.0f, sprev;
+        if (proj == 1) { dprev = 1.0f / Zprev; sprev = clamp01(Sprev * dprev); }
+        else sprev = clamp01(Sprev);
+        F4 cP = ld4(tf, (u32)(255.0f * sprev));
+        /* FP64 entry */
+        D4 a64 = xform64(view, ld4(V, T[tcur * 4 + ((fent + 0) & 3)]));
+        D4 b64 = xform64(view, ld4(V, T[tcur * 4 + ((fent + 1) & 3)]));
+        D4 c64 = xform64(view, ld4(V, T[tcur * 4 + ((fent + 2) & 3)]));
+        double Zp64 = bary64(a64, b64, c64, ox, oy, &D4::z);
+        double Sp64 = bary64(a64, b64, c64, ox, oy, &D4::w);
+        double sp64 = proj == 1 ? Sp64 / Zp64 : Sp64;
+        sp64 = sp64 < 0 ? 0 : (sp64 > 1 ? 1 : sp64);
+        F4 cP64 = ld4(tf, (u32)(255.0 * sp64));
+        float exitZ = Zprev;
+        bool anomaly = false, anom32 = false;
+        u32 iter = 0;
+        while (1) {
+          if (live32 && ++steps > MAX_STEPS) { anomaly = anom32 = true; break; }
+          if (++iter > 4 * MAX_STEPS) { anomaly = true; break; }
+          F4 Q[4];
+          for (int k = 0; k < 4; ++k) Q[k] = P[T[tcur * 4 + ((fent + k) & 3)]];
+          ExitHit h = find_exit(Q, ox, oy);
+          if (h.i == 0) { anomaly = true; anom32 = live32; break; }
+          u32 fex = (fent + (u32)h.i) & 3;
+          const F4& A = Q[h.i];
+          const F4& B = Q[(h.i + 1) & 3];
+          const F4& C = Q[(h.i + 2) & 3];
+          float Znext = bary(A.z, B.z, C.z, h);
+          float Snext = bary(A.w, B.w, C.w, h);
+          if (live32) {
+            float snext, dist;
+            if (proj == 1) {
+              float dn = 1.0f / Znext;
+              snext = clamp01(Snext * dn);
+              dist = fabsf(dn - dprev) * rayf;
+              dprev = dn;
+            } else {
+              snext = clamp01(Snext);
+              dist = fabsf(Zprev - Znext) * view_k;
+            }
+            float dl = dist * inv_me;
+            F4 cN = ld4(tf, (u32)(255.0f * snext));
+            float tf_ = cP.w * dl, tb_ = cN.w * dl;
+            float zeta = orc_expf(-(tf_ + tb_) * 0.5f);
+            float gf = tf_ / (1.0f + tf_), gb = tb_ / (1.0f + tb_);
+            float ps = psi[psi_index(gf) * 512 + psi_index(gb)];
+            float w1 = 1.0f - ps, w2 = ps - zeta;
+            ar += Tr * (cP.x * w1 + cN.x * w2);
+            ag += Tr * (cP.y * w1 + cN.y * w2);
+            ab += Tr * (cP.z * w1 + cN.z * w2);
+            Tr *= zeta;
+            cP = cN;
+            if (1.0f - Tr >= term) live32 = false;
+          }
+          if (live64) {
+            u32 ia = T[tcur * 4 + ((fent + (u32)h.i) & 3)];
+            u32 ib = T[tcur * 4 + ((fent + (u32)h.i + 1) & 3)];
+            u32 ic = T[tcur * 4 + ((fent + (u32)h.i + 2) & 3)];
+            D4 A64 = xform64(view, ld4(V, ia)), B64 = xform64(view, ld4(V, ib)),
+               C64 = xform64(view, ld4(V, ic));
+            double Zn64 = bary64(A64, B64, C64, ox, oy, &D4::z);
+            double Sn64 = bary64(A64, B64, C64, ox, oy, &D4::w);
+            double sn64 = proj == 1 ? Sn64 / Zn64 : Sn64;
+            sn64 = sn64 < 0 ? 0 : (sn64 > 1 ? 1 : sn64);
+            double dist = proj == 1 ? fabs(1.0 / Zn64 - 1.0 / Zp64) * rayf64 : fabs(Zp64 - Zn64) * (double)view_k;
+            double dl = dist / (double)maxEdge;
+            F4 cN = ld4(tf, (u32)(255.0 * sn64));
+            double tf_ = cP64.w * dl, tb_ = cN.w * dl;
+            double zeta = exp(-(tf_ + tb_) * 0.5);
+            double ps = orc_psi_exact(tf_, tb_);
+            double w1 = 1.0 - ps, w2 = ps - zeta;
+            ar64 += T64 * (cP64.x * w1 + cN.x * w2);
+            ag64 += T64 * (cP64.y * w1 + cN.y * w2);
+            ab64 += T64 * (cP64.z * w1 + cN.z * w2);
+            T64 *= zeta;
+            cP64 = cN;
+            Zp64 = Zn64;
+            if (1.0 - T64 >= (double)term) live64 = false;
+          }
+          u32 nb = con[tcur * 4 + fex];
+          if (nb == tcur) { exitZ = Znext; break; }
+          u32 k;
+          for (k = 0; k < 4; ++k) if (con[nb * 4 + k] == tcur) break;
+          if (k == 4) { anomaly = true; anom32 = live32; break; }
+          tcur = nb; fent = k; Zprev = Znext;
+          if (!live32 && !live64) break;
+        }
+        if (anom32) anomalies++;
+        if (anomaly) break;
+        if (!live32 && !live64) break;
+        /* re-entry: first entry strictly behind the exit point */
+        ++cur;
+        while (cur < nk) {
+          float z; u32 zk = (u32)(K[cur] >> 32);
+          u32 u = (zk & 0x80000000u) ? (zk & 0x7FFFFFFFu) : ~zk;
+          memcpy(&z, &u, 4);
+          if (z < exitZ) break;
+          ++cur;
+        }
+      }
+      out[0] = ar; out[1] = ag; out[2] = ab; out[3] = 1.0f - Tr;
+      out64[0] = ar64; out64[1] = ag64; out64[2] = ab64; out64[3] = 1.0 - T64;
+    }
+    if (rgba) for (int c = 0; c < 4; ++c) rgba[p * 4 + c] = out[c];
+    if (rgba64) for (int c = 0; c < 4; ++c) rgba64[p * 4 + c] = out64[c];
+    if (steps_out) steps_out[p] = steps > MAX_STEPS ? MAX_STEPS : steps;
+  }
+  return anomalies;
+}

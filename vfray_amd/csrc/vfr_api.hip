@@ -1,0 +1,633 @@
+/*
+ * vfr_api.hip -- the C ABI of libvfr.so (include/vfr.h): contexts, device
+ * meshes, transfer functions, the psi table, render plans.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/vfr.h"
+#include "vfr_internal.h"
+
+#define CHECK_HIP(x)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) {                                                               \
+      vfr_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_));       \
+      return VFR_EHIP;                                                                    \
+    }                                                                                     \
+  } while (0)
+
+#define CHECK_RC(x)              \
+  do {                           \
+    int rc_ = (x);               \
+    if (rc_ != VFR_OK) return rc_; \
+  } while (0)
+
+static const uint32_t kTfSlots = 4096;
+
+struct vfr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  float* psi = nullptr;      /* [512*512] device */
+  float4* tf = nullptr;      /* [kTfSlots*256] device */
+  uint32_t n_tf = 0;
+  int n_cu = 0;
+};
+
+struct vfr_mesh {
+  vfr_ctx* ctx = nullptr;
+  DevMesh d;
+  std::vector<uint32_t> h_extf; /* caller (tet, face) pairs */
+  float bb_lo[3] = {0, 0, 0}, bb_hi[3] = {0, 0, 0};
+};
+
+extern "C" const char* vfr_version(void) { return "vfr 0.1 gfx950"; }
+
+/* ----------------------------------------------------------------- context */
+
+extern "C" int vfr_ctx_create(int device, vfr_ctx** out) {
+  if (!out) { vfr_set_error("vfr_ctx_create: null out"); return VFR_EINVAL; }
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) {
+    vfr_set_error("no HIP device %d (found %d)", device, n);
+    return VFR_ENODEV;
+  }
+  CHECK_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  CHECK_HIP(hipGetDeviceProperties(&prop, device));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    vfr_set_error("device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+    return VFR_ENODEV;
+  }
+  vfr_ctx* c = new (std::nothrow) vfr_ctx;
+  if (!c) { vfr_set_error("out of memory"); return VFR_ENOMEM; }
+  c->device = device;
+  c->n_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc((void**)&c->psi, sizeof(float) * 512 * 512) != hipSuccess ||
+      hipMalloc((void**)&c->tf, sizeof(float4) * 256 * kTfSlots) != hipSuccess) {
+    vfr_set_error("device allocation failed");
+    vfr_ctx_destroy(c);
+    return VFR_ENOMEM;
+  }
+  int rc = vfr_psi_load(c, nullptr);
+  if (rc != VFR_OK) { vfr_ctx_destroy(c); return rc; }
+  *out = c;
+  return VFR_OK;
+}
+
+extern "C" void vfr_ctx_destroy(vfr_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->psi) (void)hipFree(c->psi);
+  if (c->tf) (void)hipFree(c->tf);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static std::once_flag g_psi_once;
+static std::vector<float> g_psi;
+
+static const float* closed_form_psi() {
+  std::call_once(g_psi_once, [] {
+    g_psi.resize(512 * 512);
+    vfr_psi_generate(g_psi.data());
+  });
+  return g_psi.data();
+}
+
+extern "C" int vfr_psi_load(vfr_ctx* c, const float* table) {
+  if (!c) { vfr_set_error("vfr_psi_load: null ctx"); return VFR_EINVAL; }
+  const float* src = table ? table : closed_form_psi();
+  CHECK_HIP(hipSetDevice(c->device));
+  CHECK_HIP(hipMemcpy(c->psi, src, sizeof(float) * 512 * 512, hipMemcpyHostToDevice));
+  return VFR_OK;
+}
+
+extern "C" int vfr_psi_read(vfr_ctx* c, float* out) {
+  if (!c || !out) { vfr_set_error("vfr_psi_read: null argument"); return VFR_EINVAL; }
+  CHECK_HIP(hipSetDevice(c->device));
+  CHECK_HIP(hipMemcpy(out, c->psi, sizeof(float) * 512 * 512, hipMemcpyDeviceToHost));
+  return VFR_OK;
+}
+
+extern "C" int vfr_tf_create(vfr_ctx* c, const float rgba[1024], int* tf_id) {
+  if (!c || !rgba || !tf_id) { vfr_set_error("vfr_tf_create: null argument"); return VFR_EINVAL; }
+  if (c->n_tf >= kTfSlots) { vfr_set_error("transfer-function table full (%u)", kTfSlots); return VFR_ENOMEM; }
+  for (int i = 0; i < 1024; ++i)
+    if (!(rgba[i] >= 0.0f && rgba[i] <= 1.0f)) {
+      vfr_set_error("transfer function entry %d component %d = %g is outside [0,1]", i / 4, i % 4,
+                    (double)rgba[i]);
+      return VFR_EINVAL;
+    }
+  CHECK_HIP(hipSetDevice(c->device));
+  CHECK_HIP(hipMemcpy(c->tf + (size_t)c->n_tf * 256, rgba, sizeof(float) * 1024, hipMemcpyHostToDevice));
+  *tf_id = (int)c->n_tf++;
+  return VFR_OK;
+}
+
+/* -------------------------------------------------------------------- mesh */
+
+extern "C" int vfr_mesh_create(vfr_ctx* c, const float* xyzs, uint32_t nV, const uint32_t* tets,
+                               uint32_t nT, const uint32_t* con, const uint32_t* extf,
+                               uint32_t n_ext, const float* lmt, uint32_t flags, vfr_mesh** out) {
+  if (!c || !xyzs || !tets || !out || nV < 4 || nT < 1) {
+    vfr_set_error("vfr_mesh_create: bad argument");
+    return VFR_EINVAL;
+  }
+  if ((uint64_t)nT * 4 >= (1ull << 31) || nT >= (1u << 30)) {
+    vfr_set_error("vfr_mesh_create: %u tets exceeds the 2^29 limit", nT);
+    return VFR_EINVAL;
+  }
+  if (extf && !con) { vfr_set_error("vfr_mesh_create: extf given without con"); return VFR_EINVAL; }
+  *out = nullptr;
+  for (size_t i = 0; i < 4 * (size_t)nT; ++i)
+    if (tets[i] >= nV) { vfr_set_error("tet %zu references vertex %u >= %u", i / 4, tets[i], nV); return VFR_EMESH; }
+  if (con)
+    for (size_t i = 0; i < 4 * (size_t)nT; ++i)
+      if (con[i] >= nT) { vfr_set_error("con entry %zu = %u >= %u", i, con[i], nT); return VFR_EMESH; }
+  if (extf)
+    for (size_t i = 0; i < (size_t)n_ext; ++i)
+      if (extf[i * 2] >= nT || extf[i * 2 + 1] > 3) { vfr_set_error("extf entry %zu invalid", i); return VFR_EMESH; }
+  CHECK_HIP(hipSetDevice(c->device));
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<float> v(xyzs, xyzs + 4 * (size_t)nV);
+  if (flags & VFR_MESH_NORMALIZE) vfr_normalize_vertices(v.data(), nV);
+  vfr_mesh* m = new (std::nothrow) vfr_mesh;
+  if (!m) { vfr_set_error("out of memory"); return VFR_ENOMEM; }
+  m->ctx = c;
+  m->d.n_verts = nV;
+  m->d.n_tets = nT;
+  for (int a = 0; a < 3; ++a) { m->bb_lo[a] = 1e30f; m->bb_hi[a] = -1e30f; }
+  for (uint32_t i = 0; i < nV; ++i)
+    for (int a = 0; a < 3; ++a) {
+      m->bb_lo[a] = std::min(m->bb_lo[a], v[(size_t)i * 4 + a]);
+      m->bb_hi[a] = std::max(m->bb_hi[a], v[(size_t)i * 4 + a]);
+    }
+  int rc = vfr_dev_build_mesh(m->d, v.data(), tets, con, extf, n_ext,
+                              (flags & VFR_MESH_MORTON) != 0, c->stream);
+  if (rc != VFR_OK) { vfr_mesh_destroy(m); return rc; }
+  if (lmt) memcpy(m->d.lmt, lmt, sizeof(float) * 3);
+  else vfr_build_lmt(v.data(), nV, tets, nT, m->d.lmt);
+  if (!(m->d.lmt[0] > 0.0f)) { vfr_set_error("max edge length %g is not positive", (double)m->d.lmt[0]); vfr_mesh_destroy(m); return VFR_EMESH; }
+  m->h_extf.resize(2 * (size_t)m->d.n_ext);
+  if (m->d.n_ext &&
+      hipMemcpy(m->h_extf.data(), m->d.extf, sizeof(uint32_t) * 2 * m->d.n_ext, hipMemcpyDeviceToHost) != hipSuccess) {
+    vfr_set_error("copy of external faces failed");
+    vfr_mesh_destroy(m);
+    return VFR_EHIP;
+  }
+  m->d.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  *out = m;
+  return VFR_OK;
+}
+
+extern "C" void vfr_mesh_destroy(vfr_mesh* m) {
+  if (!m) return;
+  if (m->ctx) (void)hipSetDevice(m->ctx->device);
+  void* ps[] = {m->d.rec, m->d.vert, m->d.ext_int, m->d.extf, m->d.perm, m->d.vperm, m->d.con};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  delete m;
+}
+
+extern "C" int vfr_mesh_get_info(const vfr_mesh* m, vfr_mesh_info* out) {
+  if (!m || !out) { vfr_set_error("vfr_mesh_get_info: null argument"); return VFR_EINVAL; }
+  out->n_verts = m->d.n_verts;
+  out->n_tets = m->d.n_tets;
+  out->n_ext_faces = m->d.n_ext;
+  out->max_edge_length = m->d.lmt[0];
+  out->max_z = m->d.lmt[1];
+  out->min_z = m->d.lmt[2];
+  out->device_bytes = m->d.bytes;
+  out->build_ms = m->d.build_ms;
+  return VFR_OK;
+}
+
+extern "C" int vfr_mesh_connectivity(const vfr_mesh* m, uint32_t* con_out, uint32_t* extf_out,
+                                     float* lmt_out) {
+  if (!m) { vfr_set_error("vfr_mesh_connectivity: null mesh"); return VFR_EINVAL; }
+  CHECK_HIP(hipSetDevice(m->ctx->device));
+  if (con_out)
+    CHECK_HIP(hipMemcpy(con_out, m->d.con, sizeof(uint32_t) * 4 * (size_t)m->d.n_tets, hipMemcpyDeviceToHost));
+  if (extf_out && m->d.n_ext) memcpy(extf_out, m->h_extf.data(), sizeof(uint32_t) * 2 * m->d.n_ext);
+  if (lmt_out) memcpy(lmt_out, m->d.lmt, sizeof(float) * 3);
+  return VFR_OK;
+}
+
+extern "C" int vfr_mesh_vertices(const vfr_mesh* m, float* out) {
+  if (!m || !out) { vfr_set_error("vfr_mesh_vertices: null argument"); return VFR_EINVAL; }
+  CHECK_HIP(hipSetDevice(m->ctx->device));
+  std::vector<float> v(4 * (size_t)m->d.n_verts);
+  std::vector<uint32_t> vp(m->d.n_verts);
+  CHECK_HIP(hipMemcpy(v.data(), m->d.vert, sizeof(float) * v.size(), hipMemcpyDeviceToHost));
+  CHECK_HIP(hipMemcpy(vp.data(), m->d.vperm, sizeof(uint32_t) * vp.size(), hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < m->d.n_verts; ++i) memcpy(out + (size_t)vp[i] * 4, &v[(size_t)i * 4], 16);
+  return VFR_OK;
+}
+
+/* -------------------------------------------------------------------- plan */
+
+struct vfr_plan {
+  vfr_ctx* ctx = nullptr;
+  const vfr_mesh* mesh = nullptr;
+  vfr_render_opts opts{};
+  std::vector<DevView> hv;
+  uint32_t n_views = 0;
+  uint64_t n_pix = 0;
+  uint32_t n_tiles = 0;
+  DevView* views = nullptr;
+  uint32_t* owner = nullptr;   /* REF owner or FTB count, [n_pix] */
+  uint64_t* entries = nullptr; /* FTB, [n_pix * VFR_K] */
+  uint64_t* rays = nullptr;    /* [n_pix] */
+  float4* rgba = nullptr;
+  uint32_t* steps = nullptr;
+  uint32_t* entry = nullptr;
+  uint32_t* counters = nullptr;           /* [n_views + 1] ray counts */
+  unsigned long long* stats = nullptr;    /* [ST_N] */
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  std::vector<hipEvent_t> fev;            /* per-view frame events */
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool executed = false;
+};
+
+/* View set-up on the host, in double, rounded once to float. */
+static int make_view(const vfr_view& in, const vfr_mesh* m, DevView* o) {
+  memset(o, 0, sizeof(*o));
+  if (in.width < 2 || in.height < 2 || in.width > 16384 || in.height > 16384) {
+    vfr_set_error("image size %ux%u outside [2,16384]", in.width, in.height);
+    return VFR_EINVAL;
+  }
+  o->W = in.width;
+  o->H = in.height;
+  o->proj = in.projection;
+  o->tf = (uint32_t)in.tf_id;
+  if (in.projection == VFR_PROJ_ORTHO) {
+    double A[3][3];
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 4; ++c) o->M[r * 4 + c] = in.ortho[r * 4 + c];
+      for (int c = 0; c < 3; ++c) A[r][c] = in.ortho[r * 4 + c];
+    }
+    double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                 A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                 A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    if (!(fabs(det) > 1e-12)) { vfr_set_error("ortho transform is singular"); return VFR_EINVAL; }
+    /* world direction of a unit step along -Z: third column of A^-1 */
+    double w0 = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+    double w1 = -(A[0][0] * A[1][2] - A[0][2] * A[1][0]) / det;
+    double w2 = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+    o->k = (float)sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+  } else if (in.projection == VFR_PROJ_PERSPECTIVE) {
+    if (!(in.fov_y_deg > 1.0f && in.fov_y_deg < 170.0f)) {
+      vfr_set_error("fov %g outside (1,170) degrees", (double)in.fov_y_deg);
+      return VFR_EINVAL;
+    }
+    double E[3], f[3], up[3];
+    for (int a = 0; a < 3; ++a) { E[a] = in.eye[a]; f[a] = (double)in.target[a] - in.eye[a]; up[a] = in.up[a]; }
+    double fl = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    if (!(fl > 0)) { vfr_set_error("eye equals target"); return VFR_EINVAL; }
+    for (int a = 0; a < 3; ++a) f[a] /= fl;
+    double r[3] = {f[1] * up[2] - f[2] * up[1], f[2] * up[0] - f[0] * up[2], f[0] * up[1] - f[1] * up[0]};
+    double rl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    if (!(rl > 1e-9)) { vfr_set_error("up vector parallel to the view direction"); return VFR_EINVAL; }
+    for (int a = 0; a < 3; ++a) r[a] /= rl;
+    double u[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0]};
+    double k = tan(in.fov_y_deg * M_PI / 360.0), cot = 1.0 / k;
+    const double* rows[3] = {r, u, f};
+    const double sc[3] = {cot, cot, 1.0};
+    for (int i = 0; i < 3; ++i) {
+      for (int a = 0; a < 3; ++a) o->M[i * 4 + a] = (float)(sc[i] * rows[i][a]);
+      o->M[i * 4 + 3] = (float)(-sc[i] * (rows[i][0] * E[0] + rows[i][1] * E[1] + rows[i][2] * E[2]));
+    }
+    o->k = (float)k;
+    /* every vertex must lie in front of the eye: check the bbox corners */
+    for (int q = 0; q < 8; ++q) {
+      double p[3] = {(q & 1) ? m->bb_hi[0] : m->bb_lo[0], (q & 2) ? m->bb_hi[1] : m->bb_lo[1],
+                     (q & 4) ? m->bb_hi[2] : m->bb_lo[2]};
+      double dc = f[0] * (p[0] - E[0]) + f[1] * (p[1] - E[1]) + f[2] * (p[2] - E[2]);
+      if (!(dc > 1e-3)) {
+        vfr_set_error("eye is inside or beside the mesh bounding box (depth %g)", dc);
+        return VFR_EINVAL;
+      }
+    }
+  } else {
+    vfr_set_error("unknown projection %u", in.projection);
+    return VFR_EINVAL;
+  }
+  return VFR_OK;
+}
+
+extern "C" void vfr_plan_destroy(vfr_plan* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->ctx->device);
+  (void)hipStreamSynchronize(p->ctx->stream);
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  void* ps[] = {p->views, p->owner, p->entries, p->rays, p->rgba, p->steps, p->entry, p->counters, p->stats};
+  for (void* q : ps)
+    if (q) (void)hipFree(q);
+  for (auto& e : p->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : p->fev)
+    if (e) (void)hipEventDestroy(e);
+  delete p;
+}
+
+static int enqueue_batched(vfr_plan* p) {
+  hipStream_t st = p->ctx->stream;
+  const vfr_mesh* m = p->mesh;
+  CHECK_RC(vfr_k_clear_u32(p->owner, p->n_pix, st));
+  CHECK_RC(vfr_k_clear_u32(p->counters, p->n_views + 1, st));
+  CHECK_RC(vfr_k_clear_u32((uint32_t*)p->stats, 2 * ST_N, st));
+  if (p->opts.mode == VFR_MODE_REF)
+    CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, m->d, p->owner, st));
+  else
+    CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d, p->owner, p->entries, p->stats, st));
+  CHECK_RC(vfr_k_compact(p->views, p->n_views, 0, p->n_tiles, p->opts.mode, p->owner, p->owner,
+                         p->entries, p->rgba, p->steps, p->entry, p->rays, p->counters + p->n_views, st));
+  return VFR_OK;
+}
+
+static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
+  TravArgs a;
+  const vfr_mesh* m = p->mesh;
+  a.rec = m->d.rec;
+  a.vert = (const float4*)m->d.vert;
+  a.ext_int = m->d.ext_int;
+  a.tf = p->ctx->tf;
+  a.psi = p->ctx->psi;
+  a.views = p->views;
+  a.rays = rays;
+  a.n_rays = n_rays;
+  a.owner = p->owner;
+  a.cnt = p->owner;
+  a.entries = p->entries;
+  a.rgba = p->rgba;
+  a.steps = p->steps;
+  a.entry_out = p->entry;
+  a.stats = p->stats;
+  a.queue = p->counters;
+  a.max_edge = m->d.lmt[0];
+  a.inv_me = 1.0f / m->d.lmt[0];
+  a.term = p->opts.term_alpha;
+  a.refill = p->opts.refill;
+  return a;
+}
+
+extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* views, int n_views,
+                               const vfr_render_opts* opts, vfr_plan** out) {
+  if (!c || !m || !views || n_views < 1 || !out) { vfr_set_error("vfr_plan_create: bad argument"); return VFR_EINVAL; }
+  if (m->ctx != c) { vfr_set_error("mesh belongs to another context"); return VFR_EINVAL; }
+  *out = nullptr;
+  vfr_render_opts o{};
+  if (opts) o = *opts;
+  if (o.mode > VFR_MODE_FTB || o.schedule > VFR_SCHED_BATCHED) { vfr_set_error("bad mode/schedule"); return VFR_EINVAL; }
+  if (!(o.term_alpha > 0.0f)) o.term_alpha = 0.99f;
+  if (o.term_alpha > 1.0f) { vfr_set_error("term_alpha %g > 1", (double)o.term_alpha); return VFR_EINVAL; }
+  if (o.max_entries == 0) o.max_entries = VFR_K;
+  if (o.max_entries != VFR_K) { vfr_set_error("max_entries must be %u", VFR_K); return VFR_EINVAL; }
+  vfr_plan* p = new (std::nothrow) vfr_plan;
+  if (!p) { vfr_set_error("out of memory"); return VFR_ENOMEM; }
+  p->ctx = c;
+  p->mesh = m;
+  p->opts = o;
+  p->n_views = (uint32_t)n_views;
+  p->hv.resize(n_views);
+  uint64_t pix = 0, tiles = 0;
+  for (int i = 0; i < n_views; ++i) {
+    int rc = make_view(views[i], m, &p->hv[i]);
+    if (rc == VFR_OK && (views[i].tf_id < 0 || (uint32_t)views[i].tf_id >= c->n_tf)) {
+      vfr_set_error("view %d: unknown transfer function %d", i, views[i].tf_id);
+      rc = VFR_EINVAL;
+    }
+    if (rc == VFR_OK && o.mode == VFR_MODE_REF && views[i].projection != VFR_PROJ_ORTHO) {
+      vfr_set_error("view %d: REF mode renders orthographic views only", i);
+      rc = VFR_EINVAL;
+    }
+    if (rc != VFR_OK) { delete p; return rc; }
+    p->hv[i].pix_off = (uint32_t)pix;
+    p->hv[i].tile_off = (uint32_t)tiles;
+    p->hv[i].tiles_x = (views[i].width + 7) / 8;
+    pix += (uint64_t)views[i].width * views[i].height;
+    tiles += (uint64_t)p->hv[i].tiles_x * ((views[i].height + 7) / 8);
+    if (pix >= (1ull << 31) || tiles >= (1ull << 31)) {
+      vfr_set_error("plan exceeds 2^31 pixels");
+      delete p;
+      return VFR_EINVAL;
+    }
+  }
+  p->n_pix = pix;
+  p->n_tiles = (uint32_t)tiles;
+  int rc = VFR_OK;
+  (void)hipSetDevice(c->device);
+#define PALLOC(ptr, bytes)                                                        \
+  if (rc == VFR_OK && hipMalloc((void**)&(ptr), (bytes) ? (bytes) : 16) != hipSuccess) { \
+    vfr_set_error("plan allocation of %zu bytes failed", (size_t)(bytes));       \
+    rc = VFR_ENOMEM;                                                              \
+  }
+  PALLOC(p->views, sizeof(DevView) * n_views);
+  PALLOC(p->owner, sizeof(uint32_t) * pix);
+  if (o.mode == VFR_MODE_FTB) PALLOC(p->entries, sizeof(uint64_t) * VFR_K * pix);
+  PALLOC(p->rays, sizeof(uint64_t) * pix);
+  PALLOC(p->rgba, sizeof(float4) * pix);
+  PALLOC(p->steps, sizeof(uint32_t) * pix);
+  PALLOC(p->entry, sizeof(uint32_t) * pix);
+  PALLOC(p->counters, sizeof(uint32_t) * (n_views + 1));
+  PALLOC(p->stats, sizeof(unsigned long long) * ST_N);
+#undef PALLOC
+  if (rc == VFR_OK && hipMemcpy(p->views, p->hv.data(), sizeof(DevView) * n_views, hipMemcpyHostToDevice) != hipSuccess) {
+    vfr_set_error("view upload failed");
+    rc = VFR_EHIP;
+  }
+  for (int i = 0; rc == VFR_OK && i < 3; ++i)
+    if (hipEventCreate(&p->ev[i]) != hipSuccess) { vfr_set_error("event creation failed"); rc = VFR_EHIP; }
+  if (rc == VFR_OK && o.schedule == VFR_SCHED_PER_VIEW) {
+    p->fev.assign(2 * (size_t)n_views, nullptr);
+    for (auto& e : p->fev)
+      if (rc == VFR_OK && hipEventCreate(&e) != hipSuccess) { vfr_set_error("event creation failed"); rc = VFR_EHIP; }
+  }
+  if (rc == VFR_OK && o.schedule == VFR_SCHED_BATCHED && o.use_graph) {
+    hipStream_t st = c->stream;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      vfr_set_error("graph capture failed to start");
+      rc = VFR_EHIP;
+    } else {
+      int rc2 = enqueue_batched(p);
+      TravArgs a = trav_args(p, p->rays, p->counters + p->n_views);
+      if (rc2 == VFR_OK) rc2 = vfr_k_traverse(o.mode, 0, a, (uint32_t)p->n_pix, 0, 0, st);
+      hipError_t e = hipStreamEndCapture(st, &p->graph);
+      if (rc2 != VFR_OK) rc = rc2;
+      else if (e != hipSuccess || hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0) != hipSuccess) {
+        vfr_set_error("graph capture failed");
+        rc = VFR_EHIP;
+      }
+    }
+  }
+  if (rc != VFR_OK) { vfr_plan_destroy(p); return rc; }
+  *out = p;
+  return VFR_OK;
+}
+
+static double pct(std::vector<double> v, double q) {
+  if (v.empty()) return 0.0;
+  std::sort(v.begin(), v.end());
+  double pos = q * (v.size() - 1);
+  size_t lo = (size_t)pos;
+  size_t hi = std::min(lo + 1, v.size() - 1);
+  return v[lo] + (pos - lo) * (v[hi] - v[lo]);
+}
+
+extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
+  if (!p) { vfr_set_error("vfr_plan_execute: null plan"); return VFR_EINVAL; }
+  vfr_ctx* c = p->ctx;
+  hipStream_t st = c->stream;
+  CHECK_HIP(hipSetDevice(c->device));
+  double ms_total = 0, ms_vis = 0, ms_trav = 0, p50 = 0, p99 = 0;
+  if (p->opts.schedule == VFR_SCHED_BATCHED) {
+    CHECK_HIP(hipEventRecord(p->ev[0], st));
+    if (p->gexec) {
+      CHECK_HIP(hipGraphLaunch(p->gexec, st));
+      CHECK_HIP(hipEventRecord(p->ev[1], st));
+    } else {
+      CHECK_RC(enqueue_batched(p));
+      CHECK_HIP(hipEventRecord(p->ev[1], st));
+      TravArgs a = trav_args(p, p->rays, p->counters + p->n_views);
+      CHECK_RC(vfr_k_traverse(p->opts.mode, 0, a, (uint32_t)p->n_pix, 0, 0, st));
+    }
+    CHECK_HIP(hipEventRecord(p->ev[2], st));
+    CHECK_HIP(hipEventSynchronize(p->ev[2]));
+    float a = 0, b = 0;
+    CHECK_HIP(hipEventElapsedTime(&a, p->ev[0], p->ev[2]));
+    ms_total = a;
+    if (!p->gexec) {
+      CHECK_HIP(hipEventElapsedTime(&b, p->ev[0], p->ev[1]));
+      ms_vis = b;
+      ms_trav = a - b;
+    }
+  } else {
+    const vfr_mesh* m = p->mesh;
+    CHECK_RC(vfr_k_clear_u32((uint32_t*)p->stats, 2 * ST_N, st));
+    CHECK_RC(vfr_k_clear_u32(p->counters, p->n_views + 1, st));
+    std::vector<double> frame(p->n_views), vis(p->n_views);
+    CHECK_HIP(hipEventRecord(p->ev[0], st));
+    for (uint32_t v = 0; v < p->n_views; ++v) {
+      const DevView& hv = p->hv[v];
+      uint32_t npx = hv.W * hv.H;
+      uint32_t ntl = hv.tiles_x * ((hv.H + 7) / 8);
+      CHECK_HIP(hipEventRecord(p->fev[2 * v], st));
+      CHECK_RC(vfr_k_clear_u32(p->owner + hv.pix_off, npx, st));
+      if (p->opts.mode == VFR_MODE_REF)
+        CHECK_RC(vfr_k_raster_ref(p->views, v, 1, m->d, p->owner, st));
+      else
+        CHECK_RC(vfr_k_raster_ftb(p->views, v, 1, m->d, p->owner, p->entries, p->stats, st));
+      CHECK_RC(vfr_k_compact(p->views, p->n_views, hv.tile_off, ntl, p->opts.mode, p->owner, p->owner,
+                             p->entries, p->rgba, p->steps, p->entry, p->rays + hv.pix_off,
+                             p->counters + v, st));
+      CHECK_HIP(hipEventRecord(p->ev[1], st));
+      TravArgs a = trav_args(p, p->rays + hv.pix_off, p->counters + v);
+      CHECK_RC(vfr_k_traverse(p->opts.mode, 0, a, npx, 0, 0, st));
+      CHECK_HIP(hipEventRecord(p->fev[2 * v + 1], st));
+      CHECK_HIP(hipEventSynchronize(p->fev[2 * v + 1]));  /* a frame ends before the next starts */
+      float f = 0, g = 0;
+      CHECK_HIP(hipEventElapsedTime(&f, p->fev[2 * v], p->fev[2 * v + 1]));
+      CHECK_HIP(hipEventElapsedTime(&g, p->fev[2 * v], p->ev[1]));
+      frame[v] = f;
+      vis[v] = g;
+    }
+    CHECK_HIP(hipEventRecord(p->ev[2], st));
+    CHECK_HIP(hipEventSynchronize(p->ev[2]));
+    float a = 0;
+    CHECK_HIP(hipEventElapsedTime(&a, p->ev[0], p->ev[2]));
+    ms_total = a;
+    for (uint32_t v = 0; v < p->n_views; ++v) { ms_vis += vis[v]; ms_trav += frame[v] - vis[v]; }
+    p50 = pct(frame, 0.50);
+    p99 = pct(frame, 0.99);
+  }
+  p->executed = true;
+  if (stats) {
+    unsigned long long s[ST_N];
+    std::vector<uint32_t> cnt(p->n_views + 1);
+    CHECK_HIP(hipMemcpy(s, p->stats, sizeof(s), hipMemcpyDeviceToHost));
+    CHECK_HIP(hipMemcpy(cnt.data(), p->counters, sizeof(uint32_t) * cnt.size(), hipMemcpyDeviceToHost));
+    uint64_t active = 0;
+    if (p->opts.schedule == VFR_SCHED_BATCHED) active = cnt[p->n_views];
+    else for (uint32_t v = 0; v < p->n_views; ++v) active += cnt[v];
+    memset(stats, 0, sizeof(*stats));
+    stats->rays = p->n_pix;
+    stats->active_rays = active;
+    stats->steps = s[ST_STEPS];
+    stats->anomalies = s[ST_ANOM];
+    stats->ms_total = ms_total;
+    stats->ms_visibility = ms_vis;
+    stats->ms_traversal = ms_trav;
+    stats->ms_frame_p50 = p50;
+    stats->ms_frame_p99 = p99;
+    if (s[ST_OVERFLOW]) {
+      vfr_set_error("%llu pixel entries beyond %u per pixel were dropped", s[ST_OVERFLOW], VFR_K);
+      return VFR_EMESH;
+    }
+    if (s[4]) {
+      vfr_set_error("%llu rays met a neighbour without a back link (non-manifold mesh)", s[4]);
+      return VFR_EMESH;
+    }
+  }
+  return VFR_OK;
+}
+
+extern "C" int vfr_plan_read(vfr_plan* p, int view, vfr_frame_out* out) {
+  if (!p || !out || view < 0 || (uint32_t)view >= p->n_views) { vfr_set_error("vfr_plan_read: bad argument"); return VFR_EINVAL; }
+  if (!p->executed) { vfr_set_error("plan not executed"); return VFR_ESTATE; }
+  CHECK_HIP(hipSetDevice(p->ctx->device));
+  const DevView& hv = p->hv[view];
+  const size_t n = (size_t)hv.W * hv.H;
+  std::vector<float> rgba;
+  if (out->rgba || out->rgb8) {
+    float* dst = out->rgba;
+    if (!dst) { rgba.resize(4 * n); dst = rgba.data(); }
+    CHECK_HIP(hipMemcpy(dst, p->rgba + hv.pix_off, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    if (out->rgb8)
+      for (size_t i = 0; i < n; ++i)
+        for (int ch = 0; ch < 3; ++ch) {
+          float x = dst[i * 4 + ch];
+          x = x > 1.0f ? 1.0f : x;
+          out->rgb8[i * 3 + ch] = (uint8_t)(int)(255.0f * x);
+        }
+  }
+  std::vector<uint32_t> ent;
+  if (out->entry_ext || out->entry_tet) {
+    uint32_t* dst = out->entry_ext;
+    if (!dst) { ent.resize(n); dst = ent.data(); }
+    CHECK_HIP(hipMemcpy(dst, p->entry + hv.pix_off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    if (out->entry_tet)
+      for (size_t i = 0; i < n; ++i)
+        out->entry_tet[i] = dst[i] == VFR_NO_ID ? VFR_NO_ID : p->mesh->h_extf[(size_t)dst[i] * 2];
+  }
+  if (out->steps)
+    CHECK_HIP(hipMemcpy(out->steps, p->steps + hv.pix_off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  return VFR_OK;
+}
+
+extern "C" int vfr_render(vfr_ctx* c, const vfr_mesh* m, const vfr_view* views, int n_views,
+                          const vfr_render_opts* opts, vfr_frame_out* outs, vfr_stats* stats) {
+  vfr_plan* p = nullptr;
+  CHECK_RC(vfr_plan_create(c, m, views, n_views, opts, &p));
+  vfr_stats s;
+  int rc = vfr_plan_execute(p, &s);
+  if (rc == VFR_OK && outs)
+    for (int v = 0; v < n_views && rc == VFR_OK; ++v) rc = vfr_plan_read(p, v, &outs[v]);
+  if (stats) *stats = s;
+  vfr_plan_destroy(p);
+  return rc;
+}

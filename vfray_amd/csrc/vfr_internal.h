@@ -1,0 +1,105 @@
+/*
+ * vfr_internal.h -- structures shared by the host API and the HIP kernels.
+ */
+#ifndef VFR_INTERNAL_H
+#define VFR_INTERNAL_H
+
+#include <cstddef>
+#include <cstdint>
+
+void vfr_set_error(const char* fmt, ...);
+double vfr_psi_integral(double tf, double tb);
+
+/* Packed neighbour word of a tet face: (next_tet << 2) | entry_face_in_next,
+ * or one of these codes. */
+#define VFR_NB_EXT 0xFFFFFFFFu    /* external face (the .con file's "itself") */
+#define VFR_NB_BROKEN 0xFFFFFFFEu /* neighbour whose faces do not link back   */
+
+/* A walk that visits more tets than this is stopped as an anomaly (a ray
+ * cycling around a degenerate edge would otherwise never end). */
+#define VFR_MAX_STEPS 65536u
+
+/* Entry-list slots per pixel in front-to-back mode. */
+#define VFR_K 8u
+
+#define VFR_NO_ID 0xFFFFFFFFu
+
+/* 32-byte tet record: the 4 vertex ids and the 4 packed neighbour words,
+ * read by one lane as two 16-byte loads of one 32-byte sector. */
+struct alignas(32) TetRec {
+  uint32_t v[4];
+  uint32_t n[4];
+};
+
+/* One view of a render plan. */
+struct alignas(16) DevView {
+  float M[12];       /* row-major 3x4 view transform                       */
+  float k;           /* ortho: world length per unit Z; persp: tan(fov/2)  */
+  uint32_t W, H;
+  uint32_t proj;     /* VFR_PROJ_*                                          */
+  uint32_t tf;       /* transfer-function slot                              */
+  uint32_t pix_off;  /* first pixel of this view in the per-pixel arrays    */
+  uint32_t tile_off; /* first 8x8 tile of this view                         */
+  uint32_t tiles_x;  /* 8x8 tiles per image row                             */
+};
+
+struct DevMesh {
+  uint32_t n_verts = 0, n_tets = 0, n_ext = 0;
+  float lmt[3] = {0, 0, 0};
+  TetRec* rec = nullptr;       /* [n_tets] internal order                   */
+  float* vert = nullptr;       /* [n_verts*4] internal order (x,y,z,s)      */
+  uint32_t* ext_int = nullptr; /* [n_ext*2] (internal tet, face)            */
+  uint32_t* extf = nullptr;    /* [n_ext*2] (caller tet, face), .extf order */
+  uint32_t* perm = nullptr;    /* [n_tets] internal -> caller tet id        */
+  uint32_t* vperm = nullptr;   /* [n_verts] internal -> caller vertex id    */
+  uint32_t* con = nullptr;     /* [n_tets*4] caller ids, .con body          */
+  uint64_t bytes = 0;
+  double build_ms = 0;
+};
+
+/* Statistics counters a render accumulates on the device. */
+enum { ST_STEPS = 0, ST_ANOM = 1, ST_OVERFLOW = 2, ST_RAYS = 3, ST_N = 8 };
+
+#ifdef __HIPCC__
+struct TravArgs {
+  const TetRec* rec;
+  const float4* vert;
+  const uint32_t* ext_int;
+  const float4* tf;        /* [slots * 256] */
+  const float* psi;        /* [512 * 512]   */
+  const DevView* views;
+  const uint64_t* rays;    /* (view << 32) | pixel */
+  const uint32_t* n_rays;  /* device count of rays */
+  const uint32_t* owner;   /* REF: ext index + 1, 0 = none */
+  const uint32_t* cnt;     /* FTB: entries per pixel */
+  const uint64_t* entries; /* FTB: [pixels * VFR_K] keys (z << 32 | ext) */
+  float4* rgba;
+  uint32_t* steps;
+  uint32_t* entry_out;
+  unsigned long long* stats;
+  uint32_t* queue;         /* persistent kernel work-queue head */
+  float max_edge, inv_me, term;
+  uint32_t refill;         /* persistent: refill when this many lanes idle */
+};
+
+/* ---- launchers (vfr_kernels.hip) ---- */
+
+int vfr_k_clear_u32(uint32_t* p, size_t n, void* stream);
+int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m,
+                     uint32_t* owner, void* stream);
+int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m,
+                     uint32_t* cnt, uint64_t* entries, unsigned long long* stats, void* stream);
+int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, uint32_t ntiles,
+                  uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries,
+                  float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
+                  uint32_t* n_rays, void* stream);
+int vfr_k_traverse(uint32_t mode, uint32_t proj_mask, const TravArgs& a, uint32_t max_rays,
+                   int persistent, uint32_t grid_waves, void* stream);
+
+/* preprocessing on the device */
+int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
+                       const uint32_t* h_con, const uint32_t* h_extf, uint32_t n_ext_given,
+                       bool morton, void* stream);
+#endif /* __HIPCC__ */
+
+#endif

@@ -1,0 +1,1022 @@
+/*
+ * vfr_kernels.hip -- CDNA4 (gfx950) kernels of the VF-Ray renderer.
+ *
+ *   preprocessing : face keys -> radix sort -> adjacent match -> external
+ *                   faces (stream compaction) -> Z-curve reorder -> packed
+ *                   32-byte tet records.  Replaces buildIncid/buildCon/
+ *                   buildExtF (vfRayPreComp.cc:219-270, 424-501, 710-737).
+ *   raster        : per (view, external face): visibility (the Cramer plane
+ *                   test of findVisibleFaces, vfRayKernelCPU.cc:51-132), the
+ *                   projected bounding box (projectVisibleFaces, :135-169) and
+ *                   the per-pixel inside test of vfRay (:248-268), writing
+ *                   per-pixel entries.
+ *   compact       : 8x8 pixel tiles -> wave ballot -> ray list; sorts each
+ *                   pixel's entry list front to back.
+ *   traverse      : one ray per lane, tet walk through packed neighbour words
+ *                   with the reference's exit-face test (:302-367) and
+ *                   partial pre-integration (:448-482).
+ *
+ * All kernels are compiled with -ffp-contract=off: every float operation that
+ * decides an entry face, an exit face or a step count is the same IEEE
+ * operation, in the same order, as in the CPU oracle.
+ */
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <vector>
+
+#include "../../include/vfr.h"
+#include "vfr_internal.h"
+
+#define CHECK_HIP(x)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (x);                                                      \
+    if (e_ != hipSuccess) {                                                   \
+      vfr_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+      return VFR_EHIP;                                                        \
+    }                                                                         \
+  } while (0)
+
+#define DEV __device__ __forceinline__
+
+/* ------------------------------------------------------------ device maths */
+
+/* |x| < RDELTA with x promoted to double (vfRayKernelCPU.cc:85, 206, 323). */
+DEV bool tiny(float x) { return (double)fabsf(x) < 1e-12; }
+
+DEV uint32_t sel4(const uint4& v, uint32_t i) {
+  return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+
+DEV float4 ld_vert(const float4* vert, uint32_t i) { return vert[i]; }
+
+/* Projective view transform; identical operation order to the oracle. */
+struct ViewRegs {
+  float M[12];
+  uint32_t proj;
+};
+
+DEV float4 xform(const ViewRegs& v, float4 p) {
+  const float* M = v.M;
+  float a = ((M[0] * p.x + M[1] * p.y) + M[2] * p.z) + M[3];
+  float b = ((M[4] * p.x + M[5] * p.y) + M[6] * p.z) + M[7];
+  float c = ((M[8] * p.x + M[9] * p.y) + M[10] * p.z) + M[11];
+  float4 r;
+  if (v.proj == VFR_PROJ_ORTHO) {
+    r.x = a; r.y = b; r.z = c; r.w = p.w;
+  } else {
+    r.z = 1.0f / c; r.x = a * r.z; r.y = b * r.z; r.w = p.w * r.z;
+  }
+  return r;
+}
+
+DEV ViewRegs load_view(const DevView* views, uint32_t vi) {
+  ViewRegs r;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) r.M[k] = views[vi].M[k];
+  r.proj = views[vi].proj;
+  return r;
+}
+
+/* obj = (pix*2)/(real)(res-1) - 1.0 (vfRayKernelCPU.cc:250): the double
+ * subtraction rounded to float equals the float subtraction, because q - 1
+ * is exact in double for q in [0, 2]. */
+DEV float pix_to_obj(uint32_t pix, uint32_t res) {
+  float q = (float)(pix * 2u) / (float)(res - 1u);
+  return q - 1.0f;
+}
+
+struct Planes {
+  float z[3], s[3];
+};
+
+DEV void cramer(float c0x, float c0y, float c0z, float c1x, float c1y, float c1z, float tx,
+                float ty, float tz, float det, float out[3]) {
+  float tmpx = ty - tz;
+  float tmpy = (ty * c1z) - (c1y * tz);
+  float tmpz = (c0y * tz) - (ty * c0z);
+  out[0] = ((tx * (c1y - c1z)) - (c1x * tmpx) + tmpy) / det;
+  out[1] = ((c0x * tmpx) - tx * (c0y - c0z) + tmpz) / det;
+  out[2] = (c0x * (-tmpy) - (c1x * tmpz) + tx * (c0y * c1z - c1y * c0z)) / det;
+}
+
+/* Face planes of vfRayKernelCPU.cc:73-122 (z and s as functions of x,y). */
+DEV Planes face_planes(float4 a, float4 b, float4 c) {
+  float c0x = a.x, c0y = b.x, c0z = c.x;
+  float c1x = a.y, c1y = b.y, c1z = c.y;
+  float det = c0x * (c1y - c1z) - c1x * (c0y - c0z) + ((c0y * c1z) - (c1y * c0z));
+  det = tiny(det) ? (float)1e-12 : det;
+  Planes p;
+  cramer(c0x, c0y, c0z, c1x, c1y, c1z, a.z, b.z, c.z, det, p.z);
+  cramer(c0x, c0y, c0z, c1x, c1y, c1z, a.w, b.w, c.w, det, p.s);
+  return p;
+}
+
+DEV float plane_eval(const float p[3], float ox, float oy) { return ox * p[0] + oy * p[1] + p[2]; }
+
+struct Hit {
+  float cy, cz, D;
+  bool sw, ok;
+};
+
+/* One candidate of the exit-face test (vfRayKernelCPU.cc:316-348). */
+DEV Hit tri_test(float4 a, float4 b, float4 c, float ox, float oy) {
+  Hit h;
+  float v1x = b.x - a.x, v1y = b.y - a.y;
+  float v2x = c.x - a.x, v2y = c.y - a.y;
+  float cx = (v1x * v2y) - (v1y * v2x);
+  h.ok = false;
+  h.sw = false;
+  h.cy = h.cz = 0.0f;
+  h.D = cx;
+  if (tiny(cx)) return h;
+  if (cx < 0.0f) {
+    float tx = v1x, ty = v1y;
+    v1x = v2x; v1y = v2y; v2x = tx; v2y = ty;
+    cx = -cx; h.sw = true;
+  }
+  float v0x = ox - a.x, v0y = oy - a.y;
+  float cy = (v1x * v0y) - (v1y * v0x);
+  float cz = (v0x * v2y) - (v0y * v2x);
+  h.cy = cy; h.cz = cz; h.D = cx;
+  h.ok = (cy >= 0.0f) && (cz >= 0.0f) && ((double)(cy + cz) <= (double)cx + 1e-12);
+  return h;
+}
+
+/* The entry inside test of vfRayKernelCPU.cc:196-268 (no slack, degenerate
+ * cross replaced by RDELTA). */
+DEV Hit entry_test(float4 a, float4 b, float4 c, float ox, float oy) {
+  Hit h;
+  float v01x = b.x - a.x, v01y = b.y - a.y, v02x = c.x - a.x, v02y = c.y - a.y;
+  float cx = (v01x * v02y) - (v01y * v02x);
+  cx = tiny(cx) ? (float)1e-12 : cx;
+  h.sw = false;
+  if (cx < 0.0f) {
+    float tx = v01x, ty = v01y;
+    v01x = v02x; v01y = v02y; v02x = tx; v02y = ty;
+    cx = -cx; h.sw = true;
+  }
+  float v0px = ox - a.x, v0py = oy - a.y;
+  h.cy = (v01x * v0py) - (v01y * v0px);
+  h.cz = (v0px * v02y) - (v0py * v02x);
+  h.D = cx;
+  h.ok = (h.cy >= 0.0f) && (h.cz >= 0.0f) && ((h.cy + h.cz) <= cx);
+  return h;
+}
+
+/* Value at the pixel from the inside-test crosses (weights cz/D for b and
+ * cy/D for c, exchanged when the edge vectors were swapped). */
+DEV float bary(float A, float B, float C, const Hit& h) {
+  float wb = h.sw ? h.cy : h.cz;
+  float wc = h.sw ? h.cz : h.cy;
+  return A + ((wb * (B - A) + wc * (C - A)) / h.D);
+}
+
+DEV float clamp01(float s) { return s < 0.0f ? 0.0f : (s > 1.0f ? 1.0f : s); }
+
+DEV uint32_t fkey(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+DEV float funkey(uint32_t k) {
+  uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+  return __uint_as_float(u);
+}
+
+/* exp(x), x <= 0: Cody-Waite reduction with a two-part ln2 and a degree-7
+ * Taylor polynomial on |r| <= ln2/2, scaled by 2^n. */
+DEV float vfr_expf(float x) {
+  if (x < -87.0f) return 0.0f;
+  float n = __builtin_rintf(x * 1.44269504f);
+  float r = (x - n * 0.693145751953125f) - n * 1.42860682030941723e-6f;
+  float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.166666672f + r * (0.0416666679f +
+            r * (0.00833333377f + r * (0.00138888892f + r * 0.000198412701f))))));
+  return __builtin_ldexpf(p, (int)n);
+}
+
+DEV uint32_t psi_index(float g) {
+  uint32_t i = (uint32_t)(g * 512.0f + 0.5f);
+  return i > 511u ? 511u : i;
+}
+
+DEV uint32_t img_coord_clamped(float x, uint32_t res) {
+  double f = ((double)res - 1.0) * ((double)x + 1.0) / 2.0;
+  if (f <= 0.0) return 0u;
+  if (f >= (double)res - 1.0) return res - 1u;
+  return (uint32_t)f;
+}
+
+/* ---------------------------------------------------------- wave helpers */
+
+DEV unsigned long long wave_sum_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+/* ------------------------------------------------------------------ clear */
+
+__global__ void k_clear(uint32_t* __restrict__ p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = 0u;
+}
+
+int vfr_k_clear_u32(uint32_t* p, size_t n, void* stream) {
+  if (n == 0) return VFR_OK;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_clear, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, p, n);
+  CHECK_HIP(hipGetLastError());
+  return VFR_OK;
+}
+
+/* ----------------------------------------------------------------- raster */
+
+/* Loads the ext face (internal tet, face) vertices in P space: slots
+ * (f+0..3)&3, the last one the opposite vertex. */
+DEV void face_verts(const TetRec* rec, const float4* vert, const uint32_t* ext_int, uint32_t e,
+                    const ViewRegs& vr, float4 out[4], uint32_t* tet, uint32_t* face) {
+  uint32_t t = ext_int[e * 2], f = ext_int[e * 2 + 1];
+  uint4 rv = *reinterpret_cast<const uint4*>(rec[t].v);
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) out[k] = xform(vr, ld_vert(vert, sel4(rv, (f + k) & 3u)));
+  *tet = t;
+  *face = f;
+}
+
+/* REF raster: one thread per (view, external face).  Visible faces write
+ * atomicMax(owner, ext+1) into every pixel the reference's loop would write:
+ * x outer, y inner over the inclusive bbox, inside test, and the column ends
+ * at the first inside pixel whose entry scalar is outside [0,1]
+ * (vfRayKernelCPU.cc:274).  The largest visible index wins, as the last face
+ * in main.cc's loop (main.cc:209-218) overwrites the pixel. */
+__global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ views, uint32_t v0,
+                                                    uint32_t nv, const TetRec* __restrict__ rec,
+                                                    const float4* __restrict__ vert,
+                                                    const uint32_t* __restrict__ ext_int,
+                                                    uint32_t n_ext, uint32_t* __restrict__ owner) {
+  uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (uint64_t)nv * n_ext) return;
+  uint32_t vi = v0 + (uint32_t)(gid / n_ext);
+  uint32_t e = (uint32_t)(gid % n_ext);
+  ViewRegs vr = load_view(views, vi);
+  const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
+  float4 fv[4];
+  uint32_t t, f;
+  face_verts(rec, vert, ext_int, e, vr, fv, &t, &f);
+  Planes pl = face_planes(fv[0], fv[1], fv[2]);
+  float faceZ = fv[3].x * pl.z[0] + fv[3].y * pl.z[1] + pl.z[2];
+  if (faceZ < fv[3].z) return;   /* vfRayKernelCPU.cc:128 */
+  if (pl.s[0] == -1.0f) return;  /* main.cc:97 drops sParams.x == -1 */
+  uint32_t bx0 = W - 1, by0 = H - 1, bx1 = 0, by1 = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    uint32_t ix = img_coord_clamped(fv[k].x, W), iy = img_coord_clamped(fv[k].y, H);
+    bx0 = min(bx0, ix); by0 = min(by0, iy); bx1 = max(bx1, ix); by1 = max(by1, iy);
+  }
+  for (uint32_t px = bx0; px <= bx1; ++px) {
+    float ox = pix_to_obj(px, W);
+    for (uint32_t py = by0; py <= by1; ++py) {
+      float oy = pix_to_obj(py, H);
+      Hit h = entry_test(fv[0], fv[1], fv[2], ox, oy);
+      if (!h.ok) continue;
+      float s = plane_eval(pl.s, ox, oy);
+      if ((s < 0.0f) || (s > 1.0f)) break;
+      atomicMax(&owner[off + py * W + px], e + 1u);
+    }
+  }
+}
+
+int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m,
+                     uint32_t* owner, void* stream) {
+  uint64_t n = (uint64_t)nv * m.n_ext;
+  if (n == 0) return VFR_OK;
+  hipLaunchKernelGGL(k_raster_ref, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, views, v0, nv, m.rec, (const float4*)m.vert, m.ext_int,
+                     m.n_ext, owner);
+  CHECK_HIP(hipGetLastError());
+  return VFR_OK;
+}
+
+/* FTB raster: every visible external face (front facing in P space) appends
+ * the key (Z at the pixel, ext index) to each pixel its projection holds. */
+__global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ views, uint32_t v0,
+                                                    uint32_t nv, const TetRec* __restrict__ rec,
+                                                    const float4* __restrict__ vert,
+                                                    const uint32_t* __restrict__ ext_int,
+                                                    uint32_t n_ext, uint32_t* __restrict__ cnt,
+                                                    uint64_t* __restrict__ entries,
+                                                    unsigned long long* __restrict__ stats) {
+  uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (uint64_t)nv * n_ext) return;
+  uint32_t vi = v0 + (uint32_t)(gid / n_ext);
+  uint32_t e = (uint32_t)(gid % n_ext);
+  ViewRegs vr = load_view(views, vi);
+  const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
+  float4 fv[4];
+  uint32_t t, f;
+  face_verts(rec, vert, ext_int, e, vr, fv, &t, &f);
+  Planes pl = face_planes(fv[0], fv[1], fv[2]);
+  float faceZ = fv[3].x * pl.z[0] + fv[3].y * pl.z[1] + pl.z[2];
+  if (faceZ < fv[3].z) return;
+  double lo0 = 1e30, lo1 = 1e30, hi0 = -1e30, hi1 = -1e30;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double fx = ((double)W - 1.0) * ((double)fv[k].x + 1.0) / 2.0;
+    double fy = ((double)H - 1.0) * ((double)fv[k].y + 1.0) / 2.0;
+    lo0 = fmin(lo0, fx); hi0 = fmax(hi0, fx);
+    lo1 = fmin(lo1, fy); hi1 = fmax(hi1, fy);
+  }
+  if (hi0 < 0.0 || hi1 < 0.0 || lo0 > W - 1.0 || lo1 > H - 1.0) return;
+  uint32_t bx0 = lo0 < 0.0 ? 0u : (uint32_t)lo0, by0 = lo1 < 0.0 ? 0u : (uint32_t)lo1;
+  uint32_t bx1 = hi0 > W - 1.0 ? W - 1u : (uint32_t)hi0, by1 = hi1 > H - 1.0 ? H - 1u : (uint32_t)hi1;
+  unsigned long long over = 0;
+  for (uint32_t px = bx0; px <= bx1; ++px) {
+    float ox = pix_to_obj(px, W);
+    for (uint32_t py = by0; py <= by1; ++py) {
+      float oy = pix_to_obj(py, H);
+      Hit h = entry_test(fv[0], fv[1], fv[2], ox, oy);
+      if (!h.ok) continue;
+      float Zp = bary(fv[0].z, fv[1].z, fv[2].z, h);
+      uint32_t gp = off + py * W + px;
+      uint32_t slot = atomicAdd(&cnt[gp], 1u);
+      if (slot < VFR_K) entries[(size_t)gp * VFR_K + slot] = ((uint64_t)fkey(Zp) << 32) | e;
+      else over++;
+    }
+  }
+  if (over) atomicAdd(&stats[ST_OVERFLOW], over);
+}
+
+int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m,
+                     uint32_t* cnt, uint64_t* entries, unsigned long long* stats, void* stream) {
+  uint64_t n = (uint64_t)nv * m.n_ext;
+  if (n == 0) return VFR_OK;
+  hipLaunchKernelGGL(k_raster_ftb, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, views, v0, nv, m.rec, (const float4*)m.vert, m.ext_int,
+                     m.n_ext, cnt, entries, stats);
+  CHECK_HIP(hipGetLastError());
+  return VFR_OK;
+}
+
+/* ---------------------------------------------------------------- compact */
+
+#define CS(a, b)                         \
+  {                                      \
+    uint64_t x_ = e[a], y_ = e[b];       \
+    e[a] = x_ > y_ ? x_ : y_;            \
+    e[b] = x_ > y_ ? y_ : x_;            \
+  }
+
+/* One wave per 8x8 tile: pixels with an entry become rays (wave ballot,
+ * one atomic per wave), inactive pixels get zero outputs, and each active
+ * pixel's entry list is sorted front to back (Batcher's 19-comparator
+ * network on 8 keys, descending). */
+__global__ __launch_bounds__(256) void k_compact(const DevView* __restrict__ views,
+                                                 uint32_t n_views, uint32_t tile0, uint32_t ntiles,
+                                                 uint32_t mode, const uint32_t* __restrict__ owner,
+                                                 uint32_t* __restrict__ cnt,
+                                                 uint64_t* __restrict__ entries,
+                                                 float4* __restrict__ rgba, uint32_t* __restrict__ steps,
+                                                 uint32_t* __restrict__ entry_out,
+                                                 uint64_t* __restrict__ rays,
+                                                 uint32_t* __restrict__ n_rays) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wt = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wt >= ntiles) return;
+  const uint32_t gt = tile0 + wt;
+  /* view owning tile gt: last view with tile_off <= gt */
+  uint32_t lo = 0, hi = n_views;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (views[mid].tile_off <= gt) lo = mid; else hi = mid;
+  }
+  const uint32_t vi = lo;
+  const uint32_t W = views[vi].W, H = views[vi].H;
+  const uint32_t lt = gt - views[vi].tile_off, tx = lt % views[vi].tiles_x, ty = lt / views[vi].tiles_x;
+  const uint32_t px = tx * 8u + (lane & 7u), py = ty * 8u + (lane >> 3);
+  const bool valid = px < W && py < H;
+  const uint32_t pix = py * W + px;
+  const uint32_t gp = views[vi].pix_off + pix;
+  bool active = false;
+  if (valid) {
+    if (mode == VFR_MODE_REF) {
+      active = owner[gp] != 0u;
+    } else {
+      uint32_t nk = cnt[gp];
+      active = nk != 0u;
+      if (nk > 1u) {
+        nk = nk > VFR_K ? VFR_K : nk;
+        uint64_t e[8];
+        uint64_t* ent = entries + (size_t)gp * VFR_K;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) e[k] = k < nk ? ent[k] : 0ull;
+        CS(0, 1) CS(2, 3) CS(4, 5) CS(6, 7) CS(0, 2) CS(1, 3) CS(4, 6) CS(5, 7) CS(1, 2) CS(5, 6)
+        CS(0, 4) CS(1, 5) CS(2, 6) CS(3, 7) CS(2, 4) CS(3, 5) CS(1, 2) CS(3, 4) CS(5, 6)
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k)
+          if (k < nk) ent[k] = e[k];
+      }
+    }
+    if (!active) {
+      rgba[gp] = make_float4(0.f, 0.f, 0.f, 0.f);
+      steps[gp] = 0u;
+      entry_out[gp] = VFR_NO_ID;
+    }
+  }
+  const uint64_t m = __ballot(active);
+  const uint32_t n = (uint32_t)__popcll(m);
+  if (n == 0) return;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(n_rays, n);
+  base = __shfl(base, 0, 64);
+  if (active) {
+    const uint32_t before = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    rays[base + before] = ((uint64_t)vi << 32) | pix;
+  }
+}
+
+int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, uint32_t ntiles,
+                  uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries,
+                  float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
+                  uint32_t* n_rays, void* stream) {
+  if (ntiles == 0) return VFR_OK;
+  hipLaunchKernelGGL(k_compact, dim3((ntiles + 3) / 4), dim3(256), 0, (hipStream_t)stream, views,
+                     n_views_total, tile0, ntiles, mode, owner, cnt, entries, rgba, steps,
+                     entry_out, rays, n_rays);
+  CHECK_HIP(hipGetLastError());
+  return VFR_OK;
+}
+
+/* --------------------------------------------------------------- traverse */
+
+/* Per-lane walk state.  H[0..2] are the entry face's vertices in P space with
+ * their ids; the tet record supplies the fourth. */
+struct Walk {
+  float4 H0, H1, H2;
+  uint32_t h0, h1, h2;
+  uint32_t t, fe;
+};
+
+struct Step {
+  float4 A, B, C;      /* exit face vertices, the face's slot order */
+  uint32_t a, b, c;    /* their ids */
+  uint32_t nb;         /* packed neighbour word across the exit face */
+  Hit h;
+  int i;               /* 1..3, 0 = no exit found */
+};
+
+/* Loads the current tet, rebuilds Q[k] = slot (fe+k)&3 from the carried
+ * entry-face vertices plus the one new vertex, and runs the reference's
+ * exit-face test on faces (fe+1..3)&3 in that order. */
+DEV Step walk_step(const Walk& w, const TetRec* __restrict__ rec, const float4* __restrict__ vert,
+                   const ViewRegs& vr, float ox, float oy) {
+  const uint4 rv = *reinterpret_cast<const uint4*>(rec[w.t].v);
+  const uint4 rn = *reinterpret_cast<const uint4*>(rec[w.t].n);
+  const uint32_t fe = w.fe;
+  const uint32_t q0 = sel4(rv, fe), q1 = sel4(rv, (fe + 1u) & 3u), q2 = sel4(rv, (fe + 2u) & 3u),
+                 q3 = sel4(rv, (fe + 3u) & 3u);
+  const float4 Q3 = xform(vr, ld_vert(vert, q3));
+  const float4 Q0 = q0 == w.h0 ? w.H0 : (q0 == w.h1 ? w.H1 : w.H2);
+  const float4 Q1 = q1 == w.h0 ? w.H0 : (q1 == w.h1 ? w.H1 : w.H2);
+  const float4 Q2 = q2 == w.h0 ? w.H0 : (q2 == w.h1 ? w.H1 : w.H2);
+  const Hit h1 = tri_test(Q1, Q2, Q3, ox, oy);
+  const Hit h2 = tri_test(Q2, Q3, Q0, ox, oy);
+  const Hit h3 = tri_test(Q3, Q0, Q1, ox, oy);
+  Step s;
+  if (h1.ok) {
+    s.i = 1; s.h = h1; s.A = Q1; s.B = Q2; s.C = Q3; s.a = q1; s.b = q2; s.c = q3;
+    s.nb = sel4(rn, (fe + 1u) & 3u);
+  } else if (h2.ok) {
+    s.i = 2; s.h = h2; s.A = Q2; s.B = Q3; s.C = Q0; s.a = q2; s.b = q3; s.c = q0;
+    s.nb = sel4(rn, (fe + 2u) & 3u);
+  } else {
+    s.i = h3.ok ? 3 : 0; s.h = h3; s.A = Q3; s.B = Q0; s.C = Q1; s.a = q3; s.b = q0; s.c = q1;
+    s.nb = sel4(rn, (fe + 3u) & 3u);
+  }
+  return s;
+}
+
+struct RayOut {
+  float4 c;
+  uint32_t steps;
+  uint32_t entry;
+  uint32_t anomaly;
+  uint32_t broken;
+};
+
+/* REF mode: the reference's per-ray loop (vfRayKernelCPU.cc:270-552) for an
+ * owned pixel, with its double-promoted sub-expressions. */
+DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
+  RayOut o;
+  o.anomaly = 0; o.broken = 0;
+  const ViewRegs vr = load_view(a.views, vi);
+  const uint32_t W = a.views[vi].W, H = a.views[vi].H, tfo = a.views[vi].tf * 256u;
+  const uint32_t gp = a.views[vi].pix_off + pix;
+  const uint32_t px = pix % W, py = pix / W;
+  const float ox = pix_to_obj(px, W), oy = pix_to_obj(py, H);
+  const uint32_t e = a.owner[gp] - 1u;
+  Walk w;
+  w.t = a.ext_int[e * 2];
+  w.fe = a.ext_int[e * 2 + 1];
+  {
+    const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[w.t].v);
+    w.h0 = sel4(rv, w.fe); w.h1 = sel4(rv, (w.fe + 1u) & 3u); w.h2 = sel4(rv, (w.fe + 2u) & 3u);
+    w.H0 = xform(vr, ld_vert(a.vert, w.h0));
+    w.H1 = xform(vr, ld_vert(a.vert, w.h1));
+    w.H2 = xform(vr, ld_vert(a.vert, w.h2));
+  }
+  const Planes cf = face_planes(w.H0, w.H1, w.H2);
+  float zPrev = plane_eval(cf.z, ox, oy);
+  const float sPrev = plane_eval(cf.s, ox, oy);
+  float4 cB = a.tf[tfo + (uint32_t)(255.0f * sPrev)];
+  float ax = 0.0f, ay = 0.0f, az = 0.0f, Tr = 1.0f;
+  uint32_t steps = 0;
+  while (true) {
+    if (++steps > VFR_MAX_STEPS) { o.anomaly = 1; break; }
+    const Step s = walk_step(w, a.rec, a.vert, vr, ox, oy);
+    if (s.i == 0) { o.anomaly = 1; break; }         /* :353 */
+    if (s.nb == VFR_NB_BROKEN) { o.broken = 1; break; } /* :365, not emulated */
+    const Planes nf = face_planes(s.A, s.B, s.C);
+    const float zNext = plane_eval(nf.z, ox, oy);
+    const float sNext = plane_eval(nf.s, ox, oy);
+    float dz = fabsf(zPrev - zNext);
+    dz = ((double)dz < 1e-12) ? 0.0f : dz;
+    if ((sNext < 0.0f) || (sNext > 1.0f)) break;    /* :446 */
+    dz = dz / a.max_edge;
+    const float4 cF = a.tf[tfo + (uint32_t)(255.0f * sNext)];
+    const float taux = cF.w * dz, tauy = cB.w * dz;
+    const float zeta = (float)exp(-((double)taux * 0.5 + (double)tauy * 0.5));
+    const float gx = (float)((double)taux / (1.0 + (double)taux));
+    const float gy = (float)((double)tauy / (1.0 + (double)tauy));
+    const float tcx = (float)((double)gx + (0.5 / 512));
+    const float tcy = (float)((double)gy + (0.5 / 512));
+    if ((tcx < 0.0f) || (tcx > 1.0f) || (tcy < 0.0f) || (tcy > 1.0f)) break; /* :468 */
+    uint32_t ix = (uint32_t)(tcx * 512.0f), iy = (uint32_t)(tcy * 512.0f);
+    ix = ix > 511u ? 511u : ix;
+    iy = iy > 511u ? 511u : iy;
+    const float ps = a.psi[ix * 512u + iy];
+    const float d = ps - zeta;
+    const float fx = (float)((double)cF.x * (1.0 - (double)ps) + (double)(cB.x * d));
+    const float fy = (float)((double)cF.y * (1.0 - (double)ps) + (double)(cB.y * d));
+    const float fz = (float)((double)cF.z * (1.0 - (double)ps) + (double)(cB.z * d));
+    const float fw = (float)(1.0 - (double)zeta);
+    ax = (float)((double)fx + (1.0 - (double)fw) * (double)ax);
+    ay = (float)((double)fy + (1.0 - (double)fw) * (double)ay);
+    az = (float)((double)fz + (1.0 - (double)fw) * (double)az);
+    Tr *= zeta;
+    cB = cF;
+    if (s.nb == VFR_NB_EXT) break;                   /* :535 */
+    w.t = s.nb >> 2; w.fe = s.nb & 3u;
+    w.H0 = s.A; w.H1 = s.B; w.H2 = s.C; w.h0 = s.a; w.h1 = s.b; w.h2 = s.c;
+    zPrev = zNext;
+  }
+  ax = ax > 1.0f ? 1.0f : ax;
+  ay = ay > 1.0f ? 1.0f : ay;
+  az = az > 1.0f ? 1.0f : az;
+  o.c = make_float4(ax, ay, az, 1.0f - Tr);
+  o.steps = steps > VFR_MAX_STEPS ? VFR_MAX_STEPS : steps;
+  o.entry = e;
+  return o;
+}
+
+/* FTB mode: entries front to back, segments composited front to back with
+ * the front = entry point, early termination at 1 - T >= term, re-entry at
+ * the first entry behind the last exit point. */
+DEV RayOut trace_ftb(const TravArgs& a, uint32_t vi, uint32_t pix) {
+  RayOut o;
+  o.anomaly = 0; o.broken = 0;
+  const ViewRegs vr = load_view(a.views, vi);
+  const uint32_t W = a.views[vi].W, H = a.views[vi].H, tfo = a.views[vi].tf * 256u;
+  const float k = a.views[vi].k;
+  const uint32_t gp = a.views[vi].pix_off + pix;
+  const uint32_t px = pix % W, py = pix / W;
+  const float ox = pix_to_obj(px, W), oy = pix_to_obj(py, H);
+  const bool persp = vr.proj == VFR_PROJ_PERSPECTIVE;
+  const float rayf = persp ? sqrtf(1.0f + (ox * ox + oy * oy) * (k * k)) : k;
+  uint32_t nk = a.cnt[gp];
+  nk = nk > VFR_K ? VFR_K : nk;
+  const uint64_t* ent = a.entries + (size_t)gp * VFR_K;
+  float ar = 0.0f, ag = 0.0f, ab = 0.0f, Tr = 1.0f;
+  uint32_t steps = 0, cur = 0;
+  bool done = false;
+  o.entry = (uint32_t)(ent[0] & 0xFFFFFFFFull);
+  while (cur < nk && !done) {
+    const uint32_t e = (uint32_t)(ent[cur] & 0xFFFFFFFFull);
+    Walk w;
+    w.t = a.ext_int[e * 2];
+    w.fe = a.ext_int[e * 2 + 1];
+    {
+      const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[w.t].v);
+      w.h0 = sel4(rv, w.fe); w.h1 = sel4(rv, (w.fe + 1u) & 3u); w.h2 = sel4(rv, (w.fe + 2u) & 3u);
+      w.H0 = xform(vr, ld_vert(a.vert, w.h0));
+      w.H1 = xform(vr, ld_vert(a.vert, w.h1));
+      w.H2 = xform(vr, ld_vert(a.vert, w.h2));
+    }
+    const Hit he = entry_test(w.H0, w.H1, w.H2, ox, oy);
+    float Zprev = bary(w.H0.z, w.H1.z, w.H2.z, he);
+    const float Sprev = bary(w.H0.w, w.H1.w, w.H2.w, he);
+    float dprev = 0.0f, sprev;
+    if (persp) { dprev = 1.0f / Zprev; sprev = clamp01(Sprev * dprev); }
+    else sprev = clamp01(Sprev);
+    float4 cP = a.tf[tfo + (uint32_t)(255.0f * sprev)];
+    float exitZ = Zprev;
+    while (true) {
+      if (++steps > VFR_MAX_STEPS) { o.anomaly = 1; done = true; break; }
+      const Step s = walk_step(w, a.rec, a.vert, vr, ox, oy);
+      if (s.i == 0) { o.anomaly = 1; done = true; break; }
+      const float Znext = bary(s.A.z, s.B.z, s.C.z, s.h);
+      const float Snext = bary(s.A.w, s.B.w, s.C.w, s.h);
+      float snext, dist;
+      if (persp) {
+        const float dn = 1.0f / Znext;
+        snext = clamp01(Snext * dn);
+        dist = fabsf(dn - dprev) * rayf;
+        dprev = dn;
+      } else {
+        snext = clamp01(Snext);
+        dist = fabsf(Zprev - Znext) * rayf;
+      }
+      const float dl = dist * a.inv_me;
+      const float4 cN = a.tf[tfo + (uint32_t)(255.0f * snext)];
+      const float tf_ = cP.w * dl, tb_ = cN.w * dl;
+      const float zeta = vfr_expf(-(tf_ + tb_) * 0.5f);
+      const float gf = tf_ / (1.0f + tf_), gb = tb_ / (1.0f + tb_);
+      const float ps = a.psi[psi_index(gf) * 512u + psi_index(gb)];
+      const float w1 = 1.0f - ps, w2 = ps - zeta;
+      ar += Tr * (cP.x * w1 + cN.x * w2);
+      ag += Tr * (cP.y * w1 + cN.y * w2);
+      ab += Tr * (cP.z * w1 + cN.z * w2);
+      Tr *= zeta;
+      cP = cN;
+      if (1.0f - Tr >= a.term) { done = true; break; }
+      if (s.nb == VFR_NB_EXT) { exitZ = Znext; break; }
+      if (s.nb == VFR_NB_BROKEN) { o.broken = 1; done = true; break; }
+      w.t = s.nb >> 2; w.fe = s.nb & 3u;
+      w.H0 = s.A; w.H1 = s.B; w.H2 = s.C; w.h0 = s.a; w.h1 = s.b; w.h2 = s.c;
+      Zprev = Znext;
+    }
+    if (done) break;
+    ++cur;
+    while (cur < nk && !(funkey((uint32_t)(ent[cur] >> 32)) < exitZ)) ++cur;
+  }
+  o.c = make_float4(ar, ag, ab, 1.0f - Tr);
+  o.steps = steps > VFR_MAX_STEPS ? VFR_MAX_STEPS : steps;
+  return o;
+}
+
+template <uint32_t MODE>
+DEV RayOut trace(const TravArgs& a, uint32_t vi, uint32_t pix) {
+  if (MODE == VFR_MODE_REF) return trace_ref(a, vi, pix);
+  return trace_ftb(a, vi, pix);
+}
+
+DEV void store_ray(const TravArgs& a, uint32_t vi, uint32_t pix, const RayOut& o) {
+  const uint32_t gp = a.views[vi].pix_off + pix;
+  a.rgba[gp] = o.c;
+  a.steps[gp] = o.steps;
+  a.entry_out[gp] = o.entry;
+}
+
+/* One ray per thread over the compacted ray list. */
+template <uint32_t MODE>
+__global__ __launch_bounds__(256) void k_trav(TravArgs a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = *a.n_rays;
+  unsigned long long st = 0, an = 0, br = 0;
+  if (i < n) {
+    const uint64_t ray = a.rays[i];
+    const uint32_t vi = (uint32_t)(ray >> 32), pix = (uint32_t)ray;
+    const RayOut o = trace<MODE>(a, vi, pix);
+    store_ray(a, vi, pix, o);
+    st = o.steps; an = o.anomaly; br = o.broken;
+  }
+  st = wave_sum_u64(st);
+  an = wave_sum_u64(an);
+  br = wave_sum_u64(br);
+  if ((threadIdx.x & 63u) == 0) {
+    if (st) atomicAdd(&a.stats[ST_STEPS], st);
+    if (an) atomicAdd(&a.stats[ST_ANOM], an);
+    if (br) atomicAdd(&a.stats[4], br);
+  }
+}
+
+int vfr_k_traverse(uint32_t mode, uint32_t proj_mask, const TravArgs& a, uint32_t max_rays,
+                   int persistent, uint32_t grid_waves, void* stream) {
+  (void)proj_mask; (void)persistent; (void)grid_waves;
+  if (max_rays == 0) return VFR_OK;
+  dim3 grid((max_rays + 255) / 256), block(256);
+  if (mode == VFR_MODE_REF)
+    hipLaunchKernelGGL(k_trav<VFR_MODE_REF>, grid, block, 0, (hipStream_t)stream, a);
+  else
+    hipLaunchKernelGGL(k_trav<VFR_MODE_FTB>, grid, block, 0, (hipStream_t)stream, a);
+  CHECK_HIP(hipGetLastError());
+  return VFR_OK;
+}
+
+/* ---------------------------------------------------------- preprocessing */
+
+/* Face keys: for slot idx = t*4+f the face's vertex ids (tet[(f+k)%4],
+ * k = 0..2, vfRayPreComp.cc:442-446) sorted a < b < c; key = (a << bits) | b,
+ * c kept aside.  Flags tets with a repeated vertex id. */
+__global__ void k_face_keys(const uint32_t* __restrict__ tets, uint32_t nT, uint32_t bits,
+                            uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                            uint32_t* __restrict__ cval, uint32_t* __restrict__ bad) {
+  uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (uint64_t)nT * 4) return;
+  uint32_t t = (uint32_t)(idx >> 2), f = (uint32_t)(idx & 3);
+  uint32_t a = tets[t * 4 + ((f + 0) & 3)], b = tets[t * 4 + ((f + 1) & 3)],
+           c = tets[t * 4 + ((f + 2) & 3)], d = tets[t * 4 + ((f + 3) & 3)];
+  if (f == 0 && (a == b || a == c || a == d || b == c || b == d || c == d)) atomicAdd(bad, 1u);
+  uint32_t x = min(a, min(b, c)), z = max(a, max(b, c)), y = a ^ b ^ c ^ x ^ z;
+  keys[idx] = ((uint64_t)x << bits) | y;
+  vals[idx] = (uint32_t)idx;
+  cval[idx] = z;
+}
+
+/* Adjacent match over the sorted keys: within the run of equal (a,b), the
+ * slots with the same c share the face; the neighbour is the smallest other
+ * tet id among them (buildCon's ascending search, vfRayPreComp.cc:450-490),
+ * else the tet itself (:448). */
+__global__ void k_match(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                        const uint32_t* __restrict__ cval, uint64_t n, uint32_t* __restrict__ con) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t key = keys[i];
+  const uint32_t idx = vals[i], c = cval[idx], t = idx >> 2;
+  uint32_t best = 0xFFFFFFFFu;
+  for (uint64_t j = i; j > 0 && keys[j - 1] == key; --j) {
+    uint32_t o = vals[j - 1];
+    if (cval[o] == c && (o >> 2) != t) best = min(best, o >> 2);
+  }
+  for (uint64_t j = i + 1; j < n && keys[j] == key; ++j) {
+    uint32_t o = vals[j];
+    if (cval[o] == c && (o >> 2) != t) best = min(best, o >> 2);
+  }
+  con[idx] = best == 0xFFFFFFFFu ? t : best;
+}
+
+__global__ void k_ext_flags(const uint32_t* __restrict__ con, uint64_t n, uint32_t* __restrict__ flag) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flag[i] = con[i] == (uint32_t)(i >> 2) ? 1u : 0u;
+}
+
+__global__ void k_ext_write(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                            uint64_t n, uint32_t* __restrict__ extf) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  uint32_t p = pos[i];
+  extf[p * 2] = (uint32_t)(i >> 2);
+  extf[p * 2 + 1] = (uint32_t)(i & 3);
+}
+
+DEV uint32_t spread10(uint32_t x) {
+  x &= 0x3FFu;
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+DEV uint32_t morton3(float x, float y, float z, const float* bb) {
+  float sx = (x - bb[0]) * bb[3], sy = (y - bb[1]) * bb[4], sz = (z - bb[2]) * bb[5];
+  uint32_t ix = (uint32_t)fminf(fmaxf(sx, 0.0f), 1023.0f);
+  uint32_t iy = (uint32_t)fminf(fmaxf(sy, 0.0f), 1023.0f);
+  uint32_t iz = (uint32_t)fminf(fmaxf(sz, 0.0f), 1023.0f);
+  return spread10(ix) | (spread10(iy) << 1) | (spread10(iz) << 2);
+}
+
+__global__ void k_vert_morton(const float4* __restrict__ v, uint32_t nV, const float* bb,
+                              uint32_t* __restrict__ code, uint32_t* __restrict__ id) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nV) return;
+  float4 p = v[i];
+  code[i] = morton3(p.x, p.y, p.z, bb);
+  id[i] = i;
+}
+
+__global__ void k_tet_morton(const float4* __restrict__ v, const uint32_t* __restrict__ tets,
+                             uint32_t nT, const float* bb, uint32_t* __restrict__ code,
+                             uint32_t* __restrict__ id) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nT) return;
+  float4 a = v[tets[i * 4]], b = v[tets[i * 4 + 1]], c = v[tets[i * 4 + 2]], d = v[tets[i * 4 + 3]];
+  code[i] = morton3((a.x + b.x + c.x + d.x) * 0.25f, (a.y + b.y + c.y + d.y) * 0.25f,
+                    (a.z + b.z + c.z + d.z) * 0.25f, bb);
+  id[i] = i;
+}
+
+__global__ void k_invert(const uint32_t* __restrict__ perm, uint32_t n, uint32_t* __restrict__ inv) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) inv[perm[i]] = i;
+}
+
+__global__ void k_gather_vert(const float4* __restrict__ v, const uint32_t* __restrict__ vperm,
+                              uint32_t nV, float4* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nV) out[i] = v[vperm[i]];
+}
+
+/* Internal tet record: vertex ids through the vertex permutation; neighbour
+ * words (internal id << 2 | the face of the neighbour that links back, the
+ * back-lookup of vfRayKernelCPU.cc:359-366). */
+__global__ void k_build_rec(const uint32_t* __restrict__ tets, const uint32_t* __restrict__ con,
+                            const uint32_t* __restrict__ perm, const uint32_t* __restrict__ tinv,
+                            const uint32_t* __restrict__ vinv, uint32_t nT,
+                            TetRec* __restrict__ rec) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nT) return;
+  uint32_t t = perm[i];
+  TetRec r;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r.v[k] = vinv[tets[t * 4 + k]];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    uint32_t u = con[t * 4 + f];
+    if (u == t) { r.n[f] = VFR_NB_EXT; continue; }
+    uint32_t kk = 4;
+    for (uint32_t q = 0; q < 4; ++q)
+      if (con[u * 4 + q] == t) { kk = q; break; }
+    r.n[f] = kk == 4 ? VFR_NB_BROKEN : ((tinv[u] << 2) | kk);
+  }
+  rec[i] = r;
+}
+
+__global__ void k_ext_internal(const uint32_t* __restrict__ extf, uint32_t n,
+                               const uint32_t* __restrict__ tinv, uint32_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i * 2] = tinv[extf[i * 2]];
+  out[i * 2 + 1] = extf[i * 2 + 1];
+}
+
+__global__ void k_iota(uint32_t* p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  hipError_t alloc(size_t n) { return hipMalloc(&p, n ? n : 16); }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+  void* release() { void* q = p; p = nullptr; return q; }
+};
+
+inline unsigned nblk(uint64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+#define ALLOC(b, n) CHECK_HIP((b).alloc(n))
+
+/* Builds every device structure of a mesh.  h_xyzs (already normalized if
+ * requested) and h_tets are caller order; h_con/h_extf optional (the .con /
+ * .extf caches). */
+int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
+                       const uint32_t* h_con, const uint32_t* h_extf, uint32_t n_ext_given,
+                       bool morton, void* stream_v) {
+  hipStream_t st = (hipStream_t)stream_v;
+  const uint32_t nV = m.n_verts, nT = m.n_tets;
+  const uint64_t nF = (uint64_t)nT * 4;
+  DevBuf d_xyzs, d_tets, d_con, d_bad;
+  ALLOC(d_xyzs, sizeof(float) * 4 * (size_t)nV);
+  ALLOC(d_tets, sizeof(uint32_t) * 4 * (size_t)nT);
+  ALLOC(d_con, sizeof(uint32_t) * 4 * (size_t)nT);
+  ALLOC(d_bad, sizeof(uint32_t) * 4);
+  CHECK_HIP(hipMemcpyAsync(d_xyzs.p, h_xyzs, sizeof(float) * 4 * (size_t)nV, hipMemcpyHostToDevice, st));
+  CHECK_HIP(hipMemcpyAsync(d_tets.p, h_tets, sizeof(uint32_t) * 4 * (size_t)nT, hipMemcpyHostToDevice, st));
+  CHECK_HIP(hipMemsetAsync(d_bad.p, 0, 16, st));
+  uint32_t n_ext = 0;
+  DevBuf d_extf;
+  if (h_con) {
+    CHECK_HIP(hipMemcpyAsync(d_con.p, h_con, sizeof(uint32_t) * nF, hipMemcpyHostToDevice, st));
+  } else {
+    /* sort-and-match */
+    uint32_t bits = 1;
+    while (bits < 32 && (1ull << bits) < (uint64_t)nV) bits++;
+    DevBuf k0, k1, v0, v1, cv, tmp;
+    ALLOC(k0, 8 * nF); ALLOC(k1, 8 * nF); ALLOC(v0, 4 * nF); ALLOC(v1, 4 * nF); ALLOC(cv, 4 * nF);
+    hipLaunchKernelGGL(k_face_keys, dim3(nblk(nF)), dim3(256), 0, st, d_tets.as<uint32_t>(), nT,
+                       bits, k0.as<uint64_t>(), v0.as<uint32_t>(), cv.as<uint32_t>(),
+                       d_bad.as<uint32_t>());
+    CHECK_HIP(hipGetLastError());
+    size_t tb = 0;
+    CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0.as<uint64_t>(), k1.as<uint64_t>(),
+                                                 v0.as<uint32_t>(), v1.as<uint32_t>(), (int)nF, 0,
+                                                 (int)(2 * bits), st));
+    ALLOC(tmp, tb);
+    CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0.as<uint64_t>(), k1.as<uint64_t>(),
+                                                 v0.as<uint32_t>(), v1.as<uint32_t>(), (int)nF, 0,
+                                                 (int)(2 * bits), st));
+    hipLaunchKernelGGL(k_match, dim3(nblk(nF)), dim3(256), 0, st, k1.as<uint64_t>(),
+                       v1.as<uint32_t>(), cv.as<uint32_t>(), nF, d_con.as<uint32_t>());
+    CHECK_HIP(hipGetLastError());
+  }
+  uint32_t bad = 0;
+  CHECK_HIP(hipMemcpyAsync(&bad, d_bad.p, 4, hipMemcpyDeviceToHost, st));
+  CHECK_HIP(hipStreamSynchronize(st));
+  if (bad) {
+    vfr_set_error("mesh has %u tetrahedra with a repeated vertex id", bad);
+    return VFR_EMESH;
+  }
+  if (h_extf) {
+    n_ext = n_ext_given;
+    ALLOC(d_extf, sizeof(uint32_t) * 2 * (size_t)n_ext);
+    CHECK_HIP(hipMemcpyAsync(d_extf.p, h_extf, sizeof(uint32_t) * 2 * (size_t)n_ext,
+                             hipMemcpyHostToDevice, st));
+  } else {
+    DevBuf flag, pos, tmp;
+    ALLOC(flag, 4 * nF); ALLOC(pos, 4 * nF);
+    hipLaunchKernelGGL(k_ext_flags, dim3(nblk(nF)), dim3(256), 0, st, d_con.as<uint32_t>(), nF,
+                       flag.as<uint32_t>());
+    CHECK_HIP(hipGetLastError());
+    size_t tb = 0;
+    CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag.as<uint32_t>(), pos.as<uint32_t>(),
+                                               (int)nF, st));
+    ALLOC(tmp, tb);
+    CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, flag.as<uint32_t>(), pos.as<uint32_t>(),
+                                               (int)nF, st));
+    uint32_t lastp = 0, lastf = 0;
+    CHECK_HIP(hipMemcpyAsync(&lastp, pos.as<uint32_t>() + nF - 1, 4, hipMemcpyDeviceToHost, st));
+    CHECK_HIP(hipMemcpyAsync(&lastf, flag.as<uint32_t>() + nF - 1, 4, hipMemcpyDeviceToHost, st));
+    CHECK_HIP(hipStreamSynchronize(st));
+    n_ext = lastp + lastf;
+    ALLOC(d_extf, sizeof(uint32_t) * 2 * (size_t)n_ext);
+    hipLaunchKernelGGL(k_ext_write, dim3(nblk(nF)), dim3(256), 0, st, flag.as<uint32_t>(),
+                       pos.as<uint32_t>(), nF, d_extf.as<uint32_t>());
+    CHECK_HIP(hipGetLastError());
+  }
+  /* orderings */
+  DevBuf perm, tinv, vperm, vinv;
+  ALLOC(perm, 4 * (size_t)nT); ALLOC(tinv, 4 * (size_t)nT);
+  ALLOC(vperm, 4 * (size_t)nV); ALLOC(vinv, 4 * (size_t)nV);
+  if (morton) {
+    float hb[6];
+    {
+      float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+      for (uint32_t i = 0; i < nV; ++i)
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = fminf(lo[a], h_xyzs[(size_t)i * 4 + a]);
+          hi[a] = fmaxf(hi[a], h_xyzs[(size_t)i * 4 + a]);
+        }
+      for (int a = 0; a < 3; ++a) {
+        hb[a] = lo[a];
+        hb[3 + a] = hi[a] > lo[a] ? 1023.0f / (hi[a] - lo[a]) : 0.0f;
+      }
+    }
+    DevBuf bb, code0, code1, id0, tmp;
+    ALLOC(bb, sizeof(hb));
+    CHECK_HIP(hipMemcpyAsync(bb.p, hb, sizeof(hb), hipMemcpyHostToDevice, st));
+    uint32_t nmax = nT > nV ? nT : nV;
+    ALLOC(code0, 4 * (size_t)nmax); ALLOC(code1, 4 * (size_t)nmax); ALLOC(id0, 4 * (size_t)nmax);
+    size_t tb = 0, tb2 = 0;
+    CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, code0.as<uint32_t>(), code1.as<uint32_t>(),
+                                                 id0.as<uint32_t>(), perm.as<uint32_t>(), (int)nT, 0, 30, st));
+    CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, code0.as<uint32_t>(), code1.as<uint32_t>(),
+                                                 id0.as<uint32_t>(), vperm.as<uint32_t>(), (int)nV, 0, 30, st));
+    ALLOC(tmp, tb > tb2 ? tb : tb2);
+    hipLaunchKernelGGL(k_vert_morton, dim3(nblk(nV)), dim3(256), 0, st, d_xyzs.as<float4>(), nV,
+                       bb.as<float>(), code0.as<uint32_t>(), id0.as<uint32_t>());
+    CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb2, code0.as<uint32_t>(), code1.as<uint32_t>(),
+                                                 id0.as<uint32_t>(), vperm.as<uint32_t>(), (int)nV, 0, 30, st));
+    hipLaunchKernelGGL(k_tet_morton, dim3(nblk(nT)), dim3(256), 0, st, d_xyzs.as<float4>(),
+                       d_tets.as<uint32_t>(), nT, bb.as<float>(), code0.as<uint32_t>(),
+                       id0.as<uint32_t>());
+    CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, code0.as<uint32_t>(), code1.as<uint32_t>(),
+                                                 id0.as<uint32_t>(), perm.as<uint32_t>(), (int)nT, 0, 30, st));
+    CHECK_HIP(hipStreamSynchronize(st));
+  } else {
+    hipLaunchKernelGGL(k_iota, dim3(nblk(nT)), dim3(256), 0, st, perm.as<uint32_t>(), nT);
+    hipLaunchKernelGGL(k_iota, dim3(nblk(nV)), dim3(256), 0, st, vperm.as<uint32_t>(), nV);
+  }
+  hipLaunchKernelGGL(k_invert, dim3(nblk(nT)), dim3(256), 0, st, perm.as<uint32_t>(), nT, tinv.as<uint32_t>());
+  hipLaunchKernelGGL(k_invert, dim3(nblk(nV)), dim3(256), 0, st, vperm.as<uint32_t>(), nV, vinv.as<uint32_t>());
+  DevBuf vert, rec, ext_int;
+  ALLOC(vert, sizeof(float) * 4 * (size_t)nV);
+  ALLOC(rec, sizeof(TetRec) * (size_t)nT);
+  ALLOC(ext_int, sizeof(uint32_t) * 2 * (size_t)n_ext);
+  hipLaunchKernelGGL(k_gather_vert, dim3(nblk(nV)), dim3(256), 0, st, d_xyzs.as<float4>(),
+                     vperm.as<uint32_t>(), nV, vert.as<float4>());
+  hipLaunchKernelGGL(k_build_rec, dim3(nblk(nT)), dim3(256), 0, st, d_tets.as<uint32_t>(),
+                     d_con.as<uint32_t>(), perm.as<uint32_t>(), tinv.as<uint32_t>(),
+                     vinv.as<uint32_t>(), nT, rec.as<TetRec>());
+  if (n_ext)
+    hipLaunchKernelGGL(k_ext_internal, dim3(nblk(n_ext)), dim3(256), 0, st, d_extf.as<uint32_t>(),
+                       n_ext, tinv.as<uint32_t>(), ext_int.as<uint32_t>());
+  CHECK_HIP(hipGetLastError());
+  CHECK_HIP(hipStreamSynchronize(st));
+  m.n_ext = n_ext;
+  m.vert = vert.as<float>(); vert.release();
+  m.rec = rec.as<TetRec>(); rec.release();
+  m.ext_int = ext_int.as<uint32_t>(); ext_int.release();
+  m.extf = d_extf.as<uint32_t>(); d_extf.release();
+  m.perm = perm.as<uint32_t>(); perm.release();
+  m.vperm = vperm.as<uint32_t>(); vperm.release();
+  m.con = d_con.as<uint32_t>(); d_con.release();
+  m.bytes = sizeof(float) * 4ull * nV + sizeof(TetRec) * (uint64_t)nT + 16ull * n_ext +
+            4ull * nT + 4ull * nV + 16ull * nT;
+  return VFR_OK;
+}
