@@ -1,0 +1,177 @@
+"""VF-Ray throughput benchmark on one MI355X.
+
+Default workload (`batch64`): 128^3 hex grid x 6 = 12,582,912 tets, 64 seeded
+perspective views on the r = 2.5 sphere with sides drawn from
+{256, 384, 512, 768, 1024}, front-to-back compositing with early termination
+at alpha 0.99.  One step = rendering all 64 views.  `value` is the batched
+renderer's Mrays/s (sum of W*H over the views / wall time of the step, inputs
+resident on the device); the plain per-view renderer (one launch set and one
+frame sync per view) is timed on the same views and reported beside it.
+
+`plain_gpu`: 64^3 grid (1,572,864 tets), one 1024^2 perspective view.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+B_STEP, B_RAY = 96, 48  # algorithmic bytes per tet step / per active ray
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(x, t, tfa, views, seconds: float) -> dict:
+    """The CPU oracle's front-to-back port (OpenMP) on the first views of the
+    same workload until `seconds` of work; Mrays/s over the rays it did."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import vfray_amd as V
+    xn = O.normalize(x)
+    con, _ = O.connectivity(t, x.shape[0])
+    ext = O.extf(con)
+    lmt = O.lmt(xn, t)
+    psi = V.psi_generate()
+    rays, el, done = 0, 0.0, 0
+    for v in views:
+        proj, M, k = V.view_matrix(v)
+        t0 = time.perf_counter()
+        O.render_ftb(xn, t, con, ext, tfa, psi, lmt, proj, M, k, v.width, v.height)
+        el += time.perf_counter() - t0
+        rays += v.width * v.height
+        done += 1
+        if el >= seconds:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": rays / el / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} of {len(views)} views ({rays} rays), CPU oracle front-to-back "
+                      f"restatement, FP32, OpenMP {threads} threads, {el:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="batch64", choices=["batch64", "plain_gpu"])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the renderer is "
+              "single-GPU; views would shard across GPUs, see DESIGN.md", file=sys.stderr)
+        return 2
+
+    import vfray_amd as V
+    from vfray_amd import workloads as WL
+
+    t0 = time.time()
+    if args.workload == "batch64":
+        x, t, tfa, seed = WL.batch64()
+        nviews, sides = 64, list(WL.SIDES_BATCH)
+    else:
+        x, t, tfa, seed = WL.plain_gpu()
+        nviews, sides = 1, [1024]
+    log(f"mesh {x.shape[0]} verts {t.shape[0]} tets generated in {time.time() - t0:.1f}s")
+    ctx = V.Context(0)
+    t0 = time.time()
+    mesh = ctx.mesh(x, t)
+    log(f"device mesh built in {time.time() - t0:.1f}s: {mesh.info}")
+    tf = ctx.tf(tfa)
+    views = WL.sphere_views(seed, nviews, sides, tf)
+    total_rays = sum(v.width * v.height for v in views)
+
+    pb = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, use_graph=args.graph)
+    for _ in range(args.warmup):
+        pb.execute()
+    log("batched warm-up done")
+    st_list = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st_list.append(pb.execute())
+    wall = (time.perf_counter() - t0) / args.steps
+    log(f"batched: {wall * 1e3:.2f} ms/step")
+    st = st_list[-1]
+    ms_trav = float(np.mean([s["ms_traversal"] for s in st_list])) if not args.graph else None
+    ms_evt = float(np.mean([s["ms_total"] for s in st_list]))
+
+    plain = None
+    if not args.no_plain:
+        pp = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+        for _ in range(args.warmup):
+            pp.execute()
+        sp = []
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            sp.append(pp.execute())
+        pwall = (time.perf_counter() - t1) / args.steps
+        plain = {"mrays_per_s": total_rays / pwall / 1e6, "ms_per_step": pwall * 1e3,
+                 "frame_p50_ms": float(np.median([s["ms_frame_p50"] for s in sp])),
+                 "frame_p99_ms": float(np.median([s["ms_frame_p99"] for s in sp]))}
+        same = all(np.array_equal(pb.read(i)["rgba"], pp.read(i)["rgba"]) for i in range(min(3, nviews)))
+        plain["batched_equals_plain"] = bool(same)
+        log(f"plain: {pwall * 1e3:.2f} ms/step")
+        pp.close()
+
+    steps_per_launch = st["steps"]
+    rays_active = st["active_rays"]
+    alg_bytes = B_STEP * steps_per_launch + B_RAY * rays_active
+    roof = None
+    if ms_trav:
+        ach = alg_bytes / (ms_trav * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
+                "kernel": "k_trav<FTB>", "kernel_ms": round(ms_trav, 3),
+                "alg_bytes": int(alg_bytes)}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(x, t, tfa, views, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.3f} Mrays/s")
+    value = total_rays / wall / 1e6
+    out = {
+        "metric": "Mrays/s (batched renderer, front-to-back, PSNR>=50dB vs FP64)",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "tets": int(t.shape[0]), "views": nviews,
+                   "rays_per_step": int(total_rays), "active_rays": int(rays_active),
+                   "tet_steps": int(steps_per_launch), "term_alpha": 0.99, "graph": args.graph},
+        "event_ms_per_step": round(ms_evt, 3),
+        "active_mrays_per_s": round(rays_active / wall / 1e6, 2),
+        "gsteps_per_s": round(steps_per_launch / wall / 1e9, 3),
+        "plain": plain,
+        "speedup_vs_plain": round(value / plain["mrays_per_s"], 3) if plain else None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    pb.close()
+    mesh.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

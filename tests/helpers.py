@@ -1,0 +1,41 @@
+"""Shared inputs and comparisons for the parity tests."""
+from __future__ import annotations
+
+import numpy as np
+
+import vfray_amd as V
+from vfray_amd import workloads as WL
+import oracle as O
+
+
+def c1_inputs(n: int = 16):
+    """The 512^2 orthographic workload: rotated mesh, its normalized copy,
+    oracle connectivity and limits, TF."""
+    x, t, tf, side = WL.plain_cpu(n)
+    xn = O.normalize(x)
+    con, _ = O.connectivity(t, x.shape[0])
+    ext = O.extf(con)
+    lm = O.lmt(xn, t)
+    return dict(x=x, t=t, tf=tf, side=side, xn=xn, con=con, ext=ext, lmt=lm)
+
+
+def grid_inputs(n: int, seed: int, variant: int = 0):
+    x, t = V.synth_grid(n, seed, variant)
+    xn = O.normalize(x)
+    con, _ = O.connectivity(t, x.shape[0])
+    ext = O.extf(con)
+    lm = O.lmt(xn, t)
+    return dict(x=x, t=t, tf=V.synth_tf(seed), xn=xn, con=con, ext=ext, lmt=lm)
+
+
+def psnr(a: np.ndarray, b: np.ndarray, mask: np.ndarray) -> float:
+    """10 log10(1/MSE) over RGBA in [0,1] on the masked pixels."""
+    d = (a[mask].astype(np.float64) - b[mask].astype(np.float64))
+    mse = float((d * d).mean()) if d.size else 0.0
+    return float("inf") if mse == 0 else 10.0 * np.log10(1.0 / mse)
+
+
+def oracle_ftb(inp, view: V.View, psi, term=0.99, fp64=False):
+    proj, M, k = V.view_matrix(view)
+    return O.render_ftb(inp["xn"], inp["t"], inp["con"], inp["ext"], inp["tf"], psi, inp["lmt"],
+                        proj, M, k, view.width, view.height, term=term, fp64=fp64)

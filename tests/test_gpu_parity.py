@@ -1,0 +1,188 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Entry faces, entry tets and step counts must be bit-exact; RGBA within
+2e-3 max-abs per channel; PSNR >= 50 dB against the FP64 re-integration.
+"""
+import numpy as np
+import pytest
+
+import vfray_amd as V
+import oracle as O
+from helpers import c1_inputs, grid_inputs, oracle_ftb, psnr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 2e-3
+
+
+@pytest.fixture(scope="module")
+def psi():
+    return V.psi_generate()
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return c1_inputs()
+
+
+def _ext_tet(ext, entry):
+    out = np.full(entry.shape, V.NO_ID, np.uint32)
+    m = entry != V.NO_ID
+    out[m] = ext[entry[m], 0]
+    return out
+
+
+@pytest.mark.parametrize("morton", [False, True])
+def test_connectivity_matches_oracle(ctx, c1, morton):
+    print(f"connectivity 16^3 morton={morton}", flush=True)
+    m = ctx.mesh(c1["x"], c1["t"], morton=morton)
+    con, ext, lmt = m.connectivity()
+    assert np.array_equal(con, c1["con"])
+    assert np.array_equal(ext, c1["ext"])
+    assert np.array_equal(lmt.view(np.uint32), c1["lmt"].view(np.uint32))
+    assert np.array_equal(m.vertices().view(np.uint32), c1["xn"].view(np.uint32))
+    m.close()
+
+
+def test_connectivity_given_files(ctx, c1):
+    """con/extf/lmt passed in (the .con/.extf/.lmt cache path) are used as is."""
+    m = ctx.mesh(c1["x"], c1["t"], con=c1["con"], extf=c1["ext"], lmt=c1["lmt"])
+    con, ext, lmt = m.connectivity()
+    assert np.array_equal(con, c1["con"]) and np.array_equal(ext, c1["ext"])
+    m.close()
+
+
+def test_connectivity_32(ctx):
+    inp = grid_inputs(32, 0x5EED0100)
+    print("connectivity 32^3", flush=True)
+    m = ctx.mesh(inp["x"], inp["t"])
+    con, ext, lmt = m.connectivity()
+    assert np.array_equal(con, inp["con"])
+    assert np.array_equal(ext, inp["ext"])
+    assert np.array_equal(lmt.view(np.uint32), inp["lmt"].view(np.uint32))
+    m.close()
+
+
+def test_ref_mode_c1(ctx, c1, psi):
+    """512^2 orthographic REF render: the reference's semantics end to end."""
+    print("REF 16^3 512x512", flush=True)
+    orc = O.render_ref(c1["xn"], c1["t"], c1["con"], c1["ext"], c1["tf"], psi, c1["lmt"], 512, 512)
+    assert orc["face_returns"] == 0
+    m = ctx.mesh(c1["x"], c1["t"])
+    tf = ctx.tf(c1["tf"])
+    outs, st = ctx.render(m, [V.ortho_view(512, 512, tf)], mode=V.MODE_REF)
+    g = outs[0]
+    assert np.array_equal(g["entry_ext"], orc["entry_ext"])
+    assert np.array_equal(g["entry_tet"], _ext_tet(c1["ext"], orc["entry_ext"]))
+    assert np.array_equal(g["steps"], orc["steps"])
+    assert np.abs(g["rgba"] - orc["rgba"]).max() <= TOL
+    assert np.abs(V.to_rgb8(g["rgba"]).astype(int) - orc["rgb8"]).max() <= 1
+    assert st["steps"] == int(orc["steps"].sum())
+    assert st["active_rays"] == int((orc["entry_ext"] != V.NO_ID).sum())
+    m.close()
+
+
+@pytest.mark.parametrize("schedule", [V.SCHED_PER_VIEW, V.SCHED_BATCHED])
+def test_ftb_ortho_c1(ctx, c1, psi, schedule):
+    print(f"FTB ortho 16^3 schedule={schedule}", flush=True)
+    m = ctx.mesh(c1["x"], c1["t"])
+    tf = ctx.tf(c1["tf"])
+    views = [V.ortho_view(512, 512, tf), V.ortho_view(300, 200, tf)]
+    outs, st = ctx.render(m, views, mode=V.MODE_FTB, schedule=schedule)
+    for v, g in zip(views, outs):
+        orc = oracle_ftb(c1, v, psi, fp64=True)
+        assert orc["anomalies"] == 0
+        assert np.array_equal(g["entry_ext"], orc["entry_ext"])
+        assert np.array_equal(g["steps"], orc["steps"])
+        assert np.abs(g["rgba"] - orc["rgba"]).max() <= TOL
+        cov = orc["entry_ext"] != V.NO_ID
+        assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
+    m.close()
+
+
+def _sphere_views(tf, sides, seed=0x5EED0101):
+    from vfray_amd import workloads as WL
+    return WL.sphere_views(seed, len(sides), list(sides), tf)
+
+
+def test_ftb_perspective_32(ctx, psi):
+    inp = grid_inputs(32, 0x5EED0102)
+    m = ctx.mesh(inp["x"], inp["t"])
+    tf = ctx.tf(inp["tf"])
+    from vfray_amd import workloads as WL
+    views = WL.sphere_views(0x5EED0102, 3, [384], tf)
+    print("FTB perspective 32^3 3x384^2", flush=True)
+    outs, st = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+    for v, g in zip(views, outs):
+        orc = oracle_ftb(inp, v, psi, fp64=True)
+        assert orc["anomalies"] == 0
+        assert np.array_equal(g["entry_ext"], orc["entry_ext"])
+        assert np.array_equal(g["steps"], orc["steps"])
+        assert np.abs(g["rgba"] - orc["rgba"]).max() <= TOL
+        cov = orc["entry_ext"] != V.NO_ID
+        assert cov.sum() > 0.2 * cov.size
+        assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
+    m.close()
+
+
+def test_batched_equals_per_view(ctx):
+    """Each view rendered in a batch equals the same view rendered alone."""
+    inp = grid_inputs(24, 0x5EED0103)
+    m = ctx.mesh(inp["x"], inp["t"])
+    tf = ctx.tf(inp["tf"])
+    from vfray_amd import workloads as WL
+    views = WL.sphere_views(0x5EED0103, 5, [256, 384, 512], tf)
+    print("batched vs per-view 24^3", flush=True)
+    a, _ = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+    b, _ = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+    for i, v in enumerate(views):
+        alone, _ = ctx.render(m, [v], mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+        for key in ("rgba", "entry_ext", "steps"):
+            assert np.array_equal(a[i][key], b[i][key])
+            assert np.array_equal(a[i][key], alone[0][key])
+    m.close()
+
+
+def test_graph_replays(ctx):
+    """A captured batched plan replayed several times, with eager renders in
+    between, gives the eager result every time."""
+    inp = grid_inputs(20, 0x5EED0104)
+    m = ctx.mesh(inp["x"], inp["t"])
+    tf = ctx.tf(inp["tf"])
+    from vfray_amd import workloads as WL
+    views = WL.sphere_views(0x5EED0104, 3, [256, 320], tf)
+    print("graph replays 20^3", flush=True)
+    ref, _ = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+    p = ctx.plan(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, use_graph=True)
+    for rep in range(3):
+        p.execute()
+        for i in range(len(views)):
+            got = p.read(i)
+            for key in ("rgba", "entry_ext", "steps"):
+                assert np.array_equal(got[key], ref[i][key]), (rep, i, key)
+        ctx.render(m, views[:1], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+    p.close()
+    m.close()
+
+
+def test_ftb_c2_full(ctx, psi):
+    """64^3 grid (1.57M tets), one 1024^2 perspective view, alpha 0.99."""
+    from vfray_amd import workloads as WL
+    x, t, tfa, seed = WL.plain_gpu()
+    inp = dict(x=x, t=t, tf=tfa, xn=O.normalize(x))
+    inp["con"], _ = O.connectivity(t, x.shape[0])
+    inp["ext"] = O.extf(inp["con"])
+    inp["lmt"] = O.lmt(inp["xn"], t)
+    m = ctx.mesh(x, t)
+    tf = ctx.tf(tfa)
+    v = WL.sphere_views(seed, 1, [1024], tf)[0]
+    print("FTB 64^3 1024x1024", flush=True)
+    outs, st = ctx.render(m, [v], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+    g = outs[0]
+    orc = oracle_ftb(inp, v, psi, fp64=True)
+    assert np.array_equal(g["entry_ext"], orc["entry_ext"])
+    assert np.array_equal(g["steps"], orc["steps"])
+    assert np.abs(g["rgba"] - orc["rgba"]).max() <= TOL
+    cov = orc["entry_ext"] != V.NO_ID
+    assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
+    m.close()
