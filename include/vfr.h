@@ -140,8 +140,12 @@ typedef struct vfr_render_opts {
   uint32_t refill;      /* dead lanes that trigger a wave refill, 0 = tuned */
   uint32_t use_graph;   /* capture the batched launch set in a hipGraph    */
   uint32_t max_entries; /* FTB entry-list slots per pixel, 0 = 8           */
-  uint32_t reserved;
+  uint32_t flags;       /* VFR_RENDER_*                                    */
 } vfr_render_opts;
+
+/* Batched FTB renders use the persistent work-queue kernel unless this flag
+ * asks for one ray per thread over the compacted ray list. */
+#define VFR_RENDER_SIMPLE_KERNEL 0x1u
 
 typedef struct vfr_frame_out {
   float* rgba;          /* W*H*4, may be NULL                              */
@@ -170,6 +174,12 @@ int vfr_plan_create(vfr_ctx* ctx, const vfr_mesh* mesh, const vfr_view* views,
 int vfr_plan_execute(vfr_plan* plan, vfr_stats* stats);
 int vfr_plan_read(vfr_plan* plan, int view, vfr_frame_out* out);
 void vfr_plan_destroy(vfr_plan* plan);
+
+/* Times candidate persistent-kernel schedules (waves per CU x refill
+ * threshold) on the plan itself and keeps the fastest; returns the choice
+ * and its time.  Results are cached per (tets, rays, views) in the process. */
+int vfr_plan_autotune(vfr_plan* plan, uint32_t* waves_per_cu, uint32_t* refill,
+                      double* best_ms);
 
 /* create + execute + read + destroy. outs has n_views entries. */
 int vfr_render(vfr_ctx* ctx, const vfr_mesh* mesh, const vfr_view* views,

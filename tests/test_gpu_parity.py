@@ -133,12 +133,17 @@ def test_batched_equals_per_view(ctx):
     from vfray_amd import workloads as WL
     views = WL.sphere_views(0x5EED0103, 5, [256, 384, 512], tf)
     print("batched vs per-view 24^3", flush=True)
-    a, _ = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
-    b, _ = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+    a, sa = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+    b, sb = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+    c, sc = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, simple_kernel=True)
+    d, sd = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, waves_per_cu=4, refill=1)
+    assert sa["steps"] == sb["steps"] == sc["steps"] == sd["steps"]
     for i, v in enumerate(views):
         alone, _ = ctx.render(m, [v], mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
         for key in ("rgba", "entry_ext", "steps"):
             assert np.array_equal(a[i][key], b[i][key])
+            assert np.array_equal(a[i][key], c[i][key])
+            assert np.array_equal(a[i][key], d[i][key])
             assert np.array_equal(a[i][key], alone[0][key])
     m.close()
 
@@ -185,4 +190,23 @@ def test_ftb_c2_full(ctx, psi):
     assert np.abs(g["rgba"] - orc["rgba"]).max() <= TOL
     cov = orc["entry_ext"] != V.NO_ID
     assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
+    m.close()
+
+
+def test_autotune_keeps_results(ctx):
+    """The autotuner's chosen schedule renders the same bits."""
+    inp = grid_inputs(16, 0x5EED0105)
+    m = ctx.mesh(inp["x"], inp["t"])
+    tf = ctx.tf(inp["tf"])
+    from vfray_amd import workloads as WL
+    views = WL.sphere_views(0x5EED0105, 4, [128, 192], tf)
+    print("autotune 16^3", flush=True)
+    ref, _ = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+    p = ctx.plan(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+    choice = p.autotune()
+    assert 1 <= choice["waves_per_cu"] <= 64 and 1 <= choice["refill"] <= 64
+    p.execute()
+    for i in range(len(views)):
+        assert np.array_equal(p.read(i)["rgba"], ref[i]["rgba"])
+    p.close()
     m.close()

@@ -37,7 +37,7 @@ class View(C.Structure):
 class RenderOpts(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("schedule", C.c_uint32), ("term_alpha", C.c_float),
                 ("waves_per_cu", C.c_uint32), ("refill", C.c_uint32), ("use_graph", C.c_uint32),
-                ("max_entries", C.c_uint32), ("reserved", C.c_uint32)]
+                ("max_entries", C.c_uint32), ("flags", C.c_uint32)]
 
 
 class FrameOut(C.Structure):
@@ -92,6 +92,7 @@ def lib() -> C.CDLL:
             "vfr_plan_execute": (C.c_int, [P, C.POINTER(Stats)]),
             "vfr_plan_read": (C.c_int, [P, C.c_int, C.POINTER(FrameOut)]),
             "vfr_plan_destroy": (None, [P]),
+            "vfr_plan_autotune": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
             "vfr_render": (C.c_int, [P, P, C.POINTER(View), C.c_int, C.POINTER(RenderOpts), C.POINTER(FrameOut), C.POINTER(Stats)]),
             "vfr_free": (None, [P]),
             "vfr_read_off": (C.c_int, [C.c_char_p, C.POINTER(f32p), u32p, C.POINTER(u32p), u32p]),
@@ -300,8 +301,9 @@ class Context:
         return Mesh(self, xyzs, tets, con, extf, lmt, normalize, morton)
 
     def plan(self, mesh: "Mesh", views, mode=MODE_FTB, schedule=SCHED_BATCHED, term_alpha=0.99,
-             use_graph=False, waves_per_cu=0, refill=0) -> "Plan":
-        return Plan(self, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill)
+             use_graph=False, waves_per_cu=0, refill=0, simple_kernel=False) -> "Plan":
+        return Plan(self, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
+                    simple_kernel)
 
     def render(self, mesh: "Mesh", views, **kw) -> tuple[list[dict], dict]:
         p = self.plan(mesh, views, **kw)
@@ -348,13 +350,20 @@ class Mesh:
 
 
 class Plan:
-    def __init__(self, ctx, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill):
+    def __init__(self, ctx, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
+                 simple_kernel=False):
         self.ctx, self.mesh = ctx, mesh
         self.views = list(views)
         arr = (View * len(self.views))(*self.views)
-        o = RenderOpts(mode, schedule, term_alpha, waves_per_cu, refill, 1 if use_graph else 0, 0, 0)
+        o = RenderOpts(mode, schedule, term_alpha, waves_per_cu, refill, 1 if use_graph else 0, 0,
+                       1 if simple_kernel else 0)
         self._p = C.c_void_p()
         _check(lib().vfr_plan_create(ctx._p, mesh._p, arr, len(self.views), C.byref(o), C.byref(self._p)))
+
+    def autotune(self) -> dict:
+        w, r, ms = C.c_uint32(), C.c_uint32(), C.c_double()
+        _check(lib().vfr_plan_autotune(self._p, C.byref(w), C.byref(r), C.byref(ms)))
+        return {"waves_per_cu": w.value, "refill": r.value, "traversal_ms": ms.value}
 
     def execute(self) -> dict:
         st = Stats()

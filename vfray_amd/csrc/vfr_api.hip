@@ -254,7 +254,8 @@ struct vfr_plan {
   float4* rgba = nullptr;
   uint32_t* steps = nullptr;
   uint32_t* entry = nullptr;
-  uint32_t* counters = nullptr;           /* [n_views + 1] ray counts */
+  uint32_t* counters = nullptr;           /* [n_views + 2]: per-view ray counts,
+                                             batched ray count, work-queue head */
   unsigned long long* stats = nullptr;    /* [ST_N] */
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   std::vector<hipEvent_t> fev;            /* per-view frame events */
@@ -349,7 +350,7 @@ static int enqueue_batched(vfr_plan* p) {
   hipStream_t st = p->ctx->stream;
   const vfr_mesh* m = p->mesh;
   CHECK_RC(vfr_k_clear_u32(p->owner, p->n_pix, st));
-  CHECK_RC(vfr_k_clear_u32(p->counters, p->n_views + 1, st));
+  CHECK_RC(vfr_k_clear_u32(p->counters, p->n_views + 2, st));
   CHECK_RC(vfr_k_clear_u32((uint32_t*)p->stats, 2 * ST_N, st));
   if (p->opts.mode == VFR_MODE_REF)
     CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, m->d, p->owner, st));
@@ -378,12 +379,21 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   a.steps = p->steps;
   a.entry_out = p->entry;
   a.stats = p->stats;
-  a.queue = p->counters;
+  a.queue = p->counters + p->n_views + 1;
   a.max_edge = m->d.lmt[0];
   a.inv_me = 1.0f / m->d.lmt[0];
   a.term = p->opts.term_alpha;
   a.refill = p->opts.refill;
   return a;
+}
+
+/* The batched traversal: persistent work-queue kernel (FTB) or one ray per
+ * thread (REF, or when asked). */
+static int launch_batched_traverse(vfr_plan* p) {
+  TravArgs a = trav_args(p, p->rays, p->counters + p->n_views);
+  const bool persist = p->opts.mode == VFR_MODE_FTB && !(p->opts.flags & VFR_RENDER_SIMPLE_KERNEL);
+  const uint32_t waves = p->opts.waves_per_cu * (uint32_t)p->ctx->n_cu;
+  return vfr_k_traverse(p->opts.mode, 0, a, (uint32_t)p->n_pix, persist ? 1 : 0, waves, p->ctx->stream);
 }
 
 extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* views, int n_views,
@@ -398,6 +408,12 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   if (o.term_alpha > 1.0f) { vfr_set_error("term_alpha %g > 1", (double)o.term_alpha); return VFR_EINVAL; }
   if (o.max_entries == 0) o.max_entries = VFR_K;
   if (o.max_entries != VFR_K) { vfr_set_error("max_entries must be %u", VFR_K); return VFR_EINVAL; }
+  if (o.waves_per_cu == 0) o.waves_per_cu = 16;
+  if (o.refill == 0) o.refill = 32;
+  if (o.waves_per_cu > 64 || o.refill > 64) {
+    vfr_set_error("waves_per_cu %u / refill %u out of range (<= 64)", o.waves_per_cu, o.refill);
+    return VFR_EINVAL;
+  }
   vfr_plan* p = new (std::nothrow) vfr_plan;
   if (!p) { vfr_set_error("out of memory"); return VFR_ENOMEM; }
   p->ctx = c;
@@ -444,7 +460,7 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   PALLOC(p->rgba, sizeof(float4) * pix);
   PALLOC(p->steps, sizeof(uint32_t) * pix);
   PALLOC(p->entry, sizeof(uint32_t) * pix);
-  PALLOC(p->counters, sizeof(uint32_t) * (n_views + 1));
+  PALLOC(p->counters, sizeof(uint32_t) * (n_views + 2));
   PALLOC(p->stats, sizeof(unsigned long long) * ST_N);
 #undef PALLOC
   if (rc == VFR_OK && hipMemcpy(p->views, p->hv.data(), sizeof(DevView) * n_views, hipMemcpyHostToDevice) != hipSuccess) {
@@ -465,8 +481,7 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
       rc = VFR_EHIP;
     } else {
       int rc2 = enqueue_batched(p);
-      TravArgs a = trav_args(p, p->rays, p->counters + p->n_views);
-      if (rc2 == VFR_OK) rc2 = vfr_k_traverse(o.mode, 0, a, (uint32_t)p->n_pix, 0, 0, st);
+      if (rc2 == VFR_OK) rc2 = launch_batched_traverse(p);
       hipError_t e = hipStreamEndCapture(st, &p->graph);
       if (rc2 != VFR_OK) rc = rc2;
       else if (e != hipSuccess || hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0) != hipSuccess) {
@@ -503,8 +518,7 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
     } else {
       CHECK_RC(enqueue_batched(p));
       CHECK_HIP(hipEventRecord(p->ev[1], st));
-      TravArgs a = trav_args(p, p->rays, p->counters + p->n_views);
-      CHECK_RC(vfr_k_traverse(p->opts.mode, 0, a, (uint32_t)p->n_pix, 0, 0, st));
+      CHECK_RC(launch_batched_traverse(p));
     }
     CHECK_HIP(hipEventRecord(p->ev[2], st));
     CHECK_HIP(hipEventSynchronize(p->ev[2]));
@@ -519,7 +533,7 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
   } else {
     const vfr_mesh* m = p->mesh;
     CHECK_RC(vfr_k_clear_u32((uint32_t*)p->stats, 2 * ST_N, st));
-    CHECK_RC(vfr_k_clear_u32(p->counters, p->n_views + 1, st));
+    CHECK_RC(vfr_k_clear_u32(p->counters, p->n_views + 2, st));
     std::vector<double> frame(p->n_views), vis(p->n_views);
     CHECK_HIP(hipEventRecord(p->ev[0], st));
     for (uint32_t v = 0; v < p->n_views; ++v) {
@@ -558,7 +572,7 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
   p->executed = true;
   if (stats) {
     unsigned long long s[ST_N];
-    std::vector<uint32_t> cnt(p->n_views + 1);
+    std::vector<uint32_t> cnt(p->n_views + 2);
     CHECK_HIP(hipMemcpy(s, p->stats, sizeof(s), hipMemcpyDeviceToHost));
     CHECK_HIP(hipMemcpy(cnt.data(), p->counters, sizeof(uint32_t) * cnt.size(), hipMemcpyDeviceToHost));
     uint64_t active = 0;
@@ -616,6 +630,68 @@ extern "C" int vfr_plan_read(vfr_plan* p, int view, vfr_frame_out* out) {
   }
   if (out->steps)
     CHECK_HIP(hipMemcpy(out->steps, p->steps + hv.pix_off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  return VFR_OK;
+}
+
+struct TuneKey {
+  uint32_t n_tets;
+  uint64_t n_pix;
+  uint32_t n_views;
+  bool operator==(const TuneKey& o) const { return n_tets == o.n_tets && n_pix == o.n_pix && n_views == o.n_views; }
+};
+static std::mutex g_tune_mu;
+static std::vector<std::pair<TuneKey, std::pair<uint32_t, uint32_t>>> g_tune;
+
+extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* wpc_out, uint32_t* refill_out, double* best_out) {
+  if (!p) { vfr_set_error("vfr_plan_autotune: null plan"); return VFR_EINVAL; }
+  if (p->opts.schedule != VFR_SCHED_BATCHED || p->opts.mode != VFR_MODE_FTB) {
+    vfr_set_error("autotune applies to batched front-to-back plans");
+    return VFR_ESTATE;
+  }
+  if (p->gexec) { vfr_set_error("autotune before graph capture (plan has a graph)"); return VFR_ESTATE; }
+  TuneKey key{p->mesh->d.n_tets, p->n_pix, p->n_views};
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    for (auto& e : g_tune)
+      if (e.first == key) {
+        p->opts.waves_per_cu = e.second.first;
+        p->opts.refill = e.second.second;
+        if (wpc_out) *wpc_out = p->opts.waves_per_cu;
+        if (refill_out) *refill_out = p->opts.refill;
+        if (best_out) *best_out = -1.0;
+        return VFR_OK;
+      }
+  }
+  static const uint32_t kW[] = {4, 8, 12, 16, 24, 32};
+  static const uint32_t kR[] = {8, 16, 32, 48, 64};
+  const uint32_t saved_flags = p->opts.flags;
+  p->opts.flags &= ~VFR_RENDER_SIMPLE_KERNEL;
+  double best = 1e30;
+  uint32_t bw = p->opts.waves_per_cu, brf = p->opts.refill;
+  vfr_stats st;
+  CHECK_RC(vfr_plan_execute(p, &st)); /* warm caches */
+  for (uint32_t w : kW)
+    for (uint32_t r : kR) {
+      p->opts.waves_per_cu = w;
+      p->opts.refill = r;
+      double t = 1e30;
+      for (int rep = 0; rep < 2; ++rep) {
+        int rc = vfr_plan_execute(p, &st);
+        if (rc != VFR_OK) { p->opts.flags = saved_flags; return rc; }
+        t = std::min(t, st.ms_traversal);
+      }
+      if (t < best) { best = t; bw = w; brf = r; }
+    }
+  p->opts.waves_per_cu = bw;
+  p->opts.refill = brf;
+  p->opts.flags = saved_flags;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune.push_back({key, {bw, brf}});
+  }
+  if (wpc_out) *wpc_out = bw;
+  if (refill_out) *refill_out = brf;
+  if (best_out) *best_out = best;
   return VFR_OK;
 }
 

@@ -528,7 +528,7 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
   const Planes cf = face_planes(w.H0, w.H1, w.H2);
   float zPrev = plane_eval(cf.z, ox, oy);
   const float sPrev = plane_eval(cf.s, ox, oy);
-  float4 cB = a.tf[tfo + (uint32_t)(255.0f * sPrev)];
+  float4 cB = a.tf[tfo + min(255u, (uint32_t)(255.0f * sPrev))];
   float ax = 0.0f, ay = 0.0f, az = 0.0f, Tr = 1.0f;
   uint32_t steps = 0;
   while (true) {
@@ -543,7 +543,7 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
     dz = ((double)dz < 1e-12) ? 0.0f : dz;
     if ((sNext < 0.0f) || (sNext > 1.0f)) break;    /* :446 */
     dz = dz / a.max_edge;
-    const float4 cF = a.tf[tfo + (uint32_t)(255.0f * sNext)];
+    const float4 cF = a.tf[tfo + min(255u, (uint32_t)(255.0f * sNext))];
     const float taux = cF.w * dz, tauy = cB.w * dz;
     const float zeta = (float)exp(-((double)taux * 0.5 + (double)tauy * 0.5));
     const float gx = (float)((double)taux / (1.0 + (double)taux));
@@ -579,89 +579,124 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
   return o;
 }
 
-/* FTB mode: entries front to back, segments composited front to back with
- * the front = entry point, early termination at 1 - T >= term, re-entry at
- * the first entry behind the last exit point. */
-DEV RayOut trace_ftb(const TravArgs& a, uint32_t vi, uint32_t pix) {
-  RayOut o;
-  o.anomaly = 0; o.broken = 0;
-  const ViewRegs vr = load_view(a.views, vi);
-  const uint32_t W = a.views[vi].W, H = a.views[vi].H, tfo = a.views[vi].tf * 256u;
+/* FTB mode as a per-lane state machine: entries front to back, segments
+ * composited front to back with the front = entry point, early termination
+ * at 1 - T >= term, re-entry at the first entry behind the last exit point.
+ * The one-ray-per-thread kernel and the persistent work-queue kernel both
+ * drive it, so they produce the same bits. */
+struct FtbRay {
+  ViewRegs vr;
+  Walk w;
+  float4 cP;
+  float ox, oy, rayf, Zprev, dprev, exitZ;
+  float ar, ag, ab, Tr;
+  uint32_t vi, pix, gp, tfo, nk, cur, steps, entry, anomaly, broken;
+};
+
+/* Starts the walk at entry `cur` of the pixel's sorted list. */
+DEV void ftb_segment(const TravArgs& a, FtbRay& r) {
+  const uint64_t key = a.entries[(size_t)r.gp * VFR_K + r.cur];
+  const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull);
+  r.w.t = a.ext_int[e * 2];
+  r.w.fe = a.ext_int[e * 2 + 1];
+  const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[r.w.t].v);
+  r.w.h0 = sel4(rv, r.w.fe); r.w.h1 = sel4(rv, (r.w.fe + 1u) & 3u); r.w.h2 = sel4(rv, (r.w.fe + 2u) & 3u);
+  r.w.H0 = xform(r.vr, ld_vert(a.vert, r.w.h0));
+  r.w.H1 = xform(r.vr, ld_vert(a.vert, r.w.h1));
+  r.w.H2 = xform(r.vr, ld_vert(a.vert, r.w.h2));
+  const Hit he = entry_test(r.w.H0, r.w.H1, r.w.H2, r.ox, r.oy);
+  r.Zprev = bary(r.w.H0.z, r.w.H1.z, r.w.H2.z, he);
+  const float Sprev = bary(r.w.H0.w, r.w.H1.w, r.w.H2.w, he);
+  float sprev;
+  if (r.vr.proj == VFR_PROJ_PERSPECTIVE) { r.dprev = 1.0f / r.Zprev; sprev = clamp01(Sprev * r.dprev); }
+  else { r.dprev = 0.0f; sprev = clamp01(Sprev); }
+  r.cP = a.tf[r.tfo + min(255u, (uint32_t)(255.0f * sprev))];
+  r.exitZ = r.Zprev;
+}
+
+DEV void ftb_init(const TravArgs& a, uint32_t vi, uint32_t pix, FtbRay& r) {
+  r.vr = load_view(a.views, vi);
+  const uint32_t W = a.views[vi].W, H = a.views[vi].H;
   const float k = a.views[vi].k;
-  const uint32_t gp = a.views[vi].pix_off + pix;
-  const uint32_t px = pix % W, py = pix / W;
-  const float ox = pix_to_obj(px, W), oy = pix_to_obj(py, H);
-  const bool persp = vr.proj == VFR_PROJ_PERSPECTIVE;
-  const float rayf = persp ? sqrtf(1.0f + (ox * ox + oy * oy) * (k * k)) : k;
-  uint32_t nk = a.cnt[gp];
-  nk = nk > VFR_K ? VFR_K : nk;
-  const uint64_t* ent = a.entries + (size_t)gp * VFR_K;
-  float ar = 0.0f, ag = 0.0f, ab = 0.0f, Tr = 1.0f;
-  uint32_t steps = 0, cur = 0;
-  bool done = false;
-  o.entry = (uint32_t)(ent[0] & 0xFFFFFFFFull);
-  while (cur < nk && !done) {
-    const uint32_t e = (uint32_t)(ent[cur] & 0xFFFFFFFFull);
-    Walk w;
-    w.t = a.ext_int[e * 2];
-    w.fe = a.ext_int[e * 2 + 1];
-    {
-      const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[w.t].v);
-      w.h0 = sel4(rv, w.fe); w.h1 = sel4(rv, (w.fe + 1u) & 3u); w.h2 = sel4(rv, (w.fe + 2u) & 3u);
-      w.H0 = xform(vr, ld_vert(a.vert, w.h0));
-      w.H1 = xform(vr, ld_vert(a.vert, w.h1));
-      w.H2 = xform(vr, ld_vert(a.vert, w.h2));
-    }
-    const Hit he = entry_test(w.H0, w.H1, w.H2, ox, oy);
-    float Zprev = bary(w.H0.z, w.H1.z, w.H2.z, he);
-    const float Sprev = bary(w.H0.w, w.H1.w, w.H2.w, he);
-    float dprev = 0.0f, sprev;
-    if (persp) { dprev = 1.0f / Zprev; sprev = clamp01(Sprev * dprev); }
-    else sprev = clamp01(Sprev);
-    float4 cP = a.tf[tfo + (uint32_t)(255.0f * sprev)];
-    float exitZ = Zprev;
-    while (true) {
-      if (++steps > VFR_MAX_STEPS) { o.anomaly = 1; done = true; break; }
-      const Step s = walk_step(w, a.rec, a.vert, vr, ox, oy);
-      if (s.i == 0) { o.anomaly = 1; done = true; break; }
-      const float Znext = bary(s.A.z, s.B.z, s.C.z, s.h);
-      const float Snext = bary(s.A.w, s.B.w, s.C.w, s.h);
-      float snext, dist;
-      if (persp) {
-        const float dn = 1.0f / Znext;
-        snext = clamp01(Snext * dn);
-        dist = fabsf(dn - dprev) * rayf;
-        dprev = dn;
-      } else {
-        snext = clamp01(Snext);
-        dist = fabsf(Zprev - Znext) * rayf;
-      }
-      const float dl = dist * a.inv_me;
-      const float4 cN = a.tf[tfo + (uint32_t)(255.0f * snext)];
-      const float tf_ = cP.w * dl, tb_ = cN.w * dl;
-      const float zeta = vfr_expf(-(tf_ + tb_) * 0.5f);
-      const float gf = tf_ / (1.0f + tf_), gb = tb_ / (1.0f + tb_);
-      const float ps = a.psi[psi_index(gf) * 512u + psi_index(gb)];
-      const float w1 = 1.0f - ps, w2 = ps - zeta;
-      ar += Tr * (cP.x * w1 + cN.x * w2);
-      ag += Tr * (cP.y * w1 + cN.y * w2);
-      ab += Tr * (cP.z * w1 + cN.z * w2);
-      Tr *= zeta;
-      cP = cN;
-      if (1.0f - Tr >= a.term) { done = true; break; }
-      if (s.nb == VFR_NB_EXT) { exitZ = Znext; break; }
-      if (s.nb == VFR_NB_BROKEN) { o.broken = 1; done = true; break; }
-      w.t = s.nb >> 2; w.fe = s.nb & 3u;
-      w.H0 = s.A; w.H1 = s.B; w.H2 = s.C; w.h0 = s.a; w.h1 = s.b; w.h2 = s.c;
-      Zprev = Znext;
-    }
-    if (done) break;
-    ++cur;
-    while (cur < nk && !(funkey((uint32_t)(ent[cur] >> 32)) < exitZ)) ++cur;
+  r.vi = vi; r.pix = pix;
+  r.tfo = a.views[vi].tf * 256u;
+  r.gp = a.views[vi].pix_off + pix;
+  r.ox = pix_to_obj(pix % W, W);
+  r.oy = pix_to_obj(pix / W, H);
+  r.rayf = r.vr.proj == VFR_PROJ_PERSPECTIVE ? sqrtf(1.0f + (r.ox * r.ox + r.oy * r.oy) * (k * k)) : k;
+  const uint32_t nk = a.cnt[r.gp];
+  r.nk = nk > VFR_K ? VFR_K : nk;
+  r.cur = 0;
+  r.ar = r.ag = r.ab = 0.0f;
+  r.Tr = 1.0f;
+  r.steps = 0; r.anomaly = 0; r.broken = 0;
+  r.entry = (uint32_t)(a.entries[(size_t)r.gp * VFR_K] & 0xFFFFFFFFull);
+  ftb_segment(a, r);
+}
+
+/* One tet of the walk; returns true when the ray is finished. */
+DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
+  if (++r.steps > VFR_MAX_STEPS) { r.anomaly = 1; return true; }
+  const Step s = walk_step(r.w, a.rec, a.vert, r.vr, r.ox, r.oy);
+  if (s.i == 0) { r.anomaly = 1; return true; }
+  const float Znext = bary(s.A.z, s.B.z, s.C.z, s.h);
+  const float Snext = bary(s.A.w, s.B.w, s.C.w, s.h);
+  float snext, dist;
+  if (r.vr.proj == VFR_PROJ_PERSPECTIVE) {
+    const float dn = 1.0f / Znext;
+    snext = clamp01(Snext * dn);
+    dist = fabsf(dn - r.dprev) * r.rayf;
+    r.dprev = dn;
+  } else {
+    snext = clamp01(Snext);
+    dist = fabsf(r.Zprev - Znext) * r.rayf;
   }
-  o.c = make_float4(ar, ag, ab, 1.0f - Tr);
-  o.steps = steps > VFR_MAX_STEPS ? VFR_MAX_STEPS : steps;
+  const float dl = dist * a.inv_me;
+  const float4 cN = a.tf[r.tfo + min(255u, (uint32_t)(255.0f * snext))];
+  const float tf_ = r.cP.w * dl, tb_ = cN.w * dl;
+  const float zeta = vfr_expf(-(tf_ + tb_) * 0.5f);
+  const float gf = tf_ / (1.0f + tf_), gb = tb_ / (1.0f + tb_);
+  const float ps = a.psi[psi_index(gf) * 512u + psi_index(gb)];
+  const float w1 = 1.0f - ps, w2 = ps - zeta;
+  r.ar += r.Tr * (r.cP.x * w1 + cN.x * w2);
+  r.ag += r.Tr * (r.cP.y * w1 + cN.y * w2);
+  r.ab += r.Tr * (r.cP.z * w1 + cN.z * w2);
+  r.Tr *= zeta;
+  r.cP = cN;
+  if (1.0f - r.Tr >= a.term) return true;
+  if (s.nb == VFR_NB_BROKEN) { r.broken = 1; return true; }
+  if (s.nb == VFR_NB_EXT) {
+    r.exitZ = Znext;
+    ++r.cur;
+    while (r.cur < r.nk &&
+           !(funkey((uint32_t)(a.entries[(size_t)r.gp * VFR_K + r.cur] >> 32)) < r.exitZ))
+      ++r.cur;
+    if (r.cur >= r.nk) return true;
+    ftb_segment(a, r);
+    return false;
+  }
+  r.w.t = s.nb >> 2; r.w.fe = s.nb & 3u;
+  r.w.H0 = s.A; r.w.H1 = s.B; r.w.H2 = s.C; r.w.h0 = s.a; r.w.h1 = s.b; r.w.h2 = s.c;
+  r.Zprev = Znext;
+  return false;
+}
+
+DEV RayOut ftb_out(const FtbRay& r) {
+  RayOut o;
+  o.c = make_float4(r.ar, r.ag, r.ab, 1.0f - r.Tr);
+  o.steps = r.steps > VFR_MAX_STEPS ? VFR_MAX_STEPS : r.steps;
+  o.entry = r.entry;
+  o.anomaly = r.anomaly;
+  o.broken = r.broken;
   return o;
+}
+
+DEV RayOut trace_ftb(const TravArgs& a, uint32_t vi, uint32_t pix) {
+  FtbRay r;
+  ftb_init(a, vi, pix, r);
+  while (!ftb_step(a, r)) {
+  }
+  return ftb_out(r);
 }
 
 template <uint32_t MODE>
@@ -700,10 +735,73 @@ __global__ __launch_bounds__(256) void k_trav(TravArgs a) {
   }
 }
 
+/* Persistent work-queue traversal: a fixed grid of waves pulls rays from the
+ * compacted list.  Whenever at least `refill` lanes of a wave are idle (their
+ * rays ended or left through a concave boundary for good) the wave claims
+ * that many rays with one atomic and hands them to the idle lanes (ballot +
+ * prefix count), then advances every lane `burst` tets.  A wave exits once
+ * the queue is drained and all its lanes are idle; every ray ends within
+ * VFR_MAX_STEPS tets, so every wave reaches that exit. */
+__global__ __launch_bounds__(256) void k_trav_persist(TravArgs a, uint32_t burst) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t n = *a.n_rays;
+  const uint32_t refill = a.refill < 1u ? 1u : (a.refill > 64u ? 64u : a.refill);
+  FtbRay r;
+  bool active = false, drained = false;
+  unsigned long long st = 0, an = 0, br = 0;
+  while (true) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t ni = (uint32_t)__popcll(idle);
+    if (!drained && ni >= refill) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(a.queue, ni);
+      base = __shfl(base, 0, 64);
+      if (base >= n) {
+        drained = true;
+      } else if (!active) {
+        const uint32_t idx = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+        if (idx < n) {
+          const uint64_t ray = a.rays[idx];
+          ftb_init(a, (uint32_t)(ray >> 32), (uint32_t)ray, r);
+          active = true;
+        }
+      }
+    }
+    if (__ballot(active) == 0ull) {
+      if (drained) break;
+      continue;
+    }
+    for (uint32_t k = 0; k < burst; ++k) {
+      if (active && ftb_step(a, r)) {
+        const RayOut o = ftb_out(r);
+        store_ray(a, r.vi, r.pix, o);
+        st += o.steps; an += o.anomaly; br += o.broken;
+        active = false;
+      }
+    }
+  }
+  st = wave_sum_u64(st);
+  an = wave_sum_u64(an);
+  br = wave_sum_u64(br);
+  if (lane == 0) {
+    if (st) atomicAdd(&a.stats[ST_STEPS], st);
+    if (an) atomicAdd(&a.stats[ST_ANOM], an);
+    if (br) atomicAdd(&a.stats[4], br);
+  }
+}
+
 int vfr_k_traverse(uint32_t mode, uint32_t proj_mask, const TravArgs& a, uint32_t max_rays,
                    int persistent, uint32_t grid_waves, void* stream) {
-  (void)proj_mask; (void)persistent; (void)grid_waves;
+  (void)proj_mask;
   if (max_rays == 0) return VFR_OK;
+  if (persistent && mode == VFR_MODE_FTB) {
+    uint32_t blocks = (grid_waves + 3u) / 4u;
+    if (blocks == 0) blocks = 1;
+    const uint32_t burst = 8;
+    hipLaunchKernelGGL(k_trav_persist, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
+    CHECK_HIP(hipGetLastError());
+    return VFR_OK;
+  }
   dim3 grid((max_rays + 255) / 256), block(256);
   if (mode == VFR_MODE_REF)
     hipLaunchKernelGGL(k_trav<VFR_MODE_REF>, grid, block, 0, (hipStream_t)stream, a);
