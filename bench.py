@@ -70,6 +70,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--probe", type=int, default=0,
+                    help="only run the copy-kernel bandwidth probe over this many bytes")
+    ap.add_argument("--diag", action="store_true", help="per-view diagnostics on stderr")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the renderer is "
@@ -78,6 +82,12 @@ def main() -> int:
 
     import vfray_amd as V
     from vfray_amd import workloads as WL
+
+    if args.probe:
+        with V.Context(0) as c:
+            g = c.bandwidth_probe(args.probe, 20)
+        print(json.dumps({"probe_bytes": args.probe, "copy_gbps": round(g, 1)}), flush=True)
+        return 0
 
     t0 = time.time()
     if args.workload == "batch64":
@@ -95,7 +105,15 @@ def main() -> int:
     views = WL.sphere_views(seed, nviews, sides, tf)
     total_rays = sum(v.width * v.height for v in views)
 
-    pb = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, use_graph=args.graph)
+    tune = None
+    if args.autotune:
+        pt = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+        tune = pt.autotune()
+        pt.close()
+        log(f"autotune: {tune}")
+    kw = {} if tune is None else {"waves_per_cu": tune["waves_per_cu"], "refill": tune["refill"],
+                                  "occupancy": tune["occupancy"]}
+    pb = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, use_graph=args.graph, **kw)
     for _ in range(args.warmup):
         pb.execute()
     log("batched warm-up done")
@@ -124,6 +142,19 @@ def main() -> int:
                  "frame_p99_ms": float(np.median([s["ms_frame_p99"] for s in sp]))}
         same = all(np.array_equal(pb.read(i)["rgba"], pp.read(i)["rgba"]) for i in range(min(3, nviews)))
         plain["batched_equals_plain"] = bool(same)
+        if args.diag:
+            ft = pp.frame_times()
+            for i in range(nviews):
+                r = pp.read(i, rgba=False, entry=False)
+                act = r["steps"] > 0
+                log(f"view {i} {views[i].width}^2 frame {ft[i]:.2f} ms rays {int(act.sum())} "
+                    f"steps max {int(r['steps'].max())} mean {float(r['steps'][act].mean()):.1f} "
+                    f"sum {int(r['steps'].sum())}")
+            pd = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, simple_kernel=True)
+            pd.execute()
+            sd = pd.execute()
+            log(f"batched, one ray per thread: traversal {sd['ms_traversal']:.2f} ms")
+            pd.close()
         log(f"plain: {pwall * 1e3:.2f} ms/step")
         pp.close()
 
@@ -157,7 +188,9 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": args.workload, "tets": int(t.shape[0]), "views": nviews,
                    "rays_per_step": int(total_rays), "active_rays": int(rays_active),
-                   "tet_steps": int(steps_per_launch), "term_alpha": 0.99, "graph": args.graph},
+                   "tet_steps": int(steps_per_launch), "term_alpha": 0.99, "graph": args.graph,
+                   "schedule": tune or {"occupancy": 4, "waves_per_cu": 16, "refill": 32}},
+        "anomalous_rays": int(st["anomalies"]),
         "event_ms_per_step": round(ms_evt, 3),
         "active_mrays_per_s": round(rays_active / wall / 1e6, 2),
         "gsteps_per_s": round(steps_per_launch / wall / 1e9, 3),

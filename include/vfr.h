@@ -70,6 +70,12 @@ int vfr_psi_read(vfr_ctx* ctx, float* out512x512);
 /* The closed-form table on host, no device needed (512*512 floats). */
 int vfr_psi_generate(float* out512x512);
 
+/* Streams a plain device-to-device copy of `bytes` `iters` times and
+ * returns the achieved read+write bandwidth in GB/s: the device's practical
+ * HBM ceiling, and the yardstick for the profiler's byte counters.  No
+ * counterpart in the reference. */
+int vfr_bandwidth_probe(vfr_ctx* ctx, uint64_t bytes, int iters, double* gbps);
+
 /* Registers a 256-entry RGBA transfer function (1024 floats, r g b a per
  * entry, as the .tf file, vfRayPreComp.cc:551-579).  Returns its id. */
 int vfr_tf_create(vfr_ctx* ctx, const float rgba[1024], int* tf_id);
@@ -141,6 +147,10 @@ typedef struct vfr_render_opts {
   uint32_t use_graph;   /* capture the batched launch set in a hipGraph    */
   uint32_t max_entries; /* FTB entry-list slots per pixel, 0 = 8           */
   uint32_t flags;       /* VFR_RENDER_*                                    */
+  uint32_t occupancy;   /* persistent kernel variant: target waves per SIMD
+                           (0 or <= 4: compiler's choice; 5, 6: register-
+                           capped); tuned by vfr_plan_autotune            */
+  uint32_t reserved;
 } vfr_render_opts;
 
 /* Batched FTB renders use the persistent work-queue kernel unless this flag
@@ -174,12 +184,15 @@ int vfr_plan_create(vfr_ctx* ctx, const vfr_mesh* mesh, const vfr_view* views,
 int vfr_plan_execute(vfr_plan* plan, vfr_stats* stats);
 int vfr_plan_read(vfr_plan* plan, int view, vfr_frame_out* out);
 void vfr_plan_destroy(vfr_plan* plan);
+/* Per-view frame latency (ms, HIP events) of the last PER_VIEW execute. */
+int vfr_plan_frame_times(const vfr_plan* plan, double* out_ms);
 
-/* Times candidate persistent-kernel schedules (waves per CU x refill
- * threshold) on the plan itself and keeps the fastest; returns the choice
- * and its time.  Results are cached per (tets, rays, views) in the process. */
-int vfr_plan_autotune(vfr_plan* plan, uint32_t* waves_per_cu, uint32_t* refill,
-                      double* best_ms);
+/* Times candidate persistent-kernel schedules (kernel occupancy variant x
+ * waves per CU x refill threshold) on the plan itself and keeps the fastest;
+ * returns the choice and its traversal time.  Results are cached per
+ * (tets, rays, views) in the process. */
+int vfr_plan_autotune(vfr_plan* plan, uint32_t* occupancy, uint32_t* waves_per_cu,
+                      uint32_t* refill, double* best_ms);
 
 /* create + execute + read + destroy. outs has n_views entries. */
 int vfr_render(vfr_ctx* ctx, const vfr_mesh* mesh, const vfr_view* views,

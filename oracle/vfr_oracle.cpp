@@ -56,7 +56,7 @@ static const double RDELTA = 1e-12; /* vfRayKernelCPU.h:28 */
 /* A walk longer than this stops as an anomaly (the reference has no bound; a
  * ray cycling around a degenerate edge would never end). Same bound as the
  * renderer. */
-static const u32 MAX_STEPS = 65536u;
+static const u32 MAX_STEPS = 8192u;
 
 struct F4 { float x, y, z, w; };
 
@@ -265,14 +265,45 @@ static inline ExitHit find_exit(const F4 Q[4], float ox, float oy) {
   return h;
 }
 
+static inline float bary(float A, float B, float C, const ExitHit& h);
+
+/* Front-to-back mode's exit choice: of the candidates that pass the
+ * reference's test, the one whose exit point has the smallest Z (farthest
+ * along the ray); ties keep the reference's order.  With one passing face it
+ * is the reference's choice. */
+static inline ExitHit find_exit_farthest(const F4 Q[4], float ox, float oy) {
+  ExitHit hs[4];
+  int n = 0, first = 0;
+  for (int i = 1; i < 4; ++i) {
+    hs[i].i = 0;
+    if (tri_test(Q[i], Q[(i + 1) & 3], Q[(i + 2) & 3], ox, oy, 1.0f, &hs[i])) {
+      hs[i].i = i;
+      if (!n) first = i;
+      n++;
+    }
+  }
+  if (n == 0) { ExitHit h; h.i = 0; h.cy = h.cz = h.D = 0.0f; h.swapped = false; return h; }
+  if (n == 1) return hs[first];
+  int best = 0;
+  float bz = 0.0f;
+  for (int i = 1; i < 4; ++i) {
+    if (!hs[i].i) continue;
+    float z = bary(Q[i].z, Q[(i + 1) & 3].z, Q[(i + 2) & 3].z, hs[i]);
+    if (!best || z < bz) { best = i; bz = z; }
+  }
+  return hs[best];
+}
+
 /* Barycentric interpolation over the triangle (a,b,c) from the inside-test
  * crosses: with e1 = b-a, e2 = c-a (swapped when their cross is negative),
  * cy = e1 x p, cz = p x e2, D = e1 x e2 > 0.  Weight of b is cz/D unswapped,
- * cy/D swapped; weight of c the other one. */
+ * cy/D swapped; weight of c the other one.  1/D is rounded once and
+ * multiplied in. */
 static inline float bary(float A, float B, float C, const ExitHit& h) {
   float wb = h.swapped ? h.cy : h.cz;
   float wc = h.swapped ? h.cz : h.cy;
-  return A + ((wb * (B - A) + wc * (C - A)) / h.D);
+  float inv = 1.0f / h.D;
+  return A + ((wb * (B - A) + wc * (C - A)) * inv);
 }
 
 /* ------------------------------------------------------------- psi (FP64) */
@@ -715,7 +746,7 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
           if (++iter > 4 * MAX_STEPS) { anomaly = true; break; }
           F4 Q[4];
           for (int k = 0; k < 4; ++k) Q[k] = P[T[tcur * 4 + ((fent + k) & 3)]];
-          ExitHit h = find_exit(Q, ox, oy);
+          ExitHit h = find_exit_farthest(Q, ox, oy);
           if (h.i == 0) { anomaly = true; anom32 = live32; break; }
           u32 fex = (fent + (u32)h.i) & 3;
           const F4& A = Q[h.i];

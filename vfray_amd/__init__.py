@@ -37,7 +37,8 @@ class View(C.Structure):
 class RenderOpts(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("schedule", C.c_uint32), ("term_alpha", C.c_float),
                 ("waves_per_cu", C.c_uint32), ("refill", C.c_uint32), ("use_graph", C.c_uint32),
-                ("max_entries", C.c_uint32), ("flags", C.c_uint32)]
+                ("max_entries", C.c_uint32), ("flags", C.c_uint32), ("occupancy", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class FrameOut(C.Structure):
@@ -83,6 +84,7 @@ def lib() -> C.CDLL:
             "vfr_psi_read": (C.c_int, [P, f32p]),
             "vfr_psi_generate": (C.c_int, [f32p]),
             "vfr_tf_create": (C.c_int, [P, f32p, C.POINTER(C.c_int)]),
+            "vfr_bandwidth_probe": (C.c_int, [P, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
             "vfr_mesh_create": (C.c_int, [P, f32p, u32, u32p, u32, u32p, u32p, u32, f32p, u32, C.POINTER(P)]),
             "vfr_mesh_destroy": (None, [P]),
             "vfr_mesh_get_info": (C.c_int, [P, C.POINTER(MeshInfo)]),
@@ -92,7 +94,9 @@ def lib() -> C.CDLL:
             "vfr_plan_execute": (C.c_int, [P, C.POINTER(Stats)]),
             "vfr_plan_read": (C.c_int, [P, C.c_int, C.POINTER(FrameOut)]),
             "vfr_plan_destroy": (None, [P]),
-            "vfr_plan_autotune": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
+            "vfr_plan_frame_times": (C.c_int, [P, C.POINTER(C.c_double)]),
+            "vfr_plan_autotune": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                            C.POINTER(C.c_double)]),
             "vfr_render": (C.c_int, [P, P, C.POINTER(View), C.c_int, C.POINTER(RenderOpts), C.POINTER(FrameOut), C.POINTER(Stats)]),
             "vfr_free": (None, [P]),
             "vfr_read_off": (C.c_int, [C.c_char_p, C.POINTER(f32p), u32p, C.POINTER(u32p), u32p]),
@@ -289,6 +293,11 @@ class Context:
         _check(lib().vfr_tf_create(self._p, fptr(t), C.byref(i)))
         return i.value
 
+    def bandwidth_probe(self, nbytes: int = 1 << 30, iters: int = 20) -> float:
+        g = C.c_double()
+        _check(lib().vfr_bandwidth_probe(self._p, nbytes, iters, C.byref(g)))
+        return g.value
+
     def psi_load(self, table: np.ndarray | None) -> None:
         _check(lib().vfr_psi_load(self._p, None if table is None else fptr(np.ascontiguousarray(table, np.float32))))
 
@@ -301,9 +310,9 @@ class Context:
         return Mesh(self, xyzs, tets, con, extf, lmt, normalize, morton)
 
     def plan(self, mesh: "Mesh", views, mode=MODE_FTB, schedule=SCHED_BATCHED, term_alpha=0.99,
-             use_graph=False, waves_per_cu=0, refill=0, simple_kernel=False) -> "Plan":
+             use_graph=False, waves_per_cu=0, refill=0, simple_kernel=False, occupancy=0) -> "Plan":
         return Plan(self, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
-                    simple_kernel)
+                    simple_kernel, occupancy)
 
     def render(self, mesh: "Mesh", views, **kw) -> tuple[list[dict], dict]:
         p = self.plan(mesh, views, **kw)
@@ -351,24 +360,29 @@ class Mesh:
 
 class Plan:
     def __init__(self, ctx, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
-                 simple_kernel=False):
+                 simple_kernel=False, occupancy=0):
         self.ctx, self.mesh = ctx, mesh
         self.views = list(views)
         arr = (View * len(self.views))(*self.views)
         o = RenderOpts(mode, schedule, term_alpha, waves_per_cu, refill, 1 if use_graph else 0, 0,
-                       1 if simple_kernel else 0)
+                       1 if simple_kernel else 0, occupancy, 0)
         self._p = C.c_void_p()
         _check(lib().vfr_plan_create(ctx._p, mesh._p, arr, len(self.views), C.byref(o), C.byref(self._p)))
 
     def autotune(self) -> dict:
-        w, r, ms = C.c_uint32(), C.c_uint32(), C.c_double()
-        _check(lib().vfr_plan_autotune(self._p, C.byref(w), C.byref(r), C.byref(ms)))
-        return {"waves_per_cu": w.value, "refill": r.value, "traversal_ms": ms.value}
+        o, w, r, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_double()
+        _check(lib().vfr_plan_autotune(self._p, C.byref(o), C.byref(w), C.byref(r), C.byref(ms)))
+        return {"occupancy": o.value, "waves_per_cu": w.value, "refill": r.value, "traversal_ms": ms.value}
 
     def execute(self) -> dict:
         st = Stats()
         _check(lib().vfr_plan_execute(self._p, C.byref(st)))
         return st.as_dict()
+
+    def frame_times(self) -> np.ndarray:
+        out = np.zeros(len(self.views), np.float64)
+        _check(lib().vfr_plan_frame_times(self._p, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
 
     def read(self, i: int, rgba=True, entry=True, steps=True) -> dict:
         v = self.views[i]

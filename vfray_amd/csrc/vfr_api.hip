@@ -137,6 +137,34 @@ extern "C" int vfr_tf_create(vfr_ctx* c, const float rgba[1024], int* tf_id) {
   return VFR_OK;
 }
 
+extern "C" int vfr_bandwidth_probe(vfr_ctx* c, uint64_t bytes, int iters, double* gbps) {
+  if (!c || !gbps || bytes < 16 || iters < 1) { vfr_set_error("vfr_bandwidth_probe: bad argument"); return VFR_EINVAL; }
+  CHECK_HIP(hipSetDevice(c->device));
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    if (a) (void)hipFree(a);
+    vfr_set_error("probe allocation of 2 x %llu bytes failed", (unsigned long long)bytes);
+    return VFR_ENOMEM;
+  }
+  int rc = vfr_k_clear_u32((uint32_t*)a, bytes / 4, c->stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == VFR_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = VFR_EHIP;
+  if (rc == VFR_OK) rc = vfr_k_copy(a, b, bytes, c->stream);
+  if (rc == VFR_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = VFR_EHIP;
+  for (int i = 0; rc == VFR_OK && i < iters; ++i) rc = vfr_k_copy(a, b, bytes, c->stream);
+  float ms = 0;
+  if (rc == VFR_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                       hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = VFR_EHIP;
+  if (rc == VFR_OK) *gbps = 2.0 * (double)(bytes / 16 * 16) * iters / (ms * 1e-3) / 1e9;
+  else if (rc == VFR_EHIP) vfr_set_error("bandwidth probe failed");
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
+}
+
 /* -------------------------------------------------------------------- mesh */
 
 extern "C" int vfr_mesh_create(vfr_ctx* c, const float* xyzs, uint32_t nV, const uint32_t* tets,
@@ -251,6 +279,7 @@ struct vfr_plan {
   uint32_t* owner = nullptr;   /* REF owner or FTB count, [n_pix] */
   uint64_t* entries = nullptr; /* FTB, [n_pix * VFR_K] */
   uint64_t* rays = nullptr;    /* [n_pix] */
+  float4* pverts = nullptr;    /* [n_views * n_verts] P-space vertices */
   float4* rgba = nullptr;
   uint32_t* steps = nullptr;
   uint32_t* entry = nullptr;
@@ -262,6 +291,7 @@ struct vfr_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool executed = false;
+  std::vector<double> frame_ms;           /* last PER_VIEW execute, per view */
 };
 
 /* View set-up on the host, in double, rounded once to float. */
@@ -336,7 +366,8 @@ extern "C" void vfr_plan_destroy(vfr_plan* p) {
   (void)hipStreamSynchronize(p->ctx->stream);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
-  void* ps[] = {p->views, p->owner, p->entries, p->rays, p->rgba, p->steps, p->entry, p->counters, p->stats};
+  void* ps[] = {p->views, p->owner, p->entries, p->rays, p->pverts, p->rgba, p->steps, p->entry,
+                p->counters, p->stats};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   for (auto& e : p->ev)
@@ -352,10 +383,11 @@ static int enqueue_batched(vfr_plan* p) {
   CHECK_RC(vfr_k_clear_u32(p->owner, p->n_pix, st));
   CHECK_RC(vfr_k_clear_u32(p->counters, p->n_views + 2, st));
   CHECK_RC(vfr_k_clear_u32((uint32_t*)p->stats, 2 * ST_N, st));
+  CHECK_RC(vfr_k_xform_views(p->views, 0, p->n_views, m->d, p->pverts, st));
   if (p->opts.mode == VFR_MODE_REF)
-    CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, m->d, p->owner, st));
+    CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, m->d, p->pverts, p->owner, st));
   else
-    CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d, p->owner, p->entries, p->stats, st));
+    CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d, p->pverts, p->owner, p->entries, p->stats, st));
   CHECK_RC(vfr_k_compact(p->views, p->n_views, 0, p->n_tiles, p->opts.mode, p->owner, p->owner,
                          p->entries, p->rgba, p->steps, p->entry, p->rays, p->counters + p->n_views, st));
   return VFR_OK;
@@ -365,7 +397,7 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   TravArgs a;
   const vfr_mesh* m = p->mesh;
   a.rec = m->d.rec;
-  a.vert = (const float4*)m->d.vert;
+  a.vert = p->pverts;
   a.ext_int = m->d.ext_int;
   a.tf = p->ctx->tf;
   a.psi = p->ctx->psi;
@@ -393,7 +425,8 @@ static int launch_batched_traverse(vfr_plan* p) {
   TravArgs a = trav_args(p, p->rays, p->counters + p->n_views);
   const bool persist = p->opts.mode == VFR_MODE_FTB && !(p->opts.flags & VFR_RENDER_SIMPLE_KERNEL);
   const uint32_t waves = p->opts.waves_per_cu * (uint32_t)p->ctx->n_cu;
-  return vfr_k_traverse(p->opts.mode, 0, a, (uint32_t)p->n_pix, persist ? 1 : 0, waves, p->ctx->stream);
+  return vfr_k_traverse(p->opts.mode, p->opts.occupancy, a, (uint32_t)p->n_pix, persist ? 1 : 0, waves,
+                        p->ctx->stream);
 }
 
 extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* views, int n_views,
@@ -436,6 +469,7 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
     p->hv[i].pix_off = (uint32_t)pix;
     p->hv[i].tile_off = (uint32_t)tiles;
     p->hv[i].tiles_x = (views[i].width + 7) / 8;
+    p->hv[i].vert_off = (uint32_t)((uint64_t)i * m->d.n_verts);
     pix += (uint64_t)views[i].width * views[i].height;
     tiles += (uint64_t)p->hv[i].tiles_x * ((views[i].height + 7) / 8);
     if (pix >= (1ull << 31) || tiles >= (1ull << 31)) {
@@ -446,6 +480,11 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   }
   p->n_pix = pix;
   p->n_tiles = (uint32_t)tiles;
+  if ((uint64_t)n_views * m->d.n_verts >= (1ull << 32)) {
+    vfr_set_error("plan needs more than 2^32 view vertices");
+    delete p;
+    return VFR_EINVAL;
+  }
   int rc = VFR_OK;
   (void)hipSetDevice(c->device);
 #define PALLOC(ptr, bytes)                                                        \
@@ -457,6 +496,7 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   PALLOC(p->owner, sizeof(uint32_t) * pix);
   if (o.mode == VFR_MODE_FTB) PALLOC(p->entries, sizeof(uint64_t) * VFR_K * pix);
   PALLOC(p->rays, sizeof(uint64_t) * pix);
+  PALLOC(p->pverts, sizeof(float4) * (size_t)n_views * m->d.n_verts);
   PALLOC(p->rgba, sizeof(float4) * pix);
   PALLOC(p->steps, sizeof(uint32_t) * pix);
   PALLOC(p->entry, sizeof(uint32_t) * pix);
@@ -542,10 +582,11 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
       uint32_t ntl = hv.tiles_x * ((hv.H + 7) / 8);
       CHECK_HIP(hipEventRecord(p->fev[2 * v], st));
       CHECK_RC(vfr_k_clear_u32(p->owner + hv.pix_off, npx, st));
+      CHECK_RC(vfr_k_xform_views(p->views, v, 1, m->d, p->pverts, st));
       if (p->opts.mode == VFR_MODE_REF)
-        CHECK_RC(vfr_k_raster_ref(p->views, v, 1, m->d, p->owner, st));
+        CHECK_RC(vfr_k_raster_ref(p->views, v, 1, m->d, p->pverts, p->owner, st));
       else
-        CHECK_RC(vfr_k_raster_ftb(p->views, v, 1, m->d, p->owner, p->entries, p->stats, st));
+        CHECK_RC(vfr_k_raster_ftb(p->views, v, 1, m->d, p->pverts, p->owner, p->entries, p->stats, st));
       CHECK_RC(vfr_k_compact(p->views, p->n_views, hv.tile_off, ntl, p->opts.mode, p->owner, p->owner,
                              p->entries, p->rgba, p->steps, p->entry, p->rays + hv.pix_off,
                              p->counters + v, st));
@@ -568,6 +609,7 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
     for (uint32_t v = 0; v < p->n_views; ++v) { ms_vis += vis[v]; ms_trav += frame[v] - vis[v]; }
     p50 = pct(frame, 0.50);
     p99 = pct(frame, 0.99);
+    p->frame_ms = frame;
   }
   p->executed = true;
   if (stats) {
@@ -597,6 +639,13 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
       return VFR_EMESH;
     }
   }
+  return VFR_OK;
+}
+
+extern "C" int vfr_plan_frame_times(const vfr_plan* p, double* out) {
+  if (!p || !out) { vfr_set_error("vfr_plan_frame_times: null argument"); return VFR_EINVAL; }
+  if (p->frame_ms.size() != p->n_views) { vfr_set_error("no per-view frame times (PER_VIEW plan not executed)"); return VFR_ESTATE; }
+  memcpy(out, p->frame_ms.data(), sizeof(double) * p->n_views);
   return VFR_OK;
 }
 
@@ -639,10 +688,14 @@ struct TuneKey {
   uint32_t n_views;
   bool operator==(const TuneKey& o) const { return n_tets == o.n_tets && n_pix == o.n_pix && n_views == o.n_views; }
 };
+struct TuneVal {
+  uint32_t occ, wpc, refill;
+};
 static std::mutex g_tune_mu;
-static std::vector<std::pair<TuneKey, std::pair<uint32_t, uint32_t>>> g_tune;
+static std::vector<std::pair<TuneKey, TuneVal>> g_tune;
 
-extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* wpc_out, uint32_t* refill_out, double* best_out) {
+extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* occ_out, uint32_t* wpc_out, uint32_t* refill_out,
+                                 double* best_out) {
   if (!p) { vfr_set_error("vfr_plan_autotune: null plan"); return VFR_EINVAL; }
   if (p->opts.schedule != VFR_SCHED_BATCHED || p->opts.mode != VFR_MODE_FTB) {
     vfr_set_error("autotune applies to batched front-to-back plans");
@@ -650,47 +703,54 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* wpc_out, uint32_t* refil
   }
   if (p->gexec) { vfr_set_error("autotune before graph capture (plan has a graph)"); return VFR_ESTATE; }
   TuneKey key{p->mesh->d.n_tets, p->n_pix, p->n_views};
+  bool cached = false;
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     for (auto& e : g_tune)
       if (e.first == key) {
-        p->opts.waves_per_cu = e.second.first;
-        p->opts.refill = e.second.second;
-        if (wpc_out) *wpc_out = p->opts.waves_per_cu;
-        if (refill_out) *refill_out = p->opts.refill;
-        if (best_out) *best_out = -1.0;
-        return VFR_OK;
+        p->opts.occupancy = e.second.occ;
+        p->opts.waves_per_cu = e.second.wpc;
+        p->opts.refill = e.second.refill;
+        cached = true;
       }
   }
-  static const uint32_t kW[] = {4, 8, 12, 16, 24, 32};
-  static const uint32_t kR[] = {8, 16, 32, 48, 64};
-  const uint32_t saved_flags = p->opts.flags;
-  p->opts.flags &= ~VFR_RENDER_SIMPLE_KERNEL;
-  double best = 1e30;
-  uint32_t bw = p->opts.waves_per_cu, brf = p->opts.refill;
-  vfr_stats st;
-  CHECK_RC(vfr_plan_execute(p, &st)); /* warm caches */
-  for (uint32_t w : kW)
-    for (uint32_t r : kR) {
-      p->opts.waves_per_cu = w;
-      p->opts.refill = r;
-      double t = 1e30;
-      for (int rep = 0; rep < 2; ++rep) {
-        int rc = vfr_plan_execute(p, &st);
-        if (rc != VFR_OK) { p->opts.flags = saved_flags; return rc; }
-        t = std::min(t, st.ms_traversal);
-      }
-      if (t < best) { best = t; bw = w; brf = r; }
-    }
-  p->opts.waves_per_cu = bw;
-  p->opts.refill = brf;
-  p->opts.flags = saved_flags;
-  {
+  double best = -1.0;
+  if (!cached) {
+    /* occupancy variant x resident waves per CU (as multiples of the
+     * variant's 4-SIMD capacity) x refill threshold */
+    static const uint32_t kOcc[] = {4, 5, 6};
+    static const uint32_t kMul[] = {1, 2};
+    static const uint32_t kR[] = {8, 16, 32, 48, 64};
+    const uint32_t saved_flags = p->opts.flags;
+    p->opts.flags &= ~VFR_RENDER_SIMPLE_KERNEL;
+    best = 1e30;
+    TuneVal bv{p->opts.occupancy, p->opts.waves_per_cu, p->opts.refill};
+    vfr_stats st;
+    CHECK_RC(vfr_plan_execute(p, &st)); /* warm caches */
+    for (uint32_t occ : kOcc)
+      for (uint32_t mul : kMul)
+        for (uint32_t r : kR) {
+          p->opts.occupancy = occ;
+          p->opts.waves_per_cu = 4 * occ * mul;
+          p->opts.refill = r;
+          double t = 1e30;
+          for (int rep = 0; rep < 2; ++rep) {
+            int rc = vfr_plan_execute(p, &st);
+            if (rc != VFR_OK) { p->opts.flags = saved_flags; return rc; }
+            t = std::min(t, st.ms_traversal);
+          }
+          if (t < best) { best = t; bv = TuneVal{occ, 4 * occ * mul, r}; }
+        }
+    p->opts.occupancy = bv.occ;
+    p->opts.waves_per_cu = bv.wpc;
+    p->opts.refill = bv.refill;
+    p->opts.flags = saved_flags;
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune.push_back({key, {bw, brf}});
+    g_tune.push_back({key, bv});
   }
-  if (wpc_out) *wpc_out = bw;
-  if (refill_out) *refill_out = brf;
+  if (occ_out) *occ_out = p->opts.occupancy;
+  if (wpc_out) *wpc_out = p->opts.waves_per_cu;
+  if (refill_out) *refill_out = p->opts.refill;
   if (best_out) *best_out = best;
   return VFR_OK;
 }

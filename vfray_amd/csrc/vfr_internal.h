@@ -15,9 +15,10 @@ double vfr_psi_integral(double tf, double tb);
 #define VFR_NB_EXT 0xFFFFFFFFu    /* external face (the .con file's "itself") */
 #define VFR_NB_BROKEN 0xFFFFFFFEu /* neighbour whose faces do not link back   */
 
-/* A walk that visits more tets than this is stopped as an anomaly (a ray
- * cycling around a degenerate edge would otherwise never end). */
-#define VFR_MAX_STEPS 65536u
+/* A walk that visits more tets than this is stopped as an anomaly (in REF
+ * mode a ray through an edge can circle it forever, as in the reference).  A
+ * straight ray through a 400^3 grid crosses about 2,100 tets. */
+#define VFR_MAX_STEPS 8192u
 
 /* Entry-list slots per pixel in front-to-back mode. */
 #define VFR_K 8u
@@ -41,6 +42,8 @@ struct alignas(16) DevView {
   uint32_t pix_off;  /* first pixel of this view in the per-pixel arrays    */
   uint32_t tile_off; /* first 8x8 tile of this view                         */
   uint32_t tiles_x;  /* 8x8 tiles per image row                             */
+  uint32_t vert_off; /* first vertex of this view in the P-space vertex array */
+  uint32_t pad_[3];
 };
 
 struct DevMesh {
@@ -63,7 +66,7 @@ enum { ST_STEPS = 0, ST_ANOM = 1, ST_OVERFLOW = 2, ST_RAYS = 3, ST_N = 8 };
 #ifdef __HIPCC__
 struct TravArgs {
   const TetRec* rec;
-  const float4* vert;
+  const float4* vert;      /* P-space vertices of all views (+ view vert_off) */
   const uint32_t* ext_int;
   const float4* tf;        /* [slots * 256] */
   const float* psi;        /* [512 * 512]   */
@@ -85,15 +88,21 @@ struct TravArgs {
 /* ---- launchers (vfr_kernels.hip) ---- */
 
 int vfr_k_clear_u32(uint32_t* p, size_t n, void* stream);
+int vfr_k_copy(const void* src, void* dst, size_t bytes, void* stream);
+int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m, float4* pv,
+                      void* stream);
 int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m,
-                     uint32_t* owner, void* stream);
+                     const float4* pv, uint32_t* owner, void* stream);
 int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m,
-                     uint32_t* cnt, uint64_t* entries, unsigned long long* stats, void* stream);
+                     const float4* pv, uint32_t* cnt, uint64_t* entries, unsigned long long* stats,
+                     void* stream);
 int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, uint32_t ntiles,
                   uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries,
                   float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
                   uint32_t* n_rays, void* stream);
-int vfr_k_traverse(uint32_t mode, uint32_t proj_mask, const TravArgs& a, uint32_t max_rays,
+/* occupancy: target waves per SIMD of the persistent kernel variant
+ * (<= 4: as compiled, 5 or 6: register-capped variants). */
+int vfr_k_traverse(uint32_t mode, uint32_t occupancy, const TravArgs& a, uint32_t max_rays,
                    int persistent, uint32_t grid_waves, void* stream);
 
 /* preprocessing on the device */

@@ -41,8 +41,22 @@
 
 /* ------------------------------------------------------------ device maths */
 
-/* |x| < RDELTA with x promoted to double (vfRayKernelCPU.cc:85, 206, 323). */
-DEV bool tiny(float x) { return (double)fabsf(x) < 1e-12; }
+/* |x| < RDELTA with x promoted to double (vfRayKernelCPU.cc:85, 206, 323).
+ * (float)1e-12 is 0x2b8cbccc, just below the double 1e-12, so the double test
+ * holds exactly when |x| is below the next float, 0x2b8cbccd. */
+#define RDELTA_F_ABOVE 0x2b8cbccdu
+DEV bool tiny(float x) { return fabsf(x) < __uint_as_float(RDELTA_F_ABOVE); }
+
+/* (double)s <= (double)c + 1e-12 for floats s and c >= 1e-12
+ * (vfRayKernelCPU.cc:342).  If s <= 2c, s - c is exact in float and the test
+ * is s - c < 0x2b8cbccd; if s > 2c, the float s - c is >= c >= 0x2b8cbccd and
+ * the test is false either way.  This equivalence holds while s is below 937
+ * (beyond that the double sum's rounding could matter), so larger s take the
+ * double path. */
+DEV bool le_slack(float s, float c) {
+  if (s < 937.0f) return (s - c) < __uint_as_float(RDELTA_F_ABOVE);
+  return (double)s <= (double)c + 1e-12;
+}
 
 DEV uint32_t sel4(const uint4& v, uint32_t i) {
   return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
@@ -76,6 +90,27 @@ DEV ViewRegs load_view(const DevView* views, uint32_t vi) {
   for (int k = 0; k < 12; ++k) r.M[k] = views[vi].M[k];
   r.proj = views[vi].proj;
   return r;
+}
+
+/* P-space vertices of a view: pv[vert_off + i] = xform(view, vert[i]). */
+__global__ __launch_bounds__(256) void k_xform_views(const DevView* __restrict__ views, uint32_t v0,
+                                                     uint32_t nv, const float4* __restrict__ vert,
+                                                     uint32_t nV, float4* __restrict__ pv) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (uint64_t)nv * nV) return;
+  const uint32_t vi = v0 + (uint32_t)(gid / nV), i = (uint32_t)(gid % nV);
+  const ViewRegs vr = load_view(views, vi);
+  pv[(size_t)views[vi].vert_off + i] = xform(vr, vert[i]);
+}
+
+int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m, float4* pv,
+                      void* stream) {
+  const uint64_t n = (uint64_t)nv * m.n_verts;
+  if (n == 0) return VFR_OK;
+  hipLaunchKernelGGL(k_xform_views, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, views, v0, nv, (const float4*)m.vert, m.n_verts, pv);
+  CHECK_HIP(hipGetLastError());
+  return VFR_OK;
 }
 
 /* obj = (pix*2)/(real)(res-1) - 1.0 (vfRayKernelCPU.cc:250): the double
@@ -139,7 +174,7 @@ DEV Hit tri_test(float4 a, float4 b, float4 c, float ox, float oy) {
   float cy = (v1x * v0y) - (v1y * v0x);
   float cz = (v0x * v2y) - (v0y * v2x);
   h.cy = cy; h.cz = cz; h.D = cx;
-  h.ok = (cy >= 0.0f) && (cz >= 0.0f) && ((double)(cy + cz) <= (double)cx + 1e-12);
+  h.ok = (cy >= 0.0f) && (cz >= 0.0f) && le_slack(cy + cz, cx);
   return h;
 }
 
@@ -166,10 +201,10 @@ DEV Hit entry_test(float4 a, float4 b, float4 c, float ox, float oy) {
 
 /* Value at the pixel from the inside-test crosses (weights cz/D for b and
  * cy/D for c, exchanged when the edge vectors were swapped). */
-DEV float bary(float A, float B, float C, const Hit& h) {
+DEV float bary(float A, float B, float C, const Hit& h, float invD) {
   float wb = h.sw ? h.cy : h.cz;
   float wc = h.sw ? h.cz : h.cy;
-  return A + ((wb * (B - A) + wc * (C - A)) / h.D);
+  return A + ((wb * (B - A) + wc * (C - A)) * invD);
 }
 
 DEV float clamp01(float s) { return s < 0.0f ? 0.0f : (s > 1.0f ? 1.0f : s); }
@@ -230,16 +265,34 @@ int vfr_k_clear_u32(uint32_t* p, size_t n, void* stream) {
   return VFR_OK;
 }
 
+/* Plain streaming copy (16 B per lane per iteration): the reference point
+ * for HBM bandwidth and for scaling the profiler's byte counters. */
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                              size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+int vfr_k_copy(const void* src, void* dst, size_t bytes, void* stream) {
+  size_t n = bytes / 16;
+  if (n == 0) return VFR_OK;
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, (hipStream_t)stream, (const float4*)src,
+                     (float4*)dst, n);
+  CHECK_HIP(hipGetLastError());
+  return VFR_OK;
+}
+
 /* ----------------------------------------------------------------- raster */
 
 /* Loads the ext face (internal tet, face) vertices in P space: slots
  * (f+0..3)&3, the last one the opposite vertex. */
-DEV void face_verts(const TetRec* rec, const float4* vert, const uint32_t* ext_int, uint32_t e,
-                    const ViewRegs& vr, float4 out[4], uint32_t* tet, uint32_t* face) {
+DEV void face_verts(const TetRec* rec, const float4* pv, const uint32_t* ext_int, uint32_t e,
+                    float4 out[4], uint32_t* tet, uint32_t* face) {
   uint32_t t = ext_int[e * 2], f = ext_int[e * 2 + 1];
   uint4 rv = *reinterpret_cast<const uint4*>(rec[t].v);
 #pragma unroll
-  for (uint32_t k = 0; k < 4; ++k) out[k] = xform(vr, ld_vert(vert, sel4(rv, (f + k) & 3u)));
+  for (uint32_t k = 0; k < 4; ++k) out[k] = pv[sel4(rv, (f + k) & 3u)];
   *tet = t;
   *face = f;
 }
@@ -252,18 +305,17 @@ DEV void face_verts(const TetRec* rec, const float4* vert, const uint32_t* ext_i
  * in main.cc's loop (main.cc:209-218) overwrites the pixel. */
 __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ views, uint32_t v0,
                                                     uint32_t nv, const TetRec* __restrict__ rec,
-                                                    const float4* __restrict__ vert,
+                                                    const float4* __restrict__ pv,
                                                     const uint32_t* __restrict__ ext_int,
                                                     uint32_t n_ext, uint32_t* __restrict__ owner) {
   uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (uint64_t)nv * n_ext) return;
   uint32_t vi = v0 + (uint32_t)(gid / n_ext);
   uint32_t e = (uint32_t)(gid % n_ext);
-  ViewRegs vr = load_view(views, vi);
   const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
   float4 fv[4];
   uint32_t t, f;
-  face_verts(rec, vert, ext_int, e, vr, fv, &t, &f);
+  face_verts(rec, pv + views[vi].vert_off, ext_int, e, fv, &t, &f);
   Planes pl = face_planes(fv[0], fv[1], fv[2]);
   float faceZ = fv[3].x * pl.z[0] + fv[3].y * pl.z[1] + pl.z[2];
   if (faceZ < fv[3].z) return;   /* vfRayKernelCPU.cc:128 */
@@ -288,12 +340,11 @@ __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ 
 }
 
 int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m,
-                     uint32_t* owner, void* stream) {
+                     const float4* pv, uint32_t* owner, void* stream) {
   uint64_t n = (uint64_t)nv * m.n_ext;
   if (n == 0) return VFR_OK;
   hipLaunchKernelGGL(k_raster_ref, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, nv, m.rec, (const float4*)m.vert, m.ext_int,
-                     m.n_ext, owner);
+                     (hipStream_t)stream, views, v0, nv, m.rec, pv, m.ext_int, m.n_ext, owner);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -302,7 +353,7 @@ int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, const DevMe
  * the key (Z at the pixel, ext index) to each pixel its projection holds. */
 __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ views, uint32_t v0,
                                                     uint32_t nv, const TetRec* __restrict__ rec,
-                                                    const float4* __restrict__ vert,
+                                                    const float4* __restrict__ pv,
                                                     const uint32_t* __restrict__ ext_int,
                                                     uint32_t n_ext, uint32_t* __restrict__ cnt,
                                                     uint64_t* __restrict__ entries,
@@ -311,11 +362,10 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
   if (gid >= (uint64_t)nv * n_ext) return;
   uint32_t vi = v0 + (uint32_t)(gid / n_ext);
   uint32_t e = (uint32_t)(gid % n_ext);
-  ViewRegs vr = load_view(views, vi);
   const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
   float4 fv[4];
   uint32_t t, f;
-  face_verts(rec, vert, ext_int, e, vr, fv, &t, &f);
+  face_verts(rec, pv + views[vi].vert_off, ext_int, e, fv, &t, &f);
   Planes pl = face_planes(fv[0], fv[1], fv[2]);
   float faceZ = fv[3].x * pl.z[0] + fv[3].y * pl.z[1] + pl.z[2];
   if (faceZ < fv[3].z) return;
@@ -337,7 +387,7 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
       float oy = pix_to_obj(py, H);
       Hit h = entry_test(fv[0], fv[1], fv[2], ox, oy);
       if (!h.ok) continue;
-      float Zp = bary(fv[0].z, fv[1].z, fv[2].z, h);
+      float Zp = bary(fv[0].z, fv[1].z, fv[2].z, h, 1.0f / h.D);
       uint32_t gp = off + py * W + px;
       uint32_t slot = atomicAdd(&cnt[gp], 1u);
       if (slot < VFR_K) entries[(size_t)gp * VFR_K + slot] = ((uint64_t)fkey(Zp) << 32) | e;
@@ -348,12 +398,13 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
 }
 
 int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m,
-                     uint32_t* cnt, uint64_t* entries, unsigned long long* stats, void* stream) {
+                     const float4* pv, uint32_t* cnt, uint64_t* entries, unsigned long long* stats,
+                     void* stream) {
   uint64_t n = (uint64_t)nv * m.n_ext;
   if (n == 0) return VFR_OK;
   hipLaunchKernelGGL(k_raster_ftb, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, nv, m.rec, (const float4*)m.vert, m.ext_int,
-                     m.n_ext, cnt, entries, stats);
+                     (hipStream_t)stream, views, v0, nv, m.rec, pv, m.ext_int, m.n_ext, cnt, entries,
+                     stats);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -465,34 +516,57 @@ struct Step {
   int i;               /* 1..3, 0 = no exit found */
 };
 
+/* Component-wise selects: keeps every candidate in registers (a select of
+ * whole structs lets the compiler build an indexed stack array instead). */
+DEV float4 sel4f(bool c, float4 a, float4 b) {
+  return make_float4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w);
+}
+
 /* Loads the current tet, rebuilds Q[k] = slot (fe+k)&3 from the carried
  * entry-face vertices plus the one new vertex, and runs the reference's
  * exit-face test on faces (fe+1..3)&3 in that order. */
-DEV Step walk_step(const Walk& w, const TetRec* __restrict__ rec, const float4* __restrict__ vert,
-                   const ViewRegs& vr, float ox, float oy) {
+template <bool FARTHEST>
+DEV Step walk_step(const Walk& w, const TetRec* __restrict__ rec, const float4* __restrict__ pv,
+                   float ox, float oy) {
   const uint4 rv = *reinterpret_cast<const uint4*>(rec[w.t].v);
   const uint4 rn = *reinterpret_cast<const uint4*>(rec[w.t].n);
   const uint32_t fe = w.fe;
   const uint32_t q0 = sel4(rv, fe), q1 = sel4(rv, (fe + 1u) & 3u), q2 = sel4(rv, (fe + 2u) & 3u),
                  q3 = sel4(rv, (fe + 3u) & 3u);
-  const float4 Q3 = xform(vr, ld_vert(vert, q3));
-  const float4 Q0 = q0 == w.h0 ? w.H0 : (q0 == w.h1 ? w.H1 : w.H2);
-  const float4 Q1 = q1 == w.h0 ? w.H0 : (q1 == w.h1 ? w.H1 : w.H2);
-  const float4 Q2 = q2 == w.h0 ? w.H0 : (q2 == w.h1 ? w.H1 : w.H2);
+  const float4 Q3 = pv[q3];
+  const float4 Q0 = sel4f(q0 == w.h0, w.H0, sel4f(q0 == w.h1, w.H1, w.H2));
+  const float4 Q1 = sel4f(q1 == w.h0, w.H0, sel4f(q1 == w.h1, w.H1, w.H2));
+  const float4 Q2 = sel4f(q2 == w.h0, w.H0, sel4f(q2 == w.h1, w.H1, w.H2));
   const Hit h1 = tri_test(Q1, Q2, Q3, ox, oy);
   const Hit h2 = tri_test(Q2, Q3, Q0, ox, oy);
   const Hit h3 = tri_test(Q3, Q0, Q1, ox, oy);
-  Step s;
-  if (h1.ok) {
-    s.i = 1; s.h = h1; s.A = Q1; s.B = Q2; s.C = Q3; s.a = q1; s.b = q2; s.c = q3;
-    s.nb = sel4(rn, (fe + 1u) & 3u);
-  } else if (h2.ok) {
-    s.i = 2; s.h = h2; s.A = Q2; s.B = Q3; s.C = Q0; s.a = q2; s.b = q3; s.c = q0;
-    s.nb = sel4(rn, (fe + 2u) & 3u);
-  } else {
-    s.i = h3.ok ? 3 : 0; s.h = h3; s.A = Q3; s.B = Q0; s.C = Q1; s.a = q3; s.b = q0; s.c = q1;
-    s.nb = sel4(rn, (fe + 3u) & 3u);
+  bool c1 = h1.ok, c2 = !h1.ok && h2.ok;
+  if (FARTHEST && (int)h1.ok + (int)h2.ok + (int)h3.ok > 1) {
+    /* Several faces pass (the pixel is on a projected edge or vertex): take
+     * the exit point farthest along the ray (smallest Z), ties in the
+     * reference's order.  A ray through an edge then always advances; the
+     * first passing face can be a zero-length exit that circles the edge. */
+    const float inf = __uint_as_float(0x7f800000u);
+    const float z1 = h1.ok ? bary(Q1.z, Q2.z, Q3.z, h1, 1.0f / h1.D) : inf;
+    const float z2 = h2.ok ? bary(Q2.z, Q3.z, Q0.z, h2, 1.0f / h2.D) : inf;
+    const float z3 = h3.ok ? bary(Q3.z, Q0.z, Q1.z, h3, 1.0f / h3.D) : inf;
+    c1 = h1.ok && z1 <= z2 && z1 <= z3;
+    c2 = !c1 && h2.ok && z2 <= z3;
   }
+  Step s;
+  s.i = c1 ? 1 : (c2 ? 2 : (h3.ok ? 3 : 0));
+  s.A = sel4f(c1, Q1, sel4f(c2, Q2, Q3));
+  s.B = sel4f(c1, Q2, sel4f(c2, Q3, Q0));
+  s.C = sel4f(c1, Q3, sel4f(c2, Q0, Q1));
+  s.a = c1 ? q1 : (c2 ? q2 : q3);
+  s.b = c1 ? q2 : (c2 ? q3 : q0);
+  s.c = c1 ? q3 : (c2 ? q0 : q1);
+  s.nb = c1 ? sel4(rn, (fe + 1u) & 3u) : (c2 ? sel4(rn, (fe + 2u) & 3u) : sel4(rn, (fe + 3u) & 3u));
+  s.h.cy = c1 ? h1.cy : (c2 ? h2.cy : h3.cy);
+  s.h.cz = c1 ? h1.cz : (c2 ? h2.cz : h3.cz);
+  s.h.D = c1 ? h1.D : (c2 ? h2.D : h3.D);
+  s.h.sw = c1 ? h1.sw : (c2 ? h2.sw : h3.sw);
+  s.h.ok = s.i != 0;
   return s;
 }
 
@@ -509,7 +583,7 @@ struct RayOut {
 DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
   RayOut o;
   o.anomaly = 0; o.broken = 0;
-  const ViewRegs vr = load_view(a.views, vi);
+  const float4* __restrict__ pv = a.vert + a.views[vi].vert_off;
   const uint32_t W = a.views[vi].W, H = a.views[vi].H, tfo = a.views[vi].tf * 256u;
   const uint32_t gp = a.views[vi].pix_off + pix;
   const uint32_t px = pix % W, py = pix / W;
@@ -521,9 +595,9 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
   {
     const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[w.t].v);
     w.h0 = sel4(rv, w.fe); w.h1 = sel4(rv, (w.fe + 1u) & 3u); w.h2 = sel4(rv, (w.fe + 2u) & 3u);
-    w.H0 = xform(vr, ld_vert(a.vert, w.h0));
-    w.H1 = xform(vr, ld_vert(a.vert, w.h1));
-    w.H2 = xform(vr, ld_vert(a.vert, w.h2));
+    w.H0 = pv[w.h0];
+    w.H1 = pv[w.h1];
+    w.H2 = pv[w.h2];
   }
   const Planes cf = face_planes(w.H0, w.H1, w.H2);
   float zPrev = plane_eval(cf.z, ox, oy);
@@ -533,14 +607,14 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
   uint32_t steps = 0;
   while (true) {
     if (++steps > VFR_MAX_STEPS) { o.anomaly = 1; break; }
-    const Step s = walk_step(w, a.rec, a.vert, vr, ox, oy);
+    const Step s = walk_step<false>(w, a.rec, pv, ox, oy);
     if (s.i == 0) { o.anomaly = 1; break; }         /* :353 */
     if (s.nb == VFR_NB_BROKEN) { o.broken = 1; break; } /* :365, not emulated */
     const Planes nf = face_planes(s.A, s.B, s.C);
     const float zNext = plane_eval(nf.z, ox, oy);
     const float sNext = plane_eval(nf.s, ox, oy);
     float dz = fabsf(zPrev - zNext);
-    dz = ((double)dz < 1e-12) ? 0.0f : dz;
+    dz = tiny(dz) ? 0.0f : dz;
     if ((sNext < 0.0f) || (sNext > 1.0f)) break;    /* :446 */
     dz = dz / a.max_edge;
     const float4 cF = a.tf[tfo + min(255u, (uint32_t)(255.0f * sNext))];
@@ -583,66 +657,72 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
  * composited front to back with the front = entry point, early termination
  * at 1 - T >= term, re-entry at the first entry behind the last exit point.
  * The one-ray-per-thread kernel and the persistent work-queue kernel both
- * drive it, so they produce the same bits. */
+ * drive it, so they produce the same bits.  The state is kept small (about
+ * 40 registers) because occupancy hides the walk's dependent loads. */
 struct FtbRay {
-  ViewRegs vr;
   Walk w;
   float4 cP;
-  float ox, oy, rayf, Zprev, dprev, exitZ;
+  float ox, oy, rayf, Zprev, dprev;
   float ar, ag, ab, Tr;
-  uint32_t vi, pix, gp, tfo, nk, cur, steps, entry, anomaly, broken;
+  uint32_t voff, gp, tfo, steps;
+  uint32_t bits; /* cur 0..3 | nk 4..7 | persp 8 | anomaly 9 | broken 10 */
 };
+
+DEV uint32_t ftb_cur(const FtbRay& r) { return r.bits & 15u; }
+DEV uint32_t ftb_nk(const FtbRay& r) { return (r.bits >> 4) & 15u; }
+DEV bool ftb_persp(const FtbRay& r) { return (r.bits >> 8) & 1u; }
 
 /* Starts the walk at entry `cur` of the pixel's sorted list. */
 DEV void ftb_segment(const TravArgs& a, FtbRay& r) {
-  const uint64_t key = a.entries[(size_t)r.gp * VFR_K + r.cur];
+  const uint64_t key = a.entries[(size_t)r.gp * VFR_K + ftb_cur(r)];
   const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull);
+  const float4* __restrict__ pv = a.vert + r.voff;
   r.w.t = a.ext_int[e * 2];
   r.w.fe = a.ext_int[e * 2 + 1];
   const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[r.w.t].v);
   r.w.h0 = sel4(rv, r.w.fe); r.w.h1 = sel4(rv, (r.w.fe + 1u) & 3u); r.w.h2 = sel4(rv, (r.w.fe + 2u) & 3u);
-  r.w.H0 = xform(r.vr, ld_vert(a.vert, r.w.h0));
-  r.w.H1 = xform(r.vr, ld_vert(a.vert, r.w.h1));
-  r.w.H2 = xform(r.vr, ld_vert(a.vert, r.w.h2));
+  r.w.H0 = pv[r.w.h0];
+  r.w.H1 = pv[r.w.h1];
+  r.w.H2 = pv[r.w.h2];
   const Hit he = entry_test(r.w.H0, r.w.H1, r.w.H2, r.ox, r.oy);
-  r.Zprev = bary(r.w.H0.z, r.w.H1.z, r.w.H2.z, he);
-  const float Sprev = bary(r.w.H0.w, r.w.H1.w, r.w.H2.w, he);
+  const float inv = 1.0f / he.D;
+  r.Zprev = bary(r.w.H0.z, r.w.H1.z, r.w.H2.z, he, inv);
+  const float Sprev = bary(r.w.H0.w, r.w.H1.w, r.w.H2.w, he, inv);
   float sprev;
-  if (r.vr.proj == VFR_PROJ_PERSPECTIVE) { r.dprev = 1.0f / r.Zprev; sprev = clamp01(Sprev * r.dprev); }
+  if (ftb_persp(r)) { r.dprev = 1.0f / r.Zprev; sprev = clamp01(Sprev * r.dprev); }
   else { r.dprev = 0.0f; sprev = clamp01(Sprev); }
   r.cP = a.tf[r.tfo + min(255u, (uint32_t)(255.0f * sprev))];
-  r.exitZ = r.Zprev;
 }
 
 DEV void ftb_init(const TravArgs& a, uint32_t vi, uint32_t pix, FtbRay& r) {
-  r.vr = load_view(a.views, vi);
   const uint32_t W = a.views[vi].W, H = a.views[vi].H;
   const float k = a.views[vi].k;
-  r.vi = vi; r.pix = pix;
+  const bool persp = a.views[vi].proj == VFR_PROJ_PERSPECTIVE;
+  r.voff = a.views[vi].vert_off;
   r.tfo = a.views[vi].tf * 256u;
   r.gp = a.views[vi].pix_off + pix;
   r.ox = pix_to_obj(pix % W, W);
   r.oy = pix_to_obj(pix / W, H);
-  r.rayf = r.vr.proj == VFR_PROJ_PERSPECTIVE ? sqrtf(1.0f + (r.ox * r.ox + r.oy * r.oy) * (k * k)) : k;
-  const uint32_t nk = a.cnt[r.gp];
-  r.nk = nk > VFR_K ? VFR_K : nk;
-  r.cur = 0;
+  r.rayf = persp ? sqrtf(1.0f + (r.ox * r.ox + r.oy * r.oy) * (k * k)) : k;
+  uint32_t nk = a.cnt[r.gp];
+  nk = nk > VFR_K ? VFR_K : nk;
+  r.bits = (nk << 4) | (persp ? 256u : 0u);
   r.ar = r.ag = r.ab = 0.0f;
   r.Tr = 1.0f;
-  r.steps = 0; r.anomaly = 0; r.broken = 0;
-  r.entry = (uint32_t)(a.entries[(size_t)r.gp * VFR_K] & 0xFFFFFFFFull);
+  r.steps = 0;
   ftb_segment(a, r);
 }
 
 /* One tet of the walk; returns true when the ray is finished. */
 DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
-  if (++r.steps > VFR_MAX_STEPS) { r.anomaly = 1; return true; }
-  const Step s = walk_step(r.w, a.rec, a.vert, r.vr, r.ox, r.oy);
-  if (s.i == 0) { r.anomaly = 1; return true; }
-  const float Znext = bary(s.A.z, s.B.z, s.C.z, s.h);
-  const float Snext = bary(s.A.w, s.B.w, s.C.w, s.h);
+  if (++r.steps > VFR_MAX_STEPS) { r.bits |= 512u; return true; }
+  const Step s = walk_step<true>(r.w, a.rec, a.vert + r.voff, r.ox, r.oy);
+  if (s.i == 0) { r.bits |= 512u; return true; }
+  const float inv = 1.0f / s.h.D;
+  const float Znext = bary(s.A.z, s.B.z, s.C.z, s.h, inv);
+  const float Snext = bary(s.A.w, s.B.w, s.C.w, s.h, inv);
   float snext, dist;
-  if (r.vr.proj == VFR_PROJ_PERSPECTIVE) {
+  if (ftb_persp(r)) {
     const float dn = 1.0f / Znext;
     snext = clamp01(Snext * dn);
     dist = fabsf(dn - r.dprev) * r.rayf;
@@ -664,14 +744,14 @@ DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
   r.Tr *= zeta;
   r.cP = cN;
   if (1.0f - r.Tr >= a.term) return true;
-  if (s.nb == VFR_NB_BROKEN) { r.broken = 1; return true; }
+  if (s.nb == VFR_NB_BROKEN) { r.bits |= 1024u; return true; }
   if (s.nb == VFR_NB_EXT) {
-    r.exitZ = Znext;
-    ++r.cur;
-    while (r.cur < r.nk &&
-           !(funkey((uint32_t)(a.entries[(size_t)r.gp * VFR_K + r.cur] >> 32)) < r.exitZ))
-      ++r.cur;
-    if (r.cur >= r.nk) return true;
+    /* re-entry: the first entry strictly behind the exit point */
+    uint32_t cur = ftb_cur(r) + 1u;
+    const uint32_t nk = ftb_nk(r);
+    while (cur < nk && !(funkey((uint32_t)(a.entries[(size_t)r.gp * VFR_K + cur] >> 32)) < Znext)) ++cur;
+    if (cur >= nk) return true;
+    r.bits = (r.bits & ~15u) | cur;
     ftb_segment(a, r);
     return false;
   }
@@ -681,13 +761,13 @@ DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
   return false;
 }
 
-DEV RayOut ftb_out(const FtbRay& r) {
+DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
   RayOut o;
   o.c = make_float4(r.ar, r.ag, r.ab, 1.0f - r.Tr);
   o.steps = r.steps > VFR_MAX_STEPS ? VFR_MAX_STEPS : r.steps;
-  o.entry = r.entry;
-  o.anomaly = r.anomaly;
-  o.broken = r.broken;
+  o.entry = (uint32_t)(a.entries[(size_t)r.gp * VFR_K] & 0xFFFFFFFFull);
+  o.anomaly = (r.bits >> 9) & 1u;
+  o.broken = (r.bits >> 10) & 1u;
   return o;
 }
 
@@ -696,7 +776,7 @@ DEV RayOut trace_ftb(const TravArgs& a, uint32_t vi, uint32_t pix) {
   ftb_init(a, vi, pix, r);
   while (!ftb_step(a, r)) {
   }
-  return ftb_out(r);
+  return ftb_out(a, r);
 }
 
 template <uint32_t MODE>
@@ -705,8 +785,7 @@ DEV RayOut trace(const TravArgs& a, uint32_t vi, uint32_t pix) {
   return trace_ftb(a, vi, pix);
 }
 
-DEV void store_ray(const TravArgs& a, uint32_t vi, uint32_t pix, const RayOut& o) {
-  const uint32_t gp = a.views[vi].pix_off + pix;
+DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
   a.rgba[gp] = o.c;
   a.steps[gp] = o.steps;
   a.entry_out[gp] = o.entry;
@@ -722,7 +801,7 @@ __global__ __launch_bounds__(256) void k_trav(TravArgs a) {
     const uint64_t ray = a.rays[i];
     const uint32_t vi = (uint32_t)(ray >> 32), pix = (uint32_t)ray;
     const RayOut o = trace<MODE>(a, vi, pix);
-    store_ray(a, vi, pix, o);
+    store_ray(a, a.views[vi].pix_off + pix, o);
     st = o.steps; an = o.anomaly; br = o.broken;
   }
   st = wave_sum_u64(st);
@@ -742,13 +821,15 @@ __global__ __launch_bounds__(256) void k_trav(TravArgs a) {
  * prefix count), then advances every lane `burst` tets.  A wave exits once
  * the queue is drained and all its lanes are idle; every ray ends within
  * VFR_MAX_STEPS tets, so every wave reaches that exit. */
-__global__ __launch_bounds__(256) void k_trav_persist(TravArgs a, uint32_t burst) {
+template <int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_trav_persist(
+    TravArgs a, uint32_t burst) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t n = *a.n_rays;
   const uint32_t refill = a.refill < 1u ? 1u : (a.refill > 64u ? 64u : a.refill);
   FtbRay r;
   bool active = false, drained = false;
-  unsigned long long st = 0, an = 0, br = 0;
+  uint32_t st = 0, an = 0, br = 0;
   while (true) {
     const uint64_t idle = __ballot(!active);
     const uint32_t ni = (uint32_t)__popcll(idle);
@@ -773,32 +854,34 @@ __global__ __launch_bounds__(256) void k_trav_persist(TravArgs a, uint32_t burst
     }
     for (uint32_t k = 0; k < burst; ++k) {
       if (active && ftb_step(a, r)) {
-        const RayOut o = ftb_out(r);
-        store_ray(a, r.vi, r.pix, o);
+        const RayOut o = ftb_out(a, r);
+        store_ray(a, r.gp, o);
         st += o.steps; an += o.anomaly; br += o.broken;
         active = false;
       }
     }
   }
-  st = wave_sum_u64(st);
-  an = wave_sum_u64(an);
-  br = wave_sum_u64(br);
+  const unsigned long long ws = wave_sum_u64(st), wa = wave_sum_u64(an), wb = wave_sum_u64(br);
   if (lane == 0) {
-    if (st) atomicAdd(&a.stats[ST_STEPS], st);
-    if (an) atomicAdd(&a.stats[ST_ANOM], an);
-    if (br) atomicAdd(&a.stats[4], br);
+    if (ws) atomicAdd(&a.stats[ST_STEPS], ws);
+    if (wa) atomicAdd(&a.stats[ST_ANOM], wa);
+    if (wb) atomicAdd(&a.stats[4], wb);
   }
 }
 
-int vfr_k_traverse(uint32_t mode, uint32_t proj_mask, const TravArgs& a, uint32_t max_rays,
+int vfr_k_traverse(uint32_t mode, uint32_t occupancy, const TravArgs& a, uint32_t max_rays,
                    int persistent, uint32_t grid_waves, void* stream) {
-  (void)proj_mask;
   if (max_rays == 0) return VFR_OK;
   if (persistent && mode == VFR_MODE_FTB) {
     uint32_t blocks = (grid_waves + 3u) / 4u;
     if (blocks == 0) blocks = 1;
     const uint32_t burst = 8;
-    hipLaunchKernelGGL(k_trav_persist, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
+    if (occupancy >= 6)
+      hipLaunchKernelGGL(k_trav_persist<6>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
+    else if (occupancy == 5)
+      hipLaunchKernelGGL(k_trav_persist<5>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
+    else
+      hipLaunchKernelGGL(k_trav_persist<1>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
     CHECK_HIP(hipGetLastError());
     return VFR_OK;
   }
