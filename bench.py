@@ -60,6 +60,21 @@ def cpu_baseline(x, t, tfa, views, seconds: float) -> dict:
                       f"restatement, FP32, OpenMP {threads} threads, {el:.1f} s"}
 
 
+def measured_traffic(workload: str):
+    """HBM/fabric bytes per traversal launch from the committed rocprofv3
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads
+    for a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"k_trav_persist_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    txt = open(path).read()
+    i = txt.rfind("```json")
+    if i < 0:
+        return None
+    blob = json.loads(txt[i + 7: txt.index("```", i + 7)])
+    return blob
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,9 +179,13 @@ def main() -> int:
     roof = None
     if ms_trav:
         ach = alg_bytes / (ms_trav * 1e-3) / 1e9
+        tr = measured_traffic(args.workload)
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
-                "kernel": "k_trav<FTB>", "kernel_ms": round(ms_trav, 3),
+                "frac": round(ach / PEAK_HBM_GBS, 4),
+                "traffic": tr["traffic_bytes"] if tr else None,
+                "traffic_source": f"profiles/k_trav_persist_{args.workload}.md" if tr else None,
+                "l2_hit": tr["l2_hit"] if tr else None,
+                "kernel": "k_trav_persist", "kernel_ms": round(ms_trav, 3),
                 "alg_bytes": int(alg_bytes)}
     cpu = None
     if not args.no_cpu:

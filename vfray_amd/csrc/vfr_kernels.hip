@@ -418,71 +418,83 @@ int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, const DevMe
     e[b] = x_ > y_ ? y_ : x_;            \
   }
 
-/* One wave per 8x8 tile: pixels with an entry become rays (wave ballot,
- * one atomic per wave), inactive pixels get zero outputs, and each active
- * pixel's entry list is sorted front to back (Batcher's 19-comparator
- * network on 8 keys, descending). */
-__global__ __launch_bounds__(256) void k_compact(const DevView* __restrict__ views,
-                                                 uint32_t n_views, uint32_t tile0, uint32_t ntiles,
-                                                 uint32_t mode, const uint32_t* __restrict__ owner,
-                                                 uint32_t* __restrict__ cnt,
-                                                 uint64_t* __restrict__ entries,
-                                                 float4* __restrict__ rgba, uint32_t* __restrict__ steps,
-                                                 uint32_t* __restrict__ entry_out,
-                                                 uint64_t* __restrict__ rays,
-                                                 uint32_t* __restrict__ n_rays) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t wt = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (wt >= ntiles) return;
-  const uint32_t gt = tile0 + wt;
-  /* view owning tile gt: last view with tile_off <= gt */
-  uint32_t lo = 0, hi = n_views;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (views[mid].tile_off <= gt) lo = mid; else hi = mid;
-  }
-  const uint32_t vi = lo;
-  const uint32_t W = views[vi].W, H = views[vi].H;
-  const uint32_t lt = gt - views[vi].tile_off, tx = lt % views[vi].tiles_x, ty = lt / views[vi].tiles_x;
-  const uint32_t px = tx * 8u + (lane & 7u), py = ty * 8u + (lane >> 3);
-  const bool valid = px < W && py < H;
-  const uint32_t pix = py * W + px;
-  const uint32_t gp = views[vi].pix_off + pix;
+/* One wave per 8x8 tile, 16 tiles per 1024-thread block: pixels with an
+ * entry become rays (wave ballot, a prefix over the block's waves in LDS and
+ * one atomic per block, so the ray list keeps tile order within a block),
+ * inactive pixels get zero outputs, and each active pixel's entry list is
+ * sorted front to back (Batcher's 19-comparator network on 8 keys,
+ * descending). */
+#define COMPACT_WAVES 16u
+__global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ views,
+                                                  uint32_t n_views, uint32_t tile0, uint32_t ntiles,
+                                                  uint32_t mode, const uint32_t* __restrict__ owner,
+                                                  uint32_t* __restrict__ cnt,
+                                                  uint64_t* __restrict__ entries,
+                                                  float4* __restrict__ rgba, uint32_t* __restrict__ steps,
+                                                  uint32_t* __restrict__ entry_out,
+                                                  uint64_t* __restrict__ rays,
+                                                  uint32_t* __restrict__ n_rays) {
+  __shared__ uint32_t wave_n[COMPACT_WAVES];
+  __shared__ uint32_t block_base;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t wt = blockIdx.x * COMPACT_WAVES + wave;
   bool active = false;
-  if (valid) {
-    if (mode == VFR_MODE_REF) {
-      active = owner[gp] != 0u;
-    } else {
-      uint32_t nk = cnt[gp];
-      active = nk != 0u;
-      if (nk > 1u) {
-        nk = nk > VFR_K ? VFR_K : nk;
-        uint64_t e[8];
-        uint64_t* ent = entries + (size_t)gp * VFR_K;
-#pragma unroll
-        for (uint32_t k = 0; k < 8; ++k) e[k] = k < nk ? ent[k] : 0ull;
-        CS(0, 1) CS(2, 3) CS(4, 5) CS(6, 7) CS(0, 2) CS(1, 3) CS(4, 6) CS(5, 7) CS(1, 2) CS(5, 6)
-        CS(0, 4) CS(1, 5) CS(2, 6) CS(3, 7) CS(2, 4) CS(3, 5) CS(1, 2) CS(3, 4) CS(5, 6)
-#pragma unroll
-        for (uint32_t k = 0; k < 8; ++k)
-          if (k < nk) ent[k] = e[k];
-      }
+  uint32_t vi = 0, pix = 0;
+  if (wt < ntiles) {
+    const uint32_t gt = tile0 + wt;
+    /* view owning tile gt: last view with tile_off <= gt */
+    uint32_t lo = 0, hi = n_views;
+    while (hi - lo > 1) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (views[mid].tile_off <= gt) lo = mid; else hi = mid;
     }
-    if (!active) {
-      rgba[gp] = make_float4(0.f, 0.f, 0.f, 0.f);
-      steps[gp] = 0u;
-      entry_out[gp] = VFR_NO_ID;
+    vi = lo;
+    const uint32_t W = views[vi].W, H = views[vi].H;
+    const uint32_t lt = gt - views[vi].tile_off, tx = lt % views[vi].tiles_x, ty = lt / views[vi].tiles_x;
+    const uint32_t px = tx * 8u + (lane & 7u), py = ty * 8u + (lane >> 3);
+    const bool valid = px < W && py < H;
+    pix = py * W + px;
+    const uint32_t gp = views[vi].pix_off + pix;
+    if (valid) {
+      if (mode == VFR_MODE_REF) {
+        active = owner[gp] != 0u;
+      } else {
+        uint32_t nk = cnt[gp];
+        active = nk != 0u;
+        if (nk > 1u) {
+          nk = nk > VFR_K ? VFR_K : nk;
+          uint64_t e[8];
+          uint64_t* ent = entries + (size_t)gp * VFR_K;
+#pragma unroll
+          for (uint32_t k = 0; k < 8; ++k) e[k] = k < nk ? ent[k] : 0ull;
+          CS(0, 1) CS(2, 3) CS(4, 5) CS(6, 7) CS(0, 2) CS(1, 3) CS(4, 6) CS(5, 7) CS(1, 2) CS(5, 6)
+          CS(0, 4) CS(1, 5) CS(2, 6) CS(3, 7) CS(2, 4) CS(3, 5) CS(1, 2) CS(3, 4) CS(5, 6)
+#pragma unroll
+          for (uint32_t k = 0; k < 8; ++k)
+            if (k < nk) ent[k] = e[k];
+        }
+      }
+      if (!active) {
+        rgba[gp] = make_float4(0.f, 0.f, 0.f, 0.f);
+        steps[gp] = 0u;
+        entry_out[gp] = VFR_NO_ID;
+      }
     }
   }
   const uint64_t m = __ballot(active);
-  const uint32_t n = (uint32_t)__popcll(m);
-  if (n == 0) return;
-  uint32_t base = 0;
-  if (lane == 0) base = atomicAdd(n_rays, n);
-  base = __shfl(base, 0, 64);
+  if (lane == 0) wave_n[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (uint32_t w = 0; w < COMPACT_WAVES; ++w) tot += wave_n[w];
+    block_base = tot ? atomicAdd(n_rays, tot) : 0u;
+  }
+  __syncthreads();
   if (active) {
-    const uint32_t before = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    rays[base + before] = ((uint64_t)vi << 32) | pix;
+    uint32_t before = block_base;
+    for (uint32_t w = 0; w < wave; ++w) before += wave_n[w];
+    before += (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    rays[before] = ((uint64_t)vi << 32) | pix;
   }
 }
 
@@ -491,9 +503,9 @@ int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, 
                   float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
                   uint32_t* n_rays, void* stream) {
   if (ntiles == 0) return VFR_OK;
-  hipLaunchKernelGGL(k_compact, dim3((ntiles + 3) / 4), dim3(256), 0, (hipStream_t)stream, views,
-                     n_views_total, tile0, ntiles, mode, owner, cnt, entries, rgba, steps,
-                     entry_out, rays, n_rays);
+  hipLaunchKernelGGL(k_compact, dim3((ntiles + COMPACT_WAVES - 1) / COMPACT_WAVES), dim3(1024), 0,
+                     (hipStream_t)stream, views, n_views_total, tile0, ntiles, mode, owner, cnt, entries,
+                     rgba, steps, entry_out, rays, n_rays);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
