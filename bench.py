@@ -10,6 +10,14 @@ frame sync per view) is timed on the same views and reported beside it.
 
 `plain_gpu`: 64^3 grid (1,572,864 tets), one 1024^2 perspective view.
 
+`serve`: 8 resident meshes (32^3 ... 160^3 grids, 0.2M-24.6M tets), 4
+transfer functions each, a stream of requests with a random mesh, TF,
+viewpoint and side in {256..1024}.  The plain renderer (one request at a time,
+frame sync) and the batched server (all requests queued at once) are measured
+on a sample, then 10,000 requests arrive as a Poisson stream at 80% of the
+batched capacity; `value` is the served Mrays/s over that stream, with its
+p50/p99 latency (arrival -> result on the device).
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -75,17 +83,91 @@ def measured_traffic(workload: str):
     return blob
 
 
+def serve_bench(args) -> int:
+    import vfray_amd as V
+    from vfray_amd import workloads as WL
+
+    t0 = time.time()
+    ctx = V.Context(0)
+    meshes, tf_ids, tets = [], [], 0
+    for (x, t, tfs) in WL.serve_meshes():
+        meshes.append(ctx.mesh(x, t))
+        tf_ids.append([ctx.tf(tf) for tf in tfs])
+        tets += t.shape[0]
+    log(f"8 resident meshes, {tets} tets, built in {time.time() - t0:.1f}s")
+    sched = dict(occupancy=args.occupancy or 5, waves_per_cu=args.wpc or 40, refill=args.refill or 32)
+    reqs = WL.serve_requests(args.requests, len(meshes), tf_ids)
+    rays_of = lambda rs: sum(r.view.width * r.view.height for r in rs)
+    n_cap = min(len(reqs), max(200, args.requests // 5))
+    sample = reqs[:n_cap]
+    # plain: one request per launch set, one at a time
+    sp = V.Server(ctx, meshes, hbm_budget=8 << 30, streams=1, max_batch=1, **sched)
+    sp.replay(sample[:20], np.zeros(20))
+    plain = sp.replay(sample, np.zeros(len(sample)))
+    sp.close()
+    log(f"plain capacity {plain['mrays_per_s']:.1f} Mrays/s over {len(sample)} requests")
+    sb = V.Server(ctx, meshes, hbm_budget=32 << 30, streams=2, max_batch=256, **sched)
+    sb.replay(sample[:64], np.zeros(64))
+    cap = sb.replay(sample, np.zeros(len(sample)))
+    log(f"batched capacity {cap['mrays_per_s']:.1f} Mrays/s, mean batch {cap['mean_batch']:.1f}")
+    mean_rays = rays_of(reqs) / len(reqs)
+    rate = args.load * cap["mrays_per_s"] * 1e6 / mean_rays
+    offs = WL.poisson_offsets(len(reqs), rate)
+    served = sb.replay(reqs, offs)
+    sb.close()
+    log(f"served {served['mrays_per_s']:.1f} Mrays/s at {rate:.0f} req/s, "
+        f"p50 {served['latency_p50_ms']:.2f} ms p99 {served['latency_p99_ms']:.2f} ms")
+    out = {
+        "metric": "Mrays/s served (continuous batching, Poisson request stream, front-to-back)",
+        "value": round(served["mrays_per_s"], 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": round(served["makespan_s"] * 1e3, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "serve", "meshes": len(meshes), "tets": int(tets), "requests": len(reqs),
+                   "offered_req_per_s": round(rate, 1), "load_vs_capacity": args.load,
+                   "term_alpha": 0.99, "schedule": sched},
+        "latency_p50_ms": round(served["latency_p50_ms"], 3),
+        "latency_p90_ms": round(served["latency_p90_ms"], 3),
+        "latency_p99_ms": round(served["latency_p99_ms"], 3),
+        "latency_max_ms": round(served["latency_max_ms"], 3),
+        "served": served,
+        "batched_capacity": cap,
+        "plain": plain,
+        "speedup_vs_plain": round(served["mrays_per_s"] / plain["mrays_per_s"], 3),
+        "capacity_vs_plain": round(cap["mrays_per_s"] / plain["mrays_per_s"], 3),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    for m in meshes:
+        m.close()
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="batch64", choices=["batch64", "plain_gpu"])
+    ap.add_argument("--workload", default="batch64", choices=["batch64", "plain_gpu", "serve"])
+    ap.add_argument("--requests", type=int, default=10000, help="serve: requests in the stream")
+    ap.add_argument("--load", type=float, default=0.8, help="serve: offered load / batched capacity")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--occupancy", type=int, default=0)
+    ap.add_argument("--wpc", type=int, default=0)
+    ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--probe", type=int, default=0,
                     help="only run the copy-kernel bandwidth probe over this many bytes")
     ap.add_argument("--diag", action="store_true", help="per-view diagnostics on stderr")
@@ -103,6 +185,9 @@ def main() -> int:
             g = c.bandwidth_probe(args.probe, 20)
         print(json.dumps({"probe_bytes": args.probe, "copy_gbps": round(g, 1)}), flush=True)
         return 0
+
+    if args.workload == "serve":
+        return serve_bench(args)
 
     t0 = time.time()
     if args.workload == "batch64":
@@ -128,6 +213,9 @@ def main() -> int:
         log(f"autotune: {tune}")
     kw = {} if tune is None else {"waves_per_cu": tune["waves_per_cu"], "refill": tune["refill"],
                                   "occupancy": tune["occupancy"]}
+    if tune is None and (args.occupancy or args.wpc or args.refill):
+        kw = {"waves_per_cu": args.wpc, "refill": args.refill, "occupancy": args.occupancy}
+        tune = {"occupancy": args.occupancy, "waves_per_cu": args.wpc, "refill": args.refill}
     pb = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, use_graph=args.graph, **kw)
     for _ in range(args.warmup):
         pb.execute()

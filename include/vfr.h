@@ -199,6 +199,62 @@ int vfr_render(vfr_ctx* ctx, const vfr_mesh* mesh, const vfr_view* views,
                int n_views, const vfr_render_opts* opts, vfr_frame_out* outs,
                vfr_stats* stats);
 
+/* --------------------------------------------------------------- serving */
+/* A render server keeps meshes resident on the device and renders a stream of
+ * requests with continuous batching: requests queue on arrival; whenever one
+ * of `streams` slots is free, the queued requests that fit the slot's share
+ * of the HBM budget (front of the queue first, up to max_batch) are rendered
+ * together in one batched launch set on that slot's stream, so batches that
+ * arrive while others run overlap on the device.  No counterpart in the
+ * reference (one 512x512 frame per process, main.cc:242-380). */
+typedef struct vfr_server_cfg {
+  uint64_t hbm_budget;  /* bytes for per-request buffers, all slots (0 = 16 GiB) */
+  uint32_t streams;     /* slots = concurrent batches (0 = 2)                    */
+  uint32_t max_batch;   /* requests per batch (0 = 256)                          */
+  float term_alpha;     /* early termination (0 = 0.99)                          */
+  uint32_t occupancy;   /* persistent traversal schedule (0 = defaults)          */
+  uint32_t waves_per_cu;
+  uint32_t refill;
+} vfr_server_cfg;
+
+typedef struct vfr_request {
+  int32_t mesh;         /* index into the server's meshes                        */
+  uint32_t keep_image;  /* copy the RGBA result back for vfr_server_wait         */
+  vfr_view view;        /* perspective or orthographic, with its tf_id           */
+} vfr_request;
+
+typedef struct vfr_result {
+  int32_t status;       /* 0 queued, 1 running, 2 done, < 0 error code           */
+  uint32_t width, height;
+  uint32_t batch_size;  /* requests rendered in the same batch                   */
+  double latency_ms;    /* arrival -> result complete on the device              */
+  double queue_ms;      /* arrival -> admitted into a batch                      */
+} vfr_result;
+
+typedef struct vfr_server_report {
+  uint64_t requests, rays, active_rays, steps, anomalies, batches;
+  double makespan_s;    /* first arrival -> last completion                      */
+  double mrays_per_s;   /* rays / makespan                                       */
+  double latency_p50_ms, latency_p90_ms, latency_p99_ms, latency_max_ms;
+  double mean_batch;
+} vfr_server_report;
+
+/* meshes stay owned by the caller and must outlive the server; their
+ * device data is copied into the server's resident arenas. */
+int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const vfr_mesh* const* meshes,
+                      int n_meshes, vfr_server** out);
+/* Queues a request (arrival = now); VFR_EBUDGET if it can never fit a slot. */
+int vfr_server_submit(vfr_server* s, const vfr_request* req, uint64_t* ticket);
+int vfr_server_poll(vfr_server* s, uint64_t ticket, vfr_result* out);
+/* Blocks until done; copies the image (W*H*4 floats) if requested and kept,
+ * then forgets the ticket. */
+int vfr_server_wait(vfr_server* s, uint64_t ticket, vfr_result* out, float* rgba_out);
+/* Load generator: submits reqs[i] at arrival offsets_s[i] seconds after the
+ * call, waits for all, and reports throughput and latency percentiles. */
+int vfr_server_replay(vfr_server* s, const vfr_request* reqs, const double* offsets_s, int n,
+                      vfr_server_report* report);
+void vfr_server_destroy(vfr_server* s);
+
 /* --------------------------------------------------------------- formats */
 /* Text formats of vfRayPreComp.cc, byte-identical output.  Readers allocate
  * with malloc; free with vfr_free. */

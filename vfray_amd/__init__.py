@@ -57,6 +57,32 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerCfg(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("streams", C.c_uint32), ("max_batch", C.c_uint32),
+                ("term_alpha", C.c_float), ("occupancy", C.c_uint32), ("waves_per_cu", C.c_uint32),
+                ("refill", C.c_uint32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("mesh", C.c_int32), ("keep_image", C.c_uint32), ("view", View)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("batch_size", C.c_uint32), ("latency_ms", C.c_double), ("queue_ms", C.c_double)]
+
+
+class Report(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("rays", C.c_uint64), ("active_rays", C.c_uint64),
+                ("steps", C.c_uint64), ("anomalies", C.c_uint64), ("batches", C.c_uint64),
+                ("makespan_s", C.c_double), ("mrays_per_s", C.c_double), ("latency_p50_ms", C.c_double),
+                ("latency_p90_ms", C.c_double), ("latency_p99_ms", C.c_double),
+                ("latency_max_ms", C.c_double), ("mean_batch", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class MeshInfo(C.Structure):
     _fields_ = [("n_verts", C.c_uint32), ("n_tets", C.c_uint32), ("n_ext_faces", C.c_uint32),
                 ("max_edge_length", C.c_float), ("max_z", C.c_float), ("min_z", C.c_float),
@@ -99,6 +125,13 @@ def lib() -> C.CDLL:
                                             C.POINTER(C.c_double)]),
             "vfr_render": (C.c_int, [P, P, C.POINTER(View), C.c_int, C.POINTER(RenderOpts), C.POINTER(FrameOut), C.POINTER(Stats)]),
             "vfr_free": (None, [P]),
+            "vfr_server_create": (C.c_int, [P, C.POINTER(ServerCfg), C.POINTER(P), C.c_int, C.POINTER(P)]),
+            "vfr_server_submit": (C.c_int, [P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+            "vfr_server_poll": (C.c_int, [P, C.c_uint64, C.POINTER(Result)]),
+            "vfr_server_wait": (C.c_int, [P, C.c_uint64, C.POINTER(Result), f32p]),
+            "vfr_server_replay": (C.c_int, [P, C.POINTER(Request), C.POINTER(C.c_double), C.c_int,
+                                            C.POINTER(Report)]),
+            "vfr_server_destroy": (None, [P]),
             "vfr_read_off": (C.c_int, [C.c_char_p, C.POINTER(f32p), u32p, C.POINTER(u32p), u32p]),
             "vfr_write_off": (C.c_int, [C.c_char_p, f32p, u32, u32p, u32]),
             "vfr_normalize_vertices": (None, [f32p, u32]),
@@ -407,4 +440,52 @@ class Plan:
     def close(self) -> None:
         if self._p:
             lib().vfr_plan_destroy(self._p)
+            self._p = C.c_void_p()
+
+
+class Server:
+    """Resident meshes + continuous batching (vfr_server_*)."""
+
+    def __init__(self, ctx: Context, meshes, hbm_budget: int = 0, streams: int = 0, max_batch: int = 0,
+                 term_alpha: float = 0.0, occupancy: int = 0, waves_per_cu: int = 0, refill: int = 0):
+        self.ctx = ctx
+        self.meshes = list(meshes)
+        arr = (C.c_void_p * len(self.meshes))(*[m._p.value for m in self.meshes])
+        cfg = ServerCfg(hbm_budget, streams, max_batch, term_alpha, occupancy, waves_per_cu, refill)
+        self._p = C.c_void_p()
+        _check(lib().vfr_server_create(ctx._p, C.byref(cfg), arr, len(self.meshes), C.byref(self._p)))
+
+    def submit(self, mesh: int, view: View, keep_image: bool = False) -> int:
+        t = C.c_uint64()
+        _check(lib().vfr_server_submit(self._p, C.byref(Request(mesh, 1 if keep_image else 0, view)),
+                                       C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, image: bool = False):
+        r = Result()
+        img = None
+        ptr = None
+        if image:
+            r0 = Result()
+            _check(lib().vfr_server_poll(self._p, ticket, C.byref(r0)))
+            img = np.zeros((r0.height, r0.width, 4), np.float32)
+            ptr = fptr(img)
+        _check(lib().vfr_server_wait(self._p, ticket, C.byref(r), ptr))
+        out = {k: getattr(r, k) for k, _ in r._fields_}
+        if image:
+            out["rgba"] = img
+        return out
+
+    def replay(self, requests, offsets_s) -> dict:
+        n = len(requests)
+        arr = (Request * n)(*requests)
+        off = np.ascontiguousarray(offsets_s, np.float64)
+        rep = Report()
+        _check(lib().vfr_server_replay(self._p, arr, off.ctypes.data_as(C.POINTER(C.c_double)), n,
+                                       C.byref(rep)))
+        return rep.as_dict()
+
+    def close(self) -> None:
+        if self._p:
+            lib().vfr_server_destroy(self._p)
             self._p = C.c_void_p()

@@ -52,6 +52,18 @@ struct vfr_mesh {
 
 extern "C" const char* vfr_version(void) { return "vfr 0.1 gfx950"; }
 
+/* Accessors for the server (vfr_server.hip). */
+int vfr_ctx_device(const vfr_ctx* c) { return c->device; }
+int vfr_ctx_cus(const vfr_ctx* c) { return c->n_cu; }
+const float4* vfr_ctx_tf(const vfr_ctx* c) { return c->tf; }
+uint32_t vfr_ctx_n_tf(const vfr_ctx* c) { return c->n_tf; }
+const float* vfr_ctx_psi(const vfr_ctx* c) { return c->psi; }
+const vfr_ctx* vfr_mesh_ctx(const vfr_mesh* m) { return m->ctx; }
+const DevMesh* vfr_mesh_dev(const vfr_mesh* m) { return &m->d; }
+void vfr_mesh_bbox(const vfr_mesh* m, float lo[3], float hi[3]) {
+  for (int a = 0; a < 3; ++a) { lo[a] = m->bb_lo[a]; hi[a] = m->bb_hi[a]; }
+}
+
 /* ----------------------------------------------------------------- context */
 
 extern "C" int vfr_ctx_create(int device, vfr_ctx** out) {
@@ -294,8 +306,9 @@ struct vfr_plan {
   std::vector<double> frame_ms;           /* last PER_VIEW execute, per view */
 };
 
-/* View set-up on the host, in double, rounded once to float. */
-static int make_view(const vfr_view& in, const vfr_mesh* m, DevView* o) {
+/* View set-up on the host, in double, rounded once to float.  bb_lo/bb_hi:
+ * the mesh's bounding box (every vertex must lie in front of a pinhole eye). */
+int vfr_make_view(const vfr_view& in, const float* bb_lo, const float* bb_hi, DevView* o) {
   memset(o, 0, sizeof(*o));
   if (in.width < 2 || in.height < 2 || in.width > 16384 || in.height > 16384) {
     vfr_set_error("image size %ux%u outside [2,16384]", in.width, in.height);
@@ -345,8 +358,8 @@ static int make_view(const vfr_view& in, const vfr_mesh* m, DevView* o) {
     o->k = (float)k;
     /* every vertex must lie in front of the eye: check the bbox corners */
     for (int q = 0; q < 8; ++q) {
-      double p[3] = {(q & 1) ? m->bb_hi[0] : m->bb_lo[0], (q & 2) ? m->bb_hi[1] : m->bb_lo[1],
-                     (q & 4) ? m->bb_hi[2] : m->bb_lo[2]};
+      double p[3] = {(q & 1) ? bb_hi[0] : bb_lo[0], (q & 2) ? bb_hi[1] : bb_lo[1],
+                     (q & 4) ? bb_hi[2] : bb_lo[2]};
       double dc = f[0] * (p[0] - E[0]) + f[1] * (p[1] - E[1]) + f[2] * (p[2] - E[2]);
       if (!(dc > 1e-3)) {
         vfr_set_error("eye is inside or beside the mesh bounding box (depth %g)", dc);
@@ -383,11 +396,13 @@ static int enqueue_batched(vfr_plan* p) {
   CHECK_RC(vfr_k_clear_u32(p->owner, p->n_pix, st));
   CHECK_RC(vfr_k_clear_u32(p->counters, p->n_views + 2, st));
   CHECK_RC(vfr_k_clear_u32((uint32_t*)p->stats, 2 * ST_N, st));
-  CHECK_RC(vfr_k_xform_views(p->views, 0, p->n_views, m->d, p->pverts, st));
+  const uint64_t nverts = (uint64_t)p->n_views * m->d.n_verts, nfaces = (uint64_t)p->n_views * m->d.n_ext;
+  CHECK_RC(vfr_k_xform_views(p->views, 0, p->n_views, nverts, (const float4*)m->d.vert, p->pverts, st));
   if (p->opts.mode == VFR_MODE_REF)
-    CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, m->d, p->pverts, p->owner, st));
+    CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, nfaces, m->d.rec, m->d.ext_int, p->pverts, p->owner, st));
   else
-    CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d, p->pverts, p->owner, p->entries, p->stats, st));
+    CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, nfaces, m->d.rec, m->d.ext_int, p->pverts, p->owner,
+                              p->entries, p->stats, st));
   CHECK_RC(vfr_k_compact(p->views, p->n_views, 0, p->n_tiles, p->opts.mode, p->owner, p->owner,
                          p->entries, p->rgba, p->steps, p->entry, p->rays, p->counters + p->n_views, st));
   return VFR_OK;
@@ -412,8 +427,6 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   a.entry_out = p->entry;
   a.stats = p->stats;
   a.queue = p->counters + p->n_views + 1;
-  a.max_edge = m->d.lmt[0];
-  a.inv_me = 1.0f / m->d.lmt[0];
   a.term = p->opts.term_alpha;
   a.refill = p->opts.refill;
   return a;
@@ -456,7 +469,7 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   p->hv.resize(n_views);
   uint64_t pix = 0, tiles = 0;
   for (int i = 0; i < n_views; ++i) {
-    int rc = make_view(views[i], m, &p->hv[i]);
+    int rc = vfr_make_view(views[i], m->bb_lo, m->bb_hi, &p->hv[i]);
     if (rc == VFR_OK && (views[i].tf_id < 0 || (uint32_t)views[i].tf_id >= c->n_tf)) {
       vfr_set_error("view %d: unknown transfer function %d", i, views[i].tf_id);
       rc = VFR_EINVAL;
@@ -470,6 +483,14 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
     p->hv[i].tile_off = (uint32_t)tiles;
     p->hv[i].tiles_x = (views[i].width + 7) / 8;
     p->hv[i].vert_off = (uint32_t)((uint64_t)i * m->d.n_verts);
+    p->hv[i].face_pre = (uint32_t)((uint64_t)i * m->d.n_ext);
+    p->hv[i].rec_off = 0;
+    p->hv[i].ext_off = 0;
+    p->hv[i].wvert_off = 0;
+    p->hv[i].n_ext = m->d.n_ext;
+    p->hv[i].n_verts = m->d.n_verts;
+    p->hv[i].max_edge = m->d.lmt[0];
+    p->hv[i].inv_me = 1.0f / m->d.lmt[0];
     pix += (uint64_t)views[i].width * views[i].height;
     tiles += (uint64_t)p->hv[i].tiles_x * ((views[i].height + 7) / 8);
     if (pix >= (1ull << 31) || tiles >= (1ull << 31)) {
@@ -480,8 +501,8 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   }
   p->n_pix = pix;
   p->n_tiles = (uint32_t)tiles;
-  if ((uint64_t)n_views * m->d.n_verts >= (1ull << 32)) {
-    vfr_set_error("plan needs more than 2^32 view vertices");
+  if ((uint64_t)n_views * m->d.n_verts >= (1ull << 32) || (uint64_t)n_views * m->d.n_ext >= (1ull << 32)) {
+    vfr_set_error("plan needs more than 2^32 view vertices or faces");
     delete p;
     return VFR_EINVAL;
   }
@@ -582,11 +603,12 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
       uint32_t ntl = hv.tiles_x * ((hv.H + 7) / 8);
       CHECK_HIP(hipEventRecord(p->fev[2 * v], st));
       CHECK_RC(vfr_k_clear_u32(p->owner + hv.pix_off, npx, st));
-      CHECK_RC(vfr_k_xform_views(p->views, v, 1, m->d, p->pverts, st));
+      CHECK_RC(vfr_k_xform_views(p->views, v, 1, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
       if (p->opts.mode == VFR_MODE_REF)
-        CHECK_RC(vfr_k_raster_ref(p->views, v, 1, m->d, p->pverts, p->owner, st));
+        CHECK_RC(vfr_k_raster_ref(p->views, v, 1, m->d.n_ext, m->d.rec, m->d.ext_int, p->pverts, p->owner, st));
       else
-        CHECK_RC(vfr_k_raster_ftb(p->views, v, 1, m->d, p->pverts, p->owner, p->entries, p->stats, st));
+        CHECK_RC(vfr_k_raster_ftb(p->views, v, 1, m->d.n_ext, m->d.rec, m->d.ext_int, p->pverts, p->owner,
+                                  p->entries, p->stats, st));
       CHECK_RC(vfr_k_compact(p->views, p->n_views, hv.tile_off, ntl, p->opts.mode, p->owner, p->owner,
                              p->entries, p->rgba, p->steps, p->entry, p->rays + hv.pix_off,
                              p->counters + v, st));

@@ -32,19 +32,29 @@ struct alignas(32) TetRec {
   uint32_t n[4];
 };
 
-/* One view of a render plan. */
+/* One view of a render batch, with the offsets of its mesh in the device
+ * arenas (a plan's single mesh, or a server's resident meshes). */
 struct alignas(16) DevView {
-  float M[12];       /* row-major 3x4 view transform                       */
-  float k;           /* ortho: world length per unit Z; persp: tan(fov/2)  */
+  float M[12];        /* row-major 3x4 view transform                       */
+  float k;            /* ortho: world length per unit Z; persp: tan(fov/2)  */
   uint32_t W, H;
-  uint32_t proj;     /* VFR_PROJ_*                                          */
-  uint32_t tf;       /* transfer-function slot                              */
-  uint32_t pix_off;  /* first pixel of this view in the per-pixel arrays    */
-  uint32_t tile_off; /* first 8x8 tile of this view                         */
-  uint32_t tiles_x;  /* 8x8 tiles per image row                             */
-  uint32_t vert_off; /* first vertex of this view in the P-space vertex array */
+  uint32_t proj;      /* VFR_PROJ_*                                          */
+  uint32_t tf;        /* transfer-function slot                              */
+  uint32_t pix_off;   /* first pixel of this view in the per-pixel arrays    */
+  uint32_t tile_off;  /* first 8x8 tile of this view                         */
+  uint32_t tiles_x;   /* 8x8 tiles per image row                             */
+  uint32_t vert_off;  /* first P-space vertex of this view (prefix of n_verts) */
+  uint32_t face_pre;  /* prefix of n_ext over the views before this one      */
+  uint32_t rec_off;   /* mesh: first tet record in the record arena          */
+  uint32_t ext_off;   /* mesh: first external face in the face arena         */
+  uint32_t wvert_off; /* mesh: first world vertex in the vertex arena        */
+  uint32_t n_ext;     /* mesh: external faces                                */
+  uint32_t n_verts;   /* mesh: vertices                                      */
+  float max_edge;     /* mesh: .lmt max edge length                          */
+  float inv_me;       /* 1 / max_edge                                        */
   uint32_t pad_[3];
 };
+static_assert(sizeof(DevView) % 16 == 0, "DevView alignment");
 
 struct DevMesh {
   uint32_t n_verts = 0, n_tets = 0, n_ext = 0;
@@ -65,9 +75,9 @@ enum { ST_STEPS = 0, ST_ANOM = 1, ST_OVERFLOW = 2, ST_RAYS = 3, ST_N = 8 };
 
 #ifdef __HIPCC__
 struct TravArgs {
-  const TetRec* rec;
+  const TetRec* rec;       /* record arena; a view's mesh starts at rec_off   */
   const float4* vert;      /* P-space vertices of all views (+ view vert_off) */
-  const uint32_t* ext_int;
+  const uint32_t* ext_int; /* (internal tet, face) arena; + ext_off           */
   const float4* tf;        /* [slots * 256] */
   const float* psi;        /* [512 * 512]   */
   const DevView* views;
@@ -81,7 +91,7 @@ struct TravArgs {
   uint32_t* entry_out;
   unsigned long long* stats;
   uint32_t* queue;         /* persistent kernel work-queue head */
-  float max_edge, inv_me, term;
+  float term;
   uint32_t refill;         /* persistent: refill when this many lanes idle */
 };
 
@@ -89,13 +99,15 @@ struct TravArgs {
 
 int vfr_k_clear_u32(uint32_t* p, size_t n, void* stream);
 int vfr_k_copy(const void* src, void* dst, size_t bytes, void* stream);
-int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m, float4* pv,
-                      void* stream);
-int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m,
-                     const float4* pv, uint32_t* owner, void* stream);
-int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m,
-                     const float4* pv, uint32_t* cnt, uint64_t* entries, unsigned long long* stats,
-                     void* stream);
+/* Batch launches cover views [v0, v0+nv); `work` is the number of vertices
+ * (xform) or external faces (raster) summed over those views. */
+int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work,
+                      const float4* wvert, float4* pv, void* stream);
+int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const TetRec* rec,
+                     const uint32_t* ext_int, const float4* pv, uint32_t* owner, void* stream);
+int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const TetRec* rec,
+                     const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
+                     unsigned long long* stats, void* stream);
 int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, uint32_t ntiles,
                   uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries,
                   float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
@@ -104,6 +116,20 @@ int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, 
  * (<= 4: as compiled, 5 or 6: register-capped variants). */
 int vfr_k_traverse(uint32_t mode, uint32_t occupancy, const TravArgs& a, uint32_t max_rays,
                    int persistent, uint32_t grid_waves, void* stream);
+
+/* host helpers shared by plans and the server (vfr_api.hip) */
+struct vfr_ctx;
+struct vfr_mesh;
+struct vfr_view;
+int vfr_make_view(const vfr_view& in, const float* bb_lo, const float* bb_hi, DevView* o);
+int vfr_ctx_device(const vfr_ctx* c);
+int vfr_ctx_cus(const vfr_ctx* c);
+const float4* vfr_ctx_tf(const vfr_ctx* c);
+uint32_t vfr_ctx_n_tf(const vfr_ctx* c);
+const float* vfr_ctx_psi(const vfr_ctx* c);
+const vfr_ctx* vfr_mesh_ctx(const vfr_mesh* m);
+const DevMesh* vfr_mesh_dev(const vfr_mesh* m);
+void vfr_mesh_bbox(const vfr_mesh* m, float lo[3], float hi[3]);
 
 /* preprocessing on the device */
 int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,

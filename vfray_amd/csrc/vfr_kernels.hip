@@ -92,23 +92,38 @@ DEV ViewRegs load_view(const DevView* views, uint32_t vi) {
   return r;
 }
 
-/* P-space vertices of a view: pv[vert_off + i] = xform(view, vert[i]). */
-__global__ __launch_bounds__(256) void k_xform_views(const DevView* __restrict__ views, uint32_t v0,
-                                                     uint32_t nv, const float4* __restrict__ vert,
-                                                     uint32_t nV, float4* __restrict__ pv) {
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (uint64_t)nv * nV) return;
-  const uint32_t vi = v0 + (uint32_t)(gid / nV), i = (uint32_t)(gid % nV);
-  const ViewRegs vr = load_view(views, vi);
-  pv[(size_t)views[vi].vert_off + i] = xform(vr, vert[i]);
+/* View of work item `g` of a batch over views [v0, v0+nv): the last view
+ * whose prefix (vert_off or face_pre) is <= the prefix of v0 plus g. */
+DEV uint32_t view_of(const DevView* __restrict__ views, uint32_t v0, uint32_t nv, uint64_t g,
+                     bool faces) {
+  const uint32_t base = faces ? views[v0].face_pre : views[v0].vert_off;
+  uint32_t lo = v0, hi = v0 + nv;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const uint32_t pre = faces ? views[mid].face_pre : views[mid].vert_off;
+    if ((uint64_t)(pre - base) <= g) lo = mid; else hi = mid;
+  }
+  return lo;
 }
 
-int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m, float4* pv,
-                      void* stream) {
-  const uint64_t n = (uint64_t)nv * m.n_verts;
-  if (n == 0) return VFR_OK;
-  hipLaunchKernelGGL(k_xform_views, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, nv, (const float4*)m.vert, m.n_verts, pv);
+/* P-space vertices of each view: pv[vert_off + i] = xform(view, mesh vertex i). */
+__global__ __launch_bounds__(256) void k_xform_views(const DevView* __restrict__ views, uint32_t v0,
+                                                     uint32_t nv, uint64_t work,
+                                                     const float4* __restrict__ wvert,
+                                                     float4* __restrict__ pv) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= work) return;
+  const uint32_t vi = view_of(views, v0, nv, gid, false);
+  const uint32_t i = (uint32_t)(gid - (views[vi].vert_off - views[v0].vert_off));
+  const ViewRegs vr = load_view(views, vi);
+  pv[(size_t)views[vi].vert_off + i] = xform(vr, wvert[(size_t)views[vi].wvert_off + i]);
+}
+
+int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work,
+                      const float4* wvert, float4* pv, void* stream) {
+  if (work == 0) return VFR_OK;
+  hipLaunchKernelGGL(k_xform_views, dim3((unsigned)((work + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, views, v0, nv, work, wvert, pv);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -304,18 +319,20 @@ DEV void face_verts(const TetRec* rec, const float4* pv, const uint32_t* ext_int
  * (vfRayKernelCPU.cc:274).  The largest visible index wins, as the last face
  * in main.cc's loop (main.cc:209-218) overwrites the pixel. */
 __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ views, uint32_t v0,
-                                                    uint32_t nv, const TetRec* __restrict__ rec,
-                                                    const float4* __restrict__ pv,
+                                                    uint32_t nv, uint64_t work,
+                                                    const TetRec* __restrict__ rec,
                                                     const uint32_t* __restrict__ ext_int,
-                                                    uint32_t n_ext, uint32_t* __restrict__ owner) {
+                                                    const float4* __restrict__ pv,
+                                                    uint32_t* __restrict__ owner) {
   uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (uint64_t)nv * n_ext) return;
-  uint32_t vi = v0 + (uint32_t)(gid / n_ext);
-  uint32_t e = (uint32_t)(gid % n_ext);
+  if (gid >= work) return;
+  const uint32_t vi = view_of(views, v0, nv, gid, true);
+  const uint32_t e = (uint32_t)(gid - (views[vi].face_pre - views[v0].face_pre));
   const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
   float4 fv[4];
   uint32_t t, f;
-  face_verts(rec, pv + views[vi].vert_off, ext_int, e, fv, &t, &f);
+  face_verts(rec + views[vi].rec_off, pv + views[vi].vert_off, ext_int + 2 * (size_t)views[vi].ext_off, e,
+             fv, &t, &f);
   Planes pl = face_planes(fv[0], fv[1], fv[2]);
   float faceZ = fv[3].x * pl.z[0] + fv[3].y * pl.z[1] + pl.z[2];
   if (faceZ < fv[3].z) return;   /* vfRayKernelCPU.cc:128 */
@@ -339,12 +356,11 @@ __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ 
   }
 }
 
-int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m,
-                     const float4* pv, uint32_t* owner, void* stream) {
-  uint64_t n = (uint64_t)nv * m.n_ext;
-  if (n == 0) return VFR_OK;
-  hipLaunchKernelGGL(k_raster_ref, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, nv, m.rec, pv, m.ext_int, m.n_ext, owner);
+int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const TetRec* rec,
+                     const uint32_t* ext_int, const float4* pv, uint32_t* owner, void* stream) {
+  if (work == 0) return VFR_OK;
+  hipLaunchKernelGGL(k_raster_ref, dim3((unsigned)((work + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, views, v0, nv, work, rec, ext_int, pv, owner);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -352,20 +368,22 @@ int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, const DevMe
 /* FTB raster: every visible external face (front facing in P space) appends
  * the key (Z at the pixel, ext index) to each pixel its projection holds. */
 __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ views, uint32_t v0,
-                                                    uint32_t nv, const TetRec* __restrict__ rec,
-                                                    const float4* __restrict__ pv,
+                                                    uint32_t nv, uint64_t work,
+                                                    const TetRec* __restrict__ rec,
                                                     const uint32_t* __restrict__ ext_int,
-                                                    uint32_t n_ext, uint32_t* __restrict__ cnt,
+                                                    const float4* __restrict__ pv,
+                                                    uint32_t* __restrict__ cnt,
                                                     uint64_t* __restrict__ entries,
                                                     unsigned long long* __restrict__ stats) {
   uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (uint64_t)nv * n_ext) return;
-  uint32_t vi = v0 + (uint32_t)(gid / n_ext);
-  uint32_t e = (uint32_t)(gid % n_ext);
+  if (gid >= work) return;
+  const uint32_t vi = view_of(views, v0, nv, gid, true);
+  const uint32_t e = (uint32_t)(gid - (views[vi].face_pre - views[v0].face_pre));
   const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
   float4 fv[4];
   uint32_t t, f;
-  face_verts(rec, pv + views[vi].vert_off, ext_int, e, fv, &t, &f);
+  face_verts(rec + views[vi].rec_off, pv + views[vi].vert_off, ext_int + 2 * (size_t)views[vi].ext_off, e,
+             fv, &t, &f);
   Planes pl = face_planes(fv[0], fv[1], fv[2]);
   float faceZ = fv[3].x * pl.z[0] + fv[3].y * pl.z[1] + pl.z[2];
   if (faceZ < fv[3].z) return;
@@ -397,14 +415,12 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
   if (over) atomicAdd(&stats[ST_OVERFLOW], over);
 }
 
-int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, const DevMesh& m,
-                     const float4* pv, uint32_t* cnt, uint64_t* entries, unsigned long long* stats,
-                     void* stream) {
-  uint64_t n = (uint64_t)nv * m.n_ext;
-  if (n == 0) return VFR_OK;
-  hipLaunchKernelGGL(k_raster_ftb, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, nv, m.rec, pv, m.ext_int, m.n_ext, cnt, entries,
-                     stats);
+int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const TetRec* rec,
+                     const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
+                     unsigned long long* stats, void* stream) {
+  if (work == 0) return VFR_OK;
+  hipLaunchKernelGGL(k_raster_ftb, dim3((unsigned)((work + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, views, v0, nv, work, rec, ext_int, pv, cnt, entries, stats);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -596,16 +612,19 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
   RayOut o;
   o.anomaly = 0; o.broken = 0;
   const float4* __restrict__ pv = a.vert + a.views[vi].vert_off;
+  const TetRec* __restrict__ rec = a.rec + a.views[vi].rec_off;
+  const uint32_t* __restrict__ ext = a.ext_int + 2 * (size_t)a.views[vi].ext_off;
+  const float max_edge = a.views[vi].max_edge;
   const uint32_t W = a.views[vi].W, H = a.views[vi].H, tfo = a.views[vi].tf * 256u;
   const uint32_t gp = a.views[vi].pix_off + pix;
   const uint32_t px = pix % W, py = pix / W;
   const float ox = pix_to_obj(px, W), oy = pix_to_obj(py, H);
   const uint32_t e = a.owner[gp] - 1u;
   Walk w;
-  w.t = a.ext_int[e * 2];
-  w.fe = a.ext_int[e * 2 + 1];
+  w.t = ext[e * 2];
+  w.fe = ext[e * 2 + 1];
   {
-    const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[w.t].v);
+    const uint4 rv = *reinterpret_cast<const uint4*>(rec[w.t].v);
     w.h0 = sel4(rv, w.fe); w.h1 = sel4(rv, (w.fe + 1u) & 3u); w.h2 = sel4(rv, (w.fe + 2u) & 3u);
     w.H0 = pv[w.h0];
     w.H1 = pv[w.h1];
@@ -619,7 +638,7 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
   uint32_t steps = 0;
   while (true) {
     if (++steps > VFR_MAX_STEPS) { o.anomaly = 1; break; }
-    const Step s = walk_step<false>(w, a.rec, pv, ox, oy);
+    const Step s = walk_step<false>(w, rec, pv, ox, oy);
     if (s.i == 0) { o.anomaly = 1; break; }         /* :353 */
     if (s.nb == VFR_NB_BROKEN) { o.broken = 1; break; } /* :365, not emulated */
     const Planes nf = face_planes(s.A, s.B, s.C);
@@ -628,7 +647,7 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
     float dz = fabsf(zPrev - zNext);
     dz = tiny(dz) ? 0.0f : dz;
     if ((sNext < 0.0f) || (sNext > 1.0f)) break;    /* :446 */
-    dz = dz / a.max_edge;
+    dz = dz / max_edge;
     const float4 cF = a.tf[tfo + min(255u, (uint32_t)(255.0f * sNext))];
     const float taux = cF.w * dz, tauy = cB.w * dz;
     const float zeta = (float)exp(-((double)taux * 0.5 + (double)tauy * 0.5));
@@ -674,9 +693,9 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
 struct FtbRay {
   Walk w;
   float4 cP;
-  float ox, oy, rayf, Zprev, dprev;
+  float ox, oy, rayf, Zprev, dprev, inv_me;
   float ar, ag, ab, Tr;
-  uint32_t voff, gp, tfo, steps;
+  uint32_t voff, roff, eoff, gp, tfo, steps;
   uint32_t bits; /* cur 0..3 | nk 4..7 | persp 8 | anomaly 9 | broken 10 */
 };
 
@@ -687,11 +706,11 @@ DEV bool ftb_persp(const FtbRay& r) { return (r.bits >> 8) & 1u; }
 /* Starts the walk at entry `cur` of the pixel's sorted list. */
 DEV void ftb_segment(const TravArgs& a, FtbRay& r) {
   const uint64_t key = a.entries[(size_t)r.gp * VFR_K + ftb_cur(r)];
-  const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull);
+  const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull) + r.eoff;
   const float4* __restrict__ pv = a.vert + r.voff;
-  r.w.t = a.ext_int[e * 2];
-  r.w.fe = a.ext_int[e * 2 + 1];
-  const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[r.w.t].v);
+  r.w.t = a.ext_int[2 * (size_t)e];
+  r.w.fe = a.ext_int[2 * (size_t)e + 1];
+  const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[r.roff + r.w.t].v);
   r.w.h0 = sel4(rv, r.w.fe); r.w.h1 = sel4(rv, (r.w.fe + 1u) & 3u); r.w.h2 = sel4(rv, (r.w.fe + 2u) & 3u);
   r.w.H0 = pv[r.w.h0];
   r.w.H1 = pv[r.w.h1];
@@ -711,6 +730,9 @@ DEV void ftb_init(const TravArgs& a, uint32_t vi, uint32_t pix, FtbRay& r) {
   const float k = a.views[vi].k;
   const bool persp = a.views[vi].proj == VFR_PROJ_PERSPECTIVE;
   r.voff = a.views[vi].vert_off;
+  r.roff = a.views[vi].rec_off;
+  r.eoff = a.views[vi].ext_off;
+  r.inv_me = a.views[vi].inv_me;
   r.tfo = a.views[vi].tf * 256u;
   r.gp = a.views[vi].pix_off + pix;
   r.ox = pix_to_obj(pix % W, W);
@@ -728,7 +750,7 @@ DEV void ftb_init(const TravArgs& a, uint32_t vi, uint32_t pix, FtbRay& r) {
 /* One tet of the walk; returns true when the ray is finished. */
 DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
   if (++r.steps > VFR_MAX_STEPS) { r.bits |= 512u; return true; }
-  const Step s = walk_step<true>(r.w, a.rec, a.vert + r.voff, r.ox, r.oy);
+  const Step s = walk_step<true>(r.w, a.rec + r.roff, a.vert + r.voff, r.ox, r.oy);
   if (s.i == 0) { r.bits |= 512u; return true; }
   const float inv = 1.0f / s.h.D;
   const float Znext = bary(s.A.z, s.B.z, s.C.z, s.h, inv);
@@ -743,7 +765,7 @@ DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
     snext = clamp01(Snext);
     dist = fabsf(r.Zprev - Znext) * r.rayf;
   }
-  const float dl = dist * a.inv_me;
+  const float dl = dist * r.inv_me;
   const float4 cN = a.tf[r.tfo + min(255u, (uint32_t)(255.0f * snext))];
   const float tf_ = r.cP.w * dl, tb_ = cN.w * dl;
   const float zeta = vfr_expf(-(tf_ + tb_) * 0.5f);

@@ -1,0 +1,548 @@
+/*
+ * vfr_server.hip -- the render server: resident meshes, admission by HBM
+ * footprint, continuous batching over a few streams, latency accounting.
+ *
+ * Each slot owns a stream and device buffers sized to its share of the HBM
+ * budget, allocated once.  A dispatcher thread polls the slots' completion
+ * events and, whenever a slot is free, admits the queued requests that fit
+ * it (oldest first, up to max_batch) and enqueues the batched launch set:
+ * view upload, clears, per-view vertex transform, visibility/raster,
+ * compaction and the persistent traversal.  Batches on different slots
+ * overlap on the device; the dispatcher never waits on the device.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/vfr.h"
+#include "vfr_internal.h"
+
+#define CHECK_HIP(x)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) {                                                               \
+      vfr_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_));       \
+      return VFR_EHIP;                                                                    \
+    }                                                                                     \
+  } while (0)
+
+#define CHECK_RC(x)                \
+  do {                             \
+    int rc_ = (x);                 \
+    if (rc_ != VFR_OK) return rc_; \
+  } while (0)
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+inline double secs(Clock::time_point t) {
+  return std::chrono::duration<double>(t.time_since_epoch()).count();
+}
+
+/* Device bytes a request needs: per pixel the entry count, VFR_K entry keys,
+ * a ray slot, RGBA, steps and entry id; per mesh vertex one P-space vertex. */
+inline uint64_t pixel_bytes() { return 4 + 8ull * VFR_K + 8 + 16 + 4 + 4; }
+
+struct SrvMesh {
+  uint32_t rec_off, ext_off, wvert_off, n_ext, n_verts, n_tets;
+  float max_edge;
+  float bb_lo[3], bb_hi[3];
+};
+
+struct Req {
+  uint64_t ticket;
+  int mesh;
+  vfr_view view;
+  DevView dv;
+  uint32_t keep;
+  uint64_t pixels;
+  uint64_t keep_off = 0;   /* offset of the kept image in the slot's staging */
+  double t_arrival = 0, t_admit = 0, t_done = 0;
+  int32_t status = 0;
+  uint32_t batch = 0;
+  std::vector<float> image;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  uint64_t cap_pix = 0, cap_verts = 0;
+  uint32_t cap_views = 0;
+  DevView* views = nullptr;
+  DevView* h_views = nullptr;   /* pinned */
+  uint32_t* cnt = nullptr;
+  uint64_t* entries = nullptr;
+  uint64_t* rays = nullptr;
+  float4* pverts = nullptr;
+  float4* rgba = nullptr;
+  uint32_t* steps = nullptr;
+  uint32_t* entry = nullptr;
+  uint32_t* counters = nullptr; /* [0] rays, [1] queue head */
+  unsigned long long* stats = nullptr;
+  unsigned long long* h_stats = nullptr; /* pinned */
+  uint32_t* h_counters = nullptr;        /* pinned */
+  float* h_img = nullptr;                /* pinned staging for kept images */
+  uint64_t cap_keep = 0;                 /* staging capacity in pixels */
+  std::vector<Req*> batch;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct vfr_server {
+  vfr_ctx* ctx = nullptr;
+  vfr_server_cfg cfg{};
+  std::vector<SrvMesh> meshes;
+  TetRec* rec = nullptr;
+  uint32_t* ext = nullptr;
+  float4* vert = nullptr;
+  std::vector<std::unique_ptr<Slot>> slots;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Req*> queue;
+  std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
+  uint64_t next_ticket = 1;
+  std::thread worker;
+  bool stop = false;
+  int worker_error = VFR_OK;
+  /* accounting */
+  uint64_t n_batches = 0, n_batched_reqs = 0, steps = 0, anomalies = 0, active_rays = 0;
+};
+
+static void slot_free(Slot* s) {
+  void* dev[] = {s->views, s->cnt, s->entries, s->rays, s->pverts, s->rgba, s->steps, s->entry,
+                 s->counters, s->stats};
+  for (void* p : dev)
+    if (p) (void)hipFree(p);
+  void* host[] = {s->h_views, s->h_stats, s->h_counters, s->h_img};
+  for (void* p : host)
+    if (p) (void)hipHostFree(p);
+  if (s->done) (void)hipEventDestroy(s->done);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+static int slot_alloc(Slot* s, uint64_t bytes, uint32_t max_batch) {
+  /* three quarters for per-pixel buffers, one quarter for P-space vertices */
+  s->cap_pix = (bytes * 3 / 4) / pixel_bytes();
+  s->cap_verts = (bytes / 4) / 16;
+  s->cap_views = max_batch;
+  CHECK_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  CHECK_HIP(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
+  CHECK_HIP(hipMalloc((void**)&s->views, sizeof(DevView) * s->cap_views));
+  CHECK_HIP(hipMalloc((void**)&s->cnt, 4 * s->cap_pix));
+  CHECK_HIP(hipMalloc((void**)&s->entries, 8ull * VFR_K * s->cap_pix));
+  CHECK_HIP(hipMalloc((void**)&s->rays, 8 * s->cap_pix));
+  CHECK_HIP(hipMalloc((void**)&s->pverts, 16 * s->cap_verts));
+  CHECK_HIP(hipMalloc((void**)&s->rgba, 16 * s->cap_pix));
+  CHECK_HIP(hipMalloc((void**)&s->steps, 4 * s->cap_pix));
+  CHECK_HIP(hipMalloc((void**)&s->entry, 4 * s->cap_pix));
+  CHECK_HIP(hipMalloc((void**)&s->counters, 16));
+  CHECK_HIP(hipMalloc((void**)&s->stats, sizeof(unsigned long long) * ST_N));
+  CHECK_HIP(hipHostMalloc((void**)&s->h_views, sizeof(DevView) * s->cap_views, hipHostMallocDefault));
+  CHECK_HIP(hipHostMalloc((void**)&s->h_stats, sizeof(unsigned long long) * ST_N, hipHostMallocDefault));
+  CHECK_HIP(hipHostMalloc((void**)&s->h_counters, 16, hipHostMallocDefault));
+  return VFR_OK;
+}
+
+/* Enqueues one batch on the slot's stream.  Caller holds no lock. */
+static int launch_batch(vfr_server* sv, Slot* s) {
+  const uint32_t nv = (uint32_t)s->batch.size();
+  uint64_t pix = 0, tiles = 0, verts = 0, faces = 0, keep_pix = 0;
+  for (uint32_t i = 0; i < nv; ++i) {
+    Req* r = s->batch[i];
+    const SrvMesh& m = sv->meshes[r->mesh];
+    DevView d = r->dv;
+    d.pix_off = (uint32_t)pix;
+    d.tile_off = (uint32_t)tiles;
+    d.tiles_x = (r->view.width + 7) / 8;
+    d.vert_off = (uint32_t)verts;
+    d.face_pre = (uint32_t)faces;
+    s->h_views[i] = d;
+    pix += r->pixels;
+    tiles += (uint64_t)d.tiles_x * ((r->view.height + 7) / 8);
+    verts += m.n_verts;
+    faces += m.n_ext;
+    if (r->keep) {
+      r->keep_off = keep_pix;
+      keep_pix += r->pixels;
+    }
+  }
+  hipStream_t st = s->stream;
+  CHECK_HIP(hipMemcpyAsync(s->views, s->h_views, sizeof(DevView) * nv, hipMemcpyHostToDevice, st));
+  CHECK_RC(vfr_k_clear_u32(s->cnt, pix, st));
+  CHECK_RC(vfr_k_clear_u32(s->counters, 4, st));
+  CHECK_RC(vfr_k_clear_u32((uint32_t*)s->stats, 2 * ST_N, st));
+  CHECK_RC(vfr_k_xform_views(s->views, 0, nv, verts, sv->vert, s->pverts, st));
+  CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, faces, sv->rec, sv->ext, s->pverts, s->cnt, s->entries,
+                            s->stats, st));
+  CHECK_RC(vfr_k_compact(s->views, nv, 0, (uint32_t)tiles, VFR_MODE_FTB, nullptr, s->cnt, s->entries,
+                         s->rgba, s->steps, s->entry, s->rays, s->counters, st));
+  TravArgs a;
+  memset(&a, 0, sizeof(a));
+  a.rec = sv->rec;
+  a.vert = s->pverts;
+  a.ext_int = sv->ext;
+  a.tf = vfr_ctx_tf(sv->ctx);
+  a.psi = vfr_ctx_psi(sv->ctx);
+  a.views = s->views;
+  a.rays = s->rays;
+  a.n_rays = s->counters;
+  a.cnt = s->cnt;
+  a.entries = s->entries;
+  a.rgba = s->rgba;
+  a.steps = s->steps;
+  a.entry_out = s->entry;
+  a.stats = s->stats;
+  a.queue = s->counters + 1;
+  a.term = sv->cfg.term_alpha;
+  a.refill = sv->cfg.refill;
+  CHECK_RC(vfr_k_traverse(VFR_MODE_FTB, sv->cfg.occupancy, a, (uint32_t)pix, 1,
+                          sv->cfg.waves_per_cu * (uint32_t)vfr_ctx_cus(sv->ctx), st));
+  CHECK_HIP(hipMemcpyAsync(s->h_stats, s->stats, sizeof(unsigned long long) * ST_N, hipMemcpyDeviceToHost, st));
+  CHECK_HIP(hipMemcpyAsync(s->h_counters, s->counters, 16, hipMemcpyDeviceToHost, st));
+  if (keep_pix) {
+    for (uint32_t i = 0; i < nv; ++i) {
+      Req* r = s->batch[i];
+      if (!r->keep) continue;
+      const DevView& d = s->h_views[i];
+      CHECK_HIP(hipMemcpyAsync(s->h_img + 4ull * r->keep_off, s->rgba + d.pix_off, 16ull * r->pixels,
+                               hipMemcpyDeviceToHost, st));
+    }
+  }
+  CHECK_HIP(hipEventRecord(s->done, st));
+  return VFR_OK;
+}
+
+/* Results of a finished slot; caller holds the lock. */
+static void finish_batch(vfr_server* sv, Slot* s, double now) {
+  for (uint32_t i = 0; i < s->batch.size(); ++i) {
+    Req* r = s->batch[i];
+    r->t_done = now;
+    r->status = 2;
+    if (r->keep) r->image.assign(s->h_img + 4ull * r->keep_off, s->h_img + 4ull * (r->keep_off + r->pixels));
+  }
+  sv->n_batches++;
+  sv->n_batched_reqs += s->batch.size();
+  sv->steps += s->h_stats[ST_STEPS];
+  sv->anomalies += s->h_stats[ST_ANOM];
+  sv->active_rays += s->h_counters[0];
+  s->batch.clear();
+  s->busy = false;
+}
+
+static void dispatcher(vfr_server* sv) {
+  (void)hipSetDevice(vfr_ctx_device(sv->ctx));
+  std::unique_lock<std::mutex> lk(sv->mu);
+  while (true) {
+    bool any_busy = false;
+    const double now = secs(Clock::now());
+    for (auto& sp : sv->slots) {
+      Slot* s = sp.get();
+      if (!s->busy) continue;
+      hipError_t q = hipEventQuery(s->done);
+      if (q == hipSuccess) {
+        finish_batch(sv, s, now);
+        sv->cv_done.notify_all();
+      } else if (q != hipErrorNotReady) {
+        vfr_set_error("batch failed: %s", hipGetErrorString(q));
+        sv->worker_error = VFR_EHIP;
+        for (Req* r : s->batch) r->status = VFR_EHIP;
+        s->batch.clear();
+        s->busy = false;
+        sv->cv_done.notify_all();
+      } else {
+        any_busy = true;
+      }
+    }
+    /* admission: oldest requests first into each free slot */
+    for (auto& sp : sv->slots) {
+      Slot* s = sp.get();
+      if (s->busy || sv->queue.empty()) continue;
+      uint64_t pix = 0, verts = 0, keep = 0;
+      while (!sv->queue.empty() && s->batch.size() < s->cap_views) {
+        Req* r = sv->queue.front();
+        const SrvMesh& m = sv->meshes[r->mesh];
+        const uint64_t kp = r->keep ? r->pixels : 0;
+        if (pix + r->pixels > s->cap_pix || verts + m.n_verts > s->cap_verts || keep + kp > s->cap_keep) break;
+        pix += r->pixels;
+        verts += m.n_verts;
+        keep += kp;
+        r->t_admit = secs(Clock::now());
+        r->status = 1;
+        s->batch.push_back(r);
+        sv->queue.pop_front();
+      }
+      if (s->batch.empty()) continue;
+      for (Req* r : s->batch) r->batch = (uint32_t)s->batch.size();
+      s->busy = true;
+      any_busy = true;
+      lk.unlock();
+      int rc = launch_batch(sv, s);
+      lk.lock();
+      if (rc != VFR_OK) {
+        sv->worker_error = rc;
+        for (Req* r : s->batch) r->status = rc;
+        s->batch.clear();
+        s->busy = false;
+        sv->cv_done.notify_all();
+      }
+    }
+    if (sv->stop && !any_busy && sv->queue.empty()) break;
+    if (!any_busy && sv->queue.empty()) {
+      sv->cv_work.wait(lk, [sv] { return sv->stop || !sv->queue.empty(); });
+    } else {
+      /* device work in flight: poll its events at a fine grain */
+      lk.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      lk.lock();
+    }
+  }
+}
+
+extern "C" void vfr_server_destroy(vfr_server* sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->stop = true;
+  }
+  sv->cv_work.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  (void)hipSetDevice(vfr_ctx_device(sv->ctx));
+  for (auto& sp : sv->slots) {
+    if (sp->stream) (void)hipStreamSynchronize(sp->stream);
+    slot_free(sp.get());
+  }
+  if (sv->rec) (void)hipFree(sv->rec);
+  if (sv->ext) (void)hipFree(sv->ext);
+  if (sv->vert) (void)hipFree(sv->vert);
+  delete sv;
+}
+
+extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const vfr_mesh* const* meshes,
+                                 int n_meshes, vfr_server** out) {
+  if (!ctx || !meshes || n_meshes < 1 || !out) { vfr_set_error("vfr_server_create: bad argument"); return VFR_EINVAL; }
+  *out = nullptr;
+  vfr_server_cfg c{};
+  if (cfg) c = *cfg;
+  if (c.hbm_budget == 0) c.hbm_budget = 16ull << 30;
+  if (c.streams == 0) c.streams = 2;
+  if (c.max_batch == 0) c.max_batch = 256;
+  if (!(c.term_alpha > 0.0f)) c.term_alpha = 0.99f;
+  if (c.occupancy == 0) c.occupancy = 4;
+  if (c.waves_per_cu == 0) c.waves_per_cu = 16;
+  if (c.refill == 0) c.refill = 32;
+  if (c.streams > 16 || c.max_batch > 4096 || c.term_alpha > 1.0f || c.waves_per_cu > 64 || c.refill > 64) {
+    vfr_set_error("vfr_server_create: configuration out of range");
+    return VFR_EINVAL;
+  }
+  uint64_t tets = 0, exts = 0, verts = 0;
+  for (int i = 0; i < n_meshes; ++i) {
+    if (!meshes[i] || vfr_mesh_ctx(meshes[i]) != ctx) { vfr_set_error("mesh %d is null or from another context", i); return VFR_EINVAL; }
+    const DevMesh* d = vfr_mesh_dev(meshes[i]);
+    tets += d->n_tets;
+    exts += d->n_ext;
+    verts += d->n_verts;
+  }
+  if (tets >= (1ull << 31) || exts >= (1ull << 31) || verts >= (1ull << 31)) {
+    vfr_set_error("resident meshes exceed 2^31 tets, faces or vertices");
+    return VFR_EINVAL;
+  }
+  vfr_server* sv = new (std::nothrow) vfr_server;
+  if (!sv) { vfr_set_error("out of memory"); return VFR_ENOMEM; }
+  sv->ctx = ctx;
+  sv->cfg = c;
+  CHECK_HIP(hipSetDevice(vfr_ctx_device(ctx)));
+  int rc = VFR_OK;
+  if (hipMalloc((void**)&sv->rec, sizeof(TetRec) * tets) != hipSuccess ||
+      hipMalloc((void**)&sv->ext, 8 * (exts + 1)) != hipSuccess ||
+      hipMalloc((void**)&sv->vert, 16 * verts) != hipSuccess) {
+    vfr_set_error("resident mesh arena allocation failed");
+    rc = VFR_ENOMEM;
+  }
+  uint64_t ro = 0, eo = 0, vo = 0;
+  for (int i = 0; rc == VFR_OK && i < n_meshes; ++i) {
+    const DevMesh* d = vfr_mesh_dev(meshes[i]);
+    SrvMesh m;
+    m.rec_off = (uint32_t)ro; m.ext_off = (uint32_t)eo; m.wvert_off = (uint32_t)vo;
+    m.n_ext = d->n_ext; m.n_verts = d->n_verts; m.n_tets = d->n_tets;
+    m.max_edge = d->lmt[0];
+    vfr_mesh_bbox(meshes[i], m.bb_lo, m.bb_hi);
+    if (hipMemcpy(sv->rec + ro, d->rec, sizeof(TetRec) * d->n_tets, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(sv->ext + 2 * eo, d->ext_int, 8ull * d->n_ext, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(sv->vert + vo, d->vert, 16ull * d->n_verts, hipMemcpyDeviceToDevice) != hipSuccess) {
+      vfr_set_error("copy of mesh %d into the resident arena failed", i);
+      rc = VFR_EHIP;
+    }
+    sv->meshes.push_back(m);
+    ro += d->n_tets; eo += d->n_ext; vo += d->n_verts;
+  }
+  for (uint32_t i = 0; rc == VFR_OK && i < c.streams; ++i) {
+    sv->slots.emplace_back(new Slot);
+    rc = slot_alloc(sv->slots.back().get(), c.hbm_budget / c.streams, c.max_batch);
+    Slot* sl = sv->slots.back().get();
+    sl->cap_keep = std::min<uint64_t>(sl->cap_pix, 1ull << 22); /* 4 Mpixel = 64 MB of RGBA */
+    if (rc == VFR_OK && hipHostMalloc((void**)&sl->h_img, 16 * sl->cap_keep, hipHostMallocDefault) != hipSuccess) {
+      vfr_set_error("pinned staging allocation failed");
+      rc = VFR_ENOMEM;
+    }
+  }
+  if (rc != VFR_OK) { vfr_server_destroy(sv); return rc; }
+  sv->worker = std::thread(dispatcher, sv);
+  *out = sv;
+  return VFR_OK;
+}
+
+static int submit_at(vfr_server* sv, const vfr_request* rq, double t_arrival, uint64_t* ticket) {
+  if (!sv || !rq || !ticket) { vfr_set_error("vfr_server_submit: bad argument"); return VFR_EINVAL; }
+  if (rq->mesh < 0 || (size_t)rq->mesh >= sv->meshes.size()) { vfr_set_error("unknown mesh %d", rq->mesh); return VFR_EINVAL; }
+  if (rq->view.tf_id < 0 || (uint32_t)rq->view.tf_id >= vfr_ctx_n_tf(sv->ctx)) {
+    vfr_set_error("unknown transfer function %d", rq->view.tf_id);
+    return VFR_EINVAL;
+  }
+  const SrvMesh& m = sv->meshes[rq->mesh];
+  std::unique_ptr<Req> r(new (std::nothrow) Req);
+  if (!r) { vfr_set_error("out of memory"); return VFR_ENOMEM; }
+  CHECK_RC(vfr_make_view(rq->view, m.bb_lo, m.bb_hi, &r->dv));
+  r->dv.rec_off = m.rec_off;
+  r->dv.ext_off = m.ext_off;
+  r->dv.wvert_off = m.wvert_off;
+  r->dv.n_ext = m.n_ext;
+  r->dv.n_verts = m.n_verts;
+  r->dv.max_edge = m.max_edge;
+  r->dv.inv_me = 1.0f / m.max_edge;
+  r->mesh = rq->mesh;
+  r->view = rq->view;
+  r->keep = rq->keep_image;
+  r->pixels = (uint64_t)rq->view.width * rq->view.height;
+  const Slot* s0 = sv->slots[0].get();
+  if (r->pixels > s0->cap_pix || m.n_verts > s0->cap_verts || (r->keep && r->pixels > s0->cap_keep)) {
+    vfr_set_error("request of %llu pixels on a %u-vertex mesh exceeds a slot's share of the HBM budget",
+                  (unsigned long long)r->pixels, m.n_verts);
+    return VFR_EBUDGET;
+  }
+  r->t_arrival = t_arrival;
+  std::lock_guard<std::mutex> g(sv->mu);
+  if (sv->worker_error != VFR_OK) { vfr_set_error("server stopped after an error"); return sv->worker_error; }
+  r->ticket = sv->next_ticket++;
+  *ticket = r->ticket;
+  sv->queue.push_back(r.get());
+  sv->reqs.emplace(r->ticket, std::move(r));
+  sv->cv_work.notify_one();
+  return VFR_OK;
+}
+
+extern "C" int vfr_server_submit(vfr_server* sv, const vfr_request* rq, uint64_t* ticket) {
+  return submit_at(sv, rq, secs(Clock::now()), ticket);
+}
+
+static void fill_result(const Req* r, vfr_result* out) {
+  out->status = r->status;
+  out->width = r->view.width;
+  out->height = r->view.height;
+  out->batch_size = r->batch;
+  out->latency_ms = r->status == 2 ? (r->t_done - r->t_arrival) * 1e3 : 0.0;
+  out->queue_ms = r->status >= 1 ? (r->t_admit - r->t_arrival) * 1e3 : 0.0;
+}
+
+extern "C" int vfr_server_poll(vfr_server* sv, uint64_t ticket, vfr_result* out) {
+  if (!sv || !out) { vfr_set_error("vfr_server_poll: bad argument"); return VFR_EINVAL; }
+  std::lock_guard<std::mutex> g(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) { vfr_set_error("unknown ticket %llu", (unsigned long long)ticket); return VFR_EINVAL; }
+  fill_result(it->second.get(), out);
+  return VFR_OK;
+}
+
+extern "C" int vfr_server_wait(vfr_server* sv, uint64_t ticket, vfr_result* out, float* rgba_out) {
+  if (!sv) { vfr_set_error("vfr_server_wait: bad argument"); return VFR_EINVAL; }
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) { vfr_set_error("unknown ticket %llu", (unsigned long long)ticket); return VFR_EINVAL; }
+  Req* r = it->second.get();
+  sv->cv_done.wait(lk, [r] { return r->status == 2 || r->status < 0; });
+  if (out) fill_result(r, out);
+  int rc = r->status < 0 ? r->status : VFR_OK;
+  if (rc == VFR_OK && rgba_out) {
+    if (!r->keep) { vfr_set_error("request was submitted without keep_image"); rc = VFR_ESTATE; }
+    else memcpy(rgba_out, r->image.data(), sizeof(float) * r->image.size());
+  }
+  sv->reqs.erase(it);
+  return rc;
+}
+
+static double pctl(std::vector<double>& v, double q) {
+  if (v.empty()) return 0.0;
+  std::sort(v.begin(), v.end());
+  double pos = q * (v.size() - 1);
+  size_t lo = (size_t)pos, hi = std::min(lo + 1, v.size() - 1);
+  return v[lo] + (pos - lo) * (v[hi] - v[lo]);
+}
+
+extern "C" int vfr_server_replay(vfr_server* sv, const vfr_request* reqs, const double* offsets, int n,
+                                 vfr_server_report* rep) {
+  if (!sv || !reqs || !offsets || n < 1 || !rep) { vfr_set_error("vfr_server_replay: bad argument"); return VFR_EINVAL; }
+  for (int i = 1; i < n; ++i)
+    if (offsets[i] < offsets[i - 1]) { vfr_set_error("arrival offsets must be non-decreasing"); return VFR_EINVAL; }
+  uint64_t b0, s0, a0, an0, ar0;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    b0 = sv->n_batches; s0 = sv->steps; a0 = sv->anomalies; an0 = sv->n_batched_reqs; ar0 = sv->active_rays;
+  }
+  std::vector<uint64_t> tickets(n);
+  const auto t0 = Clock::now();
+  const double t0s = secs(t0);
+  int rc = VFR_OK;
+  for (int i = 0; i < n && rc == VFR_OK; ++i) {
+    std::this_thread::sleep_until(t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(offsets[i])));
+    rc = submit_at(sv, &reqs[i], t0s + offsets[i], &tickets[i]);
+  }
+  if (rc != VFR_OK) return rc;
+  std::vector<double> lat(n);
+  double last = 0;
+  uint64_t rays = 0;
+  for (int i = 0; i < n; ++i) {
+    vfr_result r;
+    double t_done = 0;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      Req* q = sv->reqs[tickets[i]].get();
+      sv->cv_done.wait(lk, [q] { return q->status == 2 || q->status < 0; });
+      t_done = q->t_done;
+    }
+    int rc2 = vfr_server_wait(sv, tickets[i], &r, nullptr);
+    if (rc2 != VFR_OK) return rc2;
+    lat[i] = r.latency_ms;
+    last = std::max(last, t_done);
+    rays += (uint64_t)r.width * r.height;
+  }
+  memset(rep, 0, sizeof(*rep));
+  rep->requests = (uint64_t)n;
+  rep->rays = rays;
+  rep->makespan_s = last - (t0s + offsets[0]);
+  rep->mrays_per_s = rays / rep->makespan_s / 1e6;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    rep->batches = sv->n_batches - b0;
+    rep->steps = sv->steps - s0;
+    rep->anomalies = sv->anomalies - a0;
+    rep->active_rays = sv->active_rays - ar0;
+    rep->mean_batch = rep->batches ? (double)(sv->n_batched_reqs - an0) / rep->batches : 0.0;
+  }
+  rep->latency_p50_ms = pctl(lat, 0.50);
+  rep->latency_p90_ms = pctl(lat, 0.90);
+  rep->latency_p99_ms = pctl(lat, 0.99);
+  rep->latency_max_ms = lat.empty() ? 0.0 : *std::max_element(lat.begin(), lat.end());
+  return VFR_OK;
+}

@@ -95,3 +95,43 @@ def concave(n: int = 96):
     seed = seed_of(3)
     x, t = synth_grid(n, seed, 1)
     return x, t, synth_tf(seed), seed
+
+
+SERVE_SIZES = (32, 48, 64, 80, 96, 112, 128, 160)
+
+
+def serve_meshes(sizes=SERVE_SIZES):
+    """The 8 resident meshes of the serving workload and 4 TFs each."""
+    seed = seed_of(4)
+    out = []
+    for k, n in enumerate(sizes):
+        x, t = synth_grid(n, seed + 0x100 * (k + 1), 0)
+        tfs = [synth_tf(seed + 0x100 * (k + 1) + j) for j in range(4)]
+        out.append((x, t, tfs))
+    return out
+
+
+def serve_requests(n: int, n_meshes: int, tf_ids, sides=SIDES_SERVE, radius: float = 2.5):
+    """n requests: uniform mesh, one of its 4 TFs, eye on the r = 2.5 sphere,
+    side from `sides`.  tf_ids[m] lists mesh m's transfer-function ids."""
+    from . import Request
+    r = Rng(seed_of(4), ROLE_REQUESTS)
+    reqs = []
+    for _ in range(n):
+        m = min(n_meshes - 1, int(r.uni() * n_meshes))
+        tf = tf_ids[m][min(3, int(r.uni() * 4))]
+        d = r.direction()
+        side = r.choice(sides)
+        v = perspective_view(side, side, tf, eye=radius * d, fov_y_deg=60.0)
+        reqs.append(Request(m, 0, v))
+    return reqs
+
+
+def poisson_offsets(n: int, rate_per_s: float) -> "np.ndarray":
+    """Arrival offsets (s) of a Poisson stream at `rate_per_s` (seeded)."""
+    r = Rng(seed_of(4), ROLE_REQUESTS + 1)
+    t, out = 0.0, []
+    for _ in range(n):
+        t += -math.log(max(1e-300, 1.0 - r.uni())) / rate_per_s
+        out.append(t)
+    return np.array(out)
