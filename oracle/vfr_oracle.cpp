@@ -16,9 +16,11 @@
  *                        vfRay                    vfRayKernelCPU.cc:172-566
  *   orc_render_ftb       the renderer's own front-to-back mode (perspective,
  *                        early termination, re-entry): no reference
- *                        counterpart; it reuses the reference's exit-face test
- *                        (vfRayKernelCPU.cc:312-350) and PPI segment formula
- *                        (vfRayKernelCPU.cc:448-482) with front = entry point.
+ *                        counterpart; entry faces by the reference's inside
+ *                        test (vfRayKernelCPU.cc:256-268), exit faces by 2-D
+ *                        edge functions oriented by vertex id (watertight),
+ *                        the PPI segment formula (vfRayKernelCPU.cc:448-482)
+ *                        with front = entry point.
  *   FP64 re-integration  (both render functions) the same tet sequence
  *                        integrated in double with the exact psi integral.
  *
@@ -263,35 +265,6 @@ static inline ExitHit find_exit(const F4 Q[4], float ox, float oy) {
     if (tri_test(Q[i], Q[(i + 1) & 3], Q[(i + 2) & 3], ox, oy, 1.0f, &h)) { h.i = i; return h; }
   }
   return h;
-}
-
-static inline float bary(float A, float B, float C, const ExitHit& h);
-
-/* Front-to-back mode's exit choice: of the candidates that pass the
- * reference's test, the one whose exit point has the smallest Z (farthest
- * along the ray); ties keep the reference's order.  With one passing face it
- * is the reference's choice. */
-static inline ExitHit find_exit_farthest(const F4 Q[4], float ox, float oy) {
-  ExitHit hs[4];
-  int n = 0, first = 0;
-  for (int i = 1; i < 4; ++i) {
-    hs[i].i = 0;
-    if (tri_test(Q[i], Q[(i + 1) & 3], Q[(i + 2) & 3], ox, oy, 1.0f, &hs[i])) {
-      hs[i].i = i;
-      if (!n) first = i;
-      n++;
-    }
-  }
-  if (n == 0) { ExitHit h; h.i = 0; h.cy = h.cz = h.D = 0.0f; h.swapped = false; return h; }
-  if (n == 1) return hs[first];
-  int best = 0;
-  float bz = 0.0f;
-  for (int i = 1; i < 4; ++i) {
-    if (!hs[i].i) continue;
-    float z = bary(Q[i].z, Q[(i + 1) & 3].z, Q[(i + 2) & 3].z, hs[i]);
-    if (!best || z < bz) { best = i; bz = z; }
-  }
-  return hs[best];
 }
 
 /* Barycentric interpolation over the triangle (a,b,c) from the inside-test
@@ -595,6 +568,22 @@ static inline u32 psi_index(float g) {
   return i > 511 ? 511 : i;
 }
 
+/* Edge function of the pixel against the edge from the lower-id vertex to
+ * the higher-id one, (b - a) x (p - a): every face sharing the edge sees the
+ * same value, so the faces behind an entry face cover it without gaps. */
+static inline float edge_canon(u32 ia, const F4& A, u32 ib, const F4& B, float ox, float oy) {
+  bool sw = ib < ia;
+  float ax = sw ? B.x : A.x, ay = sw ? B.y : A.y;
+  float bx = sw ? A.x : B.x, by = sw ? A.y : B.y;
+  return (bx - ax) * (oy - ay) - (by - ay) * (ox - ax);
+}
+static inline float orient(u32 a, u32 b, float c) { return a < b ? c : -c; }
+static inline bool inside3(float a, float b, float c) {
+  bool pos = a >= 0.0f && b >= 0.0f && c >= 0.0f;
+  bool neg = a <= 0.0f && b <= 0.0f && c <= 0.0f;
+  return (pos || neg) && !(a == 0.0f && b == 0.0f && c == 0.0f);
+}
+
 /* Front-to-back render of one view.  Entry candidates are every visible
  * external face whose projection holds the pixel (the reference's inclusive
  * inside test), keyed by (Z at the pixel, ext index) and walked front to
@@ -702,37 +691,35 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
       float ar = 0, ag = 0, ab = 0, Tr = 1.0f;
       double ar64 = 0, ag64 = 0, ab64 = 0, T64 = 1.0;
       bool live32 = true, live64 = (rgba64 != nullptr);
+      F4 cP = {0.0f, 0.0f, 0.0f, 0.0f};
       u32 cur = 0;
       while (cur < nk && (live32 || live64)) {
         u32 ei = (u32)(K[cur] & 0xFFFFFFFFu);
         u32 tcur = extf[ei * 2], fent = extf[ei * 2 + 1];
-        /* entry values */
-        const F4& a = P[T[tcur * 4 + ((fent + 0) & 3)]];
-        const F4& b = P[T[tcur * 4 + ((fent + 1) & 3)]];
-        const F4& c = P[T[tcur * 4 + ((fent + 2) & 3)]];
-        float v01x = b.x - a.x, v01y = b.y - a.y, v02x = c.x - a.x, v02y = c.y - a.y;
-        float cx = (v01x * v02y) - (v01y * v02x);
-        cx = tiny(cx) ? (float)RDELTA : cx;
-        bool sw = false;
-        if (cx < 0.0f) {
-          float tx = v01x, ty = v01y;
-          v01x = v02x; v01y = v02y; v02x = tx; v02y = ty;
-          cx = -cx; sw = true;
+        /* entry face: canonical edge functions, barycentric values */
+        u32 i0 = T[tcur * 4 + ((fent + 0) & 3)], i1 = T[tcur * 4 + ((fent + 1) & 3)],
+            i2 = T[tcur * 4 + ((fent + 2) & 3)];
+        F4 P0 = P[i0], P1 = P[i1], P2 = P[i2];
+        float c01 = edge_canon(i0, P0, i1, P1, ox, oy);
+        float c12 = edge_canon(i1, P1, i2, P2, ox, oy);
+        float c20 = edge_canon(i2, P2, i0, P0, ox, oy);
+        float e01 = orient(i0, i1, c01), e12 = orient(i1, i2, c12), e20 = orient(i2, i0, c20);
+        float Dn = e01 + e12 + e20;
+        if (Dn == 0.0f) {
+          if (live32) anomalies++;
+          break;
         }
-        float v0px = ox - a.x, v0py = oy - a.y;
-        ExitHit he; he.i = 0; he.swapped = sw; he.D = cx;
-        he.cy = (v01x * v0py) - (v01y * v0px);
-        he.cz = (v0px * v02y) - (v0py * v02x);
-        float Zprev = bary(a.z, b.z, c.z, he);
-        float Sprev = bary(a.w, b.w, c.w, he);
+        float inv0 = 1.0f / Dn;
+        float Zprev = P0.z + ((e20 * (P1.z - P0.z) + e01 * (P2.z - P0.z)) * inv0);
+        float Sprev = P0.w + ((e20 * (P1.w - P0.w) + e01 * (P2.w - P0.w)) * inv0);
         float dprev = 0.0f, sprev;
         if (proj == 1) { dprev = 1.0f / Zprev; sprev = clamp01(Sprev * dprev); }
         else sprev = clamp01(Sprev);
-        F4 cP = ld4(tf, (u32)(255.0f * sprev));
+        if (live32) cP = ld4(tf, (u32)(255.0f * sprev));
         /* FP64 entry */
-        D4 a64 = xform64(view, ld4(V, T[tcur * 4 + ((fent + 0) & 3)]));
-        D4 b64 = xform64(view, ld4(V, T[tcur * 4 + ((fent + 1) & 3)]));
-        D4 c64 = xform64(view, ld4(V, T[tcur * 4 + ((fent + 2) & 3)]));
+        D4 a64 = xform64(view, ld4(V, i0));
+        D4 b64 = xform64(view, ld4(V, i1));
+        D4 c64 = xform64(view, ld4(V, i2));
         double Zp64 = bary64(a64, b64, c64, ox, oy, &D4::z);
         double Sp64 = bary64(a64, b64, c64, ox, oy, &D4::w);
         double sp64 = proj == 1 ? Sp64 / Zp64 : Sp64;
@@ -744,16 +731,42 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
         while (1) {
           if (live32 && ++steps > MAX_STEPS) { anomaly = anom32 = true; break; }
           if (++iter > 4 * MAX_STEPS) { anomaly = true; break; }
-          F4 Q[4];
-          for (int k = 0; k < 4; ++k) Q[k] = P[T[tcur * 4 + ((fent + k) & 3)]];
-          ExitHit h = find_exit_farthest(Q, ox, oy);
-          if (h.i == 0) { anomaly = true; anom32 = live32; break; }
-          u32 fex = (fent + (u32)h.i) & 3;
-          const F4& A = Q[h.i];
-          const F4& B = Q[(h.i + 1) & 3];
-          const F4& C = Q[(h.i + 2) & 3];
-          float Znext = bary(A.z, B.z, C.z, h);
-          float Snext = bary(A.w, B.w, C.w, h);
+          u32 i3 = T[tcur * 4 + ((fent + 3) & 3)];
+          F4 P3 = P[i3];
+          float c30 = edge_canon(i3, P3, i0, P0, ox, oy);
+          float c31 = edge_canon(i3, P3, i1, P1, ox, oy);
+          float c32 = edge_canon(i3, P3, i2, P2, ox, oy);
+          float t0 = orient(i3, i0, c30), t1 = orient(i3, i1, c31), t2 = orient(i3, i2, c32);
+          float o01 = orient(i0, i1, c01), o12 = orient(i1, i2, c12), o20 = orient(i2, i0, c20);
+          /* candidate faces (P3, Pi, Pj), (i,j) = (0,1), (1,2), (2,0) */
+          bool p1 = inside3(t0, o01, -t1), p2 = inside3(t1, o12, -t2), p3 = inside3(t2, o20, -t0);
+          if (!(p1 || p2 || p3)) { anomaly = true; anom32 = live32; break; }
+          bool k1 = p1, k2 = !p1 && p2;
+          if ((int)p1 + (int)p2 + (int)p3 > 1) {
+            float inf = INFINITY;
+            float z1 = p1 ? P3.z + ((-t1 * (P0.z - P3.z) + t0 * (P1.z - P3.z)) * (1.0f / (t0 + o01 - t1))) : inf;
+            float z2 = p2 ? P3.z + ((-t2 * (P1.z - P3.z) + t1 * (P2.z - P3.z)) * (1.0f / (t1 + o12 - t2))) : inf;
+            float z3 = p3 ? P3.z + ((-t0 * (P2.z - P3.z) + t2 * (P0.z - P3.z)) * (1.0f / (t2 + o20 - t0))) : inf;
+            k1 = p1 && z1 <= z2 && z1 <= z3;
+            k2 = !k1 && p2 && z2 <= z3;
+          }
+          F4 B = k1 ? P0 : (k2 ? P1 : P2);
+          F4 C = k1 ? P1 : (k2 ? P2 : P0);
+          u32 ib = k1 ? i0 : (k2 ? i1 : i2);
+          u32 ic = k1 ? i1 : (k2 ? i2 : i0);
+          u32 ik = k1 ? i2 : (k2 ? i0 : i1);
+          float eAB = k1 ? t0 : (k2 ? t1 : t2);
+          float eBC = k1 ? o01 : (k2 ? o12 : o20);
+          float eCA = -(k1 ? t1 : (k2 ? t2 : t0));
+          float cbc = k1 ? c01 : (k2 ? c12 : c20);
+          float cab = k1 ? c30 : (k2 ? c31 : c32);
+          float cca = k1 ? c31 : (k2 ? c32 : c30);
+          float inv = 1.0f / (eAB + eBC + eCA);
+          float Znext = P3.z + ((eCA * (B.z - P3.z) + eAB * (C.z - P3.z)) * inv);
+          float Snext = P3.w + ((eCA * (B.w - P3.w) + eAB * (C.w - P3.w)) * inv);
+          u32 sk = 0;
+          while (sk < 3 && T[tcur * 4 + sk] != ik) ++sk;
+          u32 fex = (sk + 1) & 3;
           if (live32) {
             float snext, dist;
             if (proj == 1) {
@@ -769,7 +782,9 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
             F4 cN = ld4(tf, (u32)(255.0f * snext));
             float tf_ = cP.w * dl, tb_ = cN.w * dl;
             float zeta = orc_expf(-(tf_ + tb_) * 0.5f);
-            float gf = tf_ / (1.0f + tf_), gb = tb_ / (1.0f + tb_);
+            float uf = 1.0f + tf_, ub = 1.0f + tb_;
+            float q = 1.0f / (uf * ub);
+            float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
             float ps = psi[psi_index(gf) * 512 + psi_index(gb)];
             float w1 = 1.0f - ps, w2 = ps - zeta;
             ar += Tr * (cP.x * w1 + cN.x * w2);
@@ -780,10 +795,7 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
             if (1.0f - Tr >= term) live32 = false;
           }
           if (live64) {
-            u32 ia = T[tcur * 4 + ((fent + (u32)h.i) & 3)];
-            u32 ib = T[tcur * 4 + ((fent + (u32)h.i + 1) & 3)];
-            u32 ic = T[tcur * 4 + ((fent + (u32)h.i + 2) & 3)];
-            D4 A64 = xform64(view, ld4(V, ia)), B64 = xform64(view, ld4(V, ib)),
+            D4 A64 = xform64(view, ld4(V, i3)), B64 = xform64(view, ld4(V, ib)),
                C64 = xform64(view, ld4(V, ic));
             double Zn64 = bary64(A64, B64, C64, ox, oy, &D4::z);
             double Sn64 = bary64(A64, B64, C64, ox, oy, &D4::w);
@@ -804,13 +816,17 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
             Zp64 = Zn64;
             if (1.0 - T64 >= (double)term) live64 = false;
           }
+          if (!live32 && !live64) break;
           u32 nb = con[tcur * 4 + fex];
           if (nb == tcur) { exitZ = Znext; break; }
           u32 k;
           for (k = 0; k < 4; ++k) if (con[nb * 4 + k] == tcur) break;
           if (k == 4) { anomaly = true; anom32 = live32; break; }
-          tcur = nb; fent = k; Zprev = Znext;
-          if (!live32 && !live64) break;
+          tcur = nb; fent = k;
+          P0 = P3; P1 = B; P2 = C;
+          i0 = i3; i1 = ib; i2 = ic;
+          c01 = cab; c12 = cbc; c20 = cca;
+          Zprev = Znext;
         }
         if (anom32) anomalies++;
         if (anomaly) break;

@@ -550,10 +550,9 @@ DEV float4 sel4f(bool c, float4 a, float4 b) {
   return make_float4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w);
 }
 
-/* Loads the current tet, rebuilds Q[k] = slot (fe+k)&3 from the carried
- * entry-face vertices plus the one new vertex, and runs the reference's
- * exit-face test on faces (fe+1..3)&3 in that order. */
-template <bool FARTHEST>
+/* REF mode's step: loads the current tet, rebuilds Q[k] = slot (fe+k)&3 from
+ * the carried entry-face vertices plus the one new vertex, and runs the
+ * reference's exit-face test on faces (fe+1..3)&3 in that order. */
 DEV Step walk_step(const Walk& w, const TetRec* __restrict__ rec, const float4* __restrict__ pv,
                    float ox, float oy) {
   const uint4 rv = *reinterpret_cast<const uint4*>(rec[w.t].v);
@@ -568,19 +567,7 @@ DEV Step walk_step(const Walk& w, const TetRec* __restrict__ rec, const float4* 
   const Hit h1 = tri_test(Q1, Q2, Q3, ox, oy);
   const Hit h2 = tri_test(Q2, Q3, Q0, ox, oy);
   const Hit h3 = tri_test(Q3, Q0, Q1, ox, oy);
-  bool c1 = h1.ok, c2 = !h1.ok && h2.ok;
-  if (FARTHEST && (int)h1.ok + (int)h2.ok + (int)h3.ok > 1) {
-    /* Several faces pass (the pixel is on a projected edge or vertex): take
-     * the exit point farthest along the ray (smallest Z), ties in the
-     * reference's order.  A ray through an edge then always advances; the
-     * first passing face can be a zero-length exit that circles the edge. */
-    const float inf = __uint_as_float(0x7f800000u);
-    const float z1 = h1.ok ? bary(Q1.z, Q2.z, Q3.z, h1, 1.0f / h1.D) : inf;
-    const float z2 = h2.ok ? bary(Q2.z, Q3.z, Q0.z, h2, 1.0f / h2.D) : inf;
-    const float z3 = h3.ok ? bary(Q3.z, Q0.z, Q1.z, h3, 1.0f / h3.D) : inf;
-    c1 = h1.ok && z1 <= z2 && z1 <= z3;
-    c2 = !c1 && h2.ok && z2 <= z3;
-  }
+  const bool c1 = h1.ok, c2 = !h1.ok && h2.ok;
   Step s;
   s.i = c1 ? 1 : (c2 ? 2 : (h3.ok ? 3 : 0));
   s.A = sel4f(c1, Q1, sel4f(c2, Q2, Q3));
@@ -638,7 +625,7 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
   uint32_t steps = 0;
   while (true) {
     if (++steps > VFR_MAX_STEPS) { o.anomaly = 1; break; }
-    const Step s = walk_step<false>(w, rec, pv, ox, oy);
+    const Step s = walk_step(w, rec, pv, ox, oy);
     if (s.i == 0) { o.anomaly = 1; break; }         /* :353 */
     if (s.nb == VFR_NB_BROKEN) { o.broken = 1; break; } /* :365, not emulated */
     const Planes nf = face_planes(s.A, s.B, s.C);
@@ -688,41 +675,76 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
  * composited front to back with the front = entry point, early termination
  * at 1 - T >= term, re-entry at the first entry behind the last exit point.
  * The one-ray-per-thread kernel and the persistent work-queue kernel both
- * drive it, so they produce the same bits.  The state is kept small (about
- * 40 registers) because occupancy hides the walk's dependent loads. */
+ * drive it, so they produce the same bits.
+ *
+ * Exit faces come from 2-D edge functions of the pixel against the projected
+ * tet edges, each computed in a canonical direction (lower vertex id first)
+ * so that every face sharing an edge sees the same value: the three faces
+ * behind the entry face then cover it without gaps.  A step computes only the
+ * three edges to the tet's new vertex; the entry face's edges are carried. */
 struct FtbRay {
-  Walk w;
+  float4 P0, P1, P2;       /* entry face vertices (P space) */
+  uint32_t i0, i1, i2;     /* their ids */
+  float c01, c12, c20;     /* canonical edge functions of the entry face */
   float4 cP;
   float ox, oy, rayf, Zprev, dprev, inv_me;
   float ar, ag, ab, Tr;
-  uint32_t voff, roff, eoff, gp, tfo, steps;
-  uint32_t bits; /* cur 0..3 | nk 4..7 | persp 8 | anomaly 9 | broken 10 */
+  uint32_t t, voff, roff, eoff, gp, tfo, steps;
+  uint32_t bits; /* cur 0..3 | nk 4..7 | persp 8 | anomaly 9 | broken 10 | fe 11..12 */
 };
 
 DEV uint32_t ftb_cur(const FtbRay& r) { return r.bits & 15u; }
 DEV uint32_t ftb_nk(const FtbRay& r) { return (r.bits >> 4) & 15u; }
 DEV bool ftb_persp(const FtbRay& r) { return (r.bits >> 8) & 1u; }
+DEV uint32_t ftb_fe(const FtbRay& r) { return (r.bits >> 11) & 3u; }
+DEV void ftb_set_fe(FtbRay& r, uint32_t fe) { r.bits = (r.bits & ~(3u << 11)) | (fe << 11); }
+
+/* Edge function of the pixel against the edge from the lower-id vertex a
+ * to the higher-id vertex b: (b - a) x (p - a). */
+DEV float edge_canon(uint32_t ia, float4 A, uint32_t ib, float4 B, float ox, float oy) {
+  const bool sw = ib < ia;
+  const float ax = sw ? B.x : A.x, ay = sw ? B.y : A.y;
+  const float bx = sw ? A.x : B.x, by = sw ? A.y : B.y;
+  return (bx - ax) * (oy - ay) - (by - ay) * (ox - ax);
+}
+
+/* The oriented value a -> b from the canonical one. */
+DEV float orient(uint32_t ia, uint32_t ib, float c) { return ia < ib ? c : -c; }
+
+DEV bool inside3(float a, float b, float c) {
+  const bool pos = a >= 0.0f && b >= 0.0f && c >= 0.0f;
+  const bool neg = a <= 0.0f && b <= 0.0f && c <= 0.0f;
+  return (pos || neg) && !(a == 0.0f && b == 0.0f && c == 0.0f);
+}
 
 /* Starts the walk at entry `cur` of the pixel's sorted list. */
-DEV void ftb_segment(const TravArgs& a, FtbRay& r) {
+DEV bool ftb_segment(const TravArgs& a, FtbRay& r) {
   const uint64_t key = a.entries[(size_t)r.gp * VFR_K + ftb_cur(r)];
   const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull) + r.eoff;
   const float4* __restrict__ pv = a.vert + r.voff;
-  r.w.t = a.ext_int[2 * (size_t)e];
-  r.w.fe = a.ext_int[2 * (size_t)e + 1];
-  const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[r.roff + r.w.t].v);
-  r.w.h0 = sel4(rv, r.w.fe); r.w.h1 = sel4(rv, (r.w.fe + 1u) & 3u); r.w.h2 = sel4(rv, (r.w.fe + 2u) & 3u);
-  r.w.H0 = pv[r.w.h0];
-  r.w.H1 = pv[r.w.h1];
-  r.w.H2 = pv[r.w.h2];
-  const Hit he = entry_test(r.w.H0, r.w.H1, r.w.H2, r.ox, r.oy);
-  const float inv = 1.0f / he.D;
-  r.Zprev = bary(r.w.H0.z, r.w.H1.z, r.w.H2.z, he, inv);
-  const float Sprev = bary(r.w.H0.w, r.w.H1.w, r.w.H2.w, he, inv);
+  r.t = a.ext_int[2 * (size_t)e];
+  const uint32_t fe = a.ext_int[2 * (size_t)e + 1];
+  ftb_set_fe(r, fe);
+  const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[r.roff + r.t].v);
+  r.i0 = sel4(rv, fe); r.i1 = sel4(rv, (fe + 1u) & 3u); r.i2 = sel4(rv, (fe + 2u) & 3u);
+  r.P0 = pv[r.i0];
+  r.P1 = pv[r.i1];
+  r.P2 = pv[r.i2];
+  r.c01 = edge_canon(r.i0, r.P0, r.i1, r.P1, r.ox, r.oy);
+  r.c12 = edge_canon(r.i1, r.P1, r.i2, r.P2, r.ox, r.oy);
+  r.c20 = edge_canon(r.i2, r.P2, r.i0, r.P0, r.ox, r.oy);
+  const float o01 = orient(r.i0, r.i1, r.c01), o12 = orient(r.i1, r.i2, r.c12), o20 = orient(r.i2, r.i0, r.c20);
+  const float D = o01 + o12 + o20;
+  if (D == 0.0f) return false;
+  const float inv = 1.0f / D;
+  /* weights: P1 <- o20 / D, P2 <- o01 / D */
+  r.Zprev = r.P0.z + ((o20 * (r.P1.z - r.P0.z) + o01 * (r.P2.z - r.P0.z)) * inv);
+  const float Sprev = r.P0.w + ((o20 * (r.P1.w - r.P0.w) + o01 * (r.P2.w - r.P0.w)) * inv);
   float sprev;
   if (ftb_persp(r)) { r.dprev = 1.0f / r.Zprev; sprev = clamp01(Sprev * r.dprev); }
   else { r.dprev = 0.0f; sprev = clamp01(Sprev); }
   r.cP = a.tf[r.tfo + min(255u, (uint32_t)(255.0f * sprev))];
+  return true;
 }
 
 DEV void ftb_init(const TravArgs& a, uint32_t vi, uint32_t pix, FtbRay& r) {
@@ -744,17 +766,55 @@ DEV void ftb_init(const TravArgs& a, uint32_t vi, uint32_t pix, FtbRay& r) {
   r.ar = r.ag = r.ab = 0.0f;
   r.Tr = 1.0f;
   r.steps = 0;
-  ftb_segment(a, r);
+  if (!ftb_segment(a, r)) r.bits |= 512u;
 }
 
 /* One tet of the walk; returns true when the ray is finished. */
 DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
-  if (++r.steps > VFR_MAX_STEPS) { r.bits |= 512u; return true; }
-  const Step s = walk_step<true>(r.w, a.rec + r.roff, a.vert + r.voff, r.ox, r.oy);
-  if (s.i == 0) { r.bits |= 512u; return true; }
-  const float inv = 1.0f / s.h.D;
-  const float Znext = bary(s.A.z, s.B.z, s.C.z, s.h, inv);
-  const float Snext = bary(s.A.w, s.B.w, s.C.w, s.h, inv);
+  if ((r.bits & 512u) || ++r.steps > VFR_MAX_STEPS) { r.bits |= 512u; return true; }
+  const TetRec* __restrict__ rc = a.rec + r.roff + r.t;
+  const uint4 rv = *reinterpret_cast<const uint4*>(rc->v);
+  const uint4 rn = *reinterpret_cast<const uint4*>(rc->n);
+  const uint32_t i3 = sel4(rv, (ftb_fe(r) + 3u) & 3u);
+  const float4 P3 = a.vert[r.voff + i3];
+  const float c30 = edge_canon(i3, P3, r.i0, r.P0, r.ox, r.oy);
+  const float c31 = edge_canon(i3, P3, r.i1, r.P1, r.ox, r.oy);
+  const float c32 = edge_canon(i3, P3, r.i2, r.P2, r.ox, r.oy);
+  const float t0 = orient(i3, r.i0, c30), t1 = orient(i3, r.i1, c31), t2 = orient(i3, r.i2, c32);
+  const float o01 = orient(r.i0, r.i1, r.c01), o12 = orient(r.i1, r.i2, r.c12), o20 = orient(r.i2, r.i0, r.c20);
+  /* candidate k = face (P3, Pi, Pj) for (i,j) = (0,1), (1,2), (2,0):
+   * oriented edges 3->i, i->j, j->3 */
+  const bool p1 = inside3(t0, o01, -t1), p2 = inside3(t1, o12, -t2), p3 = inside3(t2, o20, -t0);
+  if (!(p1 || p2 || p3)) { r.bits |= 512u; return true; }
+  bool c1 = p1, c2 = !p1 && p2;
+  if ((int)p1 + (int)p2 + (int)p3 > 1) {
+    /* the pixel is on a projected edge: exit at the point farthest along the
+     * ray (smallest Z), ties in candidate order */
+    const float inf = __uint_as_float(0x7f800000u);
+    const float z1 = p1 ? P3.z + ((-t1 * (r.P0.z - P3.z) + t0 * (r.P1.z - P3.z)) * (1.0f / (t0 + o01 - t1))) : inf;
+    const float z2 = p2 ? P3.z + ((-t2 * (r.P1.z - P3.z) + t1 * (r.P2.z - P3.z)) * (1.0f / (t1 + o12 - t2))) : inf;
+    const float z3 = p3 ? P3.z + ((-t0 * (r.P2.z - P3.z) + t2 * (r.P0.z - P3.z)) * (1.0f / (t2 + o20 - t0))) : inf;
+    c1 = p1 && z1 <= z2 && z1 <= z3;
+    c2 = !c1 && p2 && z2 <= z3;
+  }
+  /* exit face (A, B, C) = (P3, Pi, Pj); k = the tet vertex it leaves out */
+  const float4 B = sel4f(c1, r.P0, sel4f(c2, r.P1, r.P2));
+  const float4 C = sel4f(c1, r.P1, sel4f(c2, r.P2, r.P0));
+  const uint32_t ib = c1 ? r.i0 : (c2 ? r.i1 : r.i2);
+  const uint32_t ic = c1 ? r.i1 : (c2 ? r.i2 : r.i0);
+  const uint32_t ik = c1 ? r.i2 : (c2 ? r.i0 : r.i1);
+  const float eAB = c1 ? t0 : (c2 ? t1 : t2);
+  const float eBC = c1 ? o01 : (c2 ? o12 : o20);
+  const float eCA = -(c1 ? t1 : (c2 ? t2 : t0));
+  const float cbc = c1 ? r.c01 : (c2 ? r.c12 : r.c20);
+  const float cab = c1 ? c30 : (c2 ? c31 : c32);
+  const float cca = c1 ? c31 : (c2 ? c32 : c30);
+  const float inv = 1.0f / (eAB + eBC + eCA);
+  const float Znext = P3.z + ((eCA * (B.z - P3.z) + eAB * (C.z - P3.z)) * inv);
+  const float Snext = P3.w + ((eCA * (B.w - P3.w) + eAB * (C.w - P3.w)) * inv);
+  /* the exit face is the tet face opposite vertex k: face index slot(k) + 1 */
+  const uint32_t sk = rv.x == ik ? 0u : (rv.y == ik ? 1u : (rv.z == ik ? 2u : 3u));
+  const uint32_t nb = sel4(rn, (sk + 1u) & 3u);
   float snext, dist;
   if (ftb_persp(r)) {
     const float dn = 1.0f / Znext;
@@ -769,7 +829,10 @@ DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
   const float4 cN = a.tf[r.tfo + min(255u, (uint32_t)(255.0f * snext))];
   const float tf_ = r.cP.w * dl, tb_ = cN.w * dl;
   const float zeta = vfr_expf(-(tf_ + tb_) * 0.5f);
-  const float gf = tf_ / (1.0f + tf_), gb = tb_ / (1.0f + tb_);
+  /* gamma = tau / (1 + tau) for both ends with one division */
+  const float uf = 1.0f + tf_, ub = 1.0f + tb_;
+  const float q = 1.0f / (uf * ub);
+  const float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
   const float ps = a.psi[psi_index(gf) * 512u + psi_index(gb)];
   const float w1 = 1.0f - ps, w2 = ps - zeta;
   r.ar += r.Tr * (r.cP.x * w1 + cN.x * w2);
@@ -778,19 +841,22 @@ DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
   r.Tr *= zeta;
   r.cP = cN;
   if (1.0f - r.Tr >= a.term) return true;
-  if (s.nb == VFR_NB_BROKEN) { r.bits |= 1024u; return true; }
-  if (s.nb == VFR_NB_EXT) {
+  if (nb == VFR_NB_BROKEN) { r.bits |= 1024u; return true; }
+  if (nb == VFR_NB_EXT) {
     /* re-entry: the first entry strictly behind the exit point */
     uint32_t cur = ftb_cur(r) + 1u;
     const uint32_t nk = ftb_nk(r);
     while (cur < nk && !(funkey((uint32_t)(a.entries[(size_t)r.gp * VFR_K + cur] >> 32)) < Znext)) ++cur;
     if (cur >= nk) return true;
     r.bits = (r.bits & ~15u) | cur;
-    ftb_segment(a, r);
+    if (!ftb_segment(a, r)) { r.bits |= 512u; return true; }
     return false;
   }
-  r.w.t = s.nb >> 2; r.w.fe = s.nb & 3u;
-  r.w.H0 = s.A; r.w.H1 = s.B; r.w.H2 = s.C; r.w.h0 = s.a; r.w.h1 = s.b; r.w.h2 = s.c;
+  r.t = nb >> 2;
+  ftb_set_fe(r, nb & 3u);
+  r.P0 = P3; r.P1 = B; r.P2 = C;
+  r.i0 = i3; r.i1 = ib; r.i2 = ic;
+  r.c01 = cab; r.c12 = cbc; r.c20 = cca;
   r.Zprev = Znext;
   return false;
 }
