@@ -95,7 +95,7 @@ def serve_bench(args) -> int:
         tf_ids.append([ctx.tf(tf) for tf in tfs])
         tets += t.shape[0]
     log(f"8 resident meshes, {tets} tets, built in {time.time() - t0:.1f}s")
-    sched = dict(occupancy=args.occupancy or 5, waves_per_cu=args.wpc or 40, refill=args.refill or 32)
+    sched = dict(occupancy=args.occupancy, waves_per_cu=args.wpc, refill=args.refill or 32)
     reqs = WL.serve_requests(args.requests, len(meshes), tf_ids)
     rays_of = lambda rs: sum(r.view.width * r.view.height for r in rs)
     n_cap = min(len(reqs), max(200, args.requests // 5))

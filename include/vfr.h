@@ -147,9 +147,9 @@ typedef struct vfr_render_opts {
   uint32_t use_graph;   /* capture the batched launch set in a hipGraph    */
   uint32_t max_entries; /* FTB entry-list slots per pixel, 0 = 8           */
   uint32_t flags;       /* VFR_RENDER_*                                    */
-  uint32_t occupancy;   /* persistent kernel variant: target waves per SIMD
-                           (0 or <= 4: compiler's choice; 5, 6: register-
-                           capped); tuned by vfr_plan_autotune            */
+  uint32_t occupancy;   /* persistent kernel variant: waves per SIMD (4, 5,
+                           6: register-capped; other values: the compiler's
+                           choice); tuned by vfr_plan_autotune            */
   uint32_t reserved;
 } vfr_render_opts;
 

@@ -568,20 +568,41 @@ static inline u32 psi_index(float g) {
   return i > 511 ? 511 : i;
 }
 
-/* Edge function of the pixel against the edge from the lower-id vertex to
- * the higher-id one, (b - a) x (p - a): every face sharing the edge sees the
- * same value, so the faces behind an entry face cover it without gaps. */
-static inline float edge_canon(u32 ia, const F4& A, u32 ib, const F4& B, float ox, float oy) {
-  bool sw = ib < ia;
-  float ax = sw ? B.x : A.x, ay = sw ? B.y : A.y;
-  float bx = sw ? A.x : B.x, by = sw ? A.y : B.y;
-  return (bx - ax) * (oy - ay) - (by - ay) * (ox - ax);
+/* Vertex relative to the pixel (the renderer's walk works in these). */
+static inline F4 rel(const F4& p, float ox, float oy) {
+  F4 r; r.x = p.x - ox; r.y = p.y - oy; r.z = p.z; r.w = p.w;
+  return r;
 }
-static inline float orient(u32 a, u32 b, float c) { return a < b ? c : -c; }
-static inline bool inside3(float a, float b, float c) {
-  bool pos = a >= 0.0f && b >= 0.0f && c >= 0.0f;
-  bool neg = a <= 0.0f && b <= 0.0f && c <= 0.0f;
-  return (pos || neg) && !(a == 0.0f && b == 0.0f && c == 0.0f);
+/* Edge function u -> v of the pixel with u, v relative to the pixel:
+ * u.x*v.y - u.y*v.x.  The same edge walked v -> u gives exactly the negated
+ * float, so the faces behind an entry face cover it without gaps. */
+static inline float edge_rel(const F4& u, const F4& v) { return u.x * v.y - u.y * v.x; }
+/* Inside a positively oriented face with edge functions a, b, c. */
+static inline bool inside_pos(float a, float b, float c) {
+  return std::min(a, std::min(b, c)) >= 0.0f && std::max(a, std::max(b, c)) > 0.0f;
+}
+/* Depth value and scalar where the ray crosses face (A, B, C) with edge
+ * functions eAB, eBC, eCA: ortho gives Z, perspective gives the depth 1/Z
+ * and S/Z from one reciprocal of Z*D. */
+static inline void face_values(bool persp, float Az, float Aw, float Bz, float Bw, float Cz, float Cw,
+                               float eAB, float eBC, float eCA, float* zd, float* s) {
+  float D = eAB + eBC + eCA;
+  float nz = eCA * (Bz - Az) + eAB * (Cz - Az);
+  float ns = eCA * (Bw - Aw) + eAB * (Cw - Aw);
+  if (persp) {
+    float r = 1.0f / (Az * D + nz);
+    *zd = D * r;
+    *s = clamp01((Aw * D + ns) * r);
+  } else {
+    float inv = 1.0f / D;
+    *zd = Az + nz * inv;
+    *s = clamp01(Aw + ns * inv);
+  }
+}
+static inline float pick3(bool c1, bool c2, float a, float b, float c) { return c1 ? a : (c2 ? b : c); }
+static inline u32 tf_index(float s) {
+  u32 i = (u32)(255.0f * s);
+  return i > 255 ? 255 : i;
 }
 
 /* Front-to-back render of one view.  Entry candidates are every visible
@@ -682,7 +703,7 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
     if (n_entries_out) n_entries_out[p] = nk;
     if (nk) {
       float ox = pix_to_obj(px, W), oy = pix_to_obj(py, H);
-      float rayf = 1.0f;
+      float rayf = view_k;
       double rayf64 = 1.0;
       if (proj == 1) {
         rayf = sqrtf(1.0f + (ox * ox + oy * oy) * (view_k * view_k));
@@ -693,29 +714,30 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
       bool live32 = true, live64 = (rgba64 != nullptr);
       F4 cP = {0.0f, 0.0f, 0.0f, 0.0f};
       u32 cur = 0;
+      const bool persp = proj == 1;
+      const float rf = rayf * inv_me;
       while (cur < nk && (live32 || live64)) {
         u32 ei = (u32)(K[cur] & 0xFFFFFFFFu);
         u32 tcur = extf[ei * 2], fent = extf[ei * 2 + 1];
-        /* entry face: canonical edge functions, barycentric values */
+        /* entry face relative to the pixel, ordered to positive orientation */
         u32 i0 = T[tcur * 4 + ((fent + 0) & 3)], i1 = T[tcur * 4 + ((fent + 1) & 3)],
-            i2 = T[tcur * 4 + ((fent + 2) & 3)];
-        F4 P0 = P[i0], P1 = P[i1], P2 = P[i2];
-        float c01 = edge_canon(i0, P0, i1, P1, ox, oy);
-        float c12 = edge_canon(i1, P1, i2, P2, ox, oy);
-        float c20 = edge_canon(i2, P2, i0, P0, ox, oy);
-        float e01 = orient(i0, i1, c01), e12 = orient(i1, i2, c12), e20 = orient(i2, i0, c20);
-        float Dn = e01 + e12 + e20;
+            i2 = T[tcur * 4 + ((fent + 2) & 3)], i3 = T[tcur * 4 + ((fent + 3) & 3)];
+        F4 P0 = rel(P[i0], ox, oy), P1 = rel(P[i1], ox, oy), P2 = rel(P[i2], ox, oy);
+        float o01 = edge_rel(P0, P1), o12 = edge_rel(P1, P2), o20 = edge_rel(P2, P0);
+        float Dn = o01 + o12 + o20;
         if (Dn == 0.0f) {
           if (live32) anomalies++;
           break;
         }
-        float inv0 = 1.0f / Dn;
-        float Zprev = P0.z + ((e20 * (P1.z - P0.z) + e01 * (P2.z - P0.z)) * inv0);
-        float Sprev = P0.w + ((e20 * (P1.w - P0.w) + e01 * (P2.w - P0.w)) * inv0);
-        float dprev = 0.0f, sprev;
-        if (proj == 1) { dprev = 1.0f / Zprev; sprev = clamp01(Sprev * dprev); }
-        else sprev = clamp01(Sprev);
-        if (live32) cP = ld4(tf, (u32)(255.0f * sprev));
+        if (Dn < 0.0f) {
+          std::swap(P1, P2);
+          std::swap(i1, i2);
+          float n01 = -o20, n12 = -o12, n20 = -o01;
+          o01 = n01; o12 = n12; o20 = n20;
+        }
+        float prev, sprev;
+        face_values(persp, P0.z, P0.w, P1.z, P1.w, P2.z, P2.w, o01, o12, o20, &prev, &sprev);
+        if (live32) cP = ld4(tf, tf_index(sprev));
         /* FP64 entry */
         D4 a64 = xform64(view, ld4(V, i0));
         D4 b64 = xform64(view, ld4(V, i1));
@@ -725,72 +747,51 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
         double sp64 = proj == 1 ? Sp64 / Zp64 : Sp64;
         sp64 = sp64 < 0 ? 0 : (sp64 > 1 ? 1 : sp64);
         F4 cP64 = ld4(tf, (u32)(255.0 * sp64));
-        float exitZ = Zprev;
+        float exitZ = 0.0f;
         bool anomaly = false, anom32 = false;
         u32 iter = 0;
         while (1) {
           if (live32 && ++steps > MAX_STEPS) { anomaly = anom32 = true; break; }
           if (++iter > 4 * MAX_STEPS) { anomaly = true; break; }
-          u32 i3 = T[tcur * 4 + ((fent + 3) & 3)];
-          F4 P3 = P[i3];
-          float c30 = edge_canon(i3, P3, i0, P0, ox, oy);
-          float c31 = edge_canon(i3, P3, i1, P1, ox, oy);
-          float c32 = edge_canon(i3, P3, i2, P2, ox, oy);
-          float t0 = orient(i3, i0, c30), t1 = orient(i3, i1, c31), t2 = orient(i3, i2, c32);
-          float o01 = orient(i0, i1, c01), o12 = orient(i1, i2, c12), o20 = orient(i2, i0, c20);
+          F4 P3 = rel(P[i3], ox, oy);
+          float t0 = edge_rel(P3, P0), t1 = edge_rel(P3, P1), t2 = edge_rel(P3, P2);
           /* candidate faces (P3, Pi, Pj), (i,j) = (0,1), (1,2), (2,0) */
-          bool p1 = inside3(t0, o01, -t1), p2 = inside3(t1, o12, -t2), p3 = inside3(t2, o20, -t0);
+          bool p1 = inside_pos(t0, o01, -t1), p2 = inside_pos(t1, o12, -t2), p3 = inside_pos(t2, o20, -t0);
           if (!(p1 || p2 || p3)) { anomaly = true; anom32 = live32; break; }
-          bool k1 = p1, k2 = !p1 && p2;
+          bool c1 = p1, c2 = !p1 && p2;
           if ((int)p1 + (int)p2 + (int)p3 > 1) {
             float inf = INFINITY;
-            float z1 = p1 ? P3.z + ((-t1 * (P0.z - P3.z) + t0 * (P1.z - P3.z)) * (1.0f / (t0 + o01 - t1))) : inf;
-            float z2 = p2 ? P3.z + ((-t2 * (P1.z - P3.z) + t1 * (P2.z - P3.z)) * (1.0f / (t1 + o12 - t2))) : inf;
-            float z3 = p3 ? P3.z + ((-t0 * (P2.z - P3.z) + t2 * (P0.z - P3.z)) * (1.0f / (t2 + o20 - t0))) : inf;
-            k1 = p1 && z1 <= z2 && z1 <= z3;
-            k2 = !k1 && p2 && z2 <= z3;
+            float z1 = p1 ? P3.z + (-t1 * (P0.z - P3.z) + t0 * (P1.z - P3.z)) / (t0 + o01 - t1) : inf;
+            float z2 = p2 ? P3.z + (-t2 * (P1.z - P3.z) + t1 * (P2.z - P3.z)) / (t1 + o12 - t2) : inf;
+            float z3 = p3 ? P3.z + (-t0 * (P2.z - P3.z) + t2 * (P0.z - P3.z)) / (t2 + o20 - t0) : inf;
+            c1 = p1 && z1 <= z2 && z1 <= z3;
+            c2 = !c1 && p2 && z2 <= z3;
           }
-          F4 B = k1 ? P0 : (k2 ? P1 : P2);
-          F4 C = k1 ? P1 : (k2 ? P2 : P0);
-          u32 ib = k1 ? i0 : (k2 ? i1 : i2);
-          u32 ic = k1 ? i1 : (k2 ? i2 : i0);
-          u32 ik = k1 ? i2 : (k2 ? i0 : i1);
-          float eAB = k1 ? t0 : (k2 ? t1 : t2);
-          float eBC = k1 ? o01 : (k2 ? o12 : o20);
-          float eCA = -(k1 ? t1 : (k2 ? t2 : t0));
-          float cbc = k1 ? c01 : (k2 ? c12 : c20);
-          float cab = k1 ? c30 : (k2 ? c31 : c32);
-          float cca = k1 ? c31 : (k2 ? c32 : c30);
-          float inv = 1.0f / (eAB + eBC + eCA);
-          float Znext = P3.z + ((eCA * (B.z - P3.z) + eAB * (C.z - P3.z)) * inv);
-          float Snext = P3.w + ((eCA * (B.w - P3.w) + eAB * (C.w - P3.w)) * inv);
+          float zd, snext;
+          face_values(persp, P3.z, P3.w, pick3(c1, c2, P0.z, P1.z, P2.z), pick3(c1, c2, P0.w, P1.w, P2.w),
+                      pick3(c1, c2, P1.z, P2.z, P0.z), pick3(c1, c2, P1.w, P2.w, P0.w), pick3(c1, c2, t0, t1, t2),
+                      pick3(c1, c2, o01, o12, o20), -pick3(c1, c2, t1, t2, t0), &zd, &snext);
+          bool k0 = c2, k1 = !c1 && !c2, k2 = c1;
+          u32 ib = c1 ? i0 : (c2 ? i1 : i2), ic = c1 ? i1 : (c2 ? i2 : i0);
+          u32 ik = c1 ? i2 : (c2 ? i0 : i1);
           u32 sk = 0;
           while (sk < 3 && T[tcur * 4 + sk] != ik) ++sk;
           u32 fex = (sk + 1) & 3;
           if (live32) {
-            float snext, dist;
-            if (proj == 1) {
-              float dn = 1.0f / Znext;
-              snext = clamp01(Snext * dn);
-              dist = fabsf(dn - dprev) * rayf;
-              dprev = dn;
-            } else {
-              snext = clamp01(Snext);
-              dist = fabsf(Zprev - Znext) * view_k;
-            }
-            float dl = dist * inv_me;
-            F4 cN = ld4(tf, (u32)(255.0f * snext));
+            float dl = fabsf(zd - prev) * rf;
+            prev = zd;
+            F4 cN = ld4(tf, tf_index(snext));
             float tf_ = cP.w * dl, tb_ = cN.w * dl;
-            float zeta = orc_expf(-(tf_ + tb_) * 0.5f);
             float uf = 1.0f + tf_, ub = 1.0f + tb_;
             float q = 1.0f / (uf * ub);
             float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
             float ps = psi[psi_index(gf) * 512 + psi_index(gb)];
+            float zeta = orc_expf(-(tf_ + tb_) * 0.5f);
             float w1 = 1.0f - ps, w2 = ps - zeta;
-            ar += Tr * (cP.x * w1 + cN.x * w2);
-            ag += Tr * (cP.y * w1 + cN.y * w2);
-            ab += Tr * (cP.z * w1 + cN.z * w2);
-            Tr *= zeta;
+            ar = ar + Tr * (cP.x * w1 + cN.x * w2);
+            ag = ag + Tr * (cP.y * w1 + cN.y * w2);
+            ab = ab + Tr * (cP.z * w1 + cN.z * w2);
+            Tr = Tr * zeta;
             cP = cN;
             if (1.0f - Tr >= term) live32 = false;
           }
@@ -818,15 +819,20 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
           }
           if (!live32 && !live64) break;
           u32 nb = con[tcur * 4 + fex];
-          if (nb == tcur) { exitZ = Znext; break; }
+          if (nb == tcur) { exitZ = persp ? 1.0f / zd : zd; break; }
           u32 k;
           for (k = 0; k < 4; ++k) if (con[nb * 4 + k] == tcur) break;
-          if (k == 4) { anomaly = true; anom32 = live32; break; }
-          tcur = nb; fent = k;
-          P0 = P3; P1 = B; P2 = C;
-          i0 = i3; i1 = ib; i2 = ic;
-          c01 = cab; c12 = cbc; c20 = cca;
-          Zprev = Znext;
+          if (k == 4) { anomaly = true; anom32 = false; break; }
+          /* the exit face becomes the entry face: vertex k is replaced by P3 */
+          float n01 = k0 ? t1 : (k1 ? -t0 : o01);
+          float n12 = k1 ? t2 : (k2 ? -t1 : o12);
+          float n20 = k2 ? t0 : (k0 ? -t2 : o20);
+          if (k0) { P0 = P3; i0 = i3; }
+          if (k1) { P1 = P3; i1 = i3; }
+          if (k2) { P2 = P3; i2 = i3; }
+          o01 = n01; o12 = n12; o20 = n20;
+          i3 = T[nb * 4 + ((k + 3) & 3)];
+          tcur = nb;
         }
         if (anom32) anomalies++;
         if (anomaly) break;

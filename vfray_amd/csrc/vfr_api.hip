@@ -236,7 +236,7 @@ extern "C" int vfr_mesh_create(vfr_ctx* c, const float* xyzs, uint32_t nV, const
 extern "C" void vfr_mesh_destroy(vfr_mesh* m) {
   if (!m) return;
   if (m->ctx) (void)hipSetDevice(m->ctx->device);
-  void* ps[] = {m->d.rec, m->d.vert, m->d.ext_int, m->d.extf, m->d.perm, m->d.vperm, m->d.con};
+  void* ps[] = {m->d.rec, m->d.links, m->d.vert, m->d.ext_int, m->d.extf, m->d.perm, m->d.vperm, m->d.con};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete m;
@@ -412,6 +412,7 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   TravArgs a;
   const vfr_mesh* m = p->mesh;
   a.rec = m->d.rec;
+  a.links = reinterpret_cast<const uint2*>(m->d.links);
   a.vert = p->pverts;
   a.ext_int = m->d.ext_int;
   a.tf = p->ctx->tf;
@@ -740,7 +741,7 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* occ_out, uint32_t* wpc_o
   if (!cached) {
     /* occupancy variant x resident waves per CU (as multiples of the
      * variant's 4-SIMD capacity) x refill threshold */
-    static const uint32_t kOcc[] = {4, 5, 6};
+    static const uint32_t kOcc[] = {0, 4, 5, 6};
     static const uint32_t kMul[] = {1, 2};
     static const uint32_t kR[] = {8, 16, 32, 48, 64};
     const uint32_t saved_flags = p->opts.flags;
@@ -749,11 +750,13 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* occ_out, uint32_t* wpc_o
     TuneVal bv{p->opts.occupancy, p->opts.waves_per_cu, p->opts.refill};
     vfr_stats st;
     CHECK_RC(vfr_plan_execute(p, &st)); /* warm caches */
-    for (uint32_t occ : kOcc)
+    for (uint32_t occ : kOcc) {
+      uint32_t wps = 0;
+      CHECK_RC(vfr_k_traverse_waves_per_simd(occ, &wps));
       for (uint32_t mul : kMul)
         for (uint32_t r : kR) {
           p->opts.occupancy = occ;
-          p->opts.waves_per_cu = 4 * occ * mul;
+          p->opts.waves_per_cu = 4 * wps * mul;
           p->opts.refill = r;
           double t = 1e30;
           for (int rep = 0; rep < 2; ++rep) {
@@ -761,8 +764,9 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* occ_out, uint32_t* wpc_o
             if (rc != VFR_OK) { p->opts.flags = saved_flags; return rc; }
             t = std::min(t, st.ms_traversal);
           }
-          if (t < best) { best = t; bv = TuneVal{occ, 4 * occ * mul, r}; }
+          if (t < best) { best = t; bv = TuneVal{occ, 4 * wps * mul, r}; }
         }
+    }
     p->opts.occupancy = bv.occ;
     p->opts.waves_per_cu = bv.wpc;
     p->opts.refill = bv.refill;

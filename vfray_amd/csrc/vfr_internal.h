@@ -60,6 +60,9 @@ struct DevMesh {
   uint32_t n_verts = 0, n_tets = 0, n_ext = 0;
   float lmt[3] = {0, 0, 0};
   TetRec* rec = nullptr;       /* [n_tets] internal order                   */
+  uint32_t* links = nullptr;   /* [n_tets*4] (neighbour word, the neighbour's
+                                  vertex off the shared face) per face; the
+                                  tet's own opposite vertex when unlinked     */
   float* vert = nullptr;       /* [n_verts*4] internal order (x,y,z,s)      */
   uint32_t* ext_int = nullptr; /* [n_ext*2] (internal tet, face)            */
   uint32_t* extf = nullptr;    /* [n_ext*2] (caller tet, face), .extf order */
@@ -76,6 +79,7 @@ enum { ST_STEPS = 0, ST_ANOM = 1, ST_OVERFLOW = 2, ST_RAYS = 3, ST_N = 8 };
 #ifdef __HIPCC__
 struct TravArgs {
   const TetRec* rec;       /* record arena; a view's mesh starts at rec_off   */
+  const uint2* links;      /* [records * 4] (neighbour word, fourth vertex)   */
   const float4* vert;      /* P-space vertices of all views (+ view vert_off) */
   const uint32_t* ext_int; /* (internal tet, face) arena; + ext_off           */
   const float4* tf;        /* [slots * 256] */
@@ -112,10 +116,12 @@ int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, 
                   uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries,
                   float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
                   uint32_t* n_rays, void* stream);
-/* occupancy: target waves per SIMD of the persistent kernel variant
- * (<= 4: as compiled, 5 or 6: register-capped variants). */
+/* occupancy: waves per SIMD of the persistent kernel variant (4, 5, 6:
+ * register-capped variants; anything else: as compiled). */
 int vfr_k_traverse(uint32_t mode, uint32_t occupancy, const TravArgs& a, uint32_t max_rays,
                    int persistent, uint32_t grid_waves, void* stream);
+/* Waves per SIMD the persistent variant for `occupancy` reaches. */
+int vfr_k_traverse_waves_per_simd(uint32_t occupancy, uint32_t* out);
 
 /* host helpers shared by plans and the server (vfr_api.hip) */
 struct vfr_ctx;

@@ -677,44 +677,70 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
  * The one-ray-per-thread kernel and the persistent work-queue kernel both
  * drive it, so they produce the same bits.
  *
- * Exit faces come from 2-D edge functions of the pixel against the projected
- * tet edges, each computed in a canonical direction (lower vertex id first)
- * so that every face sharing an edge sees the same value: the three faces
- * behind the entry face then cover it without gaps.  A step computes only the
- * three edges to the tet's new vertex; the entry face's edges are carried. */
+ * Exit faces come from 2-D edge functions with the face vertices stored
+ * relative to the pixel: for an edge u -> v the value is u.x*v.y - u.y*v.x,
+ * and the same edge walked from the neighbouring tet, v -> u, gives exactly
+ * the negated float, so the three faces behind an entry face cover it
+ * without gaps or overlaps.  The entry face is ordered so that its edge
+ * functions are >= 0 (positive orientation); the faces behind it keep that
+ * orientation, so an exit face is one whose three edge functions are >= 0,
+ * not all zero.  A step computes the three edges to the tet's fourth vertex;
+ * the entry face's edges are carried, and the vertex the exit face leaves
+ * out is replaced in place.
+ *
+ * Loads are issued one tet ahead: once the exit face is known, its link
+ * (neighbour word and the neighbour's vertex off the shared face), then the
+ * next tet's vertex ids and fourth vertex are requested before the segment
+ * is composited, so their latency overlaps the transfer-function and psi
+ * lookups. */
 struct FtbRay {
-  float4 P0, P1, P2;       /* entry face vertices (P space) */
-  uint32_t i0, i1, i2;     /* their ids */
-  float c01, c12, c20;     /* canonical edge functions of the entry face */
-  float4 cP;
-  float ox, oy, rayf, Zprev, dprev, inv_me;
+  float4 P0, P1, P2;       /* entry face: (X - ox, Y - oy, Z, S) */
+  float4 Q;                /* the tet's fourth vertex, P space (prefetched) */
+  uint4 rv;                /* the tet's vertex ids (prefetched) */
+  uint32_t i0, i1, i2, i3; /* vertex ids of P0, P1, P2, Q */
+  float o01, o12, o20;     /* edge functions P0->P1, P1->P2, P2->P0 (>= 0) */
+  float4 cP;               /* TF colour at the entry point */
+  float ox, oy, rf, prev;  /* rf = ray length factor / max edge; prev = Z (ortho) or depth */
   float ar, ag, ab, Tr;
   uint32_t t, voff, roff, eoff, gp, tfo, steps;
-  uint32_t bits; /* cur 0..3 | nk 4..7 | persp 8 | anomaly 9 | broken 10 | fe 11..12 */
+  uint32_t bits; /* cur 0..3 | nk 4..7 | persp 8 | anomaly 9 | broken 10 */
 };
 
 DEV uint32_t ftb_cur(const FtbRay& r) { return r.bits & 15u; }
 DEV uint32_t ftb_nk(const FtbRay& r) { return (r.bits >> 4) & 15u; }
 DEV bool ftb_persp(const FtbRay& r) { return (r.bits >> 8) & 1u; }
-DEV uint32_t ftb_fe(const FtbRay& r) { return (r.bits >> 11) & 3u; }
-DEV void ftb_set_fe(FtbRay& r, uint32_t fe) { r.bits = (r.bits & ~(3u << 11)) | (fe << 11); }
 
-/* Edge function of the pixel against the edge from the lower-id vertex a
- * to the higher-id vertex b: (b - a) x (p - a). */
-DEV float edge_canon(uint32_t ia, float4 A, uint32_t ib, float4 B, float ox, float oy) {
-  const bool sw = ib < ia;
-  const float ax = sw ? B.x : A.x, ay = sw ? B.y : A.y;
-  const float bx = sw ? A.x : B.x, by = sw ? A.y : B.y;
-  return (bx - ax) * (oy - ay) - (by - ay) * (ox - ax);
+/* u -> v edge function of the pixel (vertices relative to the pixel). */
+DEV float edge_rel(float4 u, float4 v) { return u.x * v.y - u.y * v.x; }
+
+/* Inside a positively oriented face with edge functions a, b, c. */
+DEV bool inside_pos(float a, float b, float c) {
+  return fminf(a, fminf(b, c)) >= 0.0f && fmaxf(a, fmaxf(b, c)) > 0.0f;
 }
 
-/* The oriented value a -> b from the canonical one. */
-DEV float orient(uint32_t ia, uint32_t ib, float c) { return ia < ib ? c : -c; }
+DEV float pick3(bool c1, bool c2, float a, float b, float c) { return c1 ? a : (c2 ? b : c); }
+DEV uint32_t pick3u(bool c1, bool c2, uint32_t a, uint32_t b, uint32_t c) { return c1 ? a : (c2 ? b : c); }
 
-DEV bool inside3(float a, float b, float c) {
-  const bool pos = a >= 0.0f && b >= 0.0f && c >= 0.0f;
-  const bool neg = a <= 0.0f && b <= 0.0f && c <= 0.0f;
-  return (pos || neg) && !(a == 0.0f && b == 0.0f && c == 0.0f);
+DEV float4 rel(float4 p, float ox, float oy) { return make_float4(p.x - ox, p.y - oy, p.z, p.w); }
+
+/* Depth and scalar where the ray crosses face (A, B, C) with edge functions
+ * eAB, eBC, eCA (barycentric weights eBC, eCA, eAB over their sum): ortho
+ * returns Z; perspective returns the depth 1/Z from one reciprocal of Z*D,
+ * and the scalar S/Z from the same reciprocal. */
+DEV void face_values(bool persp, float Az, float Aw, float Bz, float Bw, float Cz, float Cw, float eAB,
+                     float eBC, float eCA, float* zd, float* s) {
+  const float D = eAB + eBC + eCA;
+  const float nz = eCA * (Bz - Az) + eAB * (Cz - Az);
+  const float ns = eCA * (Bw - Aw) + eAB * (Cw - Aw);
+  if (persp) {
+    const float r = 1.0f / (Az * D + nz);
+    *zd = D * r;
+    *s = clamp01((Aw * D + ns) * r);
+  } else {
+    const float inv = 1.0f / D;
+    *zd = Az + nz * inv;
+    *s = clamp01(Aw + ns * inv);
+  }
 }
 
 /* Starts the walk at entry `cur` of the pixel's sorted list. */
@@ -724,25 +750,26 @@ DEV bool ftb_segment(const TravArgs& a, FtbRay& r) {
   const float4* __restrict__ pv = a.vert + r.voff;
   r.t = a.ext_int[2 * (size_t)e];
   const uint32_t fe = a.ext_int[2 * (size_t)e + 1];
-  ftb_set_fe(r, fe);
-  const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[r.roff + r.t].v);
-  r.i0 = sel4(rv, fe); r.i1 = sel4(rv, (fe + 1u) & 3u); r.i2 = sel4(rv, (fe + 2u) & 3u);
-  r.P0 = pv[r.i0];
-  r.P1 = pv[r.i1];
-  r.P2 = pv[r.i2];
-  r.c01 = edge_canon(r.i0, r.P0, r.i1, r.P1, r.ox, r.oy);
-  r.c12 = edge_canon(r.i1, r.P1, r.i2, r.P2, r.ox, r.oy);
-  r.c20 = edge_canon(r.i2, r.P2, r.i0, r.P0, r.ox, r.oy);
-  const float o01 = orient(r.i0, r.i1, r.c01), o12 = orient(r.i1, r.i2, r.c12), o20 = orient(r.i2, r.i0, r.c20);
+  r.rv = *reinterpret_cast<const uint4*>(a.rec[r.roff + r.t].v);
+  uint32_t i0 = sel4(r.rv, fe), i1 = sel4(r.rv, (fe + 1u) & 3u), i2 = sel4(r.rv, (fe + 2u) & 3u);
+  r.i3 = sel4(r.rv, (fe + 3u) & 3u);
+  r.Q = pv[r.i3];
+  float4 P0 = rel(pv[i0], r.ox, r.oy), P1 = rel(pv[i1], r.ox, r.oy), P2 = rel(pv[i2], r.ox, r.oy);
+  float o01 = edge_rel(P0, P1), o12 = edge_rel(P1, P2), o20 = edge_rel(P2, P0);
   const float D = o01 + o12 + o20;
   if (D == 0.0f) return false;
-  const float inv = 1.0f / D;
-  /* weights: P1 <- o20 / D, P2 <- o01 / D */
-  r.Zprev = r.P0.z + ((o20 * (r.P1.z - r.P0.z) + o01 * (r.P2.z - r.P0.z)) * inv);
-  const float Sprev = r.P0.w + ((o20 * (r.P1.w - r.P0.w) + o01 * (r.P2.w - r.P0.w)) * inv);
+  if (D < 0.0f) {
+    /* order (P0, P2, P1): every edge function flips sign exactly */
+    const float4 tP = P1; P1 = P2; P2 = tP;
+    const uint32_t ti = i1; i1 = i2; i2 = ti;
+    const float n01 = -o20, n12 = -o12, n20 = -o01;
+    o01 = n01; o12 = n12; o20 = n20;
+  }
+  r.P0 = P0; r.P1 = P1; r.P2 = P2;
+  r.i0 = i0; r.i1 = i1; r.i2 = i2;
+  r.o01 = o01; r.o12 = o12; r.o20 = o20;
   float sprev;
-  if (ftb_persp(r)) { r.dprev = 1.0f / r.Zprev; sprev = clamp01(Sprev * r.dprev); }
-  else { r.dprev = 0.0f; sprev = clamp01(Sprev); }
+  face_values(ftb_persp(r), P0.z, P0.w, P1.z, P1.w, P2.z, P2.w, o01, o12, o20, &r.prev, &sprev);
   r.cP = a.tf[r.tfo + min(255u, (uint32_t)(255.0f * sprev))];
   return true;
 }
@@ -754,12 +781,12 @@ DEV void ftb_init(const TravArgs& a, uint32_t vi, uint32_t pix, FtbRay& r) {
   r.voff = a.views[vi].vert_off;
   r.roff = a.views[vi].rec_off;
   r.eoff = a.views[vi].ext_off;
-  r.inv_me = a.views[vi].inv_me;
   r.tfo = a.views[vi].tf * 256u;
   r.gp = a.views[vi].pix_off + pix;
   r.ox = pix_to_obj(pix % W, W);
   r.oy = pix_to_obj(pix / W, H);
-  r.rayf = persp ? sqrtf(1.0f + (r.ox * r.ox + r.oy * r.oy) * (k * k)) : k;
+  const float rayf = persp ? sqrtf(1.0f + (r.ox * r.ox + r.oy * r.oy) * (k * k)) : k;
+  r.rf = rayf * a.views[vi].inv_me;
   uint32_t nk = a.cnt[r.gp];
   nk = nk > VFR_K ? VFR_K : nk;
   r.bits = (nk << 4) | (persp ? 256u : 0u);
@@ -769,95 +796,109 @@ DEV void ftb_init(const TravArgs& a, uint32_t vi, uint32_t pix, FtbRay& r) {
   if (!ftb_segment(a, r)) r.bits |= 512u;
 }
 
-/* One tet of the walk; returns true when the ray is finished. */
+/* One tet of the walk; returns true when the ray is finished.
+ *
+ * The body is one straight path (the ray state is updated in place on every
+ * path, so no copies of it are needed where paths join), and its loads go
+ * out in two rounds: the exit face's link and the transfer-function colour,
+ * then the psi entry, the next tet's vertex ids and its fourth vertex.  The
+ * psi lookup is issued first in the second round because memory results
+ * return in issue order and it is the one waited for within the step. */
 DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
   if ((r.bits & 512u) || ++r.steps > VFR_MAX_STEPS) { r.bits |= 512u; return true; }
-  const TetRec* __restrict__ rc = a.rec + r.roff + r.t;
-  const uint4 rv = *reinterpret_cast<const uint4*>(rc->v);
-  const uint4 rn = *reinterpret_cast<const uint4*>(rc->n);
-  const uint32_t i3 = sel4(rv, (ftb_fe(r) + 3u) & 3u);
-  const float4 P3 = a.vert[r.voff + i3];
-  const float c30 = edge_canon(i3, P3, r.i0, r.P0, r.ox, r.oy);
-  const float c31 = edge_canon(i3, P3, r.i1, r.P1, r.ox, r.oy);
-  const float c32 = edge_canon(i3, P3, r.i2, r.P2, r.ox, r.oy);
-  const float t0 = orient(i3, r.i0, c30), t1 = orient(i3, r.i1, c31), t2 = orient(i3, r.i2, c32);
-  const float o01 = orient(r.i0, r.i1, r.c01), o12 = orient(r.i1, r.i2, r.c12), o20 = orient(r.i2, r.i0, r.c20);
-  /* candidate k = face (P3, Pi, Pj) for (i,j) = (0,1), (1,2), (2,0):
-   * oriented edges 3->i, i->j, j->3 */
-  const bool p1 = inside3(t0, o01, -t1), p2 = inside3(t1, o12, -t2), p3 = inside3(t2, o20, -t0);
-  if (!(p1 || p2 || p3)) { r.bits |= 512u; return true; }
+  const float4 P3 = rel(r.Q, r.ox, r.oy);
+  const float t0 = edge_rel(P3, r.P0), t1 = edge_rel(P3, r.P1), t2 = edge_rel(P3, r.P2);
+  /* candidate faces (P3, Pi, Pj) for (i,j) = (0,1), (1,2), (2,0): edges
+   * P3->Pi, Pi->Pj, Pj->P3 */
+  const bool p1 = inside_pos(t0, r.o01, -t1), p2 = inside_pos(t1, r.o12, -t2), p3 = inside_pos(t2, r.o20, -t0);
+  const bool none = !(p1 || p2 || p3);
   bool c1 = p1, c2 = !p1 && p2;
   if ((int)p1 + (int)p2 + (int)p3 > 1) {
     /* the pixel is on a projected edge: exit at the point farthest along the
      * ray (smallest Z), ties in candidate order */
     const float inf = __uint_as_float(0x7f800000u);
-    const float z1 = p1 ? P3.z + ((-t1 * (r.P0.z - P3.z) + t0 * (r.P1.z - P3.z)) * (1.0f / (t0 + o01 - t1))) : inf;
-    const float z2 = p2 ? P3.z + ((-t2 * (r.P1.z - P3.z) + t1 * (r.P2.z - P3.z)) * (1.0f / (t1 + o12 - t2))) : inf;
-    const float z3 = p3 ? P3.z + ((-t0 * (r.P2.z - P3.z) + t2 * (r.P0.z - P3.z)) * (1.0f / (t2 + o20 - t0))) : inf;
+    const float z1 = p1 ? P3.z + (-t1 * (r.P0.z - P3.z) + t0 * (r.P1.z - P3.z)) / (t0 + r.o01 - t1) : inf;
+    const float z2 = p2 ? P3.z + (-t2 * (r.P1.z - P3.z) + t1 * (r.P2.z - P3.z)) / (t1 + r.o12 - t2) : inf;
+    const float z3 = p3 ? P3.z + (-t0 * (r.P2.z - P3.z) + t2 * (r.P0.z - P3.z)) / (t2 + r.o20 - t0) : inf;
     c1 = p1 && z1 <= z2 && z1 <= z3;
     c2 = !c1 && p2 && z2 <= z3;
   }
-  /* exit face (A, B, C) = (P3, Pi, Pj); k = the tet vertex it leaves out */
-  const float4 B = sel4f(c1, r.P0, sel4f(c2, r.P1, r.P2));
-  const float4 C = sel4f(c1, r.P1, sel4f(c2, r.P2, r.P0));
-  const uint32_t ib = c1 ? r.i0 : (c2 ? r.i1 : r.i2);
-  const uint32_t ic = c1 ? r.i1 : (c2 ? r.i2 : r.i0);
-  const uint32_t ik = c1 ? r.i2 : (c2 ? r.i0 : r.i1);
-  const float eAB = c1 ? t0 : (c2 ? t1 : t2);
-  const float eBC = c1 ? o01 : (c2 ? o12 : o20);
-  const float eCA = -(c1 ? t1 : (c2 ? t2 : t0));
-  const float cbc = c1 ? r.c01 : (c2 ? r.c12 : r.c20);
-  const float cab = c1 ? c30 : (c2 ? c31 : c32);
-  const float cca = c1 ? c31 : (c2 ? c32 : c30);
-  const float inv = 1.0f / (eAB + eBC + eCA);
-  const float Znext = P3.z + ((eCA * (B.z - P3.z) + eAB * (C.z - P3.z)) * inv);
-  const float Snext = P3.w + ((eCA * (B.w - P3.w) + eAB * (C.w - P3.w)) * inv);
-  /* the exit face is the tet face opposite vertex k: face index slot(k) + 1 */
-  const uint32_t sk = rv.x == ik ? 0u : (rv.y == ik ? 1u : (rv.z == ik ? 2u : 3u));
-  const uint32_t nb = sel4(rn, (sk + 1u) & 3u);
-  float snext, dist;
-  if (ftb_persp(r)) {
-    const float dn = 1.0f / Znext;
-    snext = clamp01(Snext * dn);
-    dist = fabsf(dn - r.dprev) * r.rayf;
-    r.dprev = dn;
-  } else {
-    snext = clamp01(Snext);
-    dist = fabsf(r.Zprev - Znext) * r.rayf;
-  }
-  const float dl = dist * r.inv_me;
+  /* exit face (P3, Pi, Pj): eAB = t_i, eBC = o_ij, eCA = -t_j (values are
+   * picked, never addresses, so the ray state stays in registers).  With no
+   * exit face the ray ends as an anomaly; the step still runs on in-range
+   * indices and its segment is not composited. */
+  float zd, snext;
+  face_values(ftb_persp(r), P3.z, P3.w, pick3(c1, c2, r.P0.z, r.P1.z, r.P2.z),
+              pick3(c1, c2, r.P0.w, r.P1.w, r.P2.w), pick3(c1, c2, r.P1.z, r.P2.z, r.P0.z),
+              pick3(c1, c2, r.P1.w, r.P2.w, r.P0.w), pick3(c1, c2, t0, t1, t2),
+              pick3(c1, c2, r.o01, r.o12, r.o20), -pick3(c1, c2, t1, t2, t0), &zd, &snext);
+  snext = none ? 0.0f : snext;
+  /* the exit face leaves out vertex k; its face index is slot(k) + 1 */
+  const bool k0 = c2, k1 = !c1 && !c2, k2 = c1;
+  const uint32_t ik = pick3u(c1, c2, r.i2, r.i0, r.i1);
+  const uint32_t sk = r.rv.x == ik ? 0u : (r.rv.y == ik ? 1u : (r.rv.z == ik ? 2u : 3u));
+  const uint32_t fx = (sk + 1u) & 3u;
+  /* round 1: the link of the exit face and the colour at the exit point */
+  const uint2 ln = a.links[(size_t)(r.roff + r.t) * 4u + fx];
   const float4 cN = a.tf[r.tfo + min(255u, (uint32_t)(255.0f * snext))];
+  __builtin_amdgcn_sched_barrier(0);
+  /* round 2: psi, then the next tet (the current one again at a boundary,
+   * unused) */
+  float dl = fabsf(zd - r.prev) * r.rf;
+  dl = none ? 0.0f : dl;
   const float tf_ = r.cP.w * dl, tb_ = cN.w * dl;
-  const float zeta = vfr_expf(-(tf_ + tb_) * 0.5f);
   /* gamma = tau / (1 + tau) for both ends with one division */
   const float uf = 1.0f + tf_, ub = 1.0f + tb_;
   const float q = 1.0f / (uf * ub);
   const float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
   const float ps = a.psi[psi_index(gf) * 512u + psi_index(gb)];
+  const uint32_t nb = ln.x;
+  const uint32_t tn = nb < VFR_NB_BROKEN ? (nb >> 2) : r.t;
+  const float4 Qn = a.vert[r.voff + ln.y];
+  const uint4 rvn = *reinterpret_cast<const uint4*>(a.rec[r.roff + tn].v);
+  __builtin_amdgcn_sched_barrier(0);
+  const float zeta = vfr_expf(-(tf_ + tb_) * 0.5f);
+  /* the exit face becomes the entry face: vertex k is replaced by P3 (a ray
+   * that ends here never reads it) */
+  const float n01 = pick3(k0, k1, t1, -t0, r.o01);
+  const float n12 = pick3(k1, k2, t2, -t1, r.o12);
+  const float n20 = pick3(k2, k0, t0, -t2, r.o20);
+  r.P0 = sel4f(k0, P3, r.P0);
+  r.P1 = sel4f(k1, P3, r.P1);
+  r.P2 = sel4f(k2, P3, r.P2);
+  r.i0 = k0 ? r.i3 : r.i0;
+  r.i1 = k1 ? r.i3 : r.i1;
+  r.i2 = k2 ? r.i3 : r.i2;
+  r.o01 = n01; r.o12 = n12; r.o20 = n20;
+  /* composite the segment */
   const float w1 = 1.0f - ps, w2 = ps - zeta;
-  r.ar += r.Tr * (r.cP.x * w1 + cN.x * w2);
-  r.ag += r.Tr * (r.cP.y * w1 + cN.y * w2);
-  r.ab += r.Tr * (r.cP.z * w1 + cN.z * w2);
-  r.Tr *= zeta;
+  const float ar = r.ar + r.Tr * (r.cP.x * w1 + cN.x * w2);
+  const float ag = r.ag + r.Tr * (r.cP.y * w1 + cN.y * w2);
+  const float ab = r.ab + r.Tr * (r.cP.z * w1 + cN.z * w2);
+  const float Tr = r.Tr * zeta;
+  r.ar = none ? r.ar : ar;
+  r.ag = none ? r.ag : ag;
+  r.ab = none ? r.ab : ab;
+  r.Tr = none ? r.Tr : Tr;
   r.cP = cN;
+  r.prev = zd;
+  r.t = tn;
+  r.i3 = ln.y;
+  r.Q = Qn;
+  r.rv = rvn;
+  if (none) { r.bits |= 512u; return true; }
   if (1.0f - r.Tr >= a.term) return true;
   if (nb == VFR_NB_BROKEN) { r.bits |= 1024u; return true; }
   if (nb == VFR_NB_EXT) {
     /* re-entry: the first entry strictly behind the exit point */
+    const float exitZ = ftb_persp(r) ? 1.0f / zd : zd;
     uint32_t cur = ftb_cur(r) + 1u;
     const uint32_t nk = ftb_nk(r);
-    while (cur < nk && !(funkey((uint32_t)(a.entries[(size_t)r.gp * VFR_K + cur] >> 32)) < Znext)) ++cur;
+    while (cur < nk && !(funkey((uint32_t)(a.entries[(size_t)r.gp * VFR_K + cur] >> 32)) < exitZ)) ++cur;
     if (cur >= nk) return true;
     r.bits = (r.bits & ~15u) | cur;
     if (!ftb_segment(a, r)) { r.bits |= 512u; return true; }
-    return false;
   }
-  r.t = nb >> 2;
-  ftb_set_fe(r, nb & 3u);
-  r.P0 = P3; r.P1 = B; r.P2 = C;
-  r.i0 = i3; r.i1 = ib; r.i2 = ic;
-  r.c01 = cab; r.c12 = cbc; r.c20 = cca;
-  r.Zprev = Znext;
   return false;
 }
 
@@ -969,6 +1010,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   }
 }
 
+int vfr_k_traverse_waves_per_simd(uint32_t occupancy, uint32_t* out) {
+  int blocks = 0;
+  hipError_t e;
+  if (occupancy >= 6) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trav_persist<6>, 256, 0);
+  else if (occupancy == 5) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trav_persist<5>, 256, 0);
+  else if (occupancy == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trav_persist<4>, 256, 0);
+  else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trav_persist<1>, 256, 0);
+  CHECK_HIP(e);
+  /* a 256-thread block is 4 waves, one per SIMD of the CU */
+  *out = blocks < 1 ? 1u : (uint32_t)blocks;
+  return VFR_OK;
+}
+
 int vfr_k_traverse(uint32_t mode, uint32_t occupancy, const TravArgs& a, uint32_t max_rays,
                    int persistent, uint32_t grid_waves, void* stream) {
   if (max_rays == 0) return VFR_OK;
@@ -980,6 +1034,8 @@ int vfr_k_traverse(uint32_t mode, uint32_t occupancy, const TravArgs& a, uint32_
       hipLaunchKernelGGL(k_trav_persist<6>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
     else if (occupancy == 5)
       hipLaunchKernelGGL(k_trav_persist<5>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
+    else if (occupancy == 4)
+      hipLaunchKernelGGL(k_trav_persist<4>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
     else
       hipLaunchKernelGGL(k_trav_persist<1>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
     CHECK_HIP(hipGetLastError());
@@ -1101,27 +1157,35 @@ __global__ void k_gather_vert(const float4* __restrict__ v, const uint32_t* __re
 
 /* Internal tet record: vertex ids through the vertex permutation; neighbour
  * words (internal id << 2 | the face of the neighbour that links back, the
- * back-lookup of vfRayKernelCPU.cc:359-366). */
+ * back-lookup of vfRayKernelCPU.cc:359-366).  links[4i+f] pairs the
+ * neighbour word of face f with the neighbour's vertex off that face (the
+ * tet's own vertex opposite f when there is no linked neighbour), so a walk
+ * can fetch the next tet's fourth vertex without waiting for its record. */
 __global__ void k_build_rec(const uint32_t* __restrict__ tets, const uint32_t* __restrict__ con,
                             const uint32_t* __restrict__ perm, const uint32_t* __restrict__ tinv,
                             const uint32_t* __restrict__ vinv, uint32_t nT,
-                            TetRec* __restrict__ rec) {
+                            TetRec* __restrict__ rec, uint2* __restrict__ links) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nT) return;
   uint32_t t = perm[i];
   TetRec r;
+  uint32_t o[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) r.v[k] = vinv[tets[t * 4 + k]];
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
     uint32_t u = con[t * 4 + f];
+    o[f] = r.v[(f + 3) & 3];
     if (u == t) { r.n[f] = VFR_NB_EXT; continue; }
     uint32_t kk = 4;
     for (uint32_t q = 0; q < 4; ++q)
       if (con[u * 4 + q] == t) { kk = q; break; }
     r.n[f] = kk == 4 ? VFR_NB_BROKEN : ((tinv[u] << 2) | kk);
+    if (kk != 4) o[f] = vinv[tets[u * 4 + ((kk + 3) & 3)]];
   }
   rec[i] = r;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) links[(size_t)i * 4 + f] = make_uint2(r.n[f], o[f]);
 }
 
 __global__ void k_ext_internal(const uint32_t* __restrict__ extf, uint32_t n,
@@ -1275,15 +1339,16 @@ int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
   }
   hipLaunchKernelGGL(k_invert, dim3(nblk(nT)), dim3(256), 0, st, perm.as<uint32_t>(), nT, tinv.as<uint32_t>());
   hipLaunchKernelGGL(k_invert, dim3(nblk(nV)), dim3(256), 0, st, vperm.as<uint32_t>(), nV, vinv.as<uint32_t>());
-  DevBuf vert, rec, ext_int;
+  DevBuf vert, rec, links, ext_int;
   ALLOC(vert, sizeof(float) * 4 * (size_t)nV);
   ALLOC(rec, sizeof(TetRec) * (size_t)nT);
+  ALLOC(links, sizeof(uint2) * 4 * (size_t)nT);
   ALLOC(ext_int, sizeof(uint32_t) * 2 * (size_t)n_ext);
   hipLaunchKernelGGL(k_gather_vert, dim3(nblk(nV)), dim3(256), 0, st, d_xyzs.as<float4>(),
                      vperm.as<uint32_t>(), nV, vert.as<float4>());
   hipLaunchKernelGGL(k_build_rec, dim3(nblk(nT)), dim3(256), 0, st, d_tets.as<uint32_t>(),
                      d_con.as<uint32_t>(), perm.as<uint32_t>(), tinv.as<uint32_t>(),
-                     vinv.as<uint32_t>(), nT, rec.as<TetRec>());
+                     vinv.as<uint32_t>(), nT, rec.as<TetRec>(), links.as<uint2>());
   if (n_ext)
     hipLaunchKernelGGL(k_ext_internal, dim3(nblk(n_ext)), dim3(256), 0, st, d_extf.as<uint32_t>(),
                        n_ext, tinv.as<uint32_t>(), ext_int.as<uint32_t>());
@@ -1292,12 +1357,13 @@ int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
   m.n_ext = n_ext;
   m.vert = vert.as<float>(); vert.release();
   m.rec = rec.as<TetRec>(); rec.release();
+  m.links = links.as<uint32_t>(); links.release();
   m.ext_int = ext_int.as<uint32_t>(); ext_int.release();
   m.extf = d_extf.as<uint32_t>(); d_extf.release();
   m.perm = perm.as<uint32_t>(); perm.release();
   m.vperm = vperm.as<uint32_t>(); vperm.release();
   m.con = d_con.as<uint32_t>(); d_con.release();
   m.bytes = sizeof(float) * 4ull * nV + sizeof(TetRec) * (uint64_t)nT + 16ull * n_ext +
-            4ull * nT + 4ull * nV + 16ull * nT;
+            4ull * nT + 4ull * nV + 16ull * nT + 32ull * nT;
   return VFR_OK;
 }

@@ -106,6 +106,7 @@ struct vfr_server {
   vfr_server_cfg cfg{};
   std::vector<SrvMesh> meshes;
   TetRec* rec = nullptr;
+  uint32_t* links = nullptr;
   uint32_t* ext = nullptr;
   float4* vert = nullptr;
   std::vector<std::unique_ptr<Slot>> slots;
@@ -192,6 +193,7 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   TravArgs a;
   memset(&a, 0, sizeof(a));
   a.rec = sv->rec;
+  a.links = reinterpret_cast<const uint2*>(sv->links);
   a.vert = s->pverts;
   a.ext_int = sv->ext;
   a.tf = vfr_ctx_tf(sv->ctx);
@@ -325,6 +327,7 @@ extern "C" void vfr_server_destroy(vfr_server* sv) {
     slot_free(sp.get());
   }
   if (sv->rec) (void)hipFree(sv->rec);
+  if (sv->links) (void)hipFree(sv->links);
   if (sv->ext) (void)hipFree(sv->ext);
   if (sv->vert) (void)hipFree(sv->vert);
   delete sv;
@@ -340,8 +343,11 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
   if (c.streams == 0) c.streams = 2;
   if (c.max_batch == 0) c.max_batch = 256;
   if (!(c.term_alpha > 0.0f)) c.term_alpha = 0.99f;
-  if (c.occupancy == 0) c.occupancy = 4;
-  if (c.waves_per_cu == 0) c.waves_per_cu = 16;
+  if (c.waves_per_cu == 0) {
+    uint32_t wps = 0;
+    CHECK_RC(vfr_k_traverse_waves_per_simd(c.occupancy, &wps));
+    c.waves_per_cu = std::min(64u, 8u * wps);
+  }
   if (c.refill == 0) c.refill = 32;
   if (c.streams > 16 || c.max_batch > 4096 || c.term_alpha > 1.0f || c.waves_per_cu > 64 || c.refill > 64) {
     vfr_set_error("vfr_server_create: configuration out of range");
@@ -366,6 +372,7 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
   CHECK_HIP(hipSetDevice(vfr_ctx_device(ctx)));
   int rc = VFR_OK;
   if (hipMalloc((void**)&sv->rec, sizeof(TetRec) * tets) != hipSuccess ||
+      hipMalloc((void**)&sv->links, 32 * tets) != hipSuccess ||
       hipMalloc((void**)&sv->ext, 8 * (exts + 1)) != hipSuccess ||
       hipMalloc((void**)&sv->vert, 16 * verts) != hipSuccess) {
     vfr_set_error("resident mesh arena allocation failed");
@@ -380,6 +387,7 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
     m.max_edge = d->lmt[0];
     vfr_mesh_bbox(meshes[i], m.bb_lo, m.bb_hi);
     if (hipMemcpy(sv->rec + ro, d->rec, sizeof(TetRec) * d->n_tets, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(sv->links + 8 * ro, d->links, 32ull * d->n_tets, hipMemcpyDeviceToDevice) != hipSuccess ||
         hipMemcpy(sv->ext + 2 * eo, d->ext_int, 8ull * d->n_ext, hipMemcpyDeviceToDevice) != hipSuccess ||
         hipMemcpy(sv->vert + vo, d->vert, 16ull * d->n_verts, hipMemcpyDeviceToDevice) != hipSuccess) {
       vfr_set_error("copy of mesh %d into the resident arena failed", i);
