@@ -68,11 +68,11 @@ def cpu_baseline(x, t, tfa, views, seconds: float) -> dict:
                       f"restatement, FP32, OpenMP {threads} threads, {el:.1f} s"}
 
 
-def measured_traffic(workload: str):
+def measured_traffic(kernel: str, workload: str):
     """HBM/fabric bytes per traversal launch from the committed rocprofv3
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads
     for a copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", f"k_trav_persist_{workload}.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
     if not os.path.exists(path):
         return None
     txt = open(path).read()
@@ -95,7 +95,8 @@ def serve_bench(args) -> int:
         tf_ids.append([ctx.tf(tf) for tf in tfs])
         tets += t.shape[0]
     log(f"8 resident meshes, {tets} tets, built in {time.time() - t0:.1f}s")
-    sched = dict(occupancy=args.occupancy, waves_per_cu=args.wpc, refill=args.refill or 32)
+    sched = dict(persistent=args.persistent, occupancy=args.occupancy, waves_per_cu=args.wpc,
+                 refill=args.refill or 32)
     reqs = WL.serve_requests(args.requests, len(meshes), tf_ids)
     rays_of = lambda rs: sum(r.view.width * r.view.height for r in rs)
     n_cap = min(len(reqs), max(200, args.requests // 5))
@@ -164,7 +165,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--graph", action="store_true")
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched: skip the schedule autotuner (one ray per thread unless --persistent)")
+    ap.add_argument("--persistent", action="store_true", help="persistent work-queue traversal kernel")
     ap.add_argument("--occupancy", type=int, default=0)
     ap.add_argument("--wpc", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
@@ -206,16 +209,18 @@ def main() -> int:
     total_rays = sum(v.width * v.height for v in views)
 
     tune = None
-    if args.autotune:
+    manual = args.persistent or args.occupancy or args.wpc or args.refill
+    if not args.no_autotune and not manual:
         pt = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
         tune = pt.autotune()
         pt.close()
         log(f"autotune: {tune}")
-    kw = {} if tune is None else {"waves_per_cu": tune["waves_per_cu"], "refill": tune["refill"],
-                                  "occupancy": tune["occupancy"]}
-    if tune is None and (args.occupancy or args.wpc or args.refill):
-        kw = {"waves_per_cu": args.wpc, "refill": args.refill, "occupancy": args.occupancy}
-        tune = {"occupancy": args.occupancy, "waves_per_cu": args.wpc, "refill": args.refill}
+    kw = {} if tune is None else {"persistent": tune["persistent"], "waves_per_cu": tune["waves_per_cu"],
+                                  "refill": tune["refill"], "occupancy": tune["occupancy"]}
+    if tune is None and manual:
+        kw = {"persistent": args.persistent, "waves_per_cu": args.wpc, "refill": args.refill,
+              "occupancy": args.occupancy}
+        tune = dict(kw)
     pb = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, use_graph=args.graph, **kw)
     for _ in range(args.warmup):
         pb.execute()
@@ -253,10 +258,10 @@ def main() -> int:
                 log(f"view {i} {views[i].width}^2 frame {ft[i]:.2f} ms rays {int(act.sum())} "
                     f"steps max {int(r['steps'].max())} mean {float(r['steps'][act].mean()):.1f} "
                     f"sum {int(r['steps'].sum())}")
-            pd = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, simple_kernel=True)
+            pd = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, persistent=True)
             pd.execute()
             sd = pd.execute()
-            log(f"batched, one ray per thread: traversal {sd['ms_traversal']:.2f} ms")
+            log(f"batched, persistent kernel (defaults): traversal {sd['ms_traversal']:.2f} ms")
             pd.close()
         log(f"plain: {pwall * 1e3:.2f} ms/step")
         pp.close()
@@ -267,13 +272,14 @@ def main() -> int:
     roof = None
     if ms_trav:
         ach = alg_bytes / (ms_trav * 1e-3) / 1e9
-        tr = measured_traffic(args.workload)
+        kname = "k_trav_persist" if kw.get("persistent") else "k_trav"
+        tr = measured_traffic(kname, args.workload)
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(ach / PEAK_HBM_GBS, 4),
                 "traffic": tr["traffic_bytes"] if tr else None,
-                "traffic_source": f"profiles/k_trav_persist_{args.workload}.md" if tr else None,
+                "traffic_source": f"profiles/{kname}_{args.workload}.md" if tr else None,
                 "l2_hit": tr["l2_hit"] if tr else None,
-                "kernel": "k_trav_persist", "kernel_ms": round(ms_trav, 3),
+                "kernel": kname, "kernel_ms": round(ms_trav, 3),
                 "alg_bytes": int(alg_bytes)}
     cpu = None
     if not args.no_cpu:
@@ -296,7 +302,7 @@ def main() -> int:
         "config": {"workload": args.workload, "tets": int(t.shape[0]), "views": nviews,
                    "rays_per_step": int(total_rays), "active_rays": int(rays_active),
                    "tet_steps": int(steps_per_launch), "term_alpha": 0.99, "graph": args.graph,
-                   "schedule": tune or {"occupancy": 4, "waves_per_cu": 16, "refill": 32}},
+                   "schedule": tune or {"persistent": False}},
         "anomalous_rays": int(st["anomalies"]),
         "event_ms_per_step": round(ms_evt, 3),
         "active_mrays_per_s": round(rays_active / wall / 1e6, 2),

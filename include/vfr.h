@@ -142,8 +142,9 @@ typedef struct vfr_render_opts {
   uint32_t mode;        /* VFR_MODE_*                                      */
   uint32_t schedule;    /* VFR_SCHED_*                                     */
   float term_alpha;     /* FTB early termination (0.99 default if <= 0)    */
-  uint32_t waves_per_cu;/* persistent kernel occupancy, 0 = tuned          */
-  uint32_t refill;      /* dead lanes that trigger a wave refill, 0 = tuned */
+  uint32_t waves_per_cu;/* persistent kernel: resident waves per CU (0 = 16) */
+  uint32_t refill;      /* persistent kernel: idle lanes that trigger a wave
+                           refill (0 = 32)                                 */
   uint32_t use_graph;   /* capture the batched launch set in a hipGraph    */
   uint32_t max_entries; /* FTB entry-list slots per pixel, 0 = 8           */
   uint32_t flags;       /* VFR_RENDER_*                                    */
@@ -153,9 +154,11 @@ typedef struct vfr_render_opts {
   uint32_t reserved;
 } vfr_render_opts;
 
-/* Batched FTB renders use the persistent work-queue kernel unless this flag
- * asks for one ray per thread over the compacted ray list. */
-#define VFR_RENDER_SIMPLE_KERNEL 0x1u
+/* Batched FTB renders run one ray per thread over the device-built list of
+ * active rays of all views (the hardware dispatcher refills the CUs wave by
+ * wave) unless this flag asks for the persistent work-queue kernel (a fixed
+ * grid of waves that claims rays from the list and refills idle lanes). */
+#define VFR_RENDER_PERSISTENT 0x1u
 
 typedef struct vfr_frame_out {
   float* rgba;          /* W*H*4, may be NULL                              */
@@ -187,12 +190,13 @@ void vfr_plan_destroy(vfr_plan* plan);
 /* Per-view frame latency (ms, HIP events) of the last PER_VIEW execute. */
 int vfr_plan_frame_times(const vfr_plan* plan, double* out_ms);
 
-/* Times candidate persistent-kernel schedules (kernel occupancy variant x
- * waves per CU x refill threshold) on the plan itself and keeps the fastest;
- * returns the choice and its traversal time.  Results are cached per
- * (tets, rays, views) in the process. */
-int vfr_plan_autotune(vfr_plan* plan, uint32_t* occupancy, uint32_t* waves_per_cu,
-                      uint32_t* refill, double* best_ms);
+/* Times the one-ray-per-thread traversal and candidate persistent-kernel
+ * schedules (kernel occupancy variant x waves per CU x refill threshold) on
+ * the plan itself and keeps the fastest; returns the choice (flags holds
+ * VFR_RENDER_PERSISTENT when the persistent kernel won) and its traversal
+ * time.  Results are cached per (tets, rays, views) in the process. */
+int vfr_plan_autotune(vfr_plan* plan, uint32_t* flags, uint32_t* occupancy,
+                      uint32_t* waves_per_cu, uint32_t* refill, double* best_ms);
 
 /* create + execute + read + destroy. outs has n_views entries. */
 int vfr_render(vfr_ctx* ctx, const vfr_mesh* mesh, const vfr_view* views,
@@ -212,7 +216,8 @@ typedef struct vfr_server_cfg {
   uint32_t streams;     /* slots = concurrent batches (0 = 2)                    */
   uint32_t max_batch;   /* requests per batch (0 = 256)                          */
   float term_alpha;     /* early termination (0 = 0.99)                          */
-  uint32_t occupancy;   /* persistent traversal schedule (0 = defaults)          */
+  uint32_t flags;       /* VFR_RENDER_PERSISTENT: persistent traversal kernel    */
+  uint32_t occupancy;   /* its schedule (0 = defaults, as vfr_render_opts)       */
   uint32_t waves_per_cu;
   uint32_t refill;
 } vfr_server_cfg;

@@ -537,7 +537,8 @@ static inline u32 fkey(float f) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-static inline float clamp01(float s) { return s < 0.0f ? 0.0f : (s > 1.0f ? 1.0f : s); }
+/* [0,1] clamp; NaN maps to 0. */
+static inline float clamp01(float s) { return std::fmin(std::fmax(s, 0.0f), 1.0f); }
 
 /* Barycentric value of a triangle at (ox,oy) in double. */
 static inline double bary64(const D4& a, const D4& b, const D4& c, double ox, double oy,
@@ -550,16 +551,23 @@ static inline double bary64(const D4& a, const D4& b, const D4& c, double ox, do
   return a.*m + u * (b.*m - a.*m) + w * (c.*m - a.*m);
 }
 
-/* exp(x) for x <= 0 in binary32: Cody-Waite reduction x = n ln2 + r with a
- * two-part ln2, degree-7 Taylor polynomial on |r| <= ln2/2, scaled by 2^n.
+/* exp(x) for x <= 0 in binary32 (clamped at -87): Cody-Waite reduction
+ * x = n ln2 + r with a two-part ln2, degree-7 Taylor polynomial on
+ * |r| <= ln2/2 in Horner form with fused multiply-adds, scaled by 2^n.
  * The renderer's kernels evaluate the same operations, so the two agree bit
  * for bit; accuracy against libm is checked by the tests. */
 extern "C" float orc_expf(float x) {
-  if (x < -87.0f) return 0.0f;
+  x = std::max(x, -87.0f);
   float n = rintf(x * 1.44269504f);
-  float r = (x - n * 0.693145751953125f) - n * 1.42860682030941723e-6f;
-  float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.166666672f + r * (0.0416666679f +
-            r * (0.00833333377f + r * (0.00138888892f + r * 0.000198412701f))))));
+  float r = fmaf(-n, 0.693145751953125f, x);
+  r = fmaf(-n, 1.42860682030941723e-6f, r);
+  float p = fmaf(r, 0.000198412701f, 0.00138888892f);
+  p = fmaf(r, p, 0.00833333377f);
+  p = fmaf(r, p, 0.0416666679f);
+  p = fmaf(r, p, 0.166666672f);
+  p = fmaf(r, p, 0.5f);
+  p = fmaf(r, p, 1.0f);
+  p = fmaf(r, p, 1.0f);
   return ldexpf(p, (int)n);
 }
 
@@ -581,23 +589,16 @@ static inline float edge_rel(const F4& u, const F4& v) { return u.x * v.y - u.y 
 static inline bool inside_pos(float a, float b, float c) {
   return std::min(a, std::min(b, c)) >= 0.0f && std::max(a, std::max(b, c)) > 0.0f;
 }
-/* Depth value and scalar where the ray crosses face (A, B, C) with edge
- * functions eAB, eBC, eCA: ortho gives Z, perspective gives the depth 1/Z
- * and S/Z from one reciprocal of Z*D. */
-static inline void face_values(bool persp, float Az, float Aw, float Bz, float Bw, float Cz, float Cw,
-                               float eAB, float eBC, float eCA, float* zd, float* s) {
-  float D = eAB + eBC + eCA;
-  float nz = eCA * (Bz - Az) + eAB * (Cz - Az);
-  float ns = eCA * (Bw - Aw) + eAB * (Cw - Aw);
-  if (persp) {
-    float r = 1.0f / (Az * D + nz);
-    *zd = D * r;
-    *s = clamp01((Aw * D + ns) * r);
-  } else {
-    float inv = 1.0f / D;
-    *zd = Az + nz * inv;
-    *s = clamp01(Aw + ns * inv);
-  }
+/* Depth value and scalar of the point with weights wA..wE (>= 0, D their
+ * sum) over four vertices: ortho gives Z, perspective the depth 1/Z; both
+ * from one reciprocal. */
+static inline void face_values(bool persp, float D, float wA, const F4& A, float wB, const F4& B, float wC,
+                               const F4& C, float wE, const F4& E, float* zd, float* s) {
+  float zn = fmaf(wE, E.z, fmaf(wC, C.z, fmaf(wB, B.z, wA * A.z)));
+  float sn = fmaf(wE, E.w, fmaf(wC, C.w, fmaf(wB, B.w, wA * A.w)));
+  float rc = 1.0f / (persp ? zn : D);
+  *zd = (persp ? D : zn) * rc;
+  *s = clamp01(sn * rc);
 }
 static inline float pick3(bool c1, bool c2, float a, float b, float c) { return c1 ? a : (c2 ? b : c); }
 static inline u32 tf_index(float s) {
@@ -724,19 +725,19 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
             i2 = T[tcur * 4 + ((fent + 2) & 3)], i3 = T[tcur * 4 + ((fent + 3) & 3)];
         F4 P0 = rel(P[i0], ox, oy), P1 = rel(P[i1], ox, oy), P2 = rel(P[i2], ox, oy);
         float o01 = edge_rel(P0, P1), o12 = edge_rel(P1, P2), o20 = edge_rel(P2, P0);
-        float Dn = o01 + o12 + o20;
-        if (Dn == 0.0f) {
-          if (live32) anomalies++;
-          break;
-        }
-        if (Dn < 0.0f) {
+        if (o01 + o12 + o20 < 0.0f) {
           std::swap(P1, P2);
           std::swap(i1, i2);
           float n01 = -o20, n12 = -o12, n20 = -o01;
           o01 = n01; o12 = n12; o20 = n20;
         }
+        float Dn = (o12 + o20) + o01;
+        if (!(Dn > 0.0f)) {
+          if (live32) anomalies++;
+          break;
+        }
         float prev, sprev;
-        face_values(persp, P0.z, P0.w, P1.z, P1.w, P2.z, P2.w, o01, o12, o20, &prev, &sprev);
+        face_values(persp, Dn, o12, P0, o20, P1, o01, P2, 0.0f, P2, &prev, &sprev);
         if (live32) cP = ld4(tf, tf_index(sprev));
         /* FP64 entry */
         D4 a64 = xform64(view, ld4(V, i0));
@@ -767,10 +768,14 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
             c1 = p1 && z1 <= z2 && z1 <= z3;
             c2 = !c1 && p2 && z2 <= z3;
           }
+          /* exit face (P3, Pi, Pj): P3 weighs o_ij, Pi -t_j, Pj t_i, Pk 0 */
+          float w3 = pick3(c1, c2, o01, o12, o20);
+          float w0 = c1 ? -t1 : (c2 ? 0.0f : t2);
+          float w1 = c1 ? t0 : (c2 ? -t2 : 0.0f);
+          float w2 = c1 ? 0.0f : (c2 ? t1 : -t0);
+          float Dx = ((w3 + w0) + w1) + w2;
           float zd, snext;
-          face_values(persp, P3.z, P3.w, pick3(c1, c2, P0.z, P1.z, P2.z), pick3(c1, c2, P0.w, P1.w, P2.w),
-                      pick3(c1, c2, P1.z, P2.z, P0.z), pick3(c1, c2, P1.w, P2.w, P0.w), pick3(c1, c2, t0, t1, t2),
-                      pick3(c1, c2, o01, o12, o20), -pick3(c1, c2, t1, t2, t0), &zd, &snext);
+          face_values(persp, Dx, w3, P3, w0, P0, w1, P1, w2, P2, &zd, &snext);
           bool k0 = c2, k1 = !c1 && !c2, k2 = c1;
           u32 ib = c1 ? i0 : (c2 ? i1 : i2), ic = c1 ? i1 : (c2 ? i2 : i0);
           u32 ik = c1 ? i2 : (c2 ? i0 : i1);
@@ -778,19 +783,21 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
           while (sk < 3 && T[tcur * 4 + sk] != ik) ++sk;
           u32 fex = (sk + 1) & 3;
           if (live32) {
-            float dl = fabsf(zd - prev) * rf;
+            float dl = std::fmax(fabsf(zd - prev) * rf, 0.0f);
             prev = zd;
             F4 cN = ld4(tf, tf_index(snext));
             float tf_ = cP.w * dl, tb_ = cN.w * dl;
+            /* 512 * gamma for both ends with one division */
             float uf = 1.0f + tf_, ub = 1.0f + tb_;
-            float q = 1.0f / (uf * ub);
+            float q = 512.0f / (uf * ub);
             float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
-            float ps = psi[psi_index(gf) * 512 + psi_index(gb)];
+            u32 jf = (u32)(gf + 0.5f), jb = (u32)(gb + 0.5f);
+            float ps = psi[std::min(jf, 511u) * 512 + std::min(jb, 511u)];
             float zeta = orc_expf(-(tf_ + tb_) * 0.5f);
             float w1 = 1.0f - ps, w2 = ps - zeta;
-            ar = ar + Tr * (cP.x * w1 + cN.x * w2);
-            ag = ag + Tr * (cP.y * w1 + cN.y * w2);
-            ab = ab + Tr * (cP.z * w1 + cN.z * w2);
+            ar = fmaf(Tr, fmaf(cP.x, w1, cN.x * w2), ar);
+            ag = fmaf(Tr, fmaf(cP.y, w1, cN.y * w2), ag);
+            ab = fmaf(Tr, fmaf(cP.z, w1, cN.z * w2), ab);
             Tr = Tr * zeta;
             cP = cN;
             if (1.0f - Tr >= term) live32 = false;

@@ -135,8 +135,9 @@ def test_batched_equals_per_view(ctx):
     print("batched vs per-view 24^3", flush=True)
     a, sa = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
     b, sb = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
-    c, sc = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, simple_kernel=True)
-    d, sd = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, waves_per_cu=4, refill=1)
+    c, sc = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, persistent=True)
+    d, sd = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, persistent=True, waves_per_cu=4,
+                       refill=1)
     assert sa["steps"] == sb["steps"] == sc["steps"] == sd["steps"]
     for i, v in enumerate(views):
         alone, _ = ctx.render(m, [v], mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
@@ -205,6 +206,7 @@ def test_autotune_keeps_results(ctx):
     p = ctx.plan(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
     choice = p.autotune()
     assert 1 <= choice["waves_per_cu"] <= 64 and 1 <= choice["refill"] <= 64
+    assert isinstance(choice["persistent"], bool)
     p.execute()
     for i in range(len(views)):
         assert np.array_equal(p.read(i)["rgba"], ref[i]["rgba"])

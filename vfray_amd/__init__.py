@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libvfr.so")
 VFR_OK = 0
 PROJ_ORTHO, PROJ_PERSPECTIVE = 0, 1
 MODE_REF, MODE_FTB = 0, 1
+RENDER_PERSISTENT = 0x1
 SCHED_PER_VIEW, SCHED_BATCHED = 0, 1
 MESH_NORMALIZE, MESH_MORTON = 0x1, 0x2
 NO_ID = 0xFFFFFFFF
@@ -59,8 +60,8 @@ class Stats(C.Structure):
 
 class ServerCfg(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("streams", C.c_uint32), ("max_batch", C.c_uint32),
-                ("term_alpha", C.c_float), ("occupancy", C.c_uint32), ("waves_per_cu", C.c_uint32),
-                ("refill", C.c_uint32)]
+                ("term_alpha", C.c_float), ("flags", C.c_uint32), ("occupancy", C.c_uint32),
+                ("waves_per_cu", C.c_uint32), ("refill", C.c_uint32)]
 
 
 class Request(C.Structure):
@@ -122,7 +123,7 @@ def lib() -> C.CDLL:
             "vfr_plan_destroy": (None, [P]),
             "vfr_plan_frame_times": (C.c_int, [P, C.POINTER(C.c_double)]),
             "vfr_plan_autotune": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
-                                            C.POINTER(C.c_double)]),
+                                            C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
             "vfr_render": (C.c_int, [P, P, C.POINTER(View), C.c_int, C.POINTER(RenderOpts), C.POINTER(FrameOut), C.POINTER(Stats)]),
             "vfr_free": (None, [P]),
             "vfr_server_create": (C.c_int, [P, C.POINTER(ServerCfg), C.POINTER(P), C.c_int, C.POINTER(P)]),
@@ -343,9 +344,9 @@ class Context:
         return Mesh(self, xyzs, tets, con, extf, lmt, normalize, morton)
 
     def plan(self, mesh: "Mesh", views, mode=MODE_FTB, schedule=SCHED_BATCHED, term_alpha=0.99,
-             use_graph=False, waves_per_cu=0, refill=0, simple_kernel=False, occupancy=0) -> "Plan":
+             use_graph=False, waves_per_cu=0, refill=0, persistent=False, occupancy=0) -> "Plan":
         return Plan(self, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
-                    simple_kernel, occupancy)
+                    persistent, occupancy)
 
     def render(self, mesh: "Mesh", views, **kw) -> tuple[list[dict], dict]:
         p = self.plan(mesh, views, **kw)
@@ -393,19 +394,20 @@ class Mesh:
 
 class Plan:
     def __init__(self, ctx, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
-                 simple_kernel=False, occupancy=0):
+                 persistent=False, occupancy=0):
         self.ctx, self.mesh = ctx, mesh
         self.views = list(views)
         arr = (View * len(self.views))(*self.views)
         o = RenderOpts(mode, schedule, term_alpha, waves_per_cu, refill, 1 if use_graph else 0, 0,
-                       1 if simple_kernel else 0, occupancy, 0)
+                       RENDER_PERSISTENT if persistent else 0, occupancy, 0)
         self._p = C.c_void_p()
         _check(lib().vfr_plan_create(ctx._p, mesh._p, arr, len(self.views), C.byref(o), C.byref(self._p)))
 
     def autotune(self) -> dict:
-        o, w, r, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_double()
-        _check(lib().vfr_plan_autotune(self._p, C.byref(o), C.byref(w), C.byref(r), C.byref(ms)))
-        return {"occupancy": o.value, "waves_per_cu": w.value, "refill": r.value, "traversal_ms": ms.value}
+        f, o, w, r, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_double()
+        _check(lib().vfr_plan_autotune(self._p, C.byref(f), C.byref(o), C.byref(w), C.byref(r), C.byref(ms)))
+        return {"persistent": bool(f.value & RENDER_PERSISTENT), "occupancy": o.value, "waves_per_cu": w.value,
+                "refill": r.value, "traversal_ms": ms.value}
 
     def execute(self) -> dict:
         st = Stats()
@@ -447,11 +449,13 @@ class Server:
     """Resident meshes + continuous batching (vfr_server_*)."""
 
     def __init__(self, ctx: Context, meshes, hbm_budget: int = 0, streams: int = 0, max_batch: int = 0,
-                 term_alpha: float = 0.0, occupancy: int = 0, waves_per_cu: int = 0, refill: int = 0):
+                 term_alpha: float = 0.0, persistent: bool = False, occupancy: int = 0, waves_per_cu: int = 0,
+                 refill: int = 0):
         self.ctx = ctx
         self.meshes = list(meshes)
         arr = (C.c_void_p * len(self.meshes))(*[m._p.value for m in self.meshes])
-        cfg = ServerCfg(hbm_budget, streams, max_batch, term_alpha, occupancy, waves_per_cu, refill)
+        cfg = ServerCfg(hbm_budget, streams, max_batch, term_alpha, RENDER_PERSISTENT if persistent else 0,
+                        occupancy, waves_per_cu, refill)
         self._p = C.c_void_p()
         _check(lib().vfr_server_create(ctx._p, C.byref(cfg), arr, len(self.meshes), C.byref(self._p)))
 

@@ -433,11 +433,11 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   return a;
 }
 
-/* The batched traversal: persistent work-queue kernel (FTB) or one ray per
- * thread (REF, or when asked). */
+/* The batched traversal: one ray per thread over the rays of all views, or
+ * the persistent work-queue kernel (FTB, when asked). */
 static int launch_batched_traverse(vfr_plan* p) {
   TravArgs a = trav_args(p, p->rays, p->counters + p->n_views);
-  const bool persist = p->opts.mode == VFR_MODE_FTB && !(p->opts.flags & VFR_RENDER_SIMPLE_KERNEL);
+  const bool persist = p->opts.mode == VFR_MODE_FTB && (p->opts.flags & VFR_RENDER_PERSISTENT);
   const uint32_t waves = p->opts.waves_per_cu * (uint32_t)p->ctx->n_cu;
   return vfr_k_traverse(p->opts.mode, p->opts.occupancy, a, (uint32_t)p->n_pix, persist ? 1 : 0, waves,
                         p->ctx->stream);
@@ -712,13 +712,13 @@ struct TuneKey {
   bool operator==(const TuneKey& o) const { return n_tets == o.n_tets && n_pix == o.n_pix && n_views == o.n_views; }
 };
 struct TuneVal {
-  uint32_t occ, wpc, refill;
+  uint32_t flags, occ, wpc, refill;
 };
 static std::mutex g_tune_mu;
 static std::vector<std::pair<TuneKey, TuneVal>> g_tune;
 
-extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* occ_out, uint32_t* wpc_out, uint32_t* refill_out,
-                                 double* best_out) {
+extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ_out, uint32_t* wpc_out,
+                                 uint32_t* refill_out, double* best_out) {
   if (!p) { vfr_set_error("vfr_plan_autotune: null plan"); return VFR_EINVAL; }
   if (p->opts.schedule != VFR_SCHED_BATCHED || p->opts.mode != VFR_MODE_FTB) {
     vfr_set_error("autotune applies to batched front-to-back plans");
@@ -731,6 +731,7 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* occ_out, uint32_t* wpc_o
     std::lock_guard<std::mutex> g(g_tune_mu);
     for (auto& e : g_tune)
       if (e.first == key) {
+        p->opts.flags = (p->opts.flags & ~VFR_RENDER_PERSISTENT) | e.second.flags;
         p->opts.occupancy = e.second.occ;
         p->opts.waves_per_cu = e.second.wpc;
         p->opts.refill = e.second.refill;
@@ -739,41 +740,53 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* occ_out, uint32_t* wpc_o
   }
   double best = -1.0;
   if (!cached) {
-    /* occupancy variant x resident waves per CU (as multiples of the
-     * variant's 4-SIMD capacity) x refill threshold */
-    static const uint32_t kOcc[] = {0, 4, 5, 6};
+    /* the one-ray-per-thread kernel, then persistent schedules: occupancy
+     * variant x resident waves per CU (multiples of the variant's 4-SIMD
+     * capacity) x refill threshold */
+    static const uint32_t kOcc[] = {0, 6};
     static const uint32_t kMul[] = {1, 2};
-    static const uint32_t kR[] = {8, 16, 32, 48, 64};
-    const uint32_t saved_flags = p->opts.flags;
-    p->opts.flags &= ~VFR_RENDER_SIMPLE_KERNEL;
+    static const uint32_t kR[] = {16, 64};
+    const uint32_t base_flags = p->opts.flags & ~VFR_RENDER_PERSISTENT;
     best = 1e30;
-    TuneVal bv{p->opts.occupancy, p->opts.waves_per_cu, p->opts.refill};
+    TuneVal bv{0u, p->opts.occupancy, p->opts.waves_per_cu, p->opts.refill};
     vfr_stats st;
-    CHECK_RC(vfr_plan_execute(p, &st)); /* warm caches */
+    auto time_it = [&](double* t) -> int {
+      *t = 1e30;
+      for (int rep = 0; rep < 2; ++rep) {
+        int rc = vfr_plan_execute(p, &st);
+        if (rc != VFR_OK) return rc;
+        *t = std::min(*t, st.ms_traversal);
+      }
+      return VFR_OK;
+    };
+    p->opts.flags = base_flags;
+    int rc = vfr_plan_execute(p, &st); /* warm caches */
+    double t = 0;
+    if (rc == VFR_OK) rc = time_it(&t);
+    if (rc == VFR_OK && t < best) { best = t; bv = TuneVal{0u, p->opts.occupancy, p->opts.waves_per_cu, p->opts.refill}; }
+    p->opts.flags = base_flags | VFR_RENDER_PERSISTENT;
     for (uint32_t occ : kOcc) {
       uint32_t wps = 0;
-      CHECK_RC(vfr_k_traverse_waves_per_simd(occ, &wps));
+      if (rc == VFR_OK) rc = vfr_k_traverse_waves_per_simd(occ, &wps);
       for (uint32_t mul : kMul)
         for (uint32_t r : kR) {
+          if (rc != VFR_OK) break;
           p->opts.occupancy = occ;
           p->opts.waves_per_cu = 4 * wps * mul;
           p->opts.refill = r;
-          double t = 1e30;
-          for (int rep = 0; rep < 2; ++rep) {
-            int rc = vfr_plan_execute(p, &st);
-            if (rc != VFR_OK) { p->opts.flags = saved_flags; return rc; }
-            t = std::min(t, st.ms_traversal);
-          }
-          if (t < best) { best = t; bv = TuneVal{occ, 4 * wps * mul, r}; }
+          rc = time_it(&t);
+          if (rc == VFR_OK && t < best) { best = t; bv = TuneVal{VFR_RENDER_PERSISTENT, occ, 4 * wps * mul, r}; }
         }
     }
+    p->opts.flags = base_flags | bv.flags;
     p->opts.occupancy = bv.occ;
     p->opts.waves_per_cu = bv.wpc;
     p->opts.refill = bv.refill;
-    p->opts.flags = saved_flags;
+    if (rc != VFR_OK) return rc;
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tune.push_back({key, bv});
   }
+  if (flags_out) *flags_out = p->opts.flags & VFR_RENDER_PERSISTENT;
   if (occ_out) *occ_out = p->opts.occupancy;
   if (wpc_out) *wpc_out = p->opts.waves_per_cu;
   if (refill_out) *refill_out = p->opts.refill;

@@ -222,7 +222,8 @@ DEV float bary(float A, float B, float C, const Hit& h, float invD) {
   return A + ((wb * (B - A) + wc * (C - A)) * invD);
 }
 
-DEV float clamp01(float s) { return s < 0.0f ? 0.0f : (s > 1.0f ? 1.0f : s); }
+/* [0,1] clamp; NaN maps to 0 (fmaxf returns the other operand). */
+DEV float clamp01(float s) { return fminf(fmaxf(s, 0.0f), 1.0f); }
 
 DEV uint32_t fkey(float f) {
   uint32_t u = __float_as_uint(f);
@@ -233,14 +234,21 @@ DEV float funkey(uint32_t k) {
   return __uint_as_float(u);
 }
 
-/* exp(x), x <= 0: Cody-Waite reduction with a two-part ln2 and a degree-7
- * Taylor polynomial on |r| <= ln2/2, scaled by 2^n. */
+/* exp(x), x <= 0 (clamped at -87): Cody-Waite reduction with a two-part ln2
+ * and a degree-7 Taylor polynomial on |r| <= ln2/2 in Horner form with fused
+ * multiply-adds, scaled by 2^n. */
 DEV float vfr_expf(float x) {
-  if (x < -87.0f) return 0.0f;
-  float n = __builtin_rintf(x * 1.44269504f);
-  float r = (x - n * 0.693145751953125f) - n * 1.42860682030941723e-6f;
-  float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.166666672f + r * (0.0416666679f +
-            r * (0.00833333377f + r * (0.00138888892f + r * 0.000198412701f))))));
+  x = fmaxf(x, -87.0f);
+  const float n = __builtin_rintf(x * 1.44269504f);
+  float r = __builtin_fmaf(-n, 0.693145751953125f, x);
+  r = __builtin_fmaf(-n, 1.42860682030941723e-6f, r);
+  float p = __builtin_fmaf(r, 0.000198412701f, 0.00138888892f);
+  p = __builtin_fmaf(r, p, 0.00833333377f);
+  p = __builtin_fmaf(r, p, 0.0416666679f);
+  p = __builtin_fmaf(r, p, 0.166666672f);
+  p = __builtin_fmaf(r, p, 0.5f);
+  p = __builtin_fmaf(r, p, 1.0f);
+  p = __builtin_fmaf(r, p, 1.0f);
   return __builtin_ldexpf(p, (int)n);
 }
 
@@ -686,7 +694,8 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
  * orientation, so an exit face is one whose three edge functions are >= 0,
  * not all zero.  A step computes the three edges to the tet's fourth vertex;
  * the entry face's edges are carried, and the vertex the exit face leaves
- * out is replaced in place.
+ * out is replaced in place.  Depth and scalar at the exit point are the
+ * edge-function-weighted average of the face's vertices (all weights >= 0).
  *
  * Loads are issued one tet ahead: once the exit face is known, its link
  * (neighbour word and the neighbour's vertex off the shared face), then the
@@ -723,27 +732,20 @@ DEV uint32_t pick3u(bool c1, bool c2, uint32_t a, uint32_t b, uint32_t c) { retu
 
 DEV float4 rel(float4 p, float ox, float oy) { return make_float4(p.x - ox, p.y - oy, p.z, p.w); }
 
-/* Depth and scalar where the ray crosses face (A, B, C) with edge functions
- * eAB, eBC, eCA (barycentric weights eBC, eCA, eAB over their sum): ortho
- * returns Z; perspective returns the depth 1/Z from one reciprocal of Z*D,
- * and the scalar S/Z from the same reciprocal. */
-DEV void face_values(bool persp, float Az, float Aw, float Bz, float Bw, float Cz, float Cw, float eAB,
-                     float eBC, float eCA, float* zd, float* s) {
-  const float D = eAB + eBC + eCA;
-  const float nz = eCA * (Bz - Az) + eAB * (Cz - Az);
-  const float ns = eCA * (Bw - Aw) + eAB * (Cw - Aw);
-  if (persp) {
-    const float r = 1.0f / (Az * D + nz);
-    *zd = D * r;
-    *s = clamp01((Aw * D + ns) * r);
-  } else {
-    const float inv = 1.0f / D;
-    *zd = Az + nz * inv;
-    *s = clamp01(Aw + ns * inv);
-  }
+/* Depth value and scalar of the point with weights wA..wD (>= 0, D their
+ * sum) over four vertices: ortho returns Z, perspective the depth 1/Z; both
+ * from one reciprocal. */
+DEV void face_values(bool persp, float D, float wA, float4 A, float wB, float4 B, float wC, float4 C,
+                     float wE, float4 E, float* zd, float* s) {
+  const float zn = __builtin_fmaf(wE, E.z, __builtin_fmaf(wC, C.z, __builtin_fmaf(wB, B.z, wA * A.z)));
+  const float sn = __builtin_fmaf(wE, E.w, __builtin_fmaf(wC, C.w, __builtin_fmaf(wB, B.w, wA * A.w)));
+  const float rc = 1.0f / (persp ? zn : D);
+  *zd = (persp ? D : zn) * rc;
+  *s = clamp01(sn * rc);
 }
 
-/* Starts the walk at entry `cur` of the pixel's sorted list. */
+/* Starts the walk at entry `cur` of the pixel's sorted list; false if the
+ * entry face is degenerate at the pixel. */
 DEV bool ftb_segment(const TravArgs& a, FtbRay& r) {
   const uint64_t key = a.entries[(size_t)r.gp * VFR_K + ftb_cur(r)];
   const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull) + r.eoff;
@@ -756,9 +758,7 @@ DEV bool ftb_segment(const TravArgs& a, FtbRay& r) {
   r.Q = pv[r.i3];
   float4 P0 = rel(pv[i0], r.ox, r.oy), P1 = rel(pv[i1], r.ox, r.oy), P2 = rel(pv[i2], r.ox, r.oy);
   float o01 = edge_rel(P0, P1), o12 = edge_rel(P1, P2), o20 = edge_rel(P2, P0);
-  const float D = o01 + o12 + o20;
-  if (D == 0.0f) return false;
-  if (D < 0.0f) {
+  if (o01 + o12 + o20 < 0.0f) {
     /* order (P0, P2, P1): every edge function flips sign exactly */
     const float4 tP = P1; P1 = P2; P2 = tP;
     const uint32_t ti = i1; i1 = i2; i2 = ti;
@@ -768,10 +768,12 @@ DEV bool ftb_segment(const TravArgs& a, FtbRay& r) {
   r.P0 = P0; r.P1 = P1; r.P2 = P2;
   r.i0 = i0; r.i1 = i1; r.i2 = i2;
   r.o01 = o01; r.o12 = o12; r.o20 = o20;
+  const float D = (o12 + o20) + o01;
   float sprev;
-  face_values(ftb_persp(r), P0.z, P0.w, P1.z, P1.w, P2.z, P2.w, o01, o12, o20, &r.prev, &sprev);
+  /* P0 weighs o12, P1 o20, P2 o01 */
+  face_values(ftb_persp(r), D, o12, P0, o20, P1, o01, P2, 0.0f, P2, &r.prev, &sprev);
   r.cP = a.tf[r.tfo + min(255u, (uint32_t)(255.0f * sprev))];
-  return true;
+  return D > 0.0f;
 }
 
 DEV void ftb_init(const TravArgs& a, uint32_t vi, uint32_t pix, FtbRay& r) {
@@ -796,22 +798,29 @@ DEV void ftb_init(const TravArgs& a, uint32_t vi, uint32_t pix, FtbRay& r) {
   if (!ftb_segment(a, r)) r.bits |= 512u;
 }
 
-/* One tet of the walk; returns true when the ray is finished.
+enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
+
+/* One tet of the walk: FTB_GO, FTB_DONE (the ray is finished) or
+ * FTB_REENTER (it left through an external face; ftb_reenter decides).
  *
- * The body is one straight path (the ray state is updated in place on every
- * path, so no copies of it are needed where paths join), and its loads go
- * out in two rounds: the exit face's link and the transfer-function colour,
- * then the psi entry, the next tet's vertex ids and its fourth vertex.  The
- * psi lookup is issued first in the second round because memory results
- * return in issue order and it is the one waited for within the step. */
-DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
-  if ((r.bits & 512u) || ++r.steps > VFR_MAX_STEPS) { r.bits |= 512u; return true; }
+ * The body is one straight path that updates the ray state in place, so no
+ * copies of the state are needed where paths join, and it is safe to run on
+ * a finished ray (every index it forms stays inside the mesh arrays; the
+ * result is ignored).  Its loads go out in two rounds: the exit face's link
+ * and the transfer-function colour, then the psi entry, the next tet's
+ * fourth vertex and vertex ids.  The psi lookup goes first in the second
+ * round because memory results return in issue order and it is the one
+ * waited for within the step. */
+DEV uint32_t ftb_step(const TravArgs& a, FtbRay& r) {
+  const bool dead = (r.bits & 512u) != 0u;
+  r.steps += dead ? 0u : 1u;
   const float4 P3 = rel(r.Q, r.ox, r.oy);
   const float t0 = edge_rel(P3, r.P0), t1 = edge_rel(P3, r.P1), t2 = edge_rel(P3, r.P2);
   /* candidate faces (P3, Pi, Pj) for (i,j) = (0,1), (1,2), (2,0): edges
    * P3->Pi, Pi->Pj, Pj->P3 */
   const bool p1 = inside_pos(t0, r.o01, -t1), p2 = inside_pos(t1, r.o12, -t2), p3 = inside_pos(t2, r.o20, -t0);
-  const bool none = !(p1 || p2 || p3);
+  /* no exit face, a dead ray or the step bound: the step is not composited */
+  const bool skip = dead || r.steps > VFR_MAX_STEPS || !(p1 || p2 || p3);
   bool c1 = p1, c2 = !p1 && p2;
   if ((int)p1 + (int)p2 + (int)p3 > 1) {
     /* the pixel is on a projected edge: exit at the point farthest along the
@@ -823,16 +832,15 @@ DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
     c1 = p1 && z1 <= z2 && z1 <= z3;
     c2 = !c1 && p2 && z2 <= z3;
   }
-  /* exit face (P3, Pi, Pj): eAB = t_i, eBC = o_ij, eCA = -t_j (values are
-   * picked, never addresses, so the ray state stays in registers).  With no
-   * exit face the ray ends as an anomaly; the step still runs on in-range
-   * indices and its segment is not composited. */
+  /* exit face (P3, Pi, Pj): P3 weighs o_ij, Pi weighs -t_j, Pj weighs t_i,
+   * the left-out vertex Pk weighs 0 */
+  const float w3 = pick3(c1, c2, r.o01, r.o12, r.o20);
+  const float w0 = c1 ? -t1 : (c2 ? 0.0f : t2);
+  const float w1 = c1 ? t0 : (c2 ? -t2 : 0.0f);
+  const float w2 = c1 ? 0.0f : (c2 ? t1 : -t0);
+  const float D = ((w3 + w0) + w1) + w2;
   float zd, snext;
-  face_values(ftb_persp(r), P3.z, P3.w, pick3(c1, c2, r.P0.z, r.P1.z, r.P2.z),
-              pick3(c1, c2, r.P0.w, r.P1.w, r.P2.w), pick3(c1, c2, r.P1.z, r.P2.z, r.P0.z),
-              pick3(c1, c2, r.P1.w, r.P2.w, r.P0.w), pick3(c1, c2, t0, t1, t2),
-              pick3(c1, c2, r.o01, r.o12, r.o20), -pick3(c1, c2, t1, t2, t0), &zd, &snext);
-  snext = none ? 0.0f : snext;
+  face_values(ftb_persp(r), D, w3, P3, w0, r.P0, w1, r.P1, w2, r.P2, &zd, &snext);
   /* the exit face leaves out vertex k; its face index is slot(k) + 1 */
   const bool k0 = c2, k1 = !c1 && !c2, k2 = c1;
   const uint32_t ik = pick3u(c1, c2, r.i2, r.i0, r.i1);
@@ -844,22 +852,20 @@ DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
   __builtin_amdgcn_sched_barrier(0);
   /* round 2: psi, then the next tet (the current one again at a boundary,
    * unused) */
-  float dl = fabsf(zd - r.prev) * r.rf;
-  dl = none ? 0.0f : dl;
+  const float dl = fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
   const float tf_ = r.cP.w * dl, tb_ = cN.w * dl;
-  /* gamma = tau / (1 + tau) for both ends with one division */
+  /* 512 * gamma = 512 * tau / (1 + tau) for both ends with one division */
   const float uf = 1.0f + tf_, ub = 1.0f + tb_;
-  const float q = 1.0f / (uf * ub);
+  const float q = 512.0f / (uf * ub);
   const float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
-  const float ps = a.psi[psi_index(gf) * 512u + psi_index(gb)];
+  const float ps = a.psi[min(511u, (uint32_t)(gf + 0.5f)) * 512u + min(511u, (uint32_t)(gb + 0.5f))];
   const uint32_t nb = ln.x;
   const uint32_t tn = nb < VFR_NB_BROKEN ? (nb >> 2) : r.t;
   const float4 Qn = a.vert[r.voff + ln.y];
   const uint4 rvn = *reinterpret_cast<const uint4*>(a.rec[r.roff + tn].v);
   __builtin_amdgcn_sched_barrier(0);
   const float zeta = vfr_expf(-(tf_ + tb_) * 0.5f);
-  /* the exit face becomes the entry face: vertex k is replaced by P3 (a ray
-   * that ends here never reads it) */
+  /* the exit face becomes the entry face: vertex k is replaced by P3 */
   const float n01 = pick3(k0, k1, t1, -t0, r.o01);
   const float n12 = pick3(k1, k2, t2, -t1, r.o12);
   const float n20 = pick3(k2, k0, t0, -t2, r.o20);
@@ -871,35 +877,39 @@ DEV bool ftb_step(const TravArgs& a, FtbRay& r) {
   r.i2 = k2 ? r.i3 : r.i2;
   r.o01 = n01; r.o12 = n12; r.o20 = n20;
   /* composite the segment */
-  const float w1 = 1.0f - ps, w2 = ps - zeta;
-  const float ar = r.ar + r.Tr * (r.cP.x * w1 + cN.x * w2);
-  const float ag = r.ag + r.Tr * (r.cP.y * w1 + cN.y * w2);
-  const float ab = r.ab + r.Tr * (r.cP.z * w1 + cN.z * w2);
+  const float w1c = 1.0f - ps, w2c = ps - zeta;
+  const float ar = __builtin_fmaf(r.Tr, __builtin_fmaf(r.cP.x, w1c, cN.x * w2c), r.ar);
+  const float ag = __builtin_fmaf(r.Tr, __builtin_fmaf(r.cP.y, w1c, cN.y * w2c), r.ag);
+  const float ab = __builtin_fmaf(r.Tr, __builtin_fmaf(r.cP.z, w1c, cN.z * w2c), r.ab);
   const float Tr = r.Tr * zeta;
-  r.ar = none ? r.ar : ar;
-  r.ag = none ? r.ag : ag;
-  r.ab = none ? r.ab : ab;
-  r.Tr = none ? r.Tr : Tr;
+  r.ar = skip ? r.ar : ar;
+  r.ag = skip ? r.ag : ag;
+  r.ab = skip ? r.ab : ab;
+  r.Tr = skip ? r.Tr : Tr;
   r.cP = cN;
   r.prev = zd;
   r.t = tn;
   r.i3 = ln.y;
   r.Q = Qn;
   r.rv = rvn;
-  if (none) { r.bits |= 512u; return true; }
-  if (1.0f - r.Tr >= a.term) return true;
-  if (nb == VFR_NB_BROKEN) { r.bits |= 1024u; return true; }
-  if (nb == VFR_NB_EXT) {
-    /* re-entry: the first entry strictly behind the exit point */
-    const float exitZ = ftb_persp(r) ? 1.0f / zd : zd;
-    uint32_t cur = ftb_cur(r) + 1u;
-    const uint32_t nk = ftb_nk(r);
-    while (cur < nk && !(funkey((uint32_t)(a.entries[(size_t)r.gp * VFR_K + cur] >> 32)) < exitZ)) ++cur;
-    if (cur >= nk) return true;
-    r.bits = (r.bits & ~15u) | cur;
-    if (!ftb_segment(a, r)) { r.bits |= 512u; return true; }
-  }
-  return false;
+  r.bits |= (skip && !dead) ? 512u : 0u;
+  r.bits |= (!skip && 1.0f - Tr < a.term && nb == VFR_NB_BROKEN) ? 1024u : 0u;
+  if (skip || 1.0f - Tr >= a.term || nb == VFR_NB_BROKEN) return FTB_DONE;
+  return nb == VFR_NB_EXT ? FTB_REENTER : FTB_GO;
+}
+
+/* After an exit through an external face: continue at the first entry
+ * strictly behind the exit point; false when there is none (the ray is
+ * finished). */
+DEV bool ftb_reenter(const TravArgs& a, FtbRay& r) {
+  const float exitZ = ftb_persp(r) ? 1.0f / r.prev : r.prev;
+  uint32_t cur = ftb_cur(r) + 1u;
+  const uint32_t nk = ftb_nk(r);
+  while (cur < nk && !(funkey((uint32_t)(a.entries[(size_t)r.gp * VFR_K + cur] >> 32)) < exitZ)) ++cur;
+  if (cur >= nk) return false;
+  r.bits = (r.bits & ~15u) | cur;
+  if (!ftb_segment(a, r)) { r.bits |= 512u; return false; }
+  return true;
 }
 
 DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
@@ -915,7 +925,10 @@ DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
 DEV RayOut trace_ftb(const TravArgs& a, uint32_t vi, uint32_t pix) {
   FtbRay r;
   ftb_init(a, vi, pix, r);
-  while (!ftb_step(a, r)) {
+  while (true) {
+    const uint32_t st = ftb_step(a, r);
+    if (st == FTB_GO || (st == FTB_REENTER && ftb_reenter(a, r))) continue;
+    break;
   }
   return ftb_out(a, r);
 }
@@ -968,7 +981,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t n = *a.n_rays;
   const uint32_t refill = a.refill < 1u ? 1u : (a.refill > 64u ? 64u : a.refill);
+  if (n == 0) return;
+  /* every lane starts on a copy of the first ray, so that an idle lane's
+   * state is a valid one (its steps are never stored) */
   FtbRay r;
+  {
+    const uint64_t ray = a.rays[0];
+    ftb_init(a, (uint32_t)(ray >> 32), (uint32_t)ray, r);
+  }
   bool active = false, drained = false;
   uint32_t st = 0, an = 0, br = 0;
   while (true) {
@@ -993,8 +1013,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
       if (drained) break;
       continue;
     }
+    /* every lane steps (an idle lane's stale ray stays inside the mesh
+     * arrays and its result is ignored), so the walk is one straight path */
     for (uint32_t k = 0; k < burst; ++k) {
-      if (active && ftb_step(a, r)) {
+      const uint32_t s = ftb_step(a, r);
+      if (active && s != FTB_GO && !(s == FTB_REENTER && ftb_reenter(a, r))) {
         const RayOut o = ftb_out(a, r);
         store_ray(a, r.gp, o);
         st += o.steps; an += o.anomaly; br += o.broken;

@@ -210,7 +210,8 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   a.queue = s->counters + 1;
   a.term = sv->cfg.term_alpha;
   a.refill = sv->cfg.refill;
-  CHECK_RC(vfr_k_traverse(VFR_MODE_FTB, sv->cfg.occupancy, a, (uint32_t)pix, 1,
+  CHECK_RC(vfr_k_traverse(VFR_MODE_FTB, sv->cfg.occupancy, a, (uint32_t)pix,
+                          (sv->cfg.flags & VFR_RENDER_PERSISTENT) ? 1 : 0,
                           sv->cfg.waves_per_cu * (uint32_t)vfr_ctx_cus(sv->ctx), st));
   CHECK_HIP(hipMemcpyAsync(s->h_stats, s->stats, sizeof(unsigned long long) * ST_N, hipMemcpyDeviceToHost, st));
   CHECK_HIP(hipMemcpyAsync(s->h_counters, s->counters, 16, hipMemcpyDeviceToHost, st));
