@@ -571,11 +571,6 @@ extern "C" float orc_expf(float x) {
   return ldexpf(p, (int)n);
 }
 
-static inline u32 psi_index(float g) {
-  u32 i = (u32)(g * 512.0f + 0.5f);
-  return i > 511 ? 511 : i;
-}
-
 /* Vertex relative to the pixel (the renderer's walk works in these). */
 static inline F4 rel(const F4& p, float ox, float oy) {
   F4 r; r.x = p.x - ox; r.y = p.y - oy; r.z = p.z; r.w = p.w;

@@ -252,11 +252,6 @@ DEV float vfr_expf(float x) {
   return __builtin_ldexpf(p, (int)n);
 }
 
-DEV uint32_t psi_index(float g) {
-  uint32_t i = (uint32_t)(g * 512.0f + 0.5f);
-  return i > 511u ? 511u : i;
-}
-
 DEV uint32_t img_coord_clamped(float x, uint32_t res) {
   double f = ((double)res - 1.0) * ((double)x + 1.0) / 2.0;
   if (f <= 0.0) return 0u;
