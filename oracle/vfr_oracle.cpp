@@ -580,10 +580,6 @@ static inline F4 rel(const F4& p, float ox, float oy) {
  * u.x*v.y - u.y*v.x.  The same edge walked v -> u gives exactly the negated
  * float, so the faces behind an entry face cover it without gaps. */
 static inline float edge_rel(const F4& u, const F4& v) { return u.x * v.y - u.y * v.x; }
-/* Inside a positively oriented face with edge functions a, b, c. */
-static inline bool inside_pos(float a, float b, float c) {
-  return std::min(a, std::min(b, c)) >= 0.0f && std::max(a, std::max(b, c)) > 0.0f;
-}
 /* Depth value and scalar of the point with weights wA..wE (>= 0, D their
  * sum) over four vertices: ortho gives Z, perspective the depth 1/Z; both
  * from one reciprocal. */
@@ -751,8 +747,11 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
           if (++iter > 4 * MAX_STEPS) { anomaly = true; break; }
           F4 P3 = rel(P[i3], ox, oy);
           float t0 = edge_rel(P3, P0), t1 = edge_rel(P3, P1), t2 = edge_rel(P3, P2);
-          /* candidate faces (P3, Pi, Pj), (i,j) = (0,1), (1,2), (2,0) */
-          bool p1 = inside_pos(t0, o01, -t1), p2 = inside_pos(t1, o12, -t2), p3 = inside_pos(t2, o20, -t0);
+          /* candidate faces (P3, Pi, Pj), (i,j) = (0,1), (1,2), (2,0): an exit
+           * face when t_i >= 0, t_j <= 0 and not t_i = t_j = o_ij = 0 */
+          bool p1 = t0 >= 0.0f && t1 <= 0.0f && !(t0 == 0.0f && t1 == 0.0f && o01 == 0.0f);
+          bool p2 = t1 >= 0.0f && t2 <= 0.0f && !(t1 == 0.0f && t2 == 0.0f && o12 == 0.0f);
+          bool p3 = t2 >= 0.0f && t0 <= 0.0f && !(t2 == 0.0f && t0 == 0.0f && o20 == 0.0f);
           if (!(p1 || p2 || p3)) { anomaly = true; anom32 = live32; break; }
           bool c1 = p1, c2 = !p1 && p2;
           if ((int)p1 + (int)p2 + (int)p3 > 1) {

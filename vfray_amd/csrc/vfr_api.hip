@@ -186,8 +186,9 @@ extern "C" int vfr_mesh_create(vfr_ctx* c, const float* xyzs, uint32_t nV, const
     vfr_set_error("vfr_mesh_create: bad argument");
     return VFR_EINVAL;
   }
-  if ((uint64_t)nT * 4 >= (1ull << 31) || nT >= (1u << 30)) {
-    vfr_set_error("vfr_mesh_create: %u tets exceeds the 2^29 limit", nT);
+  if (nT >= VFR_MAX_TETS) {
+    /* neighbour words hold next << 2 */
+    vfr_set_error("vfr_mesh_create: %u tets exceeds the limit of %u", nT, VFR_MAX_TETS - 1u);
     return VFR_EINVAL;
   }
   if (extf && !con) { vfr_set_error("vfr_mesh_create: extf given without con"); return VFR_EINVAL; }
@@ -236,7 +237,7 @@ extern "C" int vfr_mesh_create(vfr_ctx* c, const float* xyzs, uint32_t nV, const
 extern "C" void vfr_mesh_destroy(vfr_mesh* m) {
   if (!m) return;
   if (m->ctx) (void)hipSetDevice(m->ctx->device);
-  void* ps[] = {m->d.rec, m->d.links, m->d.vert, m->d.ext_int, m->d.extf, m->d.perm, m->d.vperm, m->d.con};
+  void* ps[] = {m->d.rec, m->d.vert, m->d.ext_int, m->d.extf, m->d.perm, m->d.vperm, m->d.con};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete m;
@@ -287,6 +288,7 @@ struct vfr_plan {
   uint32_t n_views = 0;
   uint64_t n_pix = 0;
   uint32_t n_tiles = 0;
+  uint32_t trav_proj = VFR_TRAV_MIXED;    /* projections of the plan's views */
   DevView* views = nullptr;
   uint32_t* owner = nullptr;   /* REF owner or FTB count, [n_pix] */
   uint64_t* entries = nullptr; /* FTB, [n_pix * VFR_K] */
@@ -412,7 +414,6 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   TravArgs a;
   const vfr_mesh* m = p->mesh;
   a.rec = m->d.rec;
-  a.links = reinterpret_cast<const uint2*>(m->d.links);
   a.vert = p->pverts;
   a.ext_int = m->d.ext_int;
   a.tf = p->ctx->tf;
@@ -439,8 +440,8 @@ static int launch_batched_traverse(vfr_plan* p) {
   TravArgs a = trav_args(p, p->rays, p->counters + p->n_views);
   const bool persist = p->opts.mode == VFR_MODE_FTB && (p->opts.flags & VFR_RENDER_PERSISTENT);
   const uint32_t waves = p->opts.waves_per_cu * (uint32_t)p->ctx->n_cu;
-  return vfr_k_traverse(p->opts.mode, p->opts.occupancy, a, (uint32_t)p->n_pix, persist ? 1 : 0, waves,
-                        p->ctx->stream);
+  return vfr_k_traverse(p->opts.mode, p->trav_proj, p->mesh->d.n_tets, p->opts.occupancy, a, (uint32_t)p->n_pix,
+                        persist ? 1 : 0, waves, p->ctx->stream);
 }
 
 extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* views, int n_views,
@@ -502,8 +503,15 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   }
   p->n_pix = pix;
   p->n_tiles = (uint32_t)tiles;
-  if ((uint64_t)n_views * m->d.n_verts >= (1ull << 32) || (uint64_t)n_views * m->d.n_ext >= (1ull << 32)) {
-    vfr_set_error("plan needs more than 2^32 view vertices or faces");
+  {
+    uint32_t persp = 0;
+    for (int i = 0; i < n_views; ++i) persp += views[i].projection == VFR_PROJ_PERSPECTIVE ? 1u : 0u;
+    p->trav_proj = persp == 0 ? VFR_TRAV_ORTHO : (persp == (uint32_t)n_views ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
+  }
+  if ((uint64_t)n_views * m->d.n_verts >= VFR_MAX_VIEW_VERTS || (uint64_t)n_views * m->d.n_ext >= (1ull << 32)) {
+    /* P-space vertices are addressed with 32-bit byte offsets */
+    vfr_set_error("plan needs %llu view vertices (limit %llu): split the views over several plans",
+                  (unsigned long long)n_views * m->d.n_verts, (unsigned long long)VFR_MAX_VIEW_VERTS - 1);
     delete p;
     return VFR_EINVAL;
   }
@@ -615,7 +623,8 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
                              p->counters + v, st));
       CHECK_HIP(hipEventRecord(p->ev[1], st));
       TravArgs a = trav_args(p, p->rays + hv.pix_off, p->counters + v);
-      CHECK_RC(vfr_k_traverse(p->opts.mode, 0, a, npx, 0, 0, st));
+      CHECK_RC(vfr_k_traverse(p->opts.mode, hv.proj == VFR_PROJ_PERSPECTIVE ? VFR_TRAV_PERSP : VFR_TRAV_ORTHO,
+                              m->d.n_tets, 0, a, npx, 0, 0, st));
       CHECK_HIP(hipEventRecord(p->fev[2 * v + 1], st));
       CHECK_HIP(hipEventSynchronize(p->fev[2 * v + 1]));  /* a frame ends before the next starts */
       float f = 0, g = 0;

@@ -20,17 +20,33 @@ double vfr_psi_integral(double tf, double tb);
  * straight ray through a 400^3 grid crosses about 2,100 tets. */
 #define VFR_MAX_STEPS 8192u
 
+/* Neighbour words hold next << 2, so a mesh, or a server's resident meshes
+ * together, stay below VFR_MAX_TETS tets; the walk addresses P-space
+ * vertices (16 B each) with 32-bit byte offsets, so the vertices of all
+ * views of one launch stay below VFR_MAX_VIEW_VERTS. */
+#define VFR_MAX_TETS (1u << 29)
+/* Record arenas below 4 GiB (64 B per tet) are addressed with 32-bit byte
+ * offsets by the fastest traversal kernels. */
+#define VFR_BIG_RECORDS (1ull << 26)
+#define VFR_MAX_VIEW_VERTS (1ull << 28)
+
 /* Entry-list slots per pixel in front-to-back mode. */
 #define VFR_K 8u
 
 #define VFR_NO_ID 0xFFFFFFFFu
 
-/* 32-byte tet record: the 4 vertex ids and the 4 packed neighbour words,
- * read by one lane as two 16-byte loads of one 32-byte sector. */
-struct alignas(32) TetRec {
+/* 64-byte tet record, one per tet, 64-byte aligned: the 4 vertex ids, then
+ * per face f (the vertices of slots f, f+1, f+2) the packed neighbour word
+ * and the neighbour's vertex off that face (the tet's own opposite vertex
+ * when there is no linked neighbour).  A walk step reads the exit face's
+ * pair from the record it already holds in cache and the next tet's ids
+ * from one line. */
+struct alignas(64) TetRec {
   uint32_t v[4];
-  uint32_t n[4];
+  uint32_t f[8];   /* face k: f[2k] neighbour word, f[2k+1] fourth vertex */
+  uint32_t pad[4];
 };
+static_assert(sizeof(TetRec) == 64, "TetRec is one 64-byte line");
 
 /* One view of a render batch, with the offsets of its mesh in the device
  * arenas (a plan's single mesh, or a server's resident meshes). */
@@ -60,9 +76,6 @@ struct DevMesh {
   uint32_t n_verts = 0, n_tets = 0, n_ext = 0;
   float lmt[3] = {0, 0, 0};
   TetRec* rec = nullptr;       /* [n_tets] internal order                   */
-  uint32_t* links = nullptr;   /* [n_tets*4] (neighbour word, the neighbour's
-                                  vertex off the shared face) per face; the
-                                  tet's own opposite vertex when unlinked     */
   float* vert = nullptr;       /* [n_verts*4] internal order (x,y,z,s)      */
   uint32_t* ext_int = nullptr; /* [n_ext*2] (internal tet, face)            */
   uint32_t* extf = nullptr;    /* [n_ext*2] (caller tet, face), .extf order */
@@ -79,7 +92,6 @@ enum { ST_STEPS = 0, ST_ANOM = 1, ST_OVERFLOW = 2, ST_RAYS = 3, ST_N = 8 };
 #ifdef __HIPCC__
 struct TravArgs {
   const TetRec* rec;       /* record arena; a view's mesh starts at rec_off   */
-  const uint2* links;      /* [records * 4] (neighbour word, fourth vertex)   */
   const float4* vert;      /* P-space vertices of all views (+ view vert_off) */
   const uint32_t* ext_int; /* (internal tet, face) arena; + ext_off           */
   const float4* tf;        /* [slots * 256] */
@@ -118,8 +130,12 @@ int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, 
                   uint32_t* n_rays, void* stream);
 /* occupancy: waves per SIMD of the persistent kernel variant (4, 5, 6:
  * register-capped variants; anything else: as compiled). */
-int vfr_k_traverse(uint32_t mode, uint32_t occupancy, const TravArgs& a, uint32_t max_rays,
-                   int persistent, uint32_t grid_waves, void* stream);
+/* proj: VFR_TRAV_* (the projections of the views the rays belong to). */
+enum { VFR_TRAV_ORTHO = 0, VFR_TRAV_PERSP = 1, VFR_TRAV_MIXED = 2 };
+/* n_records: tet records in the arena the rays walk (from VFR_BIG_RECORDS on,
+ * the kernels address records with 64-bit arithmetic). */
+int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t occupancy, const TravArgs& a,
+                   uint32_t max_rays, int persistent, uint32_t grid_waves, void* stream);
 /* Waves per SIMD the persistent variant for `occupancy` reaches. */
 int vfr_k_traverse_waves_per_simd(uint32_t occupancy, uint32_t* out);
 

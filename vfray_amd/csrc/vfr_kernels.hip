@@ -559,7 +559,9 @@ DEV float4 sel4f(bool c, float4 a, float4 b) {
 DEV Step walk_step(const Walk& w, const TetRec* __restrict__ rec, const float4* __restrict__ pv,
                    float ox, float oy) {
   const uint4 rv = *reinterpret_cast<const uint4*>(rec[w.t].v);
-  const uint4 rn = *reinterpret_cast<const uint4*>(rec[w.t].n);
+  const uint4 f01 = *reinterpret_cast<const uint4*>(&rec[w.t].f[0]);
+  const uint4 f23 = *reinterpret_cast<const uint4*>(&rec[w.t].f[4]);
+  const uint4 rn = make_uint4(f01.x, f01.z, f23.x, f23.z);
   const uint32_t fe = w.fe;
   const uint32_t q0 = sel4(rv, fe), q1 = sel4(rv, (fe + 1u) & 3u), q2 = sel4(rv, (fe + 2u) & 3u),
                  q3 = sel4(rv, (fe + 3u) & 3u);
@@ -700,13 +702,15 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
 struct FtbRay {
   float4 P0, P1, P2;       /* entry face: (X - ox, Y - oy, Z, S) */
   float4 Q;                /* the tet's fourth vertex, P space (prefetched) */
-  uint4 rv;                /* the tet's vertex ids (prefetched) */
+  uint3 rv;                /* the tet's first three vertex ids (prefetched) */
+  uint4 fa, fb;            /* its face words f[0..7] (prefetched) */
   uint32_t i0, i1, i2, i3; /* vertex ids of P0, P1, P2, Q */
   float o01, o12, o20;     /* edge functions P0->P1, P1->P2, P2->P0 (>= 0) */
   float4 cP;               /* TF colour at the entry point */
   float ox, oy, rf, prev;  /* rf = ray length factor / max edge; prev = Z (ortho) or depth */
   float ar, ag, ab, Tr;
-  uint32_t t, voff, roff, eoff, gp, tfo, steps;
+  uint32_t t;              /* the tet, as an index into the record arena */
+  uint32_t voff, roff, eoff, gp, tfo, steps;
   uint32_t bits; /* cur 0..3 | nk 4..7 | persp 8 | anomaly 9 | broken 10 */
 };
 
@@ -717,9 +721,14 @@ DEV bool ftb_persp(const FtbRay& r) { return (r.bits >> 8) & 1u; }
 /* u -> v edge function of the pixel (vertices relative to the pixel). */
 DEV float edge_rel(float4 u, float4 v) { return u.x * v.y - u.y * v.x; }
 
-/* Inside a positively oriented face with edge functions a, b, c. */
-DEV bool inside_pos(float a, float b, float c) {
-  return fminf(a, fminf(b, c)) >= 0.0f && fmaxf(a, fmaxf(b, c)) > 0.0f;
+
+/* Load at a 32-bit byte offset from a kernel-argument base: one scalar base
+ * register pair plus one vector offset per lane, no 64-bit address math.
+ * Every array the walk reads this way stays below 4 GiB (checked when plans,
+ * meshes and servers are created). */
+template <class T>
+DEV T ld32(const void* base, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
 }
 
 DEV float pick3(bool c1, bool c2, float a, float b, float c) { return c1 ? a : (c2 ? b : c); }
@@ -745,11 +754,14 @@ DEV bool ftb_segment(const TravArgs& a, FtbRay& r) {
   const uint64_t key = a.entries[(size_t)r.gp * VFR_K + ftb_cur(r)];
   const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull) + r.eoff;
   const float4* __restrict__ pv = a.vert + r.voff;
-  r.t = a.ext_int[2 * (size_t)e];
+  r.t = r.roff + a.ext_int[2 * (size_t)e];
   const uint32_t fe = a.ext_int[2 * (size_t)e + 1];
-  r.rv = *reinterpret_cast<const uint4*>(a.rec[r.roff + r.t].v);
-  uint32_t i0 = sel4(r.rv, fe), i1 = sel4(r.rv, (fe + 1u) & 3u), i2 = sel4(r.rv, (fe + 2u) & 3u);
-  r.i3 = sel4(r.rv, (fe + 3u) & 3u);
+  const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[r.t].v);
+  r.rv = make_uint3(rv.x, rv.y, rv.z);
+  r.fa = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[0]);
+  r.fb = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[4]);
+  uint32_t i0 = sel4(rv, fe), i1 = sel4(rv, (fe + 1u) & 3u), i2 = sel4(rv, (fe + 2u) & 3u);
+  r.i3 = sel4(rv, (fe + 3u) & 3u);
   r.Q = pv[r.i3];
   float4 P0 = rel(pv[i0], r.ox, r.oy), P1 = rel(pv[i1], r.ox, r.oy), P2 = rel(pv[i2], r.ox, r.oy);
   float o01 = edge_rel(P0, P1), o12 = edge_rel(P1, P2), o20 = edge_rel(P2, P0);
@@ -795,8 +807,9 @@ DEV void ftb_init(const TravArgs& a, uint32_t vi, uint32_t pix, FtbRay& r) {
 
 enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
 
-/* One tet of the walk: FTB_GO, FTB_DONE (the ray is finished) or
- * FTB_REENTER (it left through an external face; ftb_reenter decides).
+/* One tet of the walk on a live ray: FTB_GO, FTB_DONE (the ray is
+ * finished) or FTB_REENTER (it left through an external face; ftb_reenter
+ * decides).
  *
  * The body is one straight path that updates the ray state in place, so no
  * copies of the state are needed where paths join, and it is safe to run on
@@ -806,18 +819,32 @@ enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
  * fourth vertex and vertex ids.  The psi lookup goes first in the second
  * round because memory results return in issue order and it is the one
  * waited for within the step. */
+/* PROJ: VFR_TRAV_*, the projection of every ray of the launch.  BIG: the
+ * record arena reaches 4 GiB (VFR_BIG_RECORDS tets or more), so records are
+ * addressed with 64-bit arithmetic instead of 32-bit byte offsets. */
+template <int PROJ, bool BIG>
 DEV uint32_t ftb_step(const TravArgs& a, FtbRay& r) {
-  const bool dead = (r.bits & 512u) != 0u;
-  r.steps += dead ? 0u : 1u;
+  const bool persp = PROJ == VFR_TRAV_MIXED ? ftb_persp(r) : PROJ == VFR_TRAV_PERSP;
+  r.steps += 1u;
   const float4 P3 = rel(r.Q, r.ox, r.oy);
   const float t0 = edge_rel(P3, r.P0), t1 = edge_rel(P3, r.P1), t2 = edge_rel(P3, r.P2);
-  /* candidate faces (P3, Pi, Pj) for (i,j) = (0,1), (1,2), (2,0): edges
-   * P3->Pi, Pi->Pj, Pj->P3 */
-  const bool p1 = inside_pos(t0, r.o01, -t1), p2 = inside_pos(t1, r.o12, -t2), p3 = inside_pos(t2, r.o20, -t0);
-  /* no exit face, a dead ray or the step bound: the step is not composited */
-  const bool skip = dead || r.steps > VFR_MAX_STEPS || !(p1 || p2 || p3);
+  /* candidate faces (P3, Pi, Pj) for (i,j) = (0,1), (1,2), (2,0), edges
+   * P3->Pi, Pi->Pj, Pj->P3: one is an exit face when its three edge
+   * functions are >= 0 and not all zero.  o_ij >= 0 already, so the test
+   * is t_i >= 0, t_j <= 0, and (rarely needed) not t_i = t_j = o_ij = 0. */
+  const bool ge0 = t0 >= 0.0f, ge1 = t1 >= 0.0f, ge2 = t2 >= 0.0f;
+  const bool le0 = t0 <= 0.0f, le1 = t1 <= 0.0f, le2 = t2 <= 0.0f;
+  bool p1 = ge0 && le1, p2 = ge1 && le2, p3 = ge2 && le0;
+  const bool z0 = ge0 && le0, z1 = ge1 && le1, z2 = ge2 && le2;
+  if ((z0 && z1) || (z1 && z2) || (z2 && z0)) {
+    p1 = p1 && !(z0 && z1 && r.o01 == 0.0f);
+    p2 = p2 && !(z1 && z2 && r.o12 == 0.0f);
+    p3 = p3 && !(z2 && z0 && r.o20 == 0.0f);
+  }
+  /* no exit face or the step bound: the step is not composited */
+  const bool skip = r.steps > VFR_MAX_STEPS || !(p1 || p2 || p3);
   bool c1 = p1, c2 = !p1 && p2;
-  if ((int)p1 + (int)p2 + (int)p3 > 1) {
+  if ((p1 && p2) || (p2 && p3) || (p3 && p1)) {
     /* the pixel is on a projected edge: exit at the point farthest along the
      * ray (smallest Z), ties in candidate order */
     const float inf = __uint_as_float(0x7f800000u);
@@ -835,18 +862,34 @@ DEV uint32_t ftb_step(const TravArgs& a, FtbRay& r) {
   const float w2 = c1 ? 0.0f : (c2 ? t1 : -t0);
   const float D = ((w3 + w0) + w1) + w2;
   float zd, snext;
-  face_values(ftb_persp(r), D, w3, P3, w0, r.P0, w1, r.P1, w2, r.P2, &zd, &snext);
+  face_values(persp, D, w3, P3, w0, r.P0, w1, r.P1, w2, r.P2, &zd, &snext);
   /* the exit face leaves out vertex k; its face index is slot(k) + 1 */
   const bool k0 = c2, k1 = !c1 && !c2, k2 = c1;
   const uint32_t ik = pick3u(c1, c2, r.i2, r.i0, r.i1);
-  const uint32_t sk = r.rv.x == ik ? 0u : (r.rv.y == ik ? 1u : (r.rv.z == ik ? 2u : 3u));
-  const uint32_t fx = (sk + 1u) & 3u;
-  /* round 1: the link of the exit face and the colour at the exit point */
-  const uint2 ln = a.links[(size_t)(r.roff + r.t) * 4u + fx];
-  const float4 cN = a.tf[r.tfo + min(255u, (uint32_t)(255.0f * snext))];
+  /* the exit face's link (neighbour word, fourth vertex) from the record
+   * words held in registers: slot 0 -> face 1, 1 -> 2, 2 -> 3, 3 -> 0 */
+  const bool s0 = r.rv.x == ik, s1 = r.rv.y == ik, s2 = r.rv.z == ik;
+  const uint4 fa = r.fa, fb = r.fb;
+  const uint32_t nb = pick3u(s0, s1, fa.z, fb.x, pick3u(s2, false, fb.z, 0u, fa.x));
+  const uint32_t i3n = pick3u(s0, s1, fa.w, fb.y, pick3u(s2, false, fb.w, 0u, fa.y));
+  const uint32_t tn = nb < VFR_NB_BROKEN ? r.roff + (nb >> 2) : r.t;
+  /* round 1: the colour at the exit point and the next tet (the current one
+   * again at a boundary, unused) */
+  const float4 cN = ld32<float4>(a.tf, (r.tfo + min(255u, (uint32_t)(255.0f * snext))) * 16u);
+  const float4 Qn = ld32<float4>(a.vert, (r.voff + i3n) * 16u);
+  uint3 rvn;
+  uint4 fan, fbn;
+  if (BIG) {
+    rvn = *reinterpret_cast<const uint3*>(a.rec[tn].v);
+    fan = *reinterpret_cast<const uint4*>(&a.rec[tn].f[0]);
+    fbn = *reinterpret_cast<const uint4*>(&a.rec[tn].f[4]);
+  } else {
+    rvn = ld32<uint3>(a.rec, tn * 64u);
+    fan = ld32<uint4>(a.rec, tn * 64u + 16u);
+    fbn = ld32<uint4>(a.rec, tn * 64u + 32u);
+  }
   __builtin_amdgcn_sched_barrier(0);
-  /* round 2: psi, then the next tet (the current one again at a boundary,
-   * unused) */
+  /* round 2: psi */
   const float dl = fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
   const float tf_ = r.cP.w * dl, tb_ = cN.w * dl;
   /* 512 * gamma = 512 * tau / (1 + tau) for both ends with one division */
@@ -854,10 +897,6 @@ DEV uint32_t ftb_step(const TravArgs& a, FtbRay& r) {
   const float q = 512.0f / (uf * ub);
   const float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
   const float ps = a.psi[min(511u, (uint32_t)(gf + 0.5f)) * 512u + min(511u, (uint32_t)(gb + 0.5f))];
-  const uint32_t nb = ln.x;
-  const uint32_t tn = nb < VFR_NB_BROKEN ? (nb >> 2) : r.t;
-  const float4 Qn = a.vert[r.voff + ln.y];
-  const uint4 rvn = *reinterpret_cast<const uint4*>(a.rec[r.roff + tn].v);
   __builtin_amdgcn_sched_barrier(0);
   const float zeta = vfr_expf(-(tf_ + tb_) * 0.5f);
   /* the exit face becomes the entry face: vertex k is replaced by P3 */
@@ -884,13 +923,20 @@ DEV uint32_t ftb_step(const TravArgs& a, FtbRay& r) {
   r.cP = cN;
   r.prev = zd;
   r.t = tn;
-  r.i3 = ln.y;
+  r.i3 = i3n;
   r.Q = Qn;
   r.rv = rvn;
-  r.bits |= (skip && !dead) ? 512u : 0u;
-  r.bits |= (!skip && 1.0f - Tr < a.term && nb == VFR_NB_BROKEN) ? 1024u : 0u;
-  if (skip || 1.0f - Tr >= a.term || nb == VFR_NB_BROKEN) return FTB_DONE;
-  return nb == VFR_NB_EXT ? FTB_REENTER : FTB_GO;
+  r.fa = fan;
+  r.fb = fbn;
+  const bool term = 1.0f - r.Tr >= a.term;
+  if (skip || term || nb >= VFR_NB_BROKEN) {
+    /* rare: the ray ends or leaves through the boundary */
+    if (skip) { r.bits |= 512u; return FTB_DONE; }
+    if (term) return FTB_DONE;
+    if (nb == VFR_NB_BROKEN) { r.bits |= 1024u; return FTB_DONE; }
+    return FTB_REENTER;
+  }
+  return FTB_GO;
 }
 
 /* After an exit through an external face: continue at the first entry
@@ -917,21 +963,24 @@ DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
   return o;
 }
 
+template <int PROJ, bool BIG>
 DEV RayOut trace_ftb(const TravArgs& a, uint32_t vi, uint32_t pix) {
   FtbRay r;
   ftb_init(a, vi, pix, r);
-  while (true) {
-    const uint32_t st = ftb_step(a, r);
-    if (st == FTB_GO || (st == FTB_REENTER && ftb_reenter(a, r))) continue;
-    break;
+  if (!(r.bits & 512u)) {
+    while (true) {
+      const uint32_t st = ftb_step<PROJ, BIG>(a, r);
+      if (st == FTB_GO || (st == FTB_REENTER && ftb_reenter(a, r))) continue;
+      break;
+    }
   }
   return ftb_out(a, r);
 }
 
-template <uint32_t MODE>
+template <uint32_t MODE, int PROJ, bool BIG>
 DEV RayOut trace(const TravArgs& a, uint32_t vi, uint32_t pix) {
   if (MODE == VFR_MODE_REF) return trace_ref(a, vi, pix);
-  return trace_ftb(a, vi, pix);
+  return trace_ftb<PROJ, BIG>(a, vi, pix);
 }
 
 DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
@@ -941,15 +990,15 @@ DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
 }
 
 /* One ray per thread over the compacted ray list. */
-template <uint32_t MODE>
-__global__ __launch_bounds__(256) void k_trav(TravArgs a) {
+template <uint32_t MODE, int PROJ, bool BIG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_trav(TravArgs a) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n = *a.n_rays;
   unsigned long long st = 0, an = 0, br = 0;
   if (i < n) {
     const uint64_t ray = a.rays[i];
     const uint32_t vi = (uint32_t)(ray >> 32), pix = (uint32_t)ray;
-    const RayOut o = trace<MODE>(a, vi, pix);
+    const RayOut o = trace<MODE, PROJ, BIG>(a, vi, pix);
     store_ray(a, a.views[vi].pix_off + pix, o);
     st = o.steps; an = o.anomaly; br = o.broken;
   }
@@ -1001,6 +1050,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
           const uint64_t ray = a.rays[idx];
           ftb_init(a, (uint32_t)(ray >> 32), (uint32_t)ray, r);
           active = true;
+          if (r.bits & 512u) {
+            /* degenerate entry face: the ray ends before its first step */
+            const RayOut o = ftb_out(a, r);
+            store_ray(a, r.gp, o);
+            an += o.anomaly;
+            active = false;
+          }
         }
       }
     }
@@ -1011,7 +1067,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     /* every lane steps (an idle lane's stale ray stays inside the mesh
      * arrays and its result is ignored), so the walk is one straight path */
     for (uint32_t k = 0; k < burst; ++k) {
-      const uint32_t s = ftb_step(a, r);
+      const uint32_t s = ftb_step<VFR_TRAV_MIXED, true>(a, r);
       if (active && s != FTB_GO && !(s == FTB_REENTER && ftb_reenter(a, r))) {
         const RayOut o = ftb_out(a, r);
         store_ray(a, r.gp, o);
@@ -1041,8 +1097,8 @@ int vfr_k_traverse_waves_per_simd(uint32_t occupancy, uint32_t* out) {
   return VFR_OK;
 }
 
-int vfr_k_traverse(uint32_t mode, uint32_t occupancy, const TravArgs& a, uint32_t max_rays,
-                   int persistent, uint32_t grid_waves, void* stream) {
+int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t occupancy, const TravArgs& a,
+                   uint32_t max_rays, int persistent, uint32_t grid_waves, void* stream) {
   if (max_rays == 0) return VFR_OK;
   if (persistent && mode == VFR_MODE_FTB) {
     uint32_t blocks = (grid_waves + 3u) / 4u;
@@ -1060,10 +1116,17 @@ int vfr_k_traverse(uint32_t mode, uint32_t occupancy, const TravArgs& a, uint32_
     return VFR_OK;
   }
   dim3 grid((max_rays + 255) / 256), block(256);
+  hipStream_t st = (hipStream_t)stream;
   if (mode == VFR_MODE_REF)
-    hipLaunchKernelGGL(k_trav<VFR_MODE_REF>, grid, block, 0, (hipStream_t)stream, a);
+    hipLaunchKernelGGL((k_trav<VFR_MODE_REF, VFR_TRAV_MIXED, true>), grid, block, 0, st, a);
+  else if (n_records >= VFR_BIG_RECORDS)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, true>), grid, block, 0, st, a);
+  else if (proj == VFR_TRAV_ORTHO)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_ORTHO, false>), grid, block, 0, st, a);
+  else if (proj == VFR_TRAV_PERSP)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_PERSP, false>), grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL(k_trav<VFR_MODE_FTB>, grid, block, 0, (hipStream_t)stream, a);
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false>), grid, block, 0, st, a);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -1173,37 +1236,41 @@ __global__ void k_gather_vert(const float4* __restrict__ v, const uint32_t* __re
   if (i < nV) out[i] = v[vperm[i]];
 }
 
-/* Internal tet record: vertex ids through the vertex permutation; neighbour
- * words (internal id << 2 | the face of the neighbour that links back, the
- * back-lookup of vfRayKernelCPU.cc:359-366).  links[4i+f] pairs the
- * neighbour word of face f with the neighbour's vertex off that face (the
- * tet's own vertex opposite f when there is no linked neighbour), so a walk
- * can fetch the next tet's fourth vertex without waiting for its record. */
+/* Internal tet record: vertex ids through the vertex permutation; per face
+ * the neighbour word (internal id << 2 | the face of the neighbour that
+ * links back, the back-lookup of vfRayKernelCPU.cc:359-366) and the
+ * neighbour's vertex off that face (the tet's own vertex opposite the face
+ * when there is no linked neighbour), so a walk can fetch the next tet's
+ * fourth vertex without waiting for its record. */
 __global__ void k_build_rec(const uint32_t* __restrict__ tets, const uint32_t* __restrict__ con,
                             const uint32_t* __restrict__ perm, const uint32_t* __restrict__ tinv,
                             const uint32_t* __restrict__ vinv, uint32_t nT,
-                            TetRec* __restrict__ rec, uint2* __restrict__ links) {
+                            TetRec* __restrict__ rec) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nT) return;
   uint32_t t = perm[i];
   TetRec r;
-  uint32_t o[4];
+  uint32_t n[4], o[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) r.v[k] = vinv[tets[t * 4 + k]];
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
     uint32_t u = con[t * 4 + f];
     o[f] = r.v[(f + 3) & 3];
-    if (u == t) { r.n[f] = VFR_NB_EXT; continue; }
+    if (u == t) { n[f] = VFR_NB_EXT; continue; }
     uint32_t kk = 4;
     for (uint32_t q = 0; q < 4; ++q)
       if (con[u * 4 + q] == t) { kk = q; break; }
-    r.n[f] = kk == 4 ? VFR_NB_BROKEN : ((tinv[u] << 2) | kk);
+    n[f] = kk == 4 ? VFR_NB_BROKEN : ((tinv[u] << 2) | kk);
     if (kk != 4) o[f] = vinv[tets[u * 4 + ((kk + 3) & 3)]];
   }
-  rec[i] = r;
 #pragma unroll
-  for (int f = 0; f < 4; ++f) links[(size_t)i * 4 + f] = make_uint2(r.n[f], o[f]);
+  for (int f = 0; f < 4; ++f) {
+    r.f[2 * f] = n[f];
+    r.f[2 * f + 1] = o[f];
+    r.pad[f] = 0;
+  }
+  rec[i] = r;
 }
 
 __global__ void k_ext_internal(const uint32_t* __restrict__ extf, uint32_t n,
@@ -1357,16 +1424,15 @@ int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
   }
   hipLaunchKernelGGL(k_invert, dim3(nblk(nT)), dim3(256), 0, st, perm.as<uint32_t>(), nT, tinv.as<uint32_t>());
   hipLaunchKernelGGL(k_invert, dim3(nblk(nV)), dim3(256), 0, st, vperm.as<uint32_t>(), nV, vinv.as<uint32_t>());
-  DevBuf vert, rec, links, ext_int;
+  DevBuf vert, rec, ext_int;
   ALLOC(vert, sizeof(float) * 4 * (size_t)nV);
   ALLOC(rec, sizeof(TetRec) * (size_t)nT);
-  ALLOC(links, sizeof(uint2) * 4 * (size_t)nT);
   ALLOC(ext_int, sizeof(uint32_t) * 2 * (size_t)n_ext);
   hipLaunchKernelGGL(k_gather_vert, dim3(nblk(nV)), dim3(256), 0, st, d_xyzs.as<float4>(),
                      vperm.as<uint32_t>(), nV, vert.as<float4>());
   hipLaunchKernelGGL(k_build_rec, dim3(nblk(nT)), dim3(256), 0, st, d_tets.as<uint32_t>(),
                      d_con.as<uint32_t>(), perm.as<uint32_t>(), tinv.as<uint32_t>(),
-                     vinv.as<uint32_t>(), nT, rec.as<TetRec>(), links.as<uint2>());
+                     vinv.as<uint32_t>(), nT, rec.as<TetRec>());
   if (n_ext)
     hipLaunchKernelGGL(k_ext_internal, dim3(nblk(n_ext)), dim3(256), 0, st, d_extf.as<uint32_t>(),
                        n_ext, tinv.as<uint32_t>(), ext_int.as<uint32_t>());
@@ -1375,13 +1441,12 @@ int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
   m.n_ext = n_ext;
   m.vert = vert.as<float>(); vert.release();
   m.rec = rec.as<TetRec>(); rec.release();
-  m.links = links.as<uint32_t>(); links.release();
   m.ext_int = ext_int.as<uint32_t>(); ext_int.release();
   m.extf = d_extf.as<uint32_t>(); d_extf.release();
   m.perm = perm.as<uint32_t>(); perm.release();
   m.vperm = vperm.as<uint32_t>(); vperm.release();
   m.con = d_con.as<uint32_t>(); d_con.release();
   m.bytes = sizeof(float) * 4ull * nV + sizeof(TetRec) * (uint64_t)nT + 16ull * n_ext +
-            4ull * nT + 4ull * nV + 16ull * nT + 32ull * nT;
+            4ull * nT + 4ull * nV + 16ull * nT;
   return VFR_OK;
 }

@@ -106,7 +106,7 @@ struct vfr_server {
   vfr_server_cfg cfg{};
   std::vector<SrvMesh> meshes;
   TetRec* rec = nullptr;
-  uint32_t* links = nullptr;
+  uint64_t n_records = 0;
   uint32_t* ext = nullptr;
   float4* vert = nullptr;
   std::vector<std::unique_ptr<Slot>> slots;
@@ -137,7 +137,7 @@ static void slot_free(Slot* s) {
 static int slot_alloc(Slot* s, uint64_t bytes, uint32_t max_batch) {
   /* three quarters for per-pixel buffers, one quarter for P-space vertices */
   s->cap_pix = (bytes * 3 / 4) / pixel_bytes();
-  s->cap_verts = (bytes / 4) / 16;
+  s->cap_verts = std::min<uint64_t>((bytes / 4) / 16, VFR_MAX_VIEW_VERTS - 1);
   s->cap_views = max_batch;
   CHECK_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   CHECK_HIP(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
@@ -161,10 +161,12 @@ static int slot_alloc(Slot* s, uint64_t bytes, uint32_t max_batch) {
 static int launch_batch(vfr_server* sv, Slot* s) {
   const uint32_t nv = (uint32_t)s->batch.size();
   uint64_t pix = 0, tiles = 0, verts = 0, faces = 0, keep_pix = 0;
+  uint32_t n_persp = 0;
   for (uint32_t i = 0; i < nv; ++i) {
     Req* r = s->batch[i];
     const SrvMesh& m = sv->meshes[r->mesh];
     DevView d = r->dv;
+    n_persp += d.proj == VFR_PROJ_PERSPECTIVE ? 1u : 0u;
     d.pix_off = (uint32_t)pix;
     d.tile_off = (uint32_t)tiles;
     d.tiles_x = (r->view.width + 7) / 8;
@@ -193,7 +195,6 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   TravArgs a;
   memset(&a, 0, sizeof(a));
   a.rec = sv->rec;
-  a.links = reinterpret_cast<const uint2*>(sv->links);
   a.vert = s->pverts;
   a.ext_int = sv->ext;
   a.tf = vfr_ctx_tf(sv->ctx);
@@ -210,7 +211,8 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   a.queue = s->counters + 1;
   a.term = sv->cfg.term_alpha;
   a.refill = sv->cfg.refill;
-  CHECK_RC(vfr_k_traverse(VFR_MODE_FTB, sv->cfg.occupancy, a, (uint32_t)pix,
+  const uint32_t proj = n_persp == 0 ? VFR_TRAV_ORTHO : (n_persp == nv ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
+  CHECK_RC(vfr_k_traverse(VFR_MODE_FTB, proj, sv->n_records, sv->cfg.occupancy, a, (uint32_t)pix,
                           (sv->cfg.flags & VFR_RENDER_PERSISTENT) ? 1 : 0,
                           sv->cfg.waves_per_cu * (uint32_t)vfr_ctx_cus(sv->ctx), st));
   CHECK_HIP(hipMemcpyAsync(s->h_stats, s->stats, sizeof(unsigned long long) * ST_N, hipMemcpyDeviceToHost, st));
@@ -328,7 +330,6 @@ extern "C" void vfr_server_destroy(vfr_server* sv) {
     slot_free(sp.get());
   }
   if (sv->rec) (void)hipFree(sv->rec);
-  if (sv->links) (void)hipFree(sv->links);
   if (sv->ext) (void)hipFree(sv->ext);
   if (sv->vert) (void)hipFree(sv->vert);
   delete sv;
@@ -362,8 +363,8 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
     exts += d->n_ext;
     verts += d->n_verts;
   }
-  if (tets >= (1ull << 31) || exts >= (1ull << 31) || verts >= (1ull << 31)) {
-    vfr_set_error("resident meshes exceed 2^31 tets, faces or vertices");
+  if (tets >= VFR_MAX_TETS || exts >= (1ull << 31) || verts >= (1ull << 31)) {
+    vfr_set_error("resident meshes exceed %u tets, or 2^31 faces or vertices", VFR_MAX_TETS - 1u);
     return VFR_EINVAL;
   }
   vfr_server* sv = new (std::nothrow) vfr_server;
@@ -373,7 +374,6 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
   CHECK_HIP(hipSetDevice(vfr_ctx_device(ctx)));
   int rc = VFR_OK;
   if (hipMalloc((void**)&sv->rec, sizeof(TetRec) * tets) != hipSuccess ||
-      hipMalloc((void**)&sv->links, 32 * tets) != hipSuccess ||
       hipMalloc((void**)&sv->ext, 8 * (exts + 1)) != hipSuccess ||
       hipMalloc((void**)&sv->vert, 16 * verts) != hipSuccess) {
     vfr_set_error("resident mesh arena allocation failed");
@@ -388,7 +388,6 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
     m.max_edge = d->lmt[0];
     vfr_mesh_bbox(meshes[i], m.bb_lo, m.bb_hi);
     if (hipMemcpy(sv->rec + ro, d->rec, sizeof(TetRec) * d->n_tets, hipMemcpyDeviceToDevice) != hipSuccess ||
-        hipMemcpy(sv->links + 8 * ro, d->links, 32ull * d->n_tets, hipMemcpyDeviceToDevice) != hipSuccess ||
         hipMemcpy(sv->ext + 2 * eo, d->ext_int, 8ull * d->n_ext, hipMemcpyDeviceToDevice) != hipSuccess ||
         hipMemcpy(sv->vert + vo, d->vert, 16ull * d->n_verts, hipMemcpyDeviceToDevice) != hipSuccess) {
       vfr_set_error("copy of mesh %d into the resident arena failed", i);
@@ -396,6 +395,7 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
     }
     sv->meshes.push_back(m);
     ro += d->n_tets; eo += d->n_ext; vo += d->n_verts;
+    sv->n_records = ro;
   }
   for (uint32_t i = 0; rc == VFR_OK && i < c.streams; ++i) {
     sv->slots.emplace_back(new Slot);
