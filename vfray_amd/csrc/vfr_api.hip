@@ -289,6 +289,7 @@ struct vfr_plan {
   uint64_t n_pix = 0;
   uint32_t n_tiles = 0;
   uint32_t trav_proj = VFR_TRAV_MIXED;    /* projections of the plan's views */
+  uint32_t tf_lo = 0, tf_n = 0;           /* TF slots the views use, if few  */
   DevView* views = nullptr;
   uint32_t* owner = nullptr;   /* REF owner or FTB count, [n_pix] */
   uint64_t* entries = nullptr; /* FTB, [n_pix * VFR_K] */
@@ -412,6 +413,7 @@ static int enqueue_batched(vfr_plan* p) {
 
 static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   TravArgs a;
+  memset(&a, 0, sizeof(a));
   const vfr_mesh* m = p->mesh;
   a.rec = m->d.rec;
   a.vert = p->pverts;
@@ -431,6 +433,8 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   a.queue = p->counters + p->n_views + 1;
   a.term = p->opts.term_alpha;
   a.refill = p->opts.refill;
+  a.tf_lds_base = p->tf_lo;
+  a.tf_lds_n = p->tf_n;
   return a;
 }
 
@@ -507,6 +511,13 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
     uint32_t persp = 0;
     for (int i = 0; i < n_views; ++i) persp += views[i].projection == VFR_PROJ_PERSPECTIVE ? 1u : 0u;
     p->trav_proj = persp == 0 ? VFR_TRAV_ORTHO : (persp == (uint32_t)n_views ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
+    uint32_t lo = UINT32_MAX, hi = 0;
+    for (int i = 0; i < n_views; ++i) {
+      lo = std::min(lo, (uint32_t)views[i].tf_id);
+      hi = std::max(hi, (uint32_t)views[i].tf_id);
+    }
+    /* few distinct slots: the walk reads them from LDS */
+    if (hi - lo < VFR_TF_LDS_SLOTS) { p->tf_lo = lo; p->tf_n = hi - lo + 1; }
   }
   if ((uint64_t)n_views * m->d.n_verts >= VFR_MAX_VIEW_VERTS || (uint64_t)n_views * m->d.n_ext >= (1ull << 32)) {
     /* P-space vertices are addressed with 32-bit byte offsets */
@@ -623,6 +634,8 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
                              p->counters + v, st));
       CHECK_HIP(hipEventRecord(p->ev[1], st));
       TravArgs a = trav_args(p, p->rays + hv.pix_off, p->counters + v);
+      a.tf_lds_base = hv.tf;
+      a.tf_lds_n = 1;
       CHECK_RC(vfr_k_traverse(p->opts.mode, hv.proj == VFR_PROJ_PERSPECTIVE ? VFR_TRAV_PERSP : VFR_TRAV_ORTHO,
                               m->d.n_tets, 0, a, npx, 0, 0, st));
       CHECK_HIP(hipEventRecord(p->fev[2 * v + 1], st));

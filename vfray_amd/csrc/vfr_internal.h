@@ -30,6 +30,9 @@ double vfr_psi_integral(double tf, double tb);
 #define VFR_BIG_RECORDS (1ull << 26)
 #define VFR_MAX_VIEW_VERTS (1ull << 28)
 
+/* Transfer-function slots a traversal launch can stage in LDS (4 KB each). */
+#define VFR_TF_LDS_SLOTS 4u
+
 /* Entry-list slots per pixel in front-to-back mode. */
 #define VFR_K 8u
 
@@ -109,6 +112,8 @@ struct TravArgs {
   uint32_t* queue;         /* persistent kernel work-queue head */
   float term;
   uint32_t refill;         /* persistent: refill when this many lanes idle */
+  uint32_t tf_lds_base;    /* one-ray-per-thread FTB: TF slots [base, base+n)   */
+  uint32_t tf_lds_n;       /* staged in LDS (0: read the global table)         */
 };
 
 /* ---- launchers (vfr_kernels.hip) ---- */
@@ -152,6 +157,10 @@ const float* vfr_ctx_psi(const vfr_ctx* c);
 const vfr_ctx* vfr_mesh_ctx(const vfr_mesh* m);
 const DevMesh* vfr_mesh_dev(const vfr_mesh* m);
 void vfr_mesh_bbox(const vfr_mesh* m, float lo[3], float hi[3]);
+
+/* Adds `add` to the tet index of the linked neighbour words of n records
+ * (a mesh copied into a record arena at record `add`). */
+int vfr_k_rebase_links(TetRec* rec, uint32_t n, uint32_t add, void* stream);
 
 /* preprocessing on the device */
 int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,

@@ -706,11 +706,12 @@ struct FtbRay {
   uint4 fa, fb;            /* its face words f[0..7] (prefetched) */
   uint32_t i0, i1, i2, i3; /* vertex ids of P0, P1, P2, Q */
   float o01, o12, o20;     /* edge functions P0->P1, P1->P2, P2->P0 (>= 0) */
-  float4 cP;               /* TF colour at the entry point */
+  float4 cP;               /* TF colour at the entry point ...               */
+  uint32_t cpi;            /* ... or its index, when the TF table is in LDS */
   float ox, oy, rf, prev;  /* rf = ray length factor / max edge; prev = Z (ortho) or depth */
   float ar, ag, ab, Tr;
   uint32_t t;              /* the tet, as an index into the record arena */
-  uint32_t voff, roff, eoff, gp, tfo, steps;
+  uint32_t vi, voff, gp, tfo, steps;
   uint32_t bits; /* cur 0..3 | nk 4..7 | persp 8 | anomaly 9 | broken 10 */
 };
 
@@ -748,13 +749,21 @@ DEV void face_values(bool persp, float D, float wA, float4 A, float wB, float4 B
   *s = clamp01(sn * rc);
 }
 
+/* Transfer-function colour `i` (slot offset included): from the block's LDS
+ * copy of the launch's TF slots (TFL), or from the global table. */
+template <bool TFL>
+DEV float4 tf_at(const TravArgs& a, const float4* tfl, uint32_t i) {
+  return TFL ? tfl[i] : ld32<float4>(a.tf, i * 16u);
+}
+
 /* Starts the walk at entry `cur` of the pixel's sorted list; false if the
  * entry face is degenerate at the pixel. */
-DEV bool ftb_segment(const TravArgs& a, FtbRay& r) {
+template <bool TFL>
+DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const uint64_t key = a.entries[(size_t)r.gp * VFR_K + ftb_cur(r)];
-  const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull) + r.eoff;
+  const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull) + a.views[r.vi].ext_off;
   const float4* __restrict__ pv = a.vert + r.voff;
-  r.t = r.roff + a.ext_int[2 * (size_t)e];
+  r.t = a.views[r.vi].rec_off + a.ext_int[2 * (size_t)e];
   const uint32_t fe = a.ext_int[2 * (size_t)e + 1];
   const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[r.t].v);
   r.rv = make_uint3(rv.x, rv.y, rv.z);
@@ -779,18 +788,19 @@ DEV bool ftb_segment(const TravArgs& a, FtbRay& r) {
   float sprev;
   /* P0 weighs o12, P1 o20, P2 o01 */
   face_values(ftb_persp(r), D, o12, P0, o20, P1, o01, P2, 0.0f, P2, &r.prev, &sprev);
-  r.cP = a.tf[r.tfo + min(255u, (uint32_t)(255.0f * sprev))];
+  const uint32_t ci = r.tfo + min(255u, (uint32_t)(255.0f * sprev));
+  if (TFL) r.cpi = ci; else r.cP = tf_at<false>(a, tfl, ci);
   return D > 0.0f;
 }
 
-DEV void ftb_init(const TravArgs& a, uint32_t vi, uint32_t pix, FtbRay& r) {
+template <bool TFL>
+DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pix, FtbRay& r) {
   const uint32_t W = a.views[vi].W, H = a.views[vi].H;
   const float k = a.views[vi].k;
   const bool persp = a.views[vi].proj == VFR_PROJ_PERSPECTIVE;
+  r.vi = vi;
   r.voff = a.views[vi].vert_off;
-  r.roff = a.views[vi].rec_off;
-  r.eoff = a.views[vi].ext_off;
-  r.tfo = a.views[vi].tf * 256u;
+  r.tfo = (a.views[vi].tf - (TFL ? a.tf_lds_base : 0u)) * 256u;
   r.gp = a.views[vi].pix_off + pix;
   r.ox = pix_to_obj(pix % W, W);
   r.oy = pix_to_obj(pix / W, H);
@@ -802,7 +812,7 @@ DEV void ftb_init(const TravArgs& a, uint32_t vi, uint32_t pix, FtbRay& r) {
   r.ar = r.ag = r.ab = 0.0f;
   r.Tr = 1.0f;
   r.steps = 0;
-  if (!ftb_segment(a, r)) r.bits |= 512u;
+  if (!ftb_segment<TFL>(a, tfl, r)) r.bits |= 512u;
 }
 
 enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
@@ -822,8 +832,8 @@ enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
 /* PROJ: VFR_TRAV_*, the projection of every ray of the launch.  BIG: the
  * record arena reaches 4 GiB (VFR_BIG_RECORDS tets or more), so records are
  * addressed with 64-bit arithmetic instead of 32-bit byte offsets. */
-template <int PROJ, bool BIG>
-DEV uint32_t ftb_step(const TravArgs& a, FtbRay& r) {
+template <int PROJ, bool BIG, bool TFL>
+DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const bool persp = PROJ == VFR_TRAV_MIXED ? ftb_persp(r) : PROJ == VFR_TRAV_PERSP;
   r.steps += 1u;
   const float4 P3 = rel(r.Q, r.ox, r.oy);
@@ -872,10 +882,13 @@ DEV uint32_t ftb_step(const TravArgs& a, FtbRay& r) {
   const uint4 fa = r.fa, fb = r.fb;
   const uint32_t nb = pick3u(s0, s1, fa.z, fb.x, pick3u(s2, false, fb.z, 0u, fa.x));
   const uint32_t i3n = pick3u(s0, s1, fa.w, fb.y, pick3u(s2, false, fb.w, 0u, fa.y));
-  const uint32_t tn = nb < VFR_NB_BROKEN ? r.roff + (nb >> 2) : r.t;
+  /* neighbour words index the record arena (a server rebases each resident
+   * mesh's words when it copies them in) */
+  const uint32_t tn = nb < VFR_NB_BROKEN ? (nb >> 2) : r.t;
   /* round 1: the colour at the exit point and the next tet (the current one
    * again at a boundary, unused) */
-  const float4 cN = ld32<float4>(a.tf, (r.tfo + min(255u, (uint32_t)(255.0f * snext))) * 16u);
+  const uint32_t ci = r.tfo + min(255u, (uint32_t)(255.0f * snext));
+  const float4 cN = tf_at<TFL>(a, tfl, ci);
   const float4 Qn = ld32<float4>(a.vert, (r.voff + i3n) * 16u);
   uint3 rvn;
   uint4 fan, fbn;
@@ -891,7 +904,8 @@ DEV uint32_t ftb_step(const TravArgs& a, FtbRay& r) {
   __builtin_amdgcn_sched_barrier(0);
   /* round 2: psi */
   const float dl = fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
-  const float tf_ = r.cP.w * dl, tb_ = cN.w * dl;
+  const float4 cP = TFL ? tfl[r.cpi] : r.cP;
+  const float tf_ = cP.w * dl, tb_ = cN.w * dl;
   /* 512 * gamma = 512 * tau / (1 + tau) for both ends with one division */
   const float uf = 1.0f + tf_, ub = 1.0f + tb_;
   const float q = 512.0f / (uf * ub);
@@ -912,15 +926,15 @@ DEV uint32_t ftb_step(const TravArgs& a, FtbRay& r) {
   r.o01 = n01; r.o12 = n12; r.o20 = n20;
   /* composite the segment */
   const float w1c = 1.0f - ps, w2c = ps - zeta;
-  const float ar = __builtin_fmaf(r.Tr, __builtin_fmaf(r.cP.x, w1c, cN.x * w2c), r.ar);
-  const float ag = __builtin_fmaf(r.Tr, __builtin_fmaf(r.cP.y, w1c, cN.y * w2c), r.ag);
-  const float ab = __builtin_fmaf(r.Tr, __builtin_fmaf(r.cP.z, w1c, cN.z * w2c), r.ab);
+  const float ar = __builtin_fmaf(r.Tr, __builtin_fmaf(cP.x, w1c, cN.x * w2c), r.ar);
+  const float ag = __builtin_fmaf(r.Tr, __builtin_fmaf(cP.y, w1c, cN.y * w2c), r.ag);
+  const float ab = __builtin_fmaf(r.Tr, __builtin_fmaf(cP.z, w1c, cN.z * w2c), r.ab);
   const float Tr = r.Tr * zeta;
   r.ar = skip ? r.ar : ar;
   r.ag = skip ? r.ag : ag;
   r.ab = skip ? r.ab : ab;
   r.Tr = skip ? r.Tr : Tr;
-  r.cP = cN;
+  if (TFL) r.cpi = ci; else r.cP = cN;
   r.prev = zd;
   r.t = tn;
   r.i3 = i3n;
@@ -942,14 +956,15 @@ DEV uint32_t ftb_step(const TravArgs& a, FtbRay& r) {
 /* After an exit through an external face: continue at the first entry
  * strictly behind the exit point; false when there is none (the ray is
  * finished). */
-DEV bool ftb_reenter(const TravArgs& a, FtbRay& r) {
+template <bool TFL>
+DEV bool ftb_reenter(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const float exitZ = ftb_persp(r) ? 1.0f / r.prev : r.prev;
   uint32_t cur = ftb_cur(r) + 1u;
   const uint32_t nk = ftb_nk(r);
   while (cur < nk && !(funkey((uint32_t)(a.entries[(size_t)r.gp * VFR_K + cur] >> 32)) < exitZ)) ++cur;
   if (cur >= nk) return false;
   r.bits = (r.bits & ~15u) | cur;
-  if (!ftb_segment(a, r)) { r.bits |= 512u; return false; }
+  if (!ftb_segment<TFL>(a, tfl, r)) { r.bits |= 512u; return false; }
   return true;
 }
 
@@ -963,24 +978,24 @@ DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
   return o;
 }
 
-template <int PROJ, bool BIG>
-DEV RayOut trace_ftb(const TravArgs& a, uint32_t vi, uint32_t pix) {
+template <int PROJ, bool BIG, bool TFL>
+DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pix) {
   FtbRay r;
-  ftb_init(a, vi, pix, r);
+  ftb_init<TFL>(a, tfl, vi, pix, r);
   if (!(r.bits & 512u)) {
     while (true) {
-      const uint32_t st = ftb_step<PROJ, BIG>(a, r);
-      if (st == FTB_GO || (st == FTB_REENTER && ftb_reenter(a, r))) continue;
+      const uint32_t st = ftb_step<PROJ, BIG, TFL>(a, tfl, r);
+      if (st == FTB_GO || (st == FTB_REENTER && ftb_reenter<TFL>(a, tfl, r))) continue;
       break;
     }
   }
   return ftb_out(a, r);
 }
 
-template <uint32_t MODE, int PROJ, bool BIG>
-DEV RayOut trace(const TravArgs& a, uint32_t vi, uint32_t pix) {
+template <uint32_t MODE, int PROJ, bool BIG, bool TFL>
+DEV RayOut trace(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pix) {
   if (MODE == VFR_MODE_REF) return trace_ref(a, vi, pix);
-  return trace_ftb<PROJ, BIG>(a, vi, pix);
+  return trace_ftb<PROJ, BIG, TFL>(a, tfl, vi, pix);
 }
 
 DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
@@ -990,15 +1005,24 @@ DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
 }
 
 /* One ray per thread over the compacted ray list. */
-template <uint32_t MODE, int PROJ, bool BIG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_trav(TravArgs a) {
+/* TFL: the launch's transfer-function slots [tf_lds_base, +tf_lds_n) are
+ * staged in LDS once per block (every view of the launch uses one of them). */
+template <uint32_t MODE, int PROJ, bool BIG, bool TFL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR_MODE_REF ? 5 : 6,
+                                                                     MODE == VFR_MODE_REF ? 5 : 6))) void k_trav(
+    TravArgs a) {
+  __shared__ float4 tfs[TFL ? VFR_TF_LDS_SLOTS * 256 : 1];
+  if (TFL) {
+    for (uint32_t k = threadIdx.x; k < a.tf_lds_n * 256u; k += blockDim.x) tfs[k] = a.tf[a.tf_lds_base * 256u + k];
+    __syncthreads();
+  }
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n = *a.n_rays;
   unsigned long long st = 0, an = 0, br = 0;
   if (i < n) {
     const uint64_t ray = a.rays[i];
     const uint32_t vi = (uint32_t)(ray >> 32), pix = (uint32_t)ray;
-    const RayOut o = trace<MODE, PROJ, BIG>(a, vi, pix);
+    const RayOut o = trace<MODE, PROJ, BIG, TFL>(a, tfs, vi, pix);
     store_ray(a, a.views[vi].pix_off + pix, o);
     st = o.steps; an = o.anomaly; br = o.broken;
   }
@@ -1031,7 +1055,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   FtbRay r;
   {
     const uint64_t ray = a.rays[0];
-    ftb_init(a, (uint32_t)(ray >> 32), (uint32_t)ray, r);
+    ftb_init<false>(a, nullptr, (uint32_t)(ray >> 32), (uint32_t)ray, r);
   }
   bool active = false, drained = false;
   uint32_t st = 0, an = 0, br = 0;
@@ -1048,7 +1072,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         const uint32_t idx = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
         if (idx < n) {
           const uint64_t ray = a.rays[idx];
-          ftb_init(a, (uint32_t)(ray >> 32), (uint32_t)ray, r);
+          ftb_init<false>(a, nullptr, (uint32_t)(ray >> 32), (uint32_t)ray, r);
           active = true;
           if (r.bits & 512u) {
             /* degenerate entry face: the ray ends before its first step */
@@ -1067,8 +1091,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     /* every lane steps (an idle lane's stale ray stays inside the mesh
      * arrays and its result is ignored), so the walk is one straight path */
     for (uint32_t k = 0; k < burst; ++k) {
-      const uint32_t s = ftb_step<VFR_TRAV_MIXED, true>(a, r);
-      if (active && s != FTB_GO && !(s == FTB_REENTER && ftb_reenter(a, r))) {
+      const uint32_t s = ftb_step<VFR_TRAV_MIXED, true, false>(a, nullptr, r);
+      if (active && s != FTB_GO && !(s == FTB_REENTER && ftb_reenter<false>(a, nullptr, r))) {
         const RayOut o = ftb_out(a, r);
         store_ray(a, r.gp, o);
         st += o.steps; an += o.anomaly; br += o.broken;
@@ -1117,16 +1141,23 @@ int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t oc
   }
   dim3 grid((max_rays + 255) / 256), block(256);
   hipStream_t st = (hipStream_t)stream;
+  const bool tfl = a.tf_lds_n >= 1 && a.tf_lds_n <= VFR_TF_LDS_SLOTS;
   if (mode == VFR_MODE_REF)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_REF, VFR_TRAV_MIXED, true>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_trav<VFR_MODE_REF, VFR_TRAV_MIXED, true, false>), grid, block, 0, st, a);
   else if (n_records >= VFR_BIG_RECORDS)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, true>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, true, false>), grid, block, 0, st, a);
+  else if (proj == VFR_TRAV_ORTHO && tfl)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_ORTHO, false, true>), grid, block, 0, st, a);
   else if (proj == VFR_TRAV_ORTHO)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_ORTHO, false>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_ORTHO, false, false>), grid, block, 0, st, a);
+  else if (proj == VFR_TRAV_PERSP && tfl)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_PERSP, false, true>), grid, block, 0, st, a);
   else if (proj == VFR_TRAV_PERSP)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_PERSP, false>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_PERSP, false, false>), grid, block, 0, st, a);
+  else if (tfl)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false, true>), grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false, false>), grid, block, 0, st, a);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -1271,6 +1302,25 @@ __global__ void k_build_rec(const uint32_t* __restrict__ tets, const uint32_t* _
     r.pad[f] = 0;
   }
   rec[i] = r;
+}
+
+/* Adds `add` to the tet index of every linked neighbour word of records
+ * [0, n): a mesh's records copied into an arena at record `add`. */
+__global__ void k_rebase_links(TetRec* __restrict__ rec, uint32_t n, uint32_t add) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t w = rec[i].f[2 * k];
+    if (w < VFR_NB_BROKEN) rec[i].f[2 * k] = w + (add << 2);
+  }
+}
+
+int vfr_k_rebase_links(TetRec* rec, uint32_t n, uint32_t add, void* stream) {
+  if (n == 0 || add == 0) return VFR_OK;
+  hipLaunchKernelGGL(k_rebase_links, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, rec, n, add);
+  CHECK_HIP(hipGetLastError());
+  return VFR_OK;
 }
 
 __global__ void k_ext_internal(const uint32_t* __restrict__ extf, uint32_t n,

@@ -161,12 +161,14 @@ static int slot_alloc(Slot* s, uint64_t bytes, uint32_t max_batch) {
 static int launch_batch(vfr_server* sv, Slot* s) {
   const uint32_t nv = (uint32_t)s->batch.size();
   uint64_t pix = 0, tiles = 0, verts = 0, faces = 0, keep_pix = 0;
-  uint32_t n_persp = 0;
+  uint32_t n_persp = 0, tf_lo = UINT32_MAX, tf_hi = 0;
   for (uint32_t i = 0; i < nv; ++i) {
     Req* r = s->batch[i];
     const SrvMesh& m = sv->meshes[r->mesh];
     DevView d = r->dv;
     n_persp += d.proj == VFR_PROJ_PERSPECTIVE ? 1u : 0u;
+    tf_lo = std::min(tf_lo, d.tf);
+    tf_hi = std::max(tf_hi, d.tf);
     d.pix_off = (uint32_t)pix;
     d.tile_off = (uint32_t)tiles;
     d.tiles_x = (r->view.width + 7) / 8;
@@ -211,6 +213,7 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   a.queue = s->counters + 1;
   a.term = sv->cfg.term_alpha;
   a.refill = sv->cfg.refill;
+  if (tf_hi - tf_lo < VFR_TF_LDS_SLOTS) { a.tf_lds_base = tf_lo; a.tf_lds_n = tf_hi - tf_lo + 1; }
   const uint32_t proj = n_persp == 0 ? VFR_TRAV_ORTHO : (n_persp == nv ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
   CHECK_RC(vfr_k_traverse(VFR_MODE_FTB, proj, sv->n_records, sv->cfg.occupancy, a, (uint32_t)pix,
                           (sv->cfg.flags & VFR_RENDER_PERSISTENT) ? 1 : 0,
@@ -391,6 +394,12 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
         hipMemcpy(sv->ext + 2 * eo, d->ext_int, 8ull * d->n_ext, hipMemcpyDeviceToDevice) != hipSuccess ||
         hipMemcpy(sv->vert + vo, d->vert, 16ull * d->n_verts, hipMemcpyDeviceToDevice) != hipSuccess) {
       vfr_set_error("copy of mesh %d into the resident arena failed", i);
+      rc = VFR_EHIP;
+    }
+    /* neighbour words index the whole arena */
+    if (rc == VFR_OK) rc = vfr_k_rebase_links(sv->rec + ro, d->n_tets, (uint32_t)ro, nullptr);
+    if (rc == VFR_OK && hipDeviceSynchronize() != hipSuccess) {
+      vfr_set_error("rebasing mesh %d in the resident arena failed", i);
       rc = VFR_EHIP;
     }
     sv->meshes.push_back(m);
