@@ -10,6 +10,9 @@ frame sync per view) is timed on the same views and reported beside it.
 
 `plain_gpu`: 64^3 grid (1,572,864 tets), one 1024^2 perspective view.
 
+`concave`: 96^3 grid with a graded radial map and 12 voids (rays exit and
+re-enter), 32 views of 768^2.
+
 `serve`: 8 resident meshes (32^3 ... 160^3 grids, 0.2M-24.6M tets), 4
 transfer functions each, a stream of requests with a random mesh, TF,
 viewpoint and side in {256..1024}.  The plain renderer (one request at a time,
@@ -158,7 +161,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="batch64", choices=["batch64", "plain_gpu", "serve"])
+    ap.add_argument("--workload", default="batch64", choices=["batch64", "plain_gpu", "concave", "serve"])
     ap.add_argument("--requests", type=int, default=10000, help="serve: requests in the stream")
     ap.add_argument("--load", type=float, default=0.8, help="serve: offered load / batched capacity")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -196,6 +199,9 @@ def main() -> int:
     if args.workload == "batch64":
         x, t, tfa, seed = WL.batch64()
         nviews, sides = 64, list(WL.SIDES_BATCH)
+    elif args.workload == "concave":
+        x, t, tfa, seed = WL.concave()
+        nviews, sides = 32, [768]
     else:
         x, t, tfa, seed = WL.plain_gpu()
         nviews, sides = 1, [1024]

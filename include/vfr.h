@@ -146,7 +146,9 @@ typedef struct vfr_render_opts {
   uint32_t refill;      /* persistent kernel: idle lanes that trigger a wave
                            refill (0 = 32)                                 */
   uint32_t use_graph;   /* capture the batched launch set in a hipGraph    */
-  uint32_t max_entries; /* FTB entry-list slots per pixel, 0 = 8           */
+  uint32_t max_entries; /* FTB entry-list slots per pixel: 8, 16 or 32; 0 =
+                           the fewest that hold every pixel's entry faces
+                           (counting pass at plan creation, at most 32)  */
   uint32_t flags;       /* VFR_RENDER_*                                    */
   uint32_t occupancy;   /* persistent kernel variant: waves per SIMD (4, 5,
                            6: register-capped; other values: the compiler's

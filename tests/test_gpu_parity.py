@@ -212,3 +212,60 @@ def test_autotune_keeps_results(ctx):
         assert np.array_equal(p.read(i)["rgba"], ref[i]["rgba"])
     p.close()
     m.close()
+
+
+@pytest.mark.parametrize("term", [0.99, 1.0])
+def test_ftb_concave_reentry(ctx, psi, term):
+    """Voided, radially graded mesh: rays exit through interior boundaries and
+    re-enter; batched and per-view renders equal the oracle bit for bit."""
+    inp = grid_inputs(20, 0x5EED0303, variant=1)
+    m = ctx.mesh(inp["x"], inp["t"])
+    tf = ctx.tf(inp["tf"])
+    from vfray_amd import workloads as WL
+    views = WL.sphere_views(0x5EED0303, 4, [256, 320], tf)
+    print(f"FTB concave 20^3 term={term}", flush=True)
+    a, sa = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, term_alpha=term)
+    b, sb = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW, term_alpha=term)
+    assert sa["anomalies"] == 0
+    multi = 0
+    for v, g, h in zip(views, a, b):
+        orc = oracle_ftb(inp, v, psi, term=term, fp64=True)
+        assert orc["anomalies"] == 0
+        multi += int((orc["n_entries"] > 1).sum())
+        for key in ("rgba", "entry_ext", "steps"):
+            assert np.array_equal(g[key], h[key]), key
+        assert np.array_equal(g["entry_ext"], orc["entry_ext"])
+        assert np.array_equal(g["steps"], orc["steps"])
+        assert np.abs(g["rgba"] - orc["rgba"]).max() <= TOL
+        cov = orc["entry_ext"] != V.NO_ID
+        assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
+    assert multi > 1000  # pixels whose ray meets the boundary more than once
+    m.close()
+
+
+def test_ftb_concave_c4_views(ctx, psi):
+    """The concave benchmark mesh (96^3 grid, graded, 12 voids): two of its
+    32 views at 768^2 against the oracle."""
+    from vfray_amd import workloads as WL
+    x, t, tfa, seed = WL.concave()
+    inp = dict(x=x, t=t, tf=tfa, xn=O.normalize(x))
+    inp["con"], _ = O.connectivity(t, x.shape[0])
+    inp["ext"] = O.extf(inp["con"])
+    inp["lmt"] = O.lmt(inp["xn"], t)
+    m = ctx.mesh(x, t)
+    tf = ctx.tf(tfa)
+    views = WL.sphere_views(seed, 32, [768], tf)[:2]
+    print("FTB concave 96^3 2x768^2", flush=True)
+    # up to 13 entry faces per pixel here: 8 slots drop some, and say so
+    with pytest.raises(V.VfrError, match="dropped"):
+        ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, max_entries=8)
+    outs, st = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+    for v, g in zip(views, outs):
+        orc = oracle_ftb(inp, v, psi, fp64=True)
+        assert orc["anomalies"] == 0
+        assert np.array_equal(g["entry_ext"], orc["entry_ext"])
+        assert np.array_equal(g["steps"], orc["steps"])
+        assert np.abs(g["rgba"] - orc["rgba"]).max() <= TOL
+        cov = orc["entry_ext"] != V.NO_ID
+        assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
+    m.close()

@@ -344,9 +344,10 @@ class Context:
         return Mesh(self, xyzs, tets, con, extf, lmt, normalize, morton)
 
     def plan(self, mesh: "Mesh", views, mode=MODE_FTB, schedule=SCHED_BATCHED, term_alpha=0.99,
-             use_graph=False, waves_per_cu=0, refill=0, persistent=False, occupancy=0) -> "Plan":
+             use_graph=False, waves_per_cu=0, refill=0, persistent=False, occupancy=0,
+             max_entries=0) -> "Plan":
         return Plan(self, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
-                    persistent, occupancy)
+                    persistent, occupancy, max_entries)
 
     def render(self, mesh: "Mesh", views, **kw) -> tuple[list[dict], dict]:
         p = self.plan(mesh, views, **kw)
@@ -394,11 +395,11 @@ class Mesh:
 
 class Plan:
     def __init__(self, ctx, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
-                 persistent=False, occupancy=0):
+                 persistent=False, occupancy=0, max_entries=0):
         self.ctx, self.mesh = ctx, mesh
         self.views = list(views)
         arr = (View * len(self.views))(*self.views)
-        o = RenderOpts(mode, schedule, term_alpha, waves_per_cu, refill, 1 if use_graph else 0, 0,
+        o = RenderOpts(mode, schedule, term_alpha, waves_per_cu, refill, 1 if use_graph else 0, max_entries,
                        RENDER_PERSISTENT if persistent else 0, occupancy, 0)
         self._p = C.c_void_p()
         _check(lib().vfr_plan_create(ctx._p, mesh._p, arr, len(self.views), C.byref(o), C.byref(self._p)))

@@ -292,7 +292,8 @@ struct vfr_plan {
   uint32_t tf_lo = 0, tf_n = 0;           /* TF slots the views use, if few  */
   DevView* views = nullptr;
   uint32_t* owner = nullptr;   /* REF owner or FTB count, [n_pix] */
-  uint64_t* entries = nullptr; /* FTB, [n_pix * VFR_K] */
+  uint64_t* entries = nullptr; /* FTB, [n_pix * kmax] */
+  uint32_t kmax = VFR_K;       /* FTB entry slots per pixel */
   uint64_t* rays = nullptr;    /* [n_pix] */
   float4* pverts = nullptr;    /* [n_views * n_verts] P-space vertices */
   float4* rgba = nullptr;
@@ -393,6 +394,23 @@ extern "C" void vfr_plan_destroy(vfr_plan* p) {
   delete p;
 }
 
+/* Largest number of entry faces any pixel of the plan has (device counting
+ * pass: transform + raster without storing keys). */
+static int plan_count_entries(vfr_plan* p, uint32_t* out) {
+  hipStream_t st = p->ctx->stream;
+  const vfr_mesh* m = p->mesh;
+  CHECK_RC(vfr_k_clear_u32(p->owner, p->n_pix, st));
+  CHECK_RC(vfr_k_clear_u32(p->counters, 1, st));
+  const uint64_t nverts = (uint64_t)p->n_views * m->d.n_verts, nfaces = (uint64_t)p->n_views * m->d.n_ext;
+  CHECK_RC(vfr_k_xform_views(p->views, 0, p->n_views, nverts, (const float4*)m->d.vert, p->pverts, st));
+  CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, nfaces, m->d.rec, m->d.ext_int, p->pverts, p->owner,
+                            nullptr, 0, p->stats, st));
+  CHECK_RC(vfr_k_max_u32(p->owner, p->n_pix, p->counters, st));
+  CHECK_HIP(hipMemcpyAsync(out, p->counters, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  CHECK_HIP(hipStreamSynchronize(st));
+  return VFR_OK;
+}
+
 static int enqueue_batched(vfr_plan* p) {
   hipStream_t st = p->ctx->stream;
   const vfr_mesh* m = p->mesh;
@@ -405,9 +423,9 @@ static int enqueue_batched(vfr_plan* p) {
     CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, nfaces, m->d.rec, m->d.ext_int, p->pverts, p->owner, st));
   else
     CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, nfaces, m->d.rec, m->d.ext_int, p->pverts, p->owner,
-                              p->entries, p->stats, st));
+                              p->entries, p->kmax, p->stats, st));
   CHECK_RC(vfr_k_compact(p->views, p->n_views, 0, p->n_tiles, p->opts.mode, p->owner, p->owner,
-                         p->entries, p->rgba, p->steps, p->entry, p->rays, p->counters + p->n_views, st));
+                         p->entries, p->kmax, p->rgba, p->steps, p->entry, p->rays, p->counters + p->n_views, st));
   return VFR_OK;
 }
 
@@ -426,6 +444,7 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   a.owner = p->owner;
   a.cnt = p->owner;
   a.entries = p->entries;
+  a.kmax = p->kmax;
   a.rgba = p->rgba;
   a.steps = p->steps;
   a.entry_out = p->entry;
@@ -458,8 +477,10 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   if (o.mode > VFR_MODE_FTB || o.schedule > VFR_SCHED_BATCHED) { vfr_set_error("bad mode/schedule"); return VFR_EINVAL; }
   if (!(o.term_alpha > 0.0f)) o.term_alpha = 0.99f;
   if (o.term_alpha > 1.0f) { vfr_set_error("term_alpha %g > 1", (double)o.term_alpha); return VFR_EINVAL; }
-  if (o.max_entries == 0) o.max_entries = VFR_K;
-  if (o.max_entries != VFR_K) { vfr_set_error("max_entries must be %u", VFR_K); return VFR_EINVAL; }
+  if (o.max_entries != 0 && o.max_entries != 8 && o.max_entries != 16 && o.max_entries != VFR_K_MAX) {
+    vfr_set_error("max_entries must be 0 (chosen per plan), 8, 16 or %u", VFR_K_MAX);
+    return VFR_EINVAL;
+  }
   if (o.waves_per_cu == 0) o.waves_per_cu = 16;
   if (o.refill == 0) o.refill = 32;
   if (o.waves_per_cu > 64 || o.refill > 64) {
@@ -535,7 +556,6 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   }
   PALLOC(p->views, sizeof(DevView) * n_views);
   PALLOC(p->owner, sizeof(uint32_t) * pix);
-  if (o.mode == VFR_MODE_FTB) PALLOC(p->entries, sizeof(uint64_t) * VFR_K * pix);
   PALLOC(p->rays, sizeof(uint64_t) * pix);
   PALLOC(p->pverts, sizeof(float4) * (size_t)n_views * m->d.n_verts);
   PALLOC(p->rgba, sizeof(float4) * pix);
@@ -547,6 +567,22 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   if (rc == VFR_OK && hipMemcpy(p->views, p->hv.data(), sizeof(DevView) * n_views, hipMemcpyHostToDevice) != hipSuccess) {
     vfr_set_error("view upload failed");
     rc = VFR_EHIP;
+  }
+  if (rc == VFR_OK && o.mode == VFR_MODE_FTB) {
+    /* entry slots per pixel: as asked, or the fewest of 8/16/32 that hold
+     * every pixel's entry faces, from a counting pass over all views */
+    uint32_t k = o.max_entries;
+    if (k == 0) {
+      uint32_t mx = 0;
+      rc = plan_count_entries(p, &mx);
+      k = mx <= 8 ? 8u : (mx <= 16 ? 16u : VFR_K_MAX);
+    }
+    p->kmax = k;
+    p->opts.max_entries = k;
+    if (rc == VFR_OK && hipMalloc((void**)&p->entries, sizeof(uint64_t) * k * pix) != hipSuccess) {
+      vfr_set_error("plan allocation of %zu bytes failed", (size_t)(sizeof(uint64_t) * k * pix));
+      rc = VFR_ENOMEM;
+    }
   }
   for (int i = 0; rc == VFR_OK && i < 3; ++i)
     if (hipEventCreate(&p->ev[i]) != hipSuccess) { vfr_set_error("event creation failed"); rc = VFR_EHIP; }
@@ -628,9 +664,9 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
         CHECK_RC(vfr_k_raster_ref(p->views, v, 1, m->d.n_ext, m->d.rec, m->d.ext_int, p->pverts, p->owner, st));
       else
         CHECK_RC(vfr_k_raster_ftb(p->views, v, 1, m->d.n_ext, m->d.rec, m->d.ext_int, p->pverts, p->owner,
-                                  p->entries, p->stats, st));
+                                  p->entries, p->kmax, p->stats, st));
       CHECK_RC(vfr_k_compact(p->views, p->n_views, hv.tile_off, ntl, p->opts.mode, p->owner, p->owner,
-                             p->entries, p->rgba, p->steps, p->entry, p->rays + hv.pix_off,
+                             p->entries, p->kmax, p->rgba, p->steps, p->entry, p->rays + hv.pix_off,
                              p->counters + v, st));
       CHECK_HIP(hipEventRecord(p->ev[1], st));
       TravArgs a = trav_args(p, p->rays + hv.pix_off, p->counters + v);
@@ -676,7 +712,7 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
     stats->ms_frame_p50 = p50;
     stats->ms_frame_p99 = p99;
     if (s[ST_OVERFLOW]) {
-      vfr_set_error("%llu pixel entries beyond %u per pixel were dropped", s[ST_OVERFLOW], VFR_K);
+      vfr_set_error("%llu pixel entries beyond %u per pixel were dropped", s[ST_OVERFLOW], p->kmax);
       return VFR_EMESH;
     }
     if (s[4]) {

@@ -35,6 +35,8 @@ double vfr_psi_integral(double tf, double tb);
 
 /* Entry-list slots per pixel in front-to-back mode. */
 #define VFR_K 8u
+/* Most slots a plan's entry lists can have (plans pick 8, 16 or 32). */
+#define VFR_K_MAX 32u
 
 #define VFR_NO_ID 0xFFFFFFFFu
 
@@ -104,7 +106,7 @@ struct TravArgs {
   const uint32_t* n_rays;  /* device count of rays */
   const uint32_t* owner;   /* REF: ext index + 1, 0 = none */
   const uint32_t* cnt;     /* FTB: entries per pixel */
-  const uint64_t* entries; /* FTB: [pixels * VFR_K] keys (z << 32 | ext) */
+  const uint64_t* entries; /* FTB: [pixels * kmax] keys (z << 32 | ext) */
   float4* rgba;
   uint32_t* steps;
   uint32_t* entry_out;
@@ -114,6 +116,7 @@ struct TravArgs {
   uint32_t refill;         /* persistent: refill when this many lanes idle */
   uint32_t tf_lds_base;    /* one-ray-per-thread FTB: TF slots [base, base+n)   */
   uint32_t tf_lds_n;       /* staged in LDS (0: read the global table)         */
+  uint32_t kmax;           /* FTB: entry slots per pixel (8, 16 or 32)         */
 };
 
 /* ---- launchers (vfr_kernels.hip) ---- */
@@ -126,11 +129,14 @@ int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, uint64_t w
                       const float4* wvert, float4* pv, void* stream);
 int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const TetRec* rec,
                      const uint32_t* ext_int, const float4* pv, uint32_t* owner, void* stream);
+/* kmax: entry slots per pixel; 0 only counts (cnt) without storing keys. */
 int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const TetRec* rec,
                      const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
-                     unsigned long long* stats, void* stream);
+                     uint32_t kmax, unsigned long long* stats, void* stream);
+/* *out = max(*out, cnt[i]) over n pixels (device counter). */
+int vfr_k_max_u32(const uint32_t* cnt, size_t n, uint32_t* out, void* stream);
 int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, uint32_t ntiles,
-                  uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries,
+                  uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries, uint32_t kmax,
                   float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
                   uint32_t* n_rays, void* stream);
 /* occupancy: waves per SIMD of the persistent kernel variant (4, 5, 6:

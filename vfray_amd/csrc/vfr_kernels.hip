@@ -376,7 +376,7 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
                                                     const uint32_t* __restrict__ ext_int,
                                                     const float4* __restrict__ pv,
                                                     uint32_t* __restrict__ cnt,
-                                                    uint64_t* __restrict__ entries,
+                                                    uint64_t* __restrict__ entries, uint32_t kmax,
                                                     unsigned long long* __restrict__ stats) {
   uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= work) return;
@@ -411,19 +411,19 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
       float Zp = bary(fv[0].z, fv[1].z, fv[2].z, h, 1.0f / h.D);
       uint32_t gp = off + py * W + px;
       uint32_t slot = atomicAdd(&cnt[gp], 1u);
-      if (slot < VFR_K) entries[(size_t)gp * VFR_K + slot] = ((uint64_t)fkey(Zp) << 32) | e;
+      if (slot < kmax) entries[(size_t)gp * kmax + slot] = ((uint64_t)fkey(Zp) << 32) | e;
       else over++;
     }
   }
-  if (over) atomicAdd(&stats[ST_OVERFLOW], over);
+  if (over && kmax) atomicAdd(&stats[ST_OVERFLOW], over);
 }
 
 int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const TetRec* rec,
                      const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
-                     unsigned long long* stats, void* stream) {
+                     uint32_t kmax, unsigned long long* stats, void* stream) {
   if (work == 0) return VFR_OK;
   hipLaunchKernelGGL(k_raster_ftb, dim3((unsigned)((work + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, nv, work, rec, ext_int, pv, cnt, entries, stats);
+                     (hipStream_t)stream, views, v0, nv, work, rec, ext_int, pv, cnt, entries, kmax, stats);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
                                                   uint32_t n_views, uint32_t tile0, uint32_t ntiles,
                                                   uint32_t mode, const uint32_t* __restrict__ owner,
                                                   uint32_t* __restrict__ cnt,
-                                                  uint64_t* __restrict__ entries,
+                                                  uint64_t* __restrict__ entries, uint32_t kmax,
                                                   float4* __restrict__ rgba, uint32_t* __restrict__ steps,
                                                   uint32_t* __restrict__ entry_out,
                                                   uint64_t* __restrict__ rays,
@@ -480,10 +480,10 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
       } else {
         uint32_t nk = cnt[gp];
         active = nk != 0u;
-        if (nk > 1u) {
-          nk = nk > VFR_K ? VFR_K : nk;
+        uint64_t* ent = entries + (size_t)gp * kmax;
+        nk = nk > kmax ? kmax : nk;
+        if (nk > 1u && nk <= 8u) {
           uint64_t e[8];
-          uint64_t* ent = entries + (size_t)gp * VFR_K;
 #pragma unroll
           for (uint32_t k = 0; k < 8; ++k) e[k] = k < nk ? ent[k] : 0ull;
           CS(0, 1) CS(2, 3) CS(4, 5) CS(6, 7) CS(0, 2) CS(1, 3) CS(4, 6) CS(5, 7) CS(1, 2) CS(5, 6)
@@ -491,6 +491,14 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
 #pragma unroll
           for (uint32_t k = 0; k < 8; ++k)
             if (k < nk) ent[k] = e[k];
+        } else if (nk > 8u) {
+          /* rare long list: insertion sort in place, descending */
+          for (uint32_t k = 1; k < nk; ++k) {
+            const uint64_t key = ent[k];
+            uint32_t j = k;
+            while (j > 0 && ent[j - 1] < key) { ent[j] = ent[j - 1]; --j; }
+            ent[j] = key;
+          }
         }
       }
       if (!active) {
@@ -518,13 +526,30 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
 }
 
 int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, uint32_t ntiles,
-                  uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries,
+                  uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries, uint32_t kmax,
                   float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
                   uint32_t* n_rays, void* stream) {
   if (ntiles == 0) return VFR_OK;
   hipLaunchKernelGGL(k_compact, dim3((ntiles + COMPACT_WAVES - 1) / COMPACT_WAVES), dim3(1024), 0,
                      (hipStream_t)stream, views, n_views_total, tile0, ntiles, mode, owner, cnt, entries,
-                     rgba, steps, entry_out, rays, n_rays);
+                     kmax, rgba, steps, entry_out, rays, n_rays);
+  CHECK_HIP(hipGetLastError());
+  return VFR_OK;
+}
+
+__global__ void k_max_u32(const uint32_t* __restrict__ v, size_t n, uint32_t* __restrict__ out) {
+  uint32_t m = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    m = max(m, v[i]);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor(m, d, 64));
+  if ((threadIdx.x & 63u) == 0 && m) atomicMax(out, m);
+}
+
+int vfr_k_max_u32(const uint32_t* cnt, size_t n, uint32_t* out, void* stream) {
+  if (n == 0) return VFR_OK;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_max_u32, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, cnt, n, out);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -712,12 +737,15 @@ struct FtbRay {
   float ar, ag, ab, Tr;
   uint32_t t;              /* the tet, as an index into the record arena */
   uint32_t vi, voff, gp, tfo, steps;
-  uint32_t bits; /* cur 0..3 | nk 4..7 | persp 8 | anomaly 9 | broken 10 */
+  uint32_t bits; /* cur 0..5 | nk 6..11 | FTB_PERSP | FTB_DEAD | FTB_BROKEN */
 };
 
-DEV uint32_t ftb_cur(const FtbRay& r) { return r.bits & 15u; }
-DEV uint32_t ftb_nk(const FtbRay& r) { return (r.bits >> 4) & 15u; }
-DEV bool ftb_persp(const FtbRay& r) { return (r.bits >> 8) & 1u; }
+#define FTB_PERSP (1u << 12)  /* perspective view                        */
+#define FTB_DEAD (1u << 13)   /* ended as an anomaly (no exit face)      */
+#define FTB_BROKEN (1u << 14) /* ended at a neighbour with no back link  */
+DEV uint32_t ftb_cur(const FtbRay& r) { return r.bits & 63u; }
+DEV uint32_t ftb_nk(const FtbRay& r) { return (r.bits >> 6) & 63u; }
+DEV bool ftb_persp(const FtbRay& r) { return (r.bits & FTB_PERSP) != 0u; }
 
 /* u -> v edge function of the pixel (vertices relative to the pixel). */
 DEV float edge_rel(float4 u, float4 v) { return u.x * v.y - u.y * v.x; }
@@ -760,7 +788,7 @@ DEV float4 tf_at(const TravArgs& a, const float4* tfl, uint32_t i) {
  * entry face is degenerate at the pixel. */
 template <bool TFL>
 DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
-  const uint64_t key = a.entries[(size_t)r.gp * VFR_K + ftb_cur(r)];
+  const uint64_t key = a.entries[(size_t)r.gp * a.kmax + ftb_cur(r)];
   const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull) + a.views[r.vi].ext_off;
   const float4* __restrict__ pv = a.vert + r.voff;
   r.t = a.views[r.vi].rec_off + a.ext_int[2 * (size_t)e];
@@ -807,12 +835,12 @@ DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pi
   const float rayf = persp ? sqrtf(1.0f + (r.ox * r.ox + r.oy * r.oy) * (k * k)) : k;
   r.rf = rayf * a.views[vi].inv_me;
   uint32_t nk = a.cnt[r.gp];
-  nk = nk > VFR_K ? VFR_K : nk;
-  r.bits = (nk << 4) | (persp ? 256u : 0u);
+  nk = nk > a.kmax ? a.kmax : nk;
+  r.bits = (nk << 6) | (persp ? FTB_PERSP : 0u);
   r.ar = r.ag = r.ab = 0.0f;
   r.Tr = 1.0f;
   r.steps = 0;
-  if (!ftb_segment<TFL>(a, tfl, r)) r.bits |= 512u;
+  if (!ftb_segment<TFL>(a, tfl, r)) r.bits |= FTB_DEAD;
 }
 
 enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
@@ -945,9 +973,9 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const bool term = 1.0f - r.Tr >= a.term;
   if (skip || term || nb >= VFR_NB_BROKEN) {
     /* rare: the ray ends or leaves through the boundary */
-    if (skip) { r.bits |= 512u; return FTB_DONE; }
+    if (skip) { r.bits |= FTB_DEAD; return FTB_DONE; }
     if (term) return FTB_DONE;
-    if (nb == VFR_NB_BROKEN) { r.bits |= 1024u; return FTB_DONE; }
+    if (nb == VFR_NB_BROKEN) { r.bits |= FTB_BROKEN; return FTB_DONE; }
     return FTB_REENTER;
   }
   return FTB_GO;
@@ -961,10 +989,10 @@ DEV bool ftb_reenter(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const float exitZ = ftb_persp(r) ? 1.0f / r.prev : r.prev;
   uint32_t cur = ftb_cur(r) + 1u;
   const uint32_t nk = ftb_nk(r);
-  while (cur < nk && !(funkey((uint32_t)(a.entries[(size_t)r.gp * VFR_K + cur] >> 32)) < exitZ)) ++cur;
+  while (cur < nk && !(funkey((uint32_t)(a.entries[(size_t)r.gp * a.kmax + cur] >> 32)) < exitZ)) ++cur;
   if (cur >= nk) return false;
-  r.bits = (r.bits & ~15u) | cur;
-  if (!ftb_segment<TFL>(a, tfl, r)) { r.bits |= 512u; return false; }
+  r.bits = (r.bits & ~63u) | cur;
+  if (!ftb_segment<TFL>(a, tfl, r)) { r.bits |= FTB_DEAD; return false; }
   return true;
 }
 
@@ -972,9 +1000,9 @@ DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
   RayOut o;
   o.c = make_float4(r.ar, r.ag, r.ab, 1.0f - r.Tr);
   o.steps = r.steps > VFR_MAX_STEPS ? VFR_MAX_STEPS : r.steps;
-  o.entry = (uint32_t)(a.entries[(size_t)r.gp * VFR_K] & 0xFFFFFFFFull);
-  o.anomaly = (r.bits >> 9) & 1u;
-  o.broken = (r.bits >> 10) & 1u;
+  o.entry = (uint32_t)(a.entries[(size_t)r.gp * a.kmax] & 0xFFFFFFFFull);
+  o.anomaly = (r.bits & FTB_DEAD) ? 1u : 0u;
+  o.broken = (r.bits & FTB_BROKEN) ? 1u : 0u;
   return o;
 }
 
@@ -982,7 +1010,7 @@ template <int PROJ, bool BIG, bool TFL>
 DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pix) {
   FtbRay r;
   ftb_init<TFL>(a, tfl, vi, pix, r);
-  if (!(r.bits & 512u)) {
+  if (!(r.bits & FTB_DEAD)) {
     while (true) {
       const uint32_t st = ftb_step<PROJ, BIG, TFL>(a, tfl, r);
       if (st == FTB_GO || (st == FTB_REENTER && ftb_reenter<TFL>(a, tfl, r))) continue;
@@ -1074,7 +1102,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
           const uint64_t ray = a.rays[idx];
           ftb_init<false>(a, nullptr, (uint32_t)(ray >> 32), (uint32_t)ray, r);
           active = true;
-          if (r.bits & 512u) {
+          if (r.bits & FTB_DEAD) {
             /* degenerate entry face: the ray ends before its first step */
             const RayOut o = ftb_out(a, r);
             store_ray(a, r.gp, o);

@@ -190,9 +190,9 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   CHECK_RC(vfr_k_clear_u32(s->counters, 4, st));
   CHECK_RC(vfr_k_clear_u32((uint32_t*)s->stats, 2 * ST_N, st));
   CHECK_RC(vfr_k_xform_views(s->views, 0, nv, verts, sv->vert, s->pverts, st));
-  CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, faces, sv->rec, sv->ext, s->pverts, s->cnt, s->entries,
+  CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, faces, sv->rec, sv->ext, s->pverts, s->cnt, s->entries, VFR_K,
                             s->stats, st));
-  CHECK_RC(vfr_k_compact(s->views, nv, 0, (uint32_t)tiles, VFR_MODE_FTB, nullptr, s->cnt, s->entries,
+  CHECK_RC(vfr_k_compact(s->views, nv, 0, (uint32_t)tiles, VFR_MODE_FTB, nullptr, s->cnt, s->entries, VFR_K,
                          s->rgba, s->steps, s->entry, s->rays, s->counters, st));
   TravArgs a;
   memset(&a, 0, sizeof(a));
@@ -213,6 +213,7 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   a.queue = s->counters + 1;
   a.term = sv->cfg.term_alpha;
   a.refill = sv->cfg.refill;
+  a.kmax = VFR_K;
   if (tf_hi - tf_lo < VFR_TF_LDS_SLOTS) { a.tf_lds_base = tf_lo; a.tf_lds_n = tf_hi - tf_lo + 1; }
   const uint32_t proj = n_persp == 0 ? VFR_TRAV_ORTHO : (n_persp == nv ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
   CHECK_RC(vfr_k_traverse(VFR_MODE_FTB, proj, sv->n_records, sv->cfg.occupancy, a, (uint32_t)pix,
