@@ -171,6 +171,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched: skip the schedule autotuner (one ray per thread unless --persistent)")
     ap.add_argument("--persistent", action="store_true", help="persistent work-queue traversal kernel")
+    ap.add_argument("--pipelined", action="store_true",
+                    help="batched: view chunks whose preprocessing overlaps the walk of earlier chunks")
     ap.add_argument("--occupancy", type=int, default=0)
     ap.add_argument("--wpc", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
@@ -215,17 +217,18 @@ def main() -> int:
     total_rays = sum(v.width * v.height for v in views)
 
     tune = None
-    manual = args.persistent or args.occupancy or args.wpc or args.refill
+    manual = args.persistent or args.pipelined or args.occupancy or args.wpc or args.refill
     if not args.no_autotune and not manual:
         pt = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
         tune = pt.autotune()
         pt.close()
         log(f"autotune: {tune}")
-    kw = {} if tune is None else {"persistent": tune["persistent"], "waves_per_cu": tune["waves_per_cu"],
-                                  "refill": tune["refill"], "occupancy": tune["occupancy"]}
+    kw = {} if tune is None else {"persistent": tune["persistent"], "pipelined": tune["pipelined"],
+                                  "waves_per_cu": tune["waves_per_cu"], "refill": tune["refill"],
+                                  "occupancy": tune["occupancy"]}
     if tune is None and manual:
-        kw = {"persistent": args.persistent, "waves_per_cu": args.wpc, "refill": args.refill,
-              "occupancy": args.occupancy}
+        kw = {"persistent": args.persistent, "pipelined": args.pipelined, "waves_per_cu": args.wpc,
+              "refill": args.refill, "occupancy": args.occupancy}
         tune = dict(kw)
     pb = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, use_graph=args.graph, **kw)
     for _ in range(args.warmup):
@@ -238,8 +241,17 @@ def main() -> int:
     wall = (time.perf_counter() - t0) / args.steps
     log(f"batched: {wall * 1e3:.2f} ms/step")
     st = st_list[-1]
-    ms_trav = float(np.mean([s["ms_traversal"] for s in st_list])) if not args.graph else None
     ms_evt = float(np.mean([s["ms_total"] for s in st_list]))
+    # the walk kernel alone (one launch over every view) for the roofline
+    if not kw.get("pipelined") and not args.graph:
+        ms_trav = float(np.mean([s["ms_traversal"] for s in st_list]))
+    else:
+        kw1 = dict(kw, pipelined=False)
+        ps = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, **kw1)
+        ps.execute()
+        ms_trav = float(np.mean([ps.execute()["ms_traversal"] for _ in range(3)]))
+        ps.close()
+    log(f"walk launch alone: {ms_trav:.3f} ms")
 
     plain = None
     if not args.no_plain:
@@ -308,6 +320,7 @@ def main() -> int:
         "config": {"workload": args.workload, "tets": int(t.shape[0]), "views": nviews,
                    "rays_per_step": int(total_rays), "active_rays": int(rays_active),
                    "tet_steps": int(steps_per_launch), "term_alpha": 0.99, "graph": args.graph,
+                   "pipelined": bool(kw.get("pipelined")) and not args.graph,
                    "schedule": tune or {"persistent": False}},
         "anomalous_rays": int(st["anomalies"]),
         "event_ms_per_step": round(ms_evt, 3),

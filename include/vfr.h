@@ -161,6 +161,11 @@ typedef struct vfr_render_opts {
  * wave) unless this flag asks for the persistent work-queue kernel (a fixed
  * grid of waves that claims rays from the list and refills idle lanes). */
 #define VFR_RENDER_PERSISTENT 0x1u
+/* Batched front-to-back renders (without a graph) run all preprocessing and
+ * then one walk launch over every view, unless this flag asks for view
+ * chunks whose preprocessing overlaps the walk of the chunks before them on
+ * a second stream. */
+#define VFR_RENDER_PIPELINED 0x2u
 
 typedef struct vfr_frame_out {
   float* rgba;          /* W*H*4, may be NULL                              */
@@ -194,9 +199,10 @@ int vfr_plan_frame_times(const vfr_plan* plan, double* out_ms);
 
 /* Times the one-ray-per-thread traversal and candidate persistent-kernel
  * schedules (kernel occupancy variant x waves per CU x refill threshold) on
- * the plan itself and keeps the fastest; returns the choice (flags holds
- * VFR_RENDER_PERSISTENT when the persistent kernel won) and its traversal
- * time.  Results are cached per (tets, rays, views) in the process. */
+ * the plan itself and keeps the fastest walk, then keeps the pipelined
+ * execution if it is faster end to end; returns the choice (flags holds
+ * VFR_RENDER_PERSISTENT / VFR_RENDER_PIPELINED) and the walk time.  Results
+ * are cached per (tets, rays, views) in the process. */
 int vfr_plan_autotune(vfr_plan* plan, uint32_t* flags, uint32_t* occupancy,
                       uint32_t* waves_per_cu, uint32_t* refill, double* best_ms);
 

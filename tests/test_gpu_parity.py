@@ -138,13 +138,16 @@ def test_batched_equals_per_view(ctx):
     c, sc = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, persistent=True)
     d, sd = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, persistent=True, waves_per_cu=4,
                        refill=1)
-    assert sa["steps"] == sb["steps"] == sc["steps"] == sd["steps"]
+    e, se = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, pipelined=True)
+    assert sa["steps"] == sb["steps"] == sc["steps"] == sd["steps"] == se["steps"]
+    assert sa["active_rays"] == sb["active_rays"] == se["active_rays"]
     for i, v in enumerate(views):
         alone, _ = ctx.render(m, [v], mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
         for key in ("rgba", "entry_ext", "steps"):
             assert np.array_equal(a[i][key], b[i][key])
             assert np.array_equal(a[i][key], c[i][key])
             assert np.array_equal(a[i][key], d[i][key])
+            assert np.array_equal(a[i][key], e[i][key])
             assert np.array_equal(a[i][key], alone[0][key])
     m.close()
 

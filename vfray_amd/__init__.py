@@ -18,6 +18,7 @@ VFR_OK = 0
 PROJ_ORTHO, PROJ_PERSPECTIVE = 0, 1
 MODE_REF, MODE_FTB = 0, 1
 RENDER_PERSISTENT = 0x1
+RENDER_PIPELINED = 0x2
 SCHED_PER_VIEW, SCHED_BATCHED = 0, 1
 MESH_NORMALIZE, MESH_MORTON = 0x1, 0x2
 NO_ID = 0xFFFFFFFF
@@ -345,9 +346,9 @@ class Context:
 
     def plan(self, mesh: "Mesh", views, mode=MODE_FTB, schedule=SCHED_BATCHED, term_alpha=0.99,
              use_graph=False, waves_per_cu=0, refill=0, persistent=False, occupancy=0,
-             max_entries=0) -> "Plan":
+             max_entries=0, pipelined=False) -> "Plan":
         return Plan(self, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
-                    persistent, occupancy, max_entries)
+                    persistent, occupancy, max_entries, pipelined)
 
     def render(self, mesh: "Mesh", views, **kw) -> tuple[list[dict], dict]:
         p = self.plan(mesh, views, **kw)
@@ -395,20 +396,21 @@ class Mesh:
 
 class Plan:
     def __init__(self, ctx, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
-                 persistent=False, occupancy=0, max_entries=0):
+                 persistent=False, occupancy=0, max_entries=0, pipelined=False):
         self.ctx, self.mesh = ctx, mesh
         self.views = list(views)
         arr = (View * len(self.views))(*self.views)
+        flags = (RENDER_PERSISTENT if persistent else 0) | (RENDER_PIPELINED if pipelined else 0)
         o = RenderOpts(mode, schedule, term_alpha, waves_per_cu, refill, 1 if use_graph else 0, max_entries,
-                       RENDER_PERSISTENT if persistent else 0, occupancy, 0)
+                       flags, occupancy, 0)
         self._p = C.c_void_p()
         _check(lib().vfr_plan_create(ctx._p, mesh._p, arr, len(self.views), C.byref(o), C.byref(self._p)))
 
     def autotune(self) -> dict:
         f, o, w, r, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_double()
         _check(lib().vfr_plan_autotune(self._p, C.byref(f), C.byref(o), C.byref(w), C.byref(r), C.byref(ms)))
-        return {"persistent": bool(f.value & RENDER_PERSISTENT), "occupancy": o.value, "waves_per_cu": w.value,
-                "refill": r.value, "traversal_ms": ms.value}
+        return {"persistent": bool(f.value & RENDER_PERSISTENT), "pipelined": bool(f.value & RENDER_PIPELINED),
+                "occupancy": o.value, "waves_per_cu": w.value, "refill": r.value, "traversal_ms": ms.value}
 
     def execute(self) -> dict:
         st = Stats()

@@ -308,6 +308,15 @@ struct vfr_plan {
   hipGraphExec_t gexec = nullptr;
   bool executed = false;
   std::vector<double> frame_ms;           /* last PER_VIEW execute, per view */
+  /* pipelined batched execution: view chunks whose preprocessing (stream 1)
+   * overlaps the walk of the chunks before them (stream 2) */
+  struct Chunk {
+    uint32_t v0, nv, tile0, ntiles, proj;
+    uint64_t pix0, npix, nverts, nfaces;
+  };
+  std::vector<Chunk> chunks;
+  hipStream_t st2 = nullptr;
+  std::vector<hipEvent_t> cev;            /* per chunk: ready, walk start, walk end; + join */
 };
 
 /* View set-up on the host, in double, rounded once to float.  bb_lo/bb_hi:
@@ -391,6 +400,12 @@ extern "C" void vfr_plan_destroy(vfr_plan* p) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : p->fev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : p->cev)
+    if (e) (void)hipEventDestroy(e);
+  if (p->st2) {
+    (void)hipStreamSynchronize(p->st2);
+    (void)hipStreamDestroy(p->st2);
+  }
   delete p;
 }
 
@@ -561,7 +576,7 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   PALLOC(p->rgba, sizeof(float4) * pix);
   PALLOC(p->steps, sizeof(uint32_t) * pix);
   PALLOC(p->entry, sizeof(uint32_t) * pix);
-  PALLOC(p->counters, sizeof(uint32_t) * (n_views + 2));
+  PALLOC(p->counters, sizeof(uint32_t) * (n_views + 2 + VFR_PIPE_CHUNKS));
   PALLOC(p->stats, sizeof(unsigned long long) * ST_N);
 #undef PALLOC
   if (rc == VFR_OK && hipMemcpy(p->views, p->hv.data(), sizeof(DevView) * n_views, hipMemcpyHostToDevice) != hipSuccess) {
@@ -589,6 +604,51 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   if (rc == VFR_OK && o.schedule == VFR_SCHED_PER_VIEW) {
     p->fev.assign(2 * (size_t)n_views, nullptr);
     for (auto& e : p->fev)
+      if (rc == VFR_OK && hipEventCreate(&e) != hipSuccess) { vfr_set_error("event creation failed"); rc = VFR_EHIP; }
+  }
+  if (rc == VFR_OK && o.schedule == VFR_SCHED_BATCHED && o.mode == VFR_MODE_FTB && !o.use_graph &&
+      n_views >= 2) {
+    /* chunks of consecutive views with about equal pixel counts */
+    const uint32_t nc = std::min<uint32_t>(VFR_PIPE_CHUNKS, (uint32_t)n_views);
+    uint32_t v = 0;
+    uint64_t done = 0;
+    for (uint32_t ci = 0; ci < nc && v < (uint32_t)n_views; ++ci) {
+      vfr_plan::Chunk ch{};
+      ch.v0 = v;
+      ch.pix0 = p->hv[v].pix_off;
+      ch.tile0 = p->hv[v].tile_off;
+      const uint64_t goal = (pix * (ci + 1)) / nc;
+      uint32_t persp = 0;
+      while (v < (uint32_t)n_views && (v == ch.v0 || done < goal) && (uint32_t)n_views - v > nc - 1 - ci) {
+        const uint64_t vp = (uint64_t)views[v].width * views[v].height;
+        done += vp;
+        ch.npix += vp;
+        ch.ntiles += p->hv[v].tiles_x * ((views[v].height + 7) / 8);
+        persp += views[v].projection == VFR_PROJ_PERSPECTIVE ? 1u : 0u;
+        ++v;
+        ++ch.nv;
+      }
+      if (ci == nc - 1)
+        while (v < (uint32_t)n_views) {
+          const uint64_t vp = (uint64_t)views[v].width * views[v].height;
+          done += vp;
+          ch.npix += vp;
+          ch.ntiles += p->hv[v].tiles_x * ((views[v].height + 7) / 8);
+          persp += views[v].projection == VFR_PROJ_PERSPECTIVE ? 1u : 0u;
+          ++v;
+          ++ch.nv;
+        }
+      ch.nverts = (uint64_t)ch.nv * m->d.n_verts;
+      ch.nfaces = (uint64_t)ch.nv * m->d.n_ext;
+      ch.proj = persp == 0 ? VFR_TRAV_ORTHO : (persp == ch.nv ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
+      p->chunks.push_back(ch);
+    }
+    if (hipStreamCreateWithFlags(&p->st2, hipStreamNonBlocking) != hipSuccess) {
+      vfr_set_error("stream creation failed");
+      rc = VFR_EHIP;
+    }
+    p->cev.assign(3 * p->chunks.size() + 1, nullptr);
+    for (auto& e : p->cev)
       if (rc == VFR_OK && hipEventCreate(&e) != hipSuccess) { vfr_set_error("event creation failed"); rc = VFR_EHIP; }
   }
   if (rc == VFR_OK && o.schedule == VFR_SCHED_BATCHED && o.use_graph) {
@@ -621,13 +681,65 @@ static double pct(std::vector<double> v, double q) {
   return v[lo] + (pos - lo) * (v[hi] - v[lo]);
 }
 
+/* Batched execution in view chunks: chunk c's transform, raster and
+ * compaction run on the context stream, its walk on the plan's second
+ * stream once they are done, so the preprocessing of chunk c+1 overlaps the
+ * walk of chunk c.  ms_trav sums the walk launches (HIP events on the second
+ * stream). */
+static int execute_pipelined(vfr_plan* p, double* ms_total, double* ms_vis, double* ms_trav) {
+  hipStream_t st = p->ctx->stream, s2 = p->st2;
+  const vfr_mesh* m = p->mesh;
+  const size_t nc = p->chunks.size();
+  uint32_t* ccount = p->counters + p->n_views + 2;
+  CHECK_HIP(hipEventRecord(p->ev[0], st));
+  CHECK_RC(vfr_k_clear_u32(p->owner, p->n_pix, st));
+  CHECK_RC(vfr_k_clear_u32(p->counters, p->n_views + 2 + (uint32_t)nc, st));
+  CHECK_RC(vfr_k_clear_u32((uint32_t*)p->stats, 2 * ST_N, st));
+  for (size_t c = 0; c < nc; ++c) {
+    const vfr_plan::Chunk& ch = p->chunks[c];
+    CHECK_RC(vfr_k_xform_views(p->views, ch.v0, ch.nv, ch.nverts, (const float4*)m->d.vert, p->pverts, st));
+    CHECK_RC(vfr_k_raster_ftb(p->views, ch.v0, ch.nv, ch.nfaces, m->d.rec, m->d.ext_int, p->pverts, p->owner,
+                              p->entries, p->kmax, p->stats, st));
+    CHECK_RC(vfr_k_compact(p->views, p->n_views, ch.tile0, ch.ntiles, p->opts.mode, p->owner, p->owner,
+                           p->entries, p->kmax, p->rgba, p->steps, p->entry, p->rays + ch.pix0, ccount + c, st));
+    CHECK_HIP(hipEventRecord(p->cev[3 * c], st));
+    CHECK_HIP(hipStreamWaitEvent(s2, p->cev[3 * c], 0));
+    CHECK_HIP(hipEventRecord(p->cev[3 * c + 1], s2));
+    TravArgs a = trav_args(p, p->rays + ch.pix0, ccount + c);
+    a.queue = p->counters + p->n_views + 1;
+    const bool persist = (p->opts.flags & VFR_RENDER_PERSISTENT) != 0;
+    if (persist) CHECK_RC(vfr_k_clear_u32(a.queue, 1, s2));
+    CHECK_RC(vfr_k_traverse(p->opts.mode, ch.proj, m->d.n_tets, p->opts.occupancy, a, (uint32_t)ch.npix,
+                            persist ? 1 : 0, p->opts.waves_per_cu * (uint32_t)p->ctx->n_cu, s2));
+    CHECK_HIP(hipEventRecord(p->cev[3 * c + 2], s2));
+  }
+  CHECK_HIP(hipEventRecord(p->cev[3 * nc], s2));
+  CHECK_HIP(hipStreamWaitEvent(st, p->cev[3 * nc], 0));
+  CHECK_HIP(hipEventRecord(p->ev[2], st));
+  CHECK_HIP(hipEventSynchronize(p->ev[2]));
+  float tot = 0;
+  CHECK_HIP(hipEventElapsedTime(&tot, p->ev[0], p->ev[2]));
+  double trav = 0;
+  for (size_t c = 0; c < nc; ++c) {
+    float w = 0;
+    CHECK_HIP(hipEventElapsedTime(&w, p->cev[3 * c + 1], p->cev[3 * c + 2]));
+    trav += w;
+  }
+  *ms_total = tot;
+  *ms_trav = trav;
+  *ms_vis = std::max(0.0, (double)tot - trav);
+  return VFR_OK;
+}
+
 extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
   if (!p) { vfr_set_error("vfr_plan_execute: null plan"); return VFR_EINVAL; }
   vfr_ctx* c = p->ctx;
   hipStream_t st = c->stream;
   CHECK_HIP(hipSetDevice(c->device));
   double ms_total = 0, ms_vis = 0, ms_trav = 0, p50 = 0, p99 = 0;
-  if (p->opts.schedule == VFR_SCHED_BATCHED) {
+  if (p->opts.schedule == VFR_SCHED_BATCHED && !p->chunks.empty() && (p->opts.flags & VFR_RENDER_PIPELINED)) {
+    CHECK_RC(execute_pipelined(p, &ms_total, &ms_vis, &ms_trav));
+  } else if (p->opts.schedule == VFR_SCHED_BATCHED) {
     CHECK_HIP(hipEventRecord(p->ev[0], st));
     if (p->gexec) {
       CHECK_HIP(hipGraphLaunch(p->gexec, st));
@@ -699,8 +811,16 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
     CHECK_HIP(hipMemcpy(s, p->stats, sizeof(s), hipMemcpyDeviceToHost));
     CHECK_HIP(hipMemcpy(cnt.data(), p->counters, sizeof(uint32_t) * cnt.size(), hipMemcpyDeviceToHost));
     uint64_t active = 0;
-    if (p->opts.schedule == VFR_SCHED_BATCHED) active = cnt[p->n_views];
-    else for (uint32_t v = 0; v < p->n_views; ++v) active += cnt[v];
+    if (p->opts.schedule == VFR_SCHED_BATCHED && !p->chunks.empty() && (p->opts.flags & VFR_RENDER_PIPELINED)) {
+      std::vector<uint32_t> cc(p->chunks.size());
+      CHECK_HIP(hipMemcpy(cc.data(), p->counters + p->n_views + 2, sizeof(uint32_t) * cc.size(),
+                          hipMemcpyDeviceToHost));
+      for (uint32_t x : cc) active += x;
+    } else if (p->opts.schedule == VFR_SCHED_BATCHED) {
+      active = cnt[p->n_views];
+    } else {
+      for (uint32_t v = 0; v < p->n_views; ++v) active += cnt[v];
+    }
     memset(stats, 0, sizeof(*stats));
     stats->rays = p->n_pix;
     stats->active_rays = active;
@@ -789,7 +909,7 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
     std::lock_guard<std::mutex> g(g_tune_mu);
     for (auto& e : g_tune)
       if (e.first == key) {
-        p->opts.flags = (p->opts.flags & ~VFR_RENDER_PERSISTENT) | e.second.flags;
+        p->opts.flags = (p->opts.flags & ~(VFR_RENDER_PERSISTENT | VFR_RENDER_PIPELINED)) | e.second.flags;
         p->opts.occupancy = e.second.occ;
         p->opts.waves_per_cu = e.second.wpc;
         p->opts.refill = e.second.refill;
@@ -804,7 +924,10 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
     static const uint32_t kOcc[] = {0, 6};
     static const uint32_t kMul[] = {1, 2};
     static const uint32_t kR[] = {16, 64};
-    const uint32_t base_flags = p->opts.flags & ~VFR_RENDER_PERSISTENT;
+    /* each candidate's walk is timed alone: all preprocessing first, then
+     * one walk launch */
+    const uint32_t user_flags = p->opts.flags & ~(VFR_RENDER_PERSISTENT | VFR_RENDER_PIPELINED);
+    const uint32_t base_flags = user_flags;
     best = 1e30;
     TuneVal bv{0u, p->opts.occupancy, p->opts.waves_per_cu, p->opts.refill};
     vfr_stats st;
@@ -836,15 +959,30 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
           if (rc == VFR_OK && t < best) { best = t; bv = TuneVal{VFR_RENDER_PERSISTENT, occ, 4 * wps * mul, r}; }
         }
     }
-    p->opts.flags = base_flags | bv.flags;
+    p->opts.flags = user_flags | bv.flags;
     p->opts.occupancy = bv.occ;
     p->opts.waves_per_cu = bv.wpc;
     p->opts.refill = bv.refill;
+    if (rc == VFR_OK && !p->chunks.empty()) {
+      /* end to end: one walk after all preprocessing, or view chunks */
+      double serial = 1e30, piped = 1e30;
+      for (int rep = 0; rc == VFR_OK && rep < 2; ++rep) {
+        rc = vfr_plan_execute(p, &st);
+        serial = std::min(serial, st.ms_total);
+      }
+      p->opts.flags |= VFR_RENDER_PIPELINED;
+      for (int rep = 0; rc == VFR_OK && rep < 3; ++rep) {
+        rc = vfr_plan_execute(p, &st);
+        if (rep > 0) piped = std::min(piped, st.ms_total);
+      }
+      if (piped < serial) bv.flags |= VFR_RENDER_PIPELINED;
+      p->opts.flags = user_flags | bv.flags;
+    }
     if (rc != VFR_OK) return rc;
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tune.push_back({key, bv});
   }
-  if (flags_out) *flags_out = p->opts.flags & VFR_RENDER_PERSISTENT;
+  if (flags_out) *flags_out = p->opts.flags & (VFR_RENDER_PERSISTENT | VFR_RENDER_PIPELINED);
   if (occ_out) *occ_out = p->opts.occupancy;
   if (wpc_out) *wpc_out = p->opts.waves_per_cu;
   if (refill_out) *refill_out = p->opts.refill;

@@ -30,6 +30,9 @@ double vfr_psi_integral(double tf, double tb);
 #define VFR_BIG_RECORDS (1ull << 26)
 #define VFR_MAX_VIEW_VERTS (1ull << 28)
 
+/* View chunks of a pipelined batched render. */
+#define VFR_PIPE_CHUNKS 4u
+
 /* Transfer-function slots a traversal launch can stage in LDS (4 KB each). */
 #define VFR_TF_LDS_SLOTS 4u
 
