@@ -71,6 +71,53 @@ def cpu_baseline(x, t, tfa, views, seconds: float) -> dict:
                       f"restatement, FP32, OpenMP {threads} threads, {el:.1f} s"}
 
 
+def reference_cpu_c1(ctx, reps: int = 5):
+    """The reference's own CPU path (vfRayCPU built from its sources by
+    oracle/Makefile with its -O3 -ffast-math flags, single-threaded) on its
+    native case, the 16^3 grid (24,576 tets) at 512x512 orthographic, timed
+    beside this renderer's REF mode on the same input.  Kernel times are the
+    ones the reference prints (clock(), whole milliseconds); None when the
+    compiled reference is not present."""
+    import re
+    import subprocess
+    import tempfile
+    import vfray_amd as V
+    from vfray_amd import workloads as WL
+    exe = os.path.join(ROOT, "oracle", "_ref", "fast", "vfRayCPU")
+    if not os.path.exists(exe):
+        return None
+    x, t, tfa, side = WL.plain_cpu()
+    with tempfile.TemporaryDirectory() as td:
+        os.makedirs(os.path.join(td, "tet_offs"))
+        os.makedirs(os.path.join(td, "run"))
+        V.write_off(os.path.join(td, "tet_offs", "c1.off"), x, t)
+        V.write_tf(os.path.join(td, "tet_offs", "c1.tf"), tfa)
+        out = ""
+        for _ in range(2):  # the first run builds the cache files
+            r = subprocess.run([exe, "c1"], cwd=os.path.join(td, "run"), capture_output=True, text=True,
+                               timeout=300)
+            if r.returncode != 0:
+                return None
+            out = r.stdout
+    # kernels 1-3 and the host compaction between 1 and 2 (the "Prepare" lines are allocations)
+    ms = [int(m.group(1)) for line in out.splitlines() if "Prepare" not in line
+          for m in [re.search(r"Processing time: (\d+) \(ms\)", line)] if m]
+    cpu_ms = float(sum(ms))
+    mesh = ctx.mesh(x, t)
+    tf = ctx.tf(tfa)
+    p = ctx.plan(mesh, [V.ortho_view(side, side, tf)], mode=V.MODE_REF, schedule=V.SCHED_BATCHED)
+    p.execute()
+    gms = float(np.median([p.execute()["ms_total"] for _ in range(reps)]))
+    p.close()
+    mesh.close()
+    rays = side * side
+    return {"workload": "16^3 grid x 6 = 24,576 tets, 512x512 orthographic, REF mode (the reference's case)",
+            "reference_cpu_ms": cpu_ms, "reference_cpu_mrays_per_s": round(rays / cpu_ms / 1e3, 3),
+            "reference_cores": 1, "reference_build": "vfRayCPU, -O3 -ffast-math (MakefileCPU:18)",
+            "gpu_ms": round(gms, 4), "gpu_mrays_per_s": round(rays / gms / 1e3, 1),
+            "speedup": round(cpu_ms / gms, 1)}
+
+
 def measured_traffic(kernel: str, workload: str):
     """HBM/fabric bytes per traversal launch from the committed rocprofv3
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads
@@ -299,10 +346,14 @@ def main() -> int:
                 "l2_hit": tr["l2_hit"] if tr else None,
                 "kernel": kname, "kernel_ms": round(ms_trav, 3),
                 "alg_bytes": int(alg_bytes)}
-    cpu = None
+    cpu = ref_c1 = None
     if not args.no_cpu:
         cpu = cpu_baseline(x, t, tfa, views, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']:.3f} Mrays/s")
+        ref_c1 = reference_cpu_c1(ctx)
+        if ref_c1:
+            log(f"reference CPU path on its own case: {ref_c1['reference_cpu_ms']:.0f} ms; "
+                f"REF mode on the GPU: {ref_c1['gpu_ms']:.3f} ms")
     value = total_rays / wall / 1e6
     out = {
         "metric": "Mrays/s (batched renderer, front-to-back, PSNR>=50dB vs FP64)",
@@ -330,6 +381,7 @@ def main() -> int:
         "speedup_vs_plain": round(value / plain["mrays_per_s"], 3) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "reference_cpu_c1": ref_c1,
     }
     print(json.dumps(out), flush=True)
     pb.close()

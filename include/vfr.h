@@ -286,6 +286,9 @@ int vfr_write_extf(const char* path, const uint32_t* extf, uint32_t n_ext); /* :
 int vfr_read_tf(const char* path, float rgba_out[1024]);       /* :551-590 */
 int vfr_write_tf(const char* path, const float rgba[1024]);    /* :628-660 */
 void vfr_default_tf(float rgba_out[1024]);                     /* :595-622 */
+/* cleanTF (cleanTF.cc:14-78): legacy TF file (7 header lines, 256 lines
+ * "index r g b a") to the .tf text. */
+int vfr_clean_tf(const char* legacy_path, const char* tf_path);
 int vfr_read_lmt(const char* path, float lmt_out[3]);          /* :782-803 */
 int vfr_write_lmt(const char* path, const float lmt[3]);       /* :859-880 */
 void vfr_build_lmt(const float* xyzs, uint32_t n_verts, const uint32_t* tets,

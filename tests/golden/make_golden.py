@@ -12,6 +12,10 @@ Run in a container that has the reference tree; the fixtures are data only.
   psi_ref.json    where the reference's 512x512 psi table departs from the
                   closed form by more than 1e-6 (index pairs and values), the
                   largest departure elsewhere, sha256 of the table.
+  cleantf.json    sha256 of what the reference's cleanTF writes for the
+                  seeded legacy TF file of tests/helpers.legacy_tf_text().
+
+  python3 tests/golden/make_golden.py [cleantf]   (all fixtures, or one)
 """
 from __future__ import annotations
 
@@ -28,10 +32,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import oracle as O  # noqa: E402
 import vfray_amd as V  # noqa: E402
 from vfray_amd import workloads as WL  # noqa: E402
+from helpers import legacy_tf_text  # noqa: E402
 
 REF = "/root/reference"
 
@@ -56,7 +62,24 @@ def dataset(td: str, name: str, n: int):
     return x, t, tf
 
 
+def cleantf() -> None:
+    exe = os.path.join(ROOT, "oracle", "_ref", "cleanTF")
+    assert os.path.exists(exe), "build oracle/_ref first"
+    with tempfile.TemporaryDirectory() as td:
+        src, dst = os.path.join(td, "old.off.tf"), os.path.join(td, "new.tf")
+        open(src, "w").write(legacy_tf_text())
+        import subprocess
+        subprocess.run([exe, src, dst], check=True, capture_output=True)
+        b = open(dst, "rb").read()
+    json.dump({"sha256": hashlib.sha256(b).hexdigest(), "bytes": len(b)},
+              open(os.path.join(HERE, "cleantf.json"), "w"), indent=1)
+
+
 def main() -> None:
+    if sys.argv[1:] == ["cleantf"]:
+        cleantf()
+        return
+    cleantf()
     assert O.ref_available("strict") and O.ref_available("fast"), "build oracle/_ref first"
     out = {}
     with tempfile.TemporaryDirectory() as td:

@@ -39,3 +39,15 @@ def oracle_ftb(inp, view: V.View, psi, term=0.99, fp64=False):
     proj, M, k = V.view_matrix(view)
     return O.render_ftb(inp["xn"], inp["t"], inp["con"], inp["ext"], inp["tf"], psi, inp["lmt"],
                         proj, M, k, view.width, view.height, term=term, fp64=fp64)
+
+
+def legacy_tf_text(seed: int = 7) -> str:
+    """A legacy transfer-function file as cleanTF reads it: 7 header lines,
+    then 256 lines "index r g b a" with 17-digit values (seeded)."""
+    rng = np.random.default_rng(seed)
+    head = ["# legacy transfer function", "# produced for the tests", "format 1", "entries 256",
+            "columns index r g b a", "range 0 1", "#"]
+    vals = rng.random((256, 4))
+    vals[rng.random((256, 4)) < 0.1] = 0.0
+    lines = [f"{i} " + " ".join(repr(float(v)) for v in vals[i]) for i in range(256)]
+    return "\n".join(head + lines) + "\n"

@@ -9,6 +9,9 @@ import pytest
 
 import vfray_amd as V
 from vfray_amd import workloads as WL
+from helpers import legacy_tf_text
+import hashlib
+import json
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "vfray_amd", "vfRay")
@@ -89,3 +92,53 @@ def test_cli_image_matches_strict_reference():
         img = ppm[15:].reshape(128, 128, 3).astype(int)
         diff = np.abs(img - gold["rgb128_strict"].astype(int))
         assert diff.max() <= 1 and (diff > 0).any(-1).mean() < 0.01
+
+
+CLEANTF = os.path.join(ROOT, "vfray_amd", "cleanTF")
+
+
+def test_cleantf_matches_reference():
+    """cleanTF (cleanTF.cc:14-78): the legacy file's 256 entries come out as
+    the .tf text byte for byte as the reference's converter writes them (sha256
+    recorded from it), read back by vfr_read_tf."""
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "cleantf.json")))
+    with tempfile.TemporaryDirectory() as td:
+        src, dst = os.path.join(td, "old.off.tf"), os.path.join(td, "new.tf")
+        open(src, "w").write(legacy_tf_text())
+        r = subprocess.run([CLEANTF, src, dst], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, r.stderr
+        out = open(dst, "rb").read()
+        assert hashlib.sha256(out).hexdigest() == gold["sha256"]
+        ref = os.path.join(ROOT, "oracle", "_ref", "cleanTF")
+        if os.path.exists(ref):
+            dref = os.path.join(td, "ref.tf")
+            subprocess.run([ref, src, dref], capture_output=True, timeout=60, check=True)
+            assert open(dref, "rb").read() == out
+
+
+def test_cleantf_errors():
+    """Usage error: exit 1; unreadable input: a message, exit 0 and no output,
+    as the reference's converter."""
+    with tempfile.TemporaryDirectory() as td:
+        r = subprocess.run([CLEANTF], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and "use:" in r.stderr
+        dst = os.path.join(td, "new.tf")
+        r = subprocess.run([CLEANTF, os.path.join(td, "missing.tf"), dst], capture_output=True, text=True,
+                           timeout=60)
+        assert r.returncode == 0 and not os.path.exists(dst)
+        # a header line of 200 characters stops the reader; 199 do not
+        for n, ok in ((199, True), (200, False)):
+            lines = legacy_tf_text().split("\n")
+            lines[2] = "x" * n
+            src = os.path.join(td, f"h{n}.tf")
+            open(src, "w").write("\n".join(lines))
+            out = os.path.join(td, f"h{n}.out")
+            r = subprocess.run([CLEANTF, src, out], capture_output=True, timeout=60)
+            assert r.returncode == 0 and os.path.exists(out) == ok
+            ref = os.path.join(ROOT, "oracle", "_ref", "cleanTF")
+            if os.path.exists(ref):
+                rout = os.path.join(td, f"h{n}.ref")
+                subprocess.run([ref, src, rout], capture_output=True, timeout=60)
+                assert os.path.exists(rout) == ok
+                if ok:
+                    assert open(rout, "rb").read() == open(out, "rb").read()

@@ -63,6 +63,10 @@ def build_lib(verbose: bool = True) -> str:
     _run(["g++", *CXX_FLAGS, "-o", cli + ".tmp", os.path.join(CSRC, "vfray_main.cpp"),
           f"-L{PKG}", "-lvfr", "-Wl,-rpath,$ORIGIN"])
     os.replace(cli + ".tmp", cli)
+    ctf = os.path.join(PKG, "cleanTF")
+    _run(["g++", *CXX_FLAGS, "-o", ctf + ".tmp", os.path.join(CSRC, "cleantf_main.cpp"),
+          f"-L{PKG}", "-lvfr", "-Wl,-rpath,$ORIGIN"])
+    os.replace(ctf + ".tmp", ctf)
     if verbose:
         print(f"built {lib} and {cli}")
     return lib

@@ -327,6 +327,41 @@ extern "C" int vfr_write_tf(const char* path, const float rgba[1024]) {
   return fclose(f) == 0 ? VFR_OK : VFR_EIO;
 }
 
+/* cleanTF, cleanTF.cc:14-78: a legacy transfer-function file (7 header
+ * lines, each read into a 200-character buffer, then 256 lines
+ * "index r g b a") rewritten as the .tf text: "256", then "r g b a" per
+ * entry, printed from doubles with 6 significant digits. */
+extern "C" int vfr_clean_tf(const char* in_path, const char* out_path) {
+  if (!in_path || !out_path) { vfr_set_error("vfr_clean_tf: null argument"); return VFR_EINVAL; }
+  FILE* f = fopen(in_path, "rb");
+  if (!f) { vfr_set_error("cannot open %s", in_path); return VFR_EIO; }
+  for (int l = 0; l < 7; ++l) {
+    /* a header line of 200 characters or more fails the read */
+    int c, len = 0;
+    while ((c = fgetc(f)) != EOF && c != '\n') ++len;
+    if (len >= 200 || (c == EOF && l < 6)) {
+      fclose(f);
+      vfr_set_error("%s: bad header line %d", in_path, l + 1);
+      return VFR_EIO;
+    }
+  }
+  double tf[256][4];
+  for (int i = 0; i < 256; ++i) {
+    int index;
+    if (fscanf(f, "%d %lf %lf %lf %lf", &index, &tf[i][0], &tf[i][1], &tf[i][2], &tf[i][3]) != 5) {
+      fclose(f);
+      vfr_set_error("%s: bad entry %d", in_path, i);
+      return VFR_EIO;
+    }
+  }
+  fclose(f);
+  FILE* o = fopen(out_path, "w");
+  if (!o) { vfr_set_error("cannot write %s", out_path); return VFR_EIO; }
+  fprintf(o, "256\n");
+  for (int i = 0; i < 256; ++i) fprintf(o, "%g %g %g %g\n", tf[i][0], tf[i][1], tf[i][2], tf[i][3]);
+  return fclose(o) == 0 ? VFR_OK : VFR_EIO;
+}
+
 /* buildTF, vfRayPreComp.cc:595-622: four bands of 64 entries (red, yellow,
  * cyan, blue), alpha 0.2. */
 extern "C" void vfr_default_tf(float rgba[1024]) {
