@@ -109,6 +109,15 @@ int vfr_mesh_connectivity(const vfr_mesh* mesh, uint32_t* con_out,
                           uint32_t* extf_out, float* lmt_out);
 /* Copies the (possibly normalized) vertices back, caller order, nV*4. */
 int vfr_mesh_vertices(const vfr_mesh* mesh, float* xyzs_out);
+/* The CUDA viewer's re-render step (vfRay.cu:618-640): rotateVolume
+ * (vfRayPreComp.cc:886-906, mv read row-major) on the resident vertices,
+ * then, with VFR_MESH_NORMALIZE, normalizeVertices (:84-145), both on the
+ * device and bit-exact with the host functions below.  Connectivity and the
+ * limits (.lmt) are kept, as the viewer keeps them (its maxEdgeLength goes
+ * stale once the renormalization rescales the mesh; REF mode reproduces
+ * that).  Plans created before the call return VFR_ESTATE until created
+ * again; servers hold their own copy of the mesh. */
+int vfr_mesh_rotate(vfr_mesh* mesh, const float mv[16], uint32_t flags);
 
 /* ---------------------------------------------------------------- render */
 #define VFR_PROJ_ORTHO 0u       /* X,Y,Z = A*(x,y,z)+b, rays march -Z      */
@@ -153,8 +162,23 @@ typedef struct vfr_render_opts {
   uint32_t occupancy;   /* persistent kernel variant: waves per SIMD (4, 5,
                            6: register-capped; other values: the compiler's
                            choice); tuned by vfr_plan_autotune            */
-  uint32_t reserved;
+  uint32_t composite;   /* VFR_COMPOSITE_* (REF mode; FTB takes PPI only)  */
 } vfr_render_opts;
+
+/* Segment compositing.  PPI: partial pre-integration through the psi table
+ * (the reference's USEPARTIALPREINT 1 build, vfRayKernelCPU.cc:448-482).
+ * LINEAR: the USEPARTIALPREINT 0 build's linear integration of 0..255
+ * colours, truncated to integers and clamped at 255 every step, dz in object
+ * units (vfRayKernel.cu:738-806 as the reference builds it, USEOPACITY
+ * unset: no opacity, no early termination; the CPU version at
+ * vfRayKernelCPU.cc:484-532 never stores its colour and renders black).
+ * LINEAR_OPACITY adds the USEOPACITY terms: transparency 1 - opacity, the
+ * dz^2 correction, opacity accumulation, termination past 1, and the image
+ * byte color * opacity.  rgba holds the image byte / 255 and the opacity (1
+ * for LINEAR).  Both linear modes run on the REF walk (VFR_MODE_REF). */
+#define VFR_COMPOSITE_PPI 0u
+#define VFR_COMPOSITE_LINEAR 1u
+#define VFR_COMPOSITE_LINEAR_OPACITY 2u
 
 /* Batched FTB renders run one ray per thread over the device-built list of
  * active rays of all views (the hardware dispatcher refills the CUs wave by
@@ -277,6 +301,10 @@ int vfr_read_off(const char* path, float** xyzs, uint32_t* n_verts,
 int vfr_write_off(const char* path, const float* xyzs, uint32_t n_verts,
                   const uint32_t* tets, uint32_t n_tets);
 void vfr_normalize_vertices(float* xyzs, uint32_t n_verts);    /* :84-145  */
+void vfr_rotate_volume(float* xyzs, uint32_t n_verts, const float mv[16]); /* :886-906 */
+/* The viewer's modelview for its rotation angles (vfRay.cu:618-622):
+ * Rx(rotx) Ry(roty), column-major, computed in double and rounded. */
+void vfr_gl_rotation(float rotx_deg, float roty_deg, float mv[16]);
 int vfr_read_con(const char* path, uint32_t n_tets, uint32_t** con,
                  uint32_t* n_ext);                             /* :375-419 */
 int vfr_write_con(const char* path, const uint32_t* con, uint32_t n_tets,

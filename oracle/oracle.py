@@ -28,6 +28,7 @@ def lib() -> C.CDLL:
                                  C.POINTER(C.c_double), C.POINTER(C.c_uint8))
         u32 = C.c_uint32
         L.orc_normalize.argtypes = [f32p, u32]
+        L.orc_rotate.argtypes = [f32p, u32, f32p]
         L.orc_connectivity.argtypes = [u32p, u32, u32, u32p]
         L.orc_connectivity.restype = C.c_long
         L.orc_extf.argtypes = [u32p, u32, u32p]
@@ -38,7 +39,7 @@ def lib() -> C.CDLL:
         L.orc_expf.argtypes = [C.c_float]
         L.orc_expf.restype = C.c_float
         L.orc_render_ref.argtypes = [f32p, u32, u32p, u32, u32p, u32p, u32, f32p, f32p, f32p, u32, u32,
-                                     u8p, f32p, f64p, u32p, u32p, u32p]
+                                     u8p, f32p, f64p, u32p, u32p, u32p, u32]
         L.orc_render_ref.restype = C.c_long
         L.orc_render_ftb.argtypes = [f32p, u32, u32p, u32, u32p, u32p, u32, f32p, f32p, f32p, u32, f32p,
                                      C.c_float, u32, u32, C.c_float, f32p, f64p, u32p, u32p, u32p]
@@ -65,6 +66,13 @@ def u32a(a):
 def normalize(xyzs: np.ndarray) -> np.ndarray:
     x = f32(xyzs).copy()
     lib().orc_normalize(_p(x, C.c_float), x.shape[0])
+    return x
+
+
+def rotate(xyzs: np.ndarray, mv) -> np.ndarray:
+    x = f32(xyzs).copy()
+    m = f32(mv).reshape(16)
+    lib().orc_rotate(_p(x, C.c_float), x.shape[0], _p(m, C.c_float))
     return x
 
 
@@ -100,9 +108,10 @@ def psi_exact(tf: float, tb: float) -> float:
     return lib().orc_psi_exact(tf, tb)
 
 
-def render_ref(xyzs, tets, con, ext, tf, psi, lmt_, W, H, fp64=False) -> dict:
+def render_ref(xyzs, tets, con, ext, tf, psi, lmt_, W, H, fp64=False, composite=0) -> dict:
     """main.cc's render (K1, compaction, K2, K3 per face in order) at W x H.
-    xyzs must already be normalized."""
+    xyzs must already be normalized.  composite: 0 PPI, 1 linear, 2 linear
+    with opacity (fp64 applies to PPI only)."""
     x, t, c, e = f32(xyzs), u32a(tets), u32a(con), u32a(ext)
     tfa, ps, lm = f32(tf).reshape(-1), f32(psi).reshape(-1), f32(lmt_)
     rgb8 = np.zeros((H, W, 3), np.uint8)
@@ -115,7 +124,7 @@ def render_ref(xyzs, tets, con, ext, tf, psi, lmt_, W, H, fp64=False) -> dict:
                              _p(e, C.c_uint32), e.shape[0], _p(tfa, C.c_float), _p(ps, C.c_float),
                              _p(lm, C.c_float), W, H, _p(rgb8, C.c_uint8), _p(rgba, C.c_float),
                              _p(rgba64, C.c_double), _p(ent, C.c_uint32), _p(steps, C.c_uint32),
-                             _p(nvis, C.c_uint32))
+                             _p(nvis, C.c_uint32), composite)
     if r < 0:
         raise ValueError("bad input")
     return {"rgb8": rgb8, "rgba": rgba, "rgba64": rgba64, "entry_ext": ent, "steps": steps,
@@ -161,13 +170,19 @@ def run_ref_cli(flavour: str, work: str, name: str) -> str:
     return run
 
 
-def run_ref_harness(flavour: str, prefix: str, W: int, H: int) -> dict:
+def run_ref_harness(flavour: str, prefix: str, W: int, H: int, rotations=None) -> dict:
     """Per-pixel last-writer face ids and the image from the reference CPU
-    path (see ref_harness.cpp)."""
+    path (see ref_harness.cpp); rotations: k 16-float matrices applied as the
+    viewer's rotate + renormalize before rendering."""
     exe = os.path.join(REF_DIR, flavour, "ref_harness")
     with tempfile.TemporaryDirectory() as td:
         out = os.path.join(td, "o")
-        r = subprocess.run([exe, prefix, str(W), str(H), out], stdout=subprocess.PIPE,
+        args = [exe, prefix, str(W), str(H), out]
+        if rotations is not None:
+            rf = os.path.join(td, "rot.bin")
+            np.ascontiguousarray(rotations, np.float32).tofile(rf)
+            args.append(rf)
+        r = subprocess.run(args, stdout=subprocess.PIPE,
                            stderr=subprocess.STDOUT, text=True, timeout=600)
         if r.returncode != 0:
             raise RuntimeError(f"ref_harness failed:\n{r.stdout}")

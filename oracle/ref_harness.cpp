@@ -4,7 +4,7 @@
  * record what main.cc does not print: which visible face wrote each pixel.
  * TEST INFRASTRUCTURE: used only to pin the oracle.
  *
- *   ref_harness <dataset prefix> <width> <height> <out prefix>
+ *   ref_harness <dataset prefix> <width> <height> <out prefix> [rotations]
  *
  * Runs main.cc's sequence (main.cc:257-342: readOff, normalizeVertices, the
  * .con/.incid, .tf, .extf, .lmt cache-or-build steps), K1 over the external
@@ -13,6 +13,10 @@
  * Each face is also rendered alone into two images pre-filled with 0x00 and
  * 0xFF; a pixel that holds the same value in both was written by that face,
  * so the last face that writes a pixel is its owner.
+ *
+ * [rotations]: a file of k 4x4 float32 matrices; after the cache steps each
+ * is applied as the CUDA viewer's space key does (vfRay.cu:617-640):
+ * rotateVolume(matrix), normalizeVertices(), limits kept.
  *
  * Writes <out>.rgb (width*height*3), <out>.owner (uint32 ext-face index of
  * the owning face, 0xFFFFFFFF if none), <out>.verts (the normalized vertex
@@ -37,8 +41,8 @@ uint numExtFaces;
 real h_maxEdgeLength, h_maxZ, h_minZ;
 
 int main(int argc, char** argv) {
-  if (argc != 5) {
-    fprintf(stderr, "use: %s <dataset prefix> <width> <height> <out prefix>\n", argv[0]);
+  if (argc != 5 && argc != 6) {
+    fprintf(stderr, "use: %s <dataset prefix> <width> <height> <out prefix> [rotations]\n", argv[0]);
     return 1;
   }
   std::string ds = argv[1];
@@ -66,6 +70,16 @@ int main(int argc, char** argv) {
   if (!readLmt((ds + ".lmt").c_str())) {
     buildLmt();
     writeLmt((ds + ".lmt").c_str());
+  }
+  if (argc == 6) {
+    FILE* fr = fopen(argv[5], "rb");
+    if (!fr) { fprintf(stderr, "cannot read rotations\n"); return 1; }
+    real mv[16];
+    while (fread(mv, sizeof(real), 16, fr) == 16) {
+      rotateVolume(mv);
+      normalizeVertices();
+    }
+    fclose(fr);
   }
   std::vector<faceData> vis(numExtFaces);
   for (uint i = 0; i < numExtFaces; ++i)

@@ -94,6 +94,19 @@ extern "C" void orc_normalize(float* v, u32 nV) {
   }
 }
 
+/* rotateVolume, vfRayPreComp.cc:886-906: vRot[r] += modelView[r*4+c] *
+ * vOrig[c] over r, c = 0..3 from vRot = 0, vOrig = (x, y, z, 1); x, y, z
+ * take vRot[0..2]. */
+extern "C" void orc_rotate(float* v, u32 nV, const float* mv) {
+  for (u32 i = 0; i < nV; ++i) {
+    float vOrig[4] = {v[i * 4 + 0], v[i * 4 + 1], v[i * 4 + 2], 1.0f};
+    float vRot[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (u32 r = 0; r < 4; ++r)
+      for (u32 c = 0; c < 4; ++c) vRot[r] += mv[r * 4 + c] * vOrig[c];
+    for (u32 j = 0; j < 3; ++j) v[i * 4 + j] = vRot[j];
+  }
+}
+
 /* buildIncid's per-vertex ascending tet lists (vfRayPreComp.cc:231-266, a tet
  * is listed once per vertex it contains) followed by buildCon's search
  * (vfRayPreComp.cc:438-497): the neighbour across face f of tet i is the
@@ -312,13 +325,31 @@ struct RefOut {
   uint8_t* rgb8; float* rgba; double* rgba64; u32* entry_ext; u32* steps;
 };
 
+/* double -> u32 saturating (negative and NaN give 0), the conversion the
+ * reference's GPU build applies to its linear colour sums. */
+static inline u32 sat_u32(double v) { return v > 0.0 ? (v < 4294967296.0 ? (u32)v : 0xFFFFFFFFu) : 0u; }
+
+/* One channel of the linear integration, vfRayKernel.cu:758-772 and
+ * :787-792, in the text's float/double sub-expressions. */
+static inline u32 lin_channel(u32 col, float cP, float cN, float aP, float aN, float tr, float dz) {
+  double t1 = 0.5 * (double)(cP + cN) * (double)tr * (double)dz;
+  double w = 3.0 * (double)cP * (double)aP + 5.0 * (double)cN * (double)aP + (double)(cP * aN) +
+             3.0 * (double)cN * (double)aN;
+  double t2 = 0.04166666 * w * (double)dz * (double)dz;
+  u32 v = sat_u32((double)col + (t1 - t2));
+  return v > 255u ? 255u : v;
+}
+
 /* One visible face of the reference's K3 (vfRayKernelCPU.cc:172-566), with
  * the face-plane memo (:234-245, :369-435) omitted: it is keyed by (tet, exit
  * face) and returns the values the same Cramer solve produces.  Returns 1 if
- * the face ended with the reference's face-level `return` (:365). */
+ * the face ended with the reference's face-level `return` (:365).
+ * comp 0: PPI (:448-482); 1: linear integration (vfRayKernel.cu:738-806,
+ * USEOPACITY unset: transparency 1, alpha weights 0, image = colour);
+ * 2: linear with opacity (USEOPACITY: image = colour * opacity). */
 static int ref_face(const float* V, const u32* T, const u32* con, const float* tf,
                     const float* psi, const u32* extf, u32 extIdx, const Planes& cf,
-                    const u32 bb[4], u32 W, u32 H, float maxEdge, RefOut* o) {
+                    const u32 bb[4], u32 W, u32 H, float maxEdge, u32 comp, RefOut* o) {
   u32 t0 = extf[extIdx * 2 + 0], f0 = extf[extIdx * 2 + 1];
   F4 fv0 = ld4(V, T[t0 * 4 + ((f0 + 0) & 3)]);
   F4 fv1 = ld4(V, T[t0 * 4 + ((f0 + 1) & 3)]);
@@ -348,6 +379,10 @@ static int ref_face(const float* V, const u32* T, const u32* con, const float* t
       double T64 = 1.0;
       double dB[4] = {cB.x, cB.y, cB.z, cB.w};
       double A64[3] = {0, 0, 0};
+      const bool lin = comp != 0, opac = comp == 2;
+      float cP[4] = {255.0f * cB.x, 255.0f * cB.y, 255.0f * cB.z, opac ? cB.w : 0.0f}; /* vfRayKernel.cu:463-468 */
+      u32 col[3] = {0, 0, 0};
+      float opacity = 0.0f;
       u32 tcur = t0, fent = f0, steps = 0;
       while (1) {
         if (++steps > MAX_STEPS) break;
@@ -371,6 +406,23 @@ static int ref_face(const float* V, const u32* T, const u32* con, const float* t
         float dz = fabsf(zPrev - zNext);
         dz = ((double)dz < RDELTA) ? 0.0f : dz;
         if ((sNext < 0.0f) || (sNext > 1.0f)) break; /* :446 */
+        if (lin) {
+          /* vfRayKernel.cu:738-806; dz is not divided by maxEdgeLength */
+          F4 tc = ld4(tf, (u32)(255.0f * sNext));
+          float cN[4] = {255.0f * tc.x, 255.0f * tc.y, 255.0f * tc.z, opac ? tc.w : 0.0f};
+          float tr = opac ? (float)(1.0 - (double)opacity) : 1.0f;
+          u32 nc[3];
+          for (int c = 0; c < 3; ++c) nc[c] = lin_channel(col[c], cP[c], cN[c], cP[3], cN[3], tr, dz);
+          if (opac) opacity = (float)((double)opacity + (double)((cP[3] + cN[3]) * dz) / 2.0);
+          for (int c = 0; c < 4; ++c) cP[c] = cN[c];
+          for (int c = 0; c < 3; ++c) col[c] = nc[c];
+          if (opac && opacity > 1.0f) { opacity = 1.0f; break; } /* :799-803 */
+          if (tnext == tcur) break;
+          tcur = tnext;
+          zPrev = zNext;
+          sPrev = sNext;
+          continue;
+        }
         dz /= maxEdge;
         F4 cF = ld4(tf, (u32)(255.0f * sNext));
         float taux = cF.w * dz, tauy = cB.w * dz;
@@ -417,6 +469,20 @@ static int ref_face(const float* V, const u32* T, const u32* con, const float* t
       if (ay > 1.0f) ay = 1.0f;
       if (az > 1.0f) az = 1.0f;
       u32 p = py * W + px;
+      if (lin) {
+        /* image byte: colour * opacity (vfRayKernel.cu:836-838) or colour */
+        u32 b[3];
+        for (int c = 0; c < 3; ++c) b[c] = opac ? (u32)((float)col[c] * opacity) : col[c];
+        if (o->rgb8)
+          for (int c = 0; c < 3; ++c) o->rgb8[p * 3 + c] = (uint8_t)b[c];
+        if (o->rgba) {
+          for (int c = 0; c < 3; ++c) o->rgba[p * 4 + c] = (float)b[c] / 255.0f;
+          o->rgba[p * 4 + 3] = opac ? opacity : 1.0f;
+        }
+        if (o->entry_ext) o->entry_ext[p] = extIdx;
+        if (o->steps) o->steps[p] = steps > MAX_STEPS ? MAX_STEPS : steps;
+        continue;
+      }
       if (o->rgb8) {
         o->rgb8[p * 3 + 0] = (uint8_t)(int)(255.0f * ax);
         o->rgb8[p * 3 + 1] = (uint8_t)(int)(255.0f * ay);
@@ -451,9 +517,9 @@ static inline u32 img_coord(float x, u32 res) {
 extern "C" long orc_render_ref(const float* V, u32 nV, const u32* T, u32 nT, const u32* con,
                                const u32* extf, u32 nExt, const float* tf, const float* psi,
                                const float* lmt, u32 W, u32 H, uint8_t* rgb8, float* rgba,
-                               double* rgba64, u32* entry_ext, u32* steps, u32* out_nvis) {
+                               double* rgba64, u32* entry_ext, u32* steps, u32* out_nvis, u32 comp) {
   (void)nV; (void)nT;
-  if (W < 2 || H < 2) return -1;
+  if (W < 2 || H < 2 || comp > 2) return -1;
   std::vector<Planes> pl(nExt);
   std::vector<u32> vis;
   for (u32 i = 0; i < nExt; ++i) {
@@ -490,7 +556,7 @@ extern "C" long orc_render_ref(const float* V, u32 nV, const u32* T, u32 nT, con
       if (ix > bb[2]) bb[2] = ix;
       if (iy > bb[3]) bb[3] = iy;
     }
-    returns += ref_face(V, T, con, tf, psi, extf, i, pl[i], bb, W, H, lmt[0], &o);
+    returns += ref_face(V, T, con, tf, psi, extf, i, pl[i], bb, W, H, lmt[0], comp, &o);
   }
   return returns;
 }

@@ -14,8 +14,12 @@ Run in a container that has the reference tree; the fixtures are data only.
                   largest departure elsewhere, sha256 of the table.
   cleantf.json    sha256 of what the reference's cleanTF writes for the
                   seeded legacy TF file of tests/helpers.legacy_tf_text().
+  ref_rot.npz     8^3 mesh with the seeded TF after two presses of the CUDA
+                  viewer's space key at angles (30, 20) (rotateVolume +
+                  normalizeVertices, limits kept): the vertices, per-pixel
+                  owner faces and the 128x128 strict-IEEE image.
 
-  python3 tests/golden/make_golden.py [cleantf]   (all fixtures, or one)
+  python3 tests/golden/make_golden.py [cleantf|rotated]   (all, or one)
 """
 from __future__ import annotations
 
@@ -75,11 +79,30 @@ def cleantf() -> None:
               open(os.path.join(HERE, "cleantf.json"), "w"), indent=1)
 
 
+ROT_ANGLES = [(30.0, 20.0), (30.0, 20.0)]
+
+
+def rotated() -> None:
+    assert O.ref_available("strict"), "build oracle/_ref first"
+    mv = np.stack([V.gl_rotation(a, b) for a, b in ROT_ANGLES])
+    with tempfile.TemporaryDirectory() as td:
+        x, t, tf = dataset(td, "s8", 8)
+        V.write_tf(os.path.join(td, "tet_offs", "s8.tf"), tf)
+        h = O.run_ref_harness("strict", os.path.join(td, "tet_offs", "s8"), 128, 128, rotations=mv)
+    np.savez_compressed(os.path.join(HERE, "ref_rot.npz"), angles=np.array(ROT_ANGLES, np.float32), mv=mv,
+                        verts=h["verts"], owner128=h["entry_ext"], rgb128_strict=h["rgb8"],
+                        n_vis=np.array([h["info"]["numVisFaces"]], np.uint32))
+
+
 def main() -> None:
     if sys.argv[1:] == ["cleantf"]:
         cleantf()
         return
+    if sys.argv[1:] == ["rotated"]:
+        rotated()
+        return
     cleantf()
+    rotated()
     assert O.ref_available("strict") and O.ref_available("fast"), "build oracle/_ref first"
     out = {}
     with tempfile.TemporaryDirectory() as td:

@@ -94,6 +94,63 @@ def test_cli_image_matches_strict_reference():
         assert diff.max() <= 1 and (diff > 0).any(-1).mean() < 0.01
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("composite", [("linear", 1), ("linear-opacity", 2)])
+def test_cli_linear_compositing(composite):
+    """--composite linear|linear-opacity (the reference's USEPARTIALPREINT 0
+    build switch): the image equals the oracle's linear render byte for
+    byte."""
+    import oracle as O
+    name, comp = composite
+    with tempfile.TemporaryDirectory() as td:
+        d = os.path.join(td, "tet_offs")
+        os.makedirs(d)
+        x, t, tf, _ = WL.plain_cpu(8)
+        V.write_off(os.path.join(d, "s8.off"), x, t)
+        V.write_tf(os.path.join(d, "s8.tf"), tf)
+        print(f"CLI s8 128x128 --composite {name}", flush=True)
+        r = _run(["--dir", d, "--size", "128", "128", "--composite", name, "s8"], td)
+        assert r.returncode == 0, r.stdout + r.stderr
+        ppm = np.frombuffer(open(os.path.join(td, "s8_gpu_img.ppm"), "rb").read(), np.uint8)
+        xn = O.normalize(x)
+        con, _ = O.connectivity(t, x.shape[0])
+        tfr = V.read_tf(os.path.join(d, "s8.tf"))
+        orc = O.render_ref(xn, t, con, O.extf(con), tfr, V.psi_generate(), O.lmt(xn, t), 128, 128,
+                           composite=comp)
+        assert np.array_equal(ppm[15:].reshape(128, 128, 3), orc["rgb8"])
+        assert orc["rgb8"].max() > 0
+
+
+@pytest.mark.gpu
+def test_cli_rotate():
+    """--rotate 30 20 twice (two presses of the viewer's space key): the
+    image has the compiled reference's owner faces (via the oracle) and is
+    within 1 of its strict image where the psi tables differ, equal to the
+    oracle's."""
+    import oracle as O
+    rot = dict(np.load(os.path.join(ROOT, "tests", "golden", "ref_rot.npz")))
+    with tempfile.TemporaryDirectory() as td:
+        d = os.path.join(td, "tet_offs")
+        os.makedirs(d)
+        x, t, tf, _ = WL.plain_cpu(8)
+        V.write_off(os.path.join(d, "s8.off"), x, t)
+        V.write_tf(os.path.join(d, "s8.tf"), tf)
+        print("CLI s8 128x128 --rotate 30 20 x2", flush=True)
+        args = ["--dir", d, "--size", "128", "128"]
+        for a, b in rot["angles"]:
+            args += ["--rotate", f"{a:g}", f"{b:g}"]
+        r = _run(args + ["s8"], td)
+        assert r.returncode == 0, r.stdout + r.stderr
+        img = np.frombuffer(open(os.path.join(td, "s8_gpu_img.ppm"), "rb").read(), np.uint8)[15:]
+        img = img.reshape(128, 128, 3)
+        diff = np.abs(img.astype(int) - rot["rgb128_strict"].astype(int))
+        assert diff.max() <= 1 and (diff > 0).any(-1).mean() < 0.01
+        xn = O.normalize(x)
+        con, _ = O.connectivity(t, x.shape[0])
+        orc = O.render_ref(rot["verts"], t, con, O.extf(con), tf, V.psi_generate(), O.lmt(xn, t), 128, 128)
+        assert np.array_equal(img, orc["rgb8"])
+
+
 CLEANTF = os.path.join(ROOT, "vfray_amd", "cleanTF")
 
 

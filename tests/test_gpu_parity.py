@@ -3,10 +3,13 @@
 Entry faces, entry tets and step counts must be bit-exact; RGBA within
 2e-3 max-abs per channel; PSNR >= 50 dB against the FP64 re-integration.
 """
+import os
+
 import numpy as np
 import pytest
 
 import vfray_amd as V
+from vfray_amd import workloads as WL
 import oracle as O
 from helpers import c1_inputs, grid_inputs, oracle_ftb, psnr
 
@@ -79,6 +82,62 @@ def test_ref_mode_c1(ctx, c1, psi):
     assert np.abs(V.to_rgb8(g["rgba"]).astype(int) - orc["rgb8"]).max() <= 1
     assert st["steps"] == int(orc["steps"].sum())
     assert st["active_rays"] == int((orc["entry_ext"] != V.NO_ID).sum())
+    m.close()
+
+
+@pytest.mark.parametrize("composite", [V.COMPOSITE_LINEAR, V.COMPOSITE_LINEAR_OPACITY])
+def test_ref_linear_c1(ctx, c1, psi, composite):
+    """Linear compositing (USEPARTIALPREINT 0) on the REF walk: image bytes,
+    RGBA, entry faces and step counts bit-exact with the oracle, per-view and
+    batched alike."""
+    print(f"REF linear composite={composite} 16^3 512x512", flush=True)
+    orc = O.render_ref(c1["xn"], c1["t"], c1["con"], c1["ext"], c1["tf"], psi, c1["lmt"], 512, 512,
+                       composite=composite)
+    m = ctx.mesh(c1["x"], c1["t"])
+    tf = ctx.tf(c1["tf"])
+    for sched in (V.SCHED_PER_VIEW, V.SCHED_BATCHED):
+        outs, st = ctx.render(m, [V.ortho_view(512, 512, tf)], mode=V.MODE_REF, schedule=sched,
+                              composite=composite)
+        g = outs[0]
+        assert np.array_equal(g["entry_ext"], orc["entry_ext"])
+        assert np.array_equal(g["steps"], orc["steps"])
+        assert np.array_equal(g["rgba"], orc["rgba"])
+        assert np.array_equal(V.to_rgb8(g["rgba"]), orc["rgb8"])
+        assert st["steps"] == int(orc["steps"].sum())
+    assert orc["rgb8"].max() > 0
+    with pytest.raises(V.VfrError, match="REF mode"):
+        ctx.render(m, [V.ortho_view(64, 64, tf)], mode=V.MODE_FTB, composite=composite)
+    m.close()
+
+
+def test_mesh_rotate_matches_reference(ctx, psi):
+    """vfr_mesh_rotate (the viewer's rotate + renormalize, on the device):
+    two presses at (30, 20) on the 8^3 mesh give the compiled reference's
+    vertices bit for bit; the REF render with the kept (stale) limits has its
+    owner faces and equals the oracle's image; a plan made before the
+    rotation refuses to run."""
+    rot = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_rot.npz")))
+    x, t, tf, _ = WL.plain_cpu(8)
+    print("mesh rotate 8^3 x2", flush=True)
+    m = ctx.mesh(x, t)
+    tfi = ctx.tf(tf)
+    p = ctx.plan(m, [V.ortho_view(128, 128, tfi)], mode=V.MODE_REF)
+    lmt0 = m.connectivity()[2]
+    for mv in rot["mv"]:
+        m.rotate(mv)
+    assert np.array_equal(m.vertices().view(np.uint32), rot["verts"].view(np.uint32))
+    assert np.array_equal(m.connectivity()[2], lmt0)
+    with pytest.raises(V.VfrError, match="rotated"):
+        p.execute()
+    p.close()
+    outs, _ = ctx.render(m, [V.ortho_view(128, 128, tfi)], mode=V.MODE_REF)
+    g = outs[0]
+    assert np.array_equal(g["entry_ext"], rot["owner128"])
+    xn = O.normalize(x)
+    con, _ = O.connectivity(t, x.shape[0])
+    orc = O.render_ref(rot["verts"], t, con, O.extf(con), tf, psi, O.lmt(xn, t), 128, 128)
+    assert np.array_equal(g["rgba"], orc["rgba"])
+    assert np.array_equal(g["steps"], orc["steps"])
     m.close()
 
 

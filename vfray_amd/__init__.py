@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libvfr.so")
 VFR_OK = 0
 PROJ_ORTHO, PROJ_PERSPECTIVE = 0, 1
 MODE_REF, MODE_FTB = 0, 1
+COMPOSITE_PPI, COMPOSITE_LINEAR, COMPOSITE_LINEAR_OPACITY = 0, 1, 2
 RENDER_PERSISTENT = 0x1
 RENDER_PIPELINED = 0x2
 SCHED_PER_VIEW, SCHED_BATCHED = 0, 1
@@ -40,7 +41,7 @@ class RenderOpts(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("schedule", C.c_uint32), ("term_alpha", C.c_float),
                 ("waves_per_cu", C.c_uint32), ("refill", C.c_uint32), ("use_graph", C.c_uint32),
                 ("max_entries", C.c_uint32), ("flags", C.c_uint32), ("occupancy", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("composite", C.c_uint32)]
 
 
 class FrameOut(C.Structure):
@@ -118,6 +119,7 @@ def lib() -> C.CDLL:
             "vfr_mesh_get_info": (C.c_int, [P, C.POINTER(MeshInfo)]),
             "vfr_mesh_connectivity": (C.c_int, [P, u32p, u32p, f32p]),
             "vfr_mesh_vertices": (C.c_int, [P, f32p]),
+            "vfr_mesh_rotate": (C.c_int, [P, f32p, u32]),
             "vfr_plan_create": (C.c_int, [P, P, C.POINTER(View), C.c_int, C.POINTER(RenderOpts), C.POINTER(P)]),
             "vfr_plan_execute": (C.c_int, [P, C.POINTER(Stats)]),
             "vfr_plan_read": (C.c_int, [P, C.c_int, C.POINTER(FrameOut)]),
@@ -137,6 +139,9 @@ def lib() -> C.CDLL:
             "vfr_read_off": (C.c_int, [C.c_char_p, C.POINTER(f32p), u32p, C.POINTER(u32p), u32p]),
             "vfr_write_off": (C.c_int, [C.c_char_p, f32p, u32, u32p, u32]),
             "vfr_normalize_vertices": (None, [f32p, u32]),
+            "vfr_rotate_volume": (None, [f32p, u32, f32p]),
+            "vfr_gl_rotation": (None, [C.c_float, C.c_float, f32p]),
+            "vfr_clean_tf": (C.c_int, [C.c_char_p, C.c_char_p]),
             "vfr_read_con": (C.c_int, [C.c_char_p, u32, C.POINTER(u32p), u32p]),
             "vfr_write_con": (C.c_int, [C.c_char_p, u32p, u32, u32]),
             "vfr_read_extf": (C.c_int, [C.c_char_p, u32, C.POINTER(u32p)]),
@@ -244,6 +249,25 @@ def write_tf(path: str, tf: np.ndarray) -> None:
     _check(lib().vfr_write_tf(path.encode(), fptr(np.ascontiguousarray(tf, np.float32))))
 
 
+def gl_rotation(rotx_deg: float, roty_deg: float) -> np.ndarray:
+    out = np.zeros(16, np.float32)
+    lib().vfr_gl_rotation(C.c_float(rotx_deg), C.c_float(roty_deg), fptr(out))
+    return out
+
+
+def rotate_volume(xyzs: np.ndarray, mv) -> np.ndarray:
+    x = np.ascontiguousarray(xyzs, np.float32).copy()
+    m = np.ascontiguousarray(mv, np.float32).reshape(16)
+    lib().vfr_rotate_volume(fptr(x), x.shape[0], fptr(m))
+    return x
+
+
+def read_tf(path: str) -> np.ndarray:
+    out = np.zeros(1024, np.float32)
+    _check(lib().vfr_read_tf(path.encode(), fptr(out)))
+    return out
+
+
 def to_rgb8(rgba: np.ndarray) -> np.ndarray:
     """uchar RGB the way the reference stores it: (uchar)(255*min(c,1))
     (vfRayKernelCPU.cc:546-552), computed in float32."""
@@ -346,9 +370,9 @@ class Context:
 
     def plan(self, mesh: "Mesh", views, mode=MODE_FTB, schedule=SCHED_BATCHED, term_alpha=0.99,
              use_graph=False, waves_per_cu=0, refill=0, persistent=False, occupancy=0,
-             max_entries=0, pipelined=False) -> "Plan":
+             max_entries=0, pipelined=False, composite=COMPOSITE_PPI) -> "Plan":
         return Plan(self, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
-                    persistent, occupancy, max_entries, pipelined)
+                    persistent, occupancy, max_entries, pipelined, composite)
 
     def render(self, mesh: "Mesh", views, **kw) -> tuple[list[dict], dict]:
         p = self.plan(mesh, views, **kw)
@@ -383,6 +407,11 @@ class Mesh:
         _check(lib().vfr_mesh_connectivity(self._p, uptr(con), uptr(extf), fptr(lmt)))
         return con.reshape(-1, 4), extf[: self.info["n_ext_faces"] * 2].reshape(-1, 2), lmt
 
+    def rotate(self, mv, normalize=True) -> None:
+        """rotateVolume + normalizeVertices on the resident vertices."""
+        m = np.ascontiguousarray(mv, np.float32).reshape(16)
+        _check(lib().vfr_mesh_rotate(self._p, fptr(m), MESH_NORMALIZE if normalize else 0))
+
     def vertices(self) -> np.ndarray:
         out = np.zeros(self.info["n_verts"] * 4, np.float32)
         _check(lib().vfr_mesh_vertices(self._p, fptr(out)))
@@ -396,13 +425,13 @@ class Mesh:
 
 class Plan:
     def __init__(self, ctx, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
-                 persistent=False, occupancy=0, max_entries=0, pipelined=False):
+                 persistent=False, occupancy=0, max_entries=0, pipelined=False, composite=COMPOSITE_PPI):
         self.ctx, self.mesh = ctx, mesh
         self.views = list(views)
         arr = (View * len(self.views))(*self.views)
         flags = (RENDER_PERSISTENT if persistent else 0) | (RENDER_PIPELINED if pipelined else 0)
         o = RenderOpts(mode, schedule, term_alpha, waves_per_cu, refill, 1 if use_graph else 0, max_entries,
-                       flags, occupancy, 0)
+                       flags, occupancy, composite)
         self._p = C.c_void_p()
         _check(lib().vfr_plan_create(ctx._p, mesh._p, arr, len(self.views), C.byref(o), C.byref(self._p)))
 

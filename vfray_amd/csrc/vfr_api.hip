@@ -48,6 +48,7 @@ struct vfr_mesh {
   DevMesh d;
   std::vector<uint32_t> h_extf; /* caller (tet, face) pairs */
   float bb_lo[3] = {0, 0, 0}, bb_hi[3] = {0, 0, 0};
+  uint32_t version = 0;         /* bumped by vfr_mesh_rotate */
 };
 
 extern "C" const char* vfr_version(void) { return "vfr 0.1 gfx950"; }
@@ -267,6 +268,20 @@ extern "C" int vfr_mesh_connectivity(const vfr_mesh* m, uint32_t* con_out, uint3
   return VFR_OK;
 }
 
+extern "C" int vfr_mesh_rotate(vfr_mesh* m, const float mv[16], uint32_t flags) {
+  if (!m || !mv) { vfr_set_error("vfr_mesh_rotate: null argument"); return VFR_EINVAL; }
+  if (flags & ~VFR_MESH_NORMALIZE) { vfr_set_error("vfr_mesh_rotate: unknown flags 0x%x", flags); return VFR_EINVAL; }
+  for (int k = 0; k < 16; ++k)
+    if (!std::isfinite(mv[k])) { vfr_set_error("vfr_mesh_rotate: matrix entry %d is not finite", k); return VFR_EINVAL; }
+  CHECK_HIP(hipSetDevice(m->ctx->device));
+  float lo[3], hi[3];
+  CHECK_RC(vfr_k_rotate_normalize(reinterpret_cast<float4*>(m->d.vert), m->d.n_verts, mv,
+                                  (flags & VFR_MESH_NORMALIZE) ? 1 : 0, lo, hi, m->ctx->stream));
+  for (int a = 0; a < 3; ++a) { m->bb_lo[a] = lo[a]; m->bb_hi[a] = hi[a]; }
+  ++m->version;
+  return VFR_OK;
+}
+
 extern "C" int vfr_mesh_vertices(const vfr_mesh* m, float* out) {
   if (!m || !out) { vfr_set_error("vfr_mesh_vertices: null argument"); return VFR_EINVAL; }
   CHECK_HIP(hipSetDevice(m->ctx->device));
@@ -284,6 +299,7 @@ struct vfr_plan {
   vfr_ctx* ctx = nullptr;
   const vfr_mesh* mesh = nullptr;
   vfr_render_opts opts{};
+  uint32_t mesh_version = 0;              /* the mesh's version at creation */
   std::vector<DevView> hv;
   uint32_t n_views = 0;
   uint64_t n_pix = 0;
@@ -469,6 +485,7 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   a.refill = p->opts.refill;
   a.tf_lds_base = p->tf_lo;
   a.tf_lds_n = p->tf_n;
+  a.composite = p->opts.composite;
   return a;
 }
 
@@ -498,6 +515,14 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   }
   if (o.waves_per_cu == 0) o.waves_per_cu = 16;
   if (o.refill == 0) o.refill = 32;
+  if (o.composite > VFR_COMPOSITE_LINEAR_OPACITY) {
+    vfr_set_error("unknown compositing %u", o.composite);
+    return VFR_EINVAL;
+  }
+  if (o.composite != VFR_COMPOSITE_PPI && o.mode != VFR_MODE_REF) {
+    vfr_set_error("linear compositing is defined on the reference's walk (REF mode) only");
+    return VFR_EINVAL;
+  }
   if (o.waves_per_cu > 64 || o.refill > 64) {
     vfr_set_error("waves_per_cu %u / refill %u out of range (<= 64)", o.waves_per_cu, o.refill);
     return VFR_EINVAL;
@@ -506,6 +531,7 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   if (!p) { vfr_set_error("out of memory"); return VFR_ENOMEM; }
   p->ctx = c;
   p->mesh = m;
+  p->mesh_version = m->version;
   p->opts = o;
   p->n_views = (uint32_t)n_views;
   p->hv.resize(n_views);
@@ -733,6 +759,10 @@ static int execute_pipelined(vfr_plan* p, double* ms_total, double* ms_vis, doub
 
 extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
   if (!p) { vfr_set_error("vfr_plan_execute: null plan"); return VFR_EINVAL; }
+  if (p->mesh_version != p->mesh->version) {
+    vfr_set_error("the mesh was rotated after the plan was created: create the plan again");
+    return VFR_ESTATE;
+  }
   vfr_ctx* c = p->ctx;
   hipStream_t st = c->stream;
   CHECK_HIP(hipSetDevice(c->device));
@@ -898,6 +928,10 @@ static std::vector<std::pair<TuneKey, TuneVal>> g_tune;
 extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ_out, uint32_t* wpc_out,
                                  uint32_t* refill_out, double* best_out) {
   if (!p) { vfr_set_error("vfr_plan_autotune: null plan"); return VFR_EINVAL; }
+  if (p->mesh_version != p->mesh->version) {
+    vfr_set_error("the mesh was rotated after the plan was created: create the plan again");
+    return VFR_ESTATE;
+  }
   if (p->opts.schedule != VFR_SCHED_BATCHED || p->opts.mode != VFR_MODE_FTB) {
     vfr_set_error("autotune applies to batched front-to-back plans");
     return VFR_ESTATE;

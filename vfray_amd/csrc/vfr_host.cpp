@@ -141,6 +141,20 @@ extern "C" int vfr_write_off(const char* path, const float* xyzs, uint32_t n_ver
  * origin, scale by the largest half extent, scalar to [0,1]. The centre and
  * both scale factors are computed in double and rounded to float, as the
  * reference's `/ 2.0` and `1.0 /` literals make them. */
+void vfr_normalize_params(const float mn_in[4], const float mx_in[4], float center[3], float* sc, float* ss) {
+  float mn[4], mx[4];
+  for (int i = 0; i < 4; ++i) { mn[i] = mn_in[i]; mx[i] = mx_in[i]; }
+  for (int i = 0; i < 3; ++i) {
+    center[i] = (float)((double)(mn[i] + mx[i]) / 2.0);
+    mx[i] -= center[i];
+    mn[i] -= center[i];
+  }
+  float m = (mx[1] > mx[2]) ? mx[1] : mx[2];
+  m = (mx[0] > m) ? mx[0] : m;
+  *sc = (float)(1.0 / (double)m);
+  *ss = (float)(1.0 / (double)(mx[3] - mn[3]));
+}
+
 extern "C" void vfr_normalize_vertices(float* v, uint32_t nV) {
   if (!v || nV == 0) return;
   float mn[4], mx[4], center[3];
@@ -151,15 +165,8 @@ extern "C" void vfr_normalize_vertices(float* v, uint32_t nV) {
       if (x < mn[j]) mn[j] = x;
       if (x > mx[j]) mx[j] = x;
     }
-  for (int i = 0; i < 3; ++i) {
-    center[i] = (float)((double)(mn[i] + mx[i]) / 2.0);
-    mx[i] -= center[i];
-    mn[i] -= center[i];
-  }
-  float m = (mx[1] > mx[2]) ? mx[1] : mx[2];
-  m = (mx[0] > m) ? mx[0] : m;
-  const float sc = (float)(1.0 / (double)m);
-  const float ss = (float)(1.0 / (double)(mx[3] - mn[3]));
+  float sc, ss;
+  vfr_normalize_params(mn, mx, center, &sc, &ss);
   for (uint32_t i = 0; i < nV; ++i) {
     float* p = v + (size_t)i * 4;
     p[0] = (p[0] - center[0]) * sc;
@@ -167,6 +174,42 @@ extern "C" void vfr_normalize_vertices(float* v, uint32_t nV) {
     p[2] = (p[2] - center[2]) * sc;
     p[3] = (p[3] - mn[3]) * ss;
   }
+}
+
+/* rotateVolume, vfRayPreComp.cc:886-906: row r of the new vertex is
+ * sum_c mv[r*4+c] * (x, y, z, 1)[c], accumulated from 0 in c order; x, y, z
+ * are replaced and the scalar kept.  mv is read row-major, so a column-major
+ * OpenGL matrix M applies as its transpose. */
+extern "C" void vfr_rotate_volume(float* v, uint32_t nV, const float mv[16]) {
+  if (!v || !mv) return;
+  for (uint32_t i = 0; i < nV; ++i) {
+    float* p = v + (size_t)i * 4;
+    const float o[4] = {p[0], p[1], p[2], 1.0f};
+    float r[3];
+    for (int a = 0; a < 3; ++a) {
+      float acc = 0.0f;
+      for (int c = 0; c < 4; ++c) acc = acc + mv[a * 4 + c] * o[c];
+      r[a] = acc;
+    }
+    p[0] = r[0]; p[1] = r[1]; p[2] = r[2];
+  }
+}
+
+/* The modelview matrix of the viewer's space key (vfRay.cu:618-622):
+ * identity, glRotatef(rotx, 1, 0, 0), glRotatef(roty, 0, 1, 0), read back
+ * column-major, i.e. M = Rx(rotx) Ry(roty) with mv[c*4+r] = M[r][c].  The
+ * entries are computed in double and rounded (a GL implementation's own
+ * rounding of the matrix is not reproduced). */
+extern "C" void vfr_gl_rotation(float rotx_deg, float roty_deg, float mv[16]) {
+  if (!mv) return;
+  const double k = 3.14159265358979323846 / 180.0;
+  const double cx = cos((double)rotx_deg * k), sx = sin((double)rotx_deg * k);
+  const double cy = cos((double)roty_deg * k), sy = sin((double)roty_deg * k);
+  const double M[3][3] = {{cy, 0.0, sy}, {sx * sy, cx, -sx * cy}, {-cx * sy, sx, cx * cy}};
+  for (int i = 0; i < 16; ++i) mv[i] = 0.0f;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) mv[c * 4 + r] = (float)M[r][c];
+  mv[15] = 1.0f;
 }
 
 /* buildLmt, vfRayPreComp.cc:808-853: longest tet edge (float sqrt), and the

@@ -94,6 +94,11 @@ struct DevMesh {
   double build_ms = 0;
 };
 
+/* normalizeVertices' centre and scales (vfRayPreComp.cc:113-130) from the
+ * componentwise min/max of (x, y, z, s): centre = (min + max) / 2.0 in
+ * double, sc = 1 / the largest half extent, ss = 1 / (max s - min s). */
+void vfr_normalize_params(const float mn[4], const float mx[4], float center[3], float* sc, float* ss);
+
 /* Statistics counters a render accumulates on the device. */
 enum { ST_STEPS = 0, ST_ANOM = 1, ST_OVERFLOW = 2, ST_RAYS = 3, ST_N = 8 };
 
@@ -120,6 +125,7 @@ struct TravArgs {
   uint32_t tf_lds_base;    /* one-ray-per-thread FTB: TF slots [base, base+n)   */
   uint32_t tf_lds_n;       /* staged in LDS (0: read the global table)         */
   uint32_t kmax;           /* FTB: entry slots per pixel (8, 16 or 32)         */
+  uint32_t composite;      /* REF: VFR_COMPOSITE_*                             */
 };
 
 /* ---- launchers (vfr_kernels.hip) ---- */
@@ -146,6 +152,9 @@ int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, 
  * register-capped variants; anything else: as compiled). */
 /* proj: VFR_TRAV_* (the projections of the views the rays belong to). */
 enum { VFR_TRAV_ORTHO = 0, VFR_TRAV_PERSP = 1, VFR_TRAV_MIXED = 2 };
+/* Walk kernel variant beside VFR_MODE_REF / VFR_MODE_FTB: the REF walk with
+ * linear compositing (TravArgs.composite != VFR_COMPOSITE_PPI). */
+#define VFR_MODE_REF_LINEAR 2u
 /* n_records: tet records in the arena the rays walk (from VFR_BIG_RECORDS on,
  * the kernels address records with 64-bit arithmetic). */
 int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t occupancy, const TravArgs& a,
@@ -170,6 +179,12 @@ void vfr_mesh_bbox(const vfr_mesh* m, float lo[3], float hi[3]);
 /* Adds `add` to the tet index of the linked neighbour words of n records
  * (a mesh copied into a record arena at record `add`). */
 int vfr_k_rebase_links(TetRec* rec, uint32_t n, uint32_t add, void* stream);
+
+/* Rotates the n vertices in place by mv as rotateVolume does (mv == NULL:
+ * no rotation), then, if asked, renormalizes them as normalizeVertices does;
+ * lo/hi receive the resulting x, y, z bounds.  Synchronizes the stream. */
+int vfr_k_rotate_normalize(float4* v, uint32_t n, const float* mv, int normalize, float lo[3], float hi[3],
+                           void* stream);
 
 /* preprocessing on the device */
 int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,

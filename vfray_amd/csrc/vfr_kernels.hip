@@ -3,7 +3,7 @@
  *
  *   preprocessing : face keys -> radix sort -> adjacent match -> external
  *                   faces (stream compaction) -> Z-curve reorder -> packed
- *                   32-byte tet records.  Replaces buildIncid/buildCon/
+ *                   64-byte tet records.  Replaces buildIncid/buildCon/
  *                   buildExtF (vfRayPreComp.cc:219-270, 424-501, 710-737).
  *   raster        : per (view, external face): visibility (the Cramer plane
  *                   test of findVisibleFaces, vfRayKernelCPU.cc:51-132), the
@@ -623,8 +623,39 @@ struct RayOut {
   uint32_t broken;
 };
 
+/* Linear compositing state (vfRayKernel.cu:455-470): colour as the integer
+ * 0..255 channels it accumulates, the previous sample's colour scaled to
+ * 0..255 and its opacity, the accumulated opacity. */
+struct LinState {
+  float pr, pg, pb, pw;
+  uint32_t cr, cg, cb;
+  float op;
+};
+
+/* double -> uint32 as a saturating conversion: negative and NaN give 0. */
+DEV uint32_t sat_u32(double v) { return v > 0.0 ? (v < 4294967296.0 ? (uint32_t)v : 0xFFFFFFFFu) : 0u; }
+
+/* One channel of the linear integration (vfRayKernel.cu:758-772):
+ * (uint)(color + (0.5 (cP + cN) transparency dz
+ *                 - 0.04166666 (3 cP aP + 5 cN aP + cP aN + 3 cN aN) dz^2)),
+ * with the reference's float and double sub-expressions, clamped at 255
+ * (:787-792). */
+DEV uint32_t lin_channel(uint32_t col, float cP, float cN, float aP, float aN, float tr, float dz) {
+  const double t1 = 0.5 * (double)(cP + cN) * (double)tr * (double)dz;
+  const double w = 3.0 * (double)cP * (double)aP + 5.0 * (double)cN * (double)aP + (double)(cP * aN) +
+                   3.0 * (double)cN * (double)aN;
+  const double t2 = 0.04166666 * w * (double)dz * (double)dz;
+  const uint32_t v = sat_u32((double)col + (t1 - t2));
+  return v > 255u ? 255u : v;
+}
+
 /* REF mode: the reference's per-ray loop (vfRayKernelCPU.cc:270-552) for an
- * owned pixel, with its double-promoted sub-expressions. */
+ * owned pixel, with its double-promoted sub-expressions.  LIN selects the
+ * linear integration of the reference's USEPARTIALPREINT 0 build
+ * (vfRayKernel.cu:738-806, with the colour update the CPU version omits,
+ * vfRayKernelCPU.cc:499-514); a.composite == VFR_COMPOSITE_LINEAR_OPACITY
+ * adds its opacity accumulation and early termination (USEOPACITY). */
+template <bool LIN>
 DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
   RayOut o;
   o.anomaly = 0; o.broken = 0;
@@ -652,6 +683,12 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
   const float sPrev = plane_eval(cf.s, ox, oy);
   float4 cB = a.tf[tfo + min(255u, (uint32_t)(255.0f * sPrev))];
   float ax = 0.0f, ay = 0.0f, az = 0.0f, Tr = 1.0f;
+  const bool opac = LIN && a.composite == VFR_COMPOSITE_LINEAR_OPACITY;
+  LinState L;
+  L.pr = 255.0f * cB.x; L.pg = 255.0f * cB.y; L.pb = 255.0f * cB.z; /* :463-467 */
+  L.pw = opac ? cB.w : 0.0f;
+  L.cr = L.cg = L.cb = 0u;
+  L.op = 0.0f;
   uint32_t steps = 0;
   while (true) {
     if (++steps > VFR_MAX_STEPS) { o.anomaly = 1; break; }
@@ -664,6 +701,26 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
     float dz = fabsf(zPrev - zNext);
     dz = tiny(dz) ? 0.0f : dz;
     if ((sNext < 0.0f) || (sNext > 1.0f)) break;    /* :446 */
+    if (LIN) {
+      /* vfRayKernel.cu:738-806: dz stays in object units; without opacity
+       * the transparency is 1 and the alpha weights are 0 (:755-757). */
+      const float4 cN = a.tf[tfo + min(255u, (uint32_t)(255.0f * sNext))];
+      const float nr = 255.0f * cN.x, ng = 255.0f * cN.y, nb = 255.0f * cN.z;
+      const float nw = opac ? cN.w : 0.0f;
+      const float tr = opac ? (float)(1.0 - (double)L.op) : 1.0f;
+      const uint32_t r = lin_channel(L.cr, L.pr, nr, L.pw, nw, tr, dz);
+      const uint32_t g = lin_channel(L.cg, L.pg, ng, L.pw, nw, tr, dz);
+      const uint32_t b = lin_channel(L.cb, L.pb, nb, L.pw, nw, tr, dz);
+      if (opac) L.op = (float)((double)L.op + (double)((L.pw + nw) * dz) / 2.0); /* :780 */
+      L.pr = nr; L.pg = ng; L.pb = nb; L.pw = nw;
+      L.cr = r; L.cg = g; L.cb = b;
+      if (opac && L.op > 1.0f) { L.op = 1.0f; break; }  /* :799-803 */
+      if (s.nb == VFR_NB_EXT) break;
+      w.t = s.nb >> 2; w.fe = s.nb & 3u;
+      w.H0 = s.A; w.H1 = s.B; w.H2 = s.C; w.h0 = s.a; w.h1 = s.b; w.h2 = s.c;
+      zPrev = zNext;
+      continue;
+    }
     dz = dz / max_edge;
     const float4 cF = a.tf[tfo + min(255u, (uint32_t)(255.0f * sNext))];
     const float taux = cF.w * dz, tauy = cB.w * dz;
@@ -692,10 +749,20 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
     w.H0 = s.A; w.H1 = s.B; w.H2 = s.C; w.h0 = s.a; w.h1 = s.b; w.h2 = s.c;
     zPrev = zNext;
   }
-  ax = ax > 1.0f ? 1.0f : ax;
-  ay = ay > 1.0f ? 1.0f : ay;
-  az = az > 1.0f ? 1.0f : az;
-  o.c = make_float4(ax, ay, az, 1.0f - Tr);
+  if (LIN) {
+    /* the image byte is color * opacity (:836-838) or color (:840-842); it
+     * is stored as byte / 255, which 255 * x turns back into the byte */
+    const float op = opac ? L.op : 1.0f;
+    const uint32_t r = opac ? (uint32_t)((float)L.cr * op) : L.cr;
+    const uint32_t g = opac ? (uint32_t)((float)L.cg * op) : L.cg;
+    const uint32_t b = opac ? (uint32_t)((float)L.cb * op) : L.cb;
+    o.c = make_float4((float)r / 255.0f, (float)g / 255.0f, (float)b / 255.0f, op);
+  } else {
+    ax = ax > 1.0f ? 1.0f : ax;
+    ay = ay > 1.0f ? 1.0f : ay;
+    az = az > 1.0f ? 1.0f : az;
+    o.c = make_float4(ax, ay, az, 1.0f - Tr);
+  }
   o.steps = steps > VFR_MAX_STEPS ? VFR_MAX_STEPS : steps;
   o.entry = e;
   return o;
@@ -1022,7 +1089,8 @@ DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t
 
 template <uint32_t MODE, int PROJ, bool BIG, bool TFL>
 DEV RayOut trace(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pix) {
-  if (MODE == VFR_MODE_REF) return trace_ref(a, vi, pix);
+  if (MODE == VFR_MODE_REF) return trace_ref<false>(a, vi, pix);
+  if (MODE == VFR_MODE_REF_LINEAR) return trace_ref<true>(a, vi, pix);
   return trace_ftb<PROJ, BIG, TFL>(a, tfl, vi, pix);
 }
 
@@ -1036,8 +1104,8 @@ DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
 /* TFL: the launch's transfer-function slots [tf_lds_base, +tf_lds_n) are
  * staged in LDS once per block (every view of the launch uses one of them). */
 template <uint32_t MODE, int PROJ, bool BIG, bool TFL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR_MODE_REF ? 5 : 6,
-                                                                     MODE == VFR_MODE_REF ? 5 : 6))) void k_trav(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR_MODE_FTB ? 6 : 5,
+                                                                     MODE == VFR_MODE_FTB ? 6 : 5))) void k_trav(
     TravArgs a) {
   __shared__ float4 tfs[TFL ? VFR_TF_LDS_SLOTS * 256 : 1];
   if (TFL) {
@@ -1170,7 +1238,9 @@ int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t oc
   dim3 grid((max_rays + 255) / 256), block(256);
   hipStream_t st = (hipStream_t)stream;
   const bool tfl = a.tf_lds_n >= 1 && a.tf_lds_n <= VFR_TF_LDS_SLOTS;
-  if (mode == VFR_MODE_REF)
+  if (mode == VFR_MODE_REF && a.composite != VFR_COMPOSITE_PPI)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_REF_LINEAR, VFR_TRAV_MIXED, true, false>), grid, block, 0, st, a);
+  else if (mode == VFR_MODE_REF)
     hipLaunchKernelGGL((k_trav<VFR_MODE_REF, VFR_TRAV_MIXED, true, false>), grid, block, 0, st, a);
   else if (n_records >= VFR_BIG_RECORDS)
     hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, true, false>), grid, block, 0, st, a);
@@ -1348,6 +1418,115 @@ int vfr_k_rebase_links(TetRec* rec, uint32_t n, uint32_t add, void* stream) {
   if (n == 0 || add == 0) return VFR_OK;
   hipLaunchKernelGGL(k_rebase_links, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, rec, n, add);
   CHECK_HIP(hipGetLastError());
+  return VFR_OK;
+}
+
+/* ------------------------------------------------- rotate + renormalize */
+
+/* Order-preserving integer key of a float (-0 counts as +0), so that the
+ * min/max of the keys are the keys of the min/max under `<`. */
+DEV uint32_t fkey_ord(float f) {
+  uint32_t u = __float_as_uint(f);
+  if (u == 0x80000000u) u = 0u;
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+static inline float fkey_dec(uint32_t k) {
+  uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+struct Mat16 { float m[16]; };
+
+__global__ void k_minmax_init(uint32_t* mm) {
+  if (threadIdx.x < 8) mm[threadIdx.x] = threadIdx.x < 4 ? 0xFFFFFFFFu : 0u;
+}
+
+/* rotateVolume (vfRayPreComp.cc:886-906): row r of the result is
+ * sum_c mv[r*4+c] * (x, y, z, 1)[c], accumulated from 0 in c order; rows 0-2
+ * replace x, y, z and the scalar is kept.  With `rot` clear the vertices are
+ * only scanned.  mm receives the keys of the componentwise min (0..3) and max
+ * (4..7) of the result. */
+__global__ __launch_bounds__(256) void k_rotate_verts(float4* __restrict__ v, uint32_t n, Mat16 M, int rot,
+                                                      uint32_t* __restrict__ mm) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t lo[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[4] = {0u, 0u, 0u, 0u};
+  if (i < n) {
+    float4 p = v[i];
+    if (rot) {
+      const float o[4] = {p.x, p.y, p.z, 1.0f};
+      float r[3];
+      for (int a = 0; a < 3; ++a) {
+        float acc = 0.0f;
+        for (int c = 0; c < 4; ++c) acc = acc + M.m[a * 4 + c] * o[c];
+        r[a] = acc;
+      }
+      p.x = r[0]; p.y = r[1]; p.z = r[2];
+      v[i] = p;
+    }
+    const float q[4] = {p.x, p.y, p.z, p.w};
+    for (int a = 0; a < 4; ++a) lo[a] = hi[a] = fkey_ord(q[a]);
+  }
+  for (int a = 0; a < 4; ++a)
+    for (int o = 32; o > 0; o >>= 1) {
+      lo[a] = min(lo[a], (uint32_t)__shfl_xor((int)lo[a], o, 64));
+      hi[a] = max(hi[a], (uint32_t)__shfl_xor((int)hi[a], o, 64));
+    }
+  if ((threadIdx.x & 63u) == 0)
+    for (int a = 0; a < 4; ++a) {
+      atomicMin(&mm[a], lo[a]);
+      atomicMax(&mm[4 + a], hi[a]);
+    }
+}
+
+/* normalizeVertices' update (vfRayPreComp.cc:136-143). */
+__global__ __launch_bounds__(256) void k_normalize_verts(float4* __restrict__ v, uint32_t n, float c0, float c1,
+                                                         float c2, float sc, float s0, float ss) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = v[i];
+  p.x = (p.x - c0) * sc;
+  p.y = (p.y - c1) * sc;
+  p.z = (p.z - c2) * sc;
+  p.w = (p.w - s0) * ss;
+  v[i] = p;
+}
+
+int vfr_k_rotate_normalize(float4* v, uint32_t n, const float* mv, int normalize, float lo[3], float hi[3],
+                           void* stream) {
+  if (n == 0) return VFR_OK;
+  hipStream_t st = (hipStream_t)stream;
+  uint32_t* mm = nullptr;
+  CHECK_HIP(hipMalloc(&mm, 8 * sizeof(uint32_t)));
+  Mat16 M;
+  for (int k = 0; k < 16; ++k) M.m[k] = mv ? mv[k] : 0.0f;
+  uint32_t h[8];
+  hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, st, mm);
+  hipLaunchKernelGGL(k_rotate_verts, dim3((n + 255) / 256), dim3(256), 0, st, v, n, M, mv ? 1 : 0, mm);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h, mm, sizeof(h), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(mm);
+  CHECK_HIP(e);
+  float mn[4], mx[4];
+  for (int a = 0; a < 4; ++a) { mn[a] = fkey_dec(h[a]); mx[a] = fkey_dec(h[4 + a]); }
+  if (normalize) {
+    float center[3], sc, ss;
+    vfr_normalize_params(mn, mx, center, &sc, &ss);
+    hipLaunchKernelGGL(k_normalize_verts, dim3((n + 255) / 256), dim3(256), 0, st, v, n, center[0], center[1],
+                       center[2], sc, mn[3], ss);
+    CHECK_HIP(hipGetLastError());
+    CHECK_HIP(hipStreamSynchronize(st));
+    /* the update is monotonic in each coordinate: the extremes map to the
+       extremes */
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = (mn[a] - center[a]) * sc;
+      hi[a] = (mx[a] - center[a]) * sc;
+    }
+  } else {
+    for (int a = 0; a < 3; ++a) { lo[a] = mn[a]; hi[a] = mx[a]; }
+  }
   return VFR_OK;
 }
 

@@ -2,7 +2,8 @@
  * vfray_main.cpp -- the `vfRay <volume>` command (vfRay.cu:737-936,
  * main.cc:242-380) on top of libvfr.so.
  *
- *   vfRay [--dir D] [--size W H] [--mode ref|ftb] <volume>
+ *   vfRay [--dir D] [--size W H] [--mode ref|ftb]
+ *         [--composite ppi|linear|linear-opacity] [--rotate RX RY]... <volume>
  *
  * With no options it behaves as the reference: reads D<volume>.off with
  * D = "../tet_offs/", normalizes, then for .con (.incid on the way), .tf,
@@ -10,6 +11,12 @@
  * (.extf's count is checked against .con's header); renders 512x512 along -z
  * and writes <volume>_gpu_img.ppm in the working directory.  Connectivity and
  * external faces are built on the GPU.
+ *
+ * Each --rotate RX RY is one press of the CUDA viewer's space key with
+ * rotation angles RX, RY degrees (vfRay.cu:617-640): the mesh is rotated by
+ * Rx(RX) Ry(RY) as rotateVolume applies it and renormalized, after the cache
+ * files are written; the limits stay those of the unrotated mesh, as in the
+ * viewer.
  */
 #include <chrono>
 #include <cstdio>
@@ -27,8 +34,9 @@ static int fail(const char* what) {
 
 int main(int argc, char** argv) {
   std::string dir = "../tet_offs/";
-  uint32_t W = 512, H = 512, mode = VFR_MODE_REF;
+  uint32_t W = 512, H = 512, mode = VFR_MODE_REF, composite = VFR_COMPOSITE_PPI;
   std::vector<const char*> pos;
+  std::vector<float> rot;
   for (int i = 1; i < argc; ++i) {
     if (!strcmp(argv[i], "--dir") && i + 1 < argc) {
       dir = argv[++i];
@@ -39,6 +47,16 @@ int main(int argc, char** argv) {
     } else if (!strcmp(argv[i], "--mode") && i + 1 < argc) {
       ++i;
       mode = !strcmp(argv[i], "ftb") ? VFR_MODE_FTB : VFR_MODE_REF;
+    } else if (!strcmp(argv[i], "--rotate") && i + 2 < argc) {
+      rot.push_back((float)atof(argv[++i]));
+      rot.push_back((float)atof(argv[++i]));
+    } else if (!strcmp(argv[i], "--composite") && i + 1 < argc) {
+      /* the reference's USEPARTIALPREINT / USEOPACITY build switches
+       * (vfRay.cu:38, vfRayKernel.cu:330) as an option */
+      ++i;
+      composite = !strcmp(argv[i], "linear")           ? VFR_COMPOSITE_LINEAR
+                  : !strcmp(argv[i], "linear-opacity") ? VFR_COMPOSITE_LINEAR_OPACITY
+                                                       : VFR_COMPOSITE_PPI;
     } else {
       pos.push_back(argv[i]);
     }
@@ -137,6 +155,13 @@ int main(int argc, char** argv) {
   printf("  Max Z: %f ; Min Z: %f\n", lmt[1], lmt[2]);
   printf("  Device preprocessing: %.2f ms\n", info.build_ms);
 
+  for (size_t k = 0; k < rot.size(); k += 2) {
+    float mv[16];
+    vfr_gl_rotation(rot[k], rot[k + 1], mv);
+    printf("* Rot X = %f ; Rot Y = %f\nNormalizing vertices\n", (double)rot[k], (double)rot[k + 1]);
+    if (vfr_mesh_rotate(mesh, mv, VFR_MESH_NORMALIZE) != VFR_OK) return fail("rotate");
+  }
+
   int tf_id = 0;
   if (vfr_tf_create(ctx, tf, &tf_id) != VFR_OK) return fail("tf upload");
   vfr_view v;
@@ -149,6 +174,7 @@ int main(int argc, char** argv) {
   vfr_render_opts o;
   memset(&o, 0, sizeof(o));
   o.mode = mode;
+  o.composite = composite;
   o.schedule = VFR_SCHED_BATCHED;
   std::vector<uint8_t> rgb((size_t)W * H * 3);
   vfr_frame_out fo;
