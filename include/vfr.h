@@ -190,6 +190,12 @@ typedef struct vfr_render_opts {
  * chunks whose preprocessing overlaps the walk of the chunks before them on
  * a second stream. */
 #define VFR_RENDER_PIPELINED 0x2u
+/* Front-to-back walks evaluate each segment's psi integral in registers (a
+ * 16-term series, truncation error below 3e-8 on the reachable domain)
+ * unless this flag asks for the 512x512 table lookup (nearest entry of the
+ * two gammas, the reference's vfRayKernelCPU.cc:454-470 convention; the
+ * table is the one vfr_psi_load sets).  REF mode always uses the table. */
+#define VFR_RENDER_PSI_TABLE 0x4u
 
 typedef struct vfr_frame_out {
   float* rgba;          /* W*H*4, may be NULL                              */
@@ -248,7 +254,8 @@ typedef struct vfr_server_cfg {
   uint32_t streams;     /* slots = concurrent batches (0 = 2)                    */
   uint32_t max_batch;   /* requests per batch (0 = 256)                          */
   float term_alpha;     /* early termination (0 = 0.99)                          */
-  uint32_t flags;       /* VFR_RENDER_PERSISTENT: persistent traversal kernel    */
+  uint32_t flags;       /* VFR_RENDER_PERSISTENT: persistent traversal kernel;
+                           VFR_RENDER_PSI_TABLE: psi from the table              */
   uint32_t occupancy;   /* its schedule (0 = defaults, as vfr_render_opts)       */
   uint32_t waves_per_cu;
   uint32_t refill;

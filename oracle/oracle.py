@@ -38,6 +38,8 @@ def lib() -> C.CDLL:
         L.orc_psi_exact.restype = C.c_double
         L.orc_expf.argtypes = [C.c_float]
         L.orc_expf.restype = C.c_float
+        L.orc_psi_series.argtypes = [C.c_float, C.c_float]
+        L.orc_psi_series.restype = C.c_float
         L.orc_render_ref.argtypes = [f32p, u32, u32p, u32, u32p, u32p, u32, f32p, f32p, f32p, u32, u32,
                                      u8p, f32p, f64p, u32p, u32p, u32p, u32]
         L.orc_render_ref.restype = C.c_long
@@ -104,6 +106,10 @@ def expf(x: float) -> float:
     return lib().orc_expf(x)
 
 
+def psi_series(a: float, b: float) -> float:
+    return lib().orc_psi_series(a, b)
+
+
 def psi_exact(tf: float, tb: float) -> float:
     return lib().orc_psi_exact(tf, tb)
 
@@ -131,10 +137,13 @@ def render_ref(xyzs, tets, con, ext, tf, psi, lmt_, W, H, fp64=False, composite=
             "face_returns": int(r), "n_vis": int(nvis[0])}
 
 
-def render_ftb(xyzs, tets, con, ext, tf, psi, lmt_, proj, M, k, W, H, term=0.99, fp64=False) -> dict:
-    """The renderer's front-to-back mode restated on the CPU (OpenMP)."""
+def render_ftb(xyzs, tets, con, ext, tf, psi, lmt_, proj, M, k, W, H, term=0.99, fp64=False,
+               psi_table=False) -> dict:
+    """The renderer's front-to-back mode restated on the CPU (OpenMP); psi
+    from the series, or with psi_table from the table `psi`."""
     x, t, c, e = f32(xyzs), u32a(tets), u32a(con), u32a(ext)
-    tfa, ps, lm, Mm = f32(tf).reshape(-1), f32(psi).reshape(-1), f32(lmt_), f32(M).reshape(-1)
+    tfa, lm, Mm = f32(tf).reshape(-1), f32(lmt_), f32(M).reshape(-1)
+    ps = f32(psi).reshape(-1) if psi_table else None
     rgba = np.zeros((H, W, 4), np.float32)
     rgba64 = np.zeros((H, W, 4), np.float64) if fp64 else None
     ent = np.zeros((H, W), np.uint32)

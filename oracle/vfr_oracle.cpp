@@ -670,6 +670,32 @@ static inline u32 tf_index(float s) {
  * Z < Z_exit.  Outputs rgba (W*H*4), rgba64 (FP64 re-integration, may be
  * NULL), entry_ext (front-most entry), steps.  Returns the number of
  * anomalous rays (no exit face found), or -1 on bad input. */
+/* psi(a, b) in registers, as the renderer evaluates it: exp(-(3a + b)/8)
+ * times the midpoint series J(p, q), p = (a + b)/2, q = (b - a)/2, with the
+ * terms p^(2j) q^k, 2j + k <= 6, coefficients (-1)^k / ((2j)! k! 4^(j+k)
+ * (2j + 2k + 1)) rounded to float, Horner in q then in p^2, every step a
+ * fused multiply-add. */
+extern "C" float orc_psi_series(float a, float b) {
+  float p = (a + b) * 0.5f, q = (b - a) * 0.5f;
+  float P = p * p;
+  float c0 = fmaf(q, 2.60834003e-08f, -7.39820052e-07f);
+  c0 = fmaf(q, c0, 1.80844909e-05f);
+  c0 = fmaf(q, c0, -0.000372023816f);
+  c0 = fmaf(q, c0, 0.00625000009f);
+  c0 = fmaf(q, c0, -0.0833333358f);
+  c0 = fmaf(q, c0, 1.0f);
+  float c1 = fmaf(q, 1.84955024e-06f, -3.61689818e-05f);
+  c1 = fmaf(q, c1, 0.000558035739f);
+  c1 = fmaf(q, c1, -0.00625000009f);
+  c1 = fmaf(q, c1, 0.0416666679f);
+  float c2 = fmaf(q, 9.04224544e-06f, -9.30059541e-05f);
+  c2 = fmaf(q, c2, 0.00052083336f);
+  float J = fmaf(P, 3.10019846e-06f, c2);
+  J = fmaf(P, J, c1);
+  J = fmaf(P, J, c0);
+  return orc_expf(fmaf(3.0f, a, b) * -0.125f) * J;
+}
+
 extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, const u32* con,
                                const u32* extf, u32 nExt, const float* tf, const float* psi,
                                const float* lmt, u32 proj, const float* M, float view_k,
@@ -847,12 +873,17 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
             prev = zd;
             F4 cN = ld4(tf, tf_index(snext));
             float tf_ = cP.w * dl, tb_ = cN.w * dl;
-            /* 512 * gamma for both ends with one division */
-            float uf = 1.0f + tf_, ub = 1.0f + tb_;
-            float q = 512.0f / (uf * ub);
-            float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
-            u32 jf = (u32)(gf + 0.5f), jb = (u32)(gb + 0.5f);
-            float ps = psi[std::min(jf, 511u) * 512 + std::min(jb, 511u)];
+            float ps;
+            if (psi) {
+              /* 512 * gamma for both ends with one division */
+              float uf = 1.0f + tf_, ub = 1.0f + tb_;
+              float q = 512.0f / (uf * ub);
+              float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
+              u32 jf = (u32)(gf + 0.5f), jb = (u32)(gb + 0.5f);
+              ps = psi[std::min(jf, 511u) * 512 + std::min(jb, 511u)];
+            } else {
+              ps = orc_psi_series(tf_, tb_);
+            }
             float zeta = orc_expf(-(tf_ + tb_) * 0.5f);
             float w1 = 1.0f - ps, w2 = ps - zeta;
             ar = fmaf(Tr, fmaf(cP.x, w1, cN.x * w2), ar);

@@ -156,6 +156,31 @@ def test_psi_closed_form_against_quadrature(psi):
     assert psi[0, 0] == 1.0
 
 
+def test_psi_series_against_quadrature(psi):
+    """The in-register psi (front-to-back default) against scipy quadrature of
+    the integral: within 3e-7 on the reachable domain (optical depths <= 1),
+    1e-6 up to 1.5; the diagonal closed form (1 - e^-tau)/tau; psi(0,0) = 1;
+    and within the table's quantization of the nearest table entry."""
+    from scipy.integrate import quad
+    rng = np.random.default_rng(11)
+    pts = np.concatenate([rng.uniform(0, 1, (300, 2)), rng.uniform(0, 1.5, (100, 2)),
+                          [[0, 0], [1, 1], [0, 1], [1, 0], [1e-6, 0.5], [0.5, 1e-6]]]).astype(np.float32)
+    for a, b in pts:
+        a, b = float(a), float(b)
+        val = quad(lambda s: np.exp(-a * s - (b - a) * s * s / 2), 0, 1, epsabs=1e-15, epsrel=1e-13)[0]
+        tol = 3e-7 if max(a, b) <= 1 else 1e-6
+        assert abs(O.psi_series(a, b) - val) <= tol, (a, b, O.psi_series(a, b), val)
+    for tau in (1e-4, 0.01, 0.3, 0.7, 1.0):
+        t32 = float(np.float32(tau))
+        assert abs(O.psi_series(t32, t32) - (1 - np.exp(-t32)) / t32) < 3e-7
+    assert O.psi_series(0.0, 0.0) == 1.0
+    # the table's nearest entry is within its quantization of the integral
+    for a, b in pts[:100]:
+        i = min(511, int(512 * a / (1 + a) + 0.5))
+        j = min(511, int(512 * b / (1 + b) + 0.5))
+        assert abs(psi[i, j] - O.psi_series(float(a), float(b))) < 2e-3
+
+
 def test_expf_accuracy():
     xs = np.concatenate([-np.logspace(-8, np.log10(86.9), 4000), [0.0, -1e-30, -0.5, -1.0]]).astype(np.float32)
     got = np.array([O.expf(float(v)) for v in xs], np.float64)

@@ -141,6 +141,30 @@ def test_mesh_rotate_matches_reference(ctx, psi):
     m.close()
 
 
+def test_ftb_psi_table(ctx, c1, psi):
+    """VFR_RENDER_PSI_TABLE: front-to-back with psi looked up in the table, bit
+    for bit with the oracle's table variant, for the one-ray-per-thread and
+    the persistent kernels; the series default stays within the table's
+    quantization of it and closer to the FP64 re-integration."""
+    print("FTB psi table 16^3", flush=True)
+    m = ctx.mesh(c1["x"], c1["t"])
+    tf = ctx.tf(c1["tf"])
+    views = [V.ortho_view(512, 512, tf), V.ortho_view(300, 200, tf)]
+    outs, _ = ctx.render(m, views, mode=V.MODE_FTB, psi_table=True)
+    outp, _ = ctx.render(m, views, mode=V.MODE_FTB, psi_table=True, persistent=True)
+    outs_s, _ = ctx.render(m, views, mode=V.MODE_FTB)
+    for v, g, gp, gs in zip(views, outs, outp, outs_s):
+        orc = oracle_ftb(c1, v, psi, fp64=True, psi_table=True)
+        assert np.array_equal(g["entry_ext"], orc["entry_ext"])
+        assert np.array_equal(g["steps"], orc["steps"])
+        assert np.array_equal(g["rgba"], orc["rgba"])
+        assert np.array_equal(gp["rgba"], g["rgba"]) and np.array_equal(gp["steps"], g["steps"])
+        cov = orc["entry_ext"] != V.NO_ID
+        assert np.abs(gs["rgba"] - g["rgba"]).max() <= TOL
+        assert psnr(gs["rgba"], orc["rgba64"], cov) > psnr(g["rgba"], orc["rgba64"], cov)
+    m.close()
+
+
 @pytest.mark.parametrize("schedule", [V.SCHED_PER_VIEW, V.SCHED_BATCHED])
 def test_ftb_ortho_c1(ctx, c1, psi, schedule):
     print(f"FTB ortho 16^3 schedule={schedule}", flush=True)

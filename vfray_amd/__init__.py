@@ -20,6 +20,7 @@ MODE_REF, MODE_FTB = 0, 1
 COMPOSITE_PPI, COMPOSITE_LINEAR, COMPOSITE_LINEAR_OPACITY = 0, 1, 2
 RENDER_PERSISTENT = 0x1
 RENDER_PIPELINED = 0x2
+RENDER_PSI_TABLE = 0x4
 SCHED_PER_VIEW, SCHED_BATCHED = 0, 1
 MESH_NORMALIZE, MESH_MORTON = 0x1, 0x2
 NO_ID = 0xFFFFFFFF
@@ -370,9 +371,9 @@ class Context:
 
     def plan(self, mesh: "Mesh", views, mode=MODE_FTB, schedule=SCHED_BATCHED, term_alpha=0.99,
              use_graph=False, waves_per_cu=0, refill=0, persistent=False, occupancy=0,
-             max_entries=0, pipelined=False, composite=COMPOSITE_PPI) -> "Plan":
+             max_entries=0, pipelined=False, composite=COMPOSITE_PPI, psi_table=False) -> "Plan":
         return Plan(self, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
-                    persistent, occupancy, max_entries, pipelined, composite)
+                    persistent, occupancy, max_entries, pipelined, composite, psi_table)
 
     def render(self, mesh: "Mesh", views, **kw) -> tuple[list[dict], dict]:
         p = self.plan(mesh, views, **kw)
@@ -425,11 +426,13 @@ class Mesh:
 
 class Plan:
     def __init__(self, ctx, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
-                 persistent=False, occupancy=0, max_entries=0, pipelined=False, composite=COMPOSITE_PPI):
+                 persistent=False, occupancy=0, max_entries=0, pipelined=False, composite=COMPOSITE_PPI,
+                 psi_table=False):
         self.ctx, self.mesh = ctx, mesh
         self.views = list(views)
         arr = (View * len(self.views))(*self.views)
-        flags = (RENDER_PERSISTENT if persistent else 0) | (RENDER_PIPELINED if pipelined else 0)
+        flags = ((RENDER_PERSISTENT if persistent else 0) | (RENDER_PIPELINED if pipelined else 0)
+                 | (RENDER_PSI_TABLE if psi_table else 0))
         o = RenderOpts(mode, schedule, term_alpha, waves_per_cu, refill, 1 if use_graph else 0, max_entries,
                        flags, occupancy, composite)
         self._p = C.c_void_p()
@@ -482,12 +485,12 @@ class Server:
 
     def __init__(self, ctx: Context, meshes, hbm_budget: int = 0, streams: int = 0, max_batch: int = 0,
                  term_alpha: float = 0.0, persistent: bool = False, occupancy: int = 0, waves_per_cu: int = 0,
-                 refill: int = 0):
+                 refill: int = 0, psi_table: bool = False):
         self.ctx = ctx
         self.meshes = list(meshes)
         arr = (C.c_void_p * len(self.meshes))(*[m._p.value for m in self.meshes])
-        cfg = ServerCfg(hbm_budget, streams, max_batch, term_alpha, RENDER_PERSISTENT if persistent else 0,
-                        occupancy, waves_per_cu, refill)
+        flags = (RENDER_PERSISTENT if persistent else 0) | (RENDER_PSI_TABLE if psi_table else 0)
+        cfg = ServerCfg(hbm_budget, streams, max_batch, term_alpha, flags, occupancy, waves_per_cu, refill)
         self._p = C.c_void_p()
         _check(lib().vfr_server_create(ctx._p, C.byref(cfg), arr, len(self.meshes), C.byref(self._p)))
 

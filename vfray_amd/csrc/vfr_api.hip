@@ -486,6 +486,7 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   a.tf_lds_base = p->tf_lo;
   a.tf_lds_n = p->tf_n;
   a.composite = p->opts.composite;
+  a.psi_table = (p->opts.flags & VFR_RENDER_PSI_TABLE) ? 1u : 0u;
   return a;
 }
 
@@ -917,7 +918,10 @@ struct TuneKey {
   uint32_t n_tets;
   uint64_t n_pix;
   uint32_t n_views;
-  bool operator==(const TuneKey& o) const { return n_tets == o.n_tets && n_pix == o.n_pix && n_views == o.n_views; }
+  uint32_t psi_table;
+  bool operator==(const TuneKey& o) const {
+    return n_tets == o.n_tets && n_pix == o.n_pix && n_views == o.n_views && psi_table == o.psi_table;
+  }
 };
 struct TuneVal {
   uint32_t flags, occ, wpc, refill;
@@ -937,7 +941,8 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
     return VFR_ESTATE;
   }
   if (p->gexec) { vfr_set_error("autotune before graph capture (plan has a graph)"); return VFR_ESTATE; }
-  TuneKey key{p->mesh->d.n_tets, p->n_pix, p->n_views};
+  const uint32_t table = (p->opts.flags & VFR_RENDER_PSI_TABLE) ? 1u : 0u;
+  TuneKey key{p->mesh->d.n_tets, p->n_pix, p->n_views, table};
   bool cached = false;
   {
     std::lock_guard<std::mutex> g(g_tune_mu);

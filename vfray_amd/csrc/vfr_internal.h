@@ -126,6 +126,7 @@ struct TravArgs {
   uint32_t tf_lds_n;       /* staged in LDS (0: read the global table)         */
   uint32_t kmax;           /* FTB: entry slots per pixel (8, 16 or 32)         */
   uint32_t composite;      /* REF: VFR_COMPOSITE_*                             */
+  uint32_t psi_table;      /* FTB: psi from the table (VFR_RENDER_PSI_TABLE)     */
 };
 
 /* ---- launchers (vfr_kernels.hip) ---- */

@@ -844,6 +844,36 @@ DEV void face_values(bool persp, float D, float wA, float4 A, float wB, float4 B
   *s = clamp01(sn * rc);
 }
 
+/* psi(a, b) = int_0^1 exp(-a s - (b - a) s^2 / 2) ds, the partial
+ * pre-integration integral of psiGammaTable512.h:12-19 (a = optical depth at
+ * the front end, b at the back), evaluated in registers.  With s = 1/2 + t,
+ * psi = exp(-(3a + b) / 8) J, J = int_{-1/2}^{1/2} exp(-p t - q t^2) dt,
+ * p = (a + b) / 2, q = (b - a) / 2, and
+ *   J = sum_{j,k} p^(2j) (-q)^k / ((2j)! k! 4^(j+k) (2j + 2k + 1)),
+ * kept to 2j + k <= 6 (16 terms, Horner in q, then in p^2): truncation error
+ * below 3e-8 for a, b <= 1 (a segment's depth: alpha <= 1 over at most the
+ * tet's longest edge, the limit's maxEdgeLength) and 4e-7 at 1.5. */
+DEV float psi_series(float a, float b) {
+  const float p = (a + b) * 0.5f, q = (b - a) * 0.5f;
+  const float P = p * p;
+  float c0 = __builtin_fmaf(q, 2.60834003e-08f, -7.39820052e-07f);
+  c0 = __builtin_fmaf(q, c0, 1.80844909e-05f);
+  c0 = __builtin_fmaf(q, c0, -0.000372023816f);
+  c0 = __builtin_fmaf(q, c0, 0.00625000009f);
+  c0 = __builtin_fmaf(q, c0, -0.0833333358f);
+  c0 = __builtin_fmaf(q, c0, 1.0f);
+  float c1 = __builtin_fmaf(q, 1.84955024e-06f, -3.61689818e-05f);
+  c1 = __builtin_fmaf(q, c1, 0.000558035739f);
+  c1 = __builtin_fmaf(q, c1, -0.00625000009f);
+  c1 = __builtin_fmaf(q, c1, 0.0416666679f);
+  float c2 = __builtin_fmaf(q, 9.04224544e-06f, -9.30059541e-05f);
+  c2 = __builtin_fmaf(q, c2, 0.00052083336f);
+  float J = __builtin_fmaf(P, 3.10019846e-06f, c2);
+  J = __builtin_fmaf(P, J, c1);
+  J = __builtin_fmaf(P, J, c0);
+  return vfr_expf(__builtin_fmaf(3.0f, a, b) * -0.125f) * J;
+}
+
 /* Transfer-function colour `i` (slot offset included): from the block's LDS
  * copy of the launch's TF slots (TFL), or from the global table. */
 template <bool TFL>
@@ -919,15 +949,18 @@ enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
  * The body is one straight path that updates the ray state in place, so no
  * copies of the state are needed where paths join, and it is safe to run on
  * a finished ray (every index it forms stays inside the mesh arrays; the
- * result is ignored).  Its loads go out in two rounds: the exit face's link
- * and the transfer-function colour, then the psi entry, the next tet's
- * fourth vertex and vertex ids.  The psi lookup goes first in the second
- * round because memory results return in issue order and it is the one
- * waited for within the step. */
+ * result is ignored).  Its loads go out as soon as the exit face is known:
+ * the transfer-function colour at the exit point, then the next tet's
+ * fourth vertex and record, which the next step waits for; psi is evaluated
+ * in registers meanwhile (with PSIT it is a second round of loads, the
+ * table entry, waited for within the step). */
 /* PROJ: VFR_TRAV_*, the projection of every ray of the launch.  BIG: the
  * record arena reaches 4 GiB (VFR_BIG_RECORDS tets or more), so records are
- * addressed with 64-bit arithmetic instead of 32-bit byte offsets. */
-template <int PROJ, bool BIG, bool TFL>
+ * addressed with 64-bit arithmetic instead of 32-bit byte offsets.  PSIT:
+ * psi is looked up in the 512x512 table (nearest entry of gamma = tau /
+ * (1 + tau) at both ends, the reference's convention) instead of evaluated
+ * by psi_series. */
+template <int PROJ, bool BIG, bool TFL, bool PSIT>
 DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const bool persp = PROJ == VFR_TRAV_MIXED ? ftb_persp(r) : PROJ == VFR_TRAV_PERSP;
   r.steps += 1u;
@@ -1001,12 +1034,17 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const float dl = fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
   const float4 cP = TFL ? tfl[r.cpi] : r.cP;
   const float tf_ = cP.w * dl, tb_ = cN.w * dl;
-  /* 512 * gamma = 512 * tau / (1 + tau) for both ends with one division */
-  const float uf = 1.0f + tf_, ub = 1.0f + tb_;
-  const float q = 512.0f / (uf * ub);
-  const float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
-  const float ps = a.psi[min(511u, (uint32_t)(gf + 0.5f)) * 512u + min(511u, (uint32_t)(gb + 0.5f))];
-  __builtin_amdgcn_sched_barrier(0);
+  float ps;
+  if (PSIT) {
+    /* 512 * gamma = 512 * tau / (1 + tau) for both ends with one division */
+    const float uf = 1.0f + tf_, ub = 1.0f + tb_;
+    const float q = 512.0f / (uf * ub);
+    const float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
+    ps = a.psi[min(511u, (uint32_t)(gf + 0.5f)) * 512u + min(511u, (uint32_t)(gb + 0.5f))];
+    __builtin_amdgcn_sched_barrier(0);
+  } else {
+    ps = psi_series(tf_, tb_);
+  }
   const float zeta = vfr_expf(-(tf_ + tb_) * 0.5f);
   /* the exit face becomes the entry face: vertex k is replaced by P3 */
   const float n01 = pick3(k0, k1, t1, -t0, r.o01);
@@ -1073,13 +1111,13 @@ DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
   return o;
 }
 
-template <int PROJ, bool BIG, bool TFL>
+template <int PROJ, bool BIG, bool TFL, bool PSIT>
 DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pix) {
   FtbRay r;
   ftb_init<TFL>(a, tfl, vi, pix, r);
   if (!(r.bits & FTB_DEAD)) {
     while (true) {
-      const uint32_t st = ftb_step<PROJ, BIG, TFL>(a, tfl, r);
+      const uint32_t st = ftb_step<PROJ, BIG, TFL, PSIT>(a, tfl, r);
       if (st == FTB_GO || (st == FTB_REENTER && ftb_reenter<TFL>(a, tfl, r))) continue;
       break;
     }
@@ -1087,11 +1125,11 @@ DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t
   return ftb_out(a, r);
 }
 
-template <uint32_t MODE, int PROJ, bool BIG, bool TFL>
+template <uint32_t MODE, int PROJ, bool BIG, bool TFL, bool PSIT>
 DEV RayOut trace(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pix) {
   if (MODE == VFR_MODE_REF) return trace_ref<false>(a, vi, pix);
   if (MODE == VFR_MODE_REF_LINEAR) return trace_ref<true>(a, vi, pix);
-  return trace_ftb<PROJ, BIG, TFL>(a, tfl, vi, pix);
+  return trace_ftb<PROJ, BIG, TFL, PSIT>(a, tfl, vi, pix);
 }
 
 DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
@@ -1103,7 +1141,7 @@ DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
 /* One ray per thread over the compacted ray list. */
 /* TFL: the launch's transfer-function slots [tf_lds_base, +tf_lds_n) are
  * staged in LDS once per block (every view of the launch uses one of them). */
-template <uint32_t MODE, int PROJ, bool BIG, bool TFL>
+template <uint32_t MODE, int PROJ, bool BIG, bool TFL, bool PSIT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR_MODE_FTB ? 6 : 5,
                                                                      MODE == VFR_MODE_FTB ? 6 : 5))) void k_trav(
     TravArgs a) {
@@ -1118,7 +1156,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR
   if (i < n) {
     const uint64_t ray = a.rays[i];
     const uint32_t vi = (uint32_t)(ray >> 32), pix = (uint32_t)ray;
-    const RayOut o = trace<MODE, PROJ, BIG, TFL>(a, tfs, vi, pix);
+    const RayOut o = trace<MODE, PROJ, BIG, TFL, PSIT>(a, tfs, vi, pix);
     store_ray(a, a.views[vi].pix_off + pix, o);
     st = o.steps; an = o.anomaly; br = o.broken;
   }
@@ -1139,7 +1177,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR
  * prefix count), then advances every lane `burst` tets.  A wave exits once
  * the queue is drained and all its lanes are idle; every ray ends within
  * VFR_MAX_STEPS tets, so every wave reaches that exit. */
-template <int WPE>
+template <int WPE, bool PSIT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_trav_persist(
     TravArgs a, uint32_t burst) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -1187,7 +1225,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     /* every lane steps (an idle lane's stale ray stays inside the mesh
      * arrays and its result is ignored), so the walk is one straight path */
     for (uint32_t k = 0; k < burst; ++k) {
-      const uint32_t s = ftb_step<VFR_TRAV_MIXED, true, false>(a, nullptr, r);
+      const uint32_t s = ftb_step<VFR_TRAV_MIXED, true, false, PSIT>(a, nullptr, r);
       if (active && s != FTB_GO && !(s == FTB_REENTER && ftb_reenter<false>(a, nullptr, r))) {
         const RayOut o = ftb_out(a, r);
         store_ray(a, r.gp, o);
@@ -1204,17 +1242,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   }
 }
 
+template <int WPE>
+static void launch_persist(uint32_t blocks, void* stream, const TravArgs& a, uint32_t burst) {
+  if (a.psi_table)
+    hipLaunchKernelGGL((k_trav_persist<WPE, true>), dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
+  else
+    hipLaunchKernelGGL((k_trav_persist<WPE, false>), dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
+}
+
 int vfr_k_traverse_waves_per_simd(uint32_t occupancy, uint32_t* out) {
   int blocks = 0;
   hipError_t e;
-  if (occupancy >= 6) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trav_persist<6>, 256, 0);
-  else if (occupancy == 5) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trav_persist<5>, 256, 0);
-  else if (occupancy == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trav_persist<4>, 256, 0);
-  else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trav_persist<1>, 256, 0);
+  if (occupancy >= 6) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trav_persist<6, false>, 256, 0);
+  else if (occupancy == 5) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trav_persist<5, false>, 256, 0);
+  else if (occupancy == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trav_persist<4, false>, 256, 0);
+  else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trav_persist<1, false>, 256, 0);
   CHECK_HIP(e);
   /* a 256-thread block is 4 waves, one per SIMD of the CU */
   *out = blocks < 1 ? 1u : (uint32_t)blocks;
   return VFR_OK;
+}
+
+template <bool PSIT>
+static void launch_ftb(uint32_t proj, bool tfl, dim3 grid, dim3 block, hipStream_t st, const TravArgs& a) {
+  if (proj == VFR_TRAV_ORTHO && tfl)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_ORTHO, false, true, PSIT>), grid, block, 0, st, a);
+  else if (proj == VFR_TRAV_ORTHO)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_ORTHO, false, false, PSIT>), grid, block, 0, st, a);
+  else if (proj == VFR_TRAV_PERSP && tfl)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_PERSP, false, true, PSIT>), grid, block, 0, st, a);
+  else if (proj == VFR_TRAV_PERSP)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_PERSP, false, false, PSIT>), grid, block, 0, st, a);
+  else if (tfl)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false, true, PSIT>), grid, block, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false, false, PSIT>), grid, block, 0, st, a);
 }
 
 int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t occupancy, const TravArgs& a,
@@ -1225,13 +1287,13 @@ int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t oc
     if (blocks == 0) blocks = 1;
     const uint32_t burst = 8;
     if (occupancy >= 6)
-      hipLaunchKernelGGL(k_trav_persist<6>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
+      launch_persist<6>(blocks, stream, a, burst);
     else if (occupancy == 5)
-      hipLaunchKernelGGL(k_trav_persist<5>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
+      launch_persist<5>(blocks, stream, a, burst);
     else if (occupancy == 4)
-      hipLaunchKernelGGL(k_trav_persist<4>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
+      launch_persist<4>(blocks, stream, a, burst);
     else
-      hipLaunchKernelGGL(k_trav_persist<1>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a, burst);
+      launch_persist<1>(blocks, stream, a, burst);
     CHECK_HIP(hipGetLastError());
     return VFR_OK;
   }
@@ -1239,23 +1301,17 @@ int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t oc
   hipStream_t st = (hipStream_t)stream;
   const bool tfl = a.tf_lds_n >= 1 && a.tf_lds_n <= VFR_TF_LDS_SLOTS;
   if (mode == VFR_MODE_REF && a.composite != VFR_COMPOSITE_PPI)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_REF_LINEAR, VFR_TRAV_MIXED, true, false>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_trav<VFR_MODE_REF_LINEAR, VFR_TRAV_MIXED, true, false, false>), grid, block, 0, st, a);
   else if (mode == VFR_MODE_REF)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_REF, VFR_TRAV_MIXED, true, false>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_trav<VFR_MODE_REF, VFR_TRAV_MIXED, true, false, false>), grid, block, 0, st, a);
+  else if (n_records >= VFR_BIG_RECORDS && a.psi_table)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, true, false, true>), grid, block, 0, st, a);
   else if (n_records >= VFR_BIG_RECORDS)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, true, false>), grid, block, 0, st, a);
-  else if (proj == VFR_TRAV_ORTHO && tfl)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_ORTHO, false, true>), grid, block, 0, st, a);
-  else if (proj == VFR_TRAV_ORTHO)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_ORTHO, false, false>), grid, block, 0, st, a);
-  else if (proj == VFR_TRAV_PERSP && tfl)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_PERSP, false, true>), grid, block, 0, st, a);
-  else if (proj == VFR_TRAV_PERSP)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_PERSP, false, false>), grid, block, 0, st, a);
-  else if (tfl)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false, true>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, true, false, false>), grid, block, 0, st, a);
+  else if (a.psi_table)
+    launch_ftb<true>(proj, tfl, grid, block, st, a);
   else
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false, false>), grid, block, 0, st, a);
+    launch_ftb<false>(proj, tfl, grid, block, st, a);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }

@@ -214,6 +214,7 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   a.term = sv->cfg.term_alpha;
   a.refill = sv->cfg.refill;
   a.kmax = VFR_K;
+  a.psi_table = (sv->cfg.flags & VFR_RENDER_PSI_TABLE) ? 1u : 0u;
   if (tf_hi - tf_lo < VFR_TF_LDS_SLOTS) { a.tf_lds_base = tf_lo; a.tf_lds_n = tf_hi - tf_lo + 1; }
   const uint32_t proj = n_persp == 0 ? VFR_TRAV_ORTHO : (n_persp == nv ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
   CHECK_RC(vfr_k_traverse(VFR_MODE_FTB, proj, sv->n_records, sv->cfg.occupancy, a, (uint32_t)pix,
