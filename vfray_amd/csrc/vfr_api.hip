@@ -192,6 +192,11 @@ extern "C" int vfr_mesh_create(vfr_ctx* c, const float* xyzs, uint32_t nV, const
     vfr_set_error("vfr_mesh_create: %u tets exceeds the limit of %u", nT, VFR_MAX_TETS - 1u);
     return VFR_EINVAL;
   }
+  if (nV >= VFR_MAX_VERTS) {
+    /* tet records hold vertex ids in VFR_REC_VERT_BITS bits */
+    vfr_set_error("vfr_mesh_create: %u vertices exceeds the limit of %u", nV, VFR_MAX_VERTS - 1u);
+    return VFR_EINVAL;
+  }
   if (extf && !con) { vfr_set_error("vfr_mesh_create: extf given without con"); return VFR_EINVAL; }
   *out = nullptr;
   for (size_t i = 0; i < 4 * (size_t)nT; ++i)

@@ -45,15 +45,21 @@ double vfr_psi_integral(double tf, double tb);
 
 /* 64-byte tet record, one per tet, 64-byte aligned: the 4 vertex ids, then
  * per face f (the vertices of slots f, f+1, f+2) the packed neighbour word
- * and the neighbour's vertex off that face (the tet's own opposite vertex
- * when there is no linked neighbour).  A walk step reads the exit face's
- * pair from the record it already holds in cache and the next tet's ids
- * from one line. */
+ * and the face's vertex word: the neighbour's vertex off that face (the
+ * tet's own opposite vertex when there is no linked neighbour) in the low
+ * VFR_REC_VERT_BITS bits and above them, in 2-bit fields p = 0, 1, 2, the
+ * neighbour's record slot of this tet's vertex at slot f + p.  A
+ * front-to-back step needs only the face words (32 bytes, two loads): it
+ * carries the record slots of its face vertices instead of their ids. */
 struct alignas(64) TetRec {
   uint32_t v[4];
-  uint32_t f[8];   /* face k: f[2k] neighbour word, f[2k+1] fourth vertex */
+  uint32_t f[8];   /* face k: f[2k] neighbour word, f[2k+1] vertex word */
   uint32_t pad[4];
 };
+#define VFR_REC_VERT_BITS 26u
+#define VFR_REC_VERT_MASK ((1u << VFR_REC_VERT_BITS) - 1u)
+/* A mesh's vertex ids fit the vertex word. */
+#define VFR_MAX_VERTS (1u << VFR_REC_VERT_BITS)
 static_assert(sizeof(TetRec) == 64, "TetRec is one 64-byte line");
 
 /* One view of a render batch, with the offsets of its mesh in the device

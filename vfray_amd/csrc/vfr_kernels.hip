@@ -794,9 +794,7 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
 struct FtbRay {
   float4 P0, P1, P2;       /* entry face: (X - ox, Y - oy, Z, S) */
   float4 Q;                /* the tet's fourth vertex, P space (prefetched) */
-  uint3 rv;                /* the tet's first three vertex ids (prefetched) */
   uint4 fa, fb;            /* its face words f[0..7] (prefetched) */
-  uint32_t i0, i1, i2, i3; /* vertex ids of P0, P1, P2, Q */
   float o01, o12, o20;     /* edge functions P0->P1, P1->P2, P2->P0 (>= 0) */
   float4 cP;               /* TF colour at the entry point ...               */
   uint32_t cpi;            /* ... or its index, when the TF table is in LDS */
@@ -804,7 +802,8 @@ struct FtbRay {
   float ar, ag, ab, Tr;
   uint32_t t;              /* the tet, as an index into the record arena */
   uint32_t vi, voff, gp, tfo, steps;
-  uint32_t bits; /* cur 0..5 | nk 6..11 | FTB_PERSP | FTB_DEAD | FTB_BROKEN */
+  uint32_t bits; /* cur 0..5 | nk 6..11 | FTB_PERSP | FTB_DEAD | FTB_BROKEN |
+                    record slots of P0, P1, P2, Q: 2 bits each from bit 16 */
 };
 
 #define FTB_PERSP (1u << 12)  /* perspective view                        */
@@ -891,23 +890,24 @@ DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
   r.t = a.views[r.vi].rec_off + a.ext_int[2 * (size_t)e];
   const uint32_t fe = a.ext_int[2 * (size_t)e + 1];
   const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[r.t].v);
-  r.rv = make_uint3(rv.x, rv.y, rv.z);
   r.fa = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[0]);
   r.fb = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[4]);
-  uint32_t i0 = sel4(rv, fe), i1 = sel4(rv, (fe + 1u) & 3u), i2 = sel4(rv, (fe + 2u) & 3u);
-  r.i3 = sel4(rv, (fe + 3u) & 3u);
-  r.Q = pv[r.i3];
-  float4 P0 = rel(pv[i0], r.ox, r.oy), P1 = rel(pv[i1], r.ox, r.oy), P2 = rel(pv[i2], r.ox, r.oy);
+  /* face fe = record slots fe, fe+1, fe+2; the fourth vertex is slot fe+3 */
+  uint32_t s1 = (fe + 1u) & 3u, s2 = (fe + 2u) & 3u;
+  const uint32_t s3 = (fe + 3u) & 3u;
+  r.Q = pv[sel4(rv, s3)];
+  float4 P0 = rel(pv[sel4(rv, fe)], r.ox, r.oy), P1 = rel(pv[sel4(rv, s1)], r.ox, r.oy),
+         P2 = rel(pv[sel4(rv, s2)], r.ox, r.oy);
   float o01 = edge_rel(P0, P1), o12 = edge_rel(P1, P2), o20 = edge_rel(P2, P0);
   if (o01 + o12 + o20 < 0.0f) {
     /* order (P0, P2, P1): every edge function flips sign exactly */
     const float4 tP = P1; P1 = P2; P2 = tP;
-    const uint32_t ti = i1; i1 = i2; i2 = ti;
+    const uint32_t ts = s1; s1 = s2; s2 = ts;
     const float n01 = -o20, n12 = -o12, n20 = -o01;
     o01 = n01; o12 = n12; o20 = n20;
   }
   r.P0 = P0; r.P1 = P1; r.P2 = P2;
-  r.i0 = i0; r.i1 = i1; r.i2 = i2;
+  r.bits = (r.bits & 0xFFFFu) | ((fe | (s1 << 2) | (s2 << 4) | (s3 << 6)) << 16);
   r.o01 = o01; r.o12 = o12; r.o20 = o20;
   const float D = (o12 + o20) + o01;
   float sprev;
@@ -1001,15 +1001,29 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const float D = ((w3 + w0) + w1) + w2;
   float zd, snext;
   face_values(persp, D, w3, P3, w0, r.P0, w1, r.P1, w2, r.P2, &zd, &snext);
-  /* the exit face leaves out vertex k; its face index is slot(k) + 1 */
+  /* the exit face leaves out vertex k of the entry face; in the record it
+   * is the face opposite k's slot, face (slot + 1) & 3 */
   const bool k0 = c2, k1 = !c1 && !c2, k2 = c1;
-  const uint32_t ik = pick3u(c1, c2, r.i2, r.i0, r.i1);
-  /* the exit face's link (neighbour word, fourth vertex) from the record
-   * words held in registers: slot 0 -> face 1, 1 -> 2, 2 -> 3, 3 -> 0 */
-  const bool s0 = r.rv.x == ik, s1 = r.rv.y == ik, s2 = r.rv.z == ik;
+  const uint32_t sl = r.bits >> 16;
+  const uint32_t kk = c1 ? 4u : (c2 ? 0u : 2u); /* 2k */
+  const uint32_t fx = ((sl >> kk) + 1u) & 3u;
+  /* its link (neighbour word; the neighbour's fourth vertex with, above bit
+   * 26, the neighbour's slots of this face's vertices) from the record words
+   * held in registers */
   const uint4 fa = r.fa, fb = r.fb;
-  const uint32_t nb = pick3u(s0, s1, fa.z, fb.x, pick3u(s2, false, fb.z, 0u, fa.x));
-  const uint32_t i3n = pick3u(s0, s1, fa.w, fb.y, pick3u(s2, false, fb.w, 0u, fa.y));
+  const bool e0 = fx == 0u, e1 = fx == 1u, e2 = fx == 2u;
+  const uint32_t nb = pick3u(e0, e1, fa.x, fa.z, e2 ? fb.x : fb.z);
+  const uint32_t qw = pick3u(e0, e1, fa.y, fa.w, e2 ? fb.y : fb.w);
+  const uint32_t i3n = qw & VFR_REC_VERT_MASK, pm = qw >> VFR_REC_VERT_BITS;
+  /* slots in the neighbour's record: vertex k's slot becomes the fourth
+   * vertex's, each face vertex at slot x maps to field (x - fx) & 3 of pm,
+   * and the neighbour's fourth vertex is opposite its back face */
+  const uint32_t xs = (sl & ~(3u << kk)) | (((sl >> 6) & 3u) << kk);
+  const uint32_t n0 = (pm >> (2u * ((xs - fx) & 3u))) & 3u;
+  const uint32_t n1 = (pm >> (2u * (((xs >> 2) - fx) & 3u))) & 3u;
+  const uint32_t n2 = (pm >> (2u * (((xs >> 4) - fx) & 3u))) & 3u;
+  const uint32_t n3 = (nb + 3u) & 3u;
+  const uint32_t sln = n0 | (n1 << 2) | (n2 << 4) | (n3 << 6);
   /* neighbour words index the record arena (a server rebases each resident
    * mesh's words when it copies them in) */
   const uint32_t tn = nb < VFR_NB_BROKEN ? (nb >> 2) : r.t;
@@ -1018,14 +1032,11 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const uint32_t ci = r.tfo + min(255u, (uint32_t)(255.0f * snext));
   const float4 cN = tf_at<TFL>(a, tfl, ci);
   const float4 Qn = ld32<float4>(a.vert, (r.voff + i3n) * 16u);
-  uint3 rvn;
   uint4 fan, fbn;
   if (BIG) {
-    rvn = *reinterpret_cast<const uint3*>(a.rec[tn].v);
     fan = *reinterpret_cast<const uint4*>(&a.rec[tn].f[0]);
     fbn = *reinterpret_cast<const uint4*>(&a.rec[tn].f[4]);
   } else {
-    rvn = ld32<uint3>(a.rec, tn * 64u);
     fan = ld32<uint4>(a.rec, tn * 64u + 16u);
     fbn = ld32<uint4>(a.rec, tn * 64u + 32u);
   }
@@ -1053,9 +1064,7 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
   r.P0 = sel4f(k0, P3, r.P0);
   r.P1 = sel4f(k1, P3, r.P1);
   r.P2 = sel4f(k2, P3, r.P2);
-  r.i0 = k0 ? r.i3 : r.i0;
-  r.i1 = k1 ? r.i3 : r.i1;
-  r.i2 = k2 ? r.i3 : r.i2;
+  r.bits = (r.bits & 0xFFFFu) | (sln << 16);
   r.o01 = n01; r.o12 = n12; r.o20 = n20;
   /* composite the segment */
   const float w1c = 1.0f - ps, w2c = ps - zeta;
@@ -1070,9 +1079,7 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
   if (TFL) r.cpi = ci; else r.cP = cN;
   r.prev = zd;
   r.t = tn;
-  r.i3 = i3n;
   r.Q = Qn;
-  r.rv = rvn;
   r.fa = fan;
   r.fb = fbn;
   const bool term = 1.0f - r.Tr >= a.term;
@@ -1447,7 +1454,18 @@ __global__ void k_build_rec(const uint32_t* __restrict__ tets, const uint32_t* _
     for (uint32_t q = 0; q < 4; ++q)
       if (con[u * 4 + q] == t) { kk = q; break; }
     n[f] = kk == 4 ? VFR_NB_BROKEN : ((tinv[u] << 2) | kk);
-    if (kk != 4) o[f] = vinv[tets[u * 4 + ((kk + 3) & 3)]];
+    if (kk != 4) {
+      o[f] = vinv[tets[u * 4 + ((kk + 3) & 3)]];
+      /* the neighbour's slot of each vertex of this face */
+      uint32_t pm = 0;
+      for (uint32_t p = 0; p < 3; ++p) {
+        const uint32_t id = tets[t * 4 + ((f + p) & 3)];
+        uint32_t q = 0;
+        while (q < 3 && tets[u * 4 + q] != id) ++q;
+        pm |= q << (2 * p);
+      }
+      o[f] |= pm << VFR_REC_VERT_BITS;
+    }
   }
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
