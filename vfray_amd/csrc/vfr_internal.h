@@ -25,9 +25,9 @@ double vfr_psi_integral(double tf, double tb);
  * vertices (16 B each) with 32-bit byte offsets, so the vertices of all
  * views of one launch stay below VFR_MAX_VIEW_VERTS. */
 #define VFR_MAX_TETS (1u << 29)
-/* Record arenas below 4 GiB (64 B per tet) are addressed with 32-bit byte
+/* Record arenas below 4 GiB (32 B per tet) are addressed with 32-bit byte
  * offsets by the fastest traversal kernels. */
-#define VFR_BIG_RECORDS (1ull << 26)
+#define VFR_BIG_RECORDS (1ull << 27)
 #define VFR_MAX_VIEW_VERTS (1ull << 28)
 
 /* View chunks of a pipelined batched render. */
@@ -43,24 +43,23 @@ double vfr_psi_integral(double tf, double tb);
 
 #define VFR_NO_ID 0xFFFFFFFFu
 
-/* 64-byte tet record, one per tet, 64-byte aligned: the 4 vertex ids, then
- * per face f (the vertices of slots f, f+1, f+2) the packed neighbour word
- * and the face's vertex word: the neighbour's vertex off that face (the
- * tet's own opposite vertex when there is no linked neighbour) in the low
- * VFR_REC_VERT_BITS bits and above them, in 2-bit fields p = 0, 1, 2, the
- * neighbour's record slot of this tet's vertex at slot f + p.  A
- * front-to-back step needs only the face words (32 bytes, two loads): it
- * carries the record slots of its face vertices instead of their ids. */
-struct alignas(64) TetRec {
-  uint32_t v[4];
+/* 32-byte face record, one per tet, 32-byte aligned: per face f (the
+ * vertices of slots f, f+1, f+2 of the tet's vertex ids, kept beside in a
+ * uint4 array) the packed neighbour word and the face's vertex word: the
+ * neighbour's vertex off that face (the tet's own opposite vertex when there
+ * is no linked neighbour) in the low VFR_REC_VERT_BITS bits and above them,
+ * in 2-bit fields p = 0, 1, 2, the neighbour's slot of this tet's vertex at
+ * slot f + p.  A front-to-back step reads only face records (two loads from
+ * one line; four tets per 128-byte line): it carries the slots of its face
+ * vertices instead of their ids. */
+struct alignas(32) TetRec {
   uint32_t f[8];   /* face k: f[2k] neighbour word, f[2k+1] vertex word */
-  uint32_t pad[4];
 };
 #define VFR_REC_VERT_BITS 26u
 #define VFR_REC_VERT_MASK ((1u << VFR_REC_VERT_BITS) - 1u)
 /* A mesh's vertex ids fit the vertex word. */
 #define VFR_MAX_VERTS (1u << VFR_REC_VERT_BITS)
-static_assert(sizeof(TetRec) == 64, "TetRec is one 64-byte line");
+static_assert(sizeof(TetRec) == 32, "TetRec is 32 bytes");
 
 /* One view of a render batch, with the offsets of its mesh in the device
  * arenas (a plan's single mesh, or a server's resident meshes). */
@@ -89,7 +88,8 @@ static_assert(sizeof(DevView) % 16 == 0, "DevView alignment");
 struct DevMesh {
   uint32_t n_verts = 0, n_tets = 0, n_ext = 0;
   float lmt[3] = {0, 0, 0};
-  TetRec* rec = nullptr;       /* [n_tets] internal order                   */
+  TetRec* rec = nullptr;       /* [n_tets] face records, internal order     */
+  uint32_t* tv = nullptr;      /* [n_tets*4] vertex ids (internal), same order */
   float* vert = nullptr;       /* [n_verts*4] internal order (x,y,z,s)      */
   uint32_t* ext_int = nullptr; /* [n_ext*2] (internal tet, face)            */
   uint32_t* extf = nullptr;    /* [n_ext*2] (caller tet, face), .extf order */
@@ -110,7 +110,8 @@ enum { ST_STEPS = 0, ST_ANOM = 1, ST_OVERFLOW = 2, ST_RAYS = 3, ST_N = 8 };
 
 #ifdef __HIPCC__
 struct TravArgs {
-  const TetRec* rec;       /* record arena; a view's mesh starts at rec_off   */
+  const TetRec* rec;       /* face-record arena; a view's mesh starts at rec_off */
+  const uint4* tv;         /* tet vertex ids, same indexing as rec            */
   const float4* vert;      /* P-space vertices of all views (+ view vert_off) */
   const uint32_t* ext_int; /* (internal tet, face) arena; + ext_off           */
   const float4* tf;        /* [slots * 256] */
@@ -143,10 +144,10 @@ int vfr_k_copy(const void* src, void* dst, size_t bytes, void* stream);
  * (xform) or external faces (raster) summed over those views. */
 int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work,
                       const float4* wvert, float4* pv, void* stream);
-int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const TetRec* rec,
+int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const uint4* tv,
                      const uint32_t* ext_int, const float4* pv, uint32_t* owner, void* stream);
 /* kmax: entry slots per pixel; 0 only counts (cnt) without storing keys. */
-int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const TetRec* rec,
+int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const uint4* tv,
                      const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
                      uint32_t kmax, unsigned long long* stats, void* stream);
 /* *out = max(*out, cnt[i]) over n pixels (device counter). */

@@ -3,7 +3,7 @@
  *
  *   preprocessing : face keys -> radix sort -> adjacent match -> external
  *                   faces (stream compaction) -> Z-curve reorder -> packed
- *                   64-byte tet records.  Replaces buildIncid/buildCon/
+ *                   32-byte face records + vertex ids.  Replaces buildIncid/buildCon/
  *                   buildExtF (vfRayPreComp.cc:219-270, 424-501, 710-737).
  *   raster        : per (view, external face): visibility (the Cramer plane
  *                   test of findVisibleFaces, vfRayKernelCPU.cc:51-132), the
@@ -305,10 +305,10 @@ int vfr_k_copy(const void* src, void* dst, size_t bytes, void* stream) {
 
 /* Loads the ext face (internal tet, face) vertices in P space: slots
  * (f+0..3)&3, the last one the opposite vertex. */
-DEV void face_verts(const TetRec* rec, const float4* pv, const uint32_t* ext_int, uint32_t e,
+DEV void face_verts(const uint4* tv, const float4* pv, const uint32_t* ext_int, uint32_t e,
                     float4 out[4], uint32_t* tet, uint32_t* face) {
   uint32_t t = ext_int[e * 2], f = ext_int[e * 2 + 1];
-  uint4 rv = *reinterpret_cast<const uint4*>(rec[t].v);
+  uint4 rv = tv[t];
 #pragma unroll
   for (uint32_t k = 0; k < 4; ++k) out[k] = pv[sel4(rv, (f + k) & 3u)];
   *tet = t;
@@ -323,7 +323,7 @@ DEV void face_verts(const TetRec* rec, const float4* pv, const uint32_t* ext_int
  * in main.cc's loop (main.cc:209-218) overwrites the pixel. */
 __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ views, uint32_t v0,
                                                     uint32_t nv, uint64_t work,
-                                                    const TetRec* __restrict__ rec,
+                                                    const uint4* __restrict__ tv,
                                                     const uint32_t* __restrict__ ext_int,
                                                     const float4* __restrict__ pv,
                                                     uint32_t* __restrict__ owner) {
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ 
   const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
   float4 fv[4];
   uint32_t t, f;
-  face_verts(rec + views[vi].rec_off, pv + views[vi].vert_off, ext_int + 2 * (size_t)views[vi].ext_off, e,
+  face_verts(tv + views[vi].rec_off, pv + views[vi].vert_off, ext_int + 2 * (size_t)views[vi].ext_off, e,
              fv, &t, &f);
   Planes pl = face_planes(fv[0], fv[1], fv[2]);
   float faceZ = fv[3].x * pl.z[0] + fv[3].y * pl.z[1] + pl.z[2];
@@ -359,11 +359,11 @@ __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ 
   }
 }
 
-int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const TetRec* rec,
+int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const uint4* tv,
                      const uint32_t* ext_int, const float4* pv, uint32_t* owner, void* stream) {
   if (work == 0) return VFR_OK;
   hipLaunchKernelGGL(k_raster_ref, dim3((unsigned)((work + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, nv, work, rec, ext_int, pv, owner);
+                     (hipStream_t)stream, views, v0, nv, work, tv, ext_int, pv, owner);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -372,7 +372,7 @@ int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint64_t wo
  * the key (Z at the pixel, ext index) to each pixel its projection holds. */
 __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ views, uint32_t v0,
                                                     uint32_t nv, uint64_t work,
-                                                    const TetRec* __restrict__ rec,
+                                                    const uint4* __restrict__ tv,
                                                     const uint32_t* __restrict__ ext_int,
                                                     const float4* __restrict__ pv,
                                                     uint32_t* __restrict__ cnt,
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
   const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
   float4 fv[4];
   uint32_t t, f;
-  face_verts(rec + views[vi].rec_off, pv + views[vi].vert_off, ext_int + 2 * (size_t)views[vi].ext_off, e,
+  face_verts(tv + views[vi].rec_off, pv + views[vi].vert_off, ext_int + 2 * (size_t)views[vi].ext_off, e,
              fv, &t, &f);
   Planes pl = face_planes(fv[0], fv[1], fv[2]);
   float faceZ = fv[3].x * pl.z[0] + fv[3].y * pl.z[1] + pl.z[2];
@@ -418,12 +418,12 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
   if (over && kmax) atomicAdd(&stats[ST_OVERFLOW], over);
 }
 
-int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const TetRec* rec,
+int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const uint4* tv,
                      const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
                      uint32_t kmax, unsigned long long* stats, void* stream) {
   if (work == 0) return VFR_OK;
   hipLaunchKernelGGL(k_raster_ftb, dim3((unsigned)((work + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, nv, work, rec, ext_int, pv, cnt, entries, kmax, stats);
+                     (hipStream_t)stream, views, v0, nv, work, tv, ext_int, pv, cnt, entries, kmax, stats);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -581,9 +581,9 @@ DEV float4 sel4f(bool c, float4 a, float4 b) {
 /* REF mode's step: loads the current tet, rebuilds Q[k] = slot (fe+k)&3 from
  * the carried entry-face vertices plus the one new vertex, and runs the
  * reference's exit-face test on faces (fe+1..3)&3 in that order. */
-DEV Step walk_step(const Walk& w, const TetRec* __restrict__ rec, const float4* __restrict__ pv,
-                   float ox, float oy) {
-  const uint4 rv = *reinterpret_cast<const uint4*>(rec[w.t].v);
+DEV Step walk_step(const Walk& w, const TetRec* __restrict__ rec, const uint4* __restrict__ tv,
+                   const float4* __restrict__ pv, float ox, float oy) {
+  const uint4 rv = tv[w.t];
   const uint4 f01 = *reinterpret_cast<const uint4*>(&rec[w.t].f[0]);
   const uint4 f23 = *reinterpret_cast<const uint4*>(&rec[w.t].f[4]);
   const uint4 rn = make_uint4(f01.x, f01.z, f23.x, f23.z);
@@ -661,6 +661,7 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
   o.anomaly = 0; o.broken = 0;
   const float4* __restrict__ pv = a.vert + a.views[vi].vert_off;
   const TetRec* __restrict__ rec = a.rec + a.views[vi].rec_off;
+  const uint4* __restrict__ tv = a.tv + a.views[vi].rec_off;
   const uint32_t* __restrict__ ext = a.ext_int + 2 * (size_t)a.views[vi].ext_off;
   const float max_edge = a.views[vi].max_edge;
   const uint32_t W = a.views[vi].W, H = a.views[vi].H, tfo = a.views[vi].tf * 256u;
@@ -672,7 +673,7 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
   w.t = ext[e * 2];
   w.fe = ext[e * 2 + 1];
   {
-    const uint4 rv = *reinterpret_cast<const uint4*>(rec[w.t].v);
+    const uint4 rv = tv[w.t];
     w.h0 = sel4(rv, w.fe); w.h1 = sel4(rv, (w.fe + 1u) & 3u); w.h2 = sel4(rv, (w.fe + 2u) & 3u);
     w.H0 = pv[w.h0];
     w.H1 = pv[w.h1];
@@ -692,7 +693,7 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
   uint32_t steps = 0;
   while (true) {
     if (++steps > VFR_MAX_STEPS) { o.anomaly = 1; break; }
-    const Step s = walk_step(w, rec, pv, ox, oy);
+    const Step s = walk_step(w, rec, tv, pv, ox, oy);
     if (s.i == 0) { o.anomaly = 1; break; }         /* :353 */
     if (s.nb == VFR_NB_BROKEN) { o.broken = 1; break; } /* :365, not emulated */
     const Planes nf = face_planes(s.A, s.B, s.C);
@@ -889,7 +890,7 @@ DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const float4* __restrict__ pv = a.vert + r.voff;
   r.t = a.views[r.vi].rec_off + a.ext_int[2 * (size_t)e];
   const uint32_t fe = a.ext_int[2 * (size_t)e + 1];
-  const uint4 rv = *reinterpret_cast<const uint4*>(a.rec[r.t].v);
+  const uint4 rv = a.tv[r.t];
   r.fa = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[0]);
   r.fb = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[4]);
   /* face fe = record slots fe, fe+1, fe+2; the fourth vertex is slot fe+3 */
@@ -1037,8 +1038,8 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
     fan = *reinterpret_cast<const uint4*>(&a.rec[tn].f[0]);
     fbn = *reinterpret_cast<const uint4*>(&a.rec[tn].f[4]);
   } else {
-    fan = ld32<uint4>(a.rec, tn * 64u + 16u);
-    fbn = ld32<uint4>(a.rec, tn * 64u + 32u);
+    fan = ld32<uint4>(a.rec, tn * 32u);
+    fbn = ld32<uint4>(a.rec, tn * 32u + 16u);
   }
   __builtin_amdgcn_sched_barrier(0);
   /* round 2: psi */
@@ -1428,27 +1429,28 @@ __global__ void k_gather_vert(const float4* __restrict__ v, const uint32_t* __re
   if (i < nV) out[i] = v[vperm[i]];
 }
 
-/* Internal tet record: vertex ids through the vertex permutation; per face
- * the neighbour word (internal id << 2 | the face of the neighbour that
- * links back, the back-lookup of vfRayKernelCPU.cc:359-366) and the
- * neighbour's vertex off that face (the tet's own vertex opposite the face
- * when there is no linked neighbour), so a walk can fetch the next tet's
- * fourth vertex without waiting for its record. */
+/* Internal tet records: vertex ids through the vertex permutation (tv);
+ * per face the neighbour word (internal id << 2 | the face of the neighbour
+ * that links back, the back-lookup of vfRayKernelCPU.cc:359-366) and the
+ * vertex word: the neighbour's vertex off that face (the tet's own vertex
+ * opposite the face when there is no linked neighbour), so a walk can fetch
+ * the next tet's fourth vertex without waiting for its record, and the
+ * neighbour's slots of the face's vertices. */
 __global__ void k_build_rec(const uint32_t* __restrict__ tets, const uint32_t* __restrict__ con,
                             const uint32_t* __restrict__ perm, const uint32_t* __restrict__ tinv,
                             const uint32_t* __restrict__ vinv, uint32_t nT,
-                            TetRec* __restrict__ rec) {
+                            TetRec* __restrict__ rec, uint4* __restrict__ tv) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nT) return;
   uint32_t t = perm[i];
   TetRec r;
-  uint32_t n[4], o[4];
+  uint32_t v[4], n[4], o[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) r.v[k] = vinv[tets[t * 4 + k]];
+  for (int k = 0; k < 4; ++k) v[k] = vinv[tets[t * 4 + k]];
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
     uint32_t u = con[t * 4 + f];
-    o[f] = r.v[(f + 3) & 3];
+    o[f] = v[(f + 3) & 3];
     if (u == t) { n[f] = VFR_NB_EXT; continue; }
     uint32_t kk = 4;
     for (uint32_t q = 0; q < 4; ++q)
@@ -1471,9 +1473,9 @@ __global__ void k_build_rec(const uint32_t* __restrict__ tets, const uint32_t* _
   for (int f = 0; f < 4; ++f) {
     r.f[2 * f] = n[f];
     r.f[2 * f + 1] = o[f];
-    r.pad[f] = 0;
   }
   rec[i] = r;
+  tv[i] = make_uint4(v[0], v[1], v[2], v[3]);
 }
 
 /* Adds `add` to the tet index of every linked neighbour word of records
@@ -1755,15 +1757,16 @@ int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
   }
   hipLaunchKernelGGL(k_invert, dim3(nblk(nT)), dim3(256), 0, st, perm.as<uint32_t>(), nT, tinv.as<uint32_t>());
   hipLaunchKernelGGL(k_invert, dim3(nblk(nV)), dim3(256), 0, st, vperm.as<uint32_t>(), nV, vinv.as<uint32_t>());
-  DevBuf vert, rec, ext_int;
+  DevBuf vert, rec, tv, ext_int;
   ALLOC(vert, sizeof(float) * 4 * (size_t)nV);
   ALLOC(rec, sizeof(TetRec) * (size_t)nT);
+  ALLOC(tv, sizeof(uint4) * (size_t)nT);
   ALLOC(ext_int, sizeof(uint32_t) * 2 * (size_t)n_ext);
   hipLaunchKernelGGL(k_gather_vert, dim3(nblk(nV)), dim3(256), 0, st, d_xyzs.as<float4>(),
                      vperm.as<uint32_t>(), nV, vert.as<float4>());
   hipLaunchKernelGGL(k_build_rec, dim3(nblk(nT)), dim3(256), 0, st, d_tets.as<uint32_t>(),
                      d_con.as<uint32_t>(), perm.as<uint32_t>(), tinv.as<uint32_t>(),
-                     vinv.as<uint32_t>(), nT, rec.as<TetRec>());
+                     vinv.as<uint32_t>(), nT, rec.as<TetRec>(), tv.as<uint4>());
   if (n_ext)
     hipLaunchKernelGGL(k_ext_internal, dim3(nblk(n_ext)), dim3(256), 0, st, d_extf.as<uint32_t>(),
                        n_ext, tinv.as<uint32_t>(), ext_int.as<uint32_t>());
@@ -1772,12 +1775,13 @@ int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
   m.n_ext = n_ext;
   m.vert = vert.as<float>(); vert.release();
   m.rec = rec.as<TetRec>(); rec.release();
+  m.tv = tv.as<uint32_t>(); tv.release();
   m.ext_int = ext_int.as<uint32_t>(); ext_int.release();
   m.extf = d_extf.as<uint32_t>(); d_extf.release();
   m.perm = perm.as<uint32_t>(); perm.release();
   m.vperm = vperm.as<uint32_t>(); vperm.release();
   m.con = d_con.as<uint32_t>(); d_con.release();
-  m.bytes = sizeof(float) * 4ull * nV + sizeof(TetRec) * (uint64_t)nT + 16ull * n_ext +
+  m.bytes = sizeof(float) * 4ull * nV + (sizeof(TetRec) + 16ull) * (uint64_t)nT + 16ull * n_ext +
             4ull * nT + 4ull * nV + 16ull * nT;
   return VFR_OK;
 }

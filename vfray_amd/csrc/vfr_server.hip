@@ -106,6 +106,7 @@ struct vfr_server {
   vfr_server_cfg cfg{};
   std::vector<SrvMesh> meshes;
   TetRec* rec = nullptr;
+  uint4* tv = nullptr;
   uint64_t n_records = 0;
   uint32_t* ext = nullptr;
   float4* vert = nullptr;
@@ -190,13 +191,14 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   CHECK_RC(vfr_k_clear_u32(s->counters, 4, st));
   CHECK_RC(vfr_k_clear_u32((uint32_t*)s->stats, 2 * ST_N, st));
   CHECK_RC(vfr_k_xform_views(s->views, 0, nv, verts, sv->vert, s->pverts, st));
-  CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, faces, sv->rec, sv->ext, s->pverts, s->cnt, s->entries, VFR_K,
+  CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, faces, sv->tv, sv->ext, s->pverts, s->cnt, s->entries, VFR_K,
                             s->stats, st));
   CHECK_RC(vfr_k_compact(s->views, nv, 0, (uint32_t)tiles, VFR_MODE_FTB, nullptr, s->cnt, s->entries, VFR_K,
                          s->rgba, s->steps, s->entry, s->rays, s->counters, st));
   TravArgs a;
   memset(&a, 0, sizeof(a));
   a.rec = sv->rec;
+  a.tv = sv->tv;
   a.vert = s->pverts;
   a.ext_int = sv->ext;
   a.tf = vfr_ctx_tf(sv->ctx);
@@ -335,6 +337,7 @@ extern "C" void vfr_server_destroy(vfr_server* sv) {
     slot_free(sp.get());
   }
   if (sv->rec) (void)hipFree(sv->rec);
+  if (sv->tv) (void)hipFree(sv->tv);
   if (sv->ext) (void)hipFree(sv->ext);
   if (sv->vert) (void)hipFree(sv->vert);
   delete sv;
@@ -379,6 +382,7 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
   CHECK_HIP(hipSetDevice(vfr_ctx_device(ctx)));
   int rc = VFR_OK;
   if (hipMalloc((void**)&sv->rec, sizeof(TetRec) * tets) != hipSuccess ||
+      hipMalloc((void**)&sv->tv, sizeof(uint4) * tets) != hipSuccess ||
       hipMalloc((void**)&sv->ext, 8 * (exts + 1)) != hipSuccess ||
       hipMalloc((void**)&sv->vert, 16 * verts) != hipSuccess) {
     vfr_set_error("resident mesh arena allocation failed");
@@ -393,6 +397,7 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
     m.max_edge = d->lmt[0];
     vfr_mesh_bbox(meshes[i], m.bb_lo, m.bb_hi);
     if (hipMemcpy(sv->rec + ro, d->rec, sizeof(TetRec) * d->n_tets, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(sv->tv + ro, d->tv, sizeof(uint4) * d->n_tets, hipMemcpyDeviceToDevice) != hipSuccess ||
         hipMemcpy(sv->ext + 2 * eo, d->ext_int, 8ull * d->n_ext, hipMemcpyDeviceToDevice) != hipSuccess ||
         hipMemcpy(sv->vert + vo, d->vert, 16ull * d->n_verts, hipMemcpyDeviceToDevice) != hipSuccess) {
       vfr_set_error("copy of mesh %d into the resident arena failed", i);
