@@ -196,6 +196,10 @@ typedef struct vfr_render_opts {
  * two gammas, the reference's vfRayKernelCPU.cc:454-470 convention; the
  * table is the one vfr_psi_load sets).  REF mode always uses the table. */
 #define VFR_RENDER_PSI_TABLE 0x4u
+/* Front-to-back walks use the 64-bit-address kernel variant that meshes of
+ * 2^27 tets or more need, whatever the mesh size (for testing that variant
+ * on small meshes; the results are the same). */
+#define VFR_RENDER_WIDE 0x8u
 
 typedef struct vfr_frame_out {
   float* rgba;          /* W*H*4, may be NULL                              */

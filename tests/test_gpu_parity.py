@@ -145,7 +145,8 @@ def test_ftb_psi_table(ctx, c1, psi):
     """VFR_RENDER_PSI_TABLE: front-to-back with psi looked up in the table, bit
     for bit with the oracle's table variant, for the one-ray-per-thread and
     the persistent kernels; the series default stays within the table's
-    quantization of it and closer to the FP64 re-integration."""
+    quantization of it and closer to the FP64 re-integration.  The
+    64-bit-address walk variant (VFR_RENDER_WIDE) gives the same bits."""
     print("FTB psi table 16^3", flush=True)
     m = ctx.mesh(c1["x"], c1["t"])
     tf = ctx.tf(c1["tf"])
@@ -162,6 +163,12 @@ def test_ftb_psi_table(ctx, c1, psi):
         cov = orc["entry_ext"] != V.NO_ID
         assert np.abs(gs["rgba"] - g["rgba"]).max() <= TOL
         assert psnr(gs["rgba"], orc["rgba64"], cov) > psnr(g["rgba"], orc["rgba64"], cov)
+    # the 64-bit-address variant (records of 2^27 tets or more) on this mesh
+    outw, _ = ctx.render(m, views, mode=V.MODE_FTB, wide=True)
+    outwt, _ = ctx.render(m, views, mode=V.MODE_FTB, wide=True, psi_table=True)
+    for g, gw, gs, gwt in zip(outs, outw, outs_s, outwt):
+        assert np.array_equal(gw["rgba"], gs["rgba"]) and np.array_equal(gw["steps"], gs["steps"])
+        assert np.array_equal(gwt["rgba"], g["rgba"]) and np.array_equal(gwt["steps"], g["steps"])
     m.close()
 
 

@@ -21,6 +21,7 @@ COMPOSITE_PPI, COMPOSITE_LINEAR, COMPOSITE_LINEAR_OPACITY = 0, 1, 2
 RENDER_PERSISTENT = 0x1
 RENDER_PIPELINED = 0x2
 RENDER_PSI_TABLE = 0x4
+RENDER_WIDE = 0x8
 SCHED_PER_VIEW, SCHED_BATCHED = 0, 1
 MESH_NORMALIZE, MESH_MORTON = 0x1, 0x2
 NO_ID = 0xFFFFFFFF
@@ -371,9 +372,9 @@ class Context:
 
     def plan(self, mesh: "Mesh", views, mode=MODE_FTB, schedule=SCHED_BATCHED, term_alpha=0.99,
              use_graph=False, waves_per_cu=0, refill=0, persistent=False, occupancy=0,
-             max_entries=0, pipelined=False, composite=COMPOSITE_PPI, psi_table=False) -> "Plan":
+             max_entries=0, pipelined=False, composite=COMPOSITE_PPI, psi_table=False, wide=False) -> "Plan":
         return Plan(self, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
-                    persistent, occupancy, max_entries, pipelined, composite, psi_table)
+                    persistent, occupancy, max_entries, pipelined, composite, psi_table, wide)
 
     def render(self, mesh: "Mesh", views, **kw) -> tuple[list[dict], dict]:
         p = self.plan(mesh, views, **kw)
@@ -427,12 +428,12 @@ class Mesh:
 class Plan:
     def __init__(self, ctx, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
                  persistent=False, occupancy=0, max_entries=0, pipelined=False, composite=COMPOSITE_PPI,
-                 psi_table=False):
+                 psi_table=False, wide=False):
         self.ctx, self.mesh = ctx, mesh
         self.views = list(views)
         arr = (View * len(self.views))(*self.views)
         flags = ((RENDER_PERSISTENT if persistent else 0) | (RENDER_PIPELINED if pipelined else 0)
-                 | (RENDER_PSI_TABLE if psi_table else 0))
+                 | (RENDER_PSI_TABLE if psi_table else 0) | (RENDER_WIDE if wide else 0))
         o = RenderOpts(mode, schedule, term_alpha, waves_per_cu, refill, 1 if use_graph else 0, max_entries,
                        flags, occupancy, composite)
         self._p = C.c_void_p()

@@ -466,6 +466,12 @@ static int enqueue_batched(vfr_plan* p) {
   return VFR_OK;
 }
 
+/* Records the walk must address: the mesh's, or with VFR_RENDER_WIDE as many
+ * as select the 64-bit-address walk variant. */
+static uint64_t plan_records(const vfr_plan* p) {
+  return (p->opts.flags & VFR_RENDER_WIDE) ? VFR_BIG_RECORDS : (uint64_t)p->mesh->d.n_tets;
+}
+
 static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   TravArgs a;
   memset(&a, 0, sizeof(a));
@@ -503,7 +509,7 @@ static int launch_batched_traverse(vfr_plan* p) {
   TravArgs a = trav_args(p, p->rays, p->counters + p->n_views);
   const bool persist = p->opts.mode == VFR_MODE_FTB && (p->opts.flags & VFR_RENDER_PERSISTENT);
   const uint32_t waves = p->opts.waves_per_cu * (uint32_t)p->ctx->n_cu;
-  return vfr_k_traverse(p->opts.mode, p->trav_proj, p->mesh->d.n_tets, p->opts.occupancy, a, (uint32_t)p->n_pix,
+  return vfr_k_traverse(p->opts.mode, p->trav_proj, plan_records(p), p->opts.occupancy, a, (uint32_t)p->n_pix,
                         persist ? 1 : 0, waves, p->ctx->stream);
 }
 
@@ -743,7 +749,7 @@ static int execute_pipelined(vfr_plan* p, double* ms_total, double* ms_vis, doub
     a.queue = p->counters + p->n_views + 1;
     const bool persist = (p->opts.flags & VFR_RENDER_PERSISTENT) != 0;
     if (persist) CHECK_RC(vfr_k_clear_u32(a.queue, 1, s2));
-    CHECK_RC(vfr_k_traverse(p->opts.mode, ch.proj, m->d.n_tets, p->opts.occupancy, a, (uint32_t)ch.npix,
+    CHECK_RC(vfr_k_traverse(p->opts.mode, ch.proj, plan_records(p), p->opts.occupancy, a, (uint32_t)ch.npix,
                             persist ? 1 : 0, p->opts.waves_per_cu * (uint32_t)p->ctx->n_cu, s2));
     CHECK_HIP(hipEventRecord(p->cev[3 * c + 2], s2));
   }
@@ -823,7 +829,7 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
       a.tf_lds_base = hv.tf;
       a.tf_lds_n = 1;
       CHECK_RC(vfr_k_traverse(p->opts.mode, hv.proj == VFR_PROJ_PERSPECTIVE ? VFR_TRAV_PERSP : VFR_TRAV_ORTHO,
-                              m->d.n_tets, 0, a, npx, 0, 0, st));
+                              plan_records(p), 0, a, npx, 0, 0, st));
       CHECK_HIP(hipEventRecord(p->fev[2 * v + 1], st));
       CHECK_HIP(hipEventSynchronize(p->fev[2 * v + 1]));  /* a frame ends before the next starts */
       float f = 0, g = 0;

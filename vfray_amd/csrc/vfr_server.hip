@@ -219,7 +219,8 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   a.psi_table = (sv->cfg.flags & VFR_RENDER_PSI_TABLE) ? 1u : 0u;
   if (tf_hi - tf_lo < VFR_TF_LDS_SLOTS) { a.tf_lds_base = tf_lo; a.tf_lds_n = tf_hi - tf_lo + 1; }
   const uint32_t proj = n_persp == 0 ? VFR_TRAV_ORTHO : (n_persp == nv ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
-  CHECK_RC(vfr_k_traverse(VFR_MODE_FTB, proj, sv->n_records, sv->cfg.occupancy, a, (uint32_t)pix,
+  const uint64_t records = (sv->cfg.flags & VFR_RENDER_WIDE) ? VFR_BIG_RECORDS : sv->n_records;
+  CHECK_RC(vfr_k_traverse(VFR_MODE_FTB, proj, records, sv->cfg.occupancy, a, (uint32_t)pix,
                           (sv->cfg.flags & VFR_RENDER_PERSISTENT) ? 1 : 0,
                           sv->cfg.waves_per_cu * (uint32_t)vfr_ctx_cus(sv->ctx), st));
   CHECK_HIP(hipMemcpyAsync(s->h_stats, s->stats, sizeof(unsigned long long) * ST_N, hipMemcpyDeviceToHost, st));
