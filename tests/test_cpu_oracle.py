@@ -350,12 +350,19 @@ def test_ftb_entry_faces_against_brute_force(psi):
 
 
 def test_ftb_fp32_vs_fp64_psnr(psi):
+    """Both psi variants of the front-to-back restatement against its FP64
+    re-integration (exact psi): PSNR >= 50 dB, the series well above the
+    table's nearest entry; the two agree to the table's quantization."""
     inp = grid_inputs(24, 0x5EED0301)
     for v in WL.sphere_views(0x5EED0301, 2, [160], 0):
         r = oracle_ftb(inp, v, psi, fp64=True)
+        rt = oracle_ftb(inp, v, psi, fp64=True, psi_table=True)
         cov = r["entry_ext"] != V.NO_ID
         assert r["anomalies"] == 0 and cov.mean() > 0.3
-        assert psnr(r["rgba"], r["rgba64"], cov) >= 50.0
+        assert np.array_equal(r["entry_ext"], rt["entry_ext"])
+        p_series, p_table = psnr(r["rgba"], r["rgba64"], cov), psnr(rt["rgba"], r["rgba64"], cov)
+        assert p_table >= 50.0 and p_series >= p_table + 6.0
+        assert np.abs(r["rgba"] - rt["rgba"]).max() <= 2e-3
 
 
 def test_ftb_concave_reentry(psi):
