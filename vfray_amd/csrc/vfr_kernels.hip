@@ -788,10 +788,11 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
  * edge-function-weighted average of the face's vertices (all weights >= 0).
  *
  * Loads are issued one tet ahead: once the exit face is known, its link
- * (neighbour word and the neighbour's vertex off the shared face), then the
- * next tet's vertex ids and fourth vertex are requested before the segment
- * is composited, so their latency overlaps the transfer-function and psi
- * lookups. */
+ * (neighbour word, the neighbour's vertex off the shared face and slot map)
+ * is picked from the face record held in registers, and the next tet's face
+ * record and fourth vertex are requested before the segment is composited,
+ * so their latency overlaps the compositing arithmetic.  The ray carries the
+ * record slot of each face vertex rather than its id. */
 struct FtbRay {
   float4 P0, P1, P2;       /* entry face: (X - ox, Y - oy, Z, S) */
   float4 Q;                /* the tet's fourth vertex, P space (prefetched) */
