@@ -145,8 +145,8 @@ def serve_bench(args) -> int:
         tf_ids.append([ctx.tf(tf) for tf in tfs])
         tets += t.shape[0]
     log(f"8 resident meshes, {tets} tets, built in {time.time() - t0:.1f}s")
-    sched = dict(persistent=args.persistent, occupancy=args.occupancy, waves_per_cu=args.wpc,
-                 refill=args.refill or 32)
+    sched = dict(persistent=args.persistent, wave_queue=args.wave_queue, occupancy=args.occupancy,
+                 waves_per_cu=args.wpc, refill=args.refill or 32)
     reqs = WL.serve_requests(args.requests, len(meshes), tf_ids)
     rays_of = lambda rs: sum(r.view.width * r.view.height for r in rs)
     n_cap = min(len(reqs), max(200, args.requests // 5))
@@ -217,7 +217,8 @@ def main() -> int:
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched: skip the schedule autotuner (one ray per thread unless --persistent)")
-    ap.add_argument("--persistent", action="store_true", help="persistent work-queue traversal kernel")
+    ap.add_argument("--persistent", action="store_true", help="persistent traversal kernel with lane refill")
+    ap.add_argument("--wave-queue", action="store_true", help="wave work-queue traversal kernel")
     ap.add_argument("--pipelined", action="store_true",
                     help="batched: view chunks whose preprocessing overlaps the walk of earlier chunks")
     ap.add_argument("--occupancy", type=int, default=0)
@@ -264,18 +265,18 @@ def main() -> int:
     total_rays = sum(v.width * v.height for v in views)
 
     tune = None
-    manual = args.persistent or args.pipelined or args.occupancy or args.wpc or args.refill
+    manual = args.persistent or args.wave_queue or args.pipelined or args.occupancy or args.wpc or args.refill
     if not args.no_autotune and not manual:
         pt = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
         tune = pt.autotune()
         pt.close()
         log(f"autotune: {tune}")
-    kw = {} if tune is None else {"persistent": tune["persistent"], "pipelined": tune["pipelined"],
-                                  "waves_per_cu": tune["waves_per_cu"], "refill": tune["refill"],
-                                  "occupancy": tune["occupancy"]}
+    kw = {} if tune is None else {"persistent": tune["persistent"], "wave_queue": tune["wave_queue"],
+                                  "pipelined": tune["pipelined"], "waves_per_cu": tune["waves_per_cu"],
+                                  "refill": tune["refill"], "occupancy": tune["occupancy"]}
     if tune is None and manual:
-        kw = {"persistent": args.persistent, "pipelined": args.pipelined, "waves_per_cu": args.wpc,
-              "refill": args.refill, "occupancy": args.occupancy}
+        kw = {"persistent": args.persistent, "wave_queue": args.wave_queue, "pipelined": args.pipelined,
+              "waves_per_cu": args.wpc, "refill": args.refill, "occupancy": args.occupancy}
         tune = dict(kw)
     pb = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, use_graph=args.graph, **kw)
     for _ in range(args.warmup):
@@ -337,7 +338,7 @@ def main() -> int:
     roof = None
     if ms_trav:
         ach = alg_bytes / (ms_trav * 1e-3) / 1e9
-        kname = "k_trav_persist" if kw.get("persistent") else "k_trav"
+        kname = "k_trav_persist" if kw.get("persistent") else ("k_trav_wq" if kw.get("wave_queue") else "k_trav")
         tr = measured_traffic(kname, args.workload)
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(ach / PEAK_HBM_GBS, 4),

@@ -200,6 +200,11 @@ typedef struct vfr_render_opts {
  * 2^27 tets or more need, whatever the mesh size (for testing that variant
  * on small meshes; the results are the same). */
 #define VFR_RENDER_WIDE 0x8u
+/* Batched front-to-back walks run as a wave work queue: a grid that fills
+ * the device, each wave taking the next 64 rays of the list with one atomic
+ * until the list is drained (no refill inside a wave; a wave that is done
+ * moves on without waiting for its block).  Not with VFR_RENDER_PERSISTENT. */
+#define VFR_RENDER_WAVE_QUEUE 0x10u
 
 typedef struct vfr_frame_out {
   float* rgba;          /* W*H*4, may be NULL                              */
@@ -231,12 +236,13 @@ void vfr_plan_destroy(vfr_plan* plan);
 /* Per-view frame latency (ms, HIP events) of the last PER_VIEW execute. */
 int vfr_plan_frame_times(const vfr_plan* plan, double* out_ms);
 
-/* Times the one-ray-per-thread traversal and candidate persistent-kernel
- * schedules (kernel occupancy variant x waves per CU x refill threshold) on
- * the plan itself and keeps the fastest walk, then keeps the pipelined
- * execution if it is faster end to end; returns the choice (flags holds
- * VFR_RENDER_PERSISTENT / VFR_RENDER_PIPELINED) and the walk time.  Results
- * are cached per (tets, rays, views) in the process. */
+/* Times the one-ray-per-thread traversal, the wave work queue and candidate
+ * persistent-kernel schedules (kernel occupancy variant x waves per CU x
+ * refill threshold) on the plan itself and keeps the fastest walk, then
+ * keeps the pipelined execution if it is faster end to end; returns the
+ * choice (flags holds VFR_RENDER_WAVE_QUEUE / VFR_RENDER_PERSISTENT /
+ * VFR_RENDER_PIPELINED) and the walk time.  Results are cached per (tets,
+ * rays, views, psi variant) in the process. */
 int vfr_plan_autotune(vfr_plan* plan, uint32_t* flags, uint32_t* occupancy,
                       uint32_t* waves_per_cu, uint32_t* refill, double* best_ms);
 

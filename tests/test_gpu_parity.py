@@ -216,7 +216,9 @@ def test_ftb_perspective_32(ctx, psi):
 
 
 def test_batched_equals_per_view(ctx):
-    """Each view rendered in a batch equals the same view rendered alone."""
+    """Each view rendered in a batch equals the same view rendered alone, with
+    every walk kernel (one ray per thread, lane-refill persistent, wave work
+    queue) and with pipelined view chunks."""
     inp = grid_inputs(24, 0x5EED0103)
     m = ctx.mesh(inp["x"], inp["t"])
     tf = ctx.tf(inp["tf"])
@@ -229,7 +231,9 @@ def test_batched_equals_per_view(ctx):
     d, sd = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, persistent=True, waves_per_cu=4,
                        refill=1)
     e, se = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, pipelined=True)
-    assert sa["steps"] == sb["steps"] == sc["steps"] == sd["steps"] == se["steps"]
+    f, sf = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, wave_queue=True)
+    g, sg = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, wave_queue=True, pipelined=True)
+    assert sa["steps"] == sb["steps"] == sc["steps"] == sd["steps"] == se["steps"] == sf["steps"] == sg["steps"]
     assert sa["active_rays"] == sb["active_rays"] == se["active_rays"]
     for i, v in enumerate(views):
         alone, _ = ctx.render(m, [v], mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
@@ -238,6 +242,8 @@ def test_batched_equals_per_view(ctx):
             assert np.array_equal(a[i][key], c[i][key])
             assert np.array_equal(a[i][key], d[i][key])
             assert np.array_equal(a[i][key], e[i][key])
+            assert np.array_equal(a[i][key], f[i][key])
+            assert np.array_equal(a[i][key], g[i][key])
             assert np.array_equal(a[i][key], alone[0][key])
     m.close()
 
@@ -252,15 +258,16 @@ def test_graph_replays(ctx):
     views = WL.sphere_views(0x5EED0104, 3, [256, 320], tf)
     print("graph replays 20^3", flush=True)
     ref, _ = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
-    p = ctx.plan(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, use_graph=True)
-    for rep in range(3):
-        p.execute()
-        for i in range(len(views)):
-            got = p.read(i)
-            for key in ("rgba", "entry_ext", "steps"):
-                assert np.array_equal(got[key], ref[i][key]), (rep, i, key)
-        ctx.render(m, views[:1], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
-    p.close()
+    for wq in (False, True):
+        p = ctx.plan(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, use_graph=True, wave_queue=wq)
+        for rep in range(3):
+            p.execute()
+            for i in range(len(views)):
+                got = p.read(i)
+                for key in ("rgba", "entry_ext", "steps"):
+                    assert np.array_equal(got[key], ref[i][key]), (wq, rep, i, key)
+            ctx.render(m, views[:1], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+        p.close()
     m.close()
 
 

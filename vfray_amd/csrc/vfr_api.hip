@@ -503,14 +503,22 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   return a;
 }
 
+/* The walk kernel a batched front-to-back plan asks for. */
+static int plan_walk_kernel(const vfr_plan* p) {
+  if (p->opts.mode != VFR_MODE_FTB) return VFR_WALK_THREAD;
+  if (p->opts.flags & VFR_RENDER_PERSISTENT) return VFR_WALK_REFILL;
+  if (p->opts.flags & VFR_RENDER_WAVE_QUEUE) return VFR_WALK_QUEUE;
+  return VFR_WALK_THREAD;
+}
+
 /* The batched traversal: one ray per thread over the rays of all views, or
- * the persistent work-queue kernel (FTB, when asked). */
+ * a work-queue kernel (FTB, when asked; the queue head was cleared with the
+ * plan's counters). */
 static int launch_batched_traverse(vfr_plan* p) {
   TravArgs a = trav_args(p, p->rays, p->counters + p->n_views);
-  const bool persist = p->opts.mode == VFR_MODE_FTB && (p->opts.flags & VFR_RENDER_PERSISTENT);
   const uint32_t waves = p->opts.waves_per_cu * (uint32_t)p->ctx->n_cu;
   return vfr_k_traverse(p->opts.mode, p->trav_proj, plan_records(p), p->opts.occupancy, a, (uint32_t)p->n_pix,
-                        persist ? 1 : 0, waves, p->ctx->stream);
+                        plan_walk_kernel(p), waves, p->ctx->stream);
 }
 
 extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* views, int n_views,
@@ -539,6 +547,10 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   }
   if (o.waves_per_cu > 64 || o.refill > 64) {
     vfr_set_error("waves_per_cu %u / refill %u out of range (<= 64)", o.waves_per_cu, o.refill);
+    return VFR_EINVAL;
+  }
+  if ((o.flags & VFR_RENDER_PERSISTENT) && (o.flags & VFR_RENDER_WAVE_QUEUE)) {
+    vfr_set_error("VFR_RENDER_PERSISTENT and VFR_RENDER_WAVE_QUEUE exclude each other");
     return VFR_EINVAL;
   }
   vfr_plan* p = new (std::nothrow) vfr_plan;
@@ -747,10 +759,10 @@ static int execute_pipelined(vfr_plan* p, double* ms_total, double* ms_vis, doub
     CHECK_HIP(hipEventRecord(p->cev[3 * c + 1], s2));
     TravArgs a = trav_args(p, p->rays + ch.pix0, ccount + c);
     a.queue = p->counters + p->n_views + 1;
-    const bool persist = (p->opts.flags & VFR_RENDER_PERSISTENT) != 0;
-    if (persist) CHECK_RC(vfr_k_clear_u32(a.queue, 1, s2));
+    const int kern = plan_walk_kernel(p);
+    if (kern != VFR_WALK_THREAD) CHECK_RC(vfr_k_clear_u32(a.queue, 1, s2));
     CHECK_RC(vfr_k_traverse(p->opts.mode, ch.proj, plan_records(p), p->opts.occupancy, a, (uint32_t)ch.npix,
-                            persist ? 1 : 0, p->opts.waves_per_cu * (uint32_t)p->ctx->n_cu, s2));
+                            kern, p->opts.waves_per_cu * (uint32_t)p->ctx->n_cu, s2));
     CHECK_HIP(hipEventRecord(p->cev[3 * c + 2], s2));
   }
   CHECK_HIP(hipEventRecord(p->cev[3 * nc], s2));
@@ -961,7 +973,7 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
     std::lock_guard<std::mutex> g(g_tune_mu);
     for (auto& e : g_tune)
       if (e.first == key) {
-        p->opts.flags = (p->opts.flags & ~(VFR_RENDER_PERSISTENT | VFR_RENDER_PIPELINED)) | e.second.flags;
+        p->opts.flags = (p->opts.flags & ~(VFR_RENDER_PERSISTENT | VFR_RENDER_WAVE_QUEUE | VFR_RENDER_PIPELINED)) | e.second.flags;
         p->opts.occupancy = e.second.occ;
         p->opts.waves_per_cu = e.second.wpc;
         p->opts.refill = e.second.refill;
@@ -978,7 +990,7 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
     static const uint32_t kR[] = {16, 64};
     /* each candidate's walk is timed alone: all preprocessing first, then
      * one walk launch */
-    const uint32_t user_flags = p->opts.flags & ~(VFR_RENDER_PERSISTENT | VFR_RENDER_PIPELINED);
+    const uint32_t user_flags = p->opts.flags & ~(VFR_RENDER_PERSISTENT | VFR_RENDER_WAVE_QUEUE | VFR_RENDER_PIPELINED);
     const uint32_t base_flags = user_flags;
     best = 1e30;
     TuneVal bv{0u, p->opts.occupancy, p->opts.waves_per_cu, p->opts.refill};
@@ -997,6 +1009,12 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
     double t = 0;
     if (rc == VFR_OK) rc = time_it(&t);
     if (rc == VFR_OK && t < best) { best = t; bv = TuneVal{0u, p->opts.occupancy, p->opts.waves_per_cu, p->opts.refill}; }
+    p->opts.flags = base_flags | VFR_RENDER_WAVE_QUEUE;
+    if (rc == VFR_OK) rc = time_it(&t);
+    if (rc == VFR_OK && t < best) {
+      best = t;
+      bv = TuneVal{VFR_RENDER_WAVE_QUEUE, p->opts.occupancy, p->opts.waves_per_cu, p->opts.refill};
+    }
     p->opts.flags = base_flags | VFR_RENDER_PERSISTENT;
     for (uint32_t occ : kOcc) {
       uint32_t wps = 0;
@@ -1034,7 +1052,7 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tune.push_back({key, bv});
   }
-  if (flags_out) *flags_out = p->opts.flags & (VFR_RENDER_PERSISTENT | VFR_RENDER_PIPELINED);
+  if (flags_out) *flags_out = p->opts.flags & (VFR_RENDER_PERSISTENT | VFR_RENDER_WAVE_QUEUE | VFR_RENDER_PIPELINED);
   if (occ_out) *occ_out = p->opts.occupancy;
   if (wpc_out) *wpc_out = p->opts.waves_per_cu;
   if (refill_out) *refill_out = p->opts.refill;

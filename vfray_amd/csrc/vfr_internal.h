@@ -165,8 +165,13 @@ enum { VFR_TRAV_ORTHO = 0, VFR_TRAV_PERSP = 1, VFR_TRAV_MIXED = 2 };
 #define VFR_MODE_REF_LINEAR 2u
 /* n_records: tet records in the arena the rays walk (from VFR_BIG_RECORDS on,
  * the kernels address records with 64-bit arithmetic). */
+/* Walk kernels: one ray per thread (k_trav), the persistent kernel with lane
+ * refill (k_trav_persist; grid_waves waves, `occupancy` variant), the wave
+ * work queue (k_trav_wq; a.queue must be zero).  The last two take
+ * front-to-back walks only. */
+enum { VFR_WALK_THREAD = 0, VFR_WALK_REFILL = 1, VFR_WALK_QUEUE = 2 };
 int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t occupancy, const TravArgs& a,
-                   uint32_t max_rays, int persistent, uint32_t grid_waves, void* stream);
+                   uint32_t max_rays, int kernel, uint32_t grid_waves, void* stream);
 /* Waves per SIMD the persistent variant for `occupancy` reaches. */
 int vfr_k_traverse_waves_per_simd(uint32_t occupancy, uint32_t* out);
 

@@ -1179,6 +1179,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR
   }
 }
 
+/* Wave work-queue traversal: a fixed grid of waves, each of which takes the
+ * next 64 rays of the list with one atomic (one ray per lane, as k_trav) and
+ * walks them to the end, then takes the next 64, until the list is drained.
+ * A wave whose rays end goes on at once, where a k_trav block keeps its
+ * dispatch slot until its slowest wave ends; the transfer function is
+ * staged in LDS once per block.  Every wave reaches the drained queue, so
+ * the grid ends. */
+template <int PROJ, bool BIG, bool TFL, bool PSIT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_trav_wq(TravArgs a) {
+  __shared__ float4 tfs[TFL ? VFR_TF_LDS_SLOTS * 256 : 1];
+  if (TFL) {
+    for (uint32_t k = threadIdx.x; k < a.tf_lds_n * 256u; k += blockDim.x) tfs[k] = a.tf[a.tf_lds_base * 256u + k];
+    __syncthreads();
+  }
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t n = *a.n_rays;
+  unsigned long long st = 0, an = 0, br = 0;
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(a.queue, 64u);
+    base = __shfl(base, 0, 64);
+    if (base >= n) break;
+    const uint32_t i = base + lane;
+    if (i < n) {
+      const uint64_t ray = a.rays[i];
+      const uint32_t vi = (uint32_t)(ray >> 32), pix = (uint32_t)ray;
+      const RayOut o = trace<VFR_MODE_FTB, PROJ, BIG, TFL, PSIT>(a, tfs, vi, pix);
+      store_ray(a, a.views[vi].pix_off + pix, o);
+      st += o.steps; an += o.anomaly; br += o.broken;
+    }
+  }
+  st = wave_sum_u64(st);
+  an = wave_sum_u64(an);
+  br = wave_sum_u64(br);
+  if (lane == 0) {
+    if (st) atomicAdd(&a.stats[ST_STEPS], st);
+    if (an) atomicAdd(&a.stats[ST_ANOM], an);
+    if (br) atomicAdd(&a.stats[4], br);
+  }
+}
+
 /* Persistent work-queue traversal: a fixed grid of waves pulls rays from the
  * compacted list.  Whenever at least `refill` lanes of a wave are idle (their
  * rays ended or left through a concave boundary for good) the wave claims
@@ -1272,6 +1313,35 @@ int vfr_k_traverse_waves_per_simd(uint32_t occupancy, uint32_t* out) {
   return VFR_OK;
 }
 
+/* Blocks of the wave-queue kernel that fill the device: resident blocks per
+ * CU for this kernel instance times the CUs. */
+template <class K>
+static uint32_t wq_blocks(K kernel) {
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 1;
+  return (uint32_t)per_cu * (uint32_t)cus;
+}
+
+template <int PROJ, bool BIG, bool TFL, bool PSIT>
+static void launch_wq1(hipStream_t st, const TravArgs& a) {
+  static const uint32_t blocks = wq_blocks(k_trav_wq<PROJ, BIG, TFL, PSIT>);
+  hipLaunchKernelGGL((k_trav_wq<PROJ, BIG, TFL, PSIT>), dim3(blocks), dim3(256), 0, st, a);
+}
+
+template <bool PSIT>
+static void launch_wq(uint32_t proj, bool big, bool tfl, hipStream_t st, const TravArgs& a) {
+  if (big) launch_wq1<VFR_TRAV_MIXED, true, false, PSIT>(st, a);
+  else if (proj == VFR_TRAV_ORTHO && tfl) launch_wq1<VFR_TRAV_ORTHO, false, true, PSIT>(st, a);
+  else if (proj == VFR_TRAV_ORTHO) launch_wq1<VFR_TRAV_ORTHO, false, false, PSIT>(st, a);
+  else if (proj == VFR_TRAV_PERSP && tfl) launch_wq1<VFR_TRAV_PERSP, false, true, PSIT>(st, a);
+  else if (proj == VFR_TRAV_PERSP) launch_wq1<VFR_TRAV_PERSP, false, false, PSIT>(st, a);
+  else if (tfl) launch_wq1<VFR_TRAV_MIXED, false, true, PSIT>(st, a);
+  else launch_wq1<VFR_TRAV_MIXED, false, false, PSIT>(st, a);
+}
+
 template <bool PSIT>
 static void launch_ftb(uint32_t proj, bool tfl, dim3 grid, dim3 block, hipStream_t st, const TravArgs& a) {
   if (proj == VFR_TRAV_ORTHO && tfl)
@@ -1289,9 +1359,19 @@ static void launch_ftb(uint32_t proj, bool tfl, dim3 grid, dim3 block, hipStream
 }
 
 int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t occupancy, const TravArgs& a,
-                   uint32_t max_rays, int persistent, uint32_t grid_waves, void* stream) {
+                   uint32_t max_rays, int kernel, uint32_t grid_waves, void* stream) {
   if (max_rays == 0) return VFR_OK;
-  if (persistent && mode == VFR_MODE_FTB) {
+  if (kernel == VFR_WALK_QUEUE && mode == VFR_MODE_FTB) {
+    const bool tfl = a.tf_lds_n >= 1 && a.tf_lds_n <= VFR_TF_LDS_SLOTS;
+    const bool big = n_records >= VFR_BIG_RECORDS;
+    if (a.psi_table)
+      launch_wq<true>(proj, big, tfl, (hipStream_t)stream, a);
+    else
+      launch_wq<false>(proj, big, tfl, (hipStream_t)stream, a);
+    CHECK_HIP(hipGetLastError());
+    return VFR_OK;
+  }
+  if (kernel == VFR_WALK_REFILL && mode == VFR_MODE_FTB) {
     uint32_t blocks = (grid_waves + 3u) / 4u;
     if (blocks == 0) blocks = 1;
     const uint32_t burst = 8;

@@ -221,7 +221,9 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   const uint32_t proj = n_persp == 0 ? VFR_TRAV_ORTHO : (n_persp == nv ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
   const uint64_t records = (sv->cfg.flags & VFR_RENDER_WIDE) ? VFR_BIG_RECORDS : sv->n_records;
   CHECK_RC(vfr_k_traverse(VFR_MODE_FTB, proj, records, sv->cfg.occupancy, a, (uint32_t)pix,
-                          (sv->cfg.flags & VFR_RENDER_PERSISTENT) ? 1 : 0,
+                          (sv->cfg.flags & VFR_RENDER_PERSISTENT)    ? VFR_WALK_REFILL
+                          : (sv->cfg.flags & VFR_RENDER_WAVE_QUEUE) ? VFR_WALK_QUEUE
+                                                                    : VFR_WALK_THREAD,
                           sv->cfg.waves_per_cu * (uint32_t)vfr_ctx_cus(sv->ctx), st));
   CHECK_HIP(hipMemcpyAsync(s->h_stats, s->stats, sizeof(unsigned long long) * ST_N, hipMemcpyDeviceToHost, st));
   CHECK_HIP(hipMemcpyAsync(s->h_counters, s->counters, 16, hipMemcpyDeviceToHost, st));
