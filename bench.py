@@ -206,8 +206,8 @@ def serve_bench(args) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="batch64", choices=["batch64", "plain_gpu", "concave", "serve"])
     ap.add_argument("--requests", type=int, default=10000, help="serve: requests in the stream")
     ap.add_argument("--load", type=float, default=0.8, help="serve: offered load / batched capacity")
@@ -376,6 +376,7 @@ def main() -> int:
                    "schedule": tune or {"persistent": False}},
         "anomalous_rays": int(st["anomalies"]),
         "event_ms_per_step": round(ms_evt, 3),
+        "event_ms_median": round(float(np.median([s["ms_total"] for s in st_list])), 3),
         "active_mrays_per_s": round(rays_active / wall / 1e6, 2),
         "gsteps_per_s": round(steps_per_launch / wall / 1e9, 3),
         "plain": plain,
