@@ -369,3 +369,96 @@ def test_ftb_concave_c4_views(ctx, psi):
         cov = orc["entry_ext"] != V.NO_ID
         assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
     m.close()
+
+
+def test_ftb_batch64_full(ctx, psi):
+    """The benchmark's workload at full size: 128^3 grid (12.58 M tets), all
+    64 views.  Batched equals per-view bit for bit on every view, the stats
+    add up, no anomalies; three of the views (256^2, 384^2, 512^2) equal the
+    oracle (entry ids and steps bit for bit, RGBA within 2e-3, PSNR >= 50 dB
+    against FP64)."""
+    from vfray_amd import workloads as WL
+    x, t, tfa, seed = WL.batch64()
+    m = ctx.mesh(x, t)
+    tf = ctx.tf(tfa)
+    views = WL.sphere_views(seed, 64, list(WL.SIDES_BATCH), tf)
+    print("FTB 128^3 64 views batched vs per-view", flush=True)
+    a, sa = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+    b, sb = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+    assert sa["anomalies"] == 0 and sa["steps"] == sb["steps"]
+    assert sa["steps"] == sum(int(g["steps"].sum()) for g in a)
+    assert sa["active_rays"] == sum(int((g["entry_ext"] != V.NO_ID).sum()) for g in a)
+    for ga, gb in zip(a, b):
+        for key in ("rgba", "entry_ext", "steps"):
+            assert np.array_equal(ga[key], gb[key])
+    inp = dict(x=x, t=t, tf=tfa, xn=O.normalize(x))
+    inp["con"], _ = O.connectivity(t, x.shape[0])
+    inp["ext"] = O.extf(inp["con"])
+    inp["lmt"] = O.lmt(inp["xn"], t)
+    for side in (256, 384, 512):
+        i = [k for k, v in enumerate(views) if v.width == side][0]
+        print(f"oracle view {i} {side}^2", flush=True)
+        orc = oracle_ftb(inp, views[i], psi, fp64=True)
+        g = a[i]
+        assert orc["anomalies"] == 0
+        assert np.array_equal(g["entry_ext"], orc["entry_ext"])
+        assert np.array_equal(g["steps"], orc["steps"])
+        assert np.abs(g["rgba"] - orc["rgba"]).max() <= TOL
+        cov = orc["entry_ext"] != V.NO_ID
+        assert cov.mean() > 0.2
+        assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
+    m.close()
+
+
+def test_edge_cases(ctx, psi):
+    """A camera whose field of view misses the mesh (empty image, no rays),
+    one with the mesh behind it (refused: perspective views need the whole
+    mesh in front of the eye), the smallest image (2x2, both modes), a ray
+    along the mesh's boundary edge (more entry faces than a list holds:
+    reported), and a single tet seen in perspective, against the oracle."""
+    inp = grid_inputs(8, 0x5EED0106)
+    m = ctx.mesh(inp["x"], inp["t"])
+    tf = ctx.tf(inp["tf"])
+    print("edge cases", flush=True)
+    off = V.perspective_view(64, 48, tf, eye=(0, 0, 5.0), target=(3.0, 0, 0), fov_y_deg=20.0)
+    outs, st = ctx.render(m, [off], mode=V.MODE_FTB)
+    assert st["active_rays"] == 0 and st["steps"] == 0
+    assert not outs[0]["rgba"].any() and (outs[0]["entry_ext"] == V.NO_ID).all()
+    behind = V.perspective_view(64, 48, tf, eye=(0, 0, 3.0), target=(0, 0, 6.0))
+    with pytest.raises(V.VfrError, match="in front|inside or beside"):
+        ctx.render(m, [behind], mode=V.MODE_FTB)
+    for mode in (V.MODE_FTB, V.MODE_REF):
+        # front-to-back: scaled so that the four pixel rays do not run along
+        # the mesh's boundary edges (a ray along an edge enters every face
+        # on it, more than any entry list holds: reported as an error)
+        mat = np.diag([1.2, 1.2, 1.2]).astype(np.float32) if mode == V.MODE_FTB else np.eye(3, dtype=np.float32)
+        tiny = V.ortho_view(2, 2, tf, matrix=np.hstack([mat, np.zeros((3, 1), np.float32)]))
+        g = ctx.render(m, [tiny], mode=mode)[0][0]
+        if mode == V.MODE_FTB:
+            orc = oracle_ftb(inp, tiny, psi)
+        else:
+            orc = O.render_ref(inp["xn"], inp["t"], inp["con"], inp["ext"], inp["tf"], psi, inp["lmt"], 2, 2)
+        assert np.array_equal(g["entry_ext"], orc["entry_ext"])
+        assert np.array_equal(g["steps"], orc["steps"])
+        assert np.array_equal(g["rgba"], orc["rgba"])
+    with pytest.raises(V.VfrError, match="dropped"):
+        ctx.render(m, [V.ortho_view(2, 2, tf)], mode=V.MODE_FTB)
+    m.close()
+    x = np.array([[0, 0, 0, 0.1], [1, 0, 0, 0.9], [0, 1, 0, 0.5], [0.3, 0.3, 1, 0.3]], np.float32)
+    tt = np.array([[0, 1, 2, 3]], np.uint32)
+    one = dict(x=x, t=tt, tf=inp["tf"], xn=O.normalize(x))
+    one["con"], _ = O.connectivity(tt, 4)
+    one["ext"] = O.extf(one["con"])
+    one["lmt"] = O.lmt(one["xn"], tt)
+    m1 = ctx.mesh(x, tt)
+    v = V.perspective_view(96, 96, tf, eye=(1.5, -2.0, 1.2))
+    g = ctx.render(m1, [v], mode=V.MODE_FTB)[0][0]
+    orc = oracle_ftb(one, v, psi, fp64=True)
+    cov = orc["entry_ext"] != V.NO_ID
+    assert cov.sum() > 200
+    assert np.array_equal(g["entry_ext"], orc["entry_ext"])
+    assert np.array_equal(g["steps"], orc["steps"])
+    assert np.array_equal(g["rgba"], orc["rgba"])
+    assert (g["steps"][cov] == 1).all()
+    assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
+    m1.close()
