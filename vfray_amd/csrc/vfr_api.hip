@@ -334,7 +334,7 @@ struct vfr_plan {
    * overlaps the walk of the chunks before them (stream 2) */
   struct Chunk {
     uint32_t v0, nv, tile0, ntiles, proj;
-    uint64_t pix0, npix, nverts, nfaces;
+    uint64_t pix0, npix;
   };
   std::vector<Chunk> chunks;
   hipStream_t st2 = nullptr;
@@ -438,9 +438,8 @@ static int plan_count_entries(vfr_plan* p, uint32_t* out) {
   const vfr_mesh* m = p->mesh;
   CHECK_RC(vfr_k_clear_u32(p->owner, p->n_pix, st));
   CHECK_RC(vfr_k_clear_u32(p->counters, 1, st));
-  const uint64_t nverts = (uint64_t)p->n_views * m->d.n_verts, nfaces = (uint64_t)p->n_views * m->d.n_ext;
-  CHECK_RC(vfr_k_xform_views(p->views, 0, p->n_views, nverts, (const float4*)m->d.vert, p->pverts, st));
-  CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, nfaces, m->tv4(), m->d.ext_int, p->pverts, p->owner,
+  CHECK_RC(vfr_k_xform_views(p->views, 0, p->n_views, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
+  CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner,
                             nullptr, 0, p->stats, st));
   CHECK_RC(vfr_k_max_u32(p->owner, p->n_pix, p->counters, st));
   CHECK_HIP(hipMemcpyAsync(out, p->counters, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -454,12 +453,11 @@ static int enqueue_batched(vfr_plan* p) {
   CHECK_RC(vfr_k_clear_u32(p->owner, p->n_pix, st));
   CHECK_RC(vfr_k_clear_u32(p->counters, p->n_views + 2, st));
   CHECK_RC(vfr_k_clear_u32((uint32_t*)p->stats, 2 * ST_N, st));
-  const uint64_t nverts = (uint64_t)p->n_views * m->d.n_verts, nfaces = (uint64_t)p->n_views * m->d.n_ext;
-  CHECK_RC(vfr_k_xform_views(p->views, 0, p->n_views, nverts, (const float4*)m->d.vert, p->pverts, st));
+  CHECK_RC(vfr_k_xform_views(p->views, 0, p->n_views, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
   if (p->opts.mode == VFR_MODE_REF)
-    CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, nfaces, m->tv4(), m->d.ext_int, p->pverts, p->owner, st));
+    CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner, st));
   else
-    CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, nfaces, m->tv4(), m->d.ext_int, p->pverts, p->owner,
+    CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner,
                               p->entries, p->kmax, p->stats, st));
   CHECK_RC(vfr_k_compact(p->views, p->n_views, 0, p->n_tiles, p->opts.mode, p->owner, p->owner,
                          p->entries, p->kmax, p->rgba, p->steps, p->entry, p->rays, p->counters + p->n_views, st));
@@ -690,8 +688,6 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
           ++v;
           ++ch.nv;
         }
-      ch.nverts = (uint64_t)ch.nv * m->d.n_verts;
-      ch.nfaces = (uint64_t)ch.nv * m->d.n_ext;
       ch.proj = persp == 0 ? VFR_TRAV_ORTHO : (persp == ch.nv ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
       p->chunks.push_back(ch);
     }
@@ -749,8 +745,8 @@ static int execute_pipelined(vfr_plan* p, double* ms_total, double* ms_vis, doub
   CHECK_RC(vfr_k_clear_u32((uint32_t*)p->stats, 2 * ST_N, st));
   for (size_t c = 0; c < nc; ++c) {
     const vfr_plan::Chunk& ch = p->chunks[c];
-    CHECK_RC(vfr_k_xform_views(p->views, ch.v0, ch.nv, ch.nverts, (const float4*)m->d.vert, p->pverts, st));
-    CHECK_RC(vfr_k_raster_ftb(p->views, ch.v0, ch.nv, ch.nfaces, m->tv4(), m->d.ext_int, p->pverts, p->owner,
+    CHECK_RC(vfr_k_xform_views(p->views, ch.v0, ch.nv, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
+    CHECK_RC(vfr_k_raster_ftb(p->views, ch.v0, ch.nv, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner,
                               p->entries, p->kmax, p->stats, st));
     CHECK_RC(vfr_k_compact(p->views, p->n_views, ch.tile0, ch.ntiles, p->opts.mode, p->owner, p->owner,
                            p->entries, p->kmax, p->rgba, p->steps, p->entry, p->rays + ch.pix0, ccount + c, st));

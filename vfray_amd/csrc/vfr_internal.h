@@ -140,14 +140,14 @@ struct TravArgs {
 
 int vfr_k_clear_u32(uint32_t* p, size_t n, void* stream);
 int vfr_k_copy(const void* src, void* dst, size_t bytes, void* stream);
-/* Batch launches cover views [v0, v0+nv); `work` is the number of vertices
- * (xform) or external faces (raster) summed over those views. */
-int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work,
+/* Batch launches cover views [v0, v0+nv) with a grid of (the most vertices
+ * or external faces any of those views' meshes has) x nv. */
+int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_verts,
                       const float4* wvert, float4* pv, void* stream);
-int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const uint4* tv,
+int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* tv,
                      const uint32_t* ext_int, const float4* pv, uint32_t* owner, void* stream);
 /* kmax: entry slots per pixel; 0 only counts (cnt) without storing keys. */
-int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const uint4* tv,
+int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* tv,
                      const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
                      uint32_t kmax, unsigned long long* stats, void* stream);
 /* *out = max(*out, cnt[i]) over n pixels (device counter). */

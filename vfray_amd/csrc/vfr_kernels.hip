@@ -92,38 +92,25 @@ DEV ViewRegs load_view(const DevView* views, uint32_t vi) {
   return r;
 }
 
-/* View of work item `g` of a batch over views [v0, v0+nv): the last view
- * whose prefix (vert_off or face_pre) is <= the prefix of v0 plus g. */
-DEV uint32_t view_of(const DevView* __restrict__ views, uint32_t v0, uint32_t nv, uint64_t g,
-                     bool faces) {
-  const uint32_t base = faces ? views[v0].face_pre : views[v0].vert_off;
-  uint32_t lo = v0, hi = v0 + nv;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    const uint32_t pre = faces ? views[mid].face_pre : views[mid].vert_off;
-    if ((uint64_t)(pre - base) <= g) lo = mid; else hi = mid;
-  }
-  return lo;
-}
 
 /* P-space vertices of each view: pv[vert_off + i] = xform(view, mesh vertex i). */
+/* Grid: x over a view's vertices, y over the views, so the view (and its
+ * transform) is uniform across a block and read with scalar loads. */
 __global__ __launch_bounds__(256) void k_xform_views(const DevView* __restrict__ views, uint32_t v0,
-                                                     uint32_t nv, uint64_t work,
                                                      const float4* __restrict__ wvert,
                                                      float4* __restrict__ pv) {
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= work) return;
-  const uint32_t vi = view_of(views, v0, nv, gid, false);
-  const uint32_t i = (uint32_t)(gid - (views[vi].vert_off - views[v0].vert_off));
+  const uint32_t vi = v0 + blockIdx.y;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= views[vi].n_verts) return;
   const ViewRegs vr = load_view(views, vi);
   pv[(size_t)views[vi].vert_off + i] = xform(vr, wvert[(size_t)views[vi].wvert_off + i]);
 }
 
-int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work,
+int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_verts,
                       const float4* wvert, float4* pv, void* stream) {
-  if (work == 0) return VFR_OK;
-  hipLaunchKernelGGL(k_xform_views, dim3((unsigned)((work + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, nv, work, wvert, pv);
+  if (nv == 0 || max_verts == 0) return VFR_OK;
+  hipLaunchKernelGGL(k_xform_views, dim3((max_verts + 255) / 256, nv), dim3(256), 0, (hipStream_t)stream, views, v0,
+                     wvert, pv);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -322,15 +309,13 @@ DEV void face_verts(const uint4* tv, const float4* pv, const uint32_t* ext_int, 
  * (vfRayKernelCPU.cc:274).  The largest visible index wins, as the last face
  * in main.cc's loop (main.cc:209-218) overwrites the pixel. */
 __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ views, uint32_t v0,
-                                                    uint32_t nv, uint64_t work,
                                                     const uint4* __restrict__ tv,
                                                     const uint32_t* __restrict__ ext_int,
                                                     const float4* __restrict__ pv,
                                                     uint32_t* __restrict__ owner) {
-  uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= work) return;
-  const uint32_t vi = view_of(views, v0, nv, gid, true);
-  const uint32_t e = (uint32_t)(gid - (views[vi].face_pre - views[v0].face_pre));
+  const uint32_t vi = v0 + blockIdx.y;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= views[vi].n_ext) return;
   const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
   float4 fv[4];
   uint32_t t, f;
@@ -359,11 +344,11 @@ __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ 
   }
 }
 
-int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const uint4* tv,
+int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* tv,
                      const uint32_t* ext_int, const float4* pv, uint32_t* owner, void* stream) {
-  if (work == 0) return VFR_OK;
-  hipLaunchKernelGGL(k_raster_ref, dim3((unsigned)((work + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, nv, work, tv, ext_int, pv, owner);
+  if (nv == 0 || max_faces == 0) return VFR_OK;
+  hipLaunchKernelGGL(k_raster_ref, dim3((max_faces + 255) / 256, nv), dim3(256), 0, (hipStream_t)stream, views, v0,
+                     tv, ext_int, pv, owner);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -371,17 +356,15 @@ int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint64_t wo
 /* FTB raster: every visible external face (front facing in P space) appends
  * the key (Z at the pixel, ext index) to each pixel its projection holds. */
 __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ views, uint32_t v0,
-                                                    uint32_t nv, uint64_t work,
                                                     const uint4* __restrict__ tv,
                                                     const uint32_t* __restrict__ ext_int,
                                                     const float4* __restrict__ pv,
                                                     uint32_t* __restrict__ cnt,
                                                     uint64_t* __restrict__ entries, uint32_t kmax,
                                                     unsigned long long* __restrict__ stats) {
-  uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= work) return;
-  const uint32_t vi = view_of(views, v0, nv, gid, true);
-  const uint32_t e = (uint32_t)(gid - (views[vi].face_pre - views[v0].face_pre));
+  const uint32_t vi = v0 + blockIdx.y;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= views[vi].n_ext) return;
   const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
   float4 fv[4];
   uint32_t t, f;
@@ -418,12 +401,12 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
   if (over && kmax) atomicAdd(&stats[ST_OVERFLOW], over);
 }
 
-int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint64_t work, const uint4* tv,
+int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* tv,
                      const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
                      uint32_t kmax, unsigned long long* stats, void* stream) {
-  if (work == 0) return VFR_OK;
-  hipLaunchKernelGGL(k_raster_ftb, dim3((unsigned)((work + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, nv, work, tv, ext_int, pv, cnt, entries, kmax, stats);
+  if (nv == 0 || max_faces == 0) return VFR_OK;
+  hipLaunchKernelGGL(k_raster_ftb, dim3((max_faces + 255) / 256, nv), dim3(256), 0, (hipStream_t)stream, views, v0,
+                     tv, ext_int, pv, cnt, entries, kmax, stats);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }

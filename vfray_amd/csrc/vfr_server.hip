@@ -162,7 +162,7 @@ static int slot_alloc(Slot* s, uint64_t bytes, uint32_t max_batch) {
 static int launch_batch(vfr_server* sv, Slot* s) {
   const uint32_t nv = (uint32_t)s->batch.size();
   uint64_t pix = 0, tiles = 0, verts = 0, faces = 0, keep_pix = 0;
-  uint32_t n_persp = 0, tf_lo = UINT32_MAX, tf_hi = 0;
+  uint32_t n_persp = 0, tf_lo = UINT32_MAX, tf_hi = 0, max_verts = 0, max_faces = 0;
   for (uint32_t i = 0; i < nv; ++i) {
     Req* r = s->batch[i];
     const SrvMesh& m = sv->meshes[r->mesh];
@@ -180,6 +180,8 @@ static int launch_batch(vfr_server* sv, Slot* s) {
     tiles += (uint64_t)d.tiles_x * ((r->view.height + 7) / 8);
     verts += m.n_verts;
     faces += m.n_ext;
+    max_verts = std::max(max_verts, m.n_verts);
+    max_faces = std::max(max_faces, m.n_ext);
     if (r->keep) {
       r->keep_off = keep_pix;
       keep_pix += r->pixels;
@@ -190,8 +192,8 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   CHECK_RC(vfr_k_clear_u32(s->cnt, pix, st));
   CHECK_RC(vfr_k_clear_u32(s->counters, 4, st));
   CHECK_RC(vfr_k_clear_u32((uint32_t*)s->stats, 2 * ST_N, st));
-  CHECK_RC(vfr_k_xform_views(s->views, 0, nv, verts, sv->vert, s->pverts, st));
-  CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, faces, sv->tv, sv->ext, s->pverts, s->cnt, s->entries, VFR_K,
+  CHECK_RC(vfr_k_xform_views(s->views, 0, nv, max_verts, sv->vert, s->pverts, st));
+  CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, max_faces, sv->tv, sv->ext, s->pverts, s->cnt, s->entries, VFR_K,
                             s->stats, st));
   CHECK_RC(vfr_k_compact(s->views, nv, 0, (uint32_t)tiles, VFR_MODE_FTB, nullptr, s->cnt, s->entries, VFR_K,
                          s->rgba, s->steps, s->entry, s->rays, s->counters, st));
