@@ -575,7 +575,6 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
     p->hv[i].tile_off = (uint32_t)tiles;
     p->hv[i].tiles_x = (views[i].width + 7) / 8;
     p->hv[i].vert_off = (uint32_t)((uint64_t)i * m->d.n_verts);
-    p->hv[i].face_pre = (uint32_t)((uint64_t)i * m->d.n_ext);
     p->hv[i].rec_off = 0;
     p->hv[i].ext_off = 0;
     p->hv[i].wvert_off = 0;

@@ -73,7 +73,6 @@ struct alignas(16) DevView {
   uint32_t tile_off;  /* first 8x8 tile of this view                         */
   uint32_t tiles_x;   /* 8x8 tiles per image row                             */
   uint32_t vert_off;  /* first P-space vertex of this view (prefix of n_verts) */
-  uint32_t face_pre;  /* prefix of n_ext over the views before this one      */
   uint32_t rec_off;   /* mesh: first tet record in the record arena          */
   uint32_t ext_off;   /* mesh: first external face in the face arena         */
   uint32_t wvert_off; /* mesh: first world vertex in the vertex arena        */
@@ -81,7 +80,7 @@ struct alignas(16) DevView {
   uint32_t n_verts;   /* mesh: vertices                                      */
   float max_edge;     /* mesh: .lmt max edge length                          */
   float inv_me;       /* 1 / max_edge                                        */
-  uint32_t pad_[3];
+  uint32_t pad_[4];
 };
 static_assert(sizeof(DevView) % 16 == 0, "DevView alignment");
 

@@ -161,7 +161,7 @@ static int slot_alloc(Slot* s, uint64_t bytes, uint32_t max_batch) {
 /* Enqueues one batch on the slot's stream.  Caller holds no lock. */
 static int launch_batch(vfr_server* sv, Slot* s) {
   const uint32_t nv = (uint32_t)s->batch.size();
-  uint64_t pix = 0, tiles = 0, verts = 0, faces = 0, keep_pix = 0;
+  uint64_t pix = 0, tiles = 0, verts = 0, keep_pix = 0;
   uint32_t n_persp = 0, tf_lo = UINT32_MAX, tf_hi = 0, max_verts = 0, max_faces = 0;
   for (uint32_t i = 0; i < nv; ++i) {
     Req* r = s->batch[i];
@@ -174,12 +174,10 @@ static int launch_batch(vfr_server* sv, Slot* s) {
     d.tile_off = (uint32_t)tiles;
     d.tiles_x = (r->view.width + 7) / 8;
     d.vert_off = (uint32_t)verts;
-    d.face_pre = (uint32_t)faces;
     s->h_views[i] = d;
     pix += r->pixels;
     tiles += (uint64_t)d.tiles_x * ((r->view.height + 7) / 8);
     verts += m.n_verts;
-    faces += m.n_ext;
     max_verts = std::max(max_verts, m.n_verts);
     max_faces = std::max(max_faces, m.n_ext);
     if (r->keep) {
