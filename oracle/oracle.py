@@ -40,6 +40,7 @@ def lib() -> C.CDLL:
         L.orc_expf.restype = C.c_float
         L.orc_psi_series.argtypes = [C.c_float, C.c_float]
         L.orc_psi_series.restype = C.c_float
+        L.orc_psi_zeta.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.orc_render_ref.argtypes = [f32p, u32, u32p, u32, u32p, u32p, u32, f32p, f32p, f32p, u32, u32,
                                      u8p, f32p, f64p, u32p, u32p, u32p, u32]
         L.orc_render_ref.restype = C.c_long
@@ -108,6 +109,13 @@ def expf(x: float) -> float:
 
 def psi_series(a: float, b: float) -> float:
     return lib().orc_psi_series(a, b)
+
+
+def psi_zeta(a: float, b: float) -> tuple[float, float]:
+    """(psi, zeta) of one segment as the front-to-back walk evaluates them."""
+    ps, z = C.c_float(), C.c_float()
+    lib().orc_psi_zeta(a, b, C.byref(ps), C.byref(z))
+    return ps.value, z.value
 
 
 def psi_exact(tf: float, tb: float) -> float:

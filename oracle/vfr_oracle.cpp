@@ -670,30 +670,38 @@ static inline u32 tf_index(float s) {
  * Z < Z_exit.  Outputs rgba (W*H*4), rgba64 (FP64 re-integration, may be
  * NULL), entry_ext (front-most entry), steps.  Returns the number of
  * anomalous rays (no exit face found), or -1 on bad input. */
-/* psi(a, b) in registers, as the renderer evaluates it: exp(-(3a + b)/8)
- * times the midpoint series J(p, q), p = (a + b)/2, q = (b - a)/2, with the
- * terms p^(2j) q^k, 2j + k <= 6, coefficients (-1)^k / ((2j)! k! 4^(j+k)
- * (2j + 2k + 1)) rounded to float, Horner in q then in p^2, every step a
- * fused multiply-add. */
-extern "C" float orc_psi_series(float a, float b) {
+/* psi(a, b) and zeta = exp(-(a + b)/2) of one segment, as the renderer
+ * evaluates them (psiGammaTable512.h:12-19, vfRayKernelCPU.cc:458): with
+ * p = (a + b)/2, q = (b - a)/2, E = exp(-p/2), psi = E K(p, q) and zeta = E E,
+ * K(p, q) = int_{-1/2}^{1/2} exp(-p t + q (1/4 - t^2)) dt from its series in
+ * p^(2j) q^k with coefficients M_jk / ((2j)! k!), M_jk = the integral of
+ * t^(2j) (1/4 - t^2)^k over [-1/2, 1/2], kept to q-degree 5, 3, 1, 0 for
+ * p^0, p^2, p^4, p^6, coefficients rounded to float, Horner in q then in p^2,
+ * every step a fused multiply-add. */
+extern "C" void orc_psi_zeta(float a, float b, float* psi, float* zeta) {
   float p = (a + b) * 0.5f, q = (b - a) * 0.5f;
   float P = p * p;
-  float c0 = fmaf(q, 2.60834003e-08f, -7.39820052e-07f);
-  c0 = fmaf(q, c0, 1.80844909e-05f);
-  c0 = fmaf(q, c0, -0.000372023816f);
-  c0 = fmaf(q, c0, 0.00625000009f);
-  c0 = fmaf(q, c0, -0.0833333358f);
+  float c0 = fmaf(q, 3.00625311e-06f, 6.61375670e-05f);
+  c0 = fmaf(q, c0, 0.00119047624f);
+  c0 = fmaf(q, c0, 0.0166666675f);
+  c0 = fmaf(q, c0, 0.166666672f);
   c0 = fmaf(q, c0, 1.0f);
-  float c1 = fmaf(q, 1.84955024e-06f, -3.61689818e-05f);
-  c1 = fmaf(q, c1, 0.000558035739f);
-  c1 = fmaf(q, c1, -0.00625000009f);
+  float c1 = fmaf(q, 1.65343918e-05f, 0.000297619059f);
+  c1 = fmaf(q, c1, 0.00416666688f);
   c1 = fmaf(q, c1, 0.0416666679f);
-  float c2 = fmaf(q, 9.04224544e-06f, -9.30059541e-05f);
-  c2 = fmaf(q, c2, 0.00052083336f);
-  float J = fmaf(P, 3.10019846e-06f, c2);
-  J = fmaf(P, J, c1);
-  J = fmaf(P, J, c0);
-  return orc_expf(fmaf(3.0f, a, b) * -0.125f) * J;
+  float c2 = fmaf(q, 3.72023824e-05f, 0.000520833360f);
+  float K = fmaf(P, 3.10019846e-06f, c2);
+  K = fmaf(P, K, c1);
+  K = fmaf(P, K, c0);
+  float E = orc_expf(p * -0.5f);
+  *psi = E * K;
+  *zeta = E * E;
+}
+
+extern "C" float orc_psi_series(float a, float b) {
+  float ps, zeta;
+  orc_psi_zeta(a, b, &ps, &zeta);
+  return ps;
 }
 
 extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, const u32* con,
@@ -873,7 +881,7 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
             prev = zd;
             F4 cN = ld4(tf, tf_index(snext));
             float tf_ = cP.w * dl, tb_ = cN.w * dl;
-            float ps;
+            float ps, zeta;
             if (psi) {
               /* 512 * gamma for both ends with one division */
               float uf = 1.0f + tf_, ub = 1.0f + tb_;
@@ -881,10 +889,10 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
               float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
               u32 jf = (u32)(gf + 0.5f), jb = (u32)(gb + 0.5f);
               ps = psi[std::min(jf, 511u) * 512 + std::min(jb, 511u)];
+              zeta = orc_expf(-(tf_ + tb_) * 0.5f);
             } else {
-              ps = orc_psi_series(tf_, tb_);
+              orc_psi_zeta(tf_, tb_, &ps, &zeta);
             }
-            float zeta = orc_expf(-(tf_ + tb_) * 0.5f);
             float w1 = 1.0f - ps, w2 = ps - zeta;
             ar = fmaf(Tr, fmaf(cP.x, w1, cN.x * w2), ar);
             ag = fmaf(Tr, fmaf(cP.y, w1, cN.y * w2), ag);

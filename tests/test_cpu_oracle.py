@@ -181,6 +181,23 @@ def test_psi_series_against_quadrature(psi):
         assert abs(psi[i, j] - O.psi_series(float(a), float(b))) < 2e-3
 
 
+def test_zeta_from_half_exponent():
+    """zeta = exp(-p/2)^2 (p = (a + b)/2, the walk's one exponential per
+    segment) within 4 ulp of libm's exp(-p) in double over the optical depths
+    a segment reaches, 1 at zero depth, and the psi it comes with equal to
+    psi_series."""
+    rng = np.random.default_rng(12)
+    pts = np.concatenate([rng.uniform(0, 1.5, (2000, 2)), rng.uniform(0, 1e-3, (200, 2)),
+                          [[0, 0], [1, 1], [1.5, 1.5], [0, 1e-7]]]).astype(np.float32)
+    for a, b in pts:
+        a, b = float(a), float(b)
+        ps, z = O.psi_zeta(a, b)
+        want = np.exp(-np.float64(np.float32((np.float32(a) + np.float32(b)) * np.float32(0.5))))
+        assert abs(z - want) <= 4 * float(np.spacing(np.float32(want))), (a, b, z, want)
+        assert ps == O.psi_series(a, b)
+    assert O.psi_zeta(0.0, 0.0) == (1.0, 1.0)
+
+
 def test_expf_accuracy():
     xs = np.concatenate([-np.logspace(-8, np.log10(86.9), 4000), [0.0, -1e-30, -0.5, -1.0]]).astype(np.float32)
     got = np.array([O.expf(float(v)) for v in xs], np.float64)

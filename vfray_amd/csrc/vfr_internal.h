@@ -47,11 +47,14 @@ double vfr_psi_integral(double tf, double tb);
  * vertices of slots f, f+1, f+2 of the tet's vertex ids, kept beside in a
  * uint4 array) the packed neighbour word and the face's vertex word: the
  * neighbour's vertex off that face (the tet's own opposite vertex when there
- * is no linked neighbour) in the low VFR_REC_VERT_BITS bits and above them,
- * in 2-bit fields p = 0, 1, 2, the neighbour's slot of this tet's vertex at
- * slot f + p.  A front-to-back step reads only face records (two loads from
- * one line; four tets per 128-byte line): it carries the slots of its face
- * vertices instead of their ids. */
+ * is no linked neighbour) in the low VFR_REC_VERT_BITS bits and above them
+ * the index into VFR_SLOT_PERMS of the slot map across the face: this tet's
+ * slot of each vertex of the face goes to that vertex's slot in the
+ * neighbour, and slot f + 3 (the vertex off the face) to the neighbour's own
+ * vertex off the face.  A front-to-back step reads only face records (two
+ * loads from one line; four tets per 128-byte line): it carries the slots of
+ * its face vertices, one per byte, instead of their ids, and moves them into
+ * the neighbour's numbering with one byte permute. */
 struct alignas(32) TetRec {
   uint32_t f[8];   /* face k: f[2k] neighbour word, f[2k+1] vertex word */
 };
@@ -59,6 +62,14 @@ struct alignas(32) TetRec {
 #define VFR_REC_VERT_MASK ((1u << VFR_REC_VERT_BITS) - 1u)
 /* A mesh's vertex ids fit the vertex word. */
 #define VFR_MAX_VERTS (1u << VFR_REC_VERT_BITS)
+/* The 24 permutations of four slots, lexicographic; byte i = the image of
+ * slot i. */
+#define VFR_N_SLOT_PERMS 24u
+static constexpr uint32_t VFR_SLOT_PERMS[VFR_N_SLOT_PERMS] = {
+    0x03020100u, 0x02030100u, 0x03010200u, 0x01030200u, 0x02010300u, 0x01020300u,
+    0x03020001u, 0x02030001u, 0x03000201u, 0x00030201u, 0x02000301u, 0x00020301u,
+    0x03010002u, 0x01030002u, 0x03000102u, 0x00030102u, 0x01000302u, 0x00010302u,
+    0x02010003u, 0x01020003u, 0x02000103u, 0x00020103u, 0x01000203u, 0x00010203u};
 static_assert(sizeof(TetRec) == 32, "TetRec is 32 bytes");
 
 /* One view of a render batch, with the offsets of its mesh in the device

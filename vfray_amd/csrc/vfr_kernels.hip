@@ -787,8 +787,8 @@ struct FtbRay {
   float ar, ag, ab, Tr;
   uint32_t t;              /* the tet, as an index into the record arena */
   uint32_t vi, voff, gp, tfo, steps;
-  uint32_t bits; /* cur 0..5 | nk 6..11 | FTB_PERSP | FTB_DEAD | FTB_BROKEN |
-                    record slots of P0, P1, P2, Q: 2 bits each from bit 16 */
+  uint32_t sl;             /* record slots of P0, P1, P2, Q: bytes 0..3 */
+  uint32_t bits;           /* cur 0..5 | nk 6..11 | FTB_PERSP | FTB_DEAD | FTB_BROKEN */
 };
 
 #define FTB_PERSP (1u << 12)  /* perspective view                        */
@@ -797,6 +797,12 @@ struct FtbRay {
 DEV uint32_t ftb_cur(const FtbRay& r) { return r.bits & 63u; }
 DEV uint32_t ftb_nk(const FtbRay& r) { return (r.bits >> 6) & 63u; }
 DEV bool ftb_persp(const FtbRay& r) { return (r.bits & FTB_PERSP) != 0u; }
+
+/* The slot permutations of the face records, staged in LDS once per block
+ * (the caller synchronizes before the walk reads them). */
+DEV void stage_slot_perms(uint32_t* sperm) {
+  if (threadIdx.x < VFR_N_SLOT_PERMS) sperm[threadIdx.x] = VFR_SLOT_PERMS[threadIdx.x];
+}
 
 /* u -> v edge function of the pixel (vertices relative to the pixel). */
 DEV float edge_rel(float4 u, float4 v) { return u.x * v.y - u.y * v.x; }
@@ -828,34 +834,37 @@ DEV void face_values(bool persp, float D, float wA, float4 A, float wB, float4 B
   *s = clamp01(sn * rc);
 }
 
-/* psi(a, b) = int_0^1 exp(-a s - (b - a) s^2 / 2) ds, the partial
- * pre-integration integral of psiGammaTable512.h:12-19 (a = optical depth at
- * the front end, b at the back), evaluated in registers.  With s = 1/2 + t,
- * psi = exp(-(3a + b) / 8) J, J = int_{-1/2}^{1/2} exp(-p t - q t^2) dt,
- * p = (a + b) / 2, q = (b - a) / 2, and
- *   J = sum_{j,k} p^(2j) (-q)^k / ((2j)! k! 4^(j+k) (2j + 2k + 1)),
- * kept to 2j + k <= 6 (16 terms, Horner in q, then in p^2): truncation error
- * below 3e-8 for a, b <= 1 (a segment's depth: alpha <= 1 over at most the
- * tet's longest edge, the limit's maxEdgeLength) and 4e-7 at 1.5. */
-DEV float psi_series(float a, float b) {
+/* psi and zeta of one segment from one exponential.  psi(a, b) =
+ * int_0^1 exp(-a s - (b - a) s^2 / 2) ds is the partial pre-integration
+ * integral of psiGammaTable512.h:12-19 (a = optical depth at the front end, b
+ * at the back) and zeta = exp(-(a + b) / 2) the segment's transparency
+ * (vfRayKernelCPU.cc:458).  With p = (a + b) / 2, q = (b - a) / 2 and
+ * s = 1/2 + t:
+ *   psi = exp(-p/2) K(p, q),  K = int_{-1/2}^{1/2} exp(-p t + q (1/4 - t^2)) dt,
+ *   zeta = exp(-p/2)^2,
+ *   K = sum_{j,k} p^(2j) q^k M_jk / ((2j)! k!),  M_jk = int_{-1/2}^{1/2} t^(2j) (1/4 - t^2)^k dt,
+ * kept to q-degree 5, 3, 1, 0 for p^0, p^2, p^4, p^6 (12 fused multiply-adds,
+ * Horner in q, then in p^2): truncation below 4.1e-8 for a, b <= 1 (a
+ * segment's depth: alpha <= 1 over at most the tet's longest edge, the
+ * limits' maxEdgeLength) and 4.2e-7 at 1.5. */
+DEV void psi_zeta(float a, float b, float* psi, float* zeta) {
   const float p = (a + b) * 0.5f, q = (b - a) * 0.5f;
   const float P = p * p;
-  float c0 = __builtin_fmaf(q, 2.60834003e-08f, -7.39820052e-07f);
-  c0 = __builtin_fmaf(q, c0, 1.80844909e-05f);
-  c0 = __builtin_fmaf(q, c0, -0.000372023816f);
-  c0 = __builtin_fmaf(q, c0, 0.00625000009f);
-  c0 = __builtin_fmaf(q, c0, -0.0833333358f);
+  float c0 = __builtin_fmaf(q, 3.00625311e-06f, 6.61375670e-05f);
+  c0 = __builtin_fmaf(q, c0, 0.00119047624f);
+  c0 = __builtin_fmaf(q, c0, 0.0166666675f);
+  c0 = __builtin_fmaf(q, c0, 0.166666672f);
   c0 = __builtin_fmaf(q, c0, 1.0f);
-  float c1 = __builtin_fmaf(q, 1.84955024e-06f, -3.61689818e-05f);
-  c1 = __builtin_fmaf(q, c1, 0.000558035739f);
-  c1 = __builtin_fmaf(q, c1, -0.00625000009f);
+  float c1 = __builtin_fmaf(q, 1.65343918e-05f, 0.000297619059f);
+  c1 = __builtin_fmaf(q, c1, 0.00416666688f);
   c1 = __builtin_fmaf(q, c1, 0.0416666679f);
-  float c2 = __builtin_fmaf(q, 9.04224544e-06f, -9.30059541e-05f);
-  c2 = __builtin_fmaf(q, c2, 0.00052083336f);
-  float J = __builtin_fmaf(P, 3.10019846e-06f, c2);
-  J = __builtin_fmaf(P, J, c1);
-  J = __builtin_fmaf(P, J, c0);
-  return vfr_expf(__builtin_fmaf(3.0f, a, b) * -0.125f) * J;
+  const float c2 = __builtin_fmaf(q, 3.72023824e-05f, 0.000520833360f);
+  float K = __builtin_fmaf(P, 3.10019846e-06f, c2);
+  K = __builtin_fmaf(P, K, c1);
+  K = __builtin_fmaf(P, K, c0);
+  const float E = vfr_expf(p * -0.5f);
+  *psi = E * K;
+  *zeta = E * E;
 }
 
 /* Transfer-function colour `i` (slot offset included): from the block's LDS
@@ -892,7 +901,7 @@ DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
     o01 = n01; o12 = n12; o20 = n20;
   }
   r.P0 = P0; r.P1 = P1; r.P2 = P2;
-  r.bits = (r.bits & 0xFFFFu) | ((fe | (s1 << 2) | (s2 << 4) | (s3 << 6)) << 16);
+  r.sl = fe | (s1 << 8) | (s2 << 16) | (s3 << 24);
   r.o01 = o01; r.o12 = o12; r.o20 = o20;
   const float D = (o12 + o20) + o01;
   float sprev;
@@ -944,9 +953,9 @@ enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
  * addressed with 64-bit arithmetic instead of 32-bit byte offsets.  PSIT:
  * psi is looked up in the 512x512 table (nearest entry of gamma = tau /
  * (1 + tau) at both ends, the reference's convention) instead of evaluated
- * by psi_series. */
+ * by psi_zeta. */
 template <int PROJ, bool BIG, bool TFL, bool PSIT>
-DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
+DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, FtbRay& r) {
   const bool persp = PROJ == VFR_TRAV_MIXED ? ftb_persp(r) : PROJ == VFR_TRAV_PERSP;
   r.steps += 1u;
   const float4 P3 = rel(r.Q, r.ox, r.oy);
@@ -987,28 +996,24 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
   float zd, snext;
   face_values(persp, D, w3, P3, w0, r.P0, w1, r.P1, w2, r.P2, &zd, &snext);
   /* the exit face leaves out vertex k of the entry face; in the record it
-   * is the face opposite k's slot, face (slot + 1) & 3 */
+   * is the face opposite k's slot s, face (s + 1) & 3 */
   const bool k0 = c2, k1 = !c1 && !c2, k2 = c1;
-  const uint32_t sl = r.bits >> 16;
-  const uint32_t kk = c1 ? 4u : (c2 ? 0u : 2u); /* 2k */
-  const uint32_t fx = ((sl >> kk) + 1u) & 3u;
+  const uint32_t sk = (r.sl >> (c1 ? 16u : (c2 ? 0u : 8u))) & 3u;
   /* its link (neighbour word; the neighbour's fourth vertex with, above bit
-   * 26, the neighbour's slots of this face's vertices) from the record words
-   * held in registers */
+   * 26, the slot map across the face) from the record words held in
+   * registers */
   const uint4 fa = r.fa, fb = r.fb;
-  const bool e0 = fx == 0u, e1 = fx == 1u, e2 = fx == 2u;
+  const bool e0 = sk == 3u, e1 = sk == 0u, e2 = sk == 1u;
   const uint32_t nb = pick3u(e0, e1, fa.x, fa.z, e2 ? fb.x : fb.z);
   const uint32_t qw = pick3u(e0, e1, fa.y, fa.w, e2 ? fb.y : fb.w);
-  const uint32_t i3n = qw & VFR_REC_VERT_MASK, pm = qw >> VFR_REC_VERT_BITS;
-  /* slots in the neighbour's record: vertex k's slot becomes the fourth
-   * vertex's, each face vertex at slot x maps to field (x - fx) & 3 of pm,
-   * and the neighbour's fourth vertex is opposite its back face */
-  const uint32_t xs = (sl & ~(3u << kk)) | (((sl >> 6) & 3u) << kk);
-  const uint32_t n0 = (pm >> (2u * ((xs - fx) & 3u))) & 3u;
-  const uint32_t n1 = (pm >> (2u * (((xs >> 2) - fx) & 3u))) & 3u;
-  const uint32_t n2 = (pm >> (2u * (((xs >> 4) - fx) & 3u))) & 3u;
-  const uint32_t n3 = (nb + 3u) & 3u;
-  const uint32_t sln = n0 | (n1 << 2) | (n2 << 4) | (n3 << 6);
+  const uint32_t i3n = qw & VFR_REC_VERT_MASK;
+  const uint32_t smap = sperm[qw >> VFR_REC_VERT_BITS];
+  /* slots in the neighbour's record: vertex k and the fourth vertex trade
+   * places (P3 replaces vertex k, the neighbour's vertex off the face becomes
+   * the fourth, and the map sends k's slot to that vertex's slot), then every
+   * byte goes through the map */
+  const uint32_t swp = c1 ? 0x02030100u : (c2 ? 0x00020103u : 0x01020300u);
+  const uint32_t sln = __builtin_amdgcn_perm(smap, smap, __builtin_amdgcn_perm(r.sl, r.sl, swp));
   /* neighbour words index the record arena (a server rebases each resident
    * mesh's words when it copies them in) */
   const uint32_t tn = nb < VFR_NB_BROKEN ? (nb >> 2) : r.t;
@@ -1030,7 +1035,7 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const float dl = fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
   const float4 cP = TFL ? tfl[r.cpi] : r.cP;
   const float tf_ = cP.w * dl, tb_ = cN.w * dl;
-  float ps;
+  float ps, zeta;
   if (PSIT) {
     /* 512 * gamma = 512 * tau / (1 + tau) for both ends with one division */
     const float uf = 1.0f + tf_, ub = 1.0f + tb_;
@@ -1038,10 +1043,10 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
     const float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
     ps = a.psi[min(511u, (uint32_t)(gf + 0.5f)) * 512u + min(511u, (uint32_t)(gb + 0.5f))];
     __builtin_amdgcn_sched_barrier(0);
+    zeta = vfr_expf(-(tf_ + tb_) * 0.5f);
   } else {
-    ps = psi_series(tf_, tb_);
+    psi_zeta(tf_, tb_, &ps, &zeta);
   }
-  const float zeta = vfr_expf(-(tf_ + tb_) * 0.5f);
   /* the exit face becomes the entry face: vertex k is replaced by P3 */
   const float n01 = pick3(k0, k1, t1, -t0, r.o01);
   const float n12 = pick3(k1, k2, t2, -t1, r.o12);
@@ -1049,7 +1054,7 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, FtbRay& r) {
   r.P0 = sel4f(k0, P3, r.P0);
   r.P1 = sel4f(k1, P3, r.P1);
   r.P2 = sel4f(k2, P3, r.P2);
-  r.bits = (r.bits & 0xFFFFu) | (sln << 16);
+  r.sl = sln;
   r.o01 = n01; r.o12 = n12; r.o20 = n20;
   /* composite the segment */
   const float w1c = 1.0f - ps, w2c = ps - zeta;
@@ -1104,12 +1109,12 @@ DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
 }
 
 template <int PROJ, bool BIG, bool TFL, bool PSIT>
-DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pix) {
+DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, const uint32_t* sperm, uint32_t vi, uint32_t pix) {
   FtbRay r;
   ftb_init<TFL>(a, tfl, vi, pix, r);
   if (!(r.bits & FTB_DEAD)) {
     while (true) {
-      const uint32_t st = ftb_step<PROJ, BIG, TFL, PSIT>(a, tfl, r);
+      const uint32_t st = ftb_step<PROJ, BIG, TFL, PSIT>(a, tfl, sperm, r);
       if (st == FTB_GO || (st == FTB_REENTER && ftb_reenter<TFL>(a, tfl, r))) continue;
       break;
     }
@@ -1118,10 +1123,10 @@ DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t
 }
 
 template <uint32_t MODE, int PROJ, bool BIG, bool TFL, bool PSIT>
-DEV RayOut trace(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pix) {
+DEV RayOut trace(const TravArgs& a, const float4* tfl, const uint32_t* sperm, uint32_t vi, uint32_t pix) {
   if (MODE == VFR_MODE_REF) return trace_ref<false>(a, vi, pix);
   if (MODE == VFR_MODE_REF_LINEAR) return trace_ref<true>(a, vi, pix);
-  return trace_ftb<PROJ, BIG, TFL, PSIT>(a, tfl, vi, pix);
+  return trace_ftb<PROJ, BIG, TFL, PSIT>(a, tfl, sperm, vi, pix);
 }
 
 DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
@@ -1138,17 +1143,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR
                                                                      MODE == VFR_MODE_FTB ? 6 : 5))) void k_trav(
     TravArgs a) {
   __shared__ float4 tfs[TFL ? VFR_TF_LDS_SLOTS * 256 : 1];
+  __shared__ uint32_t sperm[VFR_N_SLOT_PERMS];
+  if (MODE == VFR_MODE_FTB) stage_slot_perms(sperm);
   if (TFL) {
     for (uint32_t k = threadIdx.x; k < a.tf_lds_n * 256u; k += blockDim.x) tfs[k] = a.tf[a.tf_lds_base * 256u + k];
-    __syncthreads();
   }
+  if (TFL || MODE == VFR_MODE_FTB) __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n = *a.n_rays;
   unsigned long long st = 0, an = 0, br = 0;
   if (i < n) {
     const uint64_t ray = a.rays[i];
     const uint32_t vi = (uint32_t)(ray >> 32), pix = (uint32_t)ray;
-    const RayOut o = trace<MODE, PROJ, BIG, TFL, PSIT>(a, tfs, vi, pix);
+    const RayOut o = trace<MODE, PROJ, BIG, TFL, PSIT>(a, tfs, sperm, vi, pix);
     store_ray(a, a.views[vi].pix_off + pix, o);
     st = o.steps; an = o.anomaly; br = o.broken;
   }
@@ -1172,10 +1179,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR
 template <int PROJ, bool BIG, bool TFL, bool PSIT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_trav_wq(TravArgs a) {
   __shared__ float4 tfs[TFL ? VFR_TF_LDS_SLOTS * 256 : 1];
+  __shared__ uint32_t sperm[VFR_N_SLOT_PERMS];
+  stage_slot_perms(sperm);
   if (TFL) {
     for (uint32_t k = threadIdx.x; k < a.tf_lds_n * 256u; k += blockDim.x) tfs[k] = a.tf[a.tf_lds_base * 256u + k];
-    __syncthreads();
   }
+  __syncthreads();
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t n = *a.n_rays;
   unsigned long long st = 0, an = 0, br = 0;
@@ -1188,7 +1197,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
     if (i < n) {
       const uint64_t ray = a.rays[i];
       const uint32_t vi = (uint32_t)(ray >> 32), pix = (uint32_t)ray;
-      const RayOut o = trace<VFR_MODE_FTB, PROJ, BIG, TFL, PSIT>(a, tfs, vi, pix);
+      const RayOut o = trace<VFR_MODE_FTB, PROJ, BIG, TFL, PSIT>(a, tfs, sperm, vi, pix);
       store_ray(a, a.views[vi].pix_off + pix, o);
       st += o.steps; an += o.anomaly; br += o.broken;
     }
@@ -1213,6 +1222,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
 template <int WPE, bool PSIT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_trav_persist(
     TravArgs a, uint32_t burst) {
+  __shared__ uint32_t sperm[VFR_N_SLOT_PERMS];
+  stage_slot_perms(sperm);
+  __syncthreads();
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t n = *a.n_rays;
   const uint32_t refill = a.refill < 1u ? 1u : (a.refill > 64u ? 64u : a.refill);
@@ -1258,7 +1270,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     /* every lane steps (an idle lane's stale ray stays inside the mesh
      * arrays and its result is ignored), so the walk is one straight path */
     for (uint32_t k = 0; k < burst; ++k) {
-      const uint32_t s = ftb_step<VFR_TRAV_MIXED, true, false, PSIT>(a, nullptr, r);
+      const uint32_t s = ftb_step<VFR_TRAV_MIXED, true, false, PSIT>(a, nullptr, sperm, r);
       if (active && s != FTB_GO && !(s == FTB_REENTER && ftb_reenter<false>(a, nullptr, r))) {
         const RayOut o = ftb_out(a, r);
         store_ray(a, r.gp, o);
@@ -1522,15 +1534,21 @@ __global__ void k_build_rec(const uint32_t* __restrict__ tets, const uint32_t* _
     n[f] = kk == 4 ? VFR_NB_BROKEN : ((tinv[u] << 2) | kk);
     if (kk != 4) {
       o[f] = vinv[tets[u * 4 + ((kk + 3) & 3)]];
-      /* the neighbour's slot of each vertex of this face */
-      uint32_t pm = 0;
+      /* the slot map across the face, byte i = the image of slot i: each
+       * vertex of the face to its slot in the neighbour, the vertex off the
+       * face to the neighbour's vertex off the face */
+      uint32_t map = ((kk + 3u) & 3u) << (8u * ((f + 3u) & 3u));
       for (uint32_t p = 0; p < 3; ++p) {
         const uint32_t id = tets[t * 4 + ((f + p) & 3)];
         uint32_t q = 0;
         while (q < 3 && tets[u * 4 + q] != id) ++q;
-        pm |= q << (2 * p);
+        map |= q << (8u * ((f + p) & 3u));
       }
-      o[f] |= pm << VFR_REC_VERT_BITS;
+      uint32_t pi = 0;
+      while (pi < VFR_N_SLOT_PERMS && VFR_SLOT_PERMS[pi] != map) ++pi;
+      /* not a permutation: the neighbour's linked face is not this face */
+      if (pi == VFR_N_SLOT_PERMS) n[f] = VFR_NB_BROKEN;
+      else o[f] |= pi << VFR_REC_VERT_BITS;
     }
   }
 #pragma unroll
