@@ -38,6 +38,8 @@ def lib() -> C.CDLL:
         L.orc_psi_exact.restype = C.c_double
         L.orc_expf.argtypes = [C.c_float]
         L.orc_expf.restype = C.c_float
+        L.orc_rcp.argtypes = [C.c_float]
+        L.orc_rcp.restype = C.c_float
         L.orc_psi_series.argtypes = [C.c_float, C.c_float]
         L.orc_psi_series.restype = C.c_float
         L.orc_psi_zeta.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -105,6 +107,10 @@ def lmt(xyzs: np.ndarray, tets: np.ndarray) -> np.ndarray:
 
 def expf(x: float) -> float:
     return lib().orc_expf(x)
+
+
+def rcp(x: float) -> float:
+    return lib().orc_rcp(x)
 
 
 def psi_series(a: float, b: float) -> float:

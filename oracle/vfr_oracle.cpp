@@ -646,6 +646,19 @@ static inline F4 rel(const F4& p, float ox, float oy) {
  * u.x*v.y - u.y*v.x.  The same edge walked v -> u gives exactly the negated
  * float, so the faces behind an entry face cover it without gaps. */
 static inline float edge_rel(const F4& u, const F4& v) { return u.x * v.y - u.y * v.x; }
+/* 1/x as the renderer's walk forms it: the seed 0x7EF311C3 - bits(x) and
+ * three Newton steps y += y (1 - x y), each a pair of fused multiply-adds. */
+extern "C" float orc_rcp(float x) {
+  u32 u; memcpy(&u, &x, 4);
+  u = 0x7EF311C3u - u;
+  float y; memcpy(&y, &u, 4);
+  for (int i = 0; i < 3; ++i) {
+    float e = fmaf(-x, y, 1.0f);
+    y = fmaf(y, e, y);
+  }
+  return y;
+}
+
 /* Depth value and scalar of the point with weights wA..wE (>= 0, D their
  * sum) over four vertices: ortho gives Z, perspective the depth 1/Z; both
  * from one reciprocal. */
@@ -653,7 +666,7 @@ static inline void face_values(bool persp, float D, float wA, const F4& A, float
                                const F4& C, float wE, const F4& E, float* zd, float* s) {
   float zn = fmaf(wE, E.z, fmaf(wC, C.z, fmaf(wB, B.z, wA * A.z)));
   float sn = fmaf(wE, E.w, fmaf(wC, C.w, fmaf(wB, B.w, wA * A.w)));
-  float rc = 1.0f / (persp ? zn : D);
+  float rc = orc_rcp(persp ? zn : D);
   *zd = (persp ? D : zn) * rc;
   *s = clamp01(sn * rc);
 }

@@ -198,6 +198,22 @@ def test_zeta_from_half_exponent():
     assert O.psi_zeta(0.0, 0.0) == (1.0, 1.0)
 
 
+def test_rcp_accuracy():
+    """The walk's reciprocal (bit-pattern seed and three Newton steps) is
+    within 0.501 ulp of 1/x over the float range of either sign and exact on
+    powers of two; at +-0 it gives NaN (only discarded steps divide by 0)."""
+    rng = np.random.default_rng(13)
+    mag = 10.0 ** rng.uniform(-37, 37, 20000)
+    xs = np.concatenate([mag, -mag, rng.uniform(0.05, 4, 20000), [1.0, 2.0, 0.5, 3.0, 1e-30, 7.0]]).astype(np.float32)
+    got = np.array([O.rcp(float(v)) for v in xs], np.float64)
+    want = 1.0 / xs.astype(np.float64)
+    ulp = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64)
+    assert (np.abs(got - want) <= 0.501 * ulp).all()
+    for k in range(-20, 21):
+        assert O.rcp(2.0 ** k) == 2.0 ** -k
+    assert np.isnan(O.rcp(0.0)) and np.isnan(O.rcp(-0.0))
+
+
 def test_expf_accuracy():
     xs = np.concatenate([-np.logspace(-8, np.log10(86.9), 4000), [0.0, -1e-30, -0.5, -1.0]]).astype(np.float32)
     got = np.array([O.expf(float(v)) for v in xs], np.float64)

@@ -55,6 +55,32 @@ def test_connectivity_given_files(ctx, c1):
     m.close()
 
 
+def test_mesh_checks_and_limits(ctx):
+    """Mesh creation checks every id on the device (first offender named, as
+    the host checks did), and builds buildLmt's limits on the device bit for
+    bit, including its first-wins rule where the z extreme is a signed zero."""
+    x = np.array([[0, 0, -1.5, 0.1], [1, 0, -0.0, 0.2], [0, 1, 0.0, 0.3], [0, 0, -2, 0.4]], np.float32)
+    t = np.array([[0, 1, 2, 3]], np.uint32)
+    for z in ([-1.5, -0.0, 0.0, -2.0], [0.5, 0.0, -0.0, 2.0], [0.25, 0.5, 0.75, 0.125]):
+        xz = x.copy()
+        xz[:, 2] = np.array(z, np.float32)
+        m = ctx.mesh(xz, t, normalize=False)
+        lmt = m.connectivity()[2]
+        assert np.array_equal(lmt.view(np.uint32), O.lmt(xz, t).view(np.uint32)), (z, lmt)
+        m.close()
+    t2 = np.array([[0, 1, 2, 3], [0, 1, 2, 9]], np.uint32)
+    with pytest.raises(V.VfrError, match="tet 1 references vertex 9 >= 4"):
+        ctx.mesh(x, t2)
+    with pytest.raises(V.VfrError, match="repeated vertex"):
+        ctx.mesh(x, np.array([[0, 1, 2, 2]], np.uint32))
+    con = np.array([[0, 0, 0, 5]], np.uint32)
+    with pytest.raises(V.VfrError, match="con entry 3 = 5 >= 1"):
+        ctx.mesh(x, t, con=con)
+    con = np.zeros((1, 4), np.uint32)
+    with pytest.raises(V.VfrError, match="extf entry 2 invalid"):
+        ctx.mesh(x, t, con=con, extf=np.array([[0, 0], [0, 1], [0, 4], [0, 3]], np.uint32))
+
+
 def test_connectivity_32(ctx):
     inp = grid_inputs(32, 0x5EED0100)
     print("connectivity 32^3", flush=True)

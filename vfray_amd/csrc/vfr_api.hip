@@ -200,34 +200,17 @@ extern "C" int vfr_mesh_create(vfr_ctx* c, const float* xyzs, uint32_t nV, const
   }
   if (extf && !con) { vfr_set_error("vfr_mesh_create: extf given without con"); return VFR_EINVAL; }
   *out = nullptr;
-  for (size_t i = 0; i < 4 * (size_t)nT; ++i)
-    if (tets[i] >= nV) { vfr_set_error("tet %zu references vertex %u >= %u", i / 4, tets[i], nV); return VFR_EMESH; }
-  if (con)
-    for (size_t i = 0; i < 4 * (size_t)nT; ++i)
-      if (con[i] >= nT) { vfr_set_error("con entry %zu = %u >= %u", i, con[i], nT); return VFR_EMESH; }
-  if (extf)
-    for (size_t i = 0; i < (size_t)n_ext; ++i)
-      if (extf[i * 2] >= nT || extf[i * 2 + 1] > 3) { vfr_set_error("extf entry %zu invalid", i); return VFR_EMESH; }
   CHECK_HIP(hipSetDevice(c->device));
   auto t0 = std::chrono::steady_clock::now();
-  std::vector<float> v(xyzs, xyzs + 4 * (size_t)nV);
-  if (flags & VFR_MESH_NORMALIZE) vfr_normalize_vertices(v.data(), nV);
   vfr_mesh* m = new (std::nothrow) vfr_mesh;
   if (!m) { vfr_set_error("out of memory"); return VFR_ENOMEM; }
   m->ctx = c;
   m->d.n_verts = nV;
   m->d.n_tets = nT;
-  for (int a = 0; a < 3; ++a) { m->bb_lo[a] = 1e30f; m->bb_hi[a] = -1e30f; }
-  for (uint32_t i = 0; i < nV; ++i)
-    for (int a = 0; a < 3; ++a) {
-      m->bb_lo[a] = std::min(m->bb_lo[a], v[(size_t)i * 4 + a]);
-      m->bb_hi[a] = std::max(m->bb_hi[a], v[(size_t)i * 4 + a]);
-    }
-  int rc = vfr_dev_build_mesh(m->d, v.data(), tets, con, extf, n_ext,
-                              (flags & VFR_MESH_MORTON) != 0, c->stream);
+  /* ids are checked, vertices normalized and limits built on the device */
+  int rc = vfr_dev_build_mesh(m->d, xyzs, tets, con, extf, n_ext, (flags & VFR_MESH_NORMALIZE) != 0,
+                              (flags & VFR_MESH_MORTON) != 0, lmt, m->bb_lo, m->bb_hi, c->stream);
   if (rc != VFR_OK) { vfr_mesh_destroy(m); return rc; }
-  if (lmt) memcpy(m->d.lmt, lmt, sizeof(float) * 3);
-  else vfr_build_lmt(v.data(), nV, tets, nT, m->d.lmt);
   if (!(m->d.lmt[0] > 0.0f)) { vfr_set_error("max edge length %g is not positive", (double)m->d.lmt[0]); vfr_mesh_destroy(m); return VFR_EMESH; }
   m->h_extf.resize(2 * (size_t)m->d.n_ext);
   if (m->d.n_ext &&

@@ -209,10 +209,16 @@ int vfr_k_rebase_links(TetRec* rec, uint32_t n, uint32_t add, void* stream);
 int vfr_k_rotate_normalize(float4* v, uint32_t n, const float* mv, int normalize, float lo[3], float hi[3],
                            void* stream);
 
-/* preprocessing on the device */
+/* Preprocessing on the device: uploads the caller's vertices and tets
+ * (and .con/.extf contents when given), checks every id, normalizes the
+ * vertices (normalizeVertices) when asked, builds connectivity and external
+ * faces by sort-and-match unless given, the limits (buildLmt) unless h_lmt is
+ * given, the Morton orderings and the face records.  bb_lo/bb_hi receive the
+ * x, y, z bounds of the (normalized) vertices. */
 int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
                        const uint32_t* h_con, const uint32_t* h_extf, uint32_t n_ext_given,
-                       bool morton, void* stream);
+                       bool normalize, bool morton, const float* h_lmt, float bb_lo[3], float bb_hi[3],
+                       void* stream);
 #endif /* __HIPCC__ */
 
 #endif

@@ -822,6 +822,23 @@ DEV uint32_t pick3u(bool c1, bool c2, uint32_t a, uint32_t b, uint32_t c) { retu
 
 DEV float4 rel(float4 p, float ox, float oy) { return make_float4(p.x - ox, p.y - oy, p.z, p.w); }
 
+/* 1/x: a seed from the bit pattern (0x7EF311C3 - bits(x), within 5.1 % of
+ * 1/x for every normal x of either sign) and three Newton steps
+ * y += y (1 - x y) as fused multiply-adds: within 0.5005 ulp of 1/x
+ * (correctly rounded for 99.98 % of inputs); NaN at +-0, which the walk
+ * meets only on a step it discards (a degenerate face, D = 0).  Seven
+ * full-rate instructions where IEEE division takes ten with a quarter-rate
+ * reciprocal. */
+DEV float vfr_rcp(float x) {
+  float y = __uint_as_float(0x7EF311C3u - __float_as_uint(x));
+  float e = __builtin_fmaf(-x, y, 1.0f);
+  y = __builtin_fmaf(y, e, y);
+  e = __builtin_fmaf(-x, y, 1.0f);
+  y = __builtin_fmaf(y, e, y);
+  e = __builtin_fmaf(-x, y, 1.0f);
+  return __builtin_fmaf(y, e, y);
+}
+
 /* Depth value and scalar of the point with weights wA..wD (>= 0, D their
  * sum) over four vertices: ortho returns Z, perspective the depth 1/Z; both
  * from one reciprocal. */
@@ -829,7 +846,7 @@ DEV void face_values(bool persp, float D, float wA, float4 A, float wB, float4 B
                      float wE, float4 E, float* zd, float* s) {
   const float zn = __builtin_fmaf(wE, E.z, __builtin_fmaf(wC, C.z, __builtin_fmaf(wB, B.z, wA * A.z)));
   const float sn = __builtin_fmaf(wE, E.w, __builtin_fmaf(wC, C.w, __builtin_fmaf(wB, B.w, wA * A.w)));
-  const float rc = 1.0f / (persp ? zn : D);
+  const float rc = vfr_rcp(persp ? zn : D);
   *zd = (persp ? D : zn) * rc;
   *s = clamp01(sn * rc);
 }
@@ -1031,8 +1048,10 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
     fbn = ld32<uint4>(a.rec, tn * 32u + 16u);
   }
   __builtin_amdgcn_sched_barrier(0);
-  /* round 2: psi */
-  const float dl = fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
+  /* round 2: psi.  A skipped step composites nothing: with dl = 0 both
+   * depths are 0, psi = zeta = 1 and the update leaves colour and
+   * transparency unchanged. */
+  const float dl = skip ? 0.0f : fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
   const float4 cP = TFL ? tfl[r.cpi] : r.cP;
   const float tf_ = cP.w * dl, tb_ = cN.w * dl;
   float ps, zeta;
@@ -1061,11 +1080,10 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
   const float ar = __builtin_fmaf(r.Tr, __builtin_fmaf(cP.x, w1c, cN.x * w2c), r.ar);
   const float ag = __builtin_fmaf(r.Tr, __builtin_fmaf(cP.y, w1c, cN.y * w2c), r.ag);
   const float ab = __builtin_fmaf(r.Tr, __builtin_fmaf(cP.z, w1c, cN.z * w2c), r.ab);
-  const float Tr = r.Tr * zeta;
-  r.ar = skip ? r.ar : ar;
-  r.ag = skip ? r.ag : ag;
-  r.ab = skip ? r.ab : ab;
-  r.Tr = skip ? r.Tr : Tr;
+  r.ar = ar;
+  r.ag = ag;
+  r.ab = ab;
+  r.Tr = r.Tr * zeta;
   if (TFL) r.cpi = ci; else r.cP = cN;
   r.prev = zd;
   r.t = tn;
@@ -1688,6 +1706,49 @@ int vfr_k_rotate_normalize(float4* v, uint32_t n, const float* mv, int normalize
   return VFR_OK;
 }
 
+/* Ids out of range: `first` receives the smallest index i with
+ * a[i] >= limit, or, with `faces` set, a[i] > 3 at odd i (the face numbers of
+ * (tet, face) pairs). */
+__global__ void k_check_ids(const uint32_t* __restrict__ a, uint64_t n, uint32_t limit, int faces,
+                            unsigned long long* __restrict__ first) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t lim = (faces && (i & 1u)) ? 4u : limit;
+  if (a[i] >= lim) atomicMin(first, (unsigned long long)i);
+}
+
+/* buildLmt's longest edge (vfRayPreComp.cc:808-833): per tet the six edges
+ * (0,1) (0,2) (0,3) (1,2) (1,3) (2,3), length = sqrt in float of
+ * (dx*dx + dy*dy) + dz*dz, unfused; the maximum over every tet as the bits of
+ * a non-negative float (NaN lengths are skipped, as the reference's `>` test
+ * skips them). */
+__global__ __launch_bounds__(256) void k_max_edge(const float4* __restrict__ v, const uint4* __restrict__ tets,
+                                                  uint32_t nT, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  float m = 0.0f;
+  if (i < nT) {
+    const uint4 q = tets[i];
+    const float4 p[4] = {v[q.x], v[q.y], v[q.z], v[q.w]};
+    const int E[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const float4 a = p[E[j][0]], b = p[E[j][1]];
+      const float e0 = a.x - b.x, e1 = a.y - b.y, e2 = a.z - b.z;
+      const float len = sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+      if (len > m) m = len;
+    }
+  }
+  uint32_t bits = __float_as_uint(m);
+  for (int o = 32; o > 0; o >>= 1) bits = max(bits, (uint32_t)__shfl_xor((int)bits, o, 64));
+  if ((threadIdx.x & 63u) == 0 && bits) atomicMax(out, bits);
+}
+
+/* Smallest index of a vertex whose z is +0 or -0. */
+__global__ void k_first_zero_z(const float4* __restrict__ v, uint32_t n, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && v[i].z == 0.0f) atomicMin(out, i);
+}
+
 __global__ void k_ext_internal(const uint32_t* __restrict__ extf, uint32_t n,
                                const uint32_t* __restrict__ tinv, uint32_t* __restrict__ out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1722,23 +1783,86 @@ inline unsigned nblk(uint64_t n) { return (unsigned)((n + 255) / 256); }
  * .extf caches). */
 int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
                        const uint32_t* h_con, const uint32_t* h_extf, uint32_t n_ext_given,
-                       bool morton, void* stream_v) {
+                       bool normalize, bool morton, const float* h_lmt, float bb_lo[3], float bb_hi[3],
+                       void* stream_v) {
   hipStream_t st = (hipStream_t)stream_v;
   const uint32_t nV = m.n_verts, nT = m.n_tets;
   const uint64_t nF = (uint64_t)nT * 4;
-  DevBuf d_xyzs, d_tets, d_con, d_bad;
+  DevBuf d_xyzs, d_tets, d_con, d_bad, d_first;
   ALLOC(d_xyzs, sizeof(float) * 4 * (size_t)nV);
   ALLOC(d_tets, sizeof(uint32_t) * 4 * (size_t)nT);
   ALLOC(d_con, sizeof(uint32_t) * 4 * (size_t)nT);
   ALLOC(d_bad, sizeof(uint32_t) * 4);
+  ALLOC(d_first, sizeof(unsigned long long) * 3);
   CHECK_HIP(hipMemcpyAsync(d_xyzs.p, h_xyzs, sizeof(float) * 4 * (size_t)nV, hipMemcpyHostToDevice, st));
   CHECK_HIP(hipMemcpyAsync(d_tets.p, h_tets, sizeof(uint32_t) * 4 * (size_t)nT, hipMemcpyHostToDevice, st));
   CHECK_HIP(hipMemsetAsync(d_bad.p, 0, 16, st));
-  uint32_t n_ext = 0;
+  CHECK_HIP(hipMemsetAsync(d_first.p, 0xFF, sizeof(unsigned long long) * 3, st));
+  /* ids in range: tets' vertices, given neighbours, given external faces */
   DevBuf d_extf;
+  if (h_extf) ALLOC(d_extf, sizeof(uint32_t) * 2 * (size_t)(n_ext_given ? n_ext_given : 1));
+  unsigned long long* first = d_first.as<unsigned long long>();
+  hipLaunchKernelGGL(k_check_ids, dim3(nblk(nF)), dim3(256), 0, st, d_tets.as<uint32_t>(), nF, nV, 0, first);
   if (h_con) {
     CHECK_HIP(hipMemcpyAsync(d_con.p, h_con, sizeof(uint32_t) * nF, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_check_ids, dim3(nblk(nF)), dim3(256), 0, st, d_con.as<uint32_t>(), nF, nT, 0, first + 1);
+  }
+  if (h_extf && n_ext_given) {
+    CHECK_HIP(hipMemcpyAsync(d_extf.p, h_extf, sizeof(uint32_t) * 2 * (size_t)n_ext_given,
+                             hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_check_ids, dim3(nblk(2ull * n_ext_given)), dim3(256), 0, st, d_extf.as<uint32_t>(),
+                       2ull * n_ext_given, nT, 1, first + 2);
+  }
+  CHECK_HIP(hipGetLastError());
+  unsigned long long hf[3];
+  CHECK_HIP(hipMemcpyAsync(hf, first, sizeof(hf), hipMemcpyDeviceToHost, st));
+  CHECK_HIP(hipStreamSynchronize(st));
+  if (hf[0] != ~0ull) {
+    vfr_set_error("tet %llu references vertex %u >= %u", hf[0] / 4, h_tets[hf[0]], nV);
+    return VFR_EMESH;
+  }
+  if (hf[1] != ~0ull) {
+    vfr_set_error("con entry %llu = %u >= %u", hf[1], h_con[hf[1]], nT);
+    return VFR_EMESH;
+  }
+  if (hf[2] != ~0ull) {
+    vfr_set_error("extf entry %llu invalid", hf[2] / 2);
+    return VFR_EMESH;
+  }
+  /* normalizeVertices on the device; the bounds of the result */
+  {
+    const int rc = vfr_k_rotate_normalize(d_xyzs.as<float4>(), nV, nullptr, normalize ? 1 : 0, bb_lo, bb_hi, st);
+    if (rc != VFR_OK) return rc;
+  }
+  /* buildLmt on the device unless the limits are given: the longest edge,
+   * and the z range starting from maxZ = -1, minZ = 1 (vfRayPreComp.cc:
+   * 835-850).  An extreme z of zero is the first zero in vertex order, as the
+   * reference's strict comparisons keep it, with its sign. */
+  if (h_lmt) {
+    memcpy(m.lmt, h_lmt, sizeof(float) * 3);
   } else {
+    DevBuf d_lm;
+    ALLOC(d_lm, sizeof(uint32_t) * 2);
+    uint32_t hl[2] = {0u, 0xFFFFFFFFu};
+    CHECK_HIP(hipMemcpyAsync(d_lm.p, hl, sizeof(hl), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_max_edge, dim3(nblk(nT)), dim3(256), 0, st, d_xyzs.as<float4>(),
+                       d_tets.as<uint4>(), nT, d_lm.as<uint32_t>());
+    const bool zero = bb_hi[2] == 0.0f || bb_lo[2] == 0.0f;
+    if (zero)
+      hipLaunchKernelGGL(k_first_zero_z, dim3(nblk(nV)), dim3(256), 0, st, d_xyzs.as<float4>(), nV,
+                         d_lm.as<uint32_t>() + 1);
+    CHECK_HIP(hipGetLastError());
+    CHECK_HIP(hipMemcpyAsync(hl, d_lm.p, sizeof(hl), hipMemcpyDeviceToHost, st));
+    CHECK_HIP(hipStreamSynchronize(st));
+    float z0 = 0.0f;
+    if (zero && hl[1] < nV)
+      CHECK_HIP(hipMemcpy(&z0, d_xyzs.as<float>() + 4 * (size_t)hl[1] + 2, sizeof(float), hipMemcpyDeviceToHost));
+    memcpy(&m.lmt[0], &hl[0], sizeof(float));
+    m.lmt[1] = bb_hi[2] > -1.0f ? (bb_hi[2] == 0.0f ? z0 : bb_hi[2]) : -1.0f;
+    m.lmt[2] = bb_lo[2] < 1.0f ? (bb_lo[2] == 0.0f ? z0 : bb_lo[2]) : 1.0f;
+  }
+  uint32_t n_ext = 0;
+  if (!h_con) {
     /* sort-and-match */
     uint32_t bits = 1;
     while (bits < 32 && (1ull << bits) < (uint64_t)nV) bits++;
@@ -1769,9 +1893,6 @@ int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
   }
   if (h_extf) {
     n_ext = n_ext_given;
-    ALLOC(d_extf, sizeof(uint32_t) * 2 * (size_t)n_ext);
-    CHECK_HIP(hipMemcpyAsync(d_extf.p, h_extf, sizeof(uint32_t) * 2 * (size_t)n_ext,
-                             hipMemcpyHostToDevice, st));
   } else {
     DevBuf flag, pos, tmp;
     ALLOC(flag, 4 * nF); ALLOC(pos, 4 * nF);
@@ -1800,17 +1921,9 @@ int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
   ALLOC(vperm, 4 * (size_t)nV); ALLOC(vinv, 4 * (size_t)nV);
   if (morton) {
     float hb[6];
-    {
-      float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-      for (uint32_t i = 0; i < nV; ++i)
-        for (int a = 0; a < 3; ++a) {
-          lo[a] = fminf(lo[a], h_xyzs[(size_t)i * 4 + a]);
-          hi[a] = fmaxf(hi[a], h_xyzs[(size_t)i * 4 + a]);
-        }
-      for (int a = 0; a < 3; ++a) {
-        hb[a] = lo[a];
-        hb[3 + a] = hi[a] > lo[a] ? 1023.0f / (hi[a] - lo[a]) : 0.0f;
-      }
+    for (int a = 0; a < 3; ++a) {
+      hb[a] = bb_lo[a];
+      hb[3 + a] = bb_hi[a] > bb_lo[a] ? 1023.0f / (bb_hi[a] - bb_lo[a]) : 0.0f;
     }
     DevBuf bb, code0, code1, id0, tmp;
     ALLOC(bb, sizeof(hb));
