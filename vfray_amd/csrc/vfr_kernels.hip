@@ -1036,7 +1036,7 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
   const uint32_t tn = nb < VFR_NB_BROKEN ? (nb >> 2) : r.t;
   /* round 1: the colour at the exit point and the next tet (the current one
    * again at a boundary, unused) */
-  const uint32_t ci = r.tfo + min(255u, (uint32_t)(255.0f * snext));
+  const uint32_t ci = r.tfo + (uint32_t)(255.0f * snext); /* snext is in [0, 1] */
   const float4 cN = tf_at<TFL>(a, tfl, ci);
   const float4 Qn = ld32<float4>(a.vert, (r.voff + i3n) * 16u);
   uint4 fan, fbn;
@@ -1626,9 +1626,10 @@ __global__ void k_minmax_init(uint32_t* mm) {
  * (4..7) of the result. */
 __global__ __launch_bounds__(256) void k_rotate_verts(float4* __restrict__ v, uint32_t n, Mat16 M, int rot,
                                                       uint32_t* __restrict__ mm) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t lo[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[4] = {0u, 0u, 0u, 0u};
-  if (i < n) {
+  /* grid-stride: a few thousand blocks each reduce many vertices, so the
+   * eight global atomics run once per block */
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float4 p = v[i];
     if (rot) {
       const float o[4] = {p.x, p.y, p.z, 1.0f};
@@ -1642,18 +1643,28 @@ __global__ __launch_bounds__(256) void k_rotate_verts(float4* __restrict__ v, ui
       v[i] = p;
     }
     const float q[4] = {p.x, p.y, p.z, p.w};
-    for (int a = 0; a < 4; ++a) lo[a] = hi[a] = fkey_ord(q[a]);
+    for (int a = 0; a < 4; ++a) {
+      lo[a] = min(lo[a], fkey_ord(q[a]));
+      hi[a] = max(hi[a], fkey_ord(q[a]));
+    }
   }
   for (int a = 0; a < 4; ++a)
     for (int o = 32; o > 0; o >>= 1) {
       lo[a] = min(lo[a], (uint32_t)__shfl_xor((int)lo[a], o, 64));
       hi[a] = max(hi[a], (uint32_t)__shfl_xor((int)hi[a], o, 64));
     }
+  __shared__ uint32_t blo[4][4], bhi[4][4];
+  const uint32_t w = threadIdx.x >> 6;
   if ((threadIdx.x & 63u) == 0)
-    for (int a = 0; a < 4; ++a) {
-      atomicMin(&mm[a], lo[a]);
-      atomicMax(&mm[4 + a], hi[a]);
-    }
+    for (int a = 0; a < 4; ++a) { blo[w][a] = lo[a]; bhi[w][a] = hi[a]; }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const uint32_t a = threadIdx.x;
+    uint32_t l = blo[0][a], h = bhi[0][a];
+    for (uint32_t k = 1; k < (blockDim.x >> 6); ++k) { l = min(l, blo[k][a]); h = max(h, bhi[k][a]); }
+    atomicMin(&mm[a], l);
+    atomicMax(&mm[4 + a], h);
+  }
 }
 
 /* normalizeVertices' update (vfRayPreComp.cc:136-143). */
@@ -1679,7 +1690,8 @@ int vfr_k_rotate_normalize(float4* v, uint32_t n, const float* mv, int normalize
   for (int k = 0; k < 16; ++k) M.m[k] = mv ? mv[k] : 0.0f;
   uint32_t h[8];
   hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, st, mm);
-  hipLaunchKernelGGL(k_rotate_verts, dim3((n + 255) / 256), dim3(256), 0, st, v, n, M, mv ? 1 : 0, mm);
+  const uint32_t rb = (n + 255) / 256 < 2048u ? (n + 255) / 256 : 2048u;
+  hipLaunchKernelGGL(k_rotate_verts, dim3(rb), dim3(256), 0, st, v, n, M, mv ? 1 : 0, mm);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(h, mm, sizeof(h), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1724,9 +1736,8 @@ __global__ void k_check_ids(const uint32_t* __restrict__ a, uint64_t n, uint32_t
  * skips them). */
 __global__ __launch_bounds__(256) void k_max_edge(const float4* __restrict__ v, const uint4* __restrict__ tets,
                                                   uint32_t nT, uint32_t* __restrict__ out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   float m = 0.0f;
-  if (i < nT) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nT; i += gridDim.x * blockDim.x) {
     const uint4 q = tets[i];
     const float4 p[4] = {v[q.x], v[q.y], v[q.z], v[q.w]};
     const int E[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
@@ -1740,7 +1751,13 @@ __global__ __launch_bounds__(256) void k_max_edge(const float4* __restrict__ v, 
   }
   uint32_t bits = __float_as_uint(m);
   for (int o = 32; o > 0; o >>= 1) bits = max(bits, (uint32_t)__shfl_xor((int)bits, o, 64));
-  if ((threadIdx.x & 63u) == 0 && bits) atomicMax(out, bits);
+  __shared__ uint32_t wb[4];
+  if ((threadIdx.x & 63u) == 0) wb[threadIdx.x >> 6] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t k = 1; k < (blockDim.x >> 6); ++k) bits = max(bits, wb[k]);
+    if (bits) atomicMax(out, bits);
+  }
 }
 
 /* Smallest index of a vertex whose z is +0 or -0. */
@@ -1845,7 +1862,7 @@ int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
     ALLOC(d_lm, sizeof(uint32_t) * 2);
     uint32_t hl[2] = {0u, 0xFFFFFFFFu};
     CHECK_HIP(hipMemcpyAsync(d_lm.p, hl, sizeof(hl), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_max_edge, dim3(nblk(nT)), dim3(256), 0, st, d_xyzs.as<float4>(),
+    hipLaunchKernelGGL(k_max_edge, dim3(nblk(nT) < 4096u ? nblk(nT) : 4096u), dim3(256), 0, st, d_xyzs.as<float4>(),
                        d_tets.as<uint4>(), nT, d_lm.as<uint32_t>());
     const bool zero = bb_hi[2] == 0.0f || bb_lo[2] == 0.0f;
     if (zero)
