@@ -39,6 +39,8 @@ def lib() -> C.CDLL:
         L.orc_expf.argtypes = [C.c_float]
         L.orc_expf.restype = C.c_float
         L.orc_rcp.argtypes = [C.c_float]
+        L.orc_exp_seg.argtypes = [C.c_float]
+        L.orc_exp_seg.restype = C.c_float
         L.orc_rcp.restype = C.c_float
         L.orc_psi_series.argtypes = [C.c_float, C.c_float]
         L.orc_psi_series.restype = C.c_float
@@ -107,6 +109,10 @@ def lmt(xyzs: np.ndarray, tets: np.ndarray) -> np.ndarray:
 
 def expf(x: float) -> float:
     return lib().orc_expf(x)
+
+
+def exp_seg(x: float) -> float:
+    return lib().orc_exp_seg(x)
 
 
 def rcp(x: float) -> float:

@@ -683,9 +683,23 @@ static inline u32 tf_index(float s) {
  * Z < Z_exit.  Outputs rgba (W*H*4), rgba64 (FP64 re-integration, may be
  * NULL), entry_ext (front-most entry), steps.  Returns the number of
  * anomalous rays (no exit face found), or -1 on bad input. */
+/* exp(x) as the walk forms exp(-p/2): on [-1/2, 0] the degree-6 Taylor
+ * polynomial about -1/4 (coefficients e^(-1/4)/k! rounded to float) in
+ * Horner form with fused multiply-adds; below -1/2, orc_expf. */
+extern "C" float orc_exp_seg(float x) {
+  if (x < -0.5f) return orc_expf(x);
+  float u = x + 0.25f;
+  float e = fmaf(u, 0.00108166772f, 0.00649000658f);
+  e = fmaf(u, e, 0.0324500315f);
+  e = fmaf(u, e, 0.129800126f);
+  e = fmaf(u, e, 0.389400393f);
+  e = fmaf(u, e, 0.778800786f);
+  return fmaf(u, e, 0.778800786f);
+}
+
 /* psi(a, b) and zeta = exp(-(a + b)/2) of one segment, as the renderer
  * evaluates them (psiGammaTable512.h:12-19, vfRayKernelCPU.cc:458): with
- * p = (a + b)/2, q = (b - a)/2, E = exp(-p/2), psi = E K(p, q) and zeta = E E,
+ * p = (a + b)/2, q = (b - a)/2, E = exp(-p/2) (orc_exp_seg), psi = E K(p, q) and zeta = E E,
  * K(p, q) = int_{-1/2}^{1/2} exp(-p t + q (1/4 - t^2)) dt from its series in
  * p^(2j) q^k with coefficients M_jk / ((2j)! k!), M_jk = the integral of
  * t^(2j) (1/4 - t^2)^k over [-1/2, 1/2], kept to q-degree 5, 3, 1, 0 for
@@ -706,7 +720,7 @@ extern "C" void orc_psi_zeta(float a, float b, float* psi, float* zeta) {
   float K = fmaf(P, 3.10019846e-06f, c2);
   K = fmaf(P, K, c1);
   K = fmaf(P, K, c0);
-  float E = orc_expf(p * -0.5f);
+  float E = orc_exp_seg(p * -0.5f);
   *psi = E * K;
   *zeta = E * E;
 }

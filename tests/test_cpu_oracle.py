@@ -198,6 +198,22 @@ def test_zeta_from_half_exponent():
     assert O.psi_zeta(0.0, 0.0) == (1.0, 1.0)
 
 
+def test_exp_seg_accuracy():
+    """exp(-p/2) of a segment: within 0.85 ulp of libm's exp on [-1/2, 0]
+    (polynomial), within 2 ulp below (the range-reduced expf), exactly 1 at
+    +-0, and continuous across -1/2 to within 2 ulp."""
+    xs = np.concatenate([np.linspace(-0.5, 0, 20001), -np.logspace(-8, np.log10(0.5), 2000),
+                         np.linspace(-3, -0.5, 2001)]).astype(np.float32)
+    got = np.array([O.exp_seg(float(v)) for v in xs], np.float64)
+    want = np.exp(xs.astype(np.float64))
+    ulp = np.spacing(want.astype(np.float32)).astype(np.float64)
+    err = np.abs(got - want) / ulp
+    assert err[xs >= -0.5].max() <= 0.85 and err.max() <= 2
+    assert O.exp_seg(0.0) == 1.0 and O.exp_seg(-0.0) == 1.0
+    below = float(np.nextafter(np.float32(-0.5), np.float32(-1)))
+    assert abs(O.exp_seg(below) - O.exp_seg(-0.5)) <= 2 * np.spacing(np.float32(0.6))
+
+
 def test_rcp_accuracy():
     """The walk's reciprocal (bit-pattern seed and three Newton steps) is
     within 0.501 ulp of 1/x over the float range of either sign and exact on
