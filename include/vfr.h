@@ -162,7 +162,7 @@ typedef struct vfr_render_opts {
   uint32_t occupancy;   /* persistent kernel variant: waves per SIMD (4, 5,
                            6: register-capped; other values: the compiler's
                            choice); tuned by vfr_plan_autotune            */
-  uint32_t composite;   /* VFR_COMPOSITE_* (REF mode; FTB takes PPI only)  */
+  uint32_t composite;   /* VFR_COMPOSITE_* (both modes)                    */
 } vfr_render_opts;
 
 /* Segment compositing.  PPI: partial pre-integration through the psi table
@@ -175,7 +175,12 @@ typedef struct vfr_render_opts {
  * LINEAR_OPACITY adds the USEOPACITY terms: transparency 1 - opacity, the
  * dz^2 correction, opacity accumulation, termination past 1, and the image
  * byte color * opacity.  rgba holds the image byte / 255 and the opacity (1
- * for LINEAR).  Both linear modes run on the REF walk (VFR_MODE_REF). */
+ * for LINEAR).  In REF mode the linear modes follow the reference's walk; in
+ * FTB mode the same segment formula runs front to back on the FTB walk
+ * (perspective views, re-entry through concave boundaries) with dz the
+ * segment's length in object units, the opacity stop as the only early
+ * termination (term_alpha does not apply), on the one-ray-per-thread kernel
+ * (the persistent and wave-queue flags are ignored), and psi is not used. */
 #define VFR_COMPOSITE_PPI 0u
 #define VFR_COMPOSITE_LINEAR 1u
 #define VFR_COMPOSITE_LINEAR_OPACITY 2u

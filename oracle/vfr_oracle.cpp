@@ -735,7 +735,7 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
                                const u32* extf, u32 nExt, const float* tf, const float* psi,
                                const float* lmt, u32 proj, const float* M, float view_k,
                                u32 W, u32 H, float term, float* rgba, double* rgba64,
-                               u32* entry_ext, u32* steps_out, u32* n_entries_out) {
+                               u32* entry_ext, u32* steps_out, u32* n_entries_out, u32 composite) {
   (void)nT;
   if (W < 2 || H < 2) return -1;
   OView view; view.proj = proj; memcpy(view.M, M, sizeof(view.M)); view.k = view_k;
@@ -830,11 +830,17 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
       }
       float ar = 0, ag = 0, ab = 0, Tr = 1.0f;
       double ar64 = 0, ag64 = 0, ab64 = 0, T64 = 1.0;
+      /* linear compositing: colour bytes and opacity (32-bit path), the same
+       * integral without the per-step truncation in FP64 */
+      const bool lin = composite != 0, opac = composite == 2;
+      u32 lc[3] = {0, 0, 0};
+      float lop = 0.0f;
+      double lc64[3] = {0, 0, 0}, lop64 = 0.0;
       bool live32 = true, live64 = (rgba64 != nullptr);
       F4 cP = {0.0f, 0.0f, 0.0f, 0.0f};
       u32 cur = 0;
       const bool persp = proj == 1;
-      const float rf = rayf * inv_me;
+      const float rf = lin ? rayf : rayf * inv_me;
       while (cur < nk && (live32 || live64)) {
         u32 ei = (u32)(K[cur] & 0xFFFFFFFFu);
         u32 tcur = extf[ei * 2], fent = extf[ei * 2 + 1];
@@ -903,7 +909,19 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
           u32 sk = 0;
           while (sk < 3 && T[tcur * 4 + sk] != ik) ++sk;
           u32 fex = (sk + 1) & 3;
-          if (live32) {
+          if (live32 && lin) {
+            /* vfRayKernel.cu:738-806 over dl object units, front to back */
+            float dl = std::fmax(fabsf(zd - prev) * rf, 0.0f);
+            prev = zd;
+            F4 cN = ld4(tf, tf_index(snext));
+            float pw = opac ? cP.w : 0.0f, nw = opac ? cN.w : 0.0f;
+            float tr = opac ? (float)(1.0 - (double)lop) : 1.0f;
+            const float pc[3] = {cP.x, cP.y, cP.z}, nc[3] = {cN.x, cN.y, cN.z};
+            for (int ch = 0; ch < 3; ++ch) lc[ch] = lin_channel(lc[ch], 255.0f * pc[ch], 255.0f * nc[ch], pw, nw, tr, dl);
+            if (opac) lop = (float)((double)lop + (double)((pw + nw) * dl) / 2.0);
+            cP = cN;
+            if (opac && lop > 1.0f) { lop = 1.0f; live32 = false; }
+          } else if (live32) {
             float dl = std::fmax(fabsf(zd - prev) * rf, 0.0f);
             prev = zd;
             F4 cN = ld4(tf, tf_index(snext));
@@ -938,17 +956,34 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
             double dist = proj == 1 ? fabs(1.0 / Zn64 - 1.0 / Zp64) * rayf64 : fabs(Zp64 - Zn64) * (double)view_k;
             double dl = dist / (double)maxEdge;
             F4 cN = ld4(tf, (u32)(255.0 * sn64));
-            double tf_ = cP64.w * dl, tb_ = cN.w * dl;
-            double zeta = exp(-(tf_ + tb_) * 0.5);
-            double ps = orc_psi_exact(tf_, tb_);
-            double w1 = 1.0 - ps, w2 = ps - zeta;
-            ar64 += T64 * (cP64.x * w1 + cN.x * w2);
-            ag64 += T64 * (cP64.y * w1 + cN.y * w2);
-            ab64 += T64 * (cP64.z * w1 + cN.z * w2);
-            T64 *= zeta;
-            cP64 = cN;
-            Zp64 = Zn64;
-            if (1.0 - T64 >= (double)term) live64 = false;
+            if (lin) {
+              /* colour c(u) linear between the samples times (1 - O(u)),
+               * O(u) the linearly interpolated alpha integrated along the
+               * segment: exact integral, clamped at 255 per step */
+              double aP = opac ? cP64.w : 0.0, aN = opac ? cN.w : 0.0, tr = 1.0 - lop64;
+              const double pc[3] = {cP64.x, cP64.y, cP64.z}, nc[3] = {cN.x, cN.y, cN.z};
+              for (int ch = 0; ch < 3; ++ch) {
+                double cp = 255.0 * pc[ch], cn = 255.0 * nc[ch];
+                lc64[ch] = std::min(255.0, lc64[ch] + 0.5 * (cp + cn) * tr * dist -
+                                               (3.0 * cp * aP + 5.0 * cn * aP + cp * aN + 3.0 * cn * aN) * dist * dist / 24.0);
+              }
+              if (opac) lop64 += (aP + aN) * dist / 2.0;
+              cP64 = cN;
+              Zp64 = Zn64;
+              if (opac && lop64 > 1.0) { lop64 = 1.0; live64 = false; }
+            } else {
+              double tf_ = cP64.w * dl, tb_ = cN.w * dl;
+              double zeta = exp(-(tf_ + tb_) * 0.5);
+              double ps = orc_psi_exact(tf_, tb_);
+              double w1 = 1.0 - ps, w2 = ps - zeta;
+              ar64 += T64 * (cP64.x * w1 + cN.x * w2);
+              ag64 += T64 * (cP64.y * w1 + cN.y * w2);
+              ab64 += T64 * (cP64.z * w1 + cN.z * w2);
+              T64 *= zeta;
+              cP64 = cN;
+              Zp64 = Zn64;
+              if (1.0 - T64 >= (double)term) live64 = false;
+            }
           }
           if (!live32 && !live64) break;
           u32 nb = con[tcur * 4 + fex];
@@ -982,6 +1017,17 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
       }
       out[0] = ar; out[1] = ag; out[2] = ab; out[3] = 1.0f - Tr;
       out64[0] = ar64; out64[1] = ag64; out64[2] = ab64; out64[3] = 1.0 - T64;
+      if (lin) {
+        /* image byte: color * opacity (with opacity) or color, as byte / 255 */
+        float op = opac ? lop : 1.0f;
+        for (int ch = 0; ch < 3; ++ch) {
+          u32 b = opac ? (u32)((float)lc[ch] * op) : lc[ch];
+          out[ch] = (float)b / 255.0f;
+          out64[ch] = (opac ? lc64[ch] * lop64 : lc64[ch]) / 255.0;
+        }
+        out[3] = op;
+        out64[3] = opac ? lop64 : 1.0;
+      }
     }
     if (rgba) for (int c = 0; c < 4; ++c) rgba[p * 4 + c] = out[c];
     if (rgba64) for (int c = 0; c < 4; ++c) rgba64[p * 4 + c] = out64[c];

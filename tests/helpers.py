@@ -35,10 +35,11 @@ def psnr(a: np.ndarray, b: np.ndarray, mask: np.ndarray) -> float:
     return float("inf") if mse == 0 else 10.0 * np.log10(1.0 / mse)
 
 
-def oracle_ftb(inp, view: V.View, psi, term=0.99, fp64=False, psi_table=False):
+def oracle_ftb(inp, view: V.View, psi, term=0.99, fp64=False, psi_table=False, composite=0):
     proj, M, k = V.view_matrix(view)
     return O.render_ftb(inp["xn"], inp["t"], inp["con"], inp["ext"], inp["tf"], psi, inp["lmt"],
-                        proj, M, k, view.width, view.height, term=term, fp64=fp64, psi_table=psi_table)
+                        proj, M, k, view.width, view.height, term=term, fp64=fp64, psi_table=psi_table,
+                        composite=composite)
 
 
 def legacy_tf_text(seed: int = 7) -> str:

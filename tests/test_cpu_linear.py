@@ -172,3 +172,95 @@ def test_linear_entry_and_steps_follow_the_ref_walk(psi):
         assert ran.mean() > 0.99
         assert np.array_equal(r["steps"][ran], r0["steps"][ran])
         assert r["rgb8"].max() > 0
+
+
+# ------------------------------------------------ front-to-back (FTB) linear
+
+
+def _views(seed, n, side):
+    from vfray_amd import workloads as WL
+    return WL.sphere_views(seed, n, [side], 0)
+
+
+@pytest.mark.parametrize("composite", [1, 2])
+def test_ftb_linear_against_fp64(psi, composite):
+    """Linear compositing on the front-to-back walk, perspective views: each
+    colour byte is the FP64 integral of the same segments (no per-step
+    truncation, clamped at 255) less under 1 per step it composited; the
+    opacity is the FP64 opacity.  The walk itself is PPI mode's without
+    termination: same entry faces, same steps where the opacity stop did not
+    end the ray."""
+    from helpers import oracle_ftb
+    g = grid_inputs(16, 0x5EED0311)
+    tf = g["tf"].copy().reshape(256, 4)
+    tf[:, 3] *= 0.6 if composite == 2 else 1.0
+    g = dict(g, tf=tf.reshape(-1))
+    checked = 0
+    for v in _views(0x5EED0311, 2, 160):
+        r0 = oracle_ftb(g, v, psi, term=1.0)
+        r = oracle_ftb(g, v, psi, fp64=True, composite=composite)
+        cov = r["entry_ext"] != V.NO_ID
+        assert r["anomalies"] == 0 and cov.mean() > 0.3
+        assert np.array_equal(r["entry_ext"], r0["entry_ext"])
+        op, op64 = r["rgba"][..., 3].astype(np.float64), r["rgba64"][..., 3]
+        ran = (op < 1.0) if composite == 2 else cov
+        assert np.array_equal(r["steps"][ran], r0["steps"][ran])
+        steps = r["steps"][cov].astype(np.float64)
+        b32 = np.rint(r["rgba"][cov][:, :3].astype(np.float64) * 255.0)
+        c64 = r["rgba64"][cov][:, :3] * 255.0
+        if composite == 1:
+            assert (op[cov] == 1.0).all()
+            assert (b32 <= c64 + 1e-3).all()
+            assert (b32 >= c64 - steps[:, None] - 1e-3).all()
+        else:
+            assert np.abs(op[cov] - op64[cov]).max() < 2e-4
+            o = op64[cov][:, None]
+            assert (b32 <= c64 + 1e-2).all()
+            assert (b32 >= c64 - (steps[:, None] + 1) * o - 1.01).all()
+        assert b32.max() > 50
+        checked += int(cov.sum())
+    assert checked > 10000
+
+
+def test_ftb_linear_constant_tf_telescopes(psi):
+    """Orthographic FTB view straight down the axis-aligned 8^3 cube (every
+    covered pixel crosses thickness L = 2 in object units), constant TF
+    without opacity: the colour is 255 c L less what the per-step
+    truncation drops (under 1 per step), as on the REF walk."""
+    from helpers import oracle_ftb
+    g = grid_inputs(8, 0x5EED0007)
+    c = 0.2
+    g = dict(g, tf=np.tile(np.array([c, c / 2, c / 4, 0.0], np.float32), 256))
+    v = V.ortho_view(96, 96, 0)
+    r = oracle_ftb(g, v, psi, composite=1)
+    cov = r["entry_ext"] != V.NO_ID
+    inner = np.zeros_like(cov)
+    inner[8:-8, 8:-8] = True
+    sel = cov & inner
+    assert sel.sum() > 4000
+    steps = r["steps"][sel].astype(np.float64)
+    rgb = np.rint(r["rgba"][sel][:, :3].astype(np.float64) * 255.0)
+    L = 2.0
+    for ch, cc in enumerate([c, c / 2, c / 4]):
+        hi = 255 * np.float32(cc) * L
+        assert (rgb[:, ch] <= hi + 1e-3).all() and (rgb[:, ch] >= hi - steps - 1e-3).all()
+
+
+def test_ftb_linear_ortho_close_to_ref_linear(psi):
+    """The same orthographic 512^2 view on the 16^3 mesh in REF and FTB
+    linear modes: the two walks cross the same tets with segment lengths
+    computed two ways (plane differences vs weighted face values), so bytes
+    agree to within a few units where both own the pixel with the same
+    entry face."""
+    from helpers import c1_inputs, oracle_ftb
+    g = c1_inputs()
+    v = V.ortho_view(512, 512, 0)
+    for comp in (1, 2):
+        rf = oracle_ftb(g, v, psi, composite=comp)
+        rr = O.render_ref(g["xn"], g["t"], g["con"], g["ext"], g["tf"], psi, g["lmt"], 512, 512, composite=comp)
+        both = (rf["entry_ext"] != V.NO_ID) & (rr["entry_ext"] == rf["entry_ext"])
+        assert both.mean() > 0.5
+        bf = np.rint(rf["rgba"][both][:, :3].astype(np.float64) * 255.0)
+        br = rr["rgb8"][both].astype(np.float64)
+        d = np.abs(bf - br)
+        assert np.median(d) <= 1 and np.mean(d <= 3) > 0.99, (comp, np.median(d), np.mean(d <= 3))

@@ -131,8 +131,44 @@ def test_ref_linear_c1(ctx, c1, psi, composite):
         assert np.array_equal(V.to_rgb8(g["rgba"]), orc["rgb8"])
         assert st["steps"] == int(orc["steps"].sum())
     assert orc["rgb8"].max() > 0
-    with pytest.raises(V.VfrError, match="REF mode"):
-        ctx.render(m, [V.ortho_view(64, 64, tf)], mode=V.MODE_FTB, composite=composite)
+    m.close()
+
+
+@pytest.mark.parametrize("composite", [V.COMPOSITE_LINEAR, V.COMPOSITE_LINEAR_OPACITY])
+def test_ftb_linear_equals_oracle(ctx, psi, composite):
+    """Linear compositing on the front-to-back walk: perspective views of the
+    voided, graded mesh (rays exit and re-enter), batched and per-view, bit
+    for bit with the oracle (entry faces, steps, RGBA); the walk-kernel flags
+    for PPI mode do not change the result."""
+    inp = grid_inputs(20, 0x5EED0303, variant=1)
+    tfa = inp["tf"].copy().reshape(256, 4)
+    tfa[:, 3] *= 0.5
+    inp = dict(inp, tf=tfa.reshape(-1))
+    m = ctx.mesh(inp["x"], inp["t"])
+    tf = ctx.tf(inp["tf"])
+    from vfray_amd import workloads as WL
+    views = WL.sphere_views(0x5EED0304, 3, [192, 256], tf)
+    print(f"FTB linear composite={composite} concave 20^3", flush=True)
+    a, sa = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, composite=composite)
+    b, _ = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW, composite=composite)
+    c, _ = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, composite=composite,
+                      persistent=True)
+    assert sa["anomalies"] == 0
+    multi = 0
+    for v, g, h, k in zip(views, a, b, c):
+        orc = oracle_ftb(inp, v, psi, composite=composite)
+        multi += int((orc["n_entries"] > 1).sum())
+        for key in ("rgba", "entry_ext", "steps"):
+            assert np.array_equal(g[key], h[key]) and np.array_equal(g[key], k[key]), key
+            assert np.array_equal(g[key], orc[key]), key
+        # not a black image: most covered pixels get colour (the oracle's
+        # render of these views: above 82 %), and with opacity some alpha
+        cov = orc["entry_ext"] != V.NO_ID
+        assert (g["rgba"][cov][:, :3].max(1) > 0).mean() > 0.5
+        if composite == V.COMPOSITE_LINEAR_OPACITY:
+            assert g["rgba"][..., 3].max() > 0.1
+    assert multi > 500
+    m.close()
     m.close()
 
 

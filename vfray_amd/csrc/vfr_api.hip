@@ -522,10 +522,6 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
     vfr_set_error("unknown compositing %u", o.composite);
     return VFR_EINVAL;
   }
-  if (o.composite != VFR_COMPOSITE_PPI && o.mode != VFR_MODE_REF) {
-    vfr_set_error("linear compositing is defined on the reference's walk (REF mode) only");
-    return VFR_EINVAL;
-  }
   if (o.waves_per_cu > 64 || o.refill > 64) {
     vfr_set_error("waves_per_cu %u / refill %u out of range (<= 64)", o.waves_per_cu, o.refill);
     return VFR_EINVAL;

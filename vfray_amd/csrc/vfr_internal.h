@@ -142,7 +142,7 @@ struct TravArgs {
   uint32_t tf_lds_base;    /* one-ray-per-thread FTB: TF slots [base, base+n)   */
   uint32_t tf_lds_n;       /* staged in LDS (0: read the global table)         */
   uint32_t kmax;           /* FTB: entry slots per pixel (8, 16 or 32)         */
-  uint32_t composite;      /* REF: VFR_COMPOSITE_*                             */
+  uint32_t composite;      /* VFR_COMPOSITE_* (both walks)                     */
   uint32_t psi_table;      /* FTB: psi from the table (VFR_RENDER_PSI_TABLE)     */
 };
 
@@ -170,9 +170,11 @@ int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, 
  * register-capped variants; anything else: as compiled). */
 /* proj: VFR_TRAV_* (the projections of the views the rays belong to). */
 enum { VFR_TRAV_ORTHO = 0, VFR_TRAV_PERSP = 1, VFR_TRAV_MIXED = 2 };
-/* Walk kernel variant beside VFR_MODE_REF / VFR_MODE_FTB: the REF walk with
- * linear compositing (TravArgs.composite != VFR_COMPOSITE_PPI). */
+/* Walk kernel variants beside VFR_MODE_REF / VFR_MODE_FTB: the REF walk and
+ * the front-to-back walk with linear compositing (TravArgs.composite !=
+ * VFR_COMPOSITE_PPI). */
 #define VFR_MODE_REF_LINEAR 2u
+#define VFR_MODE_FTB_LINEAR 3u
 /* n_records: tet records in the arena the rays walk (from VFR_BIG_RECORDS on,
  * the kernels address records with 64-bit arithmetic). */
 /* Walk kernels: one ray per thread (k_trav), the persistent kernel with lane

@@ -943,7 +943,9 @@ DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
   return D > 0.0f;
 }
 
-template <bool TFL>
+/* LIN: linear compositing; the ray's colour bytes live in ar, ag, ab (exact
+ * integers), its opacity in Tr, and segment lengths stay in object units. */
+template <bool TFL, bool LIN = false>
 DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pix, FtbRay& r) {
   const uint32_t W = a.views[vi].W, H = a.views[vi].H;
   const float k = a.views[vi].k;
@@ -955,12 +957,12 @@ DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pi
   r.ox = pix_to_obj(pix % W, W);
   r.oy = pix_to_obj(pix / W, H);
   const float rayf = persp ? sqrtf(1.0f + (r.ox * r.ox + r.oy * r.oy) * (k * k)) : k;
-  r.rf = rayf * a.views[vi].inv_me;
+  r.rf = LIN ? rayf : rayf * a.views[vi].inv_me;
   uint32_t nk = a.cnt[r.gp];
   nk = nk > a.kmax ? a.kmax : nk;
   r.bits = (nk << 6) | (persp ? FTB_PERSP : 0u);
   r.ar = r.ag = r.ab = 0.0f;
-  r.Tr = 1.0f;
+  r.Tr = LIN ? 0.0f : 1.0f;
   r.steps = 0;
   if (!ftb_segment<TFL>(a, tfl, r)) r.bits |= FTB_DEAD;
 }
@@ -985,7 +987,7 @@ enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
  * psi is looked up in the 512x512 table (nearest entry of gamma = tau /
  * (1 + tau) at both ends, the reference's convention) instead of evaluated
  * by psi_zeta. */
-template <int PROJ, bool BIG, bool TFL, bool PSIT>
+template <int PROJ, bool BIG, bool TFL, bool PSIT, bool LIN = false>
 DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, FtbRay& r) {
   const bool persp = PROJ == VFR_TRAV_MIXED ? ftb_persp(r) : PROJ == VFR_TRAV_PERSP;
   r.steps += 1u;
@@ -1064,21 +1066,34 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
   __builtin_amdgcn_sched_barrier(0);
   /* round 2: psi.  A skipped step composites nothing: with dl = 0 both
    * depths are 0, psi = zeta = 1 and the update leaves colour and
-   * transparency unchanged. */
+   * transparency unchanged (in linear compositing, both terms vanish). */
   const float dl = skip ? 0.0f : fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
   const float4 cP = TFL ? tfl[r.cpi] : r.cP;
-  const float tf_ = cP.w * dl, tb_ = cN.w * dl;
-  float ps, zeta;
-  if (PSIT) {
-    /* 512 * gamma = 512 * tau / (1 + tau) for both ends with one division */
-    const float uf = 1.0f + tf_, ub = 1.0f + tb_;
-    const float q = 512.0f / (uf * ub);
-    const float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
-    ps = a.psi[min(511u, (uint32_t)(gf + 0.5f)) * 512u + min(511u, (uint32_t)(gb + 0.5f))];
-    __builtin_amdgcn_sched_barrier(0);
-    zeta = vfr_expf(-(tf_ + tb_) * 0.5f);
-  } else {
-    psi_zeta(tf_, tb_, &ps, &zeta);
+  if (LIN) {
+    /* the reference's linear integration (vfRayKernel.cu:738-806, as in
+     * trace_ref) over a segment of dl object units, front to back */
+    const bool opac = a.composite == VFR_COMPOSITE_LINEAR_OPACITY;
+    const float pw = opac ? cP.w : 0.0f, nw = opac ? cN.w : 0.0f;
+    const float tr = opac ? (float)(1.0 - (double)r.Tr) : 1.0f;
+    r.ar = (float)lin_channel((uint32_t)r.ar, 255.0f * cP.x, 255.0f * cN.x, pw, nw, tr, dl);
+    r.ag = (float)lin_channel((uint32_t)r.ag, 255.0f * cP.y, 255.0f * cN.y, pw, nw, tr, dl);
+    r.ab = (float)lin_channel((uint32_t)r.ab, 255.0f * cP.z, 255.0f * cN.z, pw, nw, tr, dl);
+    if (opac) r.Tr = (float)((double)r.Tr + (double)((pw + nw) * dl) / 2.0);
+  }
+  float ps = 1.0f, zeta = 1.0f;
+  if (!LIN) {
+    const float tf_ = cP.w * dl, tb_ = cN.w * dl;
+    if (PSIT) {
+      /* 512 * gamma = 512 * tau / (1 + tau) for both ends with one division */
+      const float uf = 1.0f + tf_, ub = 1.0f + tb_;
+      const float q = 512.0f / (uf * ub);
+      const float gf = (tf_ * ub) * q, gb = (tb_ * uf) * q;
+      ps = a.psi[min(511u, (uint32_t)(gf + 0.5f)) * 512u + min(511u, (uint32_t)(gb + 0.5f))];
+      __builtin_amdgcn_sched_barrier(0);
+      zeta = vfr_expf(-(tf_ + tb_) * 0.5f);
+    } else {
+      psi_zeta(tf_, tb_, &ps, &zeta);
+    }
   }
   /* the exit face becomes the entry face: vertex k is replaced by P3 */
   const float n01 = pick3(k0, k1, t1, -t0, r.o01);
@@ -1090,21 +1105,27 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
   r.sl = sln;
   r.o01 = n01; r.o12 = n12; r.o20 = n20;
   /* composite the segment */
-  const float w1c = 1.0f - ps, w2c = ps - zeta;
-  const float ar = __builtin_fmaf(r.Tr, __builtin_fmaf(cP.x, w1c, cN.x * w2c), r.ar);
-  const float ag = __builtin_fmaf(r.Tr, __builtin_fmaf(cP.y, w1c, cN.y * w2c), r.ag);
-  const float ab = __builtin_fmaf(r.Tr, __builtin_fmaf(cP.z, w1c, cN.z * w2c), r.ab);
-  r.ar = ar;
-  r.ag = ag;
-  r.ab = ab;
-  r.Tr = r.Tr * zeta;
+  if (!LIN) {
+    const float w1c = 1.0f - ps, w2c = ps - zeta;
+    const float ar = __builtin_fmaf(r.Tr, __builtin_fmaf(cP.x, w1c, cN.x * w2c), r.ar);
+    const float ag = __builtin_fmaf(r.Tr, __builtin_fmaf(cP.y, w1c, cN.y * w2c), r.ag);
+    const float ab = __builtin_fmaf(r.Tr, __builtin_fmaf(cP.z, w1c, cN.z * w2c), r.ab);
+    r.ar = ar;
+    r.ag = ag;
+    r.ab = ab;
+    r.Tr = r.Tr * zeta;
+  }
   if (TFL) r.cpi = ci; else r.cP = cN;
   r.prev = zd;
   r.t = tn;
   r.Q = Qn;
   r.fa = fan;
   r.fb = fbn;
-  const bool term = 1.0f - r.Tr >= a.term;
+  /* linear: the reference's stop once the opacity passes 1 (vfRayKernel.cu:
+   * 799-803), opacity clamped to 1 */
+  const bool lstop = LIN && a.composite == VFR_COMPOSITE_LINEAR_OPACITY && r.Tr > 1.0f;
+  if (lstop) r.Tr = 1.0f;
+  const bool term = LIN ? lstop : 1.0f - r.Tr >= a.term;
   if (skip || term || nb >= VFR_NB_BROKEN) {
     /* rare: the ray ends or leaves through the boundary */
     if (skip) { r.bits |= FTB_DEAD; return FTB_DONE; }
@@ -1130,9 +1151,22 @@ DEV bool ftb_reenter(const TravArgs& a, const float4* tfl, FtbRay& r) {
   return true;
 }
 
+template <bool LIN = false>
 DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
   RayOut o;
-  o.c = make_float4(r.ar, r.ag, r.ab, 1.0f - r.Tr);
+  if (LIN) {
+    /* the image byte is color * opacity with opacity, color without
+     * (vfRayKernel.cu:836-842), stored as byte / 255 */
+    const bool opac = a.composite == VFR_COMPOSITE_LINEAR_OPACITY;
+    const float op = opac ? r.Tr : 1.0f;
+    const uint32_t cr = (uint32_t)r.ar, cg = (uint32_t)r.ag, cb = (uint32_t)r.ab;
+    const uint32_t br = opac ? (uint32_t)((float)cr * op) : cr;
+    const uint32_t bg = opac ? (uint32_t)((float)cg * op) : cg;
+    const uint32_t bb = opac ? (uint32_t)((float)cb * op) : cb;
+    o.c = make_float4((float)br / 255.0f, (float)bg / 255.0f, (float)bb / 255.0f, op);
+  } else {
+    o.c = make_float4(r.ar, r.ag, r.ab, 1.0f - r.Tr);
+  }
   o.steps = r.steps > VFR_MAX_STEPS ? VFR_MAX_STEPS : r.steps;
   o.entry = (uint32_t)(a.entries[(size_t)r.gp * a.kmax] & 0xFFFFFFFFull);
   o.anomaly = (r.bits & FTB_DEAD) ? 1u : 0u;
@@ -1140,25 +1174,26 @@ DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
   return o;
 }
 
-template <int PROJ, bool BIG, bool TFL, bool PSIT>
+template <int PROJ, bool BIG, bool TFL, bool PSIT, bool LIN = false>
 DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, const uint32_t* sperm, uint32_t vi, uint32_t pix) {
   FtbRay r;
-  ftb_init<TFL>(a, tfl, vi, pix, r);
+  ftb_init<TFL, LIN>(a, tfl, vi, pix, r);
   if (!(r.bits & FTB_DEAD)) {
     while (true) {
-      const uint32_t st = ftb_step<PROJ, BIG, TFL, PSIT>(a, tfl, sperm, r);
+      const uint32_t st = ftb_step<PROJ, BIG, TFL, PSIT, LIN>(a, tfl, sperm, r);
       if (st == FTB_GO || (st == FTB_REENTER && ftb_reenter<TFL>(a, tfl, r))) continue;
       break;
     }
   }
-  return ftb_out(a, r);
+  return ftb_out<LIN>(a, r);
 }
 
 template <uint32_t MODE, int PROJ, bool BIG, bool TFL, bool PSIT>
 DEV RayOut trace(const TravArgs& a, const float4* tfl, const uint32_t* sperm, uint32_t vi, uint32_t pix) {
-  if (MODE == VFR_MODE_REF) return trace_ref<false>(a, vi, pix);
-  if (MODE == VFR_MODE_REF_LINEAR) return trace_ref<true>(a, vi, pix);
-  return trace_ftb<PROJ, BIG, TFL, PSIT>(a, tfl, sperm, vi, pix);
+  if constexpr (MODE == VFR_MODE_REF) return trace_ref<false>(a, vi, pix);
+  else if constexpr (MODE == VFR_MODE_REF_LINEAR) return trace_ref<true>(a, vi, pix);
+  else if constexpr (MODE == VFR_MODE_FTB_LINEAR) return trace_ftb<PROJ, BIG, TFL, false, true>(a, tfl, sperm, vi, pix);
+  else return trace_ftb<PROJ, BIG, TFL, PSIT>(a, tfl, sperm, vi, pix);
 }
 
 DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
@@ -1176,11 +1211,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR
     TravArgs a) {
   __shared__ float4 tfs[TFL ? VFR_TF_LDS_SLOTS * 256 : 1];
   __shared__ uint32_t sperm[VFR_N_SLOT_PERMS];
-  if (MODE == VFR_MODE_FTB) stage_slot_perms(sperm);
+  constexpr bool ftb = MODE == VFR_MODE_FTB || MODE == VFR_MODE_FTB_LINEAR;
+  if (ftb) stage_slot_perms(sperm);
   if (TFL) {
     for (uint32_t k = threadIdx.x; k < a.tf_lds_n * 256u; k += blockDim.x) tfs[k] = a.tf[a.tf_lds_base * 256u + k];
   }
-  if (TFL || MODE == VFR_MODE_FTB) __syncthreads();
+  if (TFL || ftb) __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n = *a.n_rays;
   unsigned long long st = 0, an = 0, br = 0;
@@ -1388,7 +1424,8 @@ static void launch_ftb(uint32_t proj, bool tfl, dim3 grid, dim3 block, hipStream
 int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t occupancy, const TravArgs& a,
                    uint32_t max_rays, int kernel, uint32_t grid_waves, void* stream) {
   if (max_rays == 0) return VFR_OK;
-  if (kernel == VFR_WALK_QUEUE && mode == VFR_MODE_FTB) {
+  const bool lin = a.composite != VFR_COMPOSITE_PPI;
+  if (kernel == VFR_WALK_QUEUE && mode == VFR_MODE_FTB && !lin) {
     const bool tfl = a.tf_lds_n >= 1 && a.tf_lds_n <= VFR_TF_LDS_SLOTS;
     const bool big = n_records >= VFR_BIG_RECORDS;
     if (a.psi_table)
@@ -1398,7 +1435,7 @@ int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t oc
     CHECK_HIP(hipGetLastError());
     return VFR_OK;
   }
-  if (kernel == VFR_WALK_REFILL && mode == VFR_MODE_FTB) {
+  if (kernel == VFR_WALK_REFILL && mode == VFR_MODE_FTB && !lin) {
     uint32_t blocks = (grid_waves + 3u) / 4u;
     if (blocks == 0) blocks = 1;
     const uint32_t burst = 8;
@@ -1416,8 +1453,12 @@ int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t oc
   dim3 grid((max_rays + 255) / 256), block(256);
   hipStream_t st = (hipStream_t)stream;
   const bool tfl = a.tf_lds_n >= 1 && a.tf_lds_n <= VFR_TF_LDS_SLOTS;
-  if (mode == VFR_MODE_REF && a.composite != VFR_COMPOSITE_PPI)
+  if (mode == VFR_MODE_REF && lin)
     hipLaunchKernelGGL((k_trav<VFR_MODE_REF_LINEAR, VFR_TRAV_MIXED, true, false, false>), grid, block, 0, st, a);
+  else if (lin)
+    /* linear compositing on the front-to-back walk: one general variant
+     * (any projection, 64-bit record addressing, TF from global memory) */
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB_LINEAR, VFR_TRAV_MIXED, true, false, false>), grid, block, 0, st, a);
   else if (mode == VFR_MODE_REF)
     hipLaunchKernelGGL((k_trav<VFR_MODE_REF, VFR_TRAV_MIXED, true, false, false>), grid, block, 0, st, a);
   else if (n_records >= VFR_BIG_RECORDS && a.psi_table)
