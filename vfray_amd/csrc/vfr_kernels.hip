@@ -1000,25 +1000,31 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
   const bool ge0 = t0 >= 0.0f, ge1 = t1 >= 0.0f, ge2 = t2 >= 0.0f;
   const bool le0 = t0 <= 0.0f, le1 = t1 <= 0.0f, le2 = t2 <= 0.0f;
   bool p1 = ge0 && le1, p2 = ge1 && le2, p3 = ge2 && le0;
-  const bool z0 = ge0 && le0, z1 = ge1 && le1, z2 = ge2 && le2;
-  if ((z0 && z1) || (z1 && z2) || (z2 && z0)) {
+  bool c1 = p1, c2 = !p1 && p2;
+  if ((p1 && p2) || (p2 && p3) || (p3 && p1)) {
+    /* rare: two edge functions are zero (a candidate is dropped when its
+     * third edge is degenerate too), or the pixel is on a projected edge.
+     * Two zeros always leave two candidates (t_i = t_j = 0 makes p_ij true
+     * and one of the other two), so the zero rule is only needed here. */
+    const bool z0 = ge0 && le0, z1 = ge1 && le1, z2 = ge2 && le2;
     p1 = p1 && !(z0 && z1 && r.o01 == 0.0f);
     p2 = p2 && !(z1 && z2 && r.o12 == 0.0f);
     p3 = p3 && !(z2 && z0 && r.o20 == 0.0f);
+    c1 = p1;
+    c2 = !p1 && p2;
+    if ((p1 && p2) || (p2 && p3) || (p3 && p1)) {
+      /* exit at the point farthest along the ray (smallest Z), ties in
+       * candidate order */
+      const float inf = __uint_as_float(0x7f800000u);
+      const float z1 = p1 ? P3.z + (-t1 * (r.P0.z - P3.z) + t0 * (r.P1.z - P3.z)) / (t0 + r.o01 - t1) : inf;
+      const float z2 = p2 ? P3.z + (-t2 * (r.P1.z - P3.z) + t1 * (r.P2.z - P3.z)) / (t1 + r.o12 - t2) : inf;
+      const float z3 = p3 ? P3.z + (-t0 * (r.P2.z - P3.z) + t2 * (r.P0.z - P3.z)) / (t2 + r.o20 - t0) : inf;
+      c1 = p1 && z1 <= z2 && z1 <= z3;
+      c2 = !c1 && p2 && z2 <= z3;
+    }
   }
   /* no exit face or the step bound: the step is not composited */
   const bool skip = r.steps > VFR_MAX_STEPS || !(p1 || p2 || p3);
-  bool c1 = p1, c2 = !p1 && p2;
-  if ((p1 && p2) || (p2 && p3) || (p3 && p1)) {
-    /* the pixel is on a projected edge: exit at the point farthest along the
-     * ray (smallest Z), ties in candidate order */
-    const float inf = __uint_as_float(0x7f800000u);
-    const float z1 = p1 ? P3.z + (-t1 * (r.P0.z - P3.z) + t0 * (r.P1.z - P3.z)) / (t0 + r.o01 - t1) : inf;
-    const float z2 = p2 ? P3.z + (-t2 * (r.P1.z - P3.z) + t1 * (r.P2.z - P3.z)) / (t1 + r.o12 - t2) : inf;
-    const float z3 = p3 ? P3.z + (-t0 * (r.P2.z - P3.z) + t2 * (r.P0.z - P3.z)) / (t2 + r.o20 - t0) : inf;
-    c1 = p1 && z1 <= z2 && z1 <= z3;
-    c2 = !c1 && p2 && z2 <= z3;
-  }
   /* exit face (P3, Pi, Pj): P3 weighs o_ij, Pi weighs -t_j, Pj weighs t_i,
    * the left-out vertex Pk weighs 0 */
   const float w3 = pick3(c1, c2, r.o01, r.o12, r.o20);
