@@ -354,7 +354,16 @@ int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t ma
 }
 
 /* FTB raster: every visible external face (front facing in P space) appends
- * the key (Z at the pixel, ext index) to each pixel its projection holds. */
+ * the key (Z at the pixel, ext index) to each pixel its projection holds.
+ * Wave-cooperative: each wave owns RASTER_FPW external faces of
+ * one view: lanes 0..RASTER_FPW-1 set one face up each (visibility, the
+ * reference's inclusive inside test made ready, pixel bbox), then the whole
+ * wave rasterizes the visible faces one after another, 64 pixels at a time
+ * in 8x8 tiles of the bbox, so one atomic instruction appends the face to up
+ * to 64 pixel lists.  Per pixel the arithmetic is entry_test + bary's,
+ * operation for operation; the order in which faces reach a pixel's list
+ * does not matter (k_compact sorts each list by key). */
+#define RASTER_FPW 16u
 __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ views, uint32_t v0,
                                                     const uint4* __restrict__ tv,
                                                     const uint32_t* __restrict__ ext_int,
@@ -363,50 +372,99 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
                                                     uint64_t* __restrict__ entries, uint32_t kmax,
                                                     unsigned long long* __restrict__ stats) {
   const uint32_t vi = v0 + blockIdx.y;
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= views[vi].n_ext) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t e0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * RASTER_FPW;
+  const uint32_t nE = views[vi].n_ext;
+  if (e0 >= nE) return; /* the whole wave */
   const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
-  float4 fv[4];
-  uint32_t t, f;
-  face_verts(tv + views[vi].rec_off, pv + views[vi].vert_off, ext_int + 2 * (size_t)views[vi].ext_off, e,
-             fv, &t, &f);
-  Planes pl = face_planes(fv[0], fv[1], fv[2]);
-  float faceZ = fv[3].x * pl.z[0] + fv[3].y * pl.z[1] + pl.z[2];
-  if (faceZ < fv[3].z) return;
-  double lo0 = 1e30, lo1 = 1e30, hi0 = -1e30, hi1 = -1e30;
+  /* per-lane face setup */
+  bool vis = false;
+  uint32_t bx0 = 0, by0 = 0, bx1 = 0, by1 = 0;
+  float ax = 0, ay = 0, v01x = 0, v01y = 0, v02x = 0, v02y = 0, cx = 1, zA = 0, dB = 0, dC = 0, invD = 1;
+  uint32_t sw = 0;
+  const uint32_t e = e0 + lane;
+  if (lane < RASTER_FPW && e < nE) {
+    float4 fv[4];
+    uint32_t t, f;
+    face_verts(tv + views[vi].rec_off, pv + views[vi].vert_off, ext_int + 2 * (size_t)views[vi].ext_off, e,
+               fv, &t, &f);
+    const Planes pl = face_planes(fv[0], fv[1], fv[2]);
+    const float faceZ = fv[3].x * pl.z[0] + fv[3].y * pl.z[1] + pl.z[2];
+    double lo0 = 1e30, lo1 = 1e30, hi0 = -1e30, hi1 = -1e30;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    double fx = ((double)W - 1.0) * ((double)fv[k].x + 1.0) / 2.0;
-    double fy = ((double)H - 1.0) * ((double)fv[k].y + 1.0) / 2.0;
-    lo0 = fmin(lo0, fx); hi0 = fmax(hi0, fx);
-    lo1 = fmin(lo1, fy); hi1 = fmax(hi1, fy);
+    for (int k = 0; k < 3; ++k) {
+      const double fx = ((double)W - 1.0) * ((double)fv[k].x + 1.0) / 2.0;
+      const double fy = ((double)H - 1.0) * ((double)fv[k].y + 1.0) / 2.0;
+      lo0 = fmin(lo0, fx); hi0 = fmax(hi0, fx);
+      lo1 = fmin(lo1, fy); hi1 = fmax(hi1, fy);
+    }
+    vis = !(faceZ < fv[3].z) && !(hi0 < 0.0 || hi1 < 0.0 || lo0 > W - 1.0 || lo1 > H - 1.0);
+    bx0 = lo0 < 0.0 ? 0u : (uint32_t)lo0;
+    by0 = lo1 < 0.0 ? 0u : (uint32_t)lo1;
+    bx1 = hi0 > W - 1.0 ? W - 1u : (uint32_t)hi0;
+    by1 = hi1 > H - 1.0 ? H - 1u : (uint32_t)hi1;
+    /* entry_test's face part: edge vectors from vertex 0, their cross
+     * product (degenerate -> 1e-12), swapped to a positive orientation */
+    ax = fv[0].x; ay = fv[0].y;
+    v01x = fv[1].x - fv[0].x; v01y = fv[1].y - fv[0].y;
+    v02x = fv[2].x - fv[0].x; v02y = fv[2].y - fv[0].y;
+    cx = (v01x * v02y) - (v01y * v02x);
+    cx = tiny(cx) ? (float)1e-12 : cx;
+    if (cx < 0.0f) {
+      const float tx = v01x, ty = v01y;
+      v01x = v02x; v01y = v02y; v02x = tx; v02y = ty;
+      cx = -cx; sw = 1u;
+    }
+    /* bary's face part */
+    zA = fv[0].z; dB = fv[1].z - fv[0].z; dC = fv[2].z - fv[0].z;
+    invD = 1.0f / cx;
   }
-  if (hi0 < 0.0 || hi1 < 0.0 || lo0 > W - 1.0 || lo1 > H - 1.0) return;
-  uint32_t bx0 = lo0 < 0.0 ? 0u : (uint32_t)lo0, by0 = lo1 < 0.0 ? 0u : (uint32_t)lo1;
-  uint32_t bx1 = hi0 > W - 1.0 ? W - 1u : (uint32_t)hi0, by1 = hi1 > H - 1.0 ? H - 1u : (uint32_t)hi1;
   unsigned long long over = 0;
-  for (uint32_t px = bx0; px <= bx1; ++px) {
-    float ox = pix_to_obj(px, W);
-    for (uint32_t py = by0; py <= by1; ++py) {
-      float oy = pix_to_obj(py, H);
-      Hit h = entry_test(fv[0], fv[1], fv[2], ox, oy);
-      if (!h.ok) continue;
-      float Zp = bary(fv[0].z, fv[1].z, fv[2].z, h, 1.0f / h.D);
-      uint32_t gp = off + py * W + px;
-      uint32_t slot = atomicAdd(&cnt[gp], 1u);
-      if (slot < kmax) entries[(size_t)gp * kmax + slot] = ((uint64_t)fkey(Zp) << 32) | e;
-      else over++;
+  uint64_t todo = __ballot(vis);
+  while (todo) {
+    const int j = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1ull;
+    /* face j's set-up, uniform across the wave */
+    const uint32_t fx0 = __shfl(bx0, j, 64), fy0 = __shfl(by0, j, 64);
+    const uint32_t fx1 = __shfl(bx1, j, 64), fy1 = __shfl(by1, j, 64);
+    const float fax = __shfl(ax, j, 64), fay = __shfl(ay, j, 64);
+    const float f01x = __shfl(v01x, j, 64), f01y = __shfl(v01y, j, 64);
+    const float f02x = __shfl(v02x, j, 64), f02y = __shfl(v02y, j, 64);
+    const float fcx = __shfl(cx, j, 64), fzA = __shfl(zA, j, 64), fdB = __shfl(dB, j, 64);
+    const float fdC = __shfl(dC, j, 64), finv = __shfl(invD, j, 64);
+    const bool fsw = __shfl(sw, j, 64) != 0u;
+    const uint32_t fe = e0 + (uint32_t)j;
+    for (uint32_t ty = fy0; ty <= fy1; ty += 8u) {
+      for (uint32_t tx = fx0; tx <= fx1; tx += 8u) {
+        const uint32_t px = tx + (lane & 7u), py = ty + (lane >> 3);
+        if (px > fx1 || py > fy1) continue;
+        const float ox = pix_to_obj(px, W), oy = pix_to_obj(py, H);
+        const float v0px = ox - fax, v0py = oy - fay;
+        const float cy = (f01x * v0py) - (f01y * v0px);
+        const float cz = (v0px * f02y) - (v0py * f02x);
+        if (!((cy >= 0.0f) && (cz >= 0.0f) && ((cy + cz) <= fcx))) continue;
+        const float wb = fsw ? cy : cz, wc = fsw ? cz : cy;
+        const float Zp = fzA + ((wb * fdB + wc * fdC) * finv);
+        const uint32_t gp = off + py * W + px;
+        const uint32_t slot = atomicAdd(&cnt[gp], 1u);
+        if (slot < kmax) entries[(size_t)gp * kmax + slot] = ((uint64_t)fkey(Zp) << 32) | fe;
+        else over++;
+      }
     }
   }
-  if (over && kmax) atomicAdd(&stats[ST_OVERFLOW], over);
+  if (kmax) {
+    over = wave_sum_u64(over);
+    if (lane == 0 && over) atomicAdd(&stats[ST_OVERFLOW], over);
+  }
 }
 
 int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* tv,
                      const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
                      uint32_t kmax, unsigned long long* stats, void* stream) {
   if (nv == 0 || max_faces == 0) return VFR_OK;
-  hipLaunchKernelGGL(k_raster_ftb, dim3((max_faces + 255) / 256, nv), dim3(256), 0, (hipStream_t)stream, views, v0,
-                     tv, ext_int, pv, cnt, entries, kmax, stats);
+  const uint32_t per_block = 4u * RASTER_FPW; /* 256 threads: four waves */
+  hipLaunchKernelGGL(k_raster_ftb, dim3((max_faces + per_block - 1) / per_block, nv), dim3(256), 0,
+                     (hipStream_t)stream, views, v0, tv, ext_int, pv, cnt, entries, kmax, stats);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
