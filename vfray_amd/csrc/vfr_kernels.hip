@@ -355,15 +355,45 @@ int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t ma
 
 /* FTB raster: every visible external face (front facing in P space) appends
  * the key (Z at the pixel, ext index) to each pixel its projection holds.
- * Wave-cooperative: each wave owns RASTER_FPW external faces of
- * one view: lanes 0..RASTER_FPW-1 set one face up each (visibility, the
- * reference's inclusive inside test made ready, pixel bbox), then the whole
- * wave rasterizes the visible faces one after another, 64 pixels at a time
- * in 8x8 tiles of the bbox, so one atomic instruction appends the face to up
- * to 64 pixel lists.  Per pixel the arithmetic is entry_test + bary's,
- * operation for operation; the order in which faces reach a pixel's list
- * does not matter (k_compact sorts each list by key). */
+ * Per pixel the arithmetic is entry_test + bary's, operation for operation;
+ * the order in which faces reach a pixel's list does not matter (k_compact
+ * sorts each list by key).
+ *
+ * Work split: a wave owns 64 faces of a view, or RASTER_FPW when the view's
+ * faces are large (expected area, 2 W H / faces, of RASTER_BIG pixels or
+ * more) or the launch has one view, so that a single view still fills the
+ * GPU.  Lanes set their
+ * face up (visibility, the inside test's face part, pixel bbox); a face
+ * whose bbox has at most RASTER_SMALL pixels is rasterized by its own lane,
+ * a larger one by the whole wave, 64 pixels at a time in 8x8 tiles of the
+ * bbox, so one atomic instruction appends the face to up to 64 pixel lists. */
 #define RASTER_FPW 16u
+#define RASTER_BIG 8u
+#define RASTER_SMALL 4u
+
+struct RasterFace {
+  uint32_t bx0, by0, bx1, by1;
+  float ax, ay, v01x, v01y, v02x, v02y, cx, zA, dB, dC, invD;
+  bool sw;
+};
+
+/* One pixel of a face: the inside test and, inside, the append. */
+DEV void raster_pixel(const RasterFace& f, uint32_t px, uint32_t py, uint32_t W, uint32_t H, uint32_t off,
+                      uint32_t fe, uint32_t* __restrict__ cnt, uint64_t* __restrict__ entries, uint32_t kmax,
+                      unsigned long long& over) {
+  const float ox = pix_to_obj(px, W), oy = pix_to_obj(py, H);
+  const float v0px = ox - f.ax, v0py = oy - f.ay;
+  const float cy = (f.v01x * v0py) - (f.v01y * v0px);
+  const float cz = (v0px * f.v02y) - (v0py * f.v02x);
+  if (!((cy >= 0.0f) && (cz >= 0.0f) && ((cy + cz) <= f.cx))) return;
+  const float wb = f.sw ? cy : cz, wc = f.sw ? cz : cy;
+  const float Zp = f.zA + ((wb * f.dB + wc * f.dC) * f.invD);
+  const uint32_t gp = off + py * W + px;
+  const uint32_t slot = atomicAdd(&cnt[gp], 1u);
+  if (slot < kmax) entries[(size_t)gp * kmax + slot] = ((uint64_t)fkey(Zp) << 32) | fe;
+  else over++;
+}
+
 __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ views, uint32_t v0,
                                                     const uint4* __restrict__ tv,
                                                     const uint32_t* __restrict__ ext_int,
@@ -373,21 +403,22 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
                                                     unsigned long long* __restrict__ stats) {
   const uint32_t vi = v0 + blockIdx.y;
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t e0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * RASTER_FPW;
   const uint32_t nE = views[vi].n_ext;
-  if (e0 >= nE) return; /* the whole wave */
   const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
-  /* per-lane face setup */
+  /* a launch of one view is latency-bound: the most waves; in a batch, 64
+   * faces per wave unless they are large */
+  const bool spread = gridDim.y == 1 || 2ull * W * H >= (uint64_t)RASTER_BIG * nE;
+  const uint32_t fpw = spread ? RASTER_FPW : 64u;
+  const uint32_t e0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * fpw;
+  if (e0 >= nE) return; /* the whole wave */
+  RasterFace f{};
   bool vis = false;
-  uint32_t bx0 = 0, by0 = 0, bx1 = 0, by1 = 0;
-  float ax = 0, ay = 0, v01x = 0, v01y = 0, v02x = 0, v02y = 0, cx = 1, zA = 0, dB = 0, dC = 0, invD = 1;
-  uint32_t sw = 0;
   const uint32_t e = e0 + lane;
-  if (lane < RASTER_FPW && e < nE) {
+  if (lane < fpw && e < nE) {
     float4 fv[4];
-    uint32_t t, f;
+    uint32_t t, fc;
     face_verts(tv + views[vi].rec_off, pv + views[vi].vert_off, ext_int + 2 * (size_t)views[vi].ext_off, e,
-               fv, &t, &f);
+               fv, &t, &fc);
     const Planes pl = face_planes(fv[0], fv[1], fv[2]);
     const float faceZ = fv[3].x * pl.z[0] + fv[3].y * pl.z[1] + pl.z[2];
     double lo0 = 1e30, lo1 = 1e30, hi0 = -1e30, hi1 = -1e30;
@@ -399,58 +430,53 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
       lo1 = fmin(lo1, fy); hi1 = fmax(hi1, fy);
     }
     vis = !(faceZ < fv[3].z) && !(hi0 < 0.0 || hi1 < 0.0 || lo0 > W - 1.0 || lo1 > H - 1.0);
-    bx0 = lo0 < 0.0 ? 0u : (uint32_t)lo0;
-    by0 = lo1 < 0.0 ? 0u : (uint32_t)lo1;
-    bx1 = hi0 > W - 1.0 ? W - 1u : (uint32_t)hi0;
-    by1 = hi1 > H - 1.0 ? H - 1u : (uint32_t)hi1;
+    f.bx0 = lo0 < 0.0 ? 0u : (uint32_t)lo0;
+    f.by0 = lo1 < 0.0 ? 0u : (uint32_t)lo1;
+    f.bx1 = hi0 > W - 1.0 ? W - 1u : (uint32_t)hi0;
+    f.by1 = hi1 > H - 1.0 ? H - 1u : (uint32_t)hi1;
     /* entry_test's face part: edge vectors from vertex 0, their cross
      * product (degenerate -> 1e-12), swapped to a positive orientation */
-    ax = fv[0].x; ay = fv[0].y;
-    v01x = fv[1].x - fv[0].x; v01y = fv[1].y - fv[0].y;
-    v02x = fv[2].x - fv[0].x; v02y = fv[2].y - fv[0].y;
-    cx = (v01x * v02y) - (v01y * v02x);
-    cx = tiny(cx) ? (float)1e-12 : cx;
-    if (cx < 0.0f) {
-      const float tx = v01x, ty = v01y;
-      v01x = v02x; v01y = v02y; v02x = tx; v02y = ty;
-      cx = -cx; sw = 1u;
+    f.ax = fv[0].x; f.ay = fv[0].y;
+    f.v01x = fv[1].x - fv[0].x; f.v01y = fv[1].y - fv[0].y;
+    f.v02x = fv[2].x - fv[0].x; f.v02y = fv[2].y - fv[0].y;
+    f.cx = (f.v01x * f.v02y) - (f.v01y * f.v02x);
+    f.cx = tiny(f.cx) ? (float)1e-12 : f.cx;
+    f.sw = false;
+    if (f.cx < 0.0f) {
+      const float tx = f.v01x, ty = f.v01y;
+      f.v01x = f.v02x; f.v01y = f.v02y; f.v02x = tx; f.v02y = ty;
+      f.cx = -f.cx; f.sw = true;
     }
     /* bary's face part */
-    zA = fv[0].z; dB = fv[1].z - fv[0].z; dC = fv[2].z - fv[0].z;
-    invD = 1.0f / cx;
+    f.zA = fv[0].z; f.dB = fv[1].z - fv[0].z; f.dC = fv[2].z - fv[0].z;
+    f.invD = 1.0f / f.cx;
   }
+  const uint32_t area = vis ? (f.bx1 - f.bx0 + 1u) * (f.by1 - f.by0 + 1u) : 0u;
   unsigned long long over = 0;
-  uint64_t todo = __ballot(vis);
+  /* small faces: the lane's own loop (x outer, y inner over the bbox) */
+  if (vis && area <= RASTER_SMALL)
+    for (uint32_t px = f.bx0; px <= f.bx1; ++px)
+      for (uint32_t py = f.by0; py <= f.by1; ++py) raster_pixel(f, px, py, W, H, off, e, cnt, entries, kmax, over);
+  /* larger faces: the whole wave, one face at a time */
+  uint64_t todo = __ballot(vis && area > RASTER_SMALL);
   while (todo) {
     const int j = __ffsll((unsigned long long)todo) - 1;
     todo &= todo - 1ull;
-    /* face j's set-up, uniform across the wave */
-    const uint32_t fx0 = __shfl(bx0, j, 64), fy0 = __shfl(by0, j, 64);
-    const uint32_t fx1 = __shfl(bx1, j, 64), fy1 = __shfl(by1, j, 64);
-    const float fax = __shfl(ax, j, 64), fay = __shfl(ay, j, 64);
-    const float f01x = __shfl(v01x, j, 64), f01y = __shfl(v01y, j, 64);
-    const float f02x = __shfl(v02x, j, 64), f02y = __shfl(v02y, j, 64);
-    const float fcx = __shfl(cx, j, 64), fzA = __shfl(zA, j, 64), fdB = __shfl(dB, j, 64);
-    const float fdC = __shfl(dC, j, 64), finv = __shfl(invD, j, 64);
-    const bool fsw = __shfl(sw, j, 64) != 0u;
-    const uint32_t fe = e0 + (uint32_t)j;
-    for (uint32_t ty = fy0; ty <= fy1; ty += 8u) {
-      for (uint32_t tx = fx0; tx <= fx1; tx += 8u) {
+    RasterFace g;
+    g.bx0 = __shfl(f.bx0, j, 64); g.by0 = __shfl(f.by0, j, 64);
+    g.bx1 = __shfl(f.bx1, j, 64); g.by1 = __shfl(f.by1, j, 64);
+    g.ax = __shfl(f.ax, j, 64); g.ay = __shfl(f.ay, j, 64);
+    g.v01x = __shfl(f.v01x, j, 64); g.v01y = __shfl(f.v01y, j, 64);
+    g.v02x = __shfl(f.v02x, j, 64); g.v02y = __shfl(f.v02y, j, 64);
+    g.cx = __shfl(f.cx, j, 64); g.zA = __shfl(f.zA, j, 64); g.dB = __shfl(f.dB, j, 64);
+    g.dC = __shfl(f.dC, j, 64); g.invD = __shfl(f.invD, j, 64);
+    g.sw = __shfl((uint32_t)f.sw, j, 64) != 0u;
+    const uint32_t ge = e0 + (uint32_t)j;
+    for (uint32_t ty = g.by0; ty <= g.by1; ty += 8u)
+      for (uint32_t tx = g.bx0; tx <= g.bx1; tx += 8u) {
         const uint32_t px = tx + (lane & 7u), py = ty + (lane >> 3);
-        if (px > fx1 || py > fy1) continue;
-        const float ox = pix_to_obj(px, W), oy = pix_to_obj(py, H);
-        const float v0px = ox - fax, v0py = oy - fay;
-        const float cy = (f01x * v0py) - (f01y * v0px);
-        const float cz = (v0px * f02y) - (v0py * f02x);
-        if (!((cy >= 0.0f) && (cz >= 0.0f) && ((cy + cz) <= fcx))) continue;
-        const float wb = fsw ? cy : cz, wc = fsw ? cz : cy;
-        const float Zp = fzA + ((wb * fdB + wc * fdC) * finv);
-        const uint32_t gp = off + py * W + px;
-        const uint32_t slot = atomicAdd(&cnt[gp], 1u);
-        if (slot < kmax) entries[(size_t)gp * kmax + slot] = ((uint64_t)fkey(Zp) << 32) | fe;
-        else over++;
+        if (px <= g.bx1 && py <= g.by1) raster_pixel(g, px, py, W, H, off, ge, cnt, entries, kmax, over);
       }
-    }
   }
   if (kmax) {
     over = wave_sum_u64(over);
@@ -462,7 +488,7 @@ int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t ma
                      const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
                      uint32_t kmax, unsigned long long* stats, void* stream) {
   if (nv == 0 || max_faces == 0) return VFR_OK;
-  const uint32_t per_block = 4u * RASTER_FPW; /* 256 threads: four waves */
+  const uint32_t per_block = 4u * RASTER_FPW; /* 256 threads: four waves at the smaller share */
   hipLaunchKernelGGL(k_raster_ftb, dim3((max_faces + per_block - 1) / per_block, nv), dim3(256), 0,
                      (hipStream_t)stream, views, v0, tv, ext_int, pv, cnt, entries, kmax, stats);
   CHECK_HIP(hipGetLastError());
