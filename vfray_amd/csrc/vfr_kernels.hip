@@ -302,7 +302,7 @@ DEV void face_verts(const uint4* tv, const float4* pv, const uint32_t* ext_int, 
   *face = f;
 }
 
-/* REF raster: one thread per (view, external face).  Visible faces write
+/* REF raster over (view, external face).  Visible faces write
  * atomicMax(owner, ext+1) into every pixel the reference's loop would write:
  * x outer, y inner over the inclusive bbox, inside test, and the column ends
  * at the first inside pixel whose entry scalar is outside [0,1]
@@ -313,10 +313,21 @@ __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ 
                                                     const uint32_t* __restrict__ ext_int,
                                                     const float4* __restrict__ pv,
                                                     uint32_t* __restrict__ owner) {
+  /* Column-parallel: a wave takes 64 / cpf faces, cpf lanes per face, each
+   * lane the bbox columns c, c + cpf, ... of its face, every column walked
+   * top-down with the reference's stop.  Columns are independent and the
+   * owner is a maximum, so the pixels written are exactly those of the
+   * reference's loop.  cpf (4..64) follows the view's expected face width,
+   * sqrt(2 W H / faces). */
   const uint32_t vi = v0 + blockIdx.y;
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= views[vi].n_ext) return;
+  const uint32_t nE = views[vi].n_ext;
   const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
+  const float wexp = sqrtf(2.0f * (float)W * (float)H / (float)(nE ? nE : 1u));
+  uint32_t cpf = 4;
+  while (cpf < 64u && (float)cpf < wexp) cpf <<= 1;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t e = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64u / cpf) + lane / cpf;
+  if (e >= nE) return;
   float4 fv[4];
   uint32_t t, f;
   face_verts(tv + views[vi].rec_off, pv + views[vi].vert_off, ext_int + 2 * (size_t)views[vi].ext_off, e,
@@ -331,7 +342,7 @@ __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ 
     uint32_t ix = img_coord_clamped(fv[k].x, W), iy = img_coord_clamped(fv[k].y, H);
     bx0 = min(bx0, ix); by0 = min(by0, iy); bx1 = max(bx1, ix); by1 = max(by1, iy);
   }
-  for (uint32_t px = bx0; px <= bx1; ++px) {
+  for (uint32_t px = bx0 + lane % cpf; px <= bx1; px += cpf) {
     float ox = pix_to_obj(px, W);
     for (uint32_t py = by0; py <= by1; ++py) {
       float oy = pix_to_obj(py, H);
@@ -347,7 +358,9 @@ __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ 
 int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* tv,
                      const uint32_t* ext_int, const float4* pv, uint32_t* owner, void* stream) {
   if (nv == 0 || max_faces == 0) return VFR_OK;
-  hipLaunchKernelGGL(k_raster_ref, dim3((max_faces + 255) / 256, nv), dim3(256), 0, (hipStream_t)stream, views, v0,
+  /* sized for one face per wave, the widest split; waves past a view's
+   * faces return at once */
+  hipLaunchKernelGGL(k_raster_ref, dim3((max_faces + 3) / 4, nv), dim3(256), 0, (hipStream_t)stream, views, v0,
                      tv, ext_int, pv, owner);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
