@@ -230,7 +230,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the renderer is "
-              "single-GPU; views would shard across GPUs, see DESIGN.md", file=sys.stderr)
+              "single-GPU; views would shard across GPUs, one mesh replica each", file=sys.stderr)
         return 2
 
     import vfray_amd as V

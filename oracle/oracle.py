@@ -1,6 +1,7 @@
 """ctypes wrapper of the CPU oracle (liboracle.so) and the compiled reference
 CPU path (oracle/_ref).  TEST INFRASTRUCTURE: imported only by the tests, the
-smoke check and the benchmark's CPU-baseline leg, always as the checker.
+smoke check and the benchmark's CPU-baseline leg, only to check or time the
+renderer against it.
 """
 from __future__ import annotations
 

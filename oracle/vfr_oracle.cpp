@@ -2,8 +2,8 @@
  * vfr_oracle.cpp -- CPU oracle for the VF-Ray hot path.  TEST INFRASTRUCTURE.
  *
  * Only the test suite, the smoke check and the benchmark's CPU-baseline leg
- * load this library, and only as the checker: the product path (libvfr.so)
- * never links, loads or calls it.
+ * load this library, and only to check or time the renderer against it: the
+ * product path (libvfr.so) never links, loads or calls it.
  *
  * What it restates (file:line in the VF-Ray sources):
  *   orc_normalize        normalizeVertices        vfRayPreComp.cc:84-145

@@ -1,6 +1,5 @@
 """Builds libvfr.so (HIP kernels for gfx950 + host C++), the vfRay CLI, and the
-CPU oracle.  Run as ``python -m vfray_amd.build`` or through
-``__graft_entry__.build()``; every call recompiles.
+CPU oracle.  Run as ``python -m vfray_amd.build``; every call recompiles.
 
 Compiler flags that matter for results:
   -ffp-contract=off   no FMA contraction, so every float operation that decides

@@ -1076,7 +1076,7 @@ enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
  * result is ignored).  Its loads go out as soon as the exit face is known:
  * the transfer-function colour at the exit point, then the next tet's
  * fourth vertex and record, which the next step waits for; psi is evaluated
- * in registers meanwhile (with PSIT it is a second round of loads, the
+ * in registers meanwhile (with PSIT it is a second group of loads, the
  * table entry, waited for within the step). */
 /* PROJ: VFR_TRAV_*, the projection of every ray of the launch.  BIG: the
  * record arena reaches 4 GiB (VFR_BIG_RECORDS tets or more), so records are
@@ -1153,7 +1153,7 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
   /* neighbour words index the record arena (a server rebases each resident
    * mesh's words when it copies them in) */
   const uint32_t tn = nb < VFR_NB_BROKEN ? (nb >> 2) : r.t;
-  /* round 1: the colour at the exit point and the next tet (the current one
+  /* first loads: the colour at the exit point and the next tet (the current one
    * again at a boundary, unused) */
   const uint32_t ci = r.tfo + (uint32_t)(255.0f * snext); /* snext is in [0, 1] */
   const float4 cN = tf_at<TFL>(a, tfl, ci);
@@ -1167,7 +1167,7 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
     fbn = ld32<uint4>(a.rec, tn * 32u + 16u);
   }
   __builtin_amdgcn_sched_barrier(0);
-  /* round 2: psi.  A skipped step composites nothing: with dl = 0 both
+  /* then psi.  A skipped step composites nothing: with dl = 0 both
    * depths are 0, psi = zeta = 1 and the update leaves colour and
    * transparency unchanged (in linear compositing, both terms vanish). */
   const float dl = skip ? 0.0f : fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
