@@ -423,7 +423,7 @@ static int plan_count_entries(vfr_plan* p, uint32_t* out) {
   CHECK_RC(vfr_k_clear_u32(p->counters, 1, st));
   CHECK_RC(vfr_k_xform_views(p->views, 0, p->n_views, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
   CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner,
-                            nullptr, 0, p->stats, st));
+                            nullptr, 0, 0, p->stats, st));
   CHECK_RC(vfr_k_max_u32(p->owner, p->n_pix, p->counters, st));
   CHECK_HIP(hipMemcpyAsync(out, p->counters, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   CHECK_HIP(hipStreamSynchronize(st));
@@ -441,9 +441,9 @@ static int enqueue_batched(vfr_plan* p) {
     CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner, st));
   else
     CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner,
-                              p->entries, p->kmax, p->stats, st));
+                              p->entries, p->kmax, p->n_pix, p->stats, st));
   CHECK_RC(vfr_k_compact(p->views, p->n_views, 0, p->n_tiles, p->opts.mode, p->owner, p->owner,
-                         p->entries, p->kmax, p->rgba, p->steps, p->entry, p->rays, p->counters + p->n_views, st));
+                         p->entries, p->kmax, p->n_pix, p->rgba, p->steps, p->entry, p->rays, p->counters + p->n_views, st));
   return VFR_OK;
 }
 
@@ -469,6 +469,7 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   a.owner = p->owner;
   a.cnt = p->owner;
   a.entries = p->entries;
+  a.estride = p->n_pix;
   a.kmax = p->kmax;
   a.rgba = p->rgba;
   a.steps = p->steps;
@@ -725,9 +726,9 @@ static int execute_pipelined(vfr_plan* p, double* ms_total, double* ms_vis, doub
     const vfr_plan::Chunk& ch = p->chunks[c];
     CHECK_RC(vfr_k_xform_views(p->views, ch.v0, ch.nv, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
     CHECK_RC(vfr_k_raster_ftb(p->views, ch.v0, ch.nv, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner,
-                              p->entries, p->kmax, p->stats, st));
+                              p->entries, p->kmax, p->n_pix, p->stats, st));
     CHECK_RC(vfr_k_compact(p->views, p->n_views, ch.tile0, ch.ntiles, p->opts.mode, p->owner, p->owner,
-                           p->entries, p->kmax, p->rgba, p->steps, p->entry, p->rays + ch.pix0, ccount + c, st));
+                           p->entries, p->kmax, p->n_pix, p->rgba, p->steps, p->entry, p->rays + ch.pix0, ccount + c, st));
     CHECK_HIP(hipEventRecord(p->cev[3 * c], st));
     CHECK_HIP(hipStreamWaitEvent(s2, p->cev[3 * c], 0));
     CHECK_HIP(hipEventRecord(p->cev[3 * c + 1], s2));
@@ -806,9 +807,9 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
         CHECK_RC(vfr_k_raster_ref(p->views, v, 1, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner, st));
       else
         CHECK_RC(vfr_k_raster_ftb(p->views, v, 1, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner,
-                                  p->entries, p->kmax, p->stats, st));
+                                  p->entries, p->kmax, p->n_pix, p->stats, st));
       CHECK_RC(vfr_k_compact(p->views, p->n_views, hv.tile_off, ntl, p->opts.mode, p->owner, p->owner,
-                             p->entries, p->kmax, p->rgba, p->steps, p->entry, p->rays + hv.pix_off,
+                             p->entries, p->kmax, p->n_pix, p->rgba, p->steps, p->entry, p->rays + hv.pix_off,
                              p->counters + v, st));
       CHECK_HIP(hipEventRecord(p->ev[1], st));
       TravArgs a = trav_args(p, p->rays + hv.pix_off, p->counters + v);

@@ -131,7 +131,8 @@ struct TravArgs {
   const uint32_t* n_rays;  /* device count of rays */
   const uint32_t* owner;   /* REF: ext index + 1, 0 = none */
   const uint32_t* cnt;     /* FTB: entries per pixel */
-  const uint64_t* entries; /* FTB: [pixels * kmax] keys (z << 32 | ext) */
+  const uint64_t* entries; /* FTB: keys (z << 32 | ext), slot-major: slot k of
+                              pixel g at entries[k * estride + g] */
   float4* rgba;
   uint32_t* steps;
   uint32_t* entry_out;
@@ -142,6 +143,7 @@ struct TravArgs {
   uint32_t tf_lds_base;    /* one-ray-per-thread FTB: TF slots [base, base+n)   */
   uint32_t tf_lds_n;       /* staged in LDS (0: read the global table)         */
   uint32_t kmax;           /* FTB: entry slots per pixel (8, 16 or 32)         */
+  uint64_t estride;        /* FTB: pixels per entry slot plane                 */
   uint32_t composite;      /* VFR_COMPOSITE_* (both walks)                     */
   uint32_t psi_table;      /* FTB: psi from the table (VFR_RENDER_PSI_TABLE)     */
 };
@@ -159,12 +161,12 @@ int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t ma
 /* kmax: entry slots per pixel; 0 only counts (cnt) without storing keys. */
 int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* tv,
                      const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
-                     uint32_t kmax, unsigned long long* stats, void* stream);
+                     uint32_t kmax, uint64_t estride, unsigned long long* stats, void* stream);
 /* *out = max(*out, cnt[i]) over n pixels (device counter). */
 int vfr_k_max_u32(const uint32_t* cnt, size_t n, uint32_t* out, void* stream);
 int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, uint32_t ntiles,
                   uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries, uint32_t kmax,
-                  float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
+                  uint64_t estride, float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
                   uint32_t* n_rays, void* stream);
 /* occupancy: waves per SIMD of the persistent kernel variant (4, 5, 6:
  * register-capped variants; anything else: as compiled). */

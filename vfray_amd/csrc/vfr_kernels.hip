@@ -393,7 +393,7 @@ struct RasterFace {
 /* One pixel of a face: the inside test and, inside, the append. */
 DEV void raster_pixel(const RasterFace& f, uint32_t px, uint32_t py, uint32_t W, uint32_t H, uint32_t off,
                       uint32_t fe, uint32_t* __restrict__ cnt, uint64_t* __restrict__ entries, uint32_t kmax,
-                      unsigned long long& over) {
+                      uint64_t estride, unsigned long long& over) {
   const float ox = pix_to_obj(px, W), oy = pix_to_obj(py, H);
   const float v0px = ox - f.ax, v0py = oy - f.ay;
   const float cy = (f.v01x * v0py) - (f.v01y * v0px);
@@ -403,7 +403,7 @@ DEV void raster_pixel(const RasterFace& f, uint32_t px, uint32_t py, uint32_t W,
   const float Zp = f.zA + ((wb * f.dB + wc * f.dC) * f.invD);
   const uint32_t gp = off + py * W + px;
   const uint32_t slot = atomicAdd(&cnt[gp], 1u);
-  if (slot < kmax) entries[(size_t)gp * kmax + slot] = ((uint64_t)fkey(Zp) << 32) | fe;
+  if (slot < kmax) entries[slot * estride + gp] = ((uint64_t)fkey(Zp) << 32) | fe;
   else over++;
 }
 
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
                                                     const float4* __restrict__ pv,
                                                     uint32_t* __restrict__ cnt,
                                                     uint64_t* __restrict__ entries, uint32_t kmax,
-                                                    unsigned long long* __restrict__ stats) {
+                                                    uint64_t estride, unsigned long long* __restrict__ stats) {
   const uint32_t vi = v0 + blockIdx.y;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t nE = views[vi].n_ext;
@@ -469,7 +469,7 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
   /* small faces: the lane's own loop (x outer, y inner over the bbox) */
   if (vis && area <= RASTER_SMALL)
     for (uint32_t px = f.bx0; px <= f.bx1; ++px)
-      for (uint32_t py = f.by0; py <= f.by1; ++py) raster_pixel(f, px, py, W, H, off, e, cnt, entries, kmax, over);
+      for (uint32_t py = f.by0; py <= f.by1; ++py) raster_pixel(f, px, py, W, H, off, e, cnt, entries, kmax, estride, over);
   /* larger faces: the whole wave, one face at a time */
   uint64_t todo = __ballot(vis && area > RASTER_SMALL);
   while (todo) {
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
     for (uint32_t ty = g.by0; ty <= g.by1; ty += 8u)
       for (uint32_t tx = g.bx0; tx <= g.bx1; tx += 8u) {
         const uint32_t px = tx + (lane & 7u), py = ty + (lane >> 3);
-        if (px <= g.bx1 && py <= g.by1) raster_pixel(g, px, py, W, H, off, ge, cnt, entries, kmax, over);
+        if (px <= g.bx1 && py <= g.by1) raster_pixel(g, px, py, W, H, off, ge, cnt, entries, kmax, estride, over);
       }
   }
   if (kmax) {
@@ -499,11 +499,11 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
 
 int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* tv,
                      const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
-                     uint32_t kmax, unsigned long long* stats, void* stream) {
+                     uint32_t kmax, uint64_t estride, unsigned long long* stats, void* stream) {
   if (nv == 0 || max_faces == 0) return VFR_OK;
   const uint32_t per_block = 4u * RASTER_FPW; /* 256 threads: four waves at the smaller share */
   hipLaunchKernelGGL(k_raster_ftb, dim3((max_faces + per_block - 1) / per_block, nv), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, tv, ext_int, pv, cnt, entries, kmax, stats);
+                     (hipStream_t)stream, views, v0, tv, ext_int, pv, cnt, entries, kmax, estride, stats);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -529,6 +529,7 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
                                                   uint32_t mode, const uint32_t* __restrict__ owner,
                                                   uint32_t* __restrict__ cnt,
                                                   uint64_t* __restrict__ entries, uint32_t kmax,
+                                                  uint64_t estride,
                                                   float4* __restrict__ rgba, uint32_t* __restrict__ steps,
                                                   uint32_t* __restrict__ entry_out,
                                                   uint64_t* __restrict__ rays,
@@ -560,24 +561,25 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
       } else {
         uint32_t nk = cnt[gp];
         active = nk != 0u;
-        uint64_t* ent = entries + (size_t)gp * kmax;
+        /* the pixel's slots, one per slot plane */
+        uint64_t* ent = entries + gp;
         nk = nk > kmax ? kmax : nk;
         if (nk > 1u && nk <= 8u) {
           uint64_t e[8];
 #pragma unroll
-          for (uint32_t k = 0; k < 8; ++k) e[k] = k < nk ? ent[k] : 0ull;
+          for (uint32_t k = 0; k < 8; ++k) e[k] = k < nk ? ent[k * estride] : 0ull;
           CS(0, 1) CS(2, 3) CS(4, 5) CS(6, 7) CS(0, 2) CS(1, 3) CS(4, 6) CS(5, 7) CS(1, 2) CS(5, 6)
           CS(0, 4) CS(1, 5) CS(2, 6) CS(3, 7) CS(2, 4) CS(3, 5) CS(1, 2) CS(3, 4) CS(5, 6)
 #pragma unroll
           for (uint32_t k = 0; k < 8; ++k)
-            if (k < nk) ent[k] = e[k];
+            if (k < nk) ent[k * estride] = e[k];
         } else if (nk > 8u) {
           /* rare long list: insertion sort in place, descending */
           for (uint32_t k = 1; k < nk; ++k) {
-            const uint64_t key = ent[k];
+            const uint64_t key = ent[k * estride];
             uint32_t j = k;
-            while (j > 0 && ent[j - 1] < key) { ent[j] = ent[j - 1]; --j; }
-            ent[j] = key;
+            while (j > 0 && ent[(j - 1) * estride] < key) { ent[j * estride] = ent[(j - 1) * estride]; --j; }
+            ent[j * estride] = key;
           }
         }
       }
@@ -607,12 +609,12 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
 
 int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, uint32_t ntiles,
                   uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries, uint32_t kmax,
-                  float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
+                  uint64_t estride, float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
                   uint32_t* n_rays, void* stream) {
   if (ntiles == 0) return VFR_OK;
   hipLaunchKernelGGL(k_compact, dim3((ntiles + COMPACT_WAVES - 1) / COMPACT_WAVES), dim3(1024), 0,
                      (hipStream_t)stream, views, n_views_total, tile0, ntiles, mode, owner, cnt, entries,
-                     kmax, rgba, steps, entry_out, rays, n_rays);
+                     kmax, estride, rgba, steps, entry_out, rays, n_rays);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -1006,7 +1008,7 @@ DEV float4 tf_at(const TravArgs& a, const float4* tfl, uint32_t i) {
  * entry face is degenerate at the pixel. */
 template <bool TFL>
 DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
-  const uint64_t key = a.entries[(size_t)r.gp * a.kmax + ftb_cur(r)];
+  const uint64_t key = a.entries[ftb_cur(r) * a.estride + r.gp];
   const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull) + a.views[r.vi].ext_off;
   const float4* __restrict__ pv = a.vert + r.voff;
   r.t = a.views[r.vi].rec_off + a.ext_int[2 * (size_t)e];
@@ -1247,7 +1249,7 @@ DEV bool ftb_reenter(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const float exitZ = ftb_persp(r) ? 1.0f / r.prev : r.prev;
   uint32_t cur = ftb_cur(r) + 1u;
   const uint32_t nk = ftb_nk(r);
-  while (cur < nk && !(funkey((uint32_t)(a.entries[(size_t)r.gp * a.kmax + cur] >> 32)) < exitZ)) ++cur;
+  while (cur < nk && !(funkey((uint32_t)(a.entries[cur * a.estride + r.gp] >> 32)) < exitZ)) ++cur;
   if (cur >= nk) return false;
   r.bits = (r.bits & ~63u) | cur;
   if (!ftb_segment<TFL>(a, tfl, r)) { r.bits |= FTB_DEAD; return false; }
@@ -1271,7 +1273,7 @@ DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
     o.c = make_float4(r.ar, r.ag, r.ab, 1.0f - r.Tr);
   }
   o.steps = r.steps > VFR_MAX_STEPS ? VFR_MAX_STEPS : r.steps;
-  o.entry = (uint32_t)(a.entries[(size_t)r.gp * a.kmax] & 0xFFFFFFFFull);
+  o.entry = (uint32_t)(a.entries[r.gp] & 0xFFFFFFFFull);
   o.anomaly = (r.bits & FTB_DEAD) ? 1u : 0u;
   o.broken = (r.bits & FTB_BROKEN) ? 1u : 0u;
   return o;

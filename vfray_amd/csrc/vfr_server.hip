@@ -191,9 +191,9 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   CHECK_RC(vfr_k_clear_u32(s->counters, 4, st));
   CHECK_RC(vfr_k_clear_u32((uint32_t*)s->stats, 2 * ST_N, st));
   CHECK_RC(vfr_k_xform_views(s->views, 0, nv, max_verts, sv->vert, s->pverts, st));
-  CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, max_faces, sv->tv, sv->ext, s->pverts, s->cnt, s->entries, VFR_K,
+  CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, max_faces, sv->tv, sv->ext, s->pverts, s->cnt, s->entries, VFR_K, s->cap_pix,
                             s->stats, st));
-  CHECK_RC(vfr_k_compact(s->views, nv, 0, (uint32_t)tiles, VFR_MODE_FTB, nullptr, s->cnt, s->entries, VFR_K,
+  CHECK_RC(vfr_k_compact(s->views, nv, 0, (uint32_t)tiles, VFR_MODE_FTB, nullptr, s->cnt, s->entries, VFR_K, s->cap_pix,
                          s->rgba, s->steps, s->entry, s->rays, s->counters, st));
   TravArgs a;
   memset(&a, 0, sizeof(a));
@@ -208,6 +208,7 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   a.n_rays = s->counters;
   a.cnt = s->cnt;
   a.entries = s->entries;
+  a.estride = s->cap_pix;
   a.rgba = s->rgba;
   a.steps = s->steps;
   a.entry_out = s->entry;
