@@ -1011,8 +1011,9 @@ DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const uint64_t key = a.entries[ftb_cur(r) * a.estride + r.gp];
   const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull) + a.views[r.vi].ext_off;
   const float4* __restrict__ pv = a.vert + r.voff;
-  r.t = a.views[r.vi].rec_off + a.ext_int[2 * (size_t)e];
-  const uint32_t fe = a.ext_int[2 * (size_t)e + 1];
+  const uint2 tf2 = reinterpret_cast<const uint2*>(a.ext_int)[e]; /* (tet, face) */
+  r.t = a.views[r.vi].rec_off + tf2.x;
+  const uint32_t fe = tf2.y;
   const uint4 rv = a.tv[r.t];
   r.fa = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[0]);
   r.fb = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[4]);
