@@ -422,6 +422,11 @@ def test_ftb_concave_c4_views(ctx, psi):
     with pytest.raises(V.VfrError, match="dropped"):
         ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, max_entries=8)
     outs, st = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+    # 32 slot planes instead of the 16 chosen: the same lists, the same images
+    outs32, _ = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, max_entries=32)
+    for g, h in zip(outs, outs32):
+        for key in ("rgba", "entry_ext", "steps"):
+            assert np.array_equal(g[key], h[key]), key
     for v, g in zip(views, outs):
         orc = oracle_ftb(inp, v, psi, fp64=True)
         assert orc["anomalies"] == 0
