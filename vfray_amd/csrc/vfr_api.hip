@@ -50,6 +50,7 @@ struct vfr_mesh {
   float bb_lo[3] = {0, 0, 0}, bb_hi[3] = {0, 0, 0};
   uint32_t version = 0;         /* bumped by vfr_mesh_rotate */
   const uint4* tv4() const { return reinterpret_cast<const uint4*>(d.tv); }
+  const uint4* fvid4() const { return reinterpret_cast<const uint4*>(d.fvid); }
 };
 
 extern "C" const char* vfr_version(void) { return "vfr 0.1 gfx950"; }
@@ -227,7 +228,8 @@ extern "C" int vfr_mesh_create(vfr_ctx* c, const float* xyzs, uint32_t nV, const
 extern "C" void vfr_mesh_destroy(vfr_mesh* m) {
   if (!m) return;
   if (m->ctx) (void)hipSetDevice(m->ctx->device);
-  void* ps[] = {m->d.rec, m->d.tv, m->d.vert, m->d.ext_int, m->d.extf, m->d.perm, m->d.vperm, m->d.con};
+  void* ps[] = {m->d.rec, m->d.tv, m->d.vert, m->d.ext_int, m->d.fvid, m->d.extf, m->d.perm, m->d.vperm,
+                m->d.con};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete m;
@@ -422,7 +424,7 @@ static int plan_count_entries(vfr_plan* p, uint32_t* out) {
   CHECK_RC(vfr_k_clear_u32(p->owner, p->n_pix, st));
   CHECK_RC(vfr_k_clear_u32(p->counters, 1, st));
   CHECK_RC(vfr_k_xform_views(p->views, 0, p->n_views, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
-  CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner,
+  CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d.n_ext, m->fvid4(), p->pverts, p->owner,
                             nullptr, 0, 0, p->stats, st));
   CHECK_RC(vfr_k_max_u32(p->owner, p->n_pix, p->counters, st));
   CHECK_HIP(hipMemcpyAsync(out, p->counters, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -438,9 +440,9 @@ static int enqueue_batched(vfr_plan* p) {
   CHECK_RC(vfr_k_clear_u32((uint32_t*)p->stats, 2 * ST_N, st));
   CHECK_RC(vfr_k_xform_views(p->views, 0, p->n_views, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
   if (p->opts.mode == VFR_MODE_REF)
-    CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner, st));
+    CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, m->d.n_ext, m->fvid4(), p->pverts, p->owner, st));
   else
-    CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner,
+    CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d.n_ext, m->fvid4(), p->pverts, p->owner,
                               p->entries, p->kmax, p->n_pix, p->stats, st));
   CHECK_RC(vfr_k_compact(p->views, p->n_views, 0, p->n_tiles, p->opts.mode, p->owner, p->owner,
                          p->entries, p->kmax, p->n_pix, p->rgba, p->steps, p->entry, p->rays, p->counters + p->n_views, st));
@@ -725,7 +727,7 @@ static int execute_pipelined(vfr_plan* p, double* ms_total, double* ms_vis, doub
   for (size_t c = 0; c < nc; ++c) {
     const vfr_plan::Chunk& ch = p->chunks[c];
     CHECK_RC(vfr_k_xform_views(p->views, ch.v0, ch.nv, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
-    CHECK_RC(vfr_k_raster_ftb(p->views, ch.v0, ch.nv, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner,
+    CHECK_RC(vfr_k_raster_ftb(p->views, ch.v0, ch.nv, m->d.n_ext, m->fvid4(), p->pverts, p->owner,
                               p->entries, p->kmax, p->n_pix, p->stats, st));
     CHECK_RC(vfr_k_compact(p->views, p->n_views, ch.tile0, ch.ntiles, p->opts.mode, p->owner, p->owner,
                            p->entries, p->kmax, p->n_pix, p->rgba, p->steps, p->entry, p->rays + ch.pix0, ccount + c, st));
@@ -804,9 +806,9 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
       CHECK_RC(vfr_k_clear_u32(p->owner + hv.pix_off, npx, st));
       CHECK_RC(vfr_k_xform_views(p->views, v, 1, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
       if (p->opts.mode == VFR_MODE_REF)
-        CHECK_RC(vfr_k_raster_ref(p->views, v, 1, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner, st));
+        CHECK_RC(vfr_k_raster_ref(p->views, v, 1, m->d.n_ext, m->fvid4(), p->pverts, p->owner, st));
       else
-        CHECK_RC(vfr_k_raster_ftb(p->views, v, 1, m->d.n_ext, m->tv4(), m->d.ext_int, p->pverts, p->owner,
+        CHECK_RC(vfr_k_raster_ftb(p->views, v, 1, m->d.n_ext, m->fvid4(), p->pverts, p->owner,
                                   p->entries, p->kmax, p->n_pix, p->stats, st));
       CHECK_RC(vfr_k_compact(p->views, p->n_views, hv.tile_off, ntl, p->opts.mode, p->owner, p->owner,
                              p->entries, p->kmax, p->n_pix, p->rgba, p->steps, p->entry, p->rays + hv.pix_off,

@@ -102,6 +102,8 @@ struct DevMesh {
   uint32_t* tv = nullptr;      /* [n_tets*4] vertex ids (internal), same order */
   float* vert = nullptr;       /* [n_verts*4] internal order (x,y,z,s)      */
   uint32_t* ext_int = nullptr; /* [n_ext*2] (internal tet, face)            */
+  uint32_t* fvid = nullptr;    /* [n_ext*4] internal vertex ids of each external
+                                  face's slots f, f+1, f+2 and the one off it */
   uint32_t* extf = nullptr;    /* [n_ext*2] (caller tet, face), .extf order */
   uint32_t* perm = nullptr;    /* [n_tets] internal -> caller tet id        */
   uint32_t* vperm = nullptr;   /* [n_verts] internal -> caller vertex id    */
@@ -156,11 +158,11 @@ int vfr_k_copy(const void* src, void* dst, size_t bytes, void* stream);
  * or external faces any of those views' meshes has) x nv. */
 int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_verts,
                       const float4* wvert, float4* pv, void* stream);
-int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* tv,
-                     const uint32_t* ext_int, const float4* pv, uint32_t* owner, void* stream);
+int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* fvid,
+                     const float4* pv, uint32_t* owner, void* stream);
 /* kmax: entry slots per pixel; 0 only counts (cnt) without storing keys. */
-int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* tv,
-                     const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
+int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* fvid,
+                     const float4* pv, uint32_t* cnt, uint64_t* entries,
                      uint32_t kmax, uint64_t estride, unsigned long long* stats, void* stream);
 /* *out = max(*out, cnt[i]) over n pixels (device counter). */
 int vfr_k_max_u32(const uint32_t* cnt, size_t n, uint32_t* out, void* stream);

@@ -292,14 +292,14 @@ int vfr_k_copy(const void* src, void* dst, size_t bytes, void* stream) {
 
 /* Loads the ext face (internal tet, face) vertices in P space: slots
  * (f+0..3)&3, the last one the opposite vertex. */
-DEV void face_verts(const uint4* tv, const float4* pv, const uint32_t* ext_int, uint32_t e,
-                    float4 out[4], uint32_t* tet, uint32_t* face) {
-  uint32_t t = ext_int[e * 2], f = ext_int[e * 2 + 1];
-  uint4 rv = tv[t];
-#pragma unroll
-  for (uint32_t k = 0; k < 4; ++k) out[k] = pv[sel4(rv, (f + k) & 3u)];
-  *tet = t;
-  *face = f;
+/* The four P-space vertices of external face e: slots f, f+1, f+2 and the
+ * one off the face, from the face's vertex ids (fvid, built with the mesh). */
+DEV void face_verts(const uint4* fvid, const float4* pv, uint32_t e, float4 out[4]) {
+  const uint4 q = fvid[e];
+  out[0] = pv[q.x];
+  out[1] = pv[q.y];
+  out[2] = pv[q.z];
+  out[3] = pv[q.w];
 }
 
 /* REF raster over (view, external face).  Visible faces write
@@ -309,8 +309,7 @@ DEV void face_verts(const uint4* tv, const float4* pv, const uint32_t* ext_int, 
  * (vfRayKernelCPU.cc:274).  The largest visible index wins, as the last face
  * in main.cc's loop (main.cc:209-218) overwrites the pixel. */
 __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ views, uint32_t v0,
-                                                    const uint4* __restrict__ tv,
-                                                    const uint32_t* __restrict__ ext_int,
+                                                    const uint4* __restrict__ fvid,
                                                     const float4* __restrict__ pv,
                                                     uint32_t* __restrict__ owner) {
   /* Column-parallel: a wave takes 64 / cpf faces, cpf lanes per face, each
@@ -329,9 +328,7 @@ __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ 
   const uint32_t e = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64u / cpf) + lane / cpf;
   if (e >= nE) return;
   float4 fv[4];
-  uint32_t t, f;
-  face_verts(tv + views[vi].rec_off, pv + views[vi].vert_off, ext_int + 2 * (size_t)views[vi].ext_off, e,
-             fv, &t, &f);
+  face_verts(fvid + views[vi].ext_off, pv + views[vi].vert_off, e, fv);
   Planes pl = face_planes(fv[0], fv[1], fv[2]);
   float faceZ = fv[3].x * pl.z[0] + fv[3].y * pl.z[1] + pl.z[2];
   if (faceZ < fv[3].z) return;   /* vfRayKernelCPU.cc:128 */
@@ -355,13 +352,13 @@ __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ 
   }
 }
 
-int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* tv,
-                     const uint32_t* ext_int, const float4* pv, uint32_t* owner, void* stream) {
+int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* fvid,
+                     const float4* pv, uint32_t* owner, void* stream) {
   if (nv == 0 || max_faces == 0) return VFR_OK;
   /* sized for one face per wave, the widest split; waves past a view's
    * faces return at once */
   hipLaunchKernelGGL(k_raster_ref, dim3((max_faces + 3) / 4, nv), dim3(256), 0, (hipStream_t)stream, views, v0,
-                     tv, ext_int, pv, owner);
+                     fvid, pv, owner);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -408,8 +405,7 @@ DEV void raster_pixel(const RasterFace& f, uint32_t px, uint32_t py, uint32_t W,
 }
 
 __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ views, uint32_t v0,
-                                                    const uint4* __restrict__ tv,
-                                                    const uint32_t* __restrict__ ext_int,
+                                                    const uint4* __restrict__ fvid,
                                                     const float4* __restrict__ pv,
                                                     uint32_t* __restrict__ cnt,
                                                     uint64_t* __restrict__ entries, uint32_t kmax,
@@ -429,9 +425,7 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
   const uint32_t e = e0 + lane;
   if (lane < fpw && e < nE) {
     float4 fv[4];
-    uint32_t t, fc;
-    face_verts(tv + views[vi].rec_off, pv + views[vi].vert_off, ext_int + 2 * (size_t)views[vi].ext_off, e,
-               fv, &t, &fc);
+    face_verts(fvid + views[vi].ext_off, pv + views[vi].vert_off, e, fv);
     const Planes pl = face_planes(fv[0], fv[1], fv[2]);
     const float faceZ = fv[3].x * pl.z[0] + fv[3].y * pl.z[1] + pl.z[2];
     double lo0 = 1e30, lo1 = 1e30, hi0 = -1e30, hi1 = -1e30;
@@ -497,13 +491,13 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
   }
 }
 
-int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* tv,
-                     const uint32_t* ext_int, const float4* pv, uint32_t* cnt, uint64_t* entries,
+int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* fvid,
+                     const float4* pv, uint32_t* cnt, uint64_t* entries,
                      uint32_t kmax, uint64_t estride, unsigned long long* stats, void* stream) {
   if (nv == 0 || max_faces == 0) return VFR_OK;
   const uint32_t per_block = 4u * RASTER_FPW; /* 256 threads: four waves at the smaller share */
   hipLaunchKernelGGL(k_raster_ftb, dim3((max_faces + per_block - 1) / per_block, nv), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, tv, ext_int, pv, cnt, entries, kmax, estride, stats);
+                     (hipStream_t)stream, views, v0, fvid, pv, cnt, entries, kmax, estride, stats);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -1935,6 +1929,17 @@ __global__ void k_ext_internal(const uint32_t* __restrict__ extf, uint32_t n,
   out[i * 2 + 1] = extf[i * 2 + 1];
 }
 
+/* Vertex ids of each external face's slots f, f+1, f+2 and the one off it,
+ * from its (internal tet, face) and the tet's ids. */
+__global__ void k_face_vids(const uint32_t* __restrict__ ext_int, const uint4* __restrict__ tv, uint32_t n,
+                            uint4* __restrict__ fvid) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t t = ext_int[2 * i], f = ext_int[2 * i + 1];
+  const uint4 rv = tv[t];
+  fvid[i] = make_uint4(sel4(rv, f), sel4(rv, (f + 1u) & 3u), sel4(rv, (f + 2u) & 3u), sel4(rv, (f + 3u) & 3u));
+}
+
 __global__ void k_iota(uint32_t* p, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = i;
@@ -2130,19 +2135,23 @@ int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
   }
   hipLaunchKernelGGL(k_invert, dim3(nblk(nT)), dim3(256), 0, st, perm.as<uint32_t>(), nT, tinv.as<uint32_t>());
   hipLaunchKernelGGL(k_invert, dim3(nblk(nV)), dim3(256), 0, st, vperm.as<uint32_t>(), nV, vinv.as<uint32_t>());
-  DevBuf vert, rec, tv, ext_int;
+  DevBuf vert, rec, tv, ext_int, fvid;
   ALLOC(vert, sizeof(float) * 4 * (size_t)nV);
   ALLOC(rec, sizeof(TetRec) * (size_t)nT);
   ALLOC(tv, sizeof(uint4) * (size_t)nT);
   ALLOC(ext_int, sizeof(uint32_t) * 2 * (size_t)n_ext);
+  ALLOC(fvid, sizeof(uint4) * (size_t)n_ext);
   hipLaunchKernelGGL(k_gather_vert, dim3(nblk(nV)), dim3(256), 0, st, d_xyzs.as<float4>(),
                      vperm.as<uint32_t>(), nV, vert.as<float4>());
   hipLaunchKernelGGL(k_build_rec, dim3(nblk(nT)), dim3(256), 0, st, d_tets.as<uint32_t>(),
                      d_con.as<uint32_t>(), perm.as<uint32_t>(), tinv.as<uint32_t>(),
                      vinv.as<uint32_t>(), nT, rec.as<TetRec>(), tv.as<uint4>());
-  if (n_ext)
+  if (n_ext) {
     hipLaunchKernelGGL(k_ext_internal, dim3(nblk(n_ext)), dim3(256), 0, st, d_extf.as<uint32_t>(),
                        n_ext, tinv.as<uint32_t>(), ext_int.as<uint32_t>());
+    hipLaunchKernelGGL(k_face_vids, dim3(nblk(n_ext)), dim3(256), 0, st, ext_int.as<uint32_t>(),
+                       tv.as<uint4>(), n_ext, fvid.as<uint4>());
+  }
   CHECK_HIP(hipGetLastError());
   CHECK_HIP(hipStreamSynchronize(st));
   m.n_ext = n_ext;
@@ -2150,11 +2159,12 @@ int vfr_dev_build_mesh(DevMesh& m, const float* h_xyzs, const uint32_t* h_tets,
   m.rec = rec.as<TetRec>(); rec.release();
   m.tv = tv.as<uint32_t>(); tv.release();
   m.ext_int = ext_int.as<uint32_t>(); ext_int.release();
+  m.fvid = fvid.as<uint32_t>(); fvid.release();
   m.extf = d_extf.as<uint32_t>(); d_extf.release();
   m.perm = perm.as<uint32_t>(); perm.release();
   m.vperm = vperm.as<uint32_t>(); vperm.release();
   m.con = d_con.as<uint32_t>(); d_con.release();
-  m.bytes = sizeof(float) * 4ull * nV + (sizeof(TetRec) + 16ull) * (uint64_t)nT + 16ull * n_ext +
+  m.bytes = sizeof(float) * 4ull * nV + (sizeof(TetRec) + 16ull) * (uint64_t)nT + 32ull * n_ext +
             4ull * nT + 4ull * nV + 16ull * nT;
   return VFR_OK;
 }

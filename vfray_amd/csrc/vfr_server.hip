@@ -109,6 +109,7 @@ struct vfr_server {
   uint4* tv = nullptr;
   uint64_t n_records = 0;
   uint32_t* ext = nullptr;
+  uint4* fvid = nullptr;       /* external faces' vertex ids, same offsets as ext */
   float4* vert = nullptr;
   std::vector<std::unique_ptr<Slot>> slots;
   std::mutex mu;
@@ -191,7 +192,7 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   CHECK_RC(vfr_k_clear_u32(s->counters, 4, st));
   CHECK_RC(vfr_k_clear_u32((uint32_t*)s->stats, 2 * ST_N, st));
   CHECK_RC(vfr_k_xform_views(s->views, 0, nv, max_verts, sv->vert, s->pverts, st));
-  CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, max_faces, sv->tv, sv->ext, s->pverts, s->cnt, s->entries, VFR_K, s->cap_pix,
+  CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, max_faces, sv->fvid, s->pverts, s->cnt, s->entries, VFR_K, s->cap_pix,
                             s->stats, st));
   CHECK_RC(vfr_k_compact(s->views, nv, 0, (uint32_t)tiles, VFR_MODE_FTB, nullptr, s->cnt, s->entries, VFR_K, s->cap_pix,
                          s->rgba, s->steps, s->entry, s->rays, s->counters, st));
@@ -343,6 +344,7 @@ extern "C" void vfr_server_destroy(vfr_server* sv) {
   if (sv->rec) (void)hipFree(sv->rec);
   if (sv->tv) (void)hipFree(sv->tv);
   if (sv->ext) (void)hipFree(sv->ext);
+  if (sv->fvid) (void)hipFree(sv->fvid);
   if (sv->vert) (void)hipFree(sv->vert);
   delete sv;
 }
@@ -388,6 +390,7 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
   if (hipMalloc((void**)&sv->rec, sizeof(TetRec) * tets) != hipSuccess ||
       hipMalloc((void**)&sv->tv, sizeof(uint4) * tets) != hipSuccess ||
       hipMalloc((void**)&sv->ext, 8 * (exts + 1)) != hipSuccess ||
+      hipMalloc((void**)&sv->fvid, 16 * (exts + 1)) != hipSuccess ||
       hipMalloc((void**)&sv->vert, 16 * verts) != hipSuccess) {
     vfr_set_error("resident mesh arena allocation failed");
     rc = VFR_ENOMEM;
@@ -403,6 +406,7 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
     if (hipMemcpy(sv->rec + ro, d->rec, sizeof(TetRec) * d->n_tets, hipMemcpyDeviceToDevice) != hipSuccess ||
         hipMemcpy(sv->tv + ro, d->tv, sizeof(uint4) * d->n_tets, hipMemcpyDeviceToDevice) != hipSuccess ||
         hipMemcpy(sv->ext + 2 * eo, d->ext_int, 8ull * d->n_ext, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(sv->fvid + eo, d->fvid, 16ull * d->n_ext, hipMemcpyDeviceToDevice) != hipSuccess ||
         hipMemcpy(sv->vert + vo, d->vert, 16ull * d->n_verts, hipMemcpyDeviceToDevice) != hipSuccess) {
       vfr_set_error("copy of mesh %d into the resident arena failed", i);
       rc = VFR_EHIP;
