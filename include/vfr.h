@@ -274,6 +274,8 @@ typedef struct vfr_server_cfg {
   uint32_t occupancy;   /* its schedule (0 = defaults, as vfr_render_opts)       */
   uint32_t waves_per_cu;
   uint32_t refill;
+  uint32_t composite;   /* VFR_COMPOSITE_*, for every request (0 = PPI); the
+                           linear modes use the one-ray-per-thread walk      */
 } vfr_server_cfg;
 
 typedef struct vfr_request {

@@ -82,3 +82,24 @@ def test_server_replay_report(ctx, two_meshes):
         assert rep["steps"] > 0 and rep["mrays_per_s"] > 0
     assert burst["mean_batch"] > 1.5
     assert burst["steps"] == spaced["steps"]
+
+
+@pytest.mark.parametrize("composite", [V.COMPOSITE_LINEAR, V.COMPOSITE_LINEAR_OPACITY])
+def test_server_linear_compositing(ctx, two_meshes, composite):
+    """A server built with a linear compositing mode renders every request
+    as the per-view FTB walk does in that mode, bit for bit."""
+    ms, tfs = two_meshes
+    reqs = _requests(tfs, 6, [64, 96, 128], seed=0x5EED0420 + composite)
+    print(f"server linear compositing {composite}: 6 requests", flush=True)
+    sv = V.Server(ctx, ms, hbm_budget=1 << 30, streams=2, max_batch=4, composite=composite)
+    try:
+        tickets = [sv.submit(m, v, keep_image=True) for m, v in reqs]
+        got = [sv.wait(t, image=True) for t in tickets]
+    finally:
+        sv.close()
+    for (m, v), g in zip(reqs, got):
+        alone, _ = ctx.render(ms[m], [v], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW, composite=composite)
+        ppi, _ = ctx.render(ms[m], [v], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+        assert g["status"] == 2
+        assert np.array_equal(g["rgba"], alone[0]["rgba"])
+        assert not np.array_equal(g["rgba"], ppi[0]["rgba"])

@@ -219,6 +219,7 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   a.refill = sv->cfg.refill;
   a.kmax = VFR_K;
   a.psi_table = (sv->cfg.flags & VFR_RENDER_PSI_TABLE) ? 1u : 0u;
+  a.composite = sv->cfg.composite;
   if (tf_hi - tf_lo < VFR_TF_LDS_SLOTS) { a.tf_lds_base = tf_lo; a.tf_lds_n = tf_hi - tf_lo + 1; }
   const uint32_t proj = n_persp == 0 ? VFR_TRAV_ORTHO : (n_persp == nv ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
   const uint64_t records = (sv->cfg.flags & VFR_RENDER_WIDE) ? VFR_BIG_RECORDS : sv->n_records;
@@ -365,7 +366,8 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
     c.waves_per_cu = std::min(64u, 8u * wps);
   }
   if (c.refill == 0) c.refill = 32;
-  if (c.streams > 16 || c.max_batch > 4096 || c.term_alpha > 1.0f || c.waves_per_cu > 64 || c.refill > 64) {
+  if (c.streams > 16 || c.max_batch > 4096 || c.term_alpha > 1.0f || c.waves_per_cu > 64 || c.refill > 64 ||
+      c.composite > VFR_COMPOSITE_LINEAR_OPACITY) {
     vfr_set_error("vfr_server_create: configuration out of range");
     return VFR_EINVAL;
   }
