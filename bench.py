@@ -157,7 +157,8 @@ def serve_bench(args) -> int:
     plain = sp.replay(sample, np.zeros(len(sample)))
     sp.close()
     log(f"plain capacity {plain['mrays_per_s']:.1f} Mrays/s over {len(sample)} requests")
-    sb = V.Server(ctx, meshes, hbm_budget=32 << 30, streams=2, max_batch=256, **sched)
+    sb = V.Server(ctx, meshes, hbm_budget=32 << 30, streams=args.streams, max_batch=256,
+                  max_batch_pixels=args.batch_pixels, **sched)
     sb.replay(sample[:64], np.zeros(64))
     cap = sb.replay(sample, np.zeros(len(sample)))
     log(f"batched capacity {cap['mrays_per_s']:.1f} Mrays/s, mean batch {cap['mean_batch']:.1f}")
@@ -183,7 +184,8 @@ def serve_bench(args) -> int:
         "data": "synthetic",
         "config": {"workload": "serve", "meshes": len(meshes), "tets": int(tets), "requests": len(reqs),
                    "offered_req_per_s": round(rate, 1), "load_vs_capacity": args.load,
-                   "term_alpha": 0.99, "schedule": sched},
+                   "term_alpha": 0.99, "streams": args.streams, "max_batch_pixels": args.batch_pixels,
+                   "schedule": sched},
         "latency_p50_ms": round(served["latency_p50_ms"], 3),
         "latency_p90_ms": round(served["latency_p90_ms"], 3),
         "latency_p99_ms": round(served["latency_p99_ms"], 3),
@@ -211,6 +213,8 @@ def main() -> int:
     ap.add_argument("--workload", default="batch64", choices=["batch64", "plain_gpu", "concave", "serve"])
     ap.add_argument("--requests", type=int, default=10000, help="serve: requests in the stream")
     ap.add_argument("--load", type=float, default=0.8, help="serve: offered load / batched capacity")
+    ap.add_argument("--streams", type=int, default=2, help="serve: concurrent batches")
+    ap.add_argument("--batch-pixels", type=int, default=4 << 20, help="serve: pixels per batch (0 = no cap)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")

@@ -276,6 +276,8 @@ typedef struct vfr_server_cfg {
   uint32_t refill;
   uint32_t composite;   /* VFR_COMPOSITE_*, for every request (0 = PPI); the
                            linear modes use the one-ray-per-thread walk      */
+  uint64_t max_batch_pixels; /* pixels per batch (0 = as the buffers allow); a
+                           request larger than the cap still runs, alone     */
 } vfr_server_cfg;
 
 typedef struct vfr_request {

@@ -103,3 +103,23 @@ def test_server_linear_compositing(ctx, two_meshes, composite):
         assert g["status"] == 2
         assert np.array_equal(g["rgba"], alone[0]["rgba"])
         assert not np.array_equal(g["rgba"], ppi[0]["rgba"])
+
+
+def test_server_batch_pixel_cap(ctx, two_meshes):
+    """max_batch_pixels bounds every batch (a request above the cap still
+    runs, alone) and does not change any image."""
+    ms, tfs = two_meshes
+    reqs = _requests(tfs, 10, [64, 64, 96, 160], seed=0x5EED0430)
+    print("server batch pixel cap: 10 requests", flush=True)
+    sv = V.Server(ctx, ms, hbm_budget=1 << 30, streams=1, max_batch=16, max_batch_pixels=2 * 64 * 64)
+    try:
+        tickets = [sv.submit(m, v, keep_image=True) for m, v in reqs]
+        got = [sv.wait(t, image=True) for t in tickets]
+    finally:
+        sv.close()
+    for (m, v), g in zip(reqs, got):
+        px = v.width * v.height
+        assert g["status"] == 2
+        assert g["batch_size"] <= (2 if px <= 64 * 64 else 1)
+        alone, _ = ctx.render(ms[m], [v], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+        assert np.array_equal(g["rgba"], alone[0]["rgba"])

@@ -66,7 +66,8 @@ class Stats(C.Structure):
 class ServerCfg(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("streams", C.c_uint32), ("max_batch", C.c_uint32),
                 ("term_alpha", C.c_float), ("flags", C.c_uint32), ("occupancy", C.c_uint32),
-                ("waves_per_cu", C.c_uint32), ("refill", C.c_uint32), ("composite", C.c_uint32)]
+                ("waves_per_cu", C.c_uint32), ("refill", C.c_uint32), ("composite", C.c_uint32),
+                ("max_batch_pixels", C.c_uint64)]
 
 
 class Request(C.Structure):
@@ -492,14 +493,14 @@ class Server:
     def __init__(self, ctx: Context, meshes, hbm_budget: int = 0, streams: int = 0, max_batch: int = 0,
                  term_alpha: float = 0.0, persistent: bool = False, occupancy: int = 0, waves_per_cu: int = 0,
                  refill: int = 0, psi_table: bool = False, wave_queue: bool = False,
-                 composite: int = COMPOSITE_PPI):
+                 composite: int = COMPOSITE_PPI, max_batch_pixels: int = 0):
         self.ctx = ctx
         self.meshes = list(meshes)
         arr = (C.c_void_p * len(self.meshes))(*[m._p.value for m in self.meshes])
         flags = ((RENDER_PERSISTENT if persistent else 0) | (RENDER_PSI_TABLE if psi_table else 0)
                  | (RENDER_WAVE_QUEUE if wave_queue else 0))
         cfg = ServerCfg(hbm_budget, streams, max_batch, term_alpha, flags, occupancy, waves_per_cu, refill,
-                        composite)
+                        composite, max_batch_pixels)
         self._p = C.c_void_p()
         _check(lib().vfr_server_create(ctx._p, C.byref(cfg), arr, len(self.meshes), C.byref(self._p)))
 

@@ -5,7 +5,8 @@
  * Each slot owns a stream and device buffers sized to its share of the HBM
  * budget, allocated once.  A dispatcher thread polls the slots' completion
  * events and, whenever a slot is free, admits the queued requests that fit
- * it (oldest first, up to max_batch) and enqueues the batched launch set:
+ * it (oldest first, up to max_batch requests and max_batch_pixels pixels)
+ * and enqueues the batched launch set:
  * view upload, clears, per-view vertex transform, visibility/raster,
  * compaction and the persistent traversal.  Batches on different slots
  * overlap on the device; the dispatcher never waits on the device.
@@ -294,6 +295,7 @@ static void dispatcher(vfr_server* sv) {
         const SrvMesh& m = sv->meshes[r->mesh];
         const uint64_t kp = r->keep ? r->pixels : 0;
         if (pix + r->pixels > s->cap_pix || verts + m.n_verts > s->cap_verts || keep + kp > s->cap_keep) break;
+        if (!s->batch.empty() && sv->cfg.max_batch_pixels && pix + r->pixels > sv->cfg.max_batch_pixels) break;
         pix += r->pixels;
         verts += m.n_verts;
         keep += kp;
