@@ -55,20 +55,37 @@ def cpu_baseline(x, t, tfa, views, seconds: float) -> dict:
     ext = O.extf(con)
     lmt = O.lmt(xn, t)
     psi = V.psi_generate()
-    rays, el, done = 0, 0.0, 0
-    for v in views:
-        proj, M, k = V.view_matrix(v)
-        t0 = time.perf_counter()
-        O.render_ftb(xn, t, con, ext, tfa, psi, lmt, proj, M, k, v.width, v.height)
-        el += time.perf_counter() - t0
-        rays += v.width * v.height
-        done += 1
-        if el >= seconds:
-            break
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+    def run(budget):
+        rays, el, done = 0, 0.0, 0
+        for v in views:
+            proj, M, k = V.view_matrix(v)
+            t0 = time.perf_counter()
+            O.render_ftb(xn, t, con, ext, tfa, psi, lmt, proj, M, k, v.width, v.height)
+            el += time.perf_counter() - t0
+            rays += v.width * v.height
+            done += 1
+            if el >= budget:
+                break
+        return rays, el, done
+
+    threads = O.set_threads(0)
+    rays, el, done = run(seconds)
+    O.set_threads(1)
+    rays1, el1, done1 = run(seconds)
+    O.set_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": rays / el / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"first {done} of {len(views)} views ({rays} rays), CPU oracle front-to-back "
-                      f"restatement, FP32, OpenMP {threads} threads, {el:.1f} s"}
+                      f"restatement, FP32, OpenMP {threads} threads, {el:.1f} s",
+            "single_core": {"value": rays1 / el1 / 1e6, "unit": "Mrays/s",
+                            "sample": f"first {done1} views ({rays1} rays), 1 thread, {el1:.1f} s"},
+            "host": {"cpu_model": model, "nproc": os.cpu_count()}}
 
 
 def reference_cpu_c1(ctx, reps: int = 5):

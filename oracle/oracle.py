@@ -52,8 +52,16 @@ def lib() -> C.CDLL:
         L.orc_render_ftb.argtypes = [f32p, u32, u32p, u32, u32p, u32p, u32, f32p, f32p, f32p, u32, f32p,
                                      C.c_float, u32, u32, C.c_float, f32p, f64p, u32p, u32p, u32p, u32]
         L.orc_render_ftb.restype = C.c_long
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = C.c_int
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's render loops; returns the previous
+    maximum (n <= 0 only reads it)."""
+    return lib().orc_set_threads(n)
 
 
 def _p(a, t):

@@ -50,6 +50,8 @@
 #include <cstring>
 #include <vector>
 
+#include <omp.h>
+
 typedef uint32_t u32;
 typedef uint64_t u64;
 
@@ -70,6 +72,14 @@ static inline F4 ld4(const float* v, u32 i) {
 /* ------------------------------------------------------------ preprocessing */
 
 /* normalizeVertices, vfRayPreComp.cc:84-145. */
+/* Threads of the OpenMP loops (the render's pixel loop); returns the
+ * previous maximum.  For timing the oracle on one core. */
+extern "C" int orc_set_threads(int n) {
+  const int prev = omp_get_max_threads();
+  if (n > 0) omp_set_num_threads(n);
+  return prev;
+}
+
 extern "C" void orc_normalize(float* v, u32 nV) {
   float mn[4], mx[4], center[3];
   for (int i = 0; i < 4; ++i) { mn[i] = v[i]; mx[i] = v[i]; }

@@ -423,3 +423,20 @@ def test_ftb_concave_reentry(psi):
     cov = r["entry_ext"] != V.NO_ID
     assert (r["n_entries"] > 1).sum() > 100
     assert psnr(r["rgba"], r["rgba64"], cov) >= 50.0
+
+
+def test_oracle_render_independent_of_threads(psi):
+    """The oracle's pixel loop runs under OpenMP: one thread and all threads
+    give the same image, entry faces and step counts."""
+    g = grid_inputs(10, 0x5EED0011)
+    v = V.perspective_view(72, 64, 0, eye=(1.2, -1.6, 1.5))
+    prev = O.set_threads(0)
+    try:
+        O.set_threads(1)
+        one = oracle_ftb(g, v, psi)
+    finally:
+        O.set_threads(prev)
+    many = oracle_ftb(g, v, psi)
+    for k in ("rgba", "entry_ext", "steps", "n_entries"):
+        assert np.array_equal(one[k], many[k]), k
+    assert (one["steps"] > 0).sum() > 1000
