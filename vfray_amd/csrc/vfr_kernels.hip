@@ -1002,7 +1002,7 @@ DEV float4 tf_at(const TravArgs& a, const float4* tfl, uint32_t i) {
  * entry face is degenerate at the pixel. */
 template <bool TFL>
 DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
-  const uint64_t key = a.entries[ftb_cur(r) * a.estride + r.gp];
+  const uint64_t key = __builtin_nontemporal_load(a.entries + ftb_cur(r) * a.estride + r.gp);
   const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull) + a.views[r.vi].ext_off;
   const float4* __restrict__ pv = a.vert + r.voff;
   const uint2 tf2 = reinterpret_cast<const uint2*>(a.ext_int)[e]; /* (tet, face) */
@@ -1052,7 +1052,7 @@ DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pi
   r.oy = pix_to_obj(pix / W, H);
   const float rayf = persp ? sqrtf(1.0f + (r.ox * r.ox + r.oy * r.oy) * (k * k)) : k;
   r.rf = LIN ? rayf : rayf * a.views[vi].inv_me;
-  uint32_t nk = a.cnt[r.gp];
+  uint32_t nk = __builtin_nontemporal_load(a.cnt + r.gp);
   nk = nk > a.kmax ? a.kmax : nk;
   r.bits = (nk << 6) | (persp ? FTB_PERSP : 0u);
   r.ar = r.ag = r.ab = 0.0f;
@@ -1296,10 +1296,15 @@ DEV RayOut trace(const TravArgs& a, const float4* tfl, const uint32_t* sperm, ui
   else return trace_ftb<PROJ, BIG, TFL, PSIT>(a, tfl, sperm, vi, pix);
 }
 
+/* Per-ray inputs and outputs are touched once: non-temporal accesses keep
+ * them from evicting the tet records and vertices from L2 and the
+ * Infinity Cache. */
+typedef float vfr_f4v __attribute__((ext_vector_type(4)));
 DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
-  a.rgba[gp] = o.c;
-  a.steps[gp] = o.steps;
-  a.entry_out[gp] = o.entry;
+  const vfr_f4v c = {o.c.x, o.c.y, o.c.z, o.c.w};
+  __builtin_nontemporal_store(c, reinterpret_cast<vfr_f4v*>(a.rgba) + gp);
+  __builtin_nontemporal_store(o.steps, a.steps + gp);
+  __builtin_nontemporal_store(o.entry, a.entry_out + gp);
 }
 
 /* One ray per thread over the compacted ray list. */
@@ -1321,7 +1326,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR
   const uint32_t n = *a.n_rays;
   unsigned long long st = 0, an = 0, br = 0;
   if (i < n) {
-    const uint64_t ray = a.rays[i];
+    const uint64_t ray = __builtin_nontemporal_load(a.rays + i);
     const uint32_t vi = (uint32_t)(ray >> 32), pix = (uint32_t)ray;
     const RayOut o = trace<MODE, PROJ, BIG, TFL, PSIT>(a, tfs, sperm, vi, pix);
     store_ray(a, a.views[vi].pix_off + pix, o);
