@@ -140,7 +140,8 @@ typedef struct vfr_view {
   float eye[3];         /* VFR_PROJ_PERSPECTIVE                            */
   float target[3];
   float up[3];
-  float fov_y_deg;
+  float fov_y_deg;      /* spans the image height; pixels are square, so the
+                           width spans tan(fov_y/2) * width / height     */
 } vfr_view;
 
 /* Schedules. */
@@ -157,7 +158,8 @@ typedef struct vfr_render_opts {
   uint32_t use_graph;   /* capture the batched launch set in a hipGraph    */
   uint32_t max_entries; /* FTB entry-list slots per pixel: 8, 16 or 32; 0 =
                            the fewest that hold every pixel's entry faces
-                           (counting pass at plan creation, at most 32)  */
+                           (counting pass at plan creation; a pixel with more
+                           than 32 makes vfr_plan_create fail, VFR_EMESH)   */
   uint32_t flags;       /* VFR_RENDER_*                                    */
   uint32_t occupancy;   /* persistent kernel variant: waves per SIMD (4, 5,
                            6: register-capped; other values: the compiler's
@@ -235,6 +237,9 @@ typedef struct vfr_stats {
  * nothing.  Views are copied. */
 int vfr_plan_create(vfr_ctx* ctx, const vfr_mesh* mesh, const vfr_view* views,
                     int n_views, const vfr_render_opts* opts, vfr_plan** out);
+/* VFR_EMESH (whether or not stats is given) when entry faces were dropped
+ * (a pixel with more entry faces than max_entries) or a ray met a neighbour
+ * whose faces do not link back (non-manifold mesh). */
 int vfr_plan_execute(vfr_plan* plan, vfr_stats* stats);
 int vfr_plan_read(vfr_plan* plan, int view, vfr_frame_out* out);
 void vfr_plan_destroy(vfr_plan* plan);
@@ -287,7 +292,9 @@ typedef struct vfr_request {
 } vfr_request;
 
 typedef struct vfr_result {
-  int32_t status;       /* 0 queued, 1 running, 2 done, < 0 error code           */
+  int32_t status;       /* 0 queued, 1 running, 2 done, < 0 error code: VFR_EMESH
+                           when the view has a pixel with more than 32 entry
+                           faces or a ray met a non-manifold face (no image) */
   uint32_t width, height;
   uint32_t batch_size;  /* requests rendered in the same batch                   */
   double latency_ms;    /* arrival -> result complete on the device              */
@@ -313,7 +320,8 @@ int vfr_server_poll(vfr_server* s, uint64_t ticket, vfr_result* out);
  * then forgets the ticket. */
 int vfr_server_wait(vfr_server* s, uint64_t ticket, vfr_result* out, float* rgba_out);
 /* Load generator: submits reqs[i] at arrival offsets_s[i] seconds after the
- * call, waits for all, and reports throughput and latency percentiles. */
+ * call, waits for all, and reports throughput and latency percentiles; the
+ * first request error, if any, is returned after every request is reaped. */
 int vfr_server_replay(vfr_server* s, const vfr_request* reqs, const double* offsets_s, int n,
                       vfr_server_report* report);
 void vfr_server_destroy(vfr_server* s);

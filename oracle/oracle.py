@@ -50,7 +50,8 @@ def lib() -> C.CDLL:
                                      u8p, f32p, f64p, u32p, u32p, u32p, u32]
         L.orc_render_ref.restype = C.c_long
         L.orc_render_ftb.argtypes = [f32p, u32, u32p, u32, u32p, u32p, u32, f32p, f32p, f32p, u32, f32p,
-                                     C.c_float, u32, u32, C.c_float, f32p, f64p, u32p, u32p, u32p, u32]
+                                     C.c_float, C.c_float, u32, u32, C.c_float, f32p, f64p, u32p, u32p,
+                                     u32p, u32]
         L.orc_render_ftb.restype = C.c_long
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_threads.restype = C.c_int
@@ -167,12 +168,13 @@ def render_ref(xyzs, tets, con, ext, tf, psi, lmt_, W, H, fp64=False, composite=
 
 
 def render_ftb(xyzs, tets, con, ext, tf, psi, lmt_, proj, M, k, W, H, term=0.99, fp64=False,
-               psi_table=False, composite=0) -> dict:
+               psi_table=False, composite=0, kx=None) -> dict:
     """The renderer's front-to-back mode restated on the CPU (OpenMP); psi
     from the series, or with psi_table from the table `psi`; composite 1/2:
     the reference's linear integration without / with opacity (rgba = image
     byte / 255 and opacity; rgba64 the same integral in FP64 without the
-    per-step truncation)."""
+    per-step truncation).  Perspective: k = tan(fov_y/2), kx = k W/H (None:
+    kx = k, a square view)."""
     x, t, c, e = f32(xyzs), u32a(tets), u32a(con), u32a(ext)
     tfa, lm, Mm = f32(tf).reshape(-1), f32(lmt_), f32(M).reshape(-1)
     ps = f32(psi).reshape(-1) if psi_table else None
@@ -183,7 +185,8 @@ def render_ftb(xyzs, tets, con, ext, tf, psi, lmt_, proj, M, k, W, H, term=0.99,
     nent = np.zeros((H, W), np.uint32)
     r = lib().orc_render_ftb(_p(x, C.c_float), x.shape[0], _p(t, C.c_uint32), t.shape[0], _p(c, C.c_uint32),
                              _p(e, C.c_uint32), e.shape[0], _p(tfa, C.c_float), _p(ps, C.c_float),
-                             _p(lm, C.c_float), proj, _p(Mm, C.c_float), k, W, H, term,
+                             _p(lm, C.c_float), proj, _p(Mm, C.c_float), k, k if kx is None else kx, W, H,
+                             term,
                              _p(rgba, C.c_float), _p(rgba64, C.c_double), _p(ent, C.c_uint32),
                              _p(steps, C.c_uint32), _p(nent, C.c_uint32), composite)
     if r < 0:

@@ -580,8 +580,9 @@ extern "C" long orc_render_ref(const float* V, u32 nV, const u32* T, u32 nT, con
  * Planes map to planes, so rays through pixels are the lines X = ox, Y = oy
  * and march towards decreasing Z, as the reference's rays march -z.
  * Segment length in world units: perspective |1/Z1 - 1/Z0| *
- * sqrt(1 + (ox^2 + oy^2) k^2) with k = tan(fov/2); ortho |Z1 - Z0| * k with
- * k = world length of a unit step along -Z.  k is passed as `view_k`. */
+ * sqrt(1 + (ox kx)^2 + (oy k)^2) with k = tan(fov_y/2), kx = k W/H; ortho
+ * |Z1 - Z0| * k with k = world length of a unit step along -Z.  k and kx are
+ * passed as `view_k` and `view_kx`. */
 struct OView { u32 proj; float M[12]; float k; };
 
 static inline F4 xform(const OView& v, const F4& p) {
@@ -744,7 +745,7 @@ extern "C" float orc_psi_series(float a, float b) {
 extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, const u32* con,
                                const u32* extf, u32 nExt, const float* tf, const float* psi,
                                const float* lmt, u32 proj, const float* M, float view_k,
-                               u32 W, u32 H, float term, float* rgba, double* rgba64,
+                               float view_kx, u32 W, u32 H, float term, float* rgba, double* rgba64,
                                u32* entry_ext, u32* steps_out, u32* n_entries_out, u32 composite) {
   (void)nT;
   if (W < 2 || H < 2) return -1;
@@ -835,8 +836,10 @@ extern "C" long orc_render_ftb(const float* V, u32 nV, const u32* T, u32 nT, con
       float rayf = view_k;
       double rayf64 = 1.0;
       if (proj == 1) {
-        rayf = sqrtf(1.0f + (ox * ox + oy * oy) * (view_k * view_k));
-        rayf64 = sqrt(1.0 + ((double)ox * ox + (double)oy * oy) * ((double)view_k * view_k));
+        const float ax = ox * view_kx, ay = oy * view_k;
+        rayf = sqrtf((1.0f + ax * ax) + ay * ay);
+        const double ax64 = (double)ox * view_kx, ay64 = (double)oy * view_k;
+        rayf64 = sqrt(1.0 + ax64 * ax64 + ay64 * ay64);
       }
       float ar = 0, ag = 0, ab = 0, Tr = 1.0f;
       double ar64 = 0, ag64 = 0, ab64 = 0, T64 = 1.0;

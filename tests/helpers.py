@@ -36,10 +36,10 @@ def psnr(a: np.ndarray, b: np.ndarray, mask: np.ndarray) -> float:
 
 
 def oracle_ftb(inp, view: V.View, psi, term=0.99, fp64=False, psi_table=False, composite=0):
-    proj, M, k = V.view_matrix(view)
+    proj, M, k, kx = V.view_matrix(view)
     return O.render_ftb(inp["xn"], inp["t"], inp["con"], inp["ext"], inp["tf"], psi, inp["lmt"],
                         proj, M, k, view.width, view.height, term=term, fp64=fp64, psi_table=psi_table,
-                        composite=composite)
+                        composite=composite, kx=kx)
 
 
 def legacy_tf_text(seed: int = 7) -> str:

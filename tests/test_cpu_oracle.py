@@ -366,7 +366,7 @@ def test_ftb_entry_faces_against_brute_force(psi):
     inp = grid_inputs(6, 0x5EED0300)
     v = WL.sphere_views(0x5EED0300, 1, [64], 0)[0]
     r = oracle_ftb(inp, v, psi)
-    proj, M, k = V.view_matrix(v)
+    proj, M, k, kx = V.view_matrix(v)
     M = M.astype(np.float64).reshape(3, 4)
     P = inp["xn"][:, :3].astype(np.float64) @ M[:, :3].T + M[:, 3]
     Z = 1.0 / P[:, 2]

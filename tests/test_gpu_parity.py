@@ -529,3 +529,32 @@ def test_edge_cases(ctx, psi):
     assert (g["steps"][cov] == 1).all()
     assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
     m1.close()
+
+
+def test_dropped_entries_reported_without_stats(ctx):
+    """Rays along a grid's boundary edges (a 2x2 orthographic view: 129 entry
+    faces at a pixel): a plan with 8 slots reports VFR_EMESH from
+    vfr_plan_execute whether or not the caller asks for stats, and a plan
+    left to choose its slots refuses to be created (more than 32)."""
+    import ctypes as C
+    inp = grid_inputs(8, 0x5EED0106)
+    m = ctx.mesh(inp["x"], inp["t"])
+    tf = ctx.tf(inp["tf"])
+    print("dropped entries, stats NULL", flush=True)
+    p = ctx.plan(m, [V.ortho_view(2, 2, tf)], mode=V.MODE_FTB, max_entries=8)
+    L = V.lib()
+    assert L.vfr_plan_execute(p._p, None) == -8
+    assert "dropped" in L.vfr_last_error().decode()
+    st = V.Stats()
+    assert L.vfr_plan_execute(p._p, C.byref(st)) == -8
+    p.close()
+    with pytest.raises(V.VfrError, match="more than the 32 entry slots") as e:
+        ctx.plan(m, [V.ortho_view(2, 2, tf)], mode=V.MODE_FTB)
+    assert e.value.code == -8
+    # a view whose rays stay off the boundary (scaled by 1.2: at most one entry
+    # face per pixel) renders on the same mesh with 8 slots
+    mat = np.hstack([np.diag([1.2, 1.2, 1.2]), np.zeros((3, 1))]).astype(np.float32)
+    ok = ctx.plan(m, [V.ortho_view(64, 48, tf, matrix=mat)], mode=V.MODE_FTB, max_entries=8)
+    assert L.vfr_plan_execute(ok._p, None) == 0
+    ok.close()
+    m.close()

@@ -123,3 +123,100 @@ def test_server_batch_pixel_cap(ctx, two_meshes):
         assert g["batch_size"] <= (2 if px <= 64 * 64 else 1)
         alone, _ = ctx.render(ms[m], [v], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
         assert np.array_equal(g["rgba"], alone[0]["rgba"])
+
+
+def test_server_concave_and_dropped_entries(ctx):
+    """Requests on the concave benchmark mesh (96^3 grid, graded, 12 voids:
+    pixels with up to 13 entry faces, more than 8 slots hold) render exactly
+    as the per-view walk; a request whose rays run along the boundary edges
+    of a grid (129 entry faces at a pixel, more than 32) fails with VFR_EMESH
+    instead of returning an image with entries dropped, alone and in a
+    replay, and the server goes on serving."""
+    x, t, tfa, seed = WL.concave()
+    mc = ctx.mesh(x, t)
+    xg, tg = V.synth_grid(8, 0x5EED0106, 0)
+    mg = ctx.mesh(xg, tg)
+    tf = ctx.tf(tfa)
+    views = WL.sphere_views(seed, 32, [768], tf)[:2]
+    print("server: concave 96^3 2x768^2 + an overflowing 2x2 view", flush=True)
+    with pytest.raises(V.VfrError, match="dropped"):
+        ctx.render(mc, views[:1], mode=V.MODE_FTB, max_entries=8)
+    edge = V.ortho_view(2, 2, tf)
+    sv = V.Server(ctx, [mc, mg], hbm_budget=2 << 30, streams=2, max_batch=8)
+    try:
+        tickets = [sv.submit(0, v, keep_image=True) for v in views]
+        tbad = sv.submit(1, edge, keep_image=True)
+        got = [sv.wait(tk, image=True) for tk in tickets]
+        with pytest.raises(V.VfrError) as e:
+            sv.wait(tbad, image=True)
+        assert e.value.code == -8 and "entry" in str(e.value)
+        for v, g in zip(views, got):
+            alone, _ = ctx.render(mc, [v], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+            assert g["status"] == 2
+            for key in ("rgba",):
+                assert np.array_equal(g[key], alone[0][key])
+        # a replay that contains the failing request reports it after reaping all
+        reqs = [V.Request(0, 0, views[0]), V.Request(1, 0, edge), V.Request(0, 0, views[1])]
+        with pytest.raises(V.VfrError) as e:
+            sv.replay(reqs, np.zeros(3))
+        assert e.value.code == -8
+        t2 = sv.submit(0, views[1], keep_image=True)
+        g2 = sv.wait(t2, image=True)
+        assert np.array_equal(g2["rgba"], got[1]["rgba"])
+    finally:
+        sv.close()
+        mc.close()
+        mg.close()
+
+
+def test_server_c5_stream_matches_per_view_and_oracle(ctx):
+    """The serving workload itself: its 8 resident meshes (32^3..160^3
+    grids, 0.2M-24.6M tets, 4 TFs each) and the first 200 requests of its
+    stream, all queued at once.  Every image equals the same request rendered
+    alone by the per-view walk, bit for bit; for one request of at most 512^2
+    per mesh the image meets the oracle bar (entry faces and step counts
+    bit-exact with the CPU oracle, RGBA within 2e-3, PSNR >= 50 dB against its
+    FP64 re-integration)."""
+    import oracle as O
+    from helpers import oracle_ftb, psnr
+    meshes, tf_ids, host = [], [], []
+    for (x, t, tfs) in WL.serve_meshes():
+        meshes.append(ctx.mesh(x, t))
+        tf_ids.append([ctx.tf(tf) for tf in tfs])
+        host.append((x, t, tfs))
+    print("server C5: 8 meshes built", flush=True)
+    reqs = WL.serve_requests(200, len(meshes), tf_ids)
+    sv = V.Server(ctx, meshes, hbm_budget=16 << 30, streams=2, max_batch=256, max_batch_pixels=4 << 20)
+    checked = set()
+    psi = V.psi_generate()
+    try:
+        tickets = [sv.submit(r.mesh, r.view, keep_image=True) for r in reqs]
+        sizes = []
+        for i, (r, tk) in enumerate(zip(reqs, tickets)):
+            g = sv.wait(tk, image=True)
+            assert g["status"] == 2
+            sizes.append(g["batch_size"])
+            alone = ctx.render(meshes[r.mesh], [r.view], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)[0][0]
+            assert np.array_equal(g["rgba"], alone["rgba"]), i
+            if r.mesh not in checked and r.view.width <= 512:
+                x, t, tfs = host[r.mesh]
+                print(f"request {i}: mesh {r.mesh} {r.view.width}^2 against the oracle", flush=True)
+                con, ext, _ = meshes[r.mesh].connectivity()
+                xn = O.normalize(x)
+                tfa = tfs[tf_ids[r.mesh].index(r.view.tf_id)]
+                inp = dict(xn=xn, t=t, con=con, ext=ext, lmt=O.lmt(xn, t), tf=tfa)
+                orc = oracle_ftb(inp, r.view, psi, fp64=True)
+                assert orc["anomalies"] == 0
+                assert np.array_equal(alone["entry_ext"], orc["entry_ext"])
+                assert np.array_equal(alone["steps"], orc["steps"])
+                assert np.abs(g["rgba"] - orc["rgba"]).max() <= 2e-3
+                cov = orc["entry_ext"] != V.NO_ID
+                assert cov.mean() > 0.1
+                assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
+                checked.add(r.mesh)
+        assert max(sizes) > 4
+    finally:
+        sv.close()
+        for m in meshes:
+            m.close()
+    assert len(checked) == 8

@@ -306,14 +306,17 @@ def perspective_view(width: int, height: int, tf_id: int, eye, target=(0, 0, 0),
     return v
 
 
-def view_matrix(v: View) -> tuple[int, np.ndarray, float]:
-    """The float32 3x4 matrix and k the library derives for a view (same double
-    maths, rounded once), for the oracle."""
+def view_matrix(v: View) -> tuple[int, np.ndarray, float, float]:
+    """(projection, the float32 3x4 matrix, k, kx) the library derives for a
+    view (same double maths, rounded once), for the oracle.  Perspective: k =
+    tan(fov_y/2) and kx = k W/H, the half-extents of the image plane at unit
+    depth; orthographic: k = world length of a unit step along -Z (= kx)."""
     if v.projection == PROJ_ORTHO:
         M = np.array(v.ortho[:], np.float32)
         A = M.reshape(3, 4)[:, :3].astype(np.float64)
         w = np.linalg.inv(A)[:, 2]
-        return 0, M, float(np.float32(np.sqrt((w * w).sum())))
+        k = float(np.float32(np.sqrt((w * w).sum())))
+        return 0, M, k, k
     E = np.array(v.eye[:], np.float64)
     f = np.array(v.target[:], np.float64) - E
     up = np.array(v.up[:], np.float64)
@@ -322,13 +325,13 @@ def view_matrix(v: View) -> tuple[int, np.ndarray, float]:
     r /= np.sqrt((r * r).sum())
     u = np.array([r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0]])
     k = np.tan(float(np.float32(v.fov_y_deg)) * np.pi / 360.0)
-    cot = 1.0 / k
+    kx = k * float(v.width) / float(v.height)
     M = np.zeros(12, np.float32)
-    for i, (row, sc) in enumerate(((r, cot), (u, cot), (f, 1.0))):
+    for i, (row, sc) in enumerate(((r, 1.0 / kx), (u, 1.0 / k), (f, 1.0))):
         for a in range(3):
             M[i * 4 + a] = np.float32(sc * row[a])
         M[i * 4 + 3] = np.float32(-sc * (row[0] * E[0] + row[1] * E[1] + row[2] * E[2]))
-    return 1, M, float(np.float32(k))
+    return 1, M, float(np.float32(k)), float(np.float32(kx))
 
 
 # -------------------------------------------------------------------- objects

@@ -368,14 +368,17 @@ int vfr_make_view(const vfr_view& in, const float* bb_lo, const float* bb_hi, De
     if (!(rl > 1e-9)) { vfr_set_error("up vector parallel to the view direction"); return VFR_EINVAL; }
     for (int a = 0; a < 3; ++a) r[a] /= rl;
     double u[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0]};
-    double k = tan(in.fov_y_deg * M_PI / 360.0), cot = 1.0 / k;
+    /* fov_y spans the image height; the width spans tan(fov_y/2) W/H, so
+     * pixels are square whatever the aspect ratio */
+    double k = tan(in.fov_y_deg * M_PI / 360.0), kx = k * (double)in.width / (double)in.height;
     const double* rows[3] = {r, u, f};
-    const double sc[3] = {cot, cot, 1.0};
+    const double sc[3] = {1.0 / kx, 1.0 / k, 1.0};
     for (int i = 0; i < 3; ++i) {
       for (int a = 0; a < 3; ++a) o->M[i * 4 + a] = (float)(sc[i] * rows[i][a]);
       o->M[i * 4 + 3] = (float)(-sc[i] * (rows[i][0] * E[0] + rows[i][1] * E[1] + rows[i][2] * E[2]));
     }
     o->k = (float)k;
+    o->kx = (float)kx;
     /* every vertex must lie in front of the eye: check the bbox corners */
     for (int q = 0; q < 8; ++q) {
       double p[3] = {(q & 1) ? bb_hi[0] : bb_lo[0], (q & 2) ? bb_hi[1] : bb_lo[1],
@@ -425,7 +428,7 @@ static int plan_count_entries(vfr_plan* p, uint32_t* out) {
   CHECK_RC(vfr_k_clear_u32(p->counters, 1, st));
   CHECK_RC(vfr_k_xform_views(p->views, 0, p->n_views, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
   CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d.n_ext, m->fvid4(), p->pverts, p->owner,
-                            nullptr, 0, 0, p->stats, st));
+                            nullptr, 0, 0, p->stats, nullptr, st));
   CHECK_RC(vfr_k_max_u32(p->owner, p->n_pix, p->counters, st));
   CHECK_HIP(hipMemcpyAsync(out, p->counters, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   CHECK_HIP(hipStreamSynchronize(st));
@@ -443,7 +446,7 @@ static int enqueue_batched(vfr_plan* p) {
     CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, m->d.n_ext, m->fvid4(), p->pverts, p->owner, st));
   else
     CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d.n_ext, m->fvid4(), p->pverts, p->owner,
-                              p->entries, p->kmax, p->n_pix, p->stats, st));
+                              p->entries, p->kmax, p->n_pix, p->stats, nullptr, st));
   CHECK_RC(vfr_k_compact(p->views, p->n_views, 0, p->n_tiles, p->opts.mode, p->owner, p->owner,
                          p->entries, p->kmax, p->n_pix, p->rgba, p->steps, p->entry, p->rays, p->counters + p->n_views, st));
   return VFR_OK;
@@ -621,6 +624,12 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
     if (k == 0) {
       uint32_t mx = 0;
       rc = plan_count_entries(p, &mx);
+      if (rc == VFR_OK && mx > VFR_K_MAX) {
+        /* a ray along a boundary edge enters every face on it */
+        vfr_set_error("a pixel has %u entry faces, more than the %u entry slots a plan holds; they would be "
+                      "dropped", mx, VFR_K_MAX);
+        rc = VFR_EMESH;
+      }
       k = mx <= 8 ? 8u : (mx <= 16 ? 16u : VFR_K_MAX);
     }
     p->kmax = k;
@@ -728,7 +737,7 @@ static int execute_pipelined(vfr_plan* p, double* ms_total, double* ms_vis, doub
     const vfr_plan::Chunk& ch = p->chunks[c];
     CHECK_RC(vfr_k_xform_views(p->views, ch.v0, ch.nv, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
     CHECK_RC(vfr_k_raster_ftb(p->views, ch.v0, ch.nv, m->d.n_ext, m->fvid4(), p->pverts, p->owner,
-                              p->entries, p->kmax, p->n_pix, p->stats, st));
+                              p->entries, p->kmax, p->n_pix, p->stats, nullptr, st));
     CHECK_RC(vfr_k_compact(p->views, p->n_views, ch.tile0, ch.ntiles, p->opts.mode, p->owner, p->owner,
                            p->entries, p->kmax, p->n_pix, p->rgba, p->steps, p->entry, p->rays + ch.pix0, ccount + c, st));
     CHECK_HIP(hipEventRecord(p->cev[3 * c], st));
@@ -809,7 +818,7 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
         CHECK_RC(vfr_k_raster_ref(p->views, v, 1, m->d.n_ext, m->fvid4(), p->pverts, p->owner, st));
       else
         CHECK_RC(vfr_k_raster_ftb(p->views, v, 1, m->d.n_ext, m->fvid4(), p->pverts, p->owner,
-                                  p->entries, p->kmax, p->n_pix, p->stats, st));
+                                  p->entries, p->kmax, p->n_pix, p->stats, nullptr, st));
       CHECK_RC(vfr_k_compact(p->views, p->n_views, hv.tile_off, ntl, p->opts.mode, p->owner, p->owner,
                              p->entries, p->kmax, p->n_pix, p->rgba, p->steps, p->entry, p->rays + hv.pix_off,
                              p->counters + v, st));
@@ -838,10 +847,12 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
     p->frame_ms = frame;
   }
   p->executed = true;
+  /* the device counters say whether entries were dropped or a ray met a
+   * non-manifold face: read them whether or not the caller wants stats */
+  unsigned long long s[ST_N];
+  CHECK_HIP(hipMemcpy(s, p->stats, sizeof(s), hipMemcpyDeviceToHost));
   if (stats) {
-    unsigned long long s[ST_N];
     std::vector<uint32_t> cnt(p->n_views + 2);
-    CHECK_HIP(hipMemcpy(s, p->stats, sizeof(s), hipMemcpyDeviceToHost));
     CHECK_HIP(hipMemcpy(cnt.data(), p->counters, sizeof(uint32_t) * cnt.size(), hipMemcpyDeviceToHost));
     uint64_t active = 0;
     if (p->opts.schedule == VFR_SCHED_BATCHED && !p->chunks.empty() && (p->opts.flags & VFR_RENDER_PIPELINED)) {
@@ -864,14 +875,14 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
     stats->ms_traversal = ms_trav;
     stats->ms_frame_p50 = p50;
     stats->ms_frame_p99 = p99;
-    if (s[ST_OVERFLOW]) {
-      vfr_set_error("%llu pixel entries beyond %u per pixel were dropped", s[ST_OVERFLOW], p->kmax);
-      return VFR_EMESH;
-    }
-    if (s[4]) {
-      vfr_set_error("%llu rays met a neighbour without a back link (non-manifold mesh)", s[4]);
-      return VFR_EMESH;
-    }
+  }
+  if (s[ST_OVERFLOW]) {
+    vfr_set_error("%llu pixel entries beyond %u per pixel were dropped", s[ST_OVERFLOW], p->kmax);
+    return VFR_EMESH;
+  }
+  if (s[ST_BROKEN]) {
+    vfr_set_error("%llu rays met a neighbour without a back link (non-manifold mesh)", s[ST_BROKEN]);
+    return VFR_EMESH;
   }
   return VFR_OK;
 }

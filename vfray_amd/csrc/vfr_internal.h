@@ -76,7 +76,7 @@ static_assert(sizeof(TetRec) == 32, "TetRec is 32 bytes");
  * arenas (a plan's single mesh, or a server's resident meshes). */
 struct alignas(16) DevView {
   float M[12];        /* row-major 3x4 view transform                       */
-  float k;            /* ortho: world length per unit Z; persp: tan(fov/2)  */
+  float k;            /* ortho: world length per unit Z; persp: tan(fov_y/2) */
   uint32_t W, H;
   uint32_t proj;      /* VFR_PROJ_*                                          */
   uint32_t tf;        /* transfer-function slot                              */
@@ -91,7 +91,8 @@ struct alignas(16) DevView {
   uint32_t n_verts;   /* mesh: vertices                                      */
   float max_edge;     /* mesh: .lmt max edge length                          */
   float inv_me;       /* 1 / max_edge                                        */
-  uint32_t pad_[4];
+  float kx;           /* persp: tan(fov_y/2) * W / H (horizontal half-extent) */
+  uint32_t pad_[3];
 };
 static_assert(sizeof(DevView) % 16 == 0, "DevView alignment");
 
@@ -117,8 +118,14 @@ struct DevMesh {
  * double, sc = 1 / the largest half extent, ss = 1 / (max s - min s). */
 void vfr_normalize_params(const float mn[4], const float mx[4], float center[3], float* sc, float* ss);
 
-/* Statistics counters a render accumulates on the device. */
-enum { ST_STEPS = 0, ST_ANOM = 1, ST_OVERFLOW = 2, ST_RAYS = 3, ST_N = 8 };
+/* Statistics counters a render accumulates on the device (ST_BROKEN: rays
+ * that met a neighbour whose faces do not link back). */
+enum { ST_STEPS = 0, ST_ANOM = 1, ST_OVERFLOW = 2, ST_RAYS = 3, ST_BROKEN = 4, ST_N = 8 };
+
+/* Per-view error flags a launch can record (TravArgs.vflags and the FTB
+ * raster's vflags, one word per view of the launch). */
+#define VFR_VFLAG_OVERFLOW 1u /* a pixel had more entry faces than its slots */
+#define VFR_VFLAG_BROKEN 2u   /* a ray met a neighbour without a back link   */
 
 #ifdef __HIPCC__
 struct TravArgs {
@@ -148,6 +155,8 @@ struct TravArgs {
   uint64_t estride;        /* FTB: pixels per entry slot plane                 */
   uint32_t composite;      /* VFR_COMPOSITE_* (both walks)                     */
   uint32_t psi_table;      /* FTB: psi from the table (VFR_RENDER_PSI_TABLE)     */
+  uint32_t* vflags;        /* per-view VFR_VFLAG_* words, indexed like views, or
+                              null                                             */
 };
 
 /* ---- launchers (vfr_kernels.hip) ---- */
@@ -161,9 +170,12 @@ int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, uint32_t m
 int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* fvid,
                      const float4* pv, uint32_t* owner, void* stream);
 /* kmax: entry slots per pixel; 0 only counts (cnt) without storing keys. */
+/* vflags (may be null): VFR_VFLAG_OVERFLOW is or-ed into the word of each
+ * view where a pixel's list overflows. */
 int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* fvid,
                      const float4* pv, uint32_t* cnt, uint64_t* entries,
-                     uint32_t kmax, uint64_t estride, unsigned long long* stats, void* stream);
+                     uint32_t kmax, uint64_t estride, unsigned long long* stats, uint32_t* vflags,
+                     void* stream);
 /* *out = max(*out, cnt[i]) over n pixels (device counter). */
 int vfr_k_max_u32(const uint32_t* cnt, size_t n, uint32_t* out, void* stream);
 int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, uint32_t ntiles,

@@ -390,7 +390,7 @@ struct RasterFace {
 /* One pixel of a face: the inside test and, inside, the append. */
 DEV void raster_pixel(const RasterFace& f, uint32_t px, uint32_t py, uint32_t W, uint32_t H, uint32_t off,
                       uint32_t fe, uint32_t* __restrict__ cnt, uint64_t* __restrict__ entries, uint32_t kmax,
-                      uint64_t estride, unsigned long long& over) {
+                      uint64_t estride, unsigned long long& over, uint32_t* __restrict__ vflag) {
   const float ox = pix_to_obj(px, W), oy = pix_to_obj(py, H);
   const float v0px = ox - f.ax, v0py = oy - f.ay;
   const float cy = (f.v01x * v0py) - (f.v01y * v0px);
@@ -400,8 +400,12 @@ DEV void raster_pixel(const RasterFace& f, uint32_t px, uint32_t py, uint32_t W,
   const float Zp = f.zA + ((wb * f.dB + wc * f.dC) * f.invD);
   const uint32_t gp = off + py * W + px;
   const uint32_t slot = atomicAdd(&cnt[gp], 1u);
-  if (slot < kmax) entries[slot * estride + gp] = ((uint64_t)fkey(Zp) << 32) | fe;
-  else over++;
+  if (slot < kmax) {
+    entries[slot * estride + gp] = ((uint64_t)fkey(Zp) << 32) | fe;
+  } else {
+    over++;
+    if (vflag) atomicOr(vflag, VFR_VFLAG_OVERFLOW);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ views, uint32_t v0,
@@ -409,8 +413,10 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
                                                     const float4* __restrict__ pv,
                                                     uint32_t* __restrict__ cnt,
                                                     uint64_t* __restrict__ entries, uint32_t kmax,
-                                                    uint64_t estride, unsigned long long* __restrict__ stats) {
+                                                    uint64_t estride, unsigned long long* __restrict__ stats,
+                                                    uint32_t* __restrict__ vflags) {
   const uint32_t vi = v0 + blockIdx.y;
+  uint32_t* const vflag = (vflags && kmax) ? vflags + vi : nullptr;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t nE = views[vi].n_ext;
   const uint32_t W = views[vi].W, H = views[vi].H, off = views[vi].pix_off;
@@ -463,7 +469,8 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
   /* small faces: the lane's own loop (x outer, y inner over the bbox) */
   if (vis && area <= RASTER_SMALL)
     for (uint32_t px = f.bx0; px <= f.bx1; ++px)
-      for (uint32_t py = f.by0; py <= f.by1; ++py) raster_pixel(f, px, py, W, H, off, e, cnt, entries, kmax, estride, over);
+      for (uint32_t py = f.by0; py <= f.by1; ++py)
+        raster_pixel(f, px, py, W, H, off, e, cnt, entries, kmax, estride, over, vflag);
   /* larger faces: the whole wave, one face at a time */
   uint64_t todo = __ballot(vis && area > RASTER_SMALL);
   while (todo) {
@@ -482,7 +489,8 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
     for (uint32_t ty = g.by0; ty <= g.by1; ty += 8u)
       for (uint32_t tx = g.bx0; tx <= g.bx1; tx += 8u) {
         const uint32_t px = tx + (lane & 7u), py = ty + (lane >> 3);
-        if (px <= g.bx1 && py <= g.by1) raster_pixel(g, px, py, W, H, off, ge, cnt, entries, kmax, estride, over);
+        if (px <= g.bx1 && py <= g.by1)
+          raster_pixel(g, px, py, W, H, off, ge, cnt, entries, kmax, estride, over, vflag);
       }
   }
   if (kmax) {
@@ -493,11 +501,12 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
 
 int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* fvid,
                      const float4* pv, uint32_t* cnt, uint64_t* entries,
-                     uint32_t kmax, uint64_t estride, unsigned long long* stats, void* stream) {
+                     uint32_t kmax, uint64_t estride, unsigned long long* stats, uint32_t* vflags,
+                     void* stream) {
   if (nv == 0 || max_faces == 0) return VFR_OK;
   const uint32_t per_block = 4u * RASTER_FPW; /* 256 threads: four waves at the smaller share */
   hipLaunchKernelGGL(k_raster_ftb, dim3((max_faces + per_block - 1) / per_block, nv), dim3(256), 0,
-                     (hipStream_t)stream, views, v0, fvid, pv, cnt, entries, kmax, estride, stats);
+                     (hipStream_t)stream, views, v0, fvid, pv, cnt, entries, kmax, estride, stats, vflags);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -1042,7 +1051,7 @@ DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
 template <bool TFL, bool LIN = false>
 DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pix, FtbRay& r) {
   const uint32_t W = a.views[vi].W, H = a.views[vi].H;
-  const float k = a.views[vi].k;
+  const float k = a.views[vi].k, kx = a.views[vi].kx;
   const bool persp = a.views[vi].proj == VFR_PROJ_PERSPECTIVE;
   r.vi = vi;
   r.voff = a.views[vi].vert_off;
@@ -1050,7 +1059,10 @@ DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pi
   r.gp = a.views[vi].pix_off + pix;
   r.ox = pix_to_obj(pix % W, W);
   r.oy = pix_to_obj(pix / W, H);
-  const float rayf = persp ? sqrtf(1.0f + (r.ox * r.ox + r.oy * r.oy) * (k * k)) : k;
+  /* perspective: world length per unit of depth along the pixel's ray,
+   * |(ox tan_x, oy tan_y, 1)| with tan_x = tan(fov_y/2) W/H, tan_y = tan(fov_y/2) */
+  const float ax = r.ox * kx, ay = r.oy * k;
+  const float rayf = persp ? sqrtf((1.0f + ax * ax) + ay * ay) : k;
   r.rf = LIN ? rayf : rayf * a.views[vi].inv_me;
   uint32_t nk = __builtin_nontemporal_load(a.cnt + r.gp);
   nk = nk > a.kmax ? a.kmax : nk;
@@ -1331,6 +1343,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR
     const RayOut o = trace<MODE, PROJ, BIG, TFL, PSIT>(a, tfs, sperm, vi, pix);
     store_ray(a, a.views[vi].pix_off + pix, o);
     st = o.steps; an = o.anomaly; br = o.broken;
+    if (o.broken && a.vflags) atomicOr(a.vflags + vi, VFR_VFLAG_BROKEN);
   }
   st = wave_sum_u64(st);
   an = wave_sum_u64(an);
@@ -1338,7 +1351,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR
   if ((threadIdx.x & 63u) == 0) {
     if (st) atomicAdd(&a.stats[ST_STEPS], st);
     if (an) atomicAdd(&a.stats[ST_ANOM], an);
-    if (br) atomicAdd(&a.stats[4], br);
+    if (br) atomicAdd(&a.stats[ST_BROKEN], br);
   }
 }
 
@@ -1373,6 +1386,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
       const RayOut o = trace<VFR_MODE_FTB, PROJ, BIG, TFL, PSIT>(a, tfs, sperm, vi, pix);
       store_ray(a, a.views[vi].pix_off + pix, o);
       st += o.steps; an += o.anomaly; br += o.broken;
+      if (o.broken && a.vflags) atomicOr(a.vflags + vi, VFR_VFLAG_BROKEN);
     }
   }
   st = wave_sum_u64(st);
@@ -1381,7 +1395,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
   if (lane == 0) {
     if (st) atomicAdd(&a.stats[ST_STEPS], st);
     if (an) atomicAdd(&a.stats[ST_ANOM], an);
-    if (br) atomicAdd(&a.stats[4], br);
+    if (br) atomicAdd(&a.stats[ST_BROKEN], br);
   }
 }
 
@@ -1448,6 +1462,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         const RayOut o = ftb_out(a, r);
         store_ray(a, r.gp, o);
         st += o.steps; an += o.anomaly; br += o.broken;
+        if (o.broken && a.vflags) atomicOr(a.vflags + r.vi, VFR_VFLAG_BROKEN);
         active = false;
       }
     }
@@ -1456,7 +1471,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   if (lane == 0) {
     if (ws) atomicAdd(&a.stats[ST_STEPS], ws);
     if (wa) atomicAdd(&a.stats[ST_ANOM], wa);
-    if (wb) atomicAdd(&a.stats[4], wb);
+    if (wb) atomicAdd(&a.stats[ST_BROKEN], wb);
   }
 }
 

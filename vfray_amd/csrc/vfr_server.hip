@@ -22,6 +22,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -52,9 +53,13 @@ inline double secs(Clock::time_point t) {
   return std::chrono::duration<double>(t.time_since_epoch()).count();
 }
 
-/* Device bytes a request needs: per pixel the entry count, VFR_K entry keys,
- * a ray slot, RGBA, steps and entry id; per mesh vertex one P-space vertex. */
-inline uint64_t pixel_bytes() { return 4 + 8ull * VFR_K + 8 + 16 + 4 + 4; }
+/* Device bytes a request needs: per pixel the entry count, VFR_K_MAX entry
+ * keys, a ray slot, RGBA, steps and entry id; per mesh vertex one P-space
+ * vertex.  Every batch gets the most entry slots a plan can have, so that no
+ * view a request can ask for needs a counting pass first (the slots beyond a
+ * pixel's count cost memory only: the raster, the sort and the walk touch
+ * as many as the pixel has). */
+inline uint64_t pixel_bytes() { return 4 + 8ull * VFR_K_MAX + 8 + 16 + 4 + 4; }
 
 struct SrvMesh {
   uint32_t rec_off, ext_off, wvert_off, n_ext, n_verts, n_tets;
@@ -73,6 +78,7 @@ struct Req {
   double t_arrival = 0, t_admit = 0, t_done = 0;
   int32_t status = 0;
   uint32_t batch = 0;
+  uint32_t vflag = 0;      /* VFR_VFLAG_* its view recorded */
   std::vector<float> image;
 };
 
@@ -91,6 +97,8 @@ struct Slot {
   uint32_t* steps = nullptr;
   uint32_t* entry = nullptr;
   uint32_t* counters = nullptr; /* [0] rays, [1] queue head */
+  uint32_t* vflags = nullptr;   /* [cap_views] VFR_VFLAG_* per request */
+  uint32_t* h_vflags = nullptr; /* pinned */
   unsigned long long* stats = nullptr;
   unsigned long long* h_stats = nullptr; /* pinned */
   uint32_t* h_counters = nullptr;        /* pinned */
@@ -127,10 +135,10 @@ struct vfr_server {
 
 static void slot_free(Slot* s) {
   void* dev[] = {s->views, s->cnt, s->entries, s->rays, s->pverts, s->rgba, s->steps, s->entry,
-                 s->counters, s->stats};
+                 s->counters, s->stats, s->vflags};
   for (void* p : dev)
     if (p) (void)hipFree(p);
-  void* host[] = {s->h_views, s->h_stats, s->h_counters, s->h_img};
+  void* host[] = {s->h_views, s->h_stats, s->h_counters, s->h_img, s->h_vflags};
   for (void* p : host)
     if (p) (void)hipHostFree(p);
   if (s->done) (void)hipEventDestroy(s->done);
@@ -146,7 +154,7 @@ static int slot_alloc(Slot* s, uint64_t bytes, uint32_t max_batch) {
   CHECK_HIP(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
   CHECK_HIP(hipMalloc((void**)&s->views, sizeof(DevView) * s->cap_views));
   CHECK_HIP(hipMalloc((void**)&s->cnt, 4 * s->cap_pix));
-  CHECK_HIP(hipMalloc((void**)&s->entries, 8ull * VFR_K * s->cap_pix));
+  CHECK_HIP(hipMalloc((void**)&s->entries, 8ull * VFR_K_MAX * s->cap_pix));
   CHECK_HIP(hipMalloc((void**)&s->rays, 8 * s->cap_pix));
   CHECK_HIP(hipMalloc((void**)&s->pverts, 16 * s->cap_verts));
   CHECK_HIP(hipMalloc((void**)&s->rgba, 16 * s->cap_pix));
@@ -154,7 +162,9 @@ static int slot_alloc(Slot* s, uint64_t bytes, uint32_t max_batch) {
   CHECK_HIP(hipMalloc((void**)&s->entry, 4 * s->cap_pix));
   CHECK_HIP(hipMalloc((void**)&s->counters, 16));
   CHECK_HIP(hipMalloc((void**)&s->stats, sizeof(unsigned long long) * ST_N));
+  CHECK_HIP(hipMalloc((void**)&s->vflags, sizeof(uint32_t) * s->cap_views));
   CHECK_HIP(hipHostMalloc((void**)&s->h_views, sizeof(DevView) * s->cap_views, hipHostMallocDefault));
+  CHECK_HIP(hipHostMalloc((void**)&s->h_vflags, sizeof(uint32_t) * s->cap_views, hipHostMallocDefault));
   CHECK_HIP(hipHostMalloc((void**)&s->h_stats, sizeof(unsigned long long) * ST_N, hipHostMallocDefault));
   CHECK_HIP(hipHostMalloc((void**)&s->h_counters, 16, hipHostMallocDefault));
   return VFR_OK;
@@ -192,11 +202,12 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   CHECK_RC(vfr_k_clear_u32(s->cnt, pix, st));
   CHECK_RC(vfr_k_clear_u32(s->counters, 4, st));
   CHECK_RC(vfr_k_clear_u32((uint32_t*)s->stats, 2 * ST_N, st));
+  CHECK_RC(vfr_k_clear_u32(s->vflags, nv, st));
   CHECK_RC(vfr_k_xform_views(s->views, 0, nv, max_verts, sv->vert, s->pverts, st));
-  CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, max_faces, sv->fvid, s->pverts, s->cnt, s->entries, VFR_K, s->cap_pix,
-                            s->stats, st));
-  CHECK_RC(vfr_k_compact(s->views, nv, 0, (uint32_t)tiles, VFR_MODE_FTB, nullptr, s->cnt, s->entries, VFR_K, s->cap_pix,
-                         s->rgba, s->steps, s->entry, s->rays, s->counters, st));
+  CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, max_faces, sv->fvid, s->pverts, s->cnt, s->entries, VFR_K_MAX,
+                            s->cap_pix, s->stats, s->vflags, st));
+  CHECK_RC(vfr_k_compact(s->views, nv, 0, (uint32_t)tiles, VFR_MODE_FTB, nullptr, s->cnt, s->entries, VFR_K_MAX,
+                         s->cap_pix, s->rgba, s->steps, s->entry, s->rays, s->counters, st));
   TravArgs a;
   memset(&a, 0, sizeof(a));
   a.rec = sv->rec;
@@ -218,7 +229,8 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   a.queue = s->counters + 1;
   a.term = sv->cfg.term_alpha;
   a.refill = sv->cfg.refill;
-  a.kmax = VFR_K;
+  a.kmax = VFR_K_MAX;
+  a.vflags = s->vflags;
   a.psi_table = (sv->cfg.flags & VFR_RENDER_PSI_TABLE) ? 1u : 0u;
   a.composite = sv->cfg.composite;
   if (tf_hi - tf_lo < VFR_TF_LDS_SLOTS) { a.tf_lds_base = tf_lo; a.tf_lds_n = tf_hi - tf_lo + 1; }
@@ -231,6 +243,7 @@ static int launch_batch(vfr_server* sv, Slot* s) {
                           sv->cfg.waves_per_cu * (uint32_t)vfr_ctx_cus(sv->ctx), st));
   CHECK_HIP(hipMemcpyAsync(s->h_stats, s->stats, sizeof(unsigned long long) * ST_N, hipMemcpyDeviceToHost, st));
   CHECK_HIP(hipMemcpyAsync(s->h_counters, s->counters, 16, hipMemcpyDeviceToHost, st));
+  CHECK_HIP(hipMemcpyAsync(s->h_vflags, s->vflags, sizeof(uint32_t) * nv, hipMemcpyDeviceToHost, st));
   if (keep_pix) {
     for (uint32_t i = 0; i < nv; ++i) {
       Req* r = s->batch[i];
@@ -244,13 +257,17 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   return VFR_OK;
 }
 
-/* Results of a finished slot; caller holds the lock. */
+/* Results of a finished slot; caller holds the lock.  A request whose view
+ * dropped entry faces (a pixel with more than VFR_K_MAX) or met a
+ * non-manifold face ends with VFR_EMESH instead of an image. */
 static void finish_batch(vfr_server* sv, Slot* s, double now) {
   for (uint32_t i = 0; i < s->batch.size(); ++i) {
     Req* r = s->batch[i];
     r->t_done = now;
-    r->status = 2;
-    if (r->keep) r->image.assign(s->h_img + 4ull * r->keep_off, s->h_img + 4ull * (r->keep_off + r->pixels));
+    r->vflag = s->h_vflags[i];
+    r->status = r->vflag ? VFR_EMESH : 2;
+    if (r->keep && !r->vflag)
+      r->image.assign(s->h_img + 4ull * r->keep_off, s->h_img + 4ull * (r->keep_off + r->pixels));
   }
   sv->n_batches++;
   sv->n_batched_reqs += s->batch.size();
@@ -511,6 +528,11 @@ extern "C" int vfr_server_wait(vfr_server* sv, uint64_t ticket, vfr_result* out,
   sv->cv_done.wait(lk, [r] { return r->status == 2 || r->status < 0; });
   if (out) fill_result(r, out);
   int rc = r->status < 0 ? r->status : VFR_OK;
+  if (rc == VFR_EMESH && r->vflag)
+    vfr_set_error("request %llu: %s", (unsigned long long)ticket,
+                  (r->vflag & VFR_VFLAG_OVERFLOW)
+                      ? "a pixel has more entry faces than the 32 entry slots hold (entries would be dropped)"
+                      : "a ray met a neighbour without a back link (non-manifold mesh)");
   if (rc == VFR_OK && rgba_out) {
     if (!r->keep) { vfr_set_error("request was submitted without keep_image"); rc = VFR_ESTATE; }
     else memcpy(rgba_out, r->image.data(), sizeof(float) * r->image.size());
@@ -541,25 +563,47 @@ extern "C" int vfr_server_replay(vfr_server* sv, const vfr_request* reqs, const 
   const auto t0 = Clock::now();
   const double t0s = secs(t0);
   int rc = VFR_OK;
+  int submitted = 0;
   for (int i = 0; i < n && rc == VFR_OK; ++i) {
     std::this_thread::sleep_until(t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(offsets[i])));
     rc = submit_at(sv, &reqs[i], t0s + offsets[i], &tickets[i]);
+    if (rc == VFR_OK) submitted = i + 1;
   }
-  if (rc != VFR_OK) return rc;
+  if (rc != VFR_OK) {
+    /* reap what was submitted before the failure, keeping its message */
+    std::string msg = vfr_last_error();
+    for (int i = 0; i < submitted; ++i) (void)vfr_server_wait(sv, tickets[i], nullptr, nullptr);
+    vfr_set_error("%s", msg.c_str());
+    return rc;
+  }
   std::vector<double> lat(n);
   double last = 0;
   uint64_t rays = 0;
+  int first_err = VFR_OK;
+  std::string err_msg;
   for (int i = 0; i < n; ++i) {
     vfr_result r;
     double t_done = 0;
     {
       std::unique_lock<std::mutex> lk(sv->mu);
-      Req* q = sv->reqs[tickets[i]].get();
+      auto it = sv->reqs.find(tickets[i]);
+      if (it == sv->reqs.end()) {
+        /* taken by a concurrent vfr_server_wait on the same ticket */
+        if (first_err == VFR_OK) {
+          first_err = VFR_EINVAL;
+          err_msg = "ticket " + std::to_string(tickets[i]) + " was waited for by another caller";
+        }
+        continue;
+      }
+      Req* q = it->second.get();
       sv->cv_done.wait(lk, [q] { return q->status == 2 || q->status < 0; });
       t_done = q->t_done;
     }
     int rc2 = vfr_server_wait(sv, tickets[i], &r, nullptr);
-    if (rc2 != VFR_OK) return rc2;
+    if (rc2 != VFR_OK) {
+      if (first_err == VFR_OK) { first_err = rc2; err_msg = vfr_last_error(); }
+      continue;
+    }
     lat[i] = r.latency_ms;
     last = std::max(last, t_done);
     rays += (uint64_t)r.width * r.height;
@@ -576,6 +620,10 @@ extern "C" int vfr_server_replay(vfr_server* sv, const vfr_request* reqs, const 
     rep->anomalies = sv->anomalies - a0;
     rep->active_rays = sv->active_rays - ar0;
     rep->mean_batch = rep->batches ? (double)(sv->n_batched_reqs - an0) / rep->batches : 0.0;
+  }
+  if (first_err != VFR_OK) {
+    vfr_set_error("%s", err_msg.c_str());
+    return first_err;
   }
   rep->latency_p50_ms = pctl(lat, 0.50);
   rep->latency_p90_ms = pctl(lat, 0.90);
