@@ -261,6 +261,25 @@ int vfr_render(vfr_ctx* ctx, const vfr_mesh* mesh, const vfr_view* views,
                int n_views, const vfr_render_opts* opts, vfr_frame_out* outs,
                vfr_stats* stats);
 
+/* vfr_render without any allocation: no hipMalloc/hipFree, no host heap.
+ * No counterpart in the reference; it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  The per-pixel and per-view buffers live in a fixed device
+ * scratch area of 128 MiB per device (VFR_SCRATCH_BYTES in the sources),
+ * which the library holds from its first context on that device until the
+ * process ends; its layout is the views, counters, per-pixel count, ray,
+ * RGBA, step and entry buffers, the P-space vertices (16 B x vertices x
+ * views) and the entry keys (8 B x slots x pixels).  A call that would need
+ * more returns VFR_EBUDGET with the sizes in the message.  Every call on one
+ * device shares that device's scratch area (each call clears what must start
+ * at zero): calls that overlap on one device race.  Renders one batched
+ * launch set on the context's stream and waits for it; opts as vfr_render,
+ * except use_graph and VFR_RENDER_PIPELINED (VFR_EINVAL); rgb8 outputs need
+ * rgba outputs beside them; stats carry the host wall time in ms_total. */
+int vfr_render_noalloc(vfr_ctx* ctx, const vfr_mesh* mesh, const vfr_view* views,
+                       int n_views, const vfr_render_opts* opts, vfr_frame_out* outs,
+                       vfr_stats* stats);
+
 /* --------------------------------------------------------------- serving */
 /* A render server keeps meshes resident on the device and renders a stream of
  * requests with continuous batching: requests queue on arrival; whenever one

@@ -132,6 +132,8 @@ def lib() -> C.CDLL:
             "vfr_plan_autotune": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                             C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
             "vfr_render": (C.c_int, [P, P, C.POINTER(View), C.c_int, C.POINTER(RenderOpts), C.POINTER(FrameOut), C.POINTER(Stats)]),
+            "vfr_render_noalloc": (C.c_int, [P, P, C.POINTER(View), C.c_int, C.POINTER(RenderOpts),
+                                             C.POINTER(FrameOut), C.POINTER(Stats)]),
             "vfr_free": (None, [P]),
             "vfr_server_create": (C.c_int, [P, C.POINTER(ServerCfg), C.POINTER(P), C.c_int, C.POINTER(P)]),
             "vfr_server_submit": (C.c_int, [P, C.POINTER(Request), C.POINTER(C.c_uint64)]),

@@ -57,6 +57,8 @@ extern "C" const char* vfr_version(void) { return "vfr 0.1 gfx950"; }
 
 /* Accessors for the server (vfr_server.hip). */
 int vfr_ctx_device(const vfr_ctx* c) { return c->device; }
+void* vfr_ctx_stream(const vfr_ctx* c) { return c->stream; }
+const uint32_t* vfr_mesh_host_extf(const vfr_mesh* m) { return m->h_extf.data(); }
 int vfr_ctx_cus(const vfr_ctx* c) { return c->n_cu; }
 const float4* vfr_ctx_tf(const vfr_ctx* c) { return c->tf; }
 uint32_t vfr_ctx_n_tf(const vfr_ctx* c) { return c->n_tf; }
@@ -96,6 +98,7 @@ extern "C" int vfr_ctx_create(int device, vfr_ctx** out) {
     return VFR_ENOMEM;
   }
   int rc = vfr_psi_load(c, nullptr);
+  if (rc == VFR_OK) rc = vfr_noalloc_init(device); /* device is current */
   if (rc != VFR_OK) { vfr_ctx_destroy(c); return rc; }
   *out = c;
   return VFR_OK;

@@ -209,6 +209,14 @@ struct vfr_mesh;
 struct vfr_view;
 int vfr_make_view(const vfr_view& in, const float* bb_lo, const float* bb_hi, DevView* o);
 int vfr_ctx_device(const vfr_ctx* c);
+void* vfr_ctx_stream(const vfr_ctx* c);
+/* The mesh's (caller tet, face) pairs of its external faces, on the host. */
+const uint32_t* vfr_mesh_host_extf(const vfr_mesh* m);
+/* Size of vfr_render_noalloc's device scratch area (one per device), and its
+ * set-up: the area's device address on `device` (the current device), taken
+ * once per device by vfr_ctx_create. */
+#define VFR_SCRATCH_BYTES (128ull << 20)
+int vfr_noalloc_init(int device);
 int vfr_ctx_cus(const vfr_ctx* c);
 const float4* vfr_ctx_tf(const vfr_ctx* c);
 uint32_t vfr_ctx_n_tf(const vfr_ctx* c);
