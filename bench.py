@@ -238,6 +238,7 @@ def main() -> int:
                     help="batched: skip the schedule autotuner (one ray per thread unless --persistent)")
     ap.add_argument("--persistent", action="store_true", help="persistent traversal kernel with lane refill")
     ap.add_argument("--wave-queue", action="store_true", help="wave work-queue traversal kernel")
+    ap.add_argument("--lds-records", action="store_true", help="walk with LDS-staged tet records")
     ap.add_argument("--pipelined", action="store_true",
                     help="batched: view chunks whose preprocessing overlaps the walk of earlier chunks")
     ap.add_argument("--occupancy", type=int, default=0)
@@ -284,7 +285,8 @@ def main() -> int:
     total_rays = sum(v.width * v.height for v in views)
 
     tune = None
-    manual = args.persistent or args.wave_queue or args.pipelined or args.occupancy or args.wpc or args.refill
+    manual = (args.persistent or args.wave_queue or args.pipelined or args.occupancy or args.wpc or args.refill
+              or args.lds_records)
     if not args.no_autotune and not manual:
         pt = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
         tune = pt.autotune()
@@ -292,10 +294,12 @@ def main() -> int:
         log(f"autotune: {tune}")
     kw = {} if tune is None else {"persistent": tune["persistent"], "wave_queue": tune["wave_queue"],
                                   "pipelined": tune["pipelined"], "waves_per_cu": tune["waves_per_cu"],
-                                  "refill": tune["refill"], "occupancy": tune["occupancy"]}
+                                  "refill": tune["refill"], "occupancy": tune["occupancy"],
+                                  "lds_records": tune["lds_records"]}
     if tune is None and manual:
         kw = {"persistent": args.persistent, "wave_queue": args.wave_queue, "pipelined": args.pipelined,
-              "waves_per_cu": args.wpc, "refill": args.refill, "occupancy": args.occupancy}
+              "waves_per_cu": args.wpc, "refill": args.refill, "occupancy": args.occupancy,
+              "lds_records": args.lds_records}
         tune = dict(kw)
     pb = ctx.plan(mesh, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, use_graph=args.graph, **kw)
     for _ in range(args.warmup):
@@ -356,7 +360,8 @@ def main() -> int:
     model_bytes = B_STEP * steps_per_launch + B_RAY * rays_active
     roof = None
     if ms_trav:
-        kname = "k_trav_persist" if kw.get("persistent") else ("k_trav_wq" if kw.get("wave_queue") else "k_trav")
+        kname = ("k_trav_persist" if kw.get("persistent") else "k_trav_wq" if kw.get("wave_queue")
+                 else "k_trav_ldsr" if kw.get("lds_records") else "k_trav")
         tr = measured_traffic(kname, args.workload)
         # achieved: bytes the walk moves through HBM per launch (rocprofv3
         # FETCH_SIZE + WRITE_SIZE of the same launch, copy-kernel calibrated,

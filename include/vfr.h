@@ -212,6 +212,12 @@ typedef struct vfr_render_opts {
  * until the list is drained (no refill inside a wave; a wave that is done
  * moves on without waiting for its block).  Not with VFR_RENDER_PERSISTENT. */
 #define VFR_RENDER_WAVE_QUEUE 0x10u
+/* The one-ray-per-thread front-to-back walk (psi in registers, meshes below
+ * 2^27 tets) with tet records staged in LDS per wave: per step the lanes of
+ * a wave that move into the same tet load its 32-byte record once (one owner
+ * lane per tet, elected through a 64-slot table in LDS) and the others read
+ * it from LDS.  Same results; other walks ignore the flag. */
+#define VFR_RENDER_LDS_RECORDS 0x20u
 
 typedef struct vfr_frame_out {
   float* rgba;          /* W*H*4, may be NULL                              */
@@ -246,11 +252,12 @@ void vfr_plan_destroy(vfr_plan* plan);
 /* Per-view frame latency (ms, HIP events) of the last PER_VIEW execute. */
 int vfr_plan_frame_times(const vfr_plan* plan, double* out_ms);
 
-/* Times the one-ray-per-thread traversal, the wave work queue and candidate
- * persistent-kernel schedules (kernel occupancy variant x waves per CU x
- * refill threshold) on the plan itself and keeps the fastest walk, then
- * keeps the pipelined execution if it is faster end to end; returns the
- * choice (flags holds VFR_RENDER_WAVE_QUEUE / VFR_RENDER_PERSISTENT /
+/* Times the one-ray-per-thread traversal (with and without LDS-staged
+ * records), the wave work queue and candidate persistent-kernel schedules
+ * (kernel occupancy variant x waves per CU x refill threshold) on the plan
+ * itself and keeps the fastest walk, then keeps the pipelined execution if it
+ * is faster end to end; returns the choice (flags holds
+ * VFR_RENDER_WAVE_QUEUE / VFR_RENDER_PERSISTENT / VFR_RENDER_LDS_RECORDS /
  * VFR_RENDER_PIPELINED) and the walk time.  Results are cached per (tets,
  * rays, views, psi variant) in the process. */
 int vfr_plan_autotune(vfr_plan* plan, uint32_t* flags, uint32_t* occupancy,

@@ -279,8 +279,9 @@ def test_ftb_perspective_32(ctx, psi):
 
 def test_batched_equals_per_view(ctx):
     """Each view rendered in a batch equals the same view rendered alone, with
-    every walk kernel (one ray per thread, lane-refill persistent, wave work
-    queue) and with pipelined view chunks."""
+    every walk kernel (one ray per thread with and without LDS-staged tet
+    records, lane-refill persistent, wave work queue) and with pipelined view
+    chunks."""
     inp = grid_inputs(24, 0x5EED0103)
     m = ctx.mesh(inp["x"], inp["t"])
     tf = ctx.tf(inp["tf"])
@@ -295,7 +296,10 @@ def test_batched_equals_per_view(ctx):
     e, se = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, pipelined=True)
     f, sf = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, wave_queue=True)
     g, sg = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, wave_queue=True, pipelined=True)
+    h, sh = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, lds_records=True)
+    k, sk = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW, lds_records=True)
     assert sa["steps"] == sb["steps"] == sc["steps"] == sd["steps"] == se["steps"] == sf["steps"] == sg["steps"]
+    assert sa["steps"] == sh["steps"] == sk["steps"]
     assert sa["active_rays"] == sb["active_rays"] == se["active_rays"]
     for i, v in enumerate(views):
         alone, _ = ctx.render(m, [v], mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
@@ -306,6 +310,8 @@ def test_batched_equals_per_view(ctx):
             assert np.array_equal(a[i][key], e[i][key])
             assert np.array_equal(a[i][key], f[i][key])
             assert np.array_equal(a[i][key], g[i][key])
+            assert np.array_equal(a[i][key], h[i][key])
+            assert np.array_equal(a[i][key], k[i][key])
             assert np.array_equal(a[i][key], alone[0][key])
     m.close()
 

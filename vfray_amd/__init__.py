@@ -23,6 +23,7 @@ RENDER_PIPELINED = 0x2
 RENDER_PSI_TABLE = 0x4
 RENDER_WIDE = 0x8
 RENDER_WAVE_QUEUE = 0x10
+RENDER_LDS_RECORDS = 0x20
 SCHED_PER_VIEW, SCHED_BATCHED = 0, 1
 MESH_NORMALIZE, MESH_MORTON = 0x1, 0x2
 NO_ID = 0xFFFFFFFF
@@ -380,10 +381,10 @@ class Context:
     def plan(self, mesh: "Mesh", views, mode=MODE_FTB, schedule=SCHED_BATCHED, term_alpha=0.99,
              use_graph=False, waves_per_cu=0, refill=0, persistent=False, occupancy=0,
              max_entries=0, pipelined=False, composite=COMPOSITE_PPI, psi_table=False, wide=False,
-             wave_queue=False) -> "Plan":
+             wave_queue=False, lds_records=False) -> "Plan":
         return Plan(self, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
                     persistent, occupancy, max_entries, pipelined, composite, psi_table, wide,
-                    wave_queue)
+                    wave_queue, lds_records)
 
     def render(self, mesh: "Mesh", views, **kw) -> tuple[list[dict], dict]:
         p = self.plan(mesh, views, **kw)
@@ -437,13 +438,13 @@ class Mesh:
 class Plan:
     def __init__(self, ctx, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
                  persistent=False, occupancy=0, max_entries=0, pipelined=False, composite=COMPOSITE_PPI,
-                 psi_table=False, wide=False, wave_queue=False):
+                 psi_table=False, wide=False, wave_queue=False, lds_records=False):
         self.ctx, self.mesh = ctx, mesh
         self.views = list(views)
         arr = (View * len(self.views))(*self.views)
         flags = ((RENDER_PERSISTENT if persistent else 0) | (RENDER_PIPELINED if pipelined else 0)
                  | (RENDER_PSI_TABLE if psi_table else 0) | (RENDER_WIDE if wide else 0)
-                 | (RENDER_WAVE_QUEUE if wave_queue else 0))
+                 | (RENDER_WAVE_QUEUE if wave_queue else 0) | (RENDER_LDS_RECORDS if lds_records else 0))
         o = RenderOpts(mode, schedule, term_alpha, waves_per_cu, refill, 1 if use_graph else 0, max_entries,
                        flags, occupancy, composite)
         self._p = C.c_void_p()
@@ -454,6 +455,7 @@ class Plan:
         _check(lib().vfr_plan_autotune(self._p, C.byref(f), C.byref(o), C.byref(w), C.byref(r), C.byref(ms)))
         return {"persistent": bool(f.value & RENDER_PERSISTENT), "pipelined": bool(f.value & RENDER_PIPELINED),
                 "wave_queue": bool(f.value & RENDER_WAVE_QUEUE),
+                "lds_records": bool(f.value & RENDER_LDS_RECORDS),
                 "occupancy": o.value, "waves_per_cu": w.value, "refill": r.value, "traversal_ms": ms.value}
 
     def execute(self) -> dict:
@@ -498,12 +500,12 @@ class Server:
     def __init__(self, ctx: Context, meshes, hbm_budget: int = 0, streams: int = 0, max_batch: int = 0,
                  term_alpha: float = 0.0, persistent: bool = False, occupancy: int = 0, waves_per_cu: int = 0,
                  refill: int = 0, psi_table: bool = False, wave_queue: bool = False,
-                 composite: int = COMPOSITE_PPI, max_batch_pixels: int = 0):
+                 composite: int = COMPOSITE_PPI, max_batch_pixels: int = 0, lds_records: bool = False):
         self.ctx = ctx
         self.meshes = list(meshes)
         arr = (C.c_void_p * len(self.meshes))(*[m._p.value for m in self.meshes])
         flags = ((RENDER_PERSISTENT if persistent else 0) | (RENDER_PSI_TABLE if psi_table else 0)
-                 | (RENDER_WAVE_QUEUE if wave_queue else 0))
+                 | (RENDER_WAVE_QUEUE if wave_queue else 0) | (RENDER_LDS_RECORDS if lds_records else 0))
         cfg = ServerCfg(hbm_budget, streams, max_batch, term_alpha, flags, occupancy, waves_per_cu, refill,
                         composite, max_batch_pixels)
         self._p = C.c_void_p()

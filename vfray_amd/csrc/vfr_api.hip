@@ -490,6 +490,7 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   a.tf_lds_n = p->tf_n;
   a.composite = p->opts.composite;
   a.psi_table = (p->opts.flags & VFR_RENDER_PSI_TABLE) ? 1u : 0u;
+  a.lds_records = (p->opts.flags & VFR_RENDER_LDS_RECORDS) ? 1u : 0u;
   return a;
 }
 
@@ -942,6 +943,9 @@ struct TuneKey {
 struct TuneVal {
   uint32_t flags, occ, wpc, refill;
 };
+/* The flags the autotuner chooses. */
+static const uint32_t kTuned =
+    VFR_RENDER_PERSISTENT | VFR_RENDER_WAVE_QUEUE | VFR_RENDER_PIPELINED | VFR_RENDER_LDS_RECORDS;
 static std::mutex g_tune_mu;
 static std::vector<std::pair<TuneKey, TuneVal>> g_tune;
 
@@ -964,7 +968,7 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
     std::lock_guard<std::mutex> g(g_tune_mu);
     for (auto& e : g_tune)
       if (e.first == key) {
-        p->opts.flags = (p->opts.flags & ~(VFR_RENDER_PERSISTENT | VFR_RENDER_WAVE_QUEUE | VFR_RENDER_PIPELINED)) | e.second.flags;
+        p->opts.flags = (p->opts.flags & ~kTuned) | e.second.flags;
         p->opts.occupancy = e.second.occ;
         p->opts.waves_per_cu = e.second.wpc;
         p->opts.refill = e.second.refill;
@@ -981,7 +985,7 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
     static const uint32_t kR[] = {16, 64};
     /* each candidate's walk is timed alone: all preprocessing first, then
      * one walk launch */
-    const uint32_t user_flags = p->opts.flags & ~(VFR_RENDER_PERSISTENT | VFR_RENDER_WAVE_QUEUE | VFR_RENDER_PIPELINED);
+    const uint32_t user_flags = p->opts.flags & ~kTuned;
     const uint32_t base_flags = user_flags;
     best = 1e30;
     TuneVal bv{0u, p->opts.occupancy, p->opts.waves_per_cu, p->opts.refill};
@@ -1005,6 +1009,15 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
     if (rc == VFR_OK && t < best) {
       best = t;
       bv = TuneVal{VFR_RENDER_WAVE_QUEUE, p->opts.occupancy, p->opts.waves_per_cu, p->opts.refill};
+    }
+    if (!(base_flags & VFR_RENDER_PSI_TABLE)) {
+      /* the one-ray-per-thread walk with LDS-staged tet records */
+      p->opts.flags = base_flags | VFR_RENDER_LDS_RECORDS;
+      if (rc == VFR_OK) rc = time_it(&t);
+      if (rc == VFR_OK && t < best) {
+        best = t;
+        bv = TuneVal{VFR_RENDER_LDS_RECORDS, p->opts.occupancy, p->opts.waves_per_cu, p->opts.refill};
+      }
     }
     p->opts.flags = base_flags | VFR_RENDER_PERSISTENT;
     for (uint32_t occ : kOcc) {
@@ -1043,7 +1056,7 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tune.push_back({key, bv});
   }
-  if (flags_out) *flags_out = p->opts.flags & (VFR_RENDER_PERSISTENT | VFR_RENDER_WAVE_QUEUE | VFR_RENDER_PIPELINED);
+  if (flags_out) *flags_out = p->opts.flags & kTuned;
   if (occ_out) *occ_out = p->opts.occupancy;
   if (wpc_out) *wpc_out = p->opts.waves_per_cu;
   if (refill_out) *refill_out = p->opts.refill;

@@ -157,6 +157,9 @@ struct TravArgs {
   uint32_t psi_table;      /* FTB: psi from the table (VFR_RENDER_PSI_TABLE)     */
   uint32_t* vflags;        /* per-view VFR_VFLAG_* words, indexed like views, or
                               null                                             */
+  uint32_t lds_records;    /* FTB, one ray per thread, psi in registers, records
+                              below 4 GiB: tet records staged in LDS per wave
+                              (VFR_RENDER_LDS_RECORDS)                         */
 };
 
 /* ---- launchers (vfr_kernels.hip) ---- */

@@ -527,6 +527,33 @@ int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t ma
  * sorted front to back (Batcher's 19-comparator network on 8 keys,
  * descending). */
 #define COMPACT_WAVES 16u
+
+/* Tile lt of a view's tiles_x x tiles_y 8x8 tiles, in 4x4 groups of tiles
+ * (32x32 pixels, row-major groups, row-major inside a group) over the part of
+ * the image the groups cover, then the tiles of the right and bottom strips
+ * row-major.  The ray list follows this order, so the rays the device walks
+ * at one time cover a compact patch of the image (the benchmark's 64-view
+ * walk: 1.2 % faster than row-major tiles). */
+DEV void tile_xy(uint32_t lt, uint32_t tiles_x, uint32_t tiles_y, uint32_t* tx, uint32_t* ty) {
+  const uint32_t gx = tiles_x & ~3u, gy = tiles_y & ~3u;
+  const uint32_t n_grouped = gx * gy;
+  if (lt < n_grouped) {
+    const uint32_t g = lt >> 4, w = lt & 15u, per_row = gx >> 2;
+    *tx = (g % per_row) * 4u + (w & 3u);
+    *ty = (g / per_row) * 4u + (w >> 2);
+    return;
+  }
+  lt -= n_grouped;
+  const uint32_t rw = tiles_x - gx; /* right strip: rows [0, gy) */
+  if (lt < rw * gy) {
+    *tx = gx + lt % rw;
+    *ty = lt / rw;
+    return;
+  }
+  lt -= rw * gy; /* bottom strip: rows [gy, tiles_y), every column */
+  *tx = lt % tiles_x;
+  *ty = gy + lt / tiles_x;
+}
 __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ views,
                                                   uint32_t n_views, uint32_t tile0, uint32_t ntiles,
                                                   uint32_t mode, const uint32_t* __restrict__ owner,
@@ -553,7 +580,8 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
     }
     vi = lo;
     const uint32_t W = views[vi].W, H = views[vi].H;
-    const uint32_t lt = gt - views[vi].tile_off, tx = lt % views[vi].tiles_x, ty = lt / views[vi].tiles_x;
+    uint32_t tx, ty;
+    tile_xy(gt - views[vi].tile_off, views[vi].tiles_x, (H + 7u) / 8u, &tx, &ty);
     const uint32_t px = tx * 8u + (lane & 7u), py = ty * 8u + (lane >> 3);
     const bool valid = px < W && py < H;
     pix = py * W + px;
@@ -1093,7 +1121,34 @@ enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
  * psi is looked up in the 512x512 table (nearest entry of gamma = tau /
  * (1 + tau) at both ends, the reference's convention) instead of evaluated
  * by psi_zeta. */
-template <int PROJ, bool BIG, bool TFL, bool PSIT, bool LIN = false>
+/* LDS-staged tet records (VFR_RENDER_LDS_RECORDS): after the slot maps
+ * (sperm[0, 32)) each wave of the block owns a 64-entry table: 64 tet tags
+ * and 64 owner lanes (128 words), and, further on, 64 records of 32 bytes.
+ * Per step every lane whose next tet is inside the mesh tags slot (tet & 63);
+ * among the lanes whose tet won the slot one becomes its owner and loads the
+ * record, the others take it from LDS at the start of the next step; lanes
+ * whose tet lost the slot load their own. */
+#define LDSR_WORDS (32u + 4u * 128u + 4u * 512u)
+DEV uint32_t* ldsr_tags(const uint32_t* sperm) {
+  return const_cast<uint32_t*>(sperm) + 32 + (threadIdx.x >> 6) * 128u;
+}
+DEV uint4* ldsr_recs(const uint32_t* sperm) {
+  return reinterpret_cast<uint4*>(const_cast<uint32_t*>(sperm) + 32 + 4 * 128) + (threadIdx.x >> 6) * 128u;
+}
+#define FTB_LSHARED (1u << 22) /* the next tet's record comes through LDS slot (bits >> 16) & 63 */
+#define FTB_LOWNER (1u << 23)  /* ... and this lane loaded it */
+/* The exchange of the records the wave's lanes asked for in their last step:
+ * owners store what they loaded, the other lanes of the slot read it; the
+ * flags are cleared. */
+DEV void ldsr_exchange(const uint32_t* sperm, FtbRay& r) {
+  const uint32_t h = (r.bits >> 16) & 63u;
+  uint4* rr = ldsr_recs(sperm);
+  if (r.bits & FTB_LOWNER) { rr[2 * h] = r.fa; rr[2 * h + 1] = r.fb; }
+  if ((r.bits & (FTB_LSHARED | FTB_LOWNER)) == FTB_LSHARED) { r.fa = rr[2 * h]; r.fb = rr[2 * h + 1]; }
+  r.bits &= 0xFFFFu;
+}
+
+template <int PROJ, bool BIG, bool TFL, bool PSIT, bool LIN = false, bool LDSR = false>
 DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, FtbRay& r) {
   const bool persp = PROJ == VFR_TRAV_MIXED ? ftb_persp(r) : PROJ == VFR_TRAV_PERSP;
   r.steps += 1u;
@@ -1171,6 +1226,26 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
   if (BIG) {
     fan = *reinterpret_cast<const uint4*>(&a.rec[tn].f[0]);
     fbn = *reinterpret_cast<const uint4*>(&a.rec[tn].f[4]);
+  } else if (LDSR) {
+    /* lanes at a boundary or on a skipped step keep to themselves */
+    bool own = true;
+    if (nb < VFR_NB_BROKEN && !skip) {
+      const uint32_t lane = threadIdx.x & 63u, h = tn & 63u;
+      uint32_t* tag = ldsr_tags(sperm);
+      /* the tet that wins slot h, then one lane among those asking for it */
+      tag[h] = tn;
+      if (tag[h] == tn) {
+        tag[64 + h] = lane;
+        own = tag[64 + h] == lane;
+        r.bits |= (h << 16) | FTB_LSHARED | (own ? FTB_LOWNER : 0u);
+      }
+    }
+    fan = r.fa;
+    fbn = r.fb;
+    if (own) {
+      fan = ld32<uint4>(a.rec, tn * 32u);
+      fbn = ld32<uint4>(a.rec, tn * 32u + 16u);
+    }
   } else {
     fan = ld32<uint4>(a.rec, tn * 32u);
     fbn = ld32<uint4>(a.rec, tn * 32u + 16u);
@@ -1286,11 +1361,23 @@ DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
   return o;
 }
 
-template <int PROJ, bool BIG, bool TFL, bool PSIT, bool LIN = false>
+template <int PROJ, bool BIG, bool TFL, bool PSIT, bool LIN = false, bool LDSR = false>
 DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, const uint32_t* sperm, uint32_t vi, uint32_t pix) {
   FtbRay r;
   ftb_init<TFL, LIN>(a, tfl, vi, pix, r);
-  if (!(r.bits & FTB_DEAD)) {
+  if (LDSR) {
+    /* a lane whose ray has just ended takes part in one more exchange, so
+     * that a record it loaded for other lanes reaches them */
+    bool live = !(r.bits & FTB_DEAD);
+    while (true) {
+      ldsr_exchange(sperm, r);
+      if (!live) break;
+      const uint32_t st = ftb_step<PROJ, BIG, TFL, PSIT, LIN, true>(a, tfl, sperm, r);
+      if (st == FTB_GO) continue;
+      if (st == FTB_REENTER && ftb_reenter<TFL>(a, tfl, r)) continue;
+      live = false;
+    }
+  } else if (!(r.bits & FTB_DEAD)) {
     while (true) {
       const uint32_t st = ftb_step<PROJ, BIG, TFL, PSIT, LIN>(a, tfl, sperm, r);
       if (st == FTB_GO || (st == FTB_REENTER && ftb_reenter<TFL>(a, tfl, r))) continue;
@@ -1300,12 +1387,12 @@ DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, const uint32_t* sperm
   return ftb_out<LIN>(a, r);
 }
 
-template <uint32_t MODE, int PROJ, bool BIG, bool TFL, bool PSIT>
+template <uint32_t MODE, int PROJ, bool BIG, bool TFL, bool PSIT, bool LDSR = false>
 DEV RayOut trace(const TravArgs& a, const float4* tfl, const uint32_t* sperm, uint32_t vi, uint32_t pix) {
   if constexpr (MODE == VFR_MODE_REF) return trace_ref<false>(a, vi, pix);
   else if constexpr (MODE == VFR_MODE_REF_LINEAR) return trace_ref<true>(a, vi, pix);
   else if constexpr (MODE == VFR_MODE_FTB_LINEAR) return trace_ftb<PROJ, BIG, TFL, false, true>(a, tfl, sperm, vi, pix);
-  else return trace_ftb<PROJ, BIG, TFL, PSIT>(a, tfl, sperm, vi, pix);
+  else return trace_ftb<PROJ, BIG, TFL, PSIT, false, LDSR>(a, tfl, sperm, vi, pix);
 }
 
 /* Per-ray inputs and outputs are touched once: non-temporal accesses keep
@@ -1322,12 +1409,13 @@ DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
 /* One ray per thread over the compacted ray list. */
 /* TFL: the launch's transfer-function slots [tf_lds_base, +tf_lds_n) are
  * staged in LDS once per block (every view of the launch uses one of them). */
-template <uint32_t MODE, int PROJ, bool BIG, bool TFL, bool PSIT>
+/* LDSR: the FTB walk with LDS-staged tet records (256-thread blocks). */
+template <uint32_t MODE, int PROJ, bool BIG, bool TFL, bool PSIT, bool LDSR = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR_MODE_FTB ? 6 : 5,
                                                                      MODE == VFR_MODE_FTB ? 6 : 5))) void k_trav(
     TravArgs a) {
   __shared__ float4 tfs[TFL ? VFR_TF_LDS_SLOTS * 256 : 1];
-  __shared__ uint32_t sperm[VFR_N_SLOT_PERMS];
+  __shared__ __align__(16) uint32_t sperm[LDSR ? LDSR_WORDS : VFR_N_SLOT_PERMS];
   constexpr bool ftb = MODE == VFR_MODE_FTB || MODE == VFR_MODE_FTB_LINEAR;
   if (ftb) stage_slot_perms(sperm);
   if (TFL) {
@@ -1340,7 +1428,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR
   if (i < n) {
     const uint64_t ray = __builtin_nontemporal_load(a.rays + i);
     const uint32_t vi = (uint32_t)(ray >> 32), pix = (uint32_t)ray;
-    const RayOut o = trace<MODE, PROJ, BIG, TFL, PSIT>(a, tfs, sperm, vi, pix);
+    const RayOut o = trace<MODE, PROJ, BIG, TFL, PSIT, LDSR>(a, tfs, sperm, vi, pix);
     store_ray(a, a.views[vi].pix_off + pix, o);
     st = o.steps; an = o.anomaly; br = o.broken;
     if (o.broken && a.vflags) atomicOr(a.vflags + vi, VFR_VFLAG_BROKEN);
@@ -1525,6 +1613,21 @@ static void launch_wq(uint32_t proj, bool big, bool tfl, hipStream_t st, const T
   else launch_wq1<VFR_TRAV_MIXED, false, false, PSIT>(st, a);
 }
 
+static void launch_ftb_ldsr(uint32_t proj, bool tfl, dim3 grid, dim3 block, hipStream_t st, const TravArgs& a) {
+  if (proj == VFR_TRAV_ORTHO && tfl)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_ORTHO, false, true, false, true>), grid, block, 0, st, a);
+  else if (proj == VFR_TRAV_ORTHO)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_ORTHO, false, false, false, true>), grid, block, 0, st, a);
+  else if (proj == VFR_TRAV_PERSP && tfl)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_PERSP, false, true, false, true>), grid, block, 0, st, a);
+  else if (proj == VFR_TRAV_PERSP)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_PERSP, false, false, false, true>), grid, block, 0, st, a);
+  else if (tfl)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false, true, false, true>), grid, block, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false, false, false, true>), grid, block, 0, st, a);
+}
+
 template <bool PSIT>
 static void launch_ftb(uint32_t proj, bool tfl, dim3 grid, dim3 block, hipStream_t st, const TravArgs& a) {
   if (proj == VFR_TRAV_ORTHO && tfl)
@@ -1587,6 +1690,8 @@ int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t oc
     hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, true, false, false>), grid, block, 0, st, a);
   else if (a.psi_table)
     launch_ftb<true>(proj, tfl, grid, block, st, a);
+  else if (a.lds_records)
+    launch_ftb_ldsr(proj, tfl, grid, block, st, a);
   else
     launch_ftb<false>(proj, tfl, grid, block, st, a);
   CHECK_HIP(hipGetLastError());

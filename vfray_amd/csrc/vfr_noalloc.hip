@@ -271,6 +271,7 @@ extern "C" int vfr_render_noalloc(vfr_ctx* c, const vfr_mesh* m, const vfr_view*
   if (tf_hi - tf_lo < VFR_TF_LDS_SLOTS) { a.tf_lds_base = tf_lo; a.tf_lds_n = tf_hi - tf_lo + 1; }
   a.composite = o.composite;
   a.psi_table = (o.flags & VFR_RENDER_PSI_TABLE) ? 1u : 0u;
+  a.lds_records = (o.flags & VFR_RENDER_LDS_RECORDS) ? 1u : 0u;
   const uint32_t proj = persp == 0 ? VFR_TRAV_ORTHO : (persp == (uint32_t)n_views ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
   const int kern = o.mode != VFR_MODE_FTB ? VFR_WALK_THREAD
                    : (o.flags & VFR_RENDER_PERSISTENT) ? VFR_WALK_REFILL

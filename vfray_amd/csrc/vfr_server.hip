@@ -232,6 +232,7 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   a.kmax = VFR_K_MAX;
   a.vflags = s->vflags;
   a.psi_table = (sv->cfg.flags & VFR_RENDER_PSI_TABLE) ? 1u : 0u;
+  a.lds_records = (sv->cfg.flags & VFR_RENDER_LDS_RECORDS) ? 1u : 0u;
   a.composite = sv->cfg.composite;
   if (tf_hi - tf_lo < VFR_TF_LDS_SLOTS) { a.tf_lds_base = tf_lo; a.tf_lds_n = tf_hi - tf_lo + 1; }
   const uint32_t proj = n_persp == 0 ? VFR_TRAV_ORTHO : (n_persp == nv ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
