@@ -367,6 +367,17 @@ void vfr_rotate_volume(float* xyzs, uint32_t n_verts, const float mv[16]); /* :8
 void vfr_gl_rotation(float rotx_deg, float roty_deg, float mv[16]);
 int vfr_read_con(const char* path, uint32_t n_tets, uint32_t** con,
                  uint32_t* n_ext);                             /* :375-419 */
+/* readIncid (:153-212): the per-vertex incident-tet lists of a .incid file as
+ * CSR arrays (offsets: n_verts + 1 entries, tet_ids), malloc'ed. */
+int vfr_read_incid(const char* path, uint32_t n_verts, uint32_t** offsets,
+                   uint32_t** tet_ids);
+/* buildCon (:424-501) over such lists, on the host: con_out (n_tets*4) and
+ * the number of external faces, exactly as the reference derives them from
+ * the lists it read (the GPU sort-and-match of vfr_mesh_create gives the
+ * same result for the lists buildIncid writes). */
+int vfr_con_from_incid(const uint32_t* tets, uint32_t n_tets, uint32_t n_verts,
+                       const uint32_t* offsets, const uint32_t* tet_ids,
+                       uint32_t* con_out, uint32_t* n_ext_out);
 int vfr_write_con(const char* path, const uint32_t* con, uint32_t n_tets,
                   uint32_t n_ext);                             /* :507-543 */
 int vfr_read_extf(const char* path, uint32_t n_ext, uint32_t** extf); /* :668-705 */

@@ -18,8 +18,16 @@ Run in a container that has the reference tree; the fixtures are data only.
                   viewer's space key at angles (30, 20) (rotateVolume +
                   normalizeVertices, limits kept): the vertices, per-pixel
                   owner faces and the 128x128 strict-IEEE image.
+  ref_cache.json  for each partial or damaged cache state of cache_states.py:
+  ref_cache.npz   the reference CLI's exit status, its cache-related messages
+                  and the sha256 of every cache file after the run (the
+                  -ffast-math build), and for the states that change the image the
+                  128x128 strict-IEEE image and owner faces; plus a
+                  non-manifold mesh (a face shared by three tets) with a
+                  .incid whose lists are out of order, and the .con the
+                  reference builds from it.
 
-  python3 tests/golden/make_golden.py [cleantf|rotated]   (all, or one)
+  python3 tests/golden/make_golden.py [cleantf|rotated|caches]   (all, or one)
 """
 from __future__ import annotations
 
@@ -37,6 +45,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, HERE)
 
 import oracle as O  # noqa: E402
 import vfray_amd as V  # noqa: E402
@@ -94,12 +103,85 @@ def rotated() -> None:
                         n_vis=np.array([h["info"]["numVisFaces"]], np.uint32))
 
 
+IMAGE_STATES = ("tf_255", "extf_stale")
+
+
+def nonmanifold_mesh():
+    """Five tets: three share the face (0, 1, 2) (a non-manifold mesh), one
+    more on each side elsewhere; returns (xyzs, tets, incid text with some
+    tet lists in descending order)."""
+    x = np.array([[0, 0, 0, 0.1], [1, 0, 0, 0.3], [0, 1, 0, 0.5], [0.3, 0.3, 1, 0.7], [0.3, 0.3, -1, 0.9],
+                  [0.2, 0.2, 0.6, 0.2], [1, 1, 0.5, 0.4], [-0.5, 0.2, 0.5, 0.6]], np.float32)
+    t = np.array([[0, 1, 2, 3], [0, 2, 1, 4], [1, 0, 2, 5], [1, 2, 3, 6], [2, 0, 3, 7]], np.uint32)
+    nv = x.shape[0]
+    tl = [[j for j in range(t.shape[0]) if i in t[j]] for i in range(nv)]
+    vl = [sorted({int(v) for j in tl[i] for v in t[j] if v != i}) for i in range(nv)]
+    lines = [str(nv)]
+    for i in range(nv):
+        ts = tl[i][::-1] if i in (0, 2, 3) else tl[i]
+        lines.append(" ".join([str(len(ts))] + [str(j) for j in ts]))
+        lines.append(" ".join([str(len(vl[i]))] + [str(v) for v in vl[i]]))
+    return x, t, "\n".join(lines) + "\n"
+
+
+def caches() -> None:
+    import subprocess
+    import shutil
+    from cache_states import STATES, CACHES, make_state, cache_lines
+    assert O.ref_available("strict"), "build oracle/_ref first"
+    canonical = dict(np.load(os.path.join(HERE, "ref_small.npz")))
+    # the reference's own build flags (-O3 -ffast-math): with strict IEEE its
+    # 512x512 walk meets a ray that circles an edge on this mesh and never ends
+    exe = os.path.join(ROOT, "oracle", "_ref", "fast", "vfRayCPU")
+    out, arrays = {}, {}
+    with tempfile.TemporaryDirectory() as td:
+        x, t, _, _ = WL.plain_cpu(8)
+        V.write_off(os.path.join(td, "s8.off"), x, t)
+        off = open(os.path.join(td, "s8.off"), "rb").read()
+        for st in STATES:
+            wd = os.path.join(td, st)
+            make_state(st, os.path.join(wd, "tet_offs"), canonical, off)
+            if st in IMAGE_STATES:
+                shutil.copytree(os.path.join(wd, "tet_offs"), os.path.join(wd, "h"))
+                h = O.run_ref_harness("strict", os.path.join(wd, "h", "s8"), 128, 128)
+                arrays[st + "_rgb128"] = h["rgb8"]
+                arrays[st + "_owner128"] = h["entry_ext"]
+            os.makedirs(os.path.join(wd, "run"))
+            r = subprocess.run([exe, "s8"], cwd=os.path.join(wd, "run"), capture_output=True, text=True,
+                               timeout=600)
+            files = {}
+            for ext in CACHES:
+                fp = os.path.join(wd, "tet_offs", "s8." + ext)
+                if os.path.exists(fp):
+                    files[ext] = hashlib.sha256(open(fp, "rb").read()).hexdigest()
+            out[st] = {"rc": r.returncode, "lines": cache_lines(r.stdout), "files": files}
+        # buildCon over out-of-order lists of a non-manifold mesh
+        xm, tm, incid = nonmanifold_mesh()
+        wd = os.path.join(td, "nm")
+        os.makedirs(os.path.join(wd, "tet_offs"))
+        os.makedirs(os.path.join(wd, "run"))
+        V.write_off(os.path.join(wd, "tet_offs", "nm.off"), xm, tm)
+        open(os.path.join(wd, "tet_offs", "nm.incid"), "w").write(incid)
+        subprocess.run([exe, "nm"], cwd=os.path.join(wd, "run"), capture_output=True, text=True, timeout=600)
+        con_txt = open(os.path.join(wd, "tet_offs", "nm.con")).read().split()
+        arrays["nm_xyzs"] = xm
+        arrays["nm_tets"] = tm
+        arrays["nm_incid"] = np.frombuffer(incid.encode(), np.uint8)
+        arrays["nm_con"] = np.array([int(v) for v in con_txt[2:]], np.uint32).reshape(-1, 4)
+        arrays["nm_n_ext"] = np.array([int(con_txt[0])], np.uint32)
+    json.dump(out, open(os.path.join(HERE, "ref_cache.json"), "w"), indent=1)
+    np.savez_compressed(os.path.join(HERE, "ref_cache.npz"), **arrays)
+
+
 def main() -> None:
     if sys.argv[1:] == ["cleantf"]:
         cleantf()
         return
     if sys.argv[1:] == ["rotated"]:
         rotated()
+        return
+    if sys.argv[1:] == ["caches"]:
+        caches()
         return
     cleantf()
     rotated()
@@ -150,6 +232,7 @@ def main() -> None:
                "deviations": [[int(i), int(j), float(pr[i, j])] for i, j in idx],
                "max_other_abs_diff": float(d[d <= 1e-6].max())},
               open(os.path.join(HERE, "psi_ref.json"), "w"))
+    caches()
     print("golden fixtures written")
 
 

@@ -199,3 +199,47 @@ def test_cleantf_errors():
                 assert os.path.exists(rout) == ok
                 if ok:
                     assert open(rout, "rb").read() == open(out, "rb").read()
+
+
+CACHE_STATES = ("con_count_incid", "con_count_no_incid", "incid_count", "tf_255", "extf_count", "extf_stale")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("state", CACHE_STATES)
+def test_cli_cache_states_match_reference(state):
+    """Partial and damaged cache files (tests/golden/cache_states.py): started
+    from the same state as the reference's CLI, the command prints the same
+    cache messages in the same order and leaves every cache file byte for byte
+    as the reference does: .con rebuilt from a readable .incid (count header
+    off by one), .incid rebuilt when unreadable, the default TF written over a
+    255-entry .tf, .extf rebuilt when its count is off, and a reversed .extf
+    with the right count kept after a .con rebuild (its bytes are what tell
+    a kept file from a rebuilt one).  For the default-TF and reversed-.extf
+    states the 128x128 image matches the reference's strict-IEEE image within
+    1 where the psi tables differ."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from cache_states import make_state, cache_lines
+    ref = json.load(open(os.path.join(ROOT, "tests", "golden", "ref_cache.json")))[state]
+    imgs = dict(np.load(os.path.join(ROOT, "tests", "golden", "ref_cache.npz")))
+    gold = dict(np.load(GOLD))
+    x, t, _, _ = WL.plain_cpu(8)
+    with tempfile.TemporaryDirectory() as td:
+        V.write_off(os.path.join(td, "s8.off"), x, t)
+        off = open(os.path.join(td, "s8.off"), "rb").read()
+        d = os.path.join(td, "tet_offs")
+        make_state(state, d, gold, off)
+        print(f"CLI cache state {state}", flush=True)
+        r = _run(["--dir", d, "s8"], td)
+        assert r.returncode == ref["rc"] == 0, r.stdout + r.stderr
+        assert cache_lines(r.stdout) == ref["lines"]
+        for ext, h in ref["files"].items():
+            assert hashlib.sha256(open(os.path.join(d, "s8." + ext), "rb").read()).hexdigest() == h, ext
+        if state + "_rgb128" in imgs:
+            d2 = os.path.join(td, "again")
+            make_state(state, d2, gold, off)
+            r = _run(["--dir", d2, "--size", "128", "128", "s8"], td)
+            assert r.returncode == 0, r.stdout + r.stderr
+            img = np.frombuffer(open(os.path.join(td, "s8_gpu_img.ppm"), "rb").read(), np.uint8)[15:]
+            diff = np.abs(img.reshape(128, 128, 3).astype(int) - imgs[state + "_rgb128"].astype(int))
+            assert diff.max() <= 1 and (diff > 0).any(-1).mean() < 0.01

@@ -150,6 +150,8 @@ def lib() -> C.CDLL:
             "vfr_gl_rotation": (None, [C.c_float, C.c_float, f32p]),
             "vfr_clean_tf": (C.c_int, [C.c_char_p, C.c_char_p]),
             "vfr_read_con": (C.c_int, [C.c_char_p, u32, C.POINTER(u32p), u32p]),
+            "vfr_read_incid": (C.c_int, [C.c_char_p, u32, C.POINTER(u32p), C.POINTER(u32p)]),
+            "vfr_con_from_incid": (C.c_int, [u32p, u32, u32, u32p, u32p, u32p, u32p]),
             "vfr_write_con": (C.c_int, [C.c_char_p, u32p, u32, u32]),
             "vfr_read_extf": (C.c_int, [C.c_char_p, u32, C.POINTER(u32p)]),
             "vfr_write_extf": (C.c_int, [C.c_char_p, u32p, u32]),
@@ -267,6 +269,31 @@ def rotate_volume(xyzs: np.ndarray, mv) -> np.ndarray:
     m = np.ascontiguousarray(mv, np.float32).reshape(16)
     lib().vfr_rotate_volume(fptr(x), x.shape[0], fptr(m))
     return x
+
+
+def read_incid(path: str, n_verts: int) -> tuple[np.ndarray, np.ndarray]:
+    """The per-vertex tet lists of a .incid file: (offsets [n_verts + 1], ids)."""
+    L = lib()
+    op, ip = C.POINTER(C.c_uint32)(), C.POINTER(C.c_uint32)()
+    _check(L.vfr_read_incid(path.encode(), n_verts, C.byref(op), C.byref(ip)))
+    try:
+        off = np.ctypeslib.as_array(op, shape=(n_verts + 1,)).copy()
+        ids = np.ctypeslib.as_array(ip, shape=(max(1, int(off[-1])),)).copy()[: int(off[-1])]
+    finally:
+        L.vfr_free(C.cast(op, C.c_void_p))
+        L.vfr_free(C.cast(ip, C.c_void_p))
+    return off, ids
+
+
+def con_from_incid(tets: np.ndarray, n_verts: int, offsets: np.ndarray, ids: np.ndarray) -> tuple[np.ndarray, int]:
+    """The reference's buildCon over incidence lists (host): (con [nT, 4], n_ext)."""
+    t = np.ascontiguousarray(tets, np.uint32)
+    o = np.ascontiguousarray(offsets, np.uint32)
+    i = np.ascontiguousarray(ids if ids.size else np.zeros(1), np.uint32)
+    con = np.zeros((t.shape[0], 4), np.uint32)
+    n = C.c_uint32()
+    _check(lib().vfr_con_from_incid(uptr(t), t.shape[0], n_verts, uptr(o), uptr(i), uptr(con), C.byref(n)))
+    return con, n.value
 
 
 def read_tf(path: str) -> np.ndarray:

@@ -281,6 +281,83 @@ extern "C" int vfr_write_incid(const char* path, const uint32_t* tets, uint32_t 
   return fclose(f) == 0 ? VFR_OK : VFR_EIO;
 }
 
+/* readIncid, vfRayPreComp.cc:153-212: the vertex count (must equal the
+ * mesh's), then per vertex the number of incident tets and their ids, the
+ * number of neighbouring vertices and their ids.  The tet lists come back as
+ * CSR arrays (offsets n_verts + 1, ids); the vertex lists are parsed and
+ * dropped, as buildCon does not use them. */
+extern "C" int vfr_read_incid(const char* path, uint32_t n_verts, uint32_t** offsets, uint32_t** tet_ids) {
+  if (!path || !offsets || !tet_ids) { vfr_set_error("vfr_read_incid: null argument"); return VFR_EINVAL; }
+  Tok t(path);
+  if (!t.ok) { vfr_set_error("cannot open %s", path); return VFR_EIO; }
+  uint32_t n;
+  if (!t.u32v(&n)) { vfr_set_error("%s: bad header", path); return VFR_EIO; }
+  if (n != n_verts) { vfr_set_error("%s: %u vertices, mesh has %u", path, n, n_verts); return VFR_EIO; }
+  std::vector<uint32_t> off((size_t)n_verts + 1, 0), ids;
+  for (uint32_t i = 0; i < n_verts; ++i) {
+    uint32_t nt, nv, x;
+    if (!t.u32v(&nt)) { vfr_set_error("%s: bad tet count of vertex %u", path, i); return VFR_EIO; }
+    for (uint32_t j = 0; j < nt; ++j) {
+      if (!t.u32v(&x)) { vfr_set_error("%s: bad tet id of vertex %u", path, i); return VFR_EIO; }
+      ids.push_back(x);
+    }
+    off[i + 1] = (uint32_t)ids.size();
+    if (!t.u32v(&nv)) { vfr_set_error("%s: bad vertex count of vertex %u", path, i); return VFR_EIO; }
+    for (uint32_t j = 0; j < nv; ++j)
+      if (!t.u32v(&x)) { vfr_set_error("%s: bad vertex id of vertex %u", path, i); return VFR_EIO; }
+  }
+  uint32_t* o = (uint32_t*)malloc(sizeof(uint32_t) * off.size());
+  uint32_t* d = (uint32_t*)malloc(sizeof(uint32_t) * ids.size() + 16);
+  if (!o || !d) { free(o); free(d); vfr_set_error("out of memory"); return VFR_ENOMEM; }
+  memcpy(o, off.data(), sizeof(uint32_t) * off.size());
+  if (!ids.empty()) memcpy(d, ids.data(), sizeof(uint32_t) * ids.size());
+  *offsets = o;
+  *tet_ids = d;
+  return VFR_OK;
+}
+
+/* buildCon, vfRayPreComp.cc:424-501, over the incidence lists of a .incid
+ * file: for each tet and face f (vertices a, b, c of slots f, f+1, f+2), the
+ * first tet other than itself in a's list that the lists of b and c also
+ * hold, each of those two scanned in file order and given up at the first id
+ * above the candidate; the tet itself when none (an external face, counted).
+ * With the lists buildIncid writes (ascending) this is the smallest other tet
+ * holding the face; with other lists it is what the reference computes from
+ * them. */
+extern "C" int vfr_con_from_incid(const uint32_t* tets, uint32_t n_tets, uint32_t n_verts, const uint32_t* offsets,
+                                  const uint32_t* tet_ids, uint32_t* con_out, uint32_t* n_ext_out) {
+  if (!tets || !offsets || !con_out || !n_ext_out) { vfr_set_error("vfr_con_from_incid: null argument"); return VFR_EINVAL; }
+  uint32_t n_ext = 0;
+  for (uint32_t i = 0; i < n_tets; ++i)
+    for (uint32_t f = 0; f < 4; ++f) {
+      uint32_t v[3];
+      for (uint32_t k = 0; k < 3; ++k) {
+        v[k] = tets[(size_t)i * 4 + ((k + f) & 3u)];
+        if (v[k] >= n_verts) { vfr_set_error("tet %u references vertex %u >= %u", i, v[k], n_verts); return VFR_EINVAL; }
+      }
+      uint32_t& c = con_out[(size_t)i * 4 + f];
+      c = i;
+      for (uint32_t j = offsets[v[0]]; j < offsets[v[0] + 1]; ++j) {
+        const uint32_t cur = tet_ids[j];
+        if (cur == i) continue;
+        for (uint32_t k = offsets[v[1]]; k < offsets[v[1] + 1]; ++k) {
+          if (tet_ids[k] > cur) break;
+          if (tet_ids[k] == cur) {
+            for (uint32_t l = offsets[v[2]]; l < offsets[v[2] + 1]; ++l) {
+              if (tet_ids[l] > cur) break;
+              if (tet_ids[l] == cur) { c = cur; break; }
+            }
+            if (c == cur) break;
+          }
+        }
+        if (c == cur) break;
+      }
+      if (c == i) ++n_ext;
+    }
+  *n_ext_out = n_ext;
+  return VFR_OK;
+}
+
 /* readCon, vfRayPreComp.cc:375-419: header numExtFaces, numTets (must equal
  * the mesh's), then 4 ids per tet. */
 extern "C" int vfr_read_con(const char* path, uint32_t n_tets, uint32_t** con, uint32_t* n_ext) {

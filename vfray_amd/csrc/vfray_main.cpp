@@ -6,11 +6,14 @@
  *         [--composite ppi|linear|linear-opacity] [--rotate RX RY]... <volume>
  *
  * With no options it behaves as the reference: reads D<volume>.off with
- * D = "../tet_offs/", normalizes, then for .con (.incid on the way), .tf,
- * .extf and .lmt reads the cache file or builds and writes it, in that order
- * (.extf's count is checked against .con's header); renders 512x512 along -z
- * and writes <volume>_gpu_img.ppm in the working directory.  Connectivity and
- * external faces are built on the GPU.
+ * D = "../tet_offs/", normalizes, then for .con, .tf, .extf and .lmt reads
+ * the cache file or builds and writes it, in that order and with the
+ * reference's messages.  A missing or unreadable .con is built from the
+ * .incid file when that reads (the reference's buildCon over its lists), else
+ * the .incid file is written and the connectivity built on the GPU by
+ * sort-and-match; .extf is read whenever its count equals the .con's, also
+ * after a .con rebuild, else built from .con.  Then it renders 512x512 along
+ * -z and writes <volume>_gpu_img.ppm in the working directory.
  *
  * Each --rotate RX RY is one press of the CUDA viewer's space key with
  * rotation angles RX, RY degrees (vfRay.cu:617-640): the mesh is rotated by
@@ -80,23 +83,42 @@ int main(int argc, char** argv) {
   vfr_ctx* ctx = nullptr;
   if (vfr_ctx_create(0, &ctx) != VFR_OK) return fail("device");
 
-  /* .con (and .incid) */
+  /* .con: read it, or build it from the .incid file (read, or built and
+   * written first) as main.cc:271-299 does */
   uint32_t *con = nullptr, n_ext = 0;
   const std::string fnCon = base + ".con";
   printf("Reading %s file\n", fnCon.c_str());
-  bool con_built = vfr_read_con(fnCon.c_str(), nT, &con, &n_ext) != VFR_OK;
-  vfr_mesh* mesh = nullptr;
-  if (con_built) {
+  if (vfr_read_con(fnCon.c_str(), nT, &con, &n_ext) != VFR_OK) {
     printf("%s file not found!\n Computing conTet\n", fnCon.c_str());
+    con = (uint32_t*)malloc(sizeof(uint32_t) * 4 * (size_t)nT + 16);
+    if (!con) return fail("memory");
     const std::string fnIncid = base + ".incid";
-    FILE* fi = fopen(fnIncid.c_str(), "r");
-    if (fi) {
-      fclose(fi);
+    printf("Reading %s file\n", fnIncid.c_str());
+    uint32_t *off = nullptr, *ids = nullptr;
+    if (vfr_read_incid(fnIncid.c_str(), nV, &off, &ids) == VFR_OK) {
+      /* buildCon over the lists the file holds, on the host */
+      int rc = vfr_con_from_incid(tets, nT, nV, off, ids, con, &n_ext);
+      vfr_free(off);
+      vfr_free(ids);
+      if (rc != VFR_OK) return fail("con");
     } else {
       printf("%s file not found!\n Building incid\n", fnIncid.c_str());
       printf("Writing %s file\n", fnIncid.c_str());
       if (vfr_write_incid(fnIncid.c_str(), tets, nT, nV) != VFR_OK) return fail("incid");
+      /* the same connectivity by sort-and-match on the device */
+      vfr_mesh* tmp = nullptr;
+      if (vfr_mesh_create(ctx, xyzs, nV, tets, nT, nullptr, nullptr, 0, nullptr, 0, &tmp) != VFR_OK)
+        return fail("connectivity");
+      vfr_mesh_info ti;
+      vfr_mesh_get_info(tmp, &ti);
+      n_ext = ti.n_ext_faces;
+      int rc = vfr_mesh_connectivity(tmp, con, nullptr, nullptr);
+      vfr_mesh_destroy(tmp);
+      if (rc != VFR_OK) return fail("connectivity");
     }
+    printf("Writing %s file\n", fnCon.c_str());
+    if (vfr_write_con(fnCon.c_str(), con, nT, n_ext) != VFR_OK) return fail("con");
+    printf("Deleting incidents in vertices\n");
   }
   /* .tf */
   float tf[1024];
@@ -108,51 +130,45 @@ int main(int argc, char** argv) {
     printf("Writing %s file\n", fnTF.c_str());
     if (vfr_write_tf(fnTF.c_str(), tf) != VFR_OK) return fail("tf");
   }
-  /* .extf: needs numExtFaces from .con; build on the device if absent */
+  /* .extf: read whenever it holds as many faces as .con says, also after a
+   * .con rebuild (main.cc:312-328); otherwise buildExtF's scan of .con */
   uint32_t* extf = nullptr;
   const std::string fnExtF = base + ".extf";
-  bool extf_built = true;
-  if (!con_built) {
-    printf("Reading %s file\n", fnExtF.c_str());
-    extf_built = vfr_read_extf(fnExtF.c_str(), n_ext, &extf) != VFR_OK;
+  printf("Reading %s file\n", fnExtF.c_str());
+  if (vfr_read_extf(fnExtF.c_str(), n_ext, &extf) != VFR_OK) {
+    printf("%s file not found!\n Computing extFaces\n", fnExtF.c_str());
+    std::vector<uint32_t> e;
+    for (uint32_t t = 0; t < nT; ++t)
+      for (uint32_t f = 0; f < 4; ++f)
+        if (con[(size_t)t * 4 + f] == t) { e.push_back(t); e.push_back(f); }
+    n_ext = (uint32_t)(e.size() / 2);
+    extf = (uint32_t*)malloc(sizeof(uint32_t) * e.size() + 16);
+    if (!extf) return fail("memory");
+    if (!e.empty()) memcpy(extf, e.data(), sizeof(uint32_t) * e.size());
+    printf("Writing %s file\n", fnExtF.c_str());
+    if (vfr_write_extf(fnExtF.c_str(), extf, n_ext) != VFR_OK) return fail("extf");
   }
   /* .lmt */
   float lmt[3];
   const std::string fnLmt = base + ".lmt";
-  bool lmt_built = vfr_read_lmt(fnLmt.c_str(), lmt) != VFR_OK;
-  if (lmt_built) vfr_build_lmt(xyzs, nV, tets, nT, lmt);
-
-  if (vfr_mesh_create(ctx, xyzs, nV, tets, nT, con, extf_built ? nullptr : extf, n_ext, lmt,
-                      VFR_MESH_MORTON, &mesh) != VFR_OK)
-    return fail("mesh");
-  vfr_mesh_info info;
-  vfr_mesh_get_info(mesh, &info);
-  if (con_built) {
-    std::vector<uint32_t> c((size_t)nT * 4);
-    vfr_mesh_connectivity(mesh, c.data(), nullptr, nullptr);
-    printf("Writing %s file\n", fnCon.c_str());
-    if (vfr_write_con(fnCon.c_str(), c.data(), nT, info.n_ext_faces) != VFR_OK) return fail("con");
-    printf("Deleting incidents in vertices\n");
-    printf("Reading %s file\n", fnExtF.c_str());
-  }
-  if (extf_built) {
-    printf("%s file not found!\n Computing extFaces\n", fnExtF.c_str());
-    std::vector<uint32_t> e((size_t)info.n_ext_faces * 2);
-    vfr_mesh_connectivity(mesh, nullptr, e.data(), nullptr);
-    printf("Writing %s file\n", fnExtF.c_str());
-    if (vfr_write_extf(fnExtF.c_str(), e.data(), info.n_ext_faces) != VFR_OK) return fail("extf");
-  }
   printf("Reading %s file\n", fnLmt.c_str());
-  if (lmt_built) {
+  if (vfr_read_lmt(fnLmt.c_str(), lmt) != VFR_OK) {
     printf("%s file not found!\n Computing limits\n", fnLmt.c_str());
+    vfr_build_lmt(xyzs, nV, tets, nT, lmt);
     printf("Writing %s file\n", fnLmt.c_str());
     if (vfr_write_lmt(fnLmt.c_str(), lmt) != VFR_OK) return fail("lmt");
   }
   printf("Volume dataset information: %s\n", fnOff.c_str());
   printf("  # Tets: %d ; # Verts: %d\n", nT, nV);
-  printf("  # ExtFaces: %d\n", info.n_ext_faces);
+  printf("  # ExtFaces: %d\n", n_ext);
   printf("  Max Edge Length: %f\n", lmt[0]);
   printf("  Max Z: %f ; Min Z: %f\n", lmt[1], lmt[2]);
+
+  vfr_mesh* mesh = nullptr;
+  if (vfr_mesh_create(ctx, xyzs, nV, tets, nT, con, extf, n_ext, lmt, VFR_MESH_MORTON, &mesh) != VFR_OK)
+    return fail("mesh");
+  vfr_mesh_info info;
+  vfr_mesh_get_info(mesh, &info);
   printf("  Device preprocessing: %.2f ms\n", info.build_ms);
 
   for (size_t k = 0; k < rot.size(); k += 2) {
