@@ -146,7 +146,7 @@ def serve_measure(ctx, args) -> dict:
     sp.close()
     log(f"plain capacity {plain['mrays_per_s']:.1f} Mrays/s over {len(sample)} requests")
     sb = V.Server(ctx, meshes, hbm_budget=32 << 30, streams=args.streams, max_batch=256,
-                  max_batch_pixels=args.batch_pixels, **sched)
+                  max_batch_pixels=args.batch_pixels, schedule=args.serve_schedule, **sched)
     sb.replay(sample[:64], np.zeros(64))
     cap = sb.replay(sample, np.zeros(len(sample)))
     log(f"batched capacity {cap['mrays_per_s']:.1f} Mrays/s, mean batch {cap['mean_batch']:.1f}")
@@ -187,6 +187,7 @@ def serve_measure(ctx, args) -> dict:
         "slo_search": tried,
         "stream": dict(small(stream), load=args.load, offered_req_per_s=round(rate, 1)),
         "streams": args.streams, "max_batch_pixels": args.batch_pixels, "schedule": sched,
+        "serve_schedule": ["slots", "pipeline"][args.serve_schedule],
     }
     return out
 
@@ -229,6 +230,7 @@ def main() -> int:
     ap.add_argument("--load", type=float, default=0.8, help="serve: offered load / batched capacity")
     ap.add_argument("--streams", type=int, default=2, help="serve: concurrent batches")
     ap.add_argument("--batch-pixels", type=int, default=4 << 20, help="serve: pixels per batch (0 = no cap)")
+    ap.add_argument("--serve-schedule", type=int, default=0, help="serve: 0 slots, 1 pipeline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")

@@ -289,12 +289,26 @@ int vfr_render_noalloc(vfr_ctx* ctx, const vfr_mesh* mesh, const vfr_view* views
 
 /* --------------------------------------------------------------- serving */
 /* A render server keeps meshes resident on the device and renders a stream of
- * requests with continuous batching: requests queue on arrival; whenever one
- * of `streams` slots is free, the queued requests that fit the slot's share
- * of the HBM budget (front of the queue first, up to max_batch) are rendered
- * together in one batched launch set on that slot's stream, so batches that
- * arrive while others run overlap on the device.  No counterpart in the
- * reference (one 512x512 frame per process, main.cc:242-380). */
+ * requests with continuous batching.  Two schedules:
+ *
+ *  VFR_SERVE_SLOTS     requests queue on arrival; whenever one of `streams`
+ *                      slots is free, the queued requests that fit the slot's
+ *                      share of the HBM budget (front of the queue first, up
+ *                      to max_batch) are rendered together in one batched
+ *                      launch set on that slot's stream.
+ *  VFR_SERVE_PIPELINE  requests are admitted as soon as their buffers fit in
+ *                      the server's shared arenas: their preprocessing
+ *                      (vertex transform, visibility, entry lists, ray
+ *                      compaction) runs at once on a preprocessing stream and
+ *                      appends their rays to the open walk list, while up to
+ *                      `streams` earlier walk lists walk; whenever a walk
+ *                      stream is free the open list is walked in one launch,
+ *                      however many requests joined it meanwhile.
+ *
+ * No counterpart in the reference (one 512x512 frame per process,
+ * main.cc:242-380). */
+#define VFR_SERVE_SLOTS 0u
+#define VFR_SERVE_PIPELINE 1u
 typedef struct vfr_server_cfg {
   uint64_t hbm_budget;  /* bytes for per-request buffers, all slots (0 = 16 GiB) */
   uint32_t streams;     /* slots = concurrent batches (0 = 2)                    */
@@ -307,8 +321,11 @@ typedef struct vfr_server_cfg {
   uint32_t refill;
   uint32_t composite;   /* VFR_COMPOSITE_*, for every request (0 = PPI); the
                            linear modes use the one-ray-per-thread walk      */
-  uint64_t max_batch_pixels; /* pixels per batch (0 = as the buffers allow); a
-                           request larger than the cap still runs, alone     */
+  uint64_t max_batch_pixels; /* pixels per batch (slots) or per walk list
+                           (pipeline); 0 = as the buffers allow; a request
+                           larger than the cap still runs, alone             */
+  uint32_t schedule;    /* VFR_SERVE_SLOTS (0) or VFR_SERVE_PIPELINE; with the
+                           pipeline `streams` is the number of walks in flight */
 } vfr_server_cfg;
 
 typedef struct vfr_request {

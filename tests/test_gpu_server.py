@@ -34,11 +34,12 @@ def _requests(tfs, n, sides, seed=0x5EED0410):
     return reqs
 
 
-def test_server_alone_equals_mix(ctx, two_meshes):
+@pytest.mark.parametrize("schedule", [V.SERVE_SLOTS, V.SERVE_PIPELINE])
+def test_server_alone_equals_mix(ctx, two_meshes, schedule):
     ms, tfs = two_meshes
     reqs = _requests(tfs, 12, [64, 96, 128, 160])
     print("server mix of 12 requests on 2 meshes", flush=True)
-    sv = V.Server(ctx, ms, hbm_budget=1 << 30, streams=2, max_batch=8)
+    sv = V.Server(ctx, ms, hbm_budget=1 << 30, streams=2, max_batch=8, schedule=schedule)
     try:
         tickets = [sv.submit(m, v, keep_image=True) for m, v in reqs]
         got = [sv.wait(t, image=True) for t in tickets]
@@ -65,11 +66,12 @@ def test_server_rejects_over_budget(ctx, two_meshes):
         sv.close()
 
 
-def test_server_replay_report(ctx, two_meshes):
+@pytest.mark.parametrize("schedule", [V.SERVE_SLOTS, V.SERVE_PIPELINE])
+def test_server_replay_report(ctx, two_meshes, schedule):
     ms, tfs = two_meshes
     reqs = [V.Request(m, 0, v) for m, v in _requests(tfs, 40, [32, 64, 96])]
     print("server replay of 40 requests", flush=True)
-    sv = V.Server(ctx, ms, hbm_budget=1 << 30, streams=2, max_batch=16)
+    sv = V.Server(ctx, ms, hbm_budget=1 << 30, streams=2, max_batch=16, schedule=schedule)
     try:
         burst = sv.replay(reqs, np.zeros(len(reqs)))
         spaced = sv.replay(reqs, np.arange(len(reqs)) * 2e-3)
@@ -105,13 +107,15 @@ def test_server_linear_compositing(ctx, two_meshes, composite):
         assert not np.array_equal(g["rgba"], ppi[0]["rgba"])
 
 
-def test_server_batch_pixel_cap(ctx, two_meshes):
+@pytest.mark.parametrize("schedule", [V.SERVE_SLOTS, V.SERVE_PIPELINE])
+def test_server_batch_pixel_cap(ctx, two_meshes, schedule):
     """max_batch_pixels bounds every batch (a request above the cap still
     runs, alone) and does not change any image."""
     ms, tfs = two_meshes
     reqs = _requests(tfs, 10, [64, 64, 96, 160], seed=0x5EED0430)
     print("server batch pixel cap: 10 requests", flush=True)
-    sv = V.Server(ctx, ms, hbm_budget=1 << 30, streams=1, max_batch=16, max_batch_pixels=2 * 64 * 64)
+    sv = V.Server(ctx, ms, hbm_budget=1 << 30, streams=1, max_batch=16, max_batch_pixels=2 * 64 * 64,
+                  schedule=schedule)
     try:
         tickets = [sv.submit(m, v, keep_image=True) for m, v in reqs]
         got = [sv.wait(t, image=True) for t in tickets]
@@ -125,7 +129,8 @@ def test_server_batch_pixel_cap(ctx, two_meshes):
         assert np.array_equal(g["rgba"], alone[0]["rgba"])
 
 
-def test_server_concave_and_dropped_entries(ctx):
+@pytest.mark.parametrize("schedule", [V.SERVE_SLOTS, V.SERVE_PIPELINE])
+def test_server_concave_and_dropped_entries(ctx, schedule):
     """Requests on the concave benchmark mesh (96^3 grid, graded, 12 voids:
     pixels with up to 13 entry faces, more than 8 slots hold) render exactly
     as the per-view walk; a request whose rays run along the boundary edges
@@ -142,7 +147,7 @@ def test_server_concave_and_dropped_entries(ctx):
     with pytest.raises(V.VfrError, match="dropped"):
         ctx.render(mc, views[:1], mode=V.MODE_FTB, max_entries=8)
     edge = V.ortho_view(2, 2, tf)
-    sv = V.Server(ctx, [mc, mg], hbm_budget=2 << 30, streams=2, max_batch=8)
+    sv = V.Server(ctx, [mc, mg], hbm_budget=2 << 30, streams=2, max_batch=8, schedule=schedule)
     try:
         tickets = [sv.submit(0, v, keep_image=True) for v in views]
         tbad = sv.submit(1, edge, keep_image=True)
@@ -169,7 +174,8 @@ def test_server_concave_and_dropped_entries(ctx):
         mg.close()
 
 
-def test_server_c5_stream_matches_per_view_and_oracle(ctx):
+@pytest.mark.parametrize("schedule", [V.SERVE_SLOTS, V.SERVE_PIPELINE])
+def test_server_c5_stream_matches_per_view_and_oracle(ctx, schedule):
     """The serving workload itself: its 8 resident meshes (32^3..160^3
     grids, 0.2M-24.6M tets, 4 TFs each) and the first 200 requests of its
     stream, all queued at once.  Every image equals the same request rendered
@@ -184,9 +190,10 @@ def test_server_c5_stream_matches_per_view_and_oracle(ctx):
         meshes.append(ctx.mesh(x, t))
         tf_ids.append([ctx.tf(tf) for tf in tfs])
         host.append((x, t, tfs))
-    print("server C5: 8 meshes built", flush=True)
+    print(f"server C5 schedule {schedule}: 8 meshes built", flush=True)
     reqs = WL.serve_requests(200, len(meshes), tf_ids)
-    sv = V.Server(ctx, meshes, hbm_budget=16 << 30, streams=2, max_batch=256, max_batch_pixels=4 << 20)
+    sv = V.Server(ctx, meshes, hbm_budget=16 << 30, streams=2, max_batch=256, max_batch_pixels=4 << 20,
+                  schedule=schedule)
     checked = set()
     psi = V.psi_generate()
     try:
@@ -220,3 +227,50 @@ def test_server_c5_stream_matches_per_view_and_oracle(ctx):
         for m in meshes:
             m.close()
     assert len(checked) == 8
+
+
+def test_server_pipeline_admits_during_walks(ctx):
+    """The pipeline schedule: requests submitted while a long walk is on the
+    device (eight 1024^2 views of a 128^3 grid, one walk list) are admitted
+    (status 1: preprocessing enqueued, rays in the next walk list) before that
+    walk ends, and every image equals the same request rendered alone."""
+    import time
+    x, t = V.synth_grid(128, 0x5EED0450, 0)
+    big_m = ctx.mesh(x, t)
+    xs, ts = V.synth_grid(16, 0x5EED0451, 0)
+    small_m = ctx.mesh(xs, ts)
+    tf = ctx.tf(V.synth_tf(0x5EED0452))
+    r = WL.Rng(0x5EED0453, 9)
+    big = [V.perspective_view(1024, 1024, tf, eye=2.5 * r.direction()) for _ in range(8)]
+    small = [V.perspective_view(96 + 16 * (i % 3), 96 + 16 * (i % 3), tf, eye=2.5 * r.direction()) for i in range(24)]
+    print("server pipeline: requests admitted during walks", flush=True)
+    sv = V.Server(ctx, [big_m, small_m], hbm_budget=8 << 30, streams=1, max_batch=64, schedule=V.SERVE_PIPELINE)
+    try:
+        tb = [sv.submit(0, v, keep_image=True) for v in big]
+        for _ in range(20000):
+            if all(sv.poll(k)["status"] >= 1 for k in tb):
+                break
+            time.sleep(2e-5)
+        mid, seen = [], 0
+        for v in small:
+            mid.append(sv.submit(1, v, keep_image=True))
+            for _ in range(50):
+                if sv.poll(mid[-1])["status"] >= 1:
+                    # admitted; is a big request admitted before it still walking?
+                    seen += any(sv.poll(k)["status"] == 1 for k in tb)
+                    break
+                time.sleep(2e-5)
+        gb = [sv.wait(k, image=True) for k in tb]
+        gs = [sv.wait(k, image=True) for k in mid]
+    finally:
+        sv.close()
+    assert seen > 0
+    assert max(g["batch_size"] for g in gb + gs) > 1
+    for v, g in zip(big, gb):
+        alone, _ = ctx.render(big_m, [v], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+        assert g["status"] == 2 and np.array_equal(g["rgba"], alone[0]["rgba"])
+    for v, g in zip(small, gs):
+        alone, _ = ctx.render(small_m, [v], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+        assert g["status"] == 2 and np.array_equal(g["rgba"], alone[0]["rgba"])
+    big_m.close()
+    small_m.close()

@@ -25,6 +25,7 @@ RENDER_WIDE = 0x8
 RENDER_WAVE_QUEUE = 0x10
 RENDER_LDS_RECORDS = 0x20
 SCHED_PER_VIEW, SCHED_BATCHED = 0, 1
+SERVE_SLOTS, SERVE_PIPELINE = 0, 1
 MESH_NORMALIZE, MESH_MORTON = 0x1, 0x2
 NO_ID = 0xFFFFFFFF
 
@@ -68,7 +69,7 @@ class ServerCfg(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("streams", C.c_uint32), ("max_batch", C.c_uint32),
                 ("term_alpha", C.c_float), ("flags", C.c_uint32), ("occupancy", C.c_uint32),
                 ("waves_per_cu", C.c_uint32), ("refill", C.c_uint32), ("composite", C.c_uint32),
-                ("max_batch_pixels", C.c_uint64)]
+                ("max_batch_pixels", C.c_uint64), ("schedule", C.c_uint32)]
 
 
 class Request(C.Structure):
@@ -527,14 +528,15 @@ class Server:
     def __init__(self, ctx: Context, meshes, hbm_budget: int = 0, streams: int = 0, max_batch: int = 0,
                  term_alpha: float = 0.0, persistent: bool = False, occupancy: int = 0, waves_per_cu: int = 0,
                  refill: int = 0, psi_table: bool = False, wave_queue: bool = False,
-                 composite: int = COMPOSITE_PPI, max_batch_pixels: int = 0, lds_records: bool = False):
+                 composite: int = COMPOSITE_PPI, max_batch_pixels: int = 0, lds_records: bool = False,
+                 schedule: int = 0):
         self.ctx = ctx
         self.meshes = list(meshes)
         arr = (C.c_void_p * len(self.meshes))(*[m._p.value for m in self.meshes])
         flags = ((RENDER_PERSISTENT if persistent else 0) | (RENDER_PSI_TABLE if psi_table else 0)
                  | (RENDER_WAVE_QUEUE if wave_queue else 0) | (RENDER_LDS_RECORDS if lds_records else 0))
         cfg = ServerCfg(hbm_budget, streams, max_batch, term_alpha, flags, occupancy, waves_per_cu, refill,
-                        composite, max_batch_pixels)
+                        composite, max_batch_pixels, schedule)
         self._p = C.c_void_p()
         _check(lib().vfr_server_create(ctx._p, C.byref(cfg), arr, len(self.meshes), C.byref(self._p)))
 
@@ -543,6 +545,11 @@ class Server:
         _check(lib().vfr_server_submit(self._p, C.byref(Request(mesh, 1 if keep_image else 0, view)),
                                        C.byref(t)))
         return t.value
+
+    def poll(self, ticket: int) -> dict:
+        r = Result()
+        _check(lib().vfr_server_poll(self._p, ticket, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in r._fields_}
 
     def wait(self, ticket: int, image: bool = False):
         r = Result()

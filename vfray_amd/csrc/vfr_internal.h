@@ -181,10 +181,15 @@ int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t ma
                      void* stream);
 /* *out = max(*out, cnt[i]) over n pixels (device counter). */
 int vfr_k_max_u32(const uint32_t* cnt, size_t n, uint32_t* out, void* stream);
+/* view_base: added to the view index of every ray written (the batch's views
+ * sit at [view_base, view_base + n_views_total) of the array the walk reads;
+ * `views` points at the first of them). */
 int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, uint32_t ntiles,
                   uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries, uint32_t kmax,
                   uint64_t estride, float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
-                  uint32_t* n_rays, void* stream);
+                  uint32_t* n_rays, void* stream, uint32_t view_base = 0);
+int vfr_k_clear_view_pixels(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_pix, uint32_t* a,
+                            void* stream);
 /* occupancy: waves per SIMD of the persistent kernel variant (4, 5, 6:
  * register-capped variants; anything else: as compiled). */
 /* proj: VFR_TRAV_* (the projections of the views the rays belong to). */

@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include <vector>
 
 #include "../../include/vfr.h"
@@ -563,7 +565,7 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
                                                   float4* __restrict__ rgba, uint32_t* __restrict__ steps,
                                                   uint32_t* __restrict__ entry_out,
                                                   uint64_t* __restrict__ rays,
-                                                  uint32_t* __restrict__ n_rays) {
+                                                  uint32_t* __restrict__ n_rays, uint32_t view_base) {
   __shared__ uint32_t wave_n[COMPACT_WAVES];
   __shared__ uint32_t block_base;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -634,18 +636,18 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
     uint32_t before = block_base;
     for (uint32_t w = 0; w < wave; ++w) before += wave_n[w];
     before += (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    rays[before] = ((uint64_t)vi << 32) | pix;
+    rays[before] = ((uint64_t)(vi + view_base) << 32) | pix;
   }
 }
 
 int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, uint32_t ntiles,
                   uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries, uint32_t kmax,
                   uint64_t estride, float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
-                  uint32_t* n_rays, void* stream) {
+                  uint32_t* n_rays, void* stream, uint32_t view_base) {
   if (ntiles == 0) return VFR_OK;
   hipLaunchKernelGGL(k_compact, dim3((ntiles + COMPACT_WAVES - 1) / COMPACT_WAVES), dim3(1024), 0,
                      (hipStream_t)stream, views, n_views_total, tile0, ntiles, mode, owner, cnt, entries,
-                     kmax, estride, rgba, steps, entry_out, rays, n_rays);
+                     kmax, estride, rgba, steps, entry_out, rays, n_rays, view_base);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
@@ -657,6 +659,23 @@ __global__ void k_max_u32(const uint32_t* __restrict__ v, size_t n, uint32_t* __
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor(m, d, 64));
   if ((threadIdx.x & 63u) == 0 && m) atomicMax(out, m);
+}
+
+/* Zeroes each view's pixels [pix_off, pix_off + W H) of a per-pixel array,
+ * views [v0, v0 + nv), grid (max pixels / 256) x nv. */
+__global__ void k_clear_view_pixels(const DevView* __restrict__ views, uint32_t v0, uint32_t* __restrict__ a) {
+  const DevView& v = views[v0 + blockIdx.y];
+  const uint32_t n = v.W * v.H;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[v.pix_off + i] = 0u;
+}
+
+int vfr_k_clear_view_pixels(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_pix, uint32_t* a,
+                            void* stream) {
+  if (nv == 0 || max_pix == 0) return VFR_OK;
+  const uint32_t bx = std::min<uint32_t>((max_pix + 255u) / 256u, 1024u);
+  hipLaunchKernelGGL(k_clear_view_pixels, dim3(bx, nv), dim3(256), 0, (hipStream_t)stream, views, v0, a);
+  CHECK_HIP(hipGetLastError());
+  return VFR_OK;
 }
 
 int vfr_k_max_u32(const uint32_t* cnt, size_t n, uint32_t* out, void* stream) {

@@ -2,14 +2,26 @@
  * vfr_server.hip -- the render server: resident meshes, admission by HBM
  * footprint, continuous batching over a few streams, latency accounting.
  *
- * Each slot owns a stream and device buffers sized to its share of the HBM
- * budget, allocated once.  A dispatcher thread polls the slots' completion
- * events and, whenever a slot is free, admits the queued requests that fit
- * it (oldest first, up to max_batch requests and max_batch_pixels pixels)
- * and enqueues the batched launch set:
- * view upload, clears, per-view vertex transform, visibility/raster,
- * compaction and the persistent traversal.  Batches on different slots
- * overlap on the device; the dispatcher never waits on the device.
+ * Slots (VFR_SERVE_SLOTS): each slot owns a stream and device buffers sized
+ * to its share of the HBM budget, allocated once.  A dispatcher thread polls
+ * the slots' completion events and, whenever a slot is free, admits the
+ * queued requests that fit it (oldest first, up to max_batch requests and
+ * max_batch_pixels pixels) and enqueues the batched launch set: view upload,
+ * clears, per-view vertex transform, visibility/raster, compaction and the
+ * traversal.  Batches on different slots overlap on the device.
+ *
+ * Pipeline (VFR_SERVE_PIPELINE): the per-pixel buffers, the P-space vertices
+ * and the views are shared arenas from which each admitted request takes its
+ * own ranges (first fit), so admission waits only for memory.  The
+ * dispatcher admits the front of the queue as soon as it fits and enqueues
+ * its preprocessing on the preprocessing stream; the compaction appends the
+ * rays to the open walk list (one ray list, the concatenation of every batch
+ * preprocessed since the list opened).  Whenever one of `streams` walk
+ * streams is free, the open list is closed and walked in one launch after its
+ * batches' preprocessing events; a new list opens.  So requests that arrive
+ * while walks run are preprocessed meanwhile and join the next walk, which
+ * starts while the previous one drains.  The dispatcher never waits on the
+ * device.
  */
 #include <hip/hip_runtime.h>
 
@@ -19,6 +31,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -80,6 +93,66 @@ struct Req {
   uint32_t batch = 0;
   uint32_t vflag = 0;      /* VFR_VFLAG_* its view recorded */
   std::vector<float> image;
+  /* pipeline: the request's ranges in the shared arenas */
+  uint64_t view_idx = 0, pix_off = 0, vert_off = 0;
+};
+
+/* First-fit allocator of ranges in [0, cap). */
+struct Ranges {
+  std::map<uint64_t, uint64_t> free_; /* start -> length */
+  void init(uint64_t cap) {
+    free_.clear();
+    if (cap) free_[0] = cap;
+  }
+  bool alloc(uint64_t n, uint64_t* off) {
+    if (n == 0) { *off = 0; return true; }
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= n) {
+        *off = it->first;
+        const uint64_t s = it->first + n, l = it->second - n;
+        free_.erase(it);
+        if (l) free_[s] = l;
+        return true;
+      }
+    return false;
+  }
+  void release(uint64_t off, uint64_t n) {
+    if (n == 0) return;
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && off + n == next->first) {
+      n += next->second;
+      next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        prev->second += n;
+        return;
+      }
+    }
+    free_[off] = n;
+  }
+};
+
+/* A pipeline walk list: the rays of every batch preprocessed while it was
+ * open, walked in one launch. */
+struct WalkList {
+  enum { FREE = 0, OPEN = 1, WALKING = 2 };
+  int state = FREE;
+  uint64_t* rays = nullptr;              /* [cap_pix] */
+  uint32_t* counters = nullptr;          /* [0] rays, [1] queue head */
+  unsigned long long* stats = nullptr;   /* [ST_N] */
+  unsigned long long* h_stats = nullptr; /* pinned */
+  uint32_t* h_counters = nullptr;        /* pinned */
+  uint32_t* h_vflags = nullptr;          /* pinned copy of the view flags */
+  float* h_img = nullptr;                /* pinned staging of kept images */
+  hipEvent_t done = nullptr;
+  std::vector<hipEvent_t> prep_ev;       /* the batches' preprocessing events */
+  size_t n_prep = 0;
+  std::vector<Req*> reqs;
+  uint64_t pix = 0, keep_pix = 0;
+  uint32_t n_persp = 0, tf_lo = UINT32_MAX, tf_hi = 0;
+  int stream = -1;
 };
 
 struct Slot {
@@ -121,6 +194,23 @@ struct vfr_server {
   uint4* fvid = nullptr;       /* external faces' vertex ids, same offsets as ext */
   float4* vert = nullptr;
   std::vector<std::unique_ptr<Slot>> slots;
+  /* pipeline */
+  hipStream_t prep_stream = nullptr;
+  std::vector<hipStream_t> walk_streams;
+  std::vector<int> walk_busy;
+  DevView* p_views = nullptr;     /* [cap_views] */
+  DevView* p_h_views = nullptr;   /* pinned staging, same indices */
+  uint32_t* p_vflags = nullptr;   /* [cap_views] */
+  uint32_t* p_cnt = nullptr;      /* [cap_pix] */
+  uint64_t* p_entries = nullptr;  /* [VFR_K_MAX * cap_pix] */
+  float4* p_rgba = nullptr;
+  uint32_t* p_steps = nullptr;
+  uint32_t* p_entry = nullptr;
+  float4* p_pverts = nullptr;     /* [cap_verts] */
+  uint64_t cap_pix = 0, cap_verts = 0, cap_views = 0, cap_keep = 0;
+  Ranges r_pix, r_verts, r_views;
+  std::vector<std::unique_ptr<WalkList>> lists;
+  std::vector<hipEvent_t> ev_pool;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Req*> queue;
@@ -279,7 +369,356 @@ static void finish_batch(vfr_server* sv, Slot* s, double now) {
   s->busy = false;
 }
 
+/* ----------------------------------------------------------------- pipeline */
+
+/* Per-pixel bytes of the pipeline's shared arena: entry count, VFR_K_MAX entry
+ * keys, RGBA, steps, entry id, and a ray slot in each walk list. */
+static uint64_t pipe_pixel_bytes(uint32_t lists) { return 4 + 8ull * VFR_K_MAX + 16 + 4 + 4 + 8ull * lists; }
+
+static int pipe_alloc(vfr_server* sv) {
+  const uint32_t nl = sv->cfg.streams + 1; /* one open list and one per walk stream */
+  const uint64_t b = sv->cfg.hbm_budget;
+  /* three quarters for per-pixel buffers, one quarter for P-space vertices */
+  sv->cap_pix = std::min<uint64_t>((b * 3 / 4) / pipe_pixel_bytes(nl), (1ull << 31) - 1);
+  sv->cap_verts = std::min<uint64_t>((b / 4) / 16, VFR_MAX_VIEW_VERTS - 1);
+  sv->cap_views = std::max<uint64_t>(4ull * sv->cfg.max_batch, 1024);
+  sv->cap_keep = std::min<uint64_t>(sv->cap_pix, 1ull << 22); /* 4 Mpixel = 64 MB of RGBA per list */
+  sv->r_pix.init(sv->cap_pix);
+  sv->r_verts.init(sv->cap_verts);
+  sv->r_views.init(sv->cap_views);
+  CHECK_HIP(hipStreamCreateWithFlags(&sv->prep_stream, hipStreamNonBlocking));
+  sv->walk_streams.assign(sv->cfg.streams, nullptr);
+  sv->walk_busy.assign(sv->cfg.streams, 0);
+  for (auto& st : sv->walk_streams) CHECK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  CHECK_HIP(hipMalloc((void**)&sv->p_views, sizeof(DevView) * sv->cap_views));
+  CHECK_HIP(hipHostMalloc((void**)&sv->p_h_views, sizeof(DevView) * sv->cap_views, hipHostMallocDefault));
+  CHECK_HIP(hipMalloc((void**)&sv->p_vflags, sizeof(uint32_t) * sv->cap_views));
+  CHECK_HIP(hipMalloc((void**)&sv->p_cnt, 4 * sv->cap_pix));
+  CHECK_HIP(hipMalloc((void**)&sv->p_entries, 8ull * VFR_K_MAX * sv->cap_pix));
+  CHECK_HIP(hipMalloc((void**)&sv->p_rgba, 16 * sv->cap_pix));
+  CHECK_HIP(hipMalloc((void**)&sv->p_steps, 4 * sv->cap_pix));
+  CHECK_HIP(hipMalloc((void**)&sv->p_entry, 4 * sv->cap_pix));
+  CHECK_HIP(hipMalloc((void**)&sv->p_pverts, 16 * sv->cap_verts));
+  for (uint32_t i = 0; i < nl; ++i) {
+    sv->lists.emplace_back(new WalkList);
+    WalkList* L = sv->lists.back().get();
+    CHECK_HIP(hipMalloc((void**)&L->rays, 8 * sv->cap_pix));
+    CHECK_HIP(hipMalloc((void**)&L->counters, 16));
+    CHECK_HIP(hipMalloc((void**)&L->stats, sizeof(unsigned long long) * ST_N));
+    CHECK_HIP(hipHostMalloc((void**)&L->h_stats, sizeof(unsigned long long) * ST_N, hipHostMallocDefault));
+    CHECK_HIP(hipHostMalloc((void**)&L->h_counters, 16, hipHostMallocDefault));
+    CHECK_HIP(hipHostMalloc((void**)&L->h_vflags, sizeof(uint32_t) * sv->cap_views, hipHostMallocDefault));
+    CHECK_HIP(hipHostMalloc((void**)&L->h_img, 16 * sv->cap_keep, hipHostMallocDefault));
+    CHECK_HIP(hipEventCreateWithFlags(&L->done, hipEventDisableTiming));
+  }
+  return VFR_OK;
+}
+
+static void pipe_free(vfr_server* sv) {
+  for (auto& lp : sv->lists) {
+    WalkList* L = lp.get();
+    void* dev[] = {L->rays, L->counters, L->stats};
+    for (void* q : dev)
+      if (q) (void)hipFree(q);
+    void* host[] = {L->h_stats, L->h_counters, L->h_vflags, L->h_img};
+    for (void* q : host)
+      if (q) (void)hipHostFree(q);
+    if (L->done) (void)hipEventDestroy(L->done);
+    for (hipEvent_t e : L->prep_ev) (void)hipEventDestroy(e);
+  }
+  sv->lists.clear();
+  for (hipEvent_t e : sv->ev_pool) (void)hipEventDestroy(e);
+  sv->ev_pool.clear();
+  void* dev[] = {sv->p_views, sv->p_vflags, sv->p_cnt, sv->p_entries, sv->p_rgba, sv->p_steps, sv->p_entry,
+                 sv->p_pverts};
+  for (void* q : dev)
+    if (q) (void)hipFree(q);
+  if (sv->p_h_views) (void)hipHostFree(sv->p_h_views);
+  for (hipStream_t st : sv->walk_streams)
+    if (st) (void)hipStreamDestroy(st);
+  if (sv->prep_stream) (void)hipStreamDestroy(sv->prep_stream);
+}
+
+/* A list becomes the open one: its counters and statistics are cleared on the
+ * preprocessing stream, before any batch appends to it. */
+static int pipe_open(vfr_server* sv, WalkList* L) {
+  CHECK_RC(vfr_k_clear_u32(L->counters, 4, sv->prep_stream));
+  CHECK_RC(vfr_k_clear_u32((uint32_t*)L->stats, 2 * ST_N, sv->prep_stream));
+  L->state = WalkList::OPEN;
+  L->n_prep = 0;
+  L->reqs.clear();
+  L->pix = L->keep_pix = 0;
+  L->n_persp = 0;
+  L->tf_lo = UINT32_MAX;
+  L->tf_hi = 0;
+  L->stream = -1;
+  return VFR_OK;
+}
+
+/* Preprocessing of requests [0, n) of `batch` (their arena ranges taken, views
+ * at [view_base, view_base + n)) on the preprocessing stream, appending their
+ * rays to the open list L. */
+static int pipe_prep(vfr_server* sv, WalkList* L, const std::vector<Req*>& batch, uint64_t view_base) {
+  const uint32_t nv = (uint32_t)batch.size();
+  uint64_t tiles = 0;
+  uint32_t max_verts = 0, max_faces = 0, max_pix = 0;
+  for (uint32_t i = 0; i < nv; ++i) {
+    Req* r = batch[i];
+    const SrvMesh& m = sv->meshes[r->mesh];
+    DevView d = r->dv;
+    d.pix_off = (uint32_t)r->pix_off;
+    d.tile_off = (uint32_t)tiles;
+    d.tiles_x = (r->view.width + 7) / 8;
+    d.vert_off = (uint32_t)r->vert_off;
+    sv->p_h_views[view_base + i] = d;
+    tiles += (uint64_t)d.tiles_x * ((r->view.height + 7) / 8);
+    max_verts = std::max(max_verts, m.n_verts);
+    max_faces = std::max(max_faces, m.n_ext);
+    max_pix = std::max<uint32_t>(max_pix, (uint32_t)r->pixels);
+  }
+  hipStream_t st = sv->prep_stream;
+  DevView* dv = sv->p_views + view_base;
+  CHECK_HIP(hipMemcpyAsync(dv, sv->p_h_views + view_base, sizeof(DevView) * nv, hipMemcpyHostToDevice, st));
+  CHECK_RC(vfr_k_clear_u32(sv->p_vflags + view_base, nv, st));
+  CHECK_RC(vfr_k_clear_view_pixels(sv->p_views, (uint32_t)view_base, nv, max_pix, sv->p_cnt, st));
+  CHECK_RC(vfr_k_xform_views(sv->p_views, (uint32_t)view_base, nv, max_verts, sv->vert, sv->p_pverts, st));
+  CHECK_RC(vfr_k_raster_ftb(sv->p_views, (uint32_t)view_base, nv, max_faces, sv->fvid, sv->p_pverts, sv->p_cnt,
+                            sv->p_entries, VFR_K_MAX, sv->cap_pix, L->stats, sv->p_vflags, st));
+  CHECK_RC(vfr_k_compact(dv, nv, 0, (uint32_t)tiles, VFR_MODE_FTB, nullptr, sv->p_cnt, sv->p_entries, VFR_K_MAX,
+                         sv->cap_pix, sv->p_rgba, sv->p_steps, sv->p_entry, L->rays, L->counters, st,
+                         (uint32_t)view_base));
+  if (L->n_prep == L->prep_ev.size()) {
+    hipEvent_t e = nullptr;
+    if (!sv->ev_pool.empty()) {
+      e = sv->ev_pool.back();
+      sv->ev_pool.pop_back();
+    } else {
+      CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    L->prep_ev.push_back(e);
+  }
+  CHECK_HIP(hipEventRecord(L->prep_ev[L->n_prep++], st));
+  return VFR_OK;
+}
+
+/* Walks list L on walk stream k once its batches are preprocessed, then
+ * copies back the statistics, the view flags and the kept images. */
+static int pipe_walk(vfr_server* sv, WalkList* L, int k) {
+  hipStream_t st = sv->walk_streams[k];
+  for (size_t i = 0; i < L->n_prep; ++i) CHECK_HIP(hipStreamWaitEvent(st, L->prep_ev[i], 0));
+  TravArgs a;
+  memset(&a, 0, sizeof(a));
+  a.rec = sv->rec;
+  a.tv = sv->tv;
+  a.vert = sv->p_pverts;
+  a.ext_int = sv->ext;
+  a.tf = vfr_ctx_tf(sv->ctx);
+  a.psi = vfr_ctx_psi(sv->ctx);
+  a.views = sv->p_views;
+  a.rays = L->rays;
+  a.n_rays = L->counters;
+  a.cnt = sv->p_cnt;
+  a.entries = sv->p_entries;
+  a.estride = sv->cap_pix;
+  a.rgba = sv->p_rgba;
+  a.steps = sv->p_steps;
+  a.entry_out = sv->p_entry;
+  a.stats = L->stats;
+  a.queue = L->counters + 1;
+  a.term = sv->cfg.term_alpha;
+  a.refill = sv->cfg.refill;
+  a.kmax = VFR_K_MAX;
+  a.vflags = sv->p_vflags;
+  a.psi_table = (sv->cfg.flags & VFR_RENDER_PSI_TABLE) ? 1u : 0u;
+  a.lds_records = (sv->cfg.flags & VFR_RENDER_LDS_RECORDS) ? 1u : 0u;
+  a.composite = sv->cfg.composite;
+  if (L->tf_hi - L->tf_lo < VFR_TF_LDS_SLOTS) { a.tf_lds_base = L->tf_lo; a.tf_lds_n = L->tf_hi - L->tf_lo + 1; }
+  const uint32_t nv = (uint32_t)L->reqs.size();
+  const uint32_t proj = L->n_persp == 0 ? VFR_TRAV_ORTHO : (L->n_persp == nv ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
+  const uint64_t records = (sv->cfg.flags & VFR_RENDER_WIDE) ? VFR_BIG_RECORDS : sv->n_records;
+  CHECK_RC(vfr_k_traverse(VFR_MODE_FTB, proj, records, sv->cfg.occupancy, a, (uint32_t)L->pix,
+                          (sv->cfg.flags & VFR_RENDER_PERSISTENT)    ? VFR_WALK_REFILL
+                          : (sv->cfg.flags & VFR_RENDER_WAVE_QUEUE) ? VFR_WALK_QUEUE
+                                                                    : VFR_WALK_THREAD,
+                          sv->cfg.waves_per_cu * (uint32_t)vfr_ctx_cus(sv->ctx), st));
+  CHECK_HIP(hipMemcpyAsync(L->h_stats, L->stats, sizeof(unsigned long long) * ST_N, hipMemcpyDeviceToHost, st));
+  CHECK_HIP(hipMemcpyAsync(L->h_counters, L->counters, 16, hipMemcpyDeviceToHost, st));
+  /* the flags of every view slot (a few KB; only this list's are read) */
+  CHECK_HIP(hipMemcpyAsync(L->h_vflags, sv->p_vflags, sizeof(uint32_t) * sv->cap_views, hipMemcpyDeviceToHost, st));
+  for (Req* r : L->reqs) {
+    if (r->keep)
+      CHECK_HIP(hipMemcpyAsync(L->h_img + 4ull * r->keep_off, sv->p_rgba + r->pix_off, 16ull * r->pixels,
+                               hipMemcpyDeviceToHost, st));
+  }
+  CHECK_HIP(hipEventRecord(L->done, st));
+  L->state = WalkList::WALKING;
+  L->stream = k;
+  return VFR_OK;
+}
+
+/* Results of a walked list; caller holds the lock. */
+static void pipe_finish(vfr_server* sv, WalkList* L, double now) {
+  for (Req* r : L->reqs) {
+    r->t_done = now;
+    r->vflag = L->h_vflags[r->view_idx];
+    r->status = r->vflag ? VFR_EMESH : 2;
+    r->batch = (uint32_t)L->reqs.size();
+    if (r->keep && !r->vflag)
+      r->image.assign(L->h_img + 4ull * r->keep_off, L->h_img + 4ull * (r->keep_off + r->pixels));
+    sv->r_pix.release(r->pix_off, r->pixels);
+    sv->r_verts.release(r->vert_off, sv->meshes[r->mesh].n_verts);
+    sv->r_views.release(r->view_idx, 1);
+  }
+  sv->n_batches++;
+  sv->n_batched_reqs += L->reqs.size();
+  sv->steps += L->h_stats[ST_STEPS];
+  sv->anomalies += L->h_stats[ST_ANOM];
+  sv->active_rays += L->h_counters[0];
+  sv->walk_busy[L->stream] = 0;
+  L->reqs.clear();
+  L->state = WalkList::FREE;
+}
+
+static WalkList* pipe_list(vfr_server* sv, int state) {
+  for (auto& lp : sv->lists)
+    if (lp->state == state) return lp.get();
+  return nullptr;
+}
+
+static void pipe_fail(vfr_server* sv, std::vector<Req*>& reqs, int rc) {
+  sv->worker_error = rc;
+  for (Req* r : reqs) r->status = rc;
+  sv->cv_done.notify_all();
+}
+
+static void dispatcher_pipeline(vfr_server* sv) {
+  (void)hipSetDevice(vfr_ctx_device(sv->ctx));
+  std::unique_lock<std::mutex> lk(sv->mu);
+  while (true) {
+    bool any_busy = false;
+    const double now = secs(Clock::now());
+    /* completed walks */
+    for (auto& lp : sv->lists) {
+      WalkList* L = lp.get();
+      if (L->state != WalkList::WALKING) continue;
+      hipError_t q = hipEventQuery(L->done);
+      if (q == hipSuccess) {
+        pipe_finish(sv, L, now);
+        sv->cv_done.notify_all();
+      } else if (q != hipErrorNotReady) {
+        vfr_set_error("walk failed: %s", hipGetErrorString(q));
+        pipe_fail(sv, L->reqs, VFR_EHIP);
+        sv->walk_busy[L->stream] = 0;
+        L->reqs.clear();
+        L->state = WalkList::FREE;
+      } else {
+        any_busy = true;
+      }
+    }
+    WalkList* open = pipe_list(sv, WalkList::OPEN);
+    if (!open && !sv->queue.empty()) {
+      WalkList* f = pipe_list(sv, WalkList::FREE);
+      if (f) {
+        int rc = pipe_open(sv, f);
+        if (rc != VFR_OK) {
+          std::vector<Req*> q(sv->queue.begin(), sv->queue.end());
+          sv->queue.clear();
+          pipe_fail(sv, q, rc);
+        } else {
+          open = f;
+        }
+      }
+    }
+    /* admission: the front of the queue, as long as its ranges fit */
+    if (open && !sv->queue.empty()) {
+      std::vector<Req*> batch;
+      while (!sv->queue.empty() && batch.size() < sv->cfg.max_batch) {
+        Req* r = sv->queue.front();
+        const SrvMesh& m = sv->meshes[r->mesh];
+        const uint64_t kp = r->keep ? r->pixels : 0;
+        if (open->keep_pix + kp > sv->cap_keep) break;
+        if (sv->cfg.max_batch_pixels && !open->reqs.empty() && open->pix + r->pixels > sv->cfg.max_batch_pixels) break;
+        uint64_t po = 0, vo = 0;
+        if (!sv->r_pix.alloc(r->pixels, &po)) break;
+        if (!sv->r_verts.alloc(m.n_verts, &vo)) { sv->r_pix.release(po, r->pixels); break; }
+        r->pix_off = po;
+        r->vert_off = vo;
+        r->keep_off = open->keep_pix;
+        open->keep_pix += kp;
+        open->pix += r->pixels;
+        open->reqs.push_back(r);
+        batch.push_back(r);
+        sv->queue.pop_front();
+      }
+      uint64_t vb = 0;
+      if (!batch.empty() && !sv->r_views.alloc(batch.size(), &vb)) {
+        /* no contiguous view slots: give the batch back to the queue */
+        for (size_t i = batch.size(); i-- > 0;) {
+          Req* r = batch[i];
+          sv->r_pix.release(r->pix_off, r->pixels);
+          sv->r_verts.release(r->vert_off, sv->meshes[r->mesh].n_verts);
+          open->keep_pix -= r->keep ? r->pixels : 0;
+          open->pix -= r->pixels;
+          open->reqs.pop_back();
+          sv->queue.push_front(r);
+        }
+        batch.clear();
+      }
+      if (!batch.empty()) {
+        const double ta = secs(Clock::now());
+        for (size_t i = 0; i < batch.size(); ++i) {
+          Req* r = batch[i];
+          r->view_idx = vb + i;
+          r->t_admit = ta;
+          r->status = 1;
+          open->n_persp += r->dv.proj == VFR_PROJ_PERSPECTIVE ? 1u : 0u;
+          open->tf_lo = std::min(open->tf_lo, r->dv.tf);
+          open->tf_hi = std::max(open->tf_hi, r->dv.tf);
+        }
+        lk.unlock();
+        int rc = pipe_prep(sv, open, batch, vb);
+        lk.lock();
+        if (rc != VFR_OK) pipe_fail(sv, batch, rc);
+      }
+    }
+    /* walk the open list on a free walk stream */
+    open = pipe_list(sv, WalkList::OPEN);
+    if (open && !open->reqs.empty()) {
+      int k = -1;
+      for (size_t i = 0; i < sv->walk_busy.size(); ++i)
+        if (!sv->walk_busy[i]) { k = (int)i; break; }
+      if (k >= 0) {
+        sv->walk_busy[k] = 1;
+        lk.unlock();
+        int rc = pipe_walk(sv, open, k);
+        lk.lock();
+        if (rc != VFR_OK) {
+          pipe_fail(sv, open->reqs, rc);
+          sv->walk_busy[k] = 0;
+          open->reqs.clear();
+          open->state = WalkList::FREE;
+        }
+        any_busy = true;
+      }
+    }
+    open = pipe_list(sv, WalkList::OPEN);
+    const bool pending = !sv->queue.empty() || (open && !open->reqs.empty());
+    if (sv->stop && !any_busy && !pending) break;
+    if (!any_busy && !pending) {
+      sv->cv_work.wait(lk, [sv] { return sv->stop || !sv->queue.empty(); });
+    } else {
+      lk.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      lk.lock();
+    }
+  }
+}
+
 static void dispatcher(vfr_server* sv) {
+  if (sv->cfg.schedule == VFR_SERVE_PIPELINE) {
+    dispatcher_pipeline(sv);
+    return;
+  }
   (void)hipSetDevice(vfr_ctx_device(sv->ctx));
   std::unique_lock<std::mutex> lk(sv->mu);
   while (true) {
@@ -362,6 +801,10 @@ extern "C" void vfr_server_destroy(vfr_server* sv) {
     if (sp->stream) (void)hipStreamSynchronize(sp->stream);
     slot_free(sp.get());
   }
+  if (sv->prep_stream) (void)hipStreamSynchronize(sv->prep_stream);
+  for (hipStream_t st : sv->walk_streams)
+    if (st) (void)hipStreamSynchronize(st);
+  pipe_free(sv);
   if (sv->rec) (void)hipFree(sv->rec);
   if (sv->tv) (void)hipFree(sv->tv);
   if (sv->ext) (void)hipFree(sv->ext);
@@ -387,7 +830,7 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
   }
   if (c.refill == 0) c.refill = 32;
   if (c.streams > 16 || c.max_batch > 4096 || c.term_alpha > 1.0f || c.waves_per_cu > 64 || c.refill > 64 ||
-      c.composite > VFR_COMPOSITE_LINEAR_OPACITY) {
+      c.composite > VFR_COMPOSITE_LINEAR_OPACITY || c.schedule > VFR_SERVE_PIPELINE) {
     vfr_set_error("vfr_server_create: configuration out of range");
     return VFR_EINVAL;
   }
@@ -443,7 +886,11 @@ extern "C" int vfr_server_create(vfr_ctx* ctx, const vfr_server_cfg* cfg, const 
     ro += d->n_tets; eo += d->n_ext; vo += d->n_verts;
     sv->n_records = ro;
   }
-  for (uint32_t i = 0; rc == VFR_OK && i < c.streams; ++i) {
+  if (rc == VFR_OK && c.schedule == VFR_SERVE_PIPELINE) {
+    rc = pipe_alloc(sv);
+    if (rc == VFR_ENOMEM || rc == VFR_EHIP) vfr_set_error("pipeline arena allocation failed: %s", vfr_last_error());
+  }
+  for (uint32_t i = 0; rc == VFR_OK && c.schedule == VFR_SERVE_SLOTS && i < c.streams; ++i) {
     sv->slots.emplace_back(new Slot);
     rc = slot_alloc(sv->slots.back().get(), c.hbm_budget / c.streams, c.max_batch);
     Slot* sl = sv->slots.back().get();
@@ -481,9 +928,15 @@ static int submit_at(vfr_server* sv, const vfr_request* rq, double t_arrival, ui
   r->view = rq->view;
   r->keep = rq->keep_image;
   r->pixels = (uint64_t)rq->view.width * rq->view.height;
-  const Slot* s0 = sv->slots[0].get();
-  if (r->pixels > s0->cap_pix || m.n_verts > s0->cap_verts || (r->keep && r->pixels > s0->cap_keep)) {
-    vfr_set_error("request of %llu pixels on a %u-vertex mesh exceeds a slot's share of the HBM budget",
+  uint64_t cap_pix, cap_verts, cap_keep;
+  if (sv->cfg.schedule == VFR_SERVE_PIPELINE) {
+    cap_pix = sv->cap_pix; cap_verts = sv->cap_verts; cap_keep = sv->cap_keep;
+  } else {
+    const Slot* s0 = sv->slots[0].get();
+    cap_pix = s0->cap_pix; cap_verts = s0->cap_verts; cap_keep = s0->cap_keep;
+  }
+  if (r->pixels > cap_pix || m.n_verts > cap_verts || (r->keep && r->pixels > cap_keep)) {
+    vfr_set_error("request of %llu pixels on a %u-vertex mesh exceeds the server's share of the HBM budget",
                   (unsigned long long)r->pixels, m.n_verts);
     return VFR_EBUDGET;
   }
