@@ -919,16 +919,21 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
  * out is replaced in place.  Depth and scalar at the exit point are the
  * edge-function-weighted average of the face's vertices (all weights >= 0).
  *
- * Loads are issued one tet ahead: once the exit face is known, its link
+ * Two loads per step: once the exit face is known, its 8-byte link
  * (neighbour word, the neighbour's vertex off the shared face and slot map)
- * is picked from the face record held in registers, and the next tet's face
- * record and fourth vertex are requested before the segment is composited,
- * so their latency overlaps the compositing arithmetic.  The ray carries the
- * record slot of each face vertex rather than its id. */
+ * is requested from the tet's face record, and the segment is composited
+ * while it travels; then the next tet's fourth vertex is requested, which the
+ * next step's edge functions wait for.  The walk is bound by the vector
+ * memory pipeline (one cache-line lookup per distinct line per load
+ * instruction), so one 8-byte load of the exit face's link beats
+ * prefetching the next tet's whole 32-byte record (two 16-byte loads) one
+ * step ahead, although it puts a second load on the step's dependency chain.
+ * The ray carries the record slot of each face vertex rather than its id.
+ * With LDS-staged records (LDSR) the whole record is still prefetched. */
 struct FtbRay {
   float4 P0, P1, P2;       /* entry face: (X - ox, Y - oy, Z, S) */
   float4 Q;                /* the tet's fourth vertex, P space (prefetched) */
-  uint4 fa, fb;            /* its face words f[0..7] (prefetched) */
+  uint4 fa, fb;            /* LDSR: its face words f[0..7] (prefetched) */
   float o01, o12, o20;     /* edge functions P0->P1, P1->P2, P2->P0 (>= 0) */
   float4 cP;               /* TF colour at the entry point ...               */
   uint32_t cpi;            /* ... or its index, when the TF table is in LDS */
@@ -1055,8 +1060,9 @@ DEV float4 tf_at(const TravArgs& a, const float4* tfl, uint32_t i) {
 }
 
 /* Starts the walk at entry `cur` of the pixel's sorted list; false if the
- * entry face is degenerate at the pixel. */
-template <bool TFL>
+ * entry face is degenerate at the pixel.  REC: also load the tet's face
+ * record into the ray (LDS-staged records). */
+template <bool TFL, bool REC = false>
 DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const uint64_t key = __builtin_nontemporal_load(a.entries + ftb_cur(r) * a.estride + r.gp);
   const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull) + a.views[r.vi].ext_off;
@@ -1065,8 +1071,10 @@ DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
   r.t = a.views[r.vi].rec_off + tf2.x;
   const uint32_t fe = tf2.y;
   const uint4 rv = a.tv[r.t];
-  r.fa = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[0]);
-  r.fb = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[4]);
+  if (REC) {
+    r.fa = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[0]);
+    r.fb = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[4]);
+  }
   /* face fe = record slots fe, fe+1, fe+2; the fourth vertex is slot fe+3 */
   uint32_t s1 = (fe + 1u) & 3u, s2 = (fe + 2u) & 3u;
   const uint32_t s3 = (fe + 3u) & 3u;
@@ -1095,7 +1103,7 @@ DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
 
 /* LIN: linear compositing; the ray's colour bytes live in ar, ag, ab (exact
  * integers), its opacity in Tr, and segment lengths stay in object units. */
-template <bool TFL, bool LIN = false>
+template <bool TFL, bool LIN = false, bool REC = false>
 DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pix, FtbRay& r) {
   const uint32_t W = a.views[vi].W, H = a.views[vi].H;
   const float k = a.views[vi].k, kx = a.views[vi].kx;
@@ -1117,7 +1125,7 @@ DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pi
   r.ar = r.ag = r.ab = 0.0f;
   r.Tr = LIN ? 0.0f : 1.0f;
   r.steps = 0;
-  if (!ftb_segment<TFL>(a, tfl, r)) r.bits |= FTB_DEAD;
+  if (!ftb_segment<TFL, REC>(a, tfl, r)) r.bits |= FTB_DEAD;
 }
 
 enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
@@ -1129,11 +1137,10 @@ enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
  * The body is one straight path that updates the ray state in place, so no
  * copies of the state are needed where paths join, and it is safe to run on
  * a finished ray (every index it forms stays inside the mesh arrays; the
- * result is ignored).  Its loads go out as soon as the exit face is known:
- * the transfer-function colour at the exit point, then the next tet's
- * fourth vertex and record, which the next step waits for; psi is evaluated
- * in registers meanwhile (with PSIT it is a second group of loads, the
- * table entry, waited for within the step). */
+ * result is ignored).  The exit face's link is requested as soon as the
+ * exit face is known; depth, scalar, the transfer-function colour, psi and
+ * the compositing run while it travels; the next tet's fourth vertex is
+ * requested last, and the next step waits for it. */
 /* PROJ: VFR_TRAV_*, the projection of every ray of the launch.  BIG: the
  * record arena reaches 4 GiB (VFR_BIG_RECORDS tets or more), so records are
  * addressed with 64-bit arithmetic instead of 32-bit byte offsets.  PSIT:
@@ -1205,6 +1212,26 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
   }
   /* no exit face or the step bound: the step is not composited */
   const bool skip = r.steps > VFR_MAX_STEPS || !(p1 || p2 || p3);
+  /* the exit face leaves out vertex k of the entry face; in the record it
+   * is the face opposite k's slot s, face (s + 1) & 3 */
+  const bool k0 = c2, k1 = !c1 && !c2, k2 = c1;
+  const uint32_t sk = (r.sl >> (c1 ? 16u : (c2 ? 0u : 8u))) & 3u;
+  /* its link: the neighbour word and the vertex word (the neighbour's fourth
+   * vertex with, above bit 26, the slot map across the face); requested now
+   * and used at the end of the step */
+  uint32_t nb, qw;
+  if (LDSR) {
+    const uint4 fa = r.fa, fb = r.fb;
+    const bool e0 = sk == 3u, e1 = sk == 0u, e2 = sk == 1u;
+    nb = pick3u(e0, e1, fa.x, fa.z, e2 ? fb.x : fb.z);
+    qw = pick3u(e0, e1, fa.y, fa.w, e2 ? fb.y : fb.w);
+  } else {
+    const uint32_t fx = (sk + 1u) & 3u;
+    const uint2 lk = BIG ? *reinterpret_cast<const uint2*>(&a.rec[r.t].f[2u * fx])
+                         : ld32<uint2>(a.rec, r.t * 32u + fx * 8u);
+    nb = lk.x;
+    qw = lk.y;
+  }
   /* exit face (P3, Pi, Pj): P3 weighs o_ij, Pi weighs -t_j, Pj weighs t_i,
    * the left-out vertex Pk weighs 0 */
   const float w3 = pick3(c1, c2, r.o01, r.o12, r.o20);
@@ -1214,65 +1241,13 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
   const float D = ((w3 + w0) + w1) + w2;
   float zd, snext;
   face_values(persp, D, w3, P3, w0, r.P0, w1, r.P1, w2, r.P2, &zd, &snext);
-  /* the exit face leaves out vertex k of the entry face; in the record it
-   * is the face opposite k's slot s, face (s + 1) & 3 */
-  const bool k0 = c2, k1 = !c1 && !c2, k2 = c1;
-  const uint32_t sk = (r.sl >> (c1 ? 16u : (c2 ? 0u : 8u))) & 3u;
-  /* its link (neighbour word; the neighbour's fourth vertex with, above bit
-   * 26, the slot map across the face) from the record words held in
-   * registers */
-  const uint4 fa = r.fa, fb = r.fb;
-  const bool e0 = sk == 3u, e1 = sk == 0u, e2 = sk == 1u;
-  const uint32_t nb = pick3u(e0, e1, fa.x, fa.z, e2 ? fb.x : fb.z);
-  const uint32_t qw = pick3u(e0, e1, fa.y, fa.w, e2 ? fb.y : fb.w);
-  const uint32_t i3n = qw & VFR_REC_VERT_MASK;
-  const uint32_t smap = sperm[qw >> VFR_REC_VERT_BITS];
-  /* slots in the neighbour's record: vertex k and the fourth vertex trade
-   * places (P3 replaces vertex k, the neighbour's vertex off the face becomes
-   * the fourth, and the map sends k's slot to that vertex's slot), then every
-   * byte goes through the map */
-  const uint32_t swp = c1 ? 0x02030100u : (c2 ? 0x00020103u : 0x01020300u);
-  const uint32_t sln = __builtin_amdgcn_perm(smap, smap, __builtin_amdgcn_perm(r.sl, r.sl, swp));
-  /* neighbour words index the record arena (a server rebases each resident
-   * mesh's words when it copies them in) */
-  const uint32_t tn = nb < VFR_NB_BROKEN ? (nb >> 2) : r.t;
-  /* first loads: the colour at the exit point and the next tet (the current one
-   * again at a boundary, unused) */
+  /* the colour at the exit point; psi is evaluated in registers (with PSIT
+   * it is a table entry, a second load waited for within the step).  A
+   * skipped step composites nothing: with dl = 0 both depths are 0, psi =
+   * zeta = 1 and the update leaves colour and transparency unchanged (in
+   * linear compositing, both terms vanish). */
   const uint32_t ci = r.tfo + (uint32_t)(255.0f * snext); /* snext is in [0, 1] */
   const float4 cN = tf_at<TFL>(a, tfl, ci);
-  const float4 Qn = ld32<float4>(a.vert, (r.voff + i3n) * 16u);
-  uint4 fan, fbn;
-  if (BIG) {
-    fan = *reinterpret_cast<const uint4*>(&a.rec[tn].f[0]);
-    fbn = *reinterpret_cast<const uint4*>(&a.rec[tn].f[4]);
-  } else if (LDSR) {
-    /* lanes at a boundary or on a skipped step keep to themselves */
-    bool own = true;
-    if (nb < VFR_NB_BROKEN && !skip) {
-      const uint32_t lane = threadIdx.x & 63u, h = tn & 63u;
-      uint32_t* tag = ldsr_tags(sperm);
-      /* the tet that wins slot h, then one lane among those asking for it */
-      tag[h] = tn;
-      if (tag[h] == tn) {
-        tag[64 + h] = lane;
-        own = tag[64 + h] == lane;
-        r.bits |= (h << 16) | FTB_LSHARED | (own ? FTB_LOWNER : 0u);
-      }
-    }
-    fan = r.fa;
-    fbn = r.fb;
-    if (own) {
-      fan = ld32<uint4>(a.rec, tn * 32u);
-      fbn = ld32<uint4>(a.rec, tn * 32u + 16u);
-    }
-  } else {
-    fan = ld32<uint4>(a.rec, tn * 32u);
-    fbn = ld32<uint4>(a.rec, tn * 32u + 16u);
-  }
-  __builtin_amdgcn_sched_barrier(0);
-  /* then psi.  A skipped step composites nothing: with dl = 0 both
-   * depths are 0, psi = zeta = 1 and the update leaves colour and
-   * transparency unchanged (in linear compositing, both terms vanish). */
   const float dl = skip ? 0.0f : fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
   const float4 cP = TFL ? tfl[r.cpi] : r.cP;
   if (LIN) {
@@ -1308,7 +1283,6 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
   r.P0 = sel4f(k0, P3, r.P0);
   r.P1 = sel4f(k1, P3, r.P1);
   r.P2 = sel4f(k2, P3, r.P2);
-  r.sl = sln;
   r.o01 = n01; r.o12 = n12; r.o20 = n20;
   /* composite the segment */
   if (!LIN) {
@@ -1323,10 +1297,40 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
   }
   if (TFL) r.cpi = ci; else r.cP = cN;
   r.prev = zd;
+  /* the link: the next tet (the current one again at a boundary, unused),
+   * its fourth vertex, requested here and waited for by the next step, and
+   * the slots in its record: vertex k and the fourth vertex trade places (P3
+   * replaces vertex k, the neighbour's vertex off the face becomes the
+   * fourth, and the map sends k's slot to that vertex's slot), then every
+   * byte goes through the map.  Neighbour words index the record arena (a
+   * server rebases each resident mesh's words when it copies them in). */
+  const uint32_t i3n = qw & VFR_REC_VERT_MASK;
+  const uint32_t smap = sperm[qw >> VFR_REC_VERT_BITS];
+  const uint32_t tn = nb < VFR_NB_BROKEN ? (nb >> 2) : r.t;
+  r.Q = ld32<float4>(a.vert, (r.voff + i3n) * 16u);
+  const uint32_t swp = c1 ? 0x02030100u : (c2 ? 0x00020103u : 0x01020300u);
+  r.sl = __builtin_amdgcn_perm(smap, smap, __builtin_amdgcn_perm(r.sl, r.sl, swp));
+  if (LDSR) {
+    /* the next tet's record; lanes at a boundary or on a skipped step keep to
+     * themselves */
+    bool own = true;
+    if (nb < VFR_NB_BROKEN && !skip) {
+      const uint32_t lane = threadIdx.x & 63u, h = tn & 63u;
+      uint32_t* tag = ldsr_tags(sperm);
+      /* the tet that wins slot h, then one lane among those asking for it */
+      tag[h] = tn;
+      if (tag[h] == tn) {
+        tag[64 + h] = lane;
+        own = tag[64 + h] == lane;
+        r.bits |= (h << 16) | FTB_LSHARED | (own ? FTB_LOWNER : 0u);
+      }
+    }
+    if (own) {
+      r.fa = ld32<uint4>(a.rec, tn * 32u);
+      r.fb = ld32<uint4>(a.rec, tn * 32u + 16u);
+    }
+  }
   r.t = tn;
-  r.Q = Qn;
-  r.fa = fan;
-  r.fb = fbn;
   /* linear: the reference's stop once the opacity passes 1 (vfRayKernel.cu:
    * 799-803), opacity clamped to 1 */
   const bool lstop = LIN && a.composite == VFR_COMPOSITE_LINEAR_OPACITY && r.Tr > 1.0f;
@@ -1345,7 +1349,7 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
 /* After an exit through an external face: continue at the first entry
  * strictly behind the exit point; false when there is none (the ray is
  * finished). */
-template <bool TFL>
+template <bool TFL, bool REC = false>
 DEV bool ftb_reenter(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const float exitZ = ftb_persp(r) ? 1.0f / r.prev : r.prev;
   uint32_t cur = ftb_cur(r) + 1u;
@@ -1353,7 +1357,7 @@ DEV bool ftb_reenter(const TravArgs& a, const float4* tfl, FtbRay& r) {
   while (cur < nk && !(funkey((uint32_t)(a.entries[cur * a.estride + r.gp] >> 32)) < exitZ)) ++cur;
   if (cur >= nk) return false;
   r.bits = (r.bits & ~63u) | cur;
-  if (!ftb_segment<TFL>(a, tfl, r)) { r.bits |= FTB_DEAD; return false; }
+  if (!ftb_segment<TFL, REC>(a, tfl, r)) { r.bits |= FTB_DEAD; return false; }
   return true;
 }
 
@@ -1383,7 +1387,7 @@ DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
 template <int PROJ, bool BIG, bool TFL, bool PSIT, bool LIN = false, bool LDSR = false>
 DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, const uint32_t* sperm, uint32_t vi, uint32_t pix) {
   FtbRay r;
-  ftb_init<TFL, LIN>(a, tfl, vi, pix, r);
+  ftb_init<TFL, LIN, LDSR>(a, tfl, vi, pix, r);
   if (LDSR) {
     /* a lane whose ray has just ended takes part in one more exchange, so
      * that a record it loaded for other lanes reaches them */
@@ -1393,7 +1397,7 @@ DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, const uint32_t* sperm
       if (!live) break;
       const uint32_t st = ftb_step<PROJ, BIG, TFL, PSIT, LIN, true>(a, tfl, sperm, r);
       if (st == FTB_GO) continue;
-      if (st == FTB_REENTER && ftb_reenter<TFL>(a, tfl, r)) continue;
+      if (st == FTB_REENTER && ftb_reenter<TFL, true>(a, tfl, r)) continue;
       live = false;
     }
   } else if (!(r.bits & FTB_DEAD)) {
