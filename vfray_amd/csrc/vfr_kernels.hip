@@ -1432,10 +1432,13 @@ DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
 /* One ray per thread over the compacted ray list. */
 /* TFL: the launch's transfer-function slots [tf_lds_base, +tf_lds_n) are
  * staged in LDS once per block (every view of the launch uses one of them). */
-/* LDSR: the FTB walk with LDS-staged tet records (256-thread blocks). */
+/* LDSR: the FTB walk with LDS-staged tet records (256-thread blocks).
+ * Waves per SIMD: 7 for the FTB walk (72 VGPRs, no spills; 6 measured 1.6 %
+ * slower and 8 spills), 6 with LDS-staged records, 5 for the REF walk. */
+#define VFR_TRAV_WAVES(MODE, LDSR) ((MODE) == VFR_MODE_FTB ? ((LDSR) ? 6 : 7) : 5)
 template <uint32_t MODE, int PROJ, bool BIG, bool TFL, bool PSIT, bool LDSR = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == VFR_MODE_FTB ? 6 : 5,
-                                                                     MODE == VFR_MODE_FTB ? 6 : 5))) void k_trav(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VFR_TRAV_WAVES(MODE, LDSR),
+                                                                     VFR_TRAV_WAVES(MODE, LDSR)))) void k_trav(
     TravArgs a) {
   __shared__ float4 tfs[TFL ? VFR_TF_LDS_SLOTS * 256 : 1];
   __shared__ __align__(16) uint32_t sperm[LDSR ? LDSR_WORDS : VFR_N_SLOT_PERMS];
