@@ -1448,7 +1448,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VFR_TRAV_WA
     for (uint32_t k = threadIdx.x; k < a.tf_lds_n * 256u; k += blockDim.x) tfs[k] = a.tf[a.tf_lds_base * 256u + k];
   }
   if (TFL || ftb) __syncthreads();
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  /* Blocks go to the 8 XCDs round robin (block b to XCD b % 8).  Within each
+   * run of 128 blocks, XCD x walks the consecutive ray blocks 16x .. 16x+15
+   * (neighbouring screen tiles), so its L2 holds one region of the mesh
+   * instead of 16 scattered ones (FTB walk 1-2 % faster; one contiguous
+   * eighth of the ray list per XCD was slower: the XCDs finish unevenly). */
+  uint32_t lb = blockIdx.x;
+  if (ftb && lb < (gridDim.x & ~127u)) {
+    const uint32_t w = lb & 127u;
+    lb = (lb & ~127u) | ((w & 7u) << 4) | (w >> 3);
+  }
+  const uint32_t i = lb * blockDim.x + threadIdx.x;
   const uint32_t n = *a.n_rays;
   unsigned long long st = 0, an = 0, br = 0;
   if (i < n) {
