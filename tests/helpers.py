@@ -52,3 +52,37 @@ def legacy_tf_text(seed: int = 7) -> str:
     vals[rng.random((256, 4)) < 0.1] = 0.0
     lines = [f"{i} " + " ".join(repr(float(v)) for v in vals[i]) for i in range(256)]
     return "\n".join(head + lines) + "\n"
+
+
+def brute_con(t: np.ndarray) -> np.ndarray:
+    """Neighbour = smallest other tet holding all three face vertices (O(T^2)),
+    the reference's rule (vfRayPreComp.cc:219-270) by exhaustive search."""
+    nT = t.shape[0]
+    sets = [set(map(int, r)) for r in t]
+    con = np.zeros((nT, 4), np.uint32)
+    for i in range(nT):
+        for f in range(4):
+            fv = {int(t[i, (k + f) % 4]) for k in range(3)}
+            cand = [j for j in range(nT) if j != i and fv <= sets[j]]
+            con[i, f] = min(cand) if cand else i
+    return con
+
+
+CONNECTIVITY_CASES = ("one_tet", "cube", "grid2", "nonmanifold", "grid3_jitter")
+
+
+def connectivity_case(case: str):
+    """Vertices (nV x 4: x, y, z, s) and tets of a small connectivity case;
+    "nonmanifold" puts three tets on the face (0, 1, 2)."""
+    if case == "one_tet":
+        t = np.array([[0, 1, 2, 3]], np.uint32)
+        x = np.array([[0, 0, 0, 0.1], [1, 0, 0, 0.2], [0, 1, 0, 0.3], [0, 0, 1, 0.4]], np.float32)
+    elif case == "nonmanifold":
+        t = np.array([[0, 1, 2, 3], [0, 1, 2, 4], [2, 1, 0, 5], [3, 4, 1, 2]], np.uint32)
+        x = np.array([[0, 0, 0, 0.1], [1, 0, 0, 0.2], [0, 1, 0, 0.3], [0.2, 0.2, 1, 0.4],
+                      [0.3, 0.3, -1, 0.5], [0.25, 0.2, 0.5, 0.6]], np.float32)
+    else:
+        n = {"cube": 1, "grid2": 2, "grid3_jitter": 3}[case]
+        x, t = V.synth_grid(n, 11, 0)
+    return x, t
+

@@ -16,7 +16,7 @@ import pytest
 import vfray_amd as V
 from vfray_amd import workloads as WL
 import oracle as O
-from helpers import c1_inputs, grid_inputs, oracle_ftb, psnr
+from helpers import CONNECTIVITY_CASES, brute_con, c1_inputs, connectivity_case, grid_inputs, oracle_ftb, psnr
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -242,31 +242,12 @@ def test_expf_accuracy():
 
 # ----------------------------------------------------- connectivity pins
 
-def _brute_con(t: np.ndarray) -> np.ndarray:
-    """Neighbour = smallest other tet holding all three face vertices (O(T^2))."""
-    nT = t.shape[0]
-    sets = [set(map(int, r)) for r in t]
-    con = np.zeros((nT, 4), np.uint32)
-    for i in range(nT):
-        for f in range(4):
-            fv = {int(t[i, (k + f) % 4]) for k in range(3)}
-            cand = [j for j in range(nT) if j != i and fv <= sets[j]]
-            con[i, f] = min(cand) if cand else i
-    return con
-
-
-@pytest.mark.parametrize("case", ["one_tet", "cube", "grid2", "nonmanifold", "grid3_jitter"])
+@pytest.mark.parametrize("case", CONNECTIVITY_CASES)
 def test_connectivity_bruteforce(case):
-    if case == "one_tet":
-        t = np.array([[0, 1, 2, 3]], np.uint32)
-    elif case == "nonmanifold":   # three tets on the face (0,1,2)
-        t = np.array([[0, 1, 2, 3], [0, 1, 2, 4], [2, 1, 0, 5], [3, 4, 1, 2]], np.uint32)
-    else:
-        n = {"cube": 1, "grid2": 2, "grid3_jitter": 3}[case]
-        _, t = V.synth_grid(n, 11, 0)
+    _, t = connectivity_case(case)
     nV = int(t.max()) + 1
     con, n_ext = O.connectivity(t, nV)
-    assert np.array_equal(con, _brute_con(t))
+    assert np.array_equal(con, brute_con(t))
     assert n_ext == int((con == np.arange(t.shape[0])[:, None]).sum())
     if case in ("cube", "grid2", "grid3_jitter"):
         n = {"cube": 1, "grid2": 2, "grid3_jitter": 3}[case]

@@ -11,7 +11,7 @@ import pytest
 import vfray_amd as V
 from vfray_amd import workloads as WL
 import oracle as O
-from helpers import c1_inputs, grid_inputs, oracle_ftb, psnr
+from helpers import CONNECTIVITY_CASES, brute_con, c1_inputs, connectivity_case, grid_inputs, oracle_ftb, psnr
 
 pytestmark = pytest.mark.gpu
 
@@ -89,6 +89,25 @@ def test_connectivity_32(ctx):
     assert np.array_equal(con, inp["con"])
     assert np.array_equal(ext, inp["ext"])
     assert np.array_equal(lmt.view(np.uint32), inp["lmt"].view(np.uint32))
+    m.close()
+
+
+@pytest.mark.parametrize("case", CONNECTIVITY_CASES)
+@pytest.mark.parametrize("morton", [False, True])
+def test_connectivity_brute_force(ctx, case, morton):
+    """The device's sort-and-match connectivity against exhaustive search,
+    including a face shared by three tets (the smallest other tet wins) and
+    single-tet and 1^3..3^3 grids (every face external on one tet)."""
+    print(f"connectivity {case} morton={morton}", flush=True)
+    x, t = connectivity_case(case)
+    m = ctx.mesh(x, t, morton=morton)
+    con, ext, _ = m.connectivity()
+    bc = brute_con(t)
+    assert np.array_equal(con, bc)
+    own = np.arange(t.shape[0])[:, None]
+    want_ext = np.argwhere(bc == own).astype(np.uint32)   # (tet, face), tet-major order
+    assert np.array_equal(ext, want_ext)
+    assert m.info["n_ext_faces"] == want_ext.shape[0]
     m.close()
 
 
