@@ -678,7 +678,20 @@ static void dispatcher_pipeline(vfr_server* sv) {
         lk.unlock();
         int rc = pipe_prep(sv, open, batch, vb);
         lk.lock();
-        if (rc != VFR_OK) pipe_fail(sv, batch, rc);
+        if (rc != VFR_OK) {
+          /* the failed batch leaves the open list (it was appended last), so
+           * the list's walk does not report these requests as rendered */
+          pipe_fail(sv, batch, rc);
+          for (Req* r : batch) {
+            sv->r_pix.release(r->pix_off, r->pixels);
+            sv->r_verts.release(r->vert_off, sv->meshes[r->mesh].n_verts);
+            open->keep_pix -= r->keep ? r->pixels : 0;
+            open->pix -= r->pixels;
+            open->n_persp -= r->dv.proj == VFR_PROJ_PERSPECTIVE ? 1u : 0u;
+          }
+          sv->r_views.release(vb, batch.size());
+          open->reqs.resize(open->reqs.size() - batch.size());
+        }
       }
     }
     /* walk the open list on a free walk stream */
