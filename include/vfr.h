@@ -82,7 +82,9 @@ int vfr_tf_create(vfr_ctx* ctx, const float rgba[1024], int* tf_id);
 
 /* ------------------------------------------------------------------ mesh */
 #define VFR_MESH_NORMALIZE 0x1u  /* apply normalizeVertices on upload      */
-#define VFR_MESH_MORTON 0x2u     /* reorder vertices/tets along a Z-curve  */
+#define VFR_MESH_MORTON 0x2u     /* reorder tets along a Z-curve of their
+                                     centroids, vertices in the order those
+                                     tets first use them                   */
 
 typedef struct vfr_mesh_info {
   uint32_t n_verts, n_tets, n_ext_faces;
