@@ -47,10 +47,9 @@ def build_lib(verbose: bool = True) -> str:
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "vfr_api.hip"), "-o", os.path.join(OBJ, "vfr_api.o")],
         ["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "vfr_host.cpp"), "-o", os.path.join(OBJ, "vfr_host.o")],
     ]
-    for extra in ("vfr_persist.hip", "vfr_server.hip", "vfr_noalloc.hip"):
-        if os.path.exists(os.path.join(CSRC, extra)):
-            jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, extra),
-                         "-o", os.path.join(OBJ, extra.replace(".hip", ".o"))])
+    for extra in ("vfr_server.hip", "vfr_noalloc.hip"):
+        jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, extra),
+                     "-o", os.path.join(OBJ, extra.replace(".hip", ".o"))])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         list(ex.map(_run, jobs))
     objs = [j[-1] for j in jobs]
