@@ -431,8 +431,10 @@ def test_ftb_concave_reentry(ctx, psi, term):
 
 
 def test_ftb_concave_c4_views(ctx, psi):
-    """The concave benchmark mesh (96^3 grid, graded, 12 voids): two of its
-    32 views at 768^2 against the oracle."""
+    """The concave benchmark mesh (96^3 grid, graded, 12 voids), all 32 views
+    at 768^2 against the oracle (entry ids and steps bit for bit, RGBA within
+    2e-3), two of them also against its FP64 re-integration (PSNR >= 50 dB);
+    too few entry slots are reported, more render the same."""
     from vfray_amd import workloads as WL
     x, t, tfa, seed = WL.concave()
     inp = dict(x=x, t=t, tf=tfa, xn=O.normalize(x))
@@ -441,34 +443,37 @@ def test_ftb_concave_c4_views(ctx, psi):
     inp["lmt"] = O.lmt(inp["xn"], t)
     m = ctx.mesh(x, t)
     tf = ctx.tf(tfa)
-    views = WL.sphere_views(seed, 32, [768], tf)[:2]
-    print("FTB concave 96^3 2x768^2", flush=True)
+    views = WL.sphere_views(seed, 32, [768], tf)
+    print("FTB concave 96^3 32x768^2", flush=True)
     # up to 13 entry faces per pixel here: 8 slots drop some, and say so
     with pytest.raises(V.VfrError, match="dropped"):
-        ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, max_entries=8)
+        ctx.render(m, views[:2], mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, max_entries=8)
     outs, st = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+    assert st["anomalies"] == 0
     # 32 slot planes instead of the 16 chosen: the same lists, the same images
-    outs32, _ = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, max_entries=32)
+    outs32, _ = ctx.render(m, views[:2], mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, max_entries=32)
     for g, h in zip(outs, outs32):
         for key in ("rgba", "entry_ext", "steps"):
             assert np.array_equal(g[key], h[key]), key
-    for v, g in zip(views, outs):
-        orc = oracle_ftb(inp, v, psi, fp64=True)
+    for i, (v, g) in enumerate(zip(views, outs)):
+        print(f"oracle concave view {i}", flush=True)
+        orc = oracle_ftb(inp, v, psi, fp64=i < 2)
         assert orc["anomalies"] == 0
-        assert np.array_equal(g["entry_ext"], orc["entry_ext"])
-        assert np.array_equal(g["steps"], orc["steps"])
-        assert np.abs(g["rgba"] - orc["rgba"]).max() <= TOL
-        cov = orc["entry_ext"] != V.NO_ID
-        assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
+        assert np.array_equal(g["entry_ext"], orc["entry_ext"]), i
+        assert np.array_equal(g["steps"], orc["steps"]), i
+        assert np.abs(g["rgba"] - orc["rgba"]).max() <= TOL, i
+        if i < 2:
+            cov = orc["entry_ext"] != V.NO_ID
+            assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
     m.close()
 
 
 def test_ftb_batch64_full(ctx, psi):
     """The benchmark's workload at full size: 128^3 grid (12.58 M tets), all
     64 views.  Batched equals per-view bit for bit on every view, the stats
-    add up, no anomalies; three of the views (256^2, 384^2, 512^2) equal the
-    oracle (entry ids and steps bit for bit, RGBA within 2e-3, PSNR >= 50 dB
-    against FP64)."""
+    add up, no anomalies; every view equals the oracle (entry ids and steps
+    bit for bit, RGBA within 2e-3), and three of them (256^2, 384^2, 512^2)
+    also meet PSNR >= 50 dB against its FP64 re-integration."""
     from vfray_amd import workloads as WL
     x, t, tfa, seed = WL.batch64()
     m = ctx.mesh(x, t)
@@ -487,18 +492,19 @@ def test_ftb_batch64_full(ctx, psi):
     inp["con"], _ = O.connectivity(t, x.shape[0])
     inp["ext"] = O.extf(inp["con"])
     inp["lmt"] = O.lmt(inp["xn"], t)
-    for side in (256, 384, 512):
-        i = [k for k, v in enumerate(views) if v.width == side][0]
-        print(f"oracle view {i} {side}^2", flush=True)
-        orc = oracle_ftb(inp, views[i], psi, fp64=True)
+    fp64_views = {[k for k, v in enumerate(views) if v.width == side][0] for side in (256, 384, 512)}
+    for i, v in enumerate(views):
+        print(f"oracle view {i} {v.width}^2", flush=True)
+        orc = oracle_ftb(inp, v, psi, fp64=i in fp64_views)
         g = a[i]
         assert orc["anomalies"] == 0
-        assert np.array_equal(g["entry_ext"], orc["entry_ext"])
-        assert np.array_equal(g["steps"], orc["steps"])
-        assert np.abs(g["rgba"] - orc["rgba"]).max() <= TOL
+        assert np.array_equal(g["entry_ext"], orc["entry_ext"]), i
+        assert np.array_equal(g["steps"], orc["steps"]), i
+        assert np.abs(g["rgba"] - orc["rgba"]).max() <= TOL, i
         cov = orc["entry_ext"] != V.NO_ID
         assert cov.mean() > 0.2
-        assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
+        if i in fp64_views:
+            assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
     m.close()
 
 
