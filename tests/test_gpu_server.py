@@ -179,10 +179,10 @@ def test_server_c5_stream_matches_per_view_and_oracle(ctx, schedule):
     """The serving workload itself: its 8 resident meshes (32^3..160^3
     grids, 0.2M-24.6M tets, 4 TFs each) and the first 200 requests of its
     stream, all queued at once.  Every image equals the same request rendered
-    alone by the per-view walk, bit for bit; for one request of at most 512^2
-    per mesh the image meets the oracle bar (entry faces and step counts
-    bit-exact with the CPU oracle, RGBA within 2e-3, PSNR >= 50 dB against its
-    FP64 re-integration)."""
+    alone by the per-view walk, bit for bit, and meets the oracle bar (entry
+    faces and step counts bit-exact with the CPU oracle, RGBA within 2e-3);
+    for one request of at most 512^2 per mesh also PSNR >= 50 dB against the
+    oracle's FP64 re-integration."""
     import oracle as O
     from helpers import oracle_ftb, psnr
     meshes, tf_ids, host = [], [], []
@@ -195,6 +195,7 @@ def test_server_c5_stream_matches_per_view_and_oracle(ctx, schedule):
     sv = V.Server(ctx, meshes, hbm_budget=16 << 30, streams=2, max_batch=256, max_batch_pixels=4 << 20,
                   schedule=schedule)
     checked = set()
+    inputs = {}
     psi = V.psi_generate()
     try:
         tickets = [sv.submit(r.mesh, r.view, keep_image=True) for r in reqs]
@@ -205,18 +206,20 @@ def test_server_c5_stream_matches_per_view_and_oracle(ctx, schedule):
             sizes.append(g["batch_size"])
             alone = ctx.render(meshes[r.mesh], [r.view], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)[0][0]
             assert np.array_equal(g["rgba"], alone["rgba"]), i
-            if r.mesh not in checked and r.view.width <= 512:
-                x, t, tfs = host[r.mesh]
-                print(f"request {i}: mesh {r.mesh} {r.view.width}^2 against the oracle", flush=True)
+            x, t, tfs = host[r.mesh]
+            if r.mesh not in inputs:
                 con, ext, _ = meshes[r.mesh].connectivity()
                 xn = O.normalize(x)
-                tfa = tfs[tf_ids[r.mesh].index(r.view.tf_id)]
-                inp = dict(xn=xn, t=t, con=con, ext=ext, lmt=O.lmt(xn, t), tf=tfa)
-                orc = oracle_ftb(inp, r.view, psi, fp64=True)
-                assert orc["anomalies"] == 0
-                assert np.array_equal(alone["entry_ext"], orc["entry_ext"])
-                assert np.array_equal(alone["steps"], orc["steps"])
-                assert np.abs(g["rgba"] - orc["rgba"]).max() <= 2e-3
+                inputs[r.mesh] = dict(xn=xn, t=t, con=con, ext=ext, lmt=O.lmt(xn, t))
+            inp = dict(inputs[r.mesh], tf=tfs[tf_ids[r.mesh].index(r.view.tf_id)])
+            fp64 = r.mesh not in checked and r.view.width <= 512
+            print(f"request {i}: mesh {r.mesh} {r.view.width}^2 against the oracle (fp64 {fp64})", flush=True)
+            orc = oracle_ftb(inp, r.view, psi, fp64=fp64)
+            assert orc["anomalies"] == 0
+            assert np.array_equal(alone["entry_ext"], orc["entry_ext"]), i
+            assert np.array_equal(alone["steps"], orc["steps"]), i
+            assert np.abs(g["rgba"] - orc["rgba"]).max() <= 2e-3, i
+            if fp64:
                 cov = orc["entry_ext"] != V.NO_ID
                 assert cov.mean() > 0.1
                 assert psnr(g["rgba"], orc["rgba64"], cov) >= 50.0
