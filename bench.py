@@ -374,6 +374,11 @@ def main() -> int:
                 "traffic": tr["traffic_bytes"] if tr else None,
                 "traffic_source": f"profiles/{kname}_{args.workload}.md" if tr else None,
                 "l2_hit": tr["l2_hit"] if tr else None,
+                # what bounds the walk: the vector memory pipeline between the
+                # waves and L2 (same profile; per CU, shares of the launch's cycles)
+                "memory_pipeline": None if not tr or "l1_lookups_per_cu_cycle" not in tr else {
+                    "l1_line_lookups_per_cu_cycle": tr["l1_lookups_per_cu_cycle"], "l1_hit": tr["l1_hit"],
+                    "texture_data_busy": tr["td_busy"], "l1_waiting_on_l2": tr["l1_pending_stall"]},
                 "kernel": kname, "kernel_ms": round(ms_trav, 3),
                 "model_bytes": int(model_bytes),
                 "model_gbs": round(model_bytes / (ms_trav * 1e-3) / 1e9, 1),
