@@ -584,7 +584,11 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
     const uint32_t W = views[vi].W, H = views[vi].H;
     uint32_t tx, ty;
     tile_xy(gt - views[vi].tile_off, views[vi].tiles_x, (H + 7u) / 8u, &tx, &ty);
-    const uint32_t px = tx * 8u + (lane & 7u), py = ty * 8u + (lane >> 3);
+    /* lanes in Z order over the tile: lanes 4k..4k+3 are a 2x2 quad, 8k..8k+7
+     * a 4x2 block (the lane groups a gather's addresses are looked up in) */
+    const uint32_t lx = (lane & 1u) | ((lane >> 1) & 2u) | ((lane >> 2) & 4u);
+    const uint32_t ly = ((lane >> 1) & 1u) | ((lane >> 2) & 2u) | ((lane >> 3) & 4u);
+    const uint32_t px = tx * 8u + lx, py = ty * 8u + ly;
     const bool valid = px < W && py < H;
     pix = py * W + px;
     const uint32_t gp = views[vi].pix_off + pix;
@@ -1158,6 +1162,14 @@ enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
 DEV uint32_t* ldsr_tags(const uint32_t* sperm) {
   return const_cast<uint32_t*>(sperm) + 32 + (threadIdx.x >> 6) * 128u;
 }
+/* Tag accesses are relaxed wave-scope atomics: each lane reads back what the
+ * whole wave wrote, so the compiler must not forward a lane's own store to
+ * its load (with plain accesses it merged the stores and dropped the loads,
+ * and every lane became the owner of its own record). */
+DEV void tag_put(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+DEV uint32_t tag_get(uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT); }
 DEV uint4* ldsr_recs(const uint32_t* sperm) {
   return reinterpret_cast<uint4*>(const_cast<uint32_t*>(sperm) + 32 + 4 * 128) + (threadIdx.x >> 6) * 128u;
 }
@@ -1318,10 +1330,10 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
       const uint32_t lane = threadIdx.x & 63u, h = tn & 63u;
       uint32_t* tag = ldsr_tags(sperm);
       /* the tet that wins slot h, then one lane among those asking for it */
-      tag[h] = tn;
-      if (tag[h] == tn) {
-        tag[64 + h] = lane;
-        own = tag[64 + h] == lane;
+      tag_put(tag + h, tn);
+      if (tag_get(tag + h) == tn) {
+        tag_put(tag + 64 + h, lane);
+        own = tag_get(tag + 64 + h) == lane;
         r.bits |= (h << 16) | FTB_LSHARED | (own ? FTB_LOWNER : 0u);
       }
     }
