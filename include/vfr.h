@@ -221,6 +221,15 @@ typedef struct vfr_render_opts {
  * lane per tet, elected through a 64-slot table in LDS) and the others read
  * it from LDS.  Same results; other walks ignore the flag. */
 #define VFR_RENDER_LDS_RECORDS 0x20u
+/* REF-mode walks read each exit face's z and s planes from a table of the
+ * planes of every tet face, per view, instead of solving them per step: the
+ * GPU form of the reference CPU walk's face-plane memo
+ * (vfRayKernelCPU.cc:234-245, 369-435), which keeps solved planes keyed by
+ * (tet, face) so that later rays of the same face reuse them.  The table
+ * (32 bytes per tet face per view) is filled after the vertex transform of
+ * each render with the walk's own arithmetic, so the results are the same
+ * bits.  Front-to-back walks ignore the flag (they solve no face planes). */
+#define VFR_RENDER_FACE_MEMO 0x40u
 
 typedef struct vfr_frame_out {
   float* rgba;          /* W*H*4, may be NULL                              */
@@ -255,14 +264,17 @@ void vfr_plan_destroy(vfr_plan* plan);
 /* Per-view frame latency (ms, HIP events) of the last PER_VIEW execute. */
 int vfr_plan_frame_times(const vfr_plan* plan, double* out_ms);
 
-/* Times the one-ray-per-thread traversal (with and without LDS-staged
- * records), the wave work queue and candidate persistent-kernel schedules
- * (kernel occupancy variant x waves per CU x refill threshold) on the plan
- * itself and keeps the fastest walk, then keeps the pipelined execution if it
- * is faster end to end; returns the choice (flags holds
- * VFR_RENDER_WAVE_QUEUE / VFR_RENDER_PERSISTENT / VFR_RENDER_LDS_RECORDS /
- * VFR_RENDER_PIPELINED) and the walk time.  Results are cached per (tets,
- * rays, views, psi variant) in the process. */
+/* Front-to-back plans: times the one-ray-per-thread traversal (with and
+ * without LDS-staged records), the wave work queue and candidate
+ * persistent-kernel schedules (kernel occupancy variant x waves per CU x
+ * refill threshold) on the plan itself and keeps the fastest walk, then
+ * keeps the pipelined execution if it is faster end to end.  REF plans: times
+ * the whole render with and without the face-plane memo and keeps the faster.
+ * Returns the choice (flags holds VFR_RENDER_WAVE_QUEUE /
+ * VFR_RENDER_PERSISTENT / VFR_RENDER_LDS_RECORDS / VFR_RENDER_PIPELINED /
+ * VFR_RENDER_FACE_MEMO) and the time it chose by (the walk's; the whole
+ * render's for REF plans).  Results are cached per (mode, tets, rays, views,
+ * psi variant) in the process.  Batched plans only. */
 int vfr_plan_autotune(vfr_plan* plan, uint32_t* flags, uint32_t* occupancy,
                       uint32_t* waves_per_cu, uint32_t* refill, double* best_ms);
 

@@ -130,6 +130,56 @@ def test_ref_mode_c1(ctx, c1, psi):
     m.close()
 
 
+def _rot(a, b):
+    """A 3x4 orthographic view matrix: rotation about z by a, then about x by b (radians)."""
+    ca, sa, cb, sb = np.cos(a), np.sin(a), np.cos(b), np.sin(b)
+    rz = np.array([[ca, -sa, 0], [sa, ca, 0], [0, 0, 1]])
+    rx = np.array([[1, 0, 0], [0, cb, -sb], [0, sb, cb]])
+    return np.hstack([rx @ rz, np.zeros((3, 1))]).astype(np.float32)
+
+
+def test_ref_face_memo(ctx, c1, psi):
+    """The face-plane memo (VFR_RENDER_FACE_MEMO) on the REF walk: the
+    reference's own case renders bit for bit as the walk that solves each
+    exit face (entry ids, steps, RGBA) and as the oracle, per-view and
+    batched, in PPI and linear compositing; a batch of rotated views (one
+    memo table per view) equals the solving walk view by view; the
+    autotuner's REF choice renders the same bits."""
+    print("REF face memo 16^3 512x512", flush=True)
+    m = ctx.mesh(c1["x"], c1["t"])
+    tf = ctx.tf(c1["tf"])
+    for composite in (V.COMPOSITE_PPI, V.COMPOSITE_LINEAR_OPACITY):
+        orc = O.render_ref(c1["xn"], c1["t"], c1["con"], c1["ext"], c1["tf"], psi, c1["lmt"], 512, 512,
+                           composite=composite)
+        for sched in (V.SCHED_PER_VIEW, V.SCHED_BATCHED):
+            a, sa = ctx.render(m, [V.ortho_view(512, 512, tf)], mode=V.MODE_REF, schedule=sched,
+                               composite=composite)
+            b, sb = ctx.render(m, [V.ortho_view(512, 512, tf)], mode=V.MODE_REF, schedule=sched,
+                               composite=composite, face_memo=True)
+            for key in ("rgba", "entry_ext", "steps"):
+                assert np.array_equal(a[0][key], b[0][key]), (composite, sched, key)
+            assert np.array_equal(b[0]["entry_ext"], orc["entry_ext"])
+            assert np.array_equal(b[0]["steps"], orc["steps"])
+            assert np.abs(b[0]["rgba"] - orc["rgba"]).max() <= TOL
+            assert sa["steps"] == sb["steps"] == int(orc["steps"].sum())
+    print("REF face memo, 4 rotated views", flush=True)
+    views = [V.ortho_view(160 + 16 * i, 144, tf, _rot(0.4 * i + 0.1, 0.3 * i - 0.5)) for i in range(4)]
+    a, sa = ctx.render(m, views, mode=V.MODE_REF, schedule=V.SCHED_BATCHED)
+    b, sb = ctx.render(m, views, mode=V.MODE_REF, schedule=V.SCHED_BATCHED, face_memo=True)
+    assert sa["steps"] == sb["steps"] and sa["active_rays"] > 0
+    for ga, gb in zip(a, b):
+        for key in ("rgba", "entry_ext", "steps"):
+            assert np.array_equal(ga[key], gb[key]), key
+    p = ctx.plan(m, views, mode=V.MODE_REF, schedule=V.SCHED_BATCHED)
+    choice = p.autotune()
+    assert isinstance(choice["face_memo"], bool) and choice["traversal_ms"] > 0
+    p.execute()
+    for i in range(len(views)):
+        assert np.array_equal(p.read(i)["rgba"], a[i]["rgba"])
+    p.close()
+    m.close()
+
+
 @pytest.mark.parametrize("composite", [V.COMPOSITE_LINEAR, V.COMPOSITE_LINEAR_OPACITY])
 def test_ref_linear_c1(ctx, c1, psi, composite):
     """Linear compositing (USEPARTIALPREINT 0) on the REF walk: image bytes,

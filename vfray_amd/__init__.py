@@ -24,6 +24,7 @@ RENDER_PSI_TABLE = 0x4
 RENDER_WIDE = 0x8
 RENDER_WAVE_QUEUE = 0x10
 RENDER_LDS_RECORDS = 0x20
+RENDER_FACE_MEMO = 0x40
 SCHED_PER_VIEW, SCHED_BATCHED = 0, 1
 SERVE_SLOTS, SERVE_PIPELINE = 0, 1
 MESH_NORMALIZE, MESH_MORTON = 0x1, 0x2
@@ -409,10 +410,10 @@ class Context:
     def plan(self, mesh: "Mesh", views, mode=MODE_FTB, schedule=SCHED_BATCHED, term_alpha=0.99,
              use_graph=False, waves_per_cu=0, refill=0, persistent=False, occupancy=0,
              max_entries=0, pipelined=False, composite=COMPOSITE_PPI, psi_table=False, wide=False,
-             wave_queue=False, lds_records=False) -> "Plan":
+             wave_queue=False, lds_records=False, face_memo=False) -> "Plan":
         return Plan(self, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
                     persistent, occupancy, max_entries, pipelined, composite, psi_table, wide,
-                    wave_queue, lds_records)
+                    wave_queue, lds_records, face_memo)
 
     def render(self, mesh: "Mesh", views, **kw) -> tuple[list[dict], dict]:
         p = self.plan(mesh, views, **kw)
@@ -466,13 +467,14 @@ class Mesh:
 class Plan:
     def __init__(self, ctx, mesh, views, mode, schedule, term_alpha, use_graph, waves_per_cu, refill,
                  persistent=False, occupancy=0, max_entries=0, pipelined=False, composite=COMPOSITE_PPI,
-                 psi_table=False, wide=False, wave_queue=False, lds_records=False):
+                 psi_table=False, wide=False, wave_queue=False, lds_records=False, face_memo=False):
         self.ctx, self.mesh = ctx, mesh
         self.views = list(views)
         arr = (View * len(self.views))(*self.views)
         flags = ((RENDER_PERSISTENT if persistent else 0) | (RENDER_PIPELINED if pipelined else 0)
                  | (RENDER_PSI_TABLE if psi_table else 0) | (RENDER_WIDE if wide else 0)
-                 | (RENDER_WAVE_QUEUE if wave_queue else 0) | (RENDER_LDS_RECORDS if lds_records else 0))
+                 | (RENDER_WAVE_QUEUE if wave_queue else 0) | (RENDER_LDS_RECORDS if lds_records else 0)
+                 | (RENDER_FACE_MEMO if face_memo else 0))
         o = RenderOpts(mode, schedule, term_alpha, waves_per_cu, refill, 1 if use_graph else 0, max_entries,
                        flags, occupancy, composite)
         self._p = C.c_void_p()
@@ -484,6 +486,7 @@ class Plan:
         return {"persistent": bool(f.value & RENDER_PERSISTENT), "pipelined": bool(f.value & RENDER_PIPELINED),
                 "wave_queue": bool(f.value & RENDER_WAVE_QUEUE),
                 "lds_records": bool(f.value & RENDER_LDS_RECORDS),
+                "face_memo": bool(f.value & RENDER_FACE_MEMO),
                 "occupancy": o.value, "waves_per_cu": w.value, "refill": r.value, "traversal_ms": ms.value}
 
     def execute(self) -> dict:

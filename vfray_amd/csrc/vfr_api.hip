@@ -306,6 +306,7 @@ struct vfr_plan {
   uint32_t kmax = VFR_K;       /* FTB entry slots per pixel */
   uint64_t* rays = nullptr;    /* [n_pix] */
   float4* pverts = nullptr;    /* [n_views * n_verts] P-space vertices */
+  float4* memo = nullptr;      /* REF face-plane memo, [n_views * n_tets * 4 * 2] */
   float4* rgba = nullptr;
   uint32_t* steps = nullptr;
   uint32_t* entry = nullptr;
@@ -328,6 +329,17 @@ struct vfr_plan {
   hipStream_t st2 = nullptr;
   std::vector<hipEvent_t> cev;            /* per chunk: ready, walk start, walk end; + join */
 };
+
+/* The REF walk reads the face-plane memo (VFR_RENDER_FACE_MEMO). */
+static bool plan_memo_on(const vfr_plan* p) {
+  return p->opts.mode == VFR_MODE_REF && (p->opts.flags & VFR_RENDER_FACE_MEMO) && p->memo;
+}
+
+/* Device bytes of a plan's face-plane memo: 2 float4 per tet face per view. */
+static size_t plan_memo_bytes(const vfr_plan* p) {
+  return (size_t)p->n_views * p->mesh->d.n_tets * 4u * 2u * sizeof(float4);
+}
+
 
 /* View set-up on the host, in double, rounded once to float.  bb_lo/bb_hi:
  * the mesh's bounding box (every vertex must lie in front of a pinhole eye). */
@@ -406,7 +418,7 @@ extern "C" void vfr_plan_destroy(vfr_plan* p) {
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   void* ps[] = {p->views, p->owner, p->entries, p->rays, p->pverts, p->rgba, p->steps, p->entry,
-                p->counters, p->stats};
+                p->counters, p->stats, p->memo};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   for (auto& e : p->ev)
@@ -445,6 +457,8 @@ static int enqueue_batched(vfr_plan* p) {
   CHECK_RC(vfr_k_clear_u32(p->counters, p->n_views + 2, st));
   CHECK_RC(vfr_k_clear_u32((uint32_t*)p->stats, 2 * ST_N, st));
   CHECK_RC(vfr_k_xform_views(p->views, 0, p->n_views, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
+  if (plan_memo_on(p))
+    CHECK_RC(vfr_k_face_memo(p->views, 0, p->n_views, m->d.n_tets, m->tv4(), p->pverts, p->memo, st));
   if (p->opts.mode == VFR_MODE_REF)
     CHECK_RC(vfr_k_raster_ref(p->views, 0, p->n_views, m->d.n_ext, m->fvid4(), p->pverts, p->owner, st));
   else
@@ -491,6 +505,8 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   a.composite = p->opts.composite;
   a.psi_table = (p->opts.flags & VFR_RENDER_PSI_TABLE) ? 1u : 0u;
   a.lds_records = (p->opts.flags & VFR_RENDER_LDS_RECORDS) ? 1u : 0u;
+  a.memo = plan_memo_on(p) ? p->memo : nullptr;
+  a.memo_tets = p->mesh->d.n_tets;
   return a;
 }
 
@@ -642,6 +658,11 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
       vfr_set_error("plan allocation of %zu bytes failed", (size_t)(sizeof(uint64_t) * k * pix));
       rc = VFR_ENOMEM;
     }
+  }
+  if (rc == VFR_OK && o.mode == VFR_MODE_REF && (o.flags & VFR_RENDER_FACE_MEMO) &&
+      hipMalloc((void**)&p->memo, plan_memo_bytes(p)) != hipSuccess) {
+    vfr_set_error("face-plane memo allocation of %zu bytes failed", plan_memo_bytes(p));
+    rc = VFR_ENOMEM;
   }
   for (int i = 0; rc == VFR_OK && i < 3; ++i)
     if (hipEventCreate(&p->ev[i]) != hipSuccess) { vfr_set_error("event creation failed"); rc = VFR_EHIP; }
@@ -818,6 +839,8 @@ extern "C" int vfr_plan_execute(vfr_plan* p, vfr_stats* stats) {
       CHECK_HIP(hipEventRecord(p->fev[2 * v], st));
       CHECK_RC(vfr_k_clear_u32(p->owner + hv.pix_off, npx, st));
       CHECK_RC(vfr_k_xform_views(p->views, v, 1, m->d.n_verts, (const float4*)m->d.vert, p->pverts, st));
+      if (plan_memo_on(p))
+        CHECK_RC(vfr_k_face_memo(p->views, v, 1, m->d.n_tets, m->tv4(), p->pverts, p->memo, st));
       if (p->opts.mode == VFR_MODE_REF)
         CHECK_RC(vfr_k_raster_ref(p->views, v, 1, m->d.n_ext, m->fvid4(), p->pverts, p->owner, st));
       else
@@ -932,20 +955,54 @@ extern "C" int vfr_plan_read(vfr_plan* p, int view, vfr_frame_out* out) {
 }
 
 struct TuneKey {
+  uint32_t mode;
   uint32_t n_tets;
   uint64_t n_pix;
   uint32_t n_views;
   uint32_t psi_table;
   bool operator==(const TuneKey& o) const {
-    return n_tets == o.n_tets && n_pix == o.n_pix && n_views == o.n_views && psi_table == o.psi_table;
+    return mode == o.mode && n_tets == o.n_tets && n_pix == o.n_pix && n_views == o.n_views &&
+           psi_table == o.psi_table;
   }
 };
 struct TuneVal {
   uint32_t flags, occ, wpc, refill;
 };
 /* The flags the autotuner chooses. */
-static const uint32_t kTuned =
-    VFR_RENDER_PERSISTENT | VFR_RENDER_WAVE_QUEUE | VFR_RENDER_PIPELINED | VFR_RENDER_LDS_RECORDS;
+static const uint32_t kTuned = VFR_RENDER_PERSISTENT | VFR_RENDER_WAVE_QUEUE | VFR_RENDER_PIPELINED |
+                               VFR_RENDER_LDS_RECORDS | VFR_RENDER_FACE_MEMO;
+
+/* REF plans: the whole render (the memo is filled in the preprocessing)
+ * with and without the face-plane memo; the faster stays.  The memo is
+ * allocated for the trial and kept only if chosen. */
+static int autotune_ref(vfr_plan* p, uint32_t* chosen, double* best) {
+  const uint32_t user_flags = p->opts.flags & ~kTuned;
+  const bool had = p->memo != nullptr;
+  if (!had && hipMalloc((void**)&p->memo, plan_memo_bytes(p)) != hipSuccess) {
+    /* no room for the memo: the plain walk is the only candidate */
+    (void)hipGetLastError();
+    p->memo = nullptr;
+  }
+  vfr_stats st;
+  double t[2] = {1e30, 1e30};
+  int rc = VFR_OK;
+  for (int m = 0; m < (p->memo ? 2 : 1) && rc == VFR_OK; ++m) {
+    p->opts.flags = user_flags | (m ? VFR_RENDER_FACE_MEMO : 0u);
+    for (int rep = 0; rep < 4 && rc == VFR_OK; ++rep) {
+      rc = vfr_plan_execute(p, &st);
+      if (rep > 0) t[m] = std::min(t[m], st.ms_total);
+    }
+  }
+  const bool memo = rc == VFR_OK && t[1] < t[0];
+  *chosen = memo ? VFR_RENDER_FACE_MEMO : 0u;
+  *best = memo ? t[1] : t[0];
+  p->opts.flags = user_flags | *chosen;
+  if (!memo && !had && p->memo) {
+    (void)hipFree(p->memo);
+    p->memo = nullptr;
+  }
+  return rc;
+}
 static std::mutex g_tune_mu;
 static std::vector<std::pair<TuneKey, TuneVal>> g_tune;
 
@@ -956,13 +1013,13 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
     vfr_set_error("the mesh was rotated after the plan was created: create the plan again");
     return VFR_ESTATE;
   }
-  if (p->opts.schedule != VFR_SCHED_BATCHED || p->opts.mode != VFR_MODE_FTB) {
-    vfr_set_error("autotune applies to batched front-to-back plans");
+  if (p->opts.schedule != VFR_SCHED_BATCHED) {
+    vfr_set_error("autotune applies to batched plans");
     return VFR_ESTATE;
   }
   if (p->gexec) { vfr_set_error("autotune before graph capture (plan has a graph)"); return VFR_ESTATE; }
   const uint32_t table = (p->opts.flags & VFR_RENDER_PSI_TABLE) ? 1u : 0u;
-  TuneKey key{p->mesh->d.n_tets, p->n_pix, p->n_views, table};
+  TuneKey key{p->opts.mode, p->mesh->d.n_tets, p->n_pix, p->n_views, table};
   bool cached = false;
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
@@ -976,7 +1033,19 @@ extern "C" int vfr_plan_autotune(vfr_plan* p, uint32_t* flags_out, uint32_t* occ
       }
   }
   double best = -1.0;
-  if (!cached) {
+  if (cached && p->opts.mode == VFR_MODE_REF && (p->opts.flags & VFR_RENDER_FACE_MEMO) && !p->memo &&
+      hipMalloc((void**)&p->memo, plan_memo_bytes(p)) != hipSuccess) {
+    (void)hipGetLastError();
+    p->memo = nullptr;
+    p->opts.flags &= ~VFR_RENDER_FACE_MEMO;
+  }
+  if (!cached && p->opts.mode == VFR_MODE_REF) {
+    uint32_t chosen = 0;
+    const int rc = autotune_ref(p, &chosen, &best);
+    if (rc != VFR_OK) return rc;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune.push_back({key, TuneVal{chosen, p->opts.occupancy, p->opts.waves_per_cu, p->opts.refill}});
+  } else if (!cached) {
     /* the one-ray-per-thread kernel, then persistent schedules: occupancy
      * variant x resident waves per CU (multiples of the variant's 4-SIMD
      * capacity) x refill threshold */

@@ -160,6 +160,9 @@ struct TravArgs {
   uint32_t lds_records;    /* FTB, one ray per thread, psi in registers, records
                               below 4 GiB: tet records staged in LDS per wave
                               (VFR_RENDER_LDS_RECORDS)                         */
+  const float4* memo;      /* REF: face-plane memo, 2 float4 per (view, tet,
+                              face): (z0, z1, z2, s0), (s1, s2, 0, 0); or null */
+  uint32_t memo_tets;      /* tets per view in the memo                        */
 };
 
 /* ---- launchers (vfr_kernels.hip) ---- */
@@ -170,6 +173,11 @@ int vfr_k_copy(const void* src, void* dst, size_t bytes, void* stream);
  * or external faces any of those views' meshes has) x nv. */
 int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_verts,
                       const float4* wvert, float4* pv, void* stream);
+/* The face-plane memo of views [v0, v0+nv): z and s planes of every tet face
+ * (face f = slots f, f+1, f+2 of the tet's vertex ids) from the views'
+ * P-space vertices, one record of 2 float4 per (view, tet, face). */
+int vfr_k_face_memo(const DevView* views, uint32_t v0, uint32_t nv, uint32_t n_tets, const uint4* tv,
+                    const float4* pverts, float4* memo, void* stream);
 int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_faces, const uint4* fvid,
                      const float4* pv, uint32_t* owner, void* stream);
 /* kmax: entry slots per pixel; 0 only counts (cnt) without storing keys. */

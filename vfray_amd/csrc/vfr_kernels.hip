@@ -153,6 +153,44 @@ DEV Planes face_planes(float4 a, float4 b, float4 c) {
 
 DEV float plane_eval(const float p[3], float ox, float oy) { return ox * p[0] + oy * p[1] + p[2]; }
 
+/* Face-plane memo: the z and s planes of face f of every tet (slots f, f+1,
+ * f+2, the order in which the REF walk solves an exit face), per view, so a
+ * walk reads the 6 coefficients instead of solving them.  Grid: x over tet
+ * faces, y over views. */
+__global__ __launch_bounds__(256) void k_face_memo(const DevView* __restrict__ views, uint32_t v0, uint32_t n_tets,
+                                                   const uint4* __restrict__ tv, const float4* __restrict__ pverts,
+                                                   float4* __restrict__ memo) {
+  const uint32_t vi = v0 + blockIdx.y;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 4u * n_tets) return;
+  const uint32_t t = i >> 2, f = i & 3u;
+  const uint4 rv = tv[t];
+  const float4* pv = pverts + views[vi].vert_off;
+  const Planes pl = face_planes(pv[sel4(rv, f)], pv[sel4(rv, (f + 1u) & 3u)], pv[sel4(rv, (f + 2u) & 3u)]);
+  float4* o = memo + ((size_t)vi * n_tets * 4u + i) * 2u;
+  o[0] = make_float4(pl.z[0], pl.z[1], pl.z[2], pl.s[0]);
+  o[1] = make_float4(pl.s[1], pl.s[2], 0.0f, 0.0f);
+}
+
+int vfr_k_face_memo(const DevView* views, uint32_t v0, uint32_t nv, uint32_t n_tets, const uint4* tv,
+                    const float4* pverts, float4* memo, void* stream) {
+  if (nv == 0 || n_tets == 0) return VFR_OK;
+  hipLaunchKernelGGL(k_face_memo, dim3((4u * n_tets + 255u) / 256u, nv), dim3(256), 0, (hipStream_t)stream, views,
+                     v0, n_tets, tv, pverts, memo);
+  CHECK_HIP(hipGetLastError());
+  return VFR_OK;
+}
+
+/* The memoized planes of face f of tet t in view vi. */
+DEV Planes memo_planes(const float4* __restrict__ memo, uint32_t n_tets, uint32_t vi, uint32_t t, uint32_t f) {
+  const float4* m = memo + (((size_t)vi * n_tets + t) * 4u + f) * 2u;
+  const float4 a = m[0], b = m[1];
+  Planes p;
+  p.z[0] = a.x; p.z[1] = a.y; p.z[2] = a.z;
+  p.s[0] = a.w; p.s[1] = b.x; p.s[2] = b.y;
+  return p;
+}
+
 struct Hit {
   float cy, cz, D;
   bool sw, ok;
@@ -832,7 +870,9 @@ DEV RayOut trace_ref(const TravArgs& a, uint32_t vi, uint32_t pix) {
     const Step s = walk_step(w, rec, tv, pv, ox, oy);
     if (s.i == 0) { o.anomaly = 1; break; }         /* :353 */
     if (s.nb == VFR_NB_BROKEN) { o.broken = 1; break; } /* :365, not emulated */
-    const Planes nf = face_planes(s.A, s.B, s.C);
+    /* exit face (fe + i) & 3: its planes from the memo, or solved here */
+    const Planes nf = a.memo ? memo_planes(a.memo, a.memo_tets, vi, w.t, (w.fe + (uint32_t)s.i) & 3u)
+                             : face_planes(s.A, s.B, s.C);
     const float zNext = plane_eval(nf.z, ox, oy);
     const float sNext = plane_eval(nf.s, ox, oy);
     float dz = fabsf(zPrev - zNext);
