@@ -231,7 +231,8 @@ def main() -> int:
     ap.add_argument("--streams", type=int, default=2, help="serve: concurrent batches")
     ap.add_argument("--batch-pixels", type=int, default=4 << 20, help="serve: pixels per batch (0 = no cap)")
     ap.add_argument("--serve-schedule", type=int, default=0, help="serve: 0 slots, 1 pipeline")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=6.0,
+                    help="CPU baseline: seconds of work for each of its two legs (all threads, one thread)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="batch64: skip the request-stream measurement")
