@@ -228,7 +228,8 @@ typedef struct vfr_render_opts {
  * (tet, face) so that later rays of the same face reuse them.  The table
  * (32 bytes per tet face per view) is filled after the vertex transform of
  * each render with the walk's own arithmetic, so the results are the same
- * bits.  Front-to-back walks ignore the flag (they solve no face planes). */
+ * bits.  Front-to-back walks ignore the flag (they solve no face planes), and
+ * so does vfr_render_noalloc (no room for the table in its scratch area). */
 #define VFR_RENDER_FACE_MEMO 0x40u
 
 typedef struct vfr_frame_out {
