@@ -22,6 +22,9 @@ re-enter), 32 views of 768^2.
 `serve`: the request-stream measurement alone; `value` is the served Mrays/s
 at the latency bound.
 
+`serve_capacity`: the serving workload's saturated replay alone (for kernel
+traces); prints its own JSON line (Mrays/s, batches, makespan).
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -192,6 +195,35 @@ def serve_measure(ctx, args) -> dict:
     return out
 
 
+def serve_capacity(args) -> int:
+    """The serving workload's saturated replay alone (all requests queued at
+    once, the batched server of serve_measure), for kernel traces: one JSON
+    line with its Mrays/s, batches and makespan."""
+    import vfray_amd as V
+    from vfray_amd import workloads as WL
+    ctx = V.Context(0)
+    meshes, tf_ids = [], []
+    for (x, t, tfs) in WL.serve_meshes():
+        meshes.append(ctx.mesh(x, t))
+        tf_ids.append([ctx.tf(tf) for tf in tfs])
+    reqs = WL.serve_requests(args.requests, len(meshes), tf_ids)
+    sb = V.Server(ctx, meshes, hbm_budget=32 << 30, streams=args.streams, max_batch=256,
+                  max_batch_pixels=args.batch_pixels, schedule=args.serve_schedule)
+    sb.replay(reqs[:64], np.zeros(64))
+    log("warm-up replay done")
+    cap = sb.replay(reqs, np.zeros(len(reqs)))
+    sb.close()
+    for m in meshes:
+        m.close()
+    ctx.close()
+    print(json.dumps({"workload": "serve_capacity", "requests": len(reqs), "streams": args.streams,
+                      "max_batch_pixels": args.batch_pixels,
+                      "serve_schedule": ["slots", "pipeline"][args.serve_schedule],
+                      **{k: cap[k] for k in ("mrays_per_s", "batches", "mean_batch", "makespan_s", "rays",
+                                             "active_rays", "steps")}}), flush=True)
+    return 0
+
+
 def serve_bench(args) -> int:
     import vfray_amd as V
     ctx = V.Context(0)
@@ -225,7 +257,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="batch64", choices=["batch64", "plain_gpu", "concave", "serve"])
+    ap.add_argument("--workload", default="batch64", choices=["batch64", "plain_gpu", "concave", "serve", "serve_capacity"])
     ap.add_argument("--requests", type=int, default=10000, help="serve: requests in the stream")
     ap.add_argument("--load", type=float, default=0.8, help="serve: offered load / batched capacity")
     ap.add_argument("--streams", type=int, default=2, help="serve: concurrent batches")
@@ -267,6 +299,8 @@ def main() -> int:
 
     if args.workload == "serve":
         return serve_bench(args)
+    if args.workload == "serve_capacity":
+        return serve_capacity(args)
 
     t0 = time.time()
     if args.workload == "batch64":
