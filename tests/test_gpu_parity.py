@@ -431,6 +431,30 @@ def test_ftb_c2_full(ctx, psi):
     m.close()
 
 
+def test_block_tf_staging(ctx):
+    """A batched plan whose views use more transfer functions than the walk
+    can stage in LDS for the launch: each 256-ray block stages the TFs of its
+    own rays' views, and blocks whose rays span more than four views (here
+    the run of 8x8 views) read the global table.  Every view renders bit
+    for bit as it does alone (one TF, staged for the launch)."""
+    inp = grid_inputs(14, 0x5EED0107)
+    m = ctx.mesh(inp["x"], inp["t"])
+    tfs = [ctx.tf(V.synth_tf(0x5EED0107 + k)) for k in range(7)]
+    from vfray_amd import workloads as WL
+    views = []
+    for k in range(14):
+        side = 8 if 4 <= k < 11 else 96 + 16 * (k % 3)
+        views += WL.sphere_views(0x5EED0107 + k, 1, [side], tfs[k % len(tfs)])
+    print("per-block TF staging, 14 views, 7 TFs", flush=True)
+    a, sa = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+    assert sa["anomalies"] == 0 and sa["active_rays"] > 0
+    for i, v in enumerate(views):
+        b, _ = ctx.render(m, [v], mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+        for key in ("rgba", "entry_ext", "steps"):
+            assert np.array_equal(a[i][key], b[0][key]), (i, key)
+    m.close()
+
+
 def test_autotune_keeps_results(ctx):
     """The autotuner's chosen schedule renders the same bits."""
     inp = grid_inputs(16, 0x5EED0105)

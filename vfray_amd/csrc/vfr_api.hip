@@ -507,6 +507,8 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   a.lds_records = (p->opts.flags & VFR_RENDER_LDS_RECORDS) ? 1u : 0u;
   a.memo = plan_memo_on(p) ? p->memo : nullptr;
   a.memo_tets = p->mesh->d.n_tets;
+  /* more TF slots than LDS holds for the launch: each block stages its own */
+  a.tf_block = p->tf_n == 0 ? 1u : 0u;
   return a;
 }
 

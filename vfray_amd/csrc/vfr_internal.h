@@ -163,6 +163,9 @@ struct TravArgs {
   const float4* memo;      /* REF: face-plane memo, 2 float4 per (view, tet,
                               face): (z0, z1, z2, s0), (s1, s2, 0, 0); or null */
   uint32_t memo_tets;      /* tets per view in the memo                        */
+  uint32_t tf_block;       /* FTB, one ray per thread, more TF slots than LDS
+                              holds: each block stages the TFs of its rays'
+                              views (ray list in view order)                   */
 };
 
 /* ---- launchers (vfr_kernels.hip) ---- */

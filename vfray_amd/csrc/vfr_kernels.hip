@@ -992,6 +992,7 @@ struct FtbRay {
 #define FTB_PERSP (1u << 12)  /* perspective view                        */
 #define FTB_DEAD (1u << 13)   /* ended as an anomaly (no exit face)      */
 #define FTB_BROKEN (1u << 14) /* ended at a neighbour with no back link  */
+#define FTB_TFGLOB (1u << 15) /* per-block TF staging: this block reads the global table */
 DEV uint32_t ftb_cur(const FtbRay& r) { return r.bits & 63u; }
 DEV uint32_t ftb_nk(const FtbRay& r) { return (r.bits >> 6) & 63u; }
 DEV bool ftb_persp(const FtbRay& r) { return (r.bits & FTB_PERSP) != 0u; }
@@ -1098,7 +1099,7 @@ DEV void psi_zeta(float a, float b, float* psi, float* zeta) {
 
 /* Transfer-function colour `i` (slot offset included): from the block's LDS
  * copy of the launch's TF slots (TFL), or from the global table. */
-template <bool TFL>
+template <int TFL>
 DEV float4 tf_at(const TravArgs& a, const float4* tfl, uint32_t i) {
   return TFL ? tfl[i] : ld32<float4>(a.tf, i * 16u);
 }
@@ -1106,7 +1107,7 @@ DEV float4 tf_at(const TravArgs& a, const float4* tfl, uint32_t i) {
 /* Starts the walk at entry `cur` of the pixel's sorted list; false if the
  * entry face is degenerate at the pixel.  REC: also load the tet's face
  * record into the ray (LDS-staged records). */
-template <bool TFL, bool REC = false>
+template <int TFL, bool REC = false>
 DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const uint64_t key = __builtin_nontemporal_load(a.entries + ftb_cur(r) * a.estride + r.gp);
   const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull) + a.views[r.vi].ext_off;
@@ -1147,14 +1148,25 @@ DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
 
 /* LIN: linear compositing; the ray's colour bytes live in ar, ag, ab (exact
  * integers), its opacity in Tr, and segment lengths stay in object units. */
-template <bool TFL, bool LIN = false, bool REC = false>
+template <int TFL, bool LIN = false, bool REC = false>
 DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pix, FtbRay& r) {
   const uint32_t W = a.views[vi].W, H = a.views[vi].H;
   const float k = a.views[vi].k, kx = a.views[vi].kx;
   const bool persp = a.views[vi].proj == VFR_PROJ_PERSPECTIVE;
   r.vi = vi;
   r.voff = a.views[vi].vert_off;
-  r.tfo = (a.views[vi].tf - (TFL ? a.tf_lds_base : 0u)) * 256u;
+  /* the ray's TF as an index into the table it reads: the launch's LDS
+   * slots, its block's LDS slots (one per view of the block, the block's
+   * first view at slot 0; the block record after the slots says which), or
+   * the global table */
+  uint32_t tfglob = 0;
+  if constexpr (TFL == 2) {
+    const float4 m = tfl[VFR_TF_LDS_SLOTS * 256u];
+    tfglob = __float_as_uint(m.y) ? FTB_TFGLOB : 0u;
+    r.tfo = tfglob ? a.views[vi].tf * 256u : (vi - __float_as_uint(m.x)) * 256u;
+  } else {
+    r.tfo = (a.views[vi].tf - (TFL ? a.tf_lds_base : 0u)) * 256u;
+  }
   r.gp = a.views[vi].pix_off + pix;
   r.ox = pix_to_obj(pix % W, W);
   r.oy = pix_to_obj(pix / W, H);
@@ -1165,7 +1177,7 @@ DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pi
   r.rf = LIN ? rayf : rayf * a.views[vi].inv_me;
   uint32_t nk = __builtin_nontemporal_load(a.cnt + r.gp);
   nk = nk > a.kmax ? a.kmax : nk;
-  r.bits = (nk << 6) | (persp ? FTB_PERSP : 0u);
+  r.bits = (nk << 6) | (persp ? FTB_PERSP : 0u) | tfglob;
   r.ar = r.ag = r.ab = 0.0f;
   r.Tr = LIN ? 0.0f : 1.0f;
   r.steps = 0;
@@ -1226,7 +1238,7 @@ DEV void ldsr_exchange(const uint32_t* sperm, FtbRay& r) {
   r.bits &= 0xFFFFu;
 }
 
-template <int PROJ, bool BIG, bool TFL, bool PSIT, bool LIN = false, bool LDSR = false>
+template <int PROJ, bool BIG, int TFL, bool PSIT, bool LIN = false, bool LDSR = false>
 DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, FtbRay& r) {
   const bool persp = PROJ == VFR_TRAV_MIXED ? ftb_persp(r) : PROJ == VFR_TRAV_PERSP;
   r.steps += 1u;
@@ -1299,9 +1311,19 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
    * zeta = 1 and the update leaves colour and transparency unchanged (in
    * linear compositing, both terms vanish). */
   const uint32_t ci = r.tfo + (uint32_t)(255.0f * snext); /* snext is in [0, 1] */
-  const float4 cN = tf_at<TFL>(a, tfl, ci);
+  /* a block whose rays span more views than the LDS slots reads the global
+   * table (the bit is the same on every lane of the block) */
+  const bool tglob = TFL == 2 && (__builtin_amdgcn_readfirstlane(r.bits) & FTB_TFGLOB);
+  float4 cN, cPt;
+  if (tglob) {
+    cN = ld32<float4>(a.tf, ci * 16u);
+    cPt = ld32<float4>(a.tf, r.cpi * 16u);
+  } else {
+    cN = tf_at<TFL>(a, tfl, ci);
+    cPt = TFL ? tfl[r.cpi] : r.cP;
+  }
   const float dl = skip ? 0.0f : fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
-  const float4 cP = TFL ? tfl[r.cpi] : r.cP;
+  const float4 cP = cPt;
   if (LIN) {
     /* the reference's linear integration (vfRayKernel.cu:738-806, as in
      * trace_ref) over a segment of dl object units, front to back */
@@ -1401,7 +1423,7 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
 /* After an exit through an external face: continue at the first entry
  * strictly behind the exit point; false when there is none (the ray is
  * finished). */
-template <bool TFL, bool REC = false>
+template <int TFL, bool REC = false>
 DEV bool ftb_reenter(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const float exitZ = ftb_persp(r) ? 1.0f / r.prev : r.prev;
   uint32_t cur = ftb_cur(r) + 1u;
@@ -1436,7 +1458,7 @@ DEV RayOut ftb_out(const TravArgs& a, const FtbRay& r) {
   return o;
 }
 
-template <int PROJ, bool BIG, bool TFL, bool PSIT, bool LIN = false, bool LDSR = false>
+template <int PROJ, bool BIG, int TFL, bool PSIT, bool LIN = false, bool LDSR = false>
 DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, const uint32_t* sperm, uint32_t vi, uint32_t pix) {
   FtbRay r;
   ftb_init<TFL, LIN, LDSR>(a, tfl, vi, pix, r);
@@ -1462,7 +1484,7 @@ DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, const uint32_t* sperm
   return ftb_out<LIN>(a, r);
 }
 
-template <uint32_t MODE, int PROJ, bool BIG, bool TFL, bool PSIT, bool LDSR = false>
+template <uint32_t MODE, int PROJ, bool BIG, int TFL, bool PSIT, bool LDSR = false>
 DEV RayOut trace(const TravArgs& a, const float4* tfl, const uint32_t* sperm, uint32_t vi, uint32_t pix) {
   if constexpr (MODE == VFR_MODE_REF) return trace_ref<false>(a, vi, pix);
   else if constexpr (MODE == VFR_MODE_REF_LINEAR) return trace_ref<true>(a, vi, pix);
@@ -1488,18 +1510,14 @@ DEV void store_ray(const TravArgs& a, uint32_t gp, const RayOut& o) {
  * Waves per SIMD: 7 for the FTB walk (72 VGPRs, no spills; 6 measured 1.6 %
  * slower and 8 spills), 6 with LDS-staged records, 5 for the REF walk. */
 #define VFR_TRAV_WAVES(MODE, LDSR) ((MODE) == VFR_MODE_FTB ? ((LDSR) ? 6 : 7) : 5)
-template <uint32_t MODE, int PROJ, bool BIG, bool TFL, bool PSIT, bool LDSR = false>
+template <uint32_t MODE, int PROJ, bool BIG, int TFL, bool PSIT, bool LDSR = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VFR_TRAV_WAVES(MODE, LDSR),
                                                                      VFR_TRAV_WAVES(MODE, LDSR)))) void k_trav(
     TravArgs a) {
-  __shared__ float4 tfs[TFL ? VFR_TF_LDS_SLOTS * 256 : 1];
+  /* TF slots, then (per-block staging) the block record: first view, global */
+  __shared__ float4 tfs[TFL == 2 ? VFR_TF_LDS_SLOTS * 256 + 1 : (TFL ? VFR_TF_LDS_SLOTS * 256 : 1)];
   __shared__ __align__(16) uint32_t sperm[LDSR ? LDSR_WORDS : VFR_N_SLOT_PERMS];
   constexpr bool ftb = MODE == VFR_MODE_FTB || MODE == VFR_MODE_FTB_LINEAR;
-  if (ftb) stage_slot_perms(sperm);
-  if (TFL) {
-    for (uint32_t k = threadIdx.x; k < a.tf_lds_n * 256u; k += blockDim.x) tfs[k] = a.tf[a.tf_lds_base * 256u + k];
-  }
-  if (TFL || ftb) __syncthreads();
   /* Blocks go to the 8 XCDs round robin (block b to XCD b % 8).  Within each
    * run of 128 blocks, XCD x walks the consecutive ray blocks 16x .. 16x+15
    * (neighbouring screen tiles), so its L2 holds one region of the mesh
@@ -1512,10 +1530,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VFR_TRAV_WA
   }
   const uint32_t i = lb * blockDim.x + threadIdx.x;
   const uint32_t n = *a.n_rays;
+  const uint64_t ray = i < n ? __builtin_nontemporal_load(a.rays + i) : 0ull;
+  const uint32_t vi = (uint32_t)(ray >> 32), pix = (uint32_t)ray;
+  if (ftb) stage_slot_perms(sperm);
+  if constexpr (TFL == 2) {
+    /* the TFs of the views the block's rays belong to, one LDS slot per view
+     * from the least to the greatest (the list is grouped by view, so a
+     * block spans one or two views unless views are tiny), when they fit;
+     * else the block reads the global table */
+    __shared__ uint32_t vlo[4], vhi[4], vused;
+    uint32_t lo = i < n ? vi : 0xFFFFFFFFu, hi = i < n ? vi : 0u;
+    for (int d = 32; d >= 1; d >>= 1) {
+      lo = min(lo, (uint32_t)__shfl_xor(lo, d, 64));
+      hi = max(hi, (uint32_t)__shfl_xor(hi, d, 64));
+    }
+    if ((threadIdx.x & 63u) == 0) { vlo[threadIdx.x >> 6] = lo; vhi[threadIdx.x >> 6] = hi; }
+    if (threadIdx.x == 0) vused = 0u;
+    __syncthreads();
+    lo = min(min(vlo[0], vlo[1]), min(vlo[2], vlo[3]));
+    hi = max(max(vhi[0], vhi[1]), max(vhi[2], vhi[3]));
+    const uint32_t nv = lo <= hi ? hi - lo + 1u : 0u;
+    const bool glob = nv > VFR_TF_LDS_SLOTS;
+    /* only the views some ray of the block has are read (view slots between
+     * them may be unused, as in a server's shared view arena) */
+    if (!glob && i < n) atomicOr(&vused, 1u << (vi - lo));
+    __syncthreads();
+    const uint32_t used = vused;
+    if (!glob)
+      for (uint32_t k = threadIdx.x; k < nv * 256u; k += blockDim.x)
+        if (used & (1u << (k >> 8))) tfs[k] = a.tf[a.views[lo + (k >> 8)].tf * 256u + (k & 255u)];
+    if (threadIdx.x == 0)
+      tfs[VFR_TF_LDS_SLOTS * 256] = make_float4(__uint_as_float(lo), __uint_as_float(glob ? 1u : 0u), 0.f, 0.f);
+  } else if constexpr (TFL == 1) {
+    for (uint32_t k = threadIdx.x; k < a.tf_lds_n * 256u; k += blockDim.x) tfs[k] = a.tf[a.tf_lds_base * 256u + k];
+  }
+  if (TFL || ftb) __syncthreads();
   unsigned long long st = 0, an = 0, br = 0;
   if (i < n) {
-    const uint64_t ray = __builtin_nontemporal_load(a.rays + i);
-    const uint32_t vi = (uint32_t)(ray >> 32), pix = (uint32_t)ray;
     const RayOut o = trace<MODE, PROJ, BIG, TFL, PSIT, LDSR>(a, tfs, sperm, vi, pix);
     store_ray(a, a.views[vi].pix_off + pix, o);
     st = o.steps; an = o.anomaly; br = o.broken;
@@ -1538,7 +1589,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VFR_TRAV_WA
  * dispatch slot until its slowest wave ends; the transfer function is
  * staged in LDS once per block.  Every wave reaches the drained queue, so
  * the grid ends. */
-template <int PROJ, bool BIG, bool TFL, bool PSIT>
+template <int PROJ, bool BIG, int TFL, bool PSIT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_trav_wq(TravArgs a) {
   __shared__ float4 tfs[TFL ? VFR_TF_LDS_SLOTS * 256 : 1];
   __shared__ uint32_t sperm[VFR_N_SLOT_PERMS];
@@ -1684,7 +1735,7 @@ static uint32_t wq_blocks(K kernel) {
   return (uint32_t)per_cu * (uint32_t)cus;
 }
 
-template <int PROJ, bool BIG, bool TFL, bool PSIT>
+template <int PROJ, bool BIG, int TFL, bool PSIT>
 static void launch_wq1(hipStream_t st, const TravArgs& a) {
   static const uint32_t blocks = wq_blocks(k_trav_wq<PROJ, BIG, TFL, PSIT>);
   hipLaunchKernelGGL((k_trav_wq<PROJ, BIG, TFL, PSIT>), dim3(blocks), dim3(256), 0, st, a);
@@ -1716,25 +1767,30 @@ static void launch_ftb_ldsr(uint32_t proj, bool tfl, dim3 grid, dim3 block, hipS
     hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false, false, false, true>), grid, block, 0, st, a);
 }
 
-template <bool PSIT>
-static void launch_ftb(uint32_t proj, bool tfl, dim3 grid, dim3 block, hipStream_t st, const TravArgs& a) {
-  if (proj == VFR_TRAV_ORTHO && tfl)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_ORTHO, false, true, PSIT>), grid, block, 0, st, a);
-  else if (proj == VFR_TRAV_ORTHO)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_ORTHO, false, false, PSIT>), grid, block, 0, st, a);
-  else if (proj == VFR_TRAV_PERSP && tfl)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_PERSP, false, true, PSIT>), grid, block, 0, st, a);
+/* tfl: 0 the global TF table, 1 the launch's slots in LDS, 2 per-block slots */
+template <bool PSIT, int TFL>
+static void launch_ftb_t(uint32_t proj, dim3 grid, dim3 block, hipStream_t st, const TravArgs& a) {
+  if (proj == VFR_TRAV_ORTHO)
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_ORTHO, false, TFL, PSIT>), grid, block, 0, st, a);
   else if (proj == VFR_TRAV_PERSP)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_PERSP, false, false, PSIT>), grid, block, 0, st, a);
-  else if (tfl)
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false, true, PSIT>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_PERSP, false, TFL, PSIT>), grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false, false, PSIT>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, false, TFL, PSIT>), grid, block, 0, st, a);
 }
 
-int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t occupancy, const TravArgs& a,
+template <bool PSIT>
+static void launch_ftb(uint32_t proj, int tfl, dim3 grid, dim3 block, hipStream_t st, const TravArgs& a) {
+  if (tfl == 2) launch_ftb_t<PSIT, 2>(proj, grid, block, st, a);
+  else if (tfl == 1) launch_ftb_t<PSIT, 1>(proj, grid, block, st, a);
+  else launch_ftb_t<PSIT, 0>(proj, grid, block, st, a);
+}
+
+int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t occupancy, const TravArgs& a0,
                    uint32_t max_rays, int kernel, uint32_t grid_waves, void* stream) {
   if (max_rays == 0) return VFR_OK;
+  /* per-block TF staging is the one-ray-per-thread kernel's */
+  TravArgs a = a0;
+  if (kernel != VFR_WALK_THREAD) a.tf_block = 0;
   const bool lin = a.composite != VFR_COMPOSITE_PPI;
   if (kernel == VFR_WALK_QUEUE && mode == VFR_MODE_FTB && !lin) {
     const bool tfl = a.tf_lds_n >= 1 && a.tf_lds_n <= VFR_TF_LDS_SLOTS;
@@ -1763,7 +1819,9 @@ int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t oc
   }
   dim3 grid((max_rays + 255) / 256), block(256);
   hipStream_t st = (hipStream_t)stream;
+  /* TFs in LDS: the launch's slots when they fit, or per block */
   const bool tfl = a.tf_lds_n >= 1 && a.tf_lds_n <= VFR_TF_LDS_SLOTS;
+  const int tfmode = tfl ? 1 : (a.tf_block ? 2 : 0);
   if (mode == VFR_MODE_REF && lin)
     hipLaunchKernelGGL((k_trav<VFR_MODE_REF_LINEAR, VFR_TRAV_MIXED, true, false, false>), grid, block, 0, st, a);
   else if (lin)
@@ -1777,11 +1835,11 @@ int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t oc
   else if (n_records >= VFR_BIG_RECORDS)
     hipLaunchKernelGGL((k_trav<VFR_MODE_FTB, VFR_TRAV_MIXED, true, false, false>), grid, block, 0, st, a);
   else if (a.psi_table)
-    launch_ftb<true>(proj, tfl, grid, block, st, a);
+    launch_ftb<true>(proj, tfmode, grid, block, st, a);
   else if (a.lds_records)
     launch_ftb_ldsr(proj, tfl, grid, block, st, a);
   else
-    launch_ftb<false>(proj, tfl, grid, block, st, a);
+    launch_ftb<false>(proj, tfmode, grid, block, st, a);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }

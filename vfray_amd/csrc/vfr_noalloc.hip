@@ -269,6 +269,7 @@ extern "C" int vfr_render_noalloc(vfr_ctx* c, const vfr_mesh* m, const vfr_view*
   a.term = o.term_alpha;
   a.refill = o.refill;
   if (tf_hi - tf_lo < VFR_TF_LDS_SLOTS) { a.tf_lds_base = tf_lo; a.tf_lds_n = tf_hi - tf_lo + 1; }
+  else a.tf_block = 1; /* each block stages the TFs of its rays' views */
   a.composite = o.composite;
   a.psi_table = (o.flags & VFR_RENDER_PSI_TABLE) ? 1u : 0u;
   a.lds_records = (o.flags & VFR_RENDER_LDS_RECORDS) ? 1u : 0u;

@@ -325,6 +325,7 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   a.lds_records = (sv->cfg.flags & VFR_RENDER_LDS_RECORDS) ? 1u : 0u;
   a.composite = sv->cfg.composite;
   if (tf_hi - tf_lo < VFR_TF_LDS_SLOTS) { a.tf_lds_base = tf_lo; a.tf_lds_n = tf_hi - tf_lo + 1; }
+  else a.tf_block = 1; /* each block stages the TFs of its rays' views */
   const uint32_t proj = n_persp == 0 ? VFR_TRAV_ORTHO : (n_persp == nv ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
   const uint64_t records = (sv->cfg.flags & VFR_RENDER_WIDE) ? VFR_BIG_RECORDS : sv->n_records;
   CHECK_RC(vfr_k_traverse(VFR_MODE_FTB, proj, records, sv->cfg.occupancy, a, (uint32_t)pix,
@@ -533,6 +534,7 @@ static int pipe_walk(vfr_server* sv, WalkList* L, int k) {
   a.lds_records = (sv->cfg.flags & VFR_RENDER_LDS_RECORDS) ? 1u : 0u;
   a.composite = sv->cfg.composite;
   if (L->tf_hi - L->tf_lo < VFR_TF_LDS_SLOTS) { a.tf_lds_base = L->tf_lo; a.tf_lds_n = L->tf_hi - L->tf_lo + 1; }
+  else a.tf_block = 1; /* each block stages the TFs of its rays' views */
   const uint32_t nv = (uint32_t)L->reqs.size();
   const uint32_t proj = L->n_persp == 0 ? VFR_TRAV_ORTHO : (L->n_persp == nv ? VFR_TRAV_PERSP : VFR_TRAV_MIXED);
   const uint64_t records = (sv->cfg.flags & VFR_RENDER_WIDE) ? VFR_BIG_RECORDS : sv->n_records;
