@@ -96,23 +96,27 @@ DEV ViewRegs load_view(const DevView* views, uint32_t vi) {
 
 
 /* P-space vertices of each view: pv[vert_off + i] = xform(view, mesh vertex i). */
-/* Grid: x over a view's vertices, y over the views, so the view (and its
- * transform) is uniform across a block and read with scalar loads. */
+/* Grid: x over the views, y over chunks of 256 vertices (looping when a mesh
+ * has more chunks than the grid), so the view (and its transform) is uniform
+ * across a block and read with scalar loads, and the blocks that read one
+ * chunk of world vertices for every view are dispatched one after another
+ * (the chunk comes from HBM once per XCD, not once per view). */
 __global__ __launch_bounds__(256) void k_xform_views(const DevView* __restrict__ views, uint32_t v0,
                                                      const float4* __restrict__ wvert,
                                                      float4* __restrict__ pv) {
-  const uint32_t vi = v0 + blockIdx.y;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= views[vi].n_verts) return;
+  const uint32_t vi = v0 + blockIdx.x;
+  const uint32_t n = views[vi].n_verts;
   const ViewRegs vr = load_view(views, vi);
-  pv[(size_t)views[vi].vert_off + i] = xform(vr, wvert[(size_t)views[vi].wvert_off + i]);
+  const size_t out = views[vi].vert_off, in = views[vi].wvert_off;
+  for (uint32_t i = blockIdx.y * blockDim.x + threadIdx.x; i < n; i += gridDim.y * blockDim.x)
+    pv[out + i] = xform(vr, wvert[in + i]);
 }
 
 int vfr_k_xform_views(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_verts,
                       const float4* wvert, float4* pv, void* stream) {
   if (nv == 0 || max_verts == 0) return VFR_OK;
-  hipLaunchKernelGGL(k_xform_views, dim3((max_verts + 255) / 256, nv), dim3(256), 0, (hipStream_t)stream, views, v0,
-                     wvert, pv);
+  const uint32_t chunks = std::min<uint32_t>((max_verts + 255) / 256, 65535u);
+  hipLaunchKernelGGL(k_xform_views, dim3(nv, chunks), dim3(256), 0, (hipStream_t)stream, views, v0, wvert, pv);
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
