@@ -307,6 +307,8 @@ struct vfr_plan {
   uint64_t* rays = nullptr;    /* [n_pix] */
   float4* pverts = nullptr;    /* [n_views * n_verts] P-space vertices */
   float4* memo = nullptr;      /* REF face-plane memo, [n_views * n_tets * 4 * 2] */
+  uint32_t* block_off = nullptr; /* batched compaction: per-block ray counts, then offsets */
+  uint64_t* tile_mask = nullptr;  /* ... each tile's active-pixel mask, then its view */
   float4* rgba = nullptr;
   uint32_t* steps = nullptr;
   uint32_t* entry = nullptr;
@@ -418,7 +420,7 @@ extern "C" void vfr_plan_destroy(vfr_plan* p) {
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   void* ps[] = {p->views, p->owner, p->entries, p->rays, p->pverts, p->rgba, p->steps, p->entry,
-                p->counters, p->stats, p->memo};
+                p->counters, p->stats, p->memo, p->block_off, p->tile_mask};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   for (auto& e : p->ev)
@@ -465,7 +467,8 @@ static int enqueue_batched(vfr_plan* p) {
     CHECK_RC(vfr_k_raster_ftb(p->views, 0, p->n_views, m->d.n_ext, m->fvid4(), p->pverts, p->owner,
                               p->entries, p->kmax, p->n_pix, p->stats, nullptr, st));
   CHECK_RC(vfr_k_compact(p->views, p->n_views, 0, p->n_tiles, p->opts.mode, p->owner, p->owner,
-                         p->entries, p->kmax, p->n_pix, p->rgba, p->steps, p->entry, p->rays, p->counters + p->n_views, st));
+                         p->entries, p->kmax, p->n_pix, p->rgba, p->steps, p->entry, p->rays, p->counters + p->n_views, st,
+                         0, p->block_off, p->tile_mask));
   return VFR_OK;
 }
 
@@ -634,6 +637,8 @@ extern "C" int vfr_plan_create(vfr_ctx* c, const vfr_mesh* m, const vfr_view* vi
   PALLOC(p->entry, sizeof(uint32_t) * pix);
   PALLOC(p->counters, sizeof(uint32_t) * (n_views + 2 + VFR_PIPE_CHUNKS));
   PALLOC(p->stats, sizeof(unsigned long long) * ST_N);
+  PALLOC(p->block_off, sizeof(uint32_t) * vfr_compact_blocks((uint32_t)tiles));
+  PALLOC(p->tile_mask, sizeof(uint64_t) * tiles + sizeof(uint32_t) * tiles);
 #undef PALLOC
   if (rc == VFR_OK && hipMemcpy(p->views, p->hv.data(), sizeof(DevView) * n_views, hipMemcpyHostToDevice) != hipSuccess) {
     vfr_set_error("view upload failed");
@@ -766,7 +771,8 @@ static int execute_pipelined(vfr_plan* p, double* ms_total, double* ms_vis, doub
     CHECK_RC(vfr_k_raster_ftb(p->views, ch.v0, ch.nv, m->d.n_ext, m->fvid4(), p->pverts, p->owner,
                               p->entries, p->kmax, p->n_pix, p->stats, nullptr, st));
     CHECK_RC(vfr_k_compact(p->views, p->n_views, ch.tile0, ch.ntiles, p->opts.mode, p->owner, p->owner,
-                           p->entries, p->kmax, p->n_pix, p->rgba, p->steps, p->entry, p->rays + ch.pix0, ccount + c, st));
+                           p->entries, p->kmax, p->n_pix, p->rgba, p->steps, p->entry, p->rays + ch.pix0, ccount + c, st,
+                           0, p->block_off, p->tile_mask));
     CHECK_HIP(hipEventRecord(p->cev[3 * c], st));
     CHECK_HIP(hipStreamWaitEvent(s2, p->cev[3 * c], 0));
     CHECK_HIP(hipEventRecord(p->cev[3 * c + 1], s2));

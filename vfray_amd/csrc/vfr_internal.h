@@ -198,7 +198,13 @@ int vfr_k_max_u32(const uint32_t* cnt, size_t n, uint32_t* out, void* stream);
 int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, uint32_t ntiles,
                   uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries, uint32_t kmax,
                   uint64_t estride, float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
-                  uint32_t* n_rays, void* stream, uint32_t view_base = 0);
+                  uint32_t* n_rays, void* stream, uint32_t view_base = 0,
+                  uint32_t* block_off = nullptr, uint64_t* tile_mask = nullptr);
+/* Blocks of a compaction launch over ntiles tiles, the size block_off needs.
+ * With block_off (and tile_mask: ntiles uint64 masks + ntiles uint32 views),
+ * rays are written in tile order after a count pass and a scan; without, each
+ * block appends its rays with one atomic, in arrival order. */
+uint32_t vfr_compact_blocks(uint32_t ntiles);
 int vfr_k_clear_view_pixels(const DevView* views, uint32_t v0, uint32_t nv, uint32_t max_pix, uint32_t* a,
                             void* stream);
 /* occupancy: waves per SIMD of the persistent kernel variant (4, 5, 6:

@@ -607,7 +607,9 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
                                                   float4* __restrict__ rgba, uint32_t* __restrict__ steps,
                                                   uint32_t* __restrict__ entry_out,
                                                   uint64_t* __restrict__ rays,
-                                                  uint32_t* __restrict__ n_rays, uint32_t view_base) {
+                                                  uint32_t* __restrict__ n_rays, uint32_t view_base,
+                                                  uint32_t pass, uint32_t* __restrict__ block_off,
+                                                  uint64_t* __restrict__ tile_mask) {
   __shared__ uint32_t wave_n[COMPACT_WAVES];
   __shared__ uint32_t block_base;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -670,15 +672,23 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
     }
   }
   const uint64_t m = __ballot(active);
+  /* count pass: the tile's active mask, and its view in the ntiles words
+   * after the masks (the emit pass reads both instead of searching the views
+   * again) */
+  if (pass == 1 && lane == 0 && wt < ntiles) {
+    tile_mask[wt] = m;
+    reinterpret_cast<uint32_t*>(tile_mask + ntiles)[wt] = vi;
+  }
   if (lane == 0) wave_n[wave] = (uint32_t)__popcll(m);
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t tot = 0;
     for (uint32_t w = 0; w < COMPACT_WAVES; ++w) tot += wave_n[w];
-    block_base = tot ? atomicAdd(n_rays, tot) : 0u;
+    if (pass == 1) block_off[blockIdx.x] = tot;
+    else block_base = tot ? atomicAdd(n_rays, tot) : 0u;
   }
   __syncthreads();
-  if (active) {
+  if (active && pass == 0) {
     uint32_t before = block_base;
     for (uint32_t w = 0; w < wave; ++w) before += wave_n[w];
     before += (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
@@ -686,14 +696,82 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
   }
 }
 
+/* Second pass of the tile-ordered compaction: one wave per tile, as the
+ * first pass, writing the rays of its tile's active mask after those of the
+ * tiles before it (the block's offset from the scan, then the masks of the
+ * block's earlier tiles). */
+__global__ __launch_bounds__(1024) void k_compact_emit(const DevView* __restrict__ views, uint32_t n_views,
+                                                       uint32_t tile0, uint32_t ntiles,
+                                                       const uint64_t* __restrict__ tile_mask,
+                                                       const uint32_t* __restrict__ block_off,
+                                                       uint64_t* __restrict__ rays, uint32_t* __restrict__ n_rays,
+                                                       uint32_t view_base) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t t0 = blockIdx.x * COMPACT_WAVES, wt = t0 + wave;
+  if (wt >= ntiles) return;
+  uint32_t before = block_off[blockIdx.x];
+  for (uint32_t w = 0; w < wave; ++w) before += (uint32_t)__popcll(tile_mask[t0 + w]);
+  const uint64_t m = tile_mask[wt];
+  if (wt == ntiles - 1 && lane == 0) *n_rays = before + (uint32_t)__popcll(m);
+  if (!((m >> lane) & 1ull)) return;
+  const uint32_t gt = tile0 + wt;
+  const uint32_t lo = reinterpret_cast<const uint32_t*>(tile_mask + ntiles)[wt];
+  const uint32_t W = views[lo].W, H = views[lo].H;
+  uint32_t tx, ty;
+  tile_xy(gt - views[lo].tile_off, views[lo].tiles_x, (H + 7u) / 8u, &tx, &ty);
+  const uint32_t lx = (lane & 1u) | ((lane >> 1) & 2u) | ((lane >> 2) & 4u);
+  const uint32_t ly = ((lane >> 1) & 1u) | ((lane >> 2) & 2u) | ((lane >> 3) & 4u);
+  const uint32_t pix = (ty * 8u + ly) * W + tx * 8u + lx;
+  rays[before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = ((uint64_t)(lo + view_base) << 32) | pix;
+}
+
+/* In-place exclusive prefix sum of v[0, n) by one block: each thread sums a
+ * contiguous run, the runs are scanned in LDS, then each thread writes its
+ * run's prefixes. */
+__global__ __launch_bounds__(1024) void k_scan_excl(uint32_t* __restrict__ v, uint32_t n) {
+  __shared__ uint32_t part[1024];
+  const uint32_t per = (n + 1023u) / 1024u;
+  const uint32_t b = min(n, threadIdx.x * per), e = min(n, b + per);
+  uint32_t sum = 0;
+  for (uint32_t i = b; i < e; ++i) sum += v[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (uint32_t d = 1; d < 1024u; d <<= 1) {
+    const uint32_t t = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += t;
+    __syncthreads();
+  }
+  uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+  for (uint32_t i = b; i < e; ++i) {
+    const uint32_t c = v[i];
+    v[i] = run;
+    run += c;
+  }
+}
+
+uint32_t vfr_compact_blocks(uint32_t ntiles) { return (ntiles + COMPACT_WAVES - 1) / COMPACT_WAVES; }
+
 int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, uint32_t ntiles,
                   uint32_t mode, const uint32_t* owner, uint32_t* cnt, uint64_t* entries, uint32_t kmax,
                   uint64_t estride, float4* rgba, uint32_t* steps, uint32_t* entry_out, uint64_t* rays,
-                  uint32_t* n_rays, void* stream, uint32_t view_base) {
+                  uint32_t* n_rays, void* stream, uint32_t view_base, uint32_t* block_off, uint64_t* tile_mask) {
   if (ntiles == 0) return VFR_OK;
-  hipLaunchKernelGGL(k_compact, dim3((ntiles + COMPACT_WAVES - 1) / COMPACT_WAVES), dim3(1024), 0,
-                     (hipStream_t)stream, views, n_views_total, tile0, ntiles, mode, owner, cnt, entries,
-                     kmax, estride, rgba, steps, entry_out, rays, n_rays, view_base);
+  const uint32_t nb = vfr_compact_blocks(ntiles);
+  hipStream_t st = (hipStream_t)stream;
+  if (!block_off) {
+    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(1024), 0, st, views, n_views_total, tile0, ntiles, mode, owner,
+                       cnt, entries, kmax, estride, rgba, steps, entry_out, rays, n_rays, view_base, 0u,
+                       (uint32_t*)nullptr, (uint64_t*)nullptr);
+  } else {
+    /* rays in tile order: count per block, scan, then write at the offsets */
+    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(1024), 0, st, views, n_views_total, tile0, ntiles, mode, owner,
+                       cnt, entries, kmax, estride, rgba, steps, entry_out, rays, n_rays, view_base, 1u, block_off,
+                       tile_mask);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, block_off, nb);
+    hipLaunchKernelGGL(k_compact_emit, dim3(nb), dim3(1024), 0, st, views, n_views_total, tile0, ntiles, tile_mask,
+                       block_off, rays, n_rays, view_base);
+  }
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
 }
