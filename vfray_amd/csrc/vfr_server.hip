@@ -72,7 +72,11 @@ inline double secs(Clock::time_point t) {
  * view a request can ask for needs a counting pass first (the slots beyond a
  * pixel's count cost memory only: the raster, the sort and the walk touch
  * as many as the pixel has). */
-inline uint64_t pixel_bytes() { return 4 + 8ull * VFR_K_MAX + 8 + 16 + 4 + 4; }
+/* Per pixel: count, entry slots, ray, RGBA, steps, entry id, and 2 bytes for
+ * the tile-ordered compaction (12 bytes per 8x8 tile; a view of W x H pixels
+ * has at most W H / 8 + 1 tiles, because (W + 7)(H + 7) / 64 <= W H / 8 + 1
+ * whenever (W - 1)(H - 1) >= 0). */
+inline uint64_t pixel_bytes() { return 4 + 8ull * VFR_K_MAX + 8 + 16 + 4 + 4 + 2; }
 
 struct SrvMesh {
   uint32_t rec_off, ext_off, wvert_off, n_ext, n_verts, n_tets;
@@ -160,6 +164,7 @@ struct Slot {
   hipEvent_t done = nullptr;
   uint64_t cap_pix = 0, cap_verts = 0;
   uint32_t cap_views = 0;
+  uint32_t cap_tiles = 0;       /* cap_pix / 8 + cap_views */
   DevView* views = nullptr;
   DevView* h_views = nullptr;   /* pinned */
   uint32_t* cnt = nullptr;
@@ -170,6 +175,8 @@ struct Slot {
   uint32_t* steps = nullptr;
   uint32_t* entry = nullptr;
   uint32_t* counters = nullptr; /* [0] rays, [1] queue head */
+  uint32_t* block_off = nullptr; /* compaction: per-block ray counts, then offsets */
+  uint64_t* tile_mask = nullptr; /* compaction: [cap_tiles] masks, then [cap_tiles] views */
   uint32_t* vflags = nullptr;   /* [cap_views] VFR_VFLAG_* per request */
   uint32_t* h_vflags = nullptr; /* pinned */
   unsigned long long* stats = nullptr;
@@ -225,7 +232,7 @@ struct vfr_server {
 
 static void slot_free(Slot* s) {
   void* dev[] = {s->views, s->cnt, s->entries, s->rays, s->pverts, s->rgba, s->steps, s->entry,
-                 s->counters, s->stats, s->vflags};
+                 s->counters, s->block_off, s->tile_mask, s->stats, s->vflags};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   void* host[] = {s->h_views, s->h_stats, s->h_counters, s->h_img, s->h_vflags};
@@ -240,6 +247,7 @@ static int slot_alloc(Slot* s, uint64_t bytes, uint32_t max_batch) {
   s->cap_pix = (bytes * 3 / 4) / pixel_bytes();
   s->cap_verts = std::min<uint64_t>((bytes / 4) / 16, VFR_MAX_VIEW_VERTS - 1);
   s->cap_views = max_batch;
+  s->cap_tiles = (uint32_t)std::min<uint64_t>(s->cap_pix / 8 + s->cap_views, UINT32_MAX);
   CHECK_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   CHECK_HIP(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
   CHECK_HIP(hipMalloc((void**)&s->views, sizeof(DevView) * s->cap_views));
@@ -251,6 +259,8 @@ static int slot_alloc(Slot* s, uint64_t bytes, uint32_t max_batch) {
   CHECK_HIP(hipMalloc((void**)&s->steps, 4 * s->cap_pix));
   CHECK_HIP(hipMalloc((void**)&s->entry, 4 * s->cap_pix));
   CHECK_HIP(hipMalloc((void**)&s->counters, 16));
+  CHECK_HIP(hipMalloc((void**)&s->block_off, sizeof(uint32_t) * vfr_compact_blocks(s->cap_tiles)));
+  CHECK_HIP(hipMalloc((void**)&s->tile_mask, (sizeof(uint64_t) + sizeof(uint32_t)) * (size_t)s->cap_tiles));
   CHECK_HIP(hipMalloc((void**)&s->stats, sizeof(unsigned long long) * ST_N));
   CHECK_HIP(hipMalloc((void**)&s->vflags, sizeof(uint32_t) * s->cap_views));
   CHECK_HIP(hipHostMalloc((void**)&s->h_views, sizeof(DevView) * s->cap_views, hipHostMallocDefault));
@@ -296,8 +306,14 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   CHECK_RC(vfr_k_xform_views(s->views, 0, nv, max_verts, sv->vert, s->pverts, st));
   CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, max_faces, sv->fvid, s->pverts, s->cnt, s->entries, VFR_K_MAX,
                             s->cap_pix, s->stats, s->vflags, st));
+  if (tiles > s->cap_tiles) {
+    vfr_set_error("server batch: %llu tiles exceed the slot's %u", (unsigned long long)tiles, s->cap_tiles);
+    return VFR_EINVAL;
+  }
+  /* rays in tile order (count pass, block scan, emit) */
   CHECK_RC(vfr_k_compact(s->views, nv, 0, (uint32_t)tiles, VFR_MODE_FTB, nullptr, s->cnt, s->entries, VFR_K_MAX,
-                         s->cap_pix, s->rgba, s->steps, s->entry, s->rays, s->counters, st));
+                         s->cap_pix, s->rgba, s->steps, s->entry, s->rays, s->counters, st, 0, s->block_off,
+                         s->tile_mask));
   TravArgs a;
   memset(&a, 0, sizeof(a));
   a.rec = sv->rec;
