@@ -346,6 +346,33 @@ def test_ftb_perspective_32(ctx, psi):
     m.close()
 
 
+def test_batched_thin_views(ctx, psi):
+    """A batch of views whose 8x8 tiles are partly filled (2 pixels wide or
+    tall, odd sides): the batched plan builds its ray list per tile, the
+    per-view schedule in one pass; both equal each other and the oracle."""
+    inp = grid_inputs(12, 0x5EED0109)
+    m = ctx.mesh(inp["x"], inp["t"])
+    tf = ctx.tf(inp["tf"])
+    r = WL.Rng(0x5EED0109, 9)
+    sizes = [(2, 64), (64, 2), (9, 9), (7, 300), (2, 2), (130, 3), (17, 33), (3, 129)]
+    views = [V.perspective_view(w, h, tf, eye=2.5 * r.direction()) for w, h in sizes]
+    print("thin views batched vs per-view vs oracle 12^3", flush=True)
+    a, sa = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+    b, sb = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+    c, sc = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, pipelined=True)
+    assert sa["steps"] == sb["steps"] == sc["steps"] and sa["active_rays"] == sb["active_rays"] > 0
+    for i, v in enumerate(views):
+        for key in ("rgba", "entry_ext", "steps"):
+            assert np.array_equal(a[i][key], b[i][key])
+            assert np.array_equal(a[i][key], c[i][key])
+    for i in (2, 6):
+        orc = oracle_ftb(inp, views[i], psi)
+        assert np.array_equal(a[i]["entry_ext"], orc["entry_ext"])
+        assert np.array_equal(a[i]["steps"], orc["steps"])
+        assert np.abs(a[i]["rgba"] - orc["rgba"]).max() <= TOL
+    m.close()
+
+
 def test_batched_equals_per_view(ctx):
     """Each view rendered in a batch equals the same view rendered alone, with
     every walk kernel (one ray per thread with and without LDS-staged tet

@@ -52,6 +52,33 @@ def test_server_alone_equals_mix(ctx, two_meshes, schedule):
         assert np.array_equal(g["rgba"], alone[0]["rgba"])
 
 
+@pytest.mark.parametrize("schedule", [V.SERVE_SLOTS, V.SERVE_PIPELINE])
+def test_server_thin_views_equal_alone(ctx, two_meshes, schedule):
+    """Views of partly filled 8x8 tiles (2 pixels wide or tall, odd sides) in
+    one batch: the batch's ray list is built per tile, so every request must
+    still render as it does alone."""
+    ms, tfs = two_meshes
+    sizes = [(2, 64), (64, 2), (9, 9), (7, 300), (2, 2), (130, 3), (17, 33), (3, 129)]
+    r = WL.Rng(0x5EED0420, 9)
+    reqs = []
+    for i, (w, h) in enumerate(sizes * 2):
+        m = i % 2
+        reqs.append((m, V.perspective_view(w, h, tfs[m][i % 2], eye=2.5 * r.direction())))
+    print(f"server batch of {len(reqs)} thin views", flush=True)
+    sv = V.Server(ctx, ms, hbm_budget=256 << 20, streams=1, max_batch=len(reqs), schedule=schedule)
+    try:
+        tickets = [sv.submit(m, v, keep_image=True) for m, v in reqs]
+        got = [sv.wait(t, image=True) for t in tickets]
+    finally:
+        sv.close()
+    assert max(g["batch_size"] for g in got) > 1
+    for (m, v), g in zip(reqs, got):
+        alone, _ = ctx.render(ms[m], [v], mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+        assert g["status"] == 2
+        assert g["width"] == v.width and g["height"] == v.height
+        assert np.array_equal(g["rgba"], alone[0]["rgba"])
+
+
 def test_server_rejects_over_budget(ctx, two_meshes):
     ms, tfs = two_meshes
     sv = V.Server(ctx, ms, hbm_budget=64 << 20, streams=2, max_batch=4)
