@@ -369,7 +369,8 @@ __global__ __launch_bounds__(256) void k_raster_ref(const DevView* __restrict__ 
   uint32_t cpf = 4;
   while (cpf < 64u && (float)cpf < wexp) cpf <<= 1;
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t e = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64u / cpf) + lane / cpf;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); /* known uniform */
+  const uint32_t e = (blockIdx.x * (blockDim.x >> 6) + wave) * (64u / cpf) + lane / cpf;
   if (e >= nE) return;
   float4 fv[4];
   face_verts(fvid + views[vi].ext_off, pv + views[vi].vert_off, e, fv);
@@ -468,7 +469,8 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
    * faces per wave unless they are large */
   const bool spread = gridDim.y == 1 || 2ull * W * H >= (uint64_t)RASTER_BIG * nE;
   const uint32_t fpw = spread ? RASTER_FPW : 64u;
-  const uint32_t e0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * fpw;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); /* known uniform */
+  const uint32_t e0 = (blockIdx.x * (blockDim.x >> 6) + wave) * fpw;
   if (e0 >= nE) return; /* the whole wave */
   RasterFace f{};
   bool vis = false;
@@ -612,7 +614,10 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
                                                   uint64_t* __restrict__ tile_mask) {
   __shared__ uint32_t wave_n[COMPACT_WAVES];
   __shared__ uint32_t block_base;
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  /* the wave index through readfirstlane: the compiler then knows the tile,
+   * its view and the view's fields are the same on every lane, and searches
+   * and divides with scalar loads and scalar arithmetic */
+  const uint32_t lane = threadIdx.x & 63u, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t wt = blockIdx.x * COMPACT_WAVES + wave;
   bool active = false;
   uint32_t vi = 0, pix = 0;
@@ -706,7 +711,7 @@ __global__ __launch_bounds__(1024) void k_compact_emit(const DevView* __restrict
                                                        const uint32_t* __restrict__ block_off,
                                                        uint64_t* __restrict__ rays, uint32_t* __restrict__ n_rays,
                                                        uint32_t view_base) {
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63u, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t t0 = blockIdx.x * COMPACT_WAVES, wt = t0 + wave;
   if (wt >= ntiles) return;
   uint32_t before = block_off[blockIdx.x];
