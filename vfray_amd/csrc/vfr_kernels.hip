@@ -567,11 +567,13 @@ int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t ma
   }
 
 /* One wave per 8x8 tile, 16 tiles per 1024-thread block: pixels with an
- * entry become rays (wave ballot, a prefix over the block's waves in LDS and
- * one atomic per block, so the ray list keeps tile order within a block),
- * inactive pixels get zero outputs, and each active pixel's entry list is
- * sorted front to back (Batcher's 19-comparator network on 8 keys,
- * descending). */
+ * entry become rays, inactive pixels get zero outputs, and each active
+ * pixel's entry list is sorted front to back (Batcher's 19-comparator network
+ * on 8 keys, descending).  Pass 0 appends the block's rays with one atomic
+ * (wave ballot and a prefix over the block's waves in LDS: tile order within
+ * the block, blocks in the order they finish); pass 1 only counts, keeping
+ * each tile's mask and view and each block's total, for k_scan_excl and
+ * k_compact_emit to write every ray at its place in tile order. */
 #define COMPACT_WAVES 16u
 
 /* Tile lt of a view's tiles_x x tiles_y 8x8 tiles, in 4x4 groups of tiles
