@@ -1331,6 +1331,12 @@ template <int PROJ, bool BIG, int TFL, bool PSIT, bool LIN = false, bool LDSR = 
 DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, FtbRay& r) {
   const bool persp = PROJ == VFR_TRAV_MIXED ? ftb_persp(r) : PROJ == VFR_TRAV_PERSP;
   r.steps += 1u;
+  /* the colour at the entry point, read first (its index is then free for
+   * the exit point's).  A block whose rays span more views than the LDS
+   * slots reads the global table (the bit is the same on every lane of the
+   * block). */
+  const bool tglob = TFL == 2 && (__builtin_amdgcn_readfirstlane(r.bits) & FTB_TFGLOB);
+  const float4 cP = tglob ? ld32<float4>(a.tf, r.cpi * 16u) : (TFL ? tfl[r.cpi] : r.cP);
   const float4 P3 = rel(r.Q, r.ox, r.oy);
   const float t0 = edge_rel(P3, r.P0), t1 = edge_rel(P3, r.P1), t2 = edge_rel(P3, r.P2);
   /* candidate faces (P3, Pi, Pj) for (i,j) = (0,1), (1,2), (2,0), edges
@@ -1400,19 +1406,8 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
    * zeta = 1 and the update leaves colour and transparency unchanged (in
    * linear compositing, both terms vanish). */
   const uint32_t ci = r.tfo + (uint32_t)(255.0f * snext); /* snext is in [0, 1] */
-  /* a block whose rays span more views than the LDS slots reads the global
-   * table (the bit is the same on every lane of the block) */
-  const bool tglob = TFL == 2 && (__builtin_amdgcn_readfirstlane(r.bits) & FTB_TFGLOB);
-  float4 cN, cPt;
-  if (tglob) {
-    cN = ld32<float4>(a.tf, ci * 16u);
-    cPt = ld32<float4>(a.tf, r.cpi * 16u);
-  } else {
-    cN = tf_at<TFL>(a, tfl, ci);
-    cPt = TFL ? tfl[r.cpi] : r.cP;
-  }
+  const float4 cN = tglob ? ld32<float4>(a.tf, ci * 16u) : tf_at<TFL>(a, tfl, ci);
   const float dl = skip ? 0.0f : fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
-  const float4 cP = cPt;
   if (LIN) {
     /* the reference's linear integration (vfRayKernel.cu:738-806, as in
      * trace_ref) over a segment of dl object units, front to back */
