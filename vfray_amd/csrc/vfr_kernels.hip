@@ -1276,11 +1276,29 @@ DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pi
   if (!ftb_segment<TFL, REC>(a, tfl, r)) r.bits |= FTB_DEAD;
 }
 
-enum { FTB_GO = 0, FTB_DONE = 1, FTB_REENTER = 2 };
+/* After an exit through an external face: continue at the first entry
+ * strictly behind the exit point; false when there is none (the ray is
+ * finished). */
+template <int TFL, bool REC = false>
+DEV bool ftb_reenter(const TravArgs& a, const float4* tfl, FtbRay& r) {
+  const float exitZ = ftb_persp(r) ? 1.0f / r.prev : r.prev;
+  uint32_t cur = ftb_cur(r) + 1u;
+  const uint32_t nk = ftb_nk(r);
+  while (cur < nk && !(funkey((uint32_t)(a.entries[cur * a.estride + r.gp] >> 32)) < exitZ)) ++cur;
+  if (cur >= nk) return false;
+  r.bits = (r.bits & ~63u) | cur;
+  if (!ftb_segment<TFL, REC>(a, tfl, r)) { r.bits |= FTB_DEAD; return false; }
+  return true;
+}
 
-/* One tet of the walk on a live ray: FTB_GO, FTB_DONE (the ray is
- * finished) or FTB_REENTER (it left through an external face; ftb_reenter
- * decides).
+
+/* One tet of the walk on a live ray: true while the ray goes on, false
+ * when it is finished.  After an exit through an external face it continues
+ * at its next entry behind the exit point (ftb_reenter) within the step, so
+ * the step's result is one lane mask, not a status value per lane.  `it`
+ * is the step's number on the ray (1 for its first), stored in r.steps when
+ * the ray ends: callers whose lanes start together pass one counter for the
+ * whole wave, so no per-lane count is kept step by step.
  *
  * The body is one straight path that updates the ray state in place, so no
  * copies of the state are needed where paths join, and it is safe to run on
@@ -1331,9 +1349,8 @@ DEV void ldsr_exchange(const uint32_t* sperm, FtbRay& r) {
 }
 
 template <int PROJ, bool BIG, int TFL, bool PSIT, bool LIN = false, bool LDSR = false>
-DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, FtbRay& r) {
+DEV bool ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, FtbRay& r, uint32_t it) {
   const bool persp = PROJ == VFR_TRAV_MIXED ? ftb_persp(r) : PROJ == VFR_TRAV_PERSP;
-  r.steps += 1u;
   /* the colour at the entry point, read first (its index is then free for
    * the exit point's).  A block whose rays span more views than the LDS
    * slots reads the global table (the bit is the same on every lane of the
@@ -1373,7 +1390,7 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
     }
   }
   /* no exit face or the step bound: the step is not composited */
-  const bool skip = r.steps > VFR_MAX_STEPS || !(p1 || p2 || p3);
+  const bool skip = it > VFR_MAX_STEPS || !(p1 || p2 || p3);
   /* the exit face leaves out vertex k of the entry face; in the record it
    * is the face opposite k's slot s, face (s + 1) & 3 */
   const bool k0 = c2, k1 = !c1 && !c2, k2 = c1;
@@ -1511,26 +1528,12 @@ DEV uint32_t ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sper
   const bool term = LIN ? lstop : 1.0f - r.Tr >= a.term;
   if (skip || term || nb >= VFR_NB_BROKEN) {
     /* rare: the ray ends or leaves through the boundary */
-    if (skip) { r.bits |= FTB_DEAD; return FTB_DONE; }
-    if (term) return FTB_DONE;
-    if (nb == VFR_NB_BROKEN) { r.bits |= FTB_BROKEN; return FTB_DONE; }
-    return FTB_REENTER;
+    r.steps = it;
+    if (skip) { r.bits |= FTB_DEAD; return false; }
+    if (term) return false;
+    if (nb == VFR_NB_BROKEN) { r.bits |= FTB_BROKEN; return false; }
+    return ftb_reenter<TFL, LDSR>(a, tfl, r);
   }
-  return FTB_GO;
-}
-
-/* After an exit through an external face: continue at the first entry
- * strictly behind the exit point; false when there is none (the ray is
- * finished). */
-template <int TFL, bool REC = false>
-DEV bool ftb_reenter(const TravArgs& a, const float4* tfl, FtbRay& r) {
-  const float exitZ = ftb_persp(r) ? 1.0f / r.prev : r.prev;
-  uint32_t cur = ftb_cur(r) + 1u;
-  const uint32_t nk = ftb_nk(r);
-  while (cur < nk && !(funkey((uint32_t)(a.entries[cur * a.estride + r.gp] >> 32)) < exitZ)) ++cur;
-  if (cur >= nk) return false;
-  r.bits = (r.bits & ~63u) | cur;
-  if (!ftb_segment<TFL, REC>(a, tfl, r)) { r.bits |= FTB_DEAD; return false; }
   return true;
 }
 
@@ -1565,19 +1568,15 @@ DEV RayOut trace_ftb(const TravArgs& a, const float4* tfl, const uint32_t* sperm
     /* a lane whose ray has just ended takes part in one more exchange, so
      * that a record it loaded for other lanes reaches them */
     bool live = !(r.bits & FTB_DEAD);
+    uint32_t it = 0;
     while (true) {
       ldsr_exchange(sperm, r);
       if (!live) break;
-      const uint32_t st = ftb_step<PROJ, BIG, TFL, PSIT, LIN, true>(a, tfl, sperm, r);
-      if (st == FTB_GO) continue;
-      if (st == FTB_REENTER && ftb_reenter<TFL, true>(a, tfl, r)) continue;
-      live = false;
+      live = ftb_step<PROJ, BIG, TFL, PSIT, LIN, true>(a, tfl, sperm, r, ++it);
     }
   } else if (!(r.bits & FTB_DEAD)) {
-    while (true) {
-      const uint32_t st = ftb_step<PROJ, BIG, TFL, PSIT, LIN>(a, tfl, sperm, r);
-      if (st == FTB_GO || (st == FTB_REENTER && ftb_reenter<TFL>(a, tfl, r))) continue;
-      break;
+    uint32_t it = 0;
+    while (ftb_step<PROJ, BIG, TFL, PSIT, LIN>(a, tfl, sperm, r, ++it)) {
     }
   }
   return ftb_out<LIN>(a, r);
@@ -1783,8 +1782,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     /* every lane steps (an idle lane's stale ray stays inside the mesh
      * arrays and its result is ignored), so the walk is one straight path */
     for (uint32_t k = 0; k < burst; ++k) {
-      const uint32_t s = ftb_step<VFR_TRAV_MIXED, true, false, PSIT>(a, nullptr, sperm, r);
-      if (active && s != FTB_GO && !(s == FTB_REENTER && ftb_reenter<false>(a, nullptr, r))) {
+      /* an idle lane's stale ray may also try a re-entry: it reads only its
+       * own pixel's entry list, and its result is ignored */
+      const bool go = ftb_step<VFR_TRAV_MIXED, true, false, PSIT>(a, nullptr, sperm, r, ++r.steps);
+      if (active && !go) {
         const RayOut o = ftb_out(a, r);
         store_ray(a, r.gp, o);
         st += o.steps; an += o.anomaly; br += o.broken;
