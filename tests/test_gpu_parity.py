@@ -346,6 +346,36 @@ def test_ftb_perspective_32(ctx, psi):
     m.close()
 
 
+def test_ftb_rgba_bit_exact(ctx, psi):
+    """The front-to-back walk's colours equal the oracle's bit for bit, not
+    only within the tolerance: the kernel and the oracle perform the same
+    float operations in the same order (fused multiply-adds written out,
+    no contraction), so any change to the kernel's arithmetic that is not
+    mirrored in the oracle shows here.  A transfer function whose
+    neighbouring entries differ makes psi count in every segment; a concave
+    mesh adds re-entries; perspective and orthographic views."""
+    rng = np.random.default_rng(0x5EED010A)
+    tf = np.empty((256, 4), np.float32)
+    tf[:, :3] = rng.random((256, 3), dtype=np.float32)
+    tf[:, 3] = 0.6 * rng.random(256, dtype=np.float32)
+    for n, variant, seed in ((16, 0, 0x5EED010A), (12, 1, 0x5EED010B)):
+        inp = grid_inputs(n, seed, variant)
+        inp["tf"] = tf
+        m = ctx.mesh(inp["x"], inp["t"])
+        tid = ctx.tf(tf)
+        views = WL.sphere_views(seed, 3, [64, 96], tid)
+        views.append(V.ortho_view(80, 72, tid, matrix=_rot(0.3, -0.4)))
+        print(f"FTB bit-exact colours {n}^3 variant {variant}", flush=True)
+        outs, _ = ctx.render(m, views, mode=V.MODE_FTB)
+        for g, v in zip(outs, views):
+            orc = oracle_ftb(inp, v, psi)
+            assert (orc["entry_ext"] != V.NO_ID).sum() > 0
+            assert np.array_equal(g["entry_ext"], orc["entry_ext"])
+            assert np.array_equal(g["steps"], orc["steps"])
+            assert np.array_equal(g["rgba"].view(np.uint32), orc["rgba"].view(np.uint32))
+        m.close()
+
+
 def test_batched_thin_views(ctx, psi):
     """A batch of views whose 8x8 tiles are partly filled (2 pixels wide or
     tall, odd sides): the batched plan builds its ray list per tile, the
