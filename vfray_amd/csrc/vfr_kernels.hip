@@ -1082,7 +1082,6 @@ struct FtbRay {
 #define FTB_PERSP (1u << 12)  /* perspective view                        */
 #define FTB_DEAD (1u << 13)   /* ended as an anomaly (no exit face)      */
 #define FTB_BROKEN (1u << 14) /* ended at a neighbour with no back link  */
-#define FTB_TFGLOB (1u << 15) /* per-block TF staging: this block reads the global table */
 DEV uint32_t ftb_cur(const FtbRay& r) { return r.bits & 63u; }
 DEV uint32_t ftb_nk(const FtbRay& r) { return (r.bits >> 6) & 63u; }
 DEV bool ftb_persp(const FtbRay& r) { return (r.bits & FTB_PERSP) != 0u; }
@@ -1249,13 +1248,10 @@ DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pi
   r.voff = a.views[vi].vert_off;
   /* the ray's TF as a byte offset into the table it reads: the launch's LDS
    * slots, its block's LDS slots (one per view of the block, the block's
-   * first view at slot 0; the block record after the slots says which), or
+   * first view at slot 0, which the block record after the slots names), or
    * the global table */
-  uint32_t tfglob = 0;
   if constexpr (TFL == 2) {
-    const float4 m = tfl[VFR_TF_LDS_SLOTS * 256u];
-    tfglob = __float_as_uint(m.y) ? FTB_TFGLOB : 0u;
-    r.tfb = tfglob ? a.views[vi].tf * 4096u : (vi - __float_as_uint(m.x)) * 4096u;
+    r.tfb = (vi - __float_as_uint(tfl[VFR_TF_LDS_SLOTS * 256u].x)) * 4096u;
   } else {
     r.tfb = (a.views[vi].tf - (TFL ? a.tf_lds_base : 0u)) * 4096u;
   }
@@ -1269,7 +1265,7 @@ DEV void ftb_init(const TravArgs& a, const float4* tfl, uint32_t vi, uint32_t pi
   r.rf = LIN ? rayf : rayf * a.views[vi].inv_me;
   uint32_t nk = __builtin_nontemporal_load(a.cnt + r.gp);
   nk = nk > a.kmax ? a.kmax : nk;
-  r.bits = (nk << 6) | (persp ? FTB_PERSP : 0u) | tfglob;
+  r.bits = (nk << 6) | (persp ? FTB_PERSP : 0u);
   r.ar = r.ag = r.ab = 0.0f;
   r.Tr = LIN ? 0.0f : 1.0f;
   r.steps = 0;
@@ -1352,11 +1348,8 @@ template <int PROJ, bool BIG, int TFL, bool PSIT, bool LIN = false, bool LDSR = 
 DEV bool ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, FtbRay& r, uint32_t it) {
   const bool persp = PROJ == VFR_TRAV_MIXED ? ftb_persp(r) : PROJ == VFR_TRAV_PERSP;
   /* the colour at the entry point, read first (its index is then free for
-   * the exit point's).  A block whose rays span more views than the LDS
-   * slots reads the global table (the bit is the same on every lane of the
-   * block). */
-  const bool tglob = TFL == 2 && (__builtin_amdgcn_readfirstlane(r.bits) & FTB_TFGLOB);
-  const float4 cP = tglob ? ld32<float4>(a.tf, r.cpb) : (TFL ? tf_at<TFL>(a, tfl, r.cpb) : r.cP);
+   * the exit point's) */
+  const float4 cP = TFL ? tf_at<TFL>(a, tfl, r.cpb) : r.cP;
   const float4 P3 = rel(r.Q, r.ox, r.oy);
   const float t0 = edge_rel(P3, r.P0), t1 = edge_rel(P3, r.P1), t2 = edge_rel(P3, r.P2);
   /* candidate faces (P3, Pi, Pj) for (i,j) = (0,1), (1,2), (2,0), edges
@@ -1426,7 +1419,7 @@ DEV bool ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, F
    * zeta = 1 and the update leaves colour and transparency unchanged (in
    * linear compositing, both terms vanish). */
   const uint32_t cb = r.tfb + ((uint32_t)(255.0f * snext) << 4); /* snext is in [0, 1] */
-  const float4 cN = tglob ? ld32<float4>(a.tf, cb) : tf_at<TFL>(a, tfl, cb);
+  const float4 cN = tf_at<TFL>(a, tfl, cb);
   const float dl = skip ? 0.0f : fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
   if (LIN) {
     /* the reference's linear integration (vfRayKernel.cu:738-806, as in
@@ -1612,7 +1605,7 @@ template <uint32_t MODE, int PROJ, bool BIG, int TFL, bool PSIT, bool LDSR = fal
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VFR_TRAV_WAVES(MODE, LDSR),
                                                                      VFR_TRAV_WAVES(MODE, LDSR)))) void k_trav(
     TravArgs a) {
-  /* TF slots, then (per-block staging) the block record: first view, global */
+  /* TF slots, then (per-block staging) the block record: its first view */
   __shared__ float4 tfs[TFL == 2 ? VFR_TF_LDS_SLOTS * 256 + 1 : (TFL ? VFR_TF_LDS_SLOTS * 256 : 1)];
   __shared__ __align__(16) uint32_t sperm[LDSR ? LDSR_WORDS : VFR_N_SLOT_PERMS];
   constexpr bool ftb = MODE == VFR_MODE_FTB || MODE == VFR_MODE_FTB_LINEAR;
@@ -1631,6 +1624,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VFR_TRAV_WA
   const uint64_t ray = i < n ? __builtin_nontemporal_load(a.rays + i) : 0ull;
   const uint32_t vi = (uint32_t)(ray >> 32), pix = (uint32_t)ray;
   if (ftb) stage_slot_perms(sperm);
+  bool bglob = false;
   if constexpr (TFL == 2) {
     /* the TFs of the views the block's rays belong to, one LDS slot per view
      * from the least to the greatest (the list is grouped by view, so a
@@ -1649,6 +1643,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VFR_TRAV_WA
     hi = max(max(vhi[0], vhi[1]), max(vhi[2], vhi[3]));
     const uint32_t nv = lo <= hi ? hi - lo + 1u : 0u;
     const bool glob = nv > VFR_TF_LDS_SLOTS;
+    bglob = __builtin_amdgcn_readfirstlane(glob ? 1 : 0) != 0; /* the same on every lane */
     /* only the views some ray of the block has are read (view slots between
      * them may be unused, as in a server's shared view arena) */
     if (!glob && i < n) atomicOr(&vused, 1u << (vi - lo));
@@ -1658,14 +1653,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VFR_TRAV_WA
       for (uint32_t k = threadIdx.x; k < nv * 256u; k += blockDim.x)
         if (used & (1u << (k >> 8))) tfs[k] = a.tf[a.views[lo + (k >> 8)].tf * 256u + (k & 255u)];
     if (threadIdx.x == 0)
-      tfs[VFR_TF_LDS_SLOTS * 256] = make_float4(__uint_as_float(lo), __uint_as_float(glob ? 1u : 0u), 0.f, 0.f);
+      tfs[VFR_TF_LDS_SLOTS * 256] = make_float4(__uint_as_float(lo), 0.f, 0.f, 0.f);
   } else if constexpr (TFL == 1) {
     for (uint32_t k = threadIdx.x; k < a.tf_lds_n * 256u; k += blockDim.x) tfs[k] = a.tf[a.tf_lds_base * 256u + k];
   }
   if (TFL || ftb) __syncthreads();
   unsigned long long st = 0, an = 0, br = 0;
   if (i < n) {
-    const RayOut o = trace<MODE, PROJ, BIG, TFL, PSIT, LDSR>(a, tfs, sperm, vi, pix);
+    /* per-block staging: the block's choice picks one of two walks once, so
+     * no step asks which table it reads */
+    RayOut o;
+    if constexpr (TFL == 2)
+      o = bglob ? trace<MODE, PROJ, BIG, 0, PSIT, LDSR>(a, tfs, sperm, vi, pix)
+                : trace<MODE, PROJ, BIG, 2, PSIT, LDSR>(a, tfs, sperm, vi, pix);
+    else
+      o = trace<MODE, PROJ, BIG, TFL, PSIT, LDSR>(a, tfs, sperm, vi, pix);
     store_ray(a, a.views[vi].pix_off + pix, o);
     st = o.steps; an = o.anomaly; br = o.broken;
     if (o.broken && a.vflags) atomicOr(a.vflags + vi, VFR_VFLAG_BROKEN);
