@@ -148,6 +148,8 @@ struct TravArgs {
   unsigned long long* stats;
   uint32_t* queue;         /* persistent kernel work-queue head */
   float term;
+  float term_tr;           /* front to back: the largest T with 1 - T >= term in
+                              float arithmetic (set by vfr_k_traverse)         */
   uint32_t refill;         /* persistent: refill when this many lanes idle */
   uint32_t tf_lds_base;    /* one-ray-per-thread FTB: TF slots [base, base+n)   */
   uint32_t tf_lds_n;       /* staged in LDS (0: read the global table)         */

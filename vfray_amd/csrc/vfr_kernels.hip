@@ -24,6 +24,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 
 #include <vector>
 
@@ -1420,7 +1421,9 @@ DEV bool ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, F
    * linear compositing, both terms vanish). */
   const uint32_t cb = r.tfb + ((uint32_t)(255.0f * snext) << 4); /* snext is in [0, 1] */
   const float4 cN = tf_at<TFL>(a, tfl, cb);
-  const float dl = skip ? 0.0f : fmaxf(fabsf(zd - r.prev) * r.rf, 0.0f);
+  /* (the oracle's max(.., 0) changes nothing here: a step with an exit face
+   * has D > 0 and finite depths, and a skipped one is zeroed) */
+  const float dl = skip ? 0.0f : fabsf(zd - r.prev) * r.rf;
   if (LIN) {
     /* the reference's linear integration (vfRayKernel.cu:738-806, as in
      * trace_ref) over a segment of dl object units, front to back */
@@ -1518,7 +1521,9 @@ DEV bool ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, F
    * 799-803), opacity clamped to 1 */
   const bool lstop = LIN && a.composite == VFR_COMPOSITE_LINEAR_OPACITY && r.Tr > 1.0f;
   if (lstop) r.Tr = 1.0f;
-  const bool term = LIN ? lstop : 1.0f - r.Tr >= a.term;
+  /* 1 - Tr >= term, as one compare: 1 - T rounded to float never grows with
+   * T, so it holds exactly for T <= term_tr */
+  const bool term = LIN ? lstop : r.Tr <= a.term_tr;
   if (skip || term || nb >= VFR_NB_BROKEN) {
     /* rare: the ray ends or leaves through the boundary */
     r.steps = it;
@@ -1887,11 +1892,23 @@ static void launch_ftb(uint32_t proj, int tfl, dim3 grid, dim3 block, hipStream_
   else launch_ftb_t<PSIT, 0>(proj, grid, block, st, a);
 }
 
+/* The largest float T with 1 - T >= term in float arithmetic (NaN for a NaN
+ * term, which no T meets): the front-to-back walk's termination test as one
+ * compare of its transparency. */
+static float term_threshold(float term) {
+  if (term != term) return term;
+  float t = 1.0f - term;
+  while (t < 2.0f && 1.0f - std::nextafter(t, 4.0f) >= term) t = std::nextafter(t, 4.0f);
+  while (t > -2.0f && !(1.0f - t >= term)) t = std::nextafter(t, -4.0f);
+  return t;
+}
+
 int vfr_k_traverse(uint32_t mode, uint32_t proj, uint64_t n_records, uint32_t occupancy, const TravArgs& a0,
                    uint32_t max_rays, int kernel, uint32_t grid_waves, void* stream) {
   if (max_rays == 0) return VFR_OK;
   /* per-block TF staging is the one-ray-per-thread kernel's */
   TravArgs a = a0;
+  a.term_tr = term_threshold(a.term);
   if (kernel != VFR_WALK_THREAD) a.tf_block = 0;
   const bool lin = a.composite != VFR_COMPOSITE_PPI;
   if (kernel == VFR_WALK_QUEUE && mode == VFR_MODE_FTB && !lin) {
