@@ -1351,6 +1351,11 @@ DEV bool ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, F
   /* the colour at the entry point, read first (its index is then free for
    * the exit point's) */
   const float4 cP = TFL ? tf_at<TFL>(a, tfl, r.cpb) : r.cP;
+  /* Wave priority 2 from the fourth vertex's arrival to the link's request
+   * and from the link's arrival to the next vertex's request, the two short
+   * sequences each step's next load waits on; 0 for the rest (the
+   * compositing), which can wait for issue slots. */
+  __builtin_amdgcn_s_setprio(2);
   const float4 P3 = rel(r.Q, r.ox, r.oy);
   const float t0 = edge_rel(P3, r.P0), t1 = edge_rel(P3, r.P1), t2 = edge_rel(P3, r.P2);
   /* candidate faces (P3, Pi, Pj) for (i,j) = (0,1), (1,2), (2,0), edges
@@ -1405,6 +1410,7 @@ DEV bool ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, F
     nb = lk.x;
     qw = lk.y;
   }
+  __builtin_amdgcn_s_setprio(0);
   /* exit face (P3, Pi, Pj): P3 weighs o_ij, Pi weighs -t_j, Pj weighs t_i,
    * the left-out vertex Pk weighs 0 */
   const float w3 = pick3(c1, c2, r.o01, r.o12, r.o20);
@@ -1490,10 +1496,12 @@ DEV bool ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, F
    * fourth, and the map sends k's slot to that vertex's slot), then every
    * byte goes through the map.  Neighbour words index the record arena (a
    * server rebases each resident mesh's words when it copies them in). */
+  __builtin_amdgcn_s_setprio(2);
   const uint32_t i3n = qw & VFR_REC_VERT_MASK;
   const uint32_t smap = sperm[qw >> VFR_REC_VERT_BITS];
   const uint32_t tn = nb < VFR_NB_BROKEN ? (nb >> 2) : r.t;
   r.Q = ld32<float4>(a.vert, (r.voff + i3n) * 16u);
+  __builtin_amdgcn_s_setprio(0);
   const uint32_t swp = c1 ? 0x02030100u : (c2 ? 0x00020103u : 0x01020300u);
   r.sl = __builtin_amdgcn_perm(smap, smap, __builtin_amdgcn_perm(r.sl, r.sl, swp));
   if (LDSR) {
