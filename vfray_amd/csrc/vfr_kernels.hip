@@ -1366,7 +1366,31 @@ DEV bool ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, F
   const bool le0 = t0 <= 0.0f, le1 = t1 <= 0.0f, le2 = t2 <= 0.0f;
   bool p1 = ge0 && le1, p2 = ge1 && le2, p3 = ge2 && le0;
   bool c1 = p1, c2 = !p1 && p2;
+  /* the exit face's link: the neighbour word and the vertex word (the
+   * neighbour's fourth vertex with, above bit 26, the slot map across the
+   * face).  The exit face leaves out vertex k of the entry face; in the
+   * record it is the face opposite k's slot s, face (s + 1) & 3.  Requested
+   * at once for the face the common case picks, before the rare cases are
+   * even tested, and used at the end of the step. */
+  uint32_t nb, qw;
+  auto link = [&](bool f1, bool f2) {
+    const uint32_t sk = (r.sl >> (f1 ? 16u : (f2 ? 0u : 8u))) & 3u;
+    if (LDSR) {
+      const uint4 fa = r.fa, fb = r.fb;
+      const bool e0 = sk == 3u, e1 = sk == 0u, e2 = sk == 1u;
+      nb = pick3u(e0, e1, fa.x, fa.z, e2 ? fb.x : fb.z);
+      qw = pick3u(e0, e1, fa.y, fa.w, e2 ? fb.y : fb.w);
+    } else {
+      const uint32_t fx = (sk + 1u) & 3u;
+      const uint2 lk = BIG ? *reinterpret_cast<const uint2*>(&a.rec[r.t].f[2u * fx])
+                           : ld32<uint2>(a.rec, r.t * 32u + fx * 8u);
+      nb = lk.x;
+      qw = lk.y;
+    }
+  };
+  link(c1, c2);
   if ((p1 && p2) || (p2 && p3) || (p3 && p1)) {
+    const bool s1 = c1, s2 = c2;
     /* rare: two edge functions are zero (a candidate is dropped when its
      * third edge is degenerate too), or the pixel is on a projected edge.
      * Two zeros always leave two candidates (t_i = t_j = 0 makes p_ij true
@@ -1387,29 +1411,11 @@ DEV bool ftb_step(const TravArgs& a, const float4* tfl, const uint32_t* sperm, F
       c1 = p1 && z1 <= z2 && z1 <= z3;
       c2 = !c1 && p2 && z2 <= z3;
     }
+    if (c1 != s1 || c2 != s2) link(c1, c2); /* the face the rare case picked */
   }
   /* no exit face or the step bound: the step is not composited */
   const bool skip = it > VFR_MAX_STEPS || !(p1 || p2 || p3);
-  /* the exit face leaves out vertex k of the entry face; in the record it
-   * is the face opposite k's slot s, face (s + 1) & 3 */
   const bool k0 = c2, k1 = !c1 && !c2, k2 = c1;
-  const uint32_t sk = (r.sl >> (c1 ? 16u : (c2 ? 0u : 8u))) & 3u;
-  /* its link: the neighbour word and the vertex word (the neighbour's fourth
-   * vertex with, above bit 26, the slot map across the face); requested now
-   * and used at the end of the step */
-  uint32_t nb, qw;
-  if (LDSR) {
-    const uint4 fa = r.fa, fb = r.fb;
-    const bool e0 = sk == 3u, e1 = sk == 0u, e2 = sk == 1u;
-    nb = pick3u(e0, e1, fa.x, fa.z, e2 ? fb.x : fb.z);
-    qw = pick3u(e0, e1, fa.y, fa.w, e2 ? fb.y : fb.w);
-  } else {
-    const uint32_t fx = (sk + 1u) & 3u;
-    const uint2 lk = BIG ? *reinterpret_cast<const uint2*>(&a.rec[r.t].f[2u * fx])
-                         : ld32<uint2>(a.rec, r.t * 32u + fx * 8u);
-    nb = lk.x;
-    qw = lk.y;
-  }
   __builtin_amdgcn_s_setprio(0);
   /* exit face (P3, Pi, Pj): P3 weighs o_ij, Pi weighs -t_j, Pj weighs t_i,
    * the left-out vertex Pk weighs 0 */
