@@ -297,6 +297,10 @@ static int launch_batch(vfr_server* sv, Slot* s) {
       keep_pix += r->pixels;
     }
   }
+  if (tiles > s->cap_tiles) {
+    vfr_set_error("server batch: %llu tiles exceed the slot's %u", (unsigned long long)tiles, s->cap_tiles);
+    return VFR_EINVAL;
+  }
   hipStream_t st = s->stream;
   CHECK_HIP(hipMemcpyAsync(s->views, s->h_views, sizeof(DevView) * nv, hipMemcpyHostToDevice, st));
   CHECK_RC(vfr_k_clear_u32(s->cnt, pix, st));
@@ -306,10 +310,6 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   CHECK_RC(vfr_k_xform_views(s->views, 0, nv, max_verts, sv->vert, s->pverts, st));
   CHECK_RC(vfr_k_raster_ftb(s->views, 0, nv, max_faces, sv->fvid, s->pverts, s->cnt, s->entries, VFR_K_MAX,
                             s->cap_pix, s->stats, s->vflags, st));
-  if (tiles > s->cap_tiles) {
-    vfr_set_error("server batch: %llu tiles exceed the slot's %u", (unsigned long long)tiles, s->cap_tiles);
-    return VFR_EINVAL;
-  }
   /* rays in tile order (count pass, block scan, emit) */
   CHECK_RC(vfr_k_compact(s->views, nv, 0, (uint32_t)tiles, VFR_MODE_FTB, nullptr, s->cnt, s->entries, VFR_K_MAX,
                          s->cap_pix, s->rgba, s->steps, s->entry, s->rays, s->counters, st, 0, s->block_off,
