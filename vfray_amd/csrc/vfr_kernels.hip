@@ -1300,10 +1300,12 @@ DEV bool ftb_reenter(const TravArgs& a, const float4* tfl, FtbRay& r) {
  * The body is one straight path that updates the ray state in place, so no
  * copies of the state are needed where paths join, and it is safe to run on
  * a finished ray (every index it forms stays inside the mesh arrays; the
- * result is ignored).  The exit face's link is requested as soon as the
- * exit face is known; depth, scalar, the transfer-function colour, psi and
- * the compositing run while it travels; the next tet's fourth vertex is
- * requested last, and the next step waits for it. */
+ * result is ignored; a finished ray that tries a re-entry reads only its
+ * own pixel's entry list).  The exit face's link is requested as soon as the
+ * common case's exit face is known (the rare multi-candidate case requests
+ * it again if it picks another face); depth, scalar, the transfer-function
+ * colour, psi and the compositing run while it travels; the next tet's
+ * fourth vertex is requested last, and the next step waits for it. */
 /* PROJ: VFR_TRAV_*, the projection of every ray of the launch.  BIG: the
  * record arena reaches 4 GiB (VFR_BIG_RECORDS tets or more), so records are
  * addressed with 64-bit arithmetic instead of 32-bit byte offsets.  PSIT:
