@@ -486,6 +486,7 @@ static TravArgs trav_args(vfr_plan* p, uint64_t* rays, uint32_t* n_rays) {
   a.tv = m->tv4();
   a.vert = p->pverts;
   a.ext_int = m->d.ext_int;
+  a.fvid = m->fvid4();
   a.tf = p->ctx->tf;
   a.psi = p->ctx->psi;
   a.views = p->views;

@@ -133,6 +133,9 @@ struct TravArgs {
   const uint4* tv;         /* tet vertex ids, same indexing as rec            */
   const float4* vert;      /* P-space vertices of all views (+ view vert_off) */
   const uint32_t* ext_int; /* (internal tet, face) arena; + ext_off           */
+  const uint4* fvid;       /* per external face: its tet's vertex ids in slots
+                              f, f+1, f+2, f+3 (the face, then the vertex off
+                              it); same indexing as ext_int                  */
   const float4* tf;        /* [slots * 256] */
   const float* psi;        /* [512 * 512]   */
   const DevView* views;

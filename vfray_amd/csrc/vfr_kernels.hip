@@ -1205,9 +1205,11 @@ DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
   const uint32_t e = (uint32_t)(key & 0xFFFFFFFFull) + a.views[r.vi].ext_off;
   const float4* __restrict__ pv = a.vert + r.voff;
   const uint2 tf2 = reinterpret_cast<const uint2*>(a.ext_int)[e]; /* (tet, face) */
+  /* the face's vertex ids in slots fe, fe+1, fe+2 and the fourth in fe+3,
+   * read beside (tet, face) rather than after it from the tet's ids */
+  const uint4 fv = a.fvid[e];
   r.t = a.views[r.vi].rec_off + tf2.x;
   const uint32_t fe = tf2.y;
-  const uint4 rv = a.tv[r.t];
   if (REC) {
     r.fa = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[0]);
     r.fb = *reinterpret_cast<const uint4*>(&a.rec[r.t].f[4]);
@@ -1215,9 +1217,8 @@ DEV bool ftb_segment(const TravArgs& a, const float4* tfl, FtbRay& r) {
   /* face fe = record slots fe, fe+1, fe+2; the fourth vertex is slot fe+3 */
   uint32_t s1 = (fe + 1u) & 3u, s2 = (fe + 2u) & 3u;
   const uint32_t s3 = (fe + 3u) & 3u;
-  r.Q = pv[sel4(rv, s3)];
-  float4 P0 = rel(pv[sel4(rv, fe)], r.ox, r.oy), P1 = rel(pv[sel4(rv, s1)], r.ox, r.oy),
-         P2 = rel(pv[sel4(rv, s2)], r.ox, r.oy);
+  r.Q = pv[fv.w];
+  float4 P0 = rel(pv[fv.x], r.ox, r.oy), P1 = rel(pv[fv.y], r.ox, r.oy), P2 = rel(pv[fv.z], r.ox, r.oy);
   float o01 = edge_rel(P0, P1), o12 = edge_rel(P1, P2), o20 = edge_rel(P2, P0);
   if (o01 + o12 + o20 < 0.0f) {
     /* order (P0, P2, P1): every edge function flips sign exactly */

@@ -251,6 +251,7 @@ extern "C" int vfr_render_noalloc(vfr_ctx* c, const vfr_mesh* m, const vfr_view*
   a.tv = reinterpret_cast<const uint4*>(d->tv);
   a.vert = pverts;
   a.ext_int = d->ext_int;
+  a.fvid = reinterpret_cast<const uint4*>(d->fvid);
   a.tf = vfr_ctx_tf(c);
   a.psi = vfr_ctx_psi(c);
   a.views = dviews;

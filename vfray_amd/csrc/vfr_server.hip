@@ -320,6 +320,7 @@ static int launch_batch(vfr_server* sv, Slot* s) {
   a.tv = sv->tv;
   a.vert = s->pverts;
   a.ext_int = sv->ext;
+  a.fvid = sv->fvid;
   a.tf = vfr_ctx_tf(sv->ctx);
   a.psi = vfr_ctx_psi(sv->ctx);
   a.views = s->views;
@@ -529,6 +530,7 @@ static int pipe_walk(vfr_server* sv, WalkList* L, int k) {
   a.tv = sv->tv;
   a.vert = sv->p_pverts;
   a.ext_int = sv->ext;
+  a.fvid = sv->fvid;
   a.tf = vfr_ctx_tf(sv->ctx);
   a.psi = vfr_ctx_psi(sv->ctx);
   a.views = sv->p_views;
