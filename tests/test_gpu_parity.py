@@ -535,7 +535,10 @@ def test_autotune_keeps_results(ctx):
 @pytest.mark.parametrize("term", [0.99, 1.0])
 def test_ftb_concave_reentry(ctx, psi, term):
     """Voided, radially graded mesh: rays exit through interior boundaries and
-    re-enter; batched and per-view renders equal the oracle bit for bit."""
+    re-enter; batched and per-view renders equal the oracle bit for bit, and
+    so do the other walk kernels (lane-refill persistent, whose idle lanes
+    also step and may try a re-entry; wave work queue; LDS-staged records)
+    and the pipelined schedule."""
     inp = grid_inputs(20, 0x5EED0303, variant=1)
     m = ctx.mesh(inp["x"], inp["t"])
     tf = ctx.tf(inp["tf"])
@@ -545,6 +548,13 @@ def test_ftb_concave_reentry(ctx, psi, term):
     a, sa = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, term_alpha=term)
     b, sb = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW, term_alpha=term)
     assert sa["anomalies"] == 0
+    for kw in (dict(persistent=True), dict(persistent=True, waves_per_cu=4, refill=1), dict(wave_queue=True),
+               dict(lds_records=True), dict(pipelined=True)):
+        c, sc = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED, term_alpha=term, **kw)
+        assert sc["steps"] == sa["steps"] and sc["anomalies"] == 0, kw
+        for g, h in zip(a, c):
+            for key in ("rgba", "entry_ext", "steps"):
+                assert np.array_equal(g[key], h[key]), (kw, key)
     multi = 0
     for v, g, h in zip(views, a, b):
         orc = oracle_ftb(inp, v, psi, term=term, fp64=True)
