@@ -403,6 +403,32 @@ def test_batched_thin_views(ctx, psi):
     m.close()
 
 
+def test_raster_face_sizes(ctx, psi):
+    """Entry lists from faces of every size the raster splits its work by: a
+    3^3 grid at 512^2 (bounding boxes of thousands of pixels, rasterized by
+    the whole wave in 8x8 tiles), a 6^3 grid at 160^2 (a few hundred pixels
+    each, shared out over several wave iterations) and views of a few pixels
+    per face (16 and 64 faces per wave), in one launch and one per view,
+    against the oracle bit for bit."""
+    for n, seed, sides in ((3, 0x5EED010C, (512, 40)), (6, 0x5EED010D, (160, 24))):
+        inp = grid_inputs(n, seed)
+        m = ctx.mesh(inp["x"], inp["t"])
+        tf = ctx.tf(inp["tf"])
+        views = [WL.sphere_views(seed + k, 1, [s], tf)[0] for k, s in enumerate(sides)]
+        views.append(V.ortho_view(sides[0], sides[0] - 37, tf, matrix=_rot(0.7, 0.2)))
+        print(f"raster face sizes {n}^3 {sides}", flush=True)
+        a, _ = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_BATCHED)
+        b, _ = ctx.render(m, views, mode=V.MODE_FTB, schedule=V.SCHED_PER_VIEW)
+        for i, v in enumerate(views):
+            orc = oracle_ftb(inp, v, psi)
+            assert (orc["entry_ext"] != V.NO_ID).sum() > 0
+            for g in (a[i], b[i]):
+                assert np.array_equal(g["entry_ext"], orc["entry_ext"])
+                assert np.array_equal(g["steps"], orc["steps"])
+                assert np.array_equal(g["rgba"].view(np.uint32), orc["rgba"].view(np.uint32))
+        m.close()
+
+
 def test_batched_equals_per_view(ctx):
     """Each view rendered in a batch equals the same view rendered alone, with
     every walk kernel (one ray per thread with and without LDS-staged tet

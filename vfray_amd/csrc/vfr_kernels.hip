@@ -335,8 +335,6 @@ int vfr_k_copy(const void* src, void* dst, size_t bytes, void* stream) {
 
 /* ----------------------------------------------------------------- raster */
 
-/* Loads the ext face (internal tet, face) vertices in P space: slots
- * (f+0..3)&3, the last one the opposite vertex. */
 /* The four P-space vertices of external face e: slots f, f+1, f+2 and the
  * one off the face, from the face's vertex ids (fvid, built with the mesh). */
 DEV void face_verts(const uint4* fvid, const float4* pv, uint32_t e, float4 out[4]) {
@@ -419,13 +417,14 @@ int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t ma
  * faces are large (expected area, 2 W H / faces, of RASTER_BIG pixels or
  * more) or the launch has one view, so that a single view still fills the
  * GPU.  Lanes set their
- * face up (visibility, the inside test's face part, pixel bbox); a face
- * whose bbox has at most RASTER_SMALL pixels is rasterized by its own lane,
- * a larger one by the whole wave, 64 pixels at a time in 8x8 tiles of the
- * bbox, so one atomic instruction appends the face to up to 64 pixel lists. */
+ * face up (visibility, the inside test's face part, pixel bbox).  The bbox
+ * pixels of the wave's faces of up to RASTER_BAL pixels are then shared out
+ * evenly, 64 per wave iteration whatever the faces' sizes; a larger face is
+ * rasterized by the whole wave, 64 pixels at a time in 8x8 tiles of its
+ * bbox.  Either way one atomic instruction appends to up to 64 pixel lists. */
 #define RASTER_FPW 16u
 #define RASTER_BIG 8u
-#define RASTER_SMALL 4u
+#define RASTER_BAL 1024u
 
 struct RasterFace {
   uint32_t bx0, by0, bx1, by1;
@@ -513,13 +512,64 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
   }
   const uint32_t area = vis ? (f.bx1 - f.bx0 + 1u) * (f.by1 - f.by0 + 1u) : 0u;
   unsigned long long over = 0;
-  /* small faces: the lane's own loop (x outer, y inner over the bbox) */
-  if (vis && area <= RASTER_SMALL)
-    for (uint32_t px = f.bx0; px <= f.bx1; ++px)
-      for (uint32_t py = f.by0; py <= f.by1; ++py)
-        raster_pixel(f, px, py, W, H, off, e, cnt, entries, kmax, estride, over, vflag);
-  /* larger faces: the whole wave, one face at a time */
-  uint64_t todo = __ballot(vis && area > RASTER_SMALL);
+  /* Faces of up to RASTER_BAL pixels: the wave's (face, bbox pixel) pairs
+   * form one list, face by face and row by row within a bbox, and the wave
+   * takes 64 pairs at a time.  A lane finds its pair's face by binary
+   * search over the faces' running pixel counts and reads the face from LDS,
+   * where each lane has left its own. */
+  __shared__ uint32_t s_end[4][64];
+  __shared__ float4 s_face[4][64][4];
+  const uint32_t a = area <= RASTER_BAL ? area : 0u;
+  uint32_t end = a; /* inclusive running sum over the wave's lanes */
+#pragma unroll
+  for (uint32_t d = 1; d < 64u; d <<= 1) {
+    const uint32_t t = __shfl_up(end, d, 64);
+    if (lane >= d) end += t;
+  }
+  const uint32_t total = __builtin_amdgcn_readlane(end, 63);
+  if (total) {
+    const uint32_t bw = f.bx1 - f.bx0 + 1u;
+    s_end[wave][lane] = end;
+    s_face[wave][lane][0] = make_float4(__uint_as_float(f.bx0 | (f.sw ? 0x80000000u : 0u)),
+                                        __uint_as_float(f.by0), __uint_as_float(bw),
+                                        __uint_as_float(end - a));
+    s_face[wave][lane][1] = make_float4(1.0f / (float)bw, f.ax, f.ay, f.v01x);
+    s_face[wave][lane][2] = make_float4(f.v01y, f.v02x, f.v02y, f.cx);
+    s_face[wave][lane][3] = make_float4(f.zA, f.dB, f.dC, f.invD);
+    /* the wave's own LDS rows: the wave's LDS accesses complete in order */
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t base = 0; base < total; base += 64u) {
+      const uint32_t item = base + lane;
+      if (item >= total) break;
+      /* j = number of faces whose pairs end at or before item */
+      uint32_t j = 0;
+#pragma unroll
+      for (uint32_t st = 32; st; st >>= 1)
+        if (s_end[wave][j + st - 1u] <= item) j += st;
+      const float4 q0 = s_face[wave][j][0], q1 = s_face[wave][j][1];
+      const float4 q2 = s_face[wave][j][2], q3 = s_face[wave][j][3];
+      RasterFace g;
+      const uint32_t x0 = __float_as_uint(q0.x);
+      g.sw = (x0 >> 31) != 0u;
+      const uint32_t gbw = __float_as_uint(q0.z);
+      g.ax = q1.y; g.ay = q1.z; g.v01x = q1.w;
+      g.v01y = q2.x; g.v02x = q2.y; g.v02y = q2.z; g.cx = q2.w;
+      g.zA = q3.x; g.dB = q3.y; g.dC = q3.z; g.invD = q3.w;
+      /* row and column of the pair in the bbox: the float quotient of two
+       * integers below 2^11 is within one of the exact one */
+      const uint32_t k = item - __float_as_uint(q0.w);
+      uint32_t row = (uint32_t)((float)k * q1.x);
+      int32_t col = (int32_t)(k - row * gbw);
+      if (col < 0) { --row; col += (int32_t)gbw; }
+      else if (col >= (int32_t)gbw) { ++row; col -= (int32_t)gbw; }
+      raster_pixel(g, (x0 & 0x7FFFFFFFu) + (uint32_t)col, __float_as_uint(q0.y) + row, W, H, off,
+                   e0 + j, cnt, entries, kmax, estride, over, vflag);
+    }
+  }
+  /* larger faces: the whole wave, one face at a time, in 8x8 tiles */
+  uint64_t todo = __ballot(vis && area > RASTER_BAL);
   while (todo) {
     const int j = __ffsll((unsigned long long)todo) - 1;
     todo &= todo - 1ull;
