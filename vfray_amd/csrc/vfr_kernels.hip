@@ -418,13 +418,20 @@ int vfr_k_raster_ref(const DevView* views, uint32_t v0, uint32_t nv, uint32_t ma
  * more) or the launch has one view, so that a single view still fills the
  * GPU.  Lanes set their
  * face up (visibility, the inside test's face part, pixel bbox).  The bbox
- * pixels of the wave's faces of up to RASTER_BAL pixels are then shared out
+ * pixels of the wave's faces of up to RASTER_BAL pixels (RASTER_BAL_ONE in
+ * a one-view launch) are then shared out
  * evenly, 64 per wave iteration whatever the faces' sizes; a larger face is
  * rasterized by the whole wave, 64 pixels at a time in 8x8 tiles of its
  * bbox.  Either way one atomic instruction appends to up to 64 pixel lists. */
 #define RASTER_FPW 16u
 #define RASTER_BIG 8u
+/* bbox pixels up to which a face goes to the shared list: in a launch of
+ * several views, and in a one-view launch, whose few waves cannot hide the
+ * list's search (six dependent LDS reads per iteration) and whose faces of
+ * more than 64 pixels keep the tile loop's lanes busy (one 1024^2 view of
+ * the 64^3 mesh: 0.17 ms with 1024, 0.12 ms with 64) */
 #define RASTER_BAL 1024u
+#define RASTER_BAL_ONE 64u
 
 struct RasterFace {
   uint32_t bx0, by0, bx1, by1;
@@ -512,14 +519,15 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
   }
   const uint32_t area = vis ? (f.bx1 - f.bx0 + 1u) * (f.by1 - f.by0 + 1u) : 0u;
   unsigned long long over = 0;
-  /* Faces of up to RASTER_BAL pixels: the wave's (face, bbox pixel) pairs
+  /* Faces of up to bal pixels: the wave's (face, bbox pixel) pairs
    * form one list, face by face and row by row within a bbox, and the wave
    * takes 64 pairs at a time.  A lane finds its pair's face by binary
    * search over the faces' running pixel counts and reads the face from LDS,
    * where each lane has left its own. */
   __shared__ uint32_t s_end[4][64];
   __shared__ float4 s_face[4][64][4];
-  const uint32_t a = area <= RASTER_BAL ? area : 0u;
+  const uint32_t bal = gridDim.y == 1 ? RASTER_BAL_ONE : RASTER_BAL;
+  const uint32_t a = area <= bal ? area : 0u;
   uint32_t end = a; /* inclusive running sum over the wave's lanes */
 #pragma unroll
   for (uint32_t d = 1; d < 64u; d <<= 1) {
@@ -569,7 +577,7 @@ __global__ __launch_bounds__(256) void k_raster_ftb(const DevView* __restrict__ 
     }
   }
   /* larger faces: the whole wave, one face at a time, in 8x8 tiles */
-  uint64_t todo = __ballot(vis && area > RASTER_BAL);
+  uint64_t todo = __ballot(vis && area > bal);
   while (todo) {
     const int j = __ffsll((unsigned long long)todo) - 1;
     todo &= todo - 1ull;
