@@ -625,7 +625,8 @@ int vfr_k_raster_ftb(const DevView* views, uint32_t v0, uint32_t nv, uint32_t ma
     e[b] = x_ > y_ ? y_ : x_;            \
   }
 
-/* One wave per 8x8 tile, 16 tiles per 1024-thread block: pixels with an
+/* TPW 8x8 tiles per wave, 16 tiles per block (one tile per wave for pass 0,
+ * two for pass 1, which is limited by the loads a CU has in flight): pixels with an
  * entry become rays, inactive pixels get zero outputs, and each active
  * pixel's entry list is sorted front to back (Batcher's 19-comparator network
  * on 8 keys, descending).  Pass 0 appends the block's rays with one atomic
@@ -661,6 +662,7 @@ DEV void tile_xy(uint32_t lt, uint32_t tiles_x, uint32_t tiles_y, uint32_t* tx, 
   *tx = lt % tiles_x;
   *ty = gy + lt / tiles_x;
 }
+template <uint32_t TPW>
 __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ views,
                                                   uint32_t n_views, uint32_t tile0, uint32_t ntiles,
                                                   uint32_t mode, const uint32_t* __restrict__ owner,
@@ -679,50 +681,62 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
    * its view and the view's fields are the same on every lane, and searches
    * and divides with scalar loads and scalar arithmetic */
   const uint32_t lane = threadIdx.x & 63u, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t wt = blockIdx.x * COMPACT_WAVES + wave;
-  bool active = false;
-  uint32_t vi = 0, pix = 0;
-  if (wt < ntiles) {
-    const uint32_t gt = tile0 + wt;
-    /* view owning tile gt: last view with tile_off <= gt */
-    uint32_t lo = 0, hi = n_views;
-    while (hi - lo > 1) {
-      uint32_t mid = (lo + hi) >> 1;
-      if (views[mid].tile_off <= gt) lo = mid; else hi = mid;
+  /* lanes in Z order over the tile: lanes 4k..4k+3 are a 2x2 quad, 8k..8k+7
+   * a 4x2 block (the lane groups a gather's addresses are looked up in) */
+  const uint32_t lx = (lane & 1u) | ((lane >> 1) & 2u) | ((lane >> 2) & 4u);
+  const uint32_t ly = ((lane >> 1) & 1u) | ((lane >> 2) & 2u) | ((lane >> 3) & 4u);
+  /* the wave's TPW tiles: every pixel count is requested before any is used,
+   * so that a wave has TPW loads in flight */
+  uint32_t vi[TPW], pix[TPW], gp[TPW], nk[TPW];
+  bool valid[TPW];
+#pragma unroll
+  for (uint32_t t = 0; t < TPW; ++t) {
+    const uint32_t wt = blockIdx.x * COMPACT_WAVES + wave * TPW + t;
+    vi[t] = 0; pix[t] = 0; gp[t] = 0; nk[t] = 0; valid[t] = false;
+    if (wt < ntiles) {
+      const uint32_t gt = tile0 + wt;
+      /* view owning tile gt: last view with tile_off <= gt */
+      uint32_t lo = 0, hi = n_views;
+      while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (views[mid].tile_off <= gt) lo = mid; else hi = mid;
+      }
+      vi[t] = lo;
+      const uint32_t W = views[lo].W, H = views[lo].H;
+      uint32_t tx, ty;
+      tile_xy(gt - views[lo].tile_off, views[lo].tiles_x, (H + 7u) / 8u, &tx, &ty);
+      const uint32_t px = tx * 8u + lx, py = ty * 8u + ly;
+      valid[t] = px < W && py < H;
+      pix[t] = py * W + px;
+      gp[t] = views[lo].pix_off + pix[t];
+      if (valid[t]) nk[t] = mode == VFR_MODE_REF ? owner[gp[t]] : cnt[gp[t]];
     }
-    vi = lo;
-    const uint32_t W = views[vi].W, H = views[vi].H;
-    uint32_t tx, ty;
-    tile_xy(gt - views[vi].tile_off, views[vi].tiles_x, (H + 7u) / 8u, &tx, &ty);
-    /* lanes in Z order over the tile: lanes 4k..4k+3 are a 2x2 quad, 8k..8k+7
-     * a 4x2 block (the lane groups a gather's addresses are looked up in) */
-    const uint32_t lx = (lane & 1u) | ((lane >> 1) & 2u) | ((lane >> 2) & 4u);
-    const uint32_t ly = ((lane >> 1) & 1u) | ((lane >> 2) & 2u) | ((lane >> 3) & 4u);
-    const uint32_t px = tx * 8u + lx, py = ty * 8u + ly;
-    const bool valid = px < W && py < H;
-    pix = py * W + px;
-    const uint32_t gp = views[vi].pix_off + pix;
-    if (valid) {
-      if (mode == VFR_MODE_REF) {
-        active = owner[gp] != 0u;
-      } else {
-        uint32_t nk = cnt[gp];
-        active = nk != 0u;
+  }
+  uint32_t n_act = 0;
+  bool active0 = false;
+  uint64_t m0 = 0;
+#pragma unroll
+  for (uint32_t t = 0; t < TPW; ++t) {
+    const uint32_t wt = blockIdx.x * COMPACT_WAVES + wave * TPW + t;
+    bool active = false;
+    if (valid[t]) {
+      active = nk[t] != 0u;
+      if (mode != VFR_MODE_REF) {
         /* the pixel's slots, one per slot plane */
-        uint64_t* ent = entries + gp;
-        nk = nk > kmax ? kmax : nk;
-        if (nk > 1u && nk <= 8u) {
+        uint64_t* ent = entries + gp[t];
+        const uint32_t n = nk[t] > kmax ? kmax : nk[t];
+        if (n > 1u && n <= 8u) {
           uint64_t e[8];
 #pragma unroll
-          for (uint32_t k = 0; k < 8; ++k) e[k] = k < nk ? ent[k * estride] : 0ull;
+          for (uint32_t k = 0; k < 8; ++k) e[k] = k < n ? ent[k * estride] : 0ull;
           CS(0, 1) CS(2, 3) CS(4, 5) CS(6, 7) CS(0, 2) CS(1, 3) CS(4, 6) CS(5, 7) CS(1, 2) CS(5, 6)
           CS(0, 4) CS(1, 5) CS(2, 6) CS(3, 7) CS(2, 4) CS(3, 5) CS(1, 2) CS(3, 4) CS(5, 6)
 #pragma unroll
           for (uint32_t k = 0; k < 8; ++k)
-            if (k < nk) ent[k * estride] = e[k];
-        } else if (nk > 8u) {
+            if (k < n) ent[k * estride] = e[k];
+        } else if (n > 8u) {
           /* rare long list: insertion sort in place, descending */
-          for (uint32_t k = 1; k < nk; ++k) {
+          for (uint32_t k = 1; k < n; ++k) {
             const uint64_t key = ent[k * estride];
             uint32_t j = k;
             while (j > 0 && ent[(j - 1) * estride] < key) { ent[j * estride] = ent[(j - 1) * estride]; --j; }
@@ -731,34 +745,37 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
         }
       }
       if (!active) {
-        rgba[gp] = make_float4(0.f, 0.f, 0.f, 0.f);
-        steps[gp] = 0u;
-        entry_out[gp] = VFR_NO_ID;
+        rgba[gp[t]] = make_float4(0.f, 0.f, 0.f, 0.f);
+        steps[gp[t]] = 0u;
+        entry_out[gp[t]] = VFR_NO_ID;
       }
     }
+    const uint64_t m = __ballot(active);
+    /* count pass: the tile's active mask, and its view in the ntiles words
+     * after the masks (the emit pass reads both instead of searching the
+     * views again) */
+    if (pass == 1 && lane == 0 && wt < ntiles) {
+      tile_mask[wt] = m;
+      reinterpret_cast<uint32_t*>(tile_mask + ntiles)[wt] = vi[t];
+    }
+    n_act += (uint32_t)__popcll(m);
+    if (t == 0) { active0 = active; m0 = m; }
   }
-  const uint64_t m = __ballot(active);
-  /* count pass: the tile's active mask, and its view in the ntiles words
-   * after the masks (the emit pass reads both instead of searching the views
-   * again) */
-  if (pass == 1 && lane == 0 && wt < ntiles) {
-    tile_mask[wt] = m;
-    reinterpret_cast<uint32_t*>(tile_mask + ntiles)[wt] = vi;
-  }
-  if (lane == 0) wave_n[wave] = (uint32_t)__popcll(m);
+  if (lane == 0) wave_n[wave] = n_act;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t tot = 0;
-    for (uint32_t w = 0; w < COMPACT_WAVES; ++w) tot += wave_n[w];
+    for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) tot += wave_n[w];
     if (pass == 1) block_off[blockIdx.x] = tot;
     else block_base = tot ? atomicAdd(n_rays, tot) : 0u;
   }
   __syncthreads();
-  if (active && pass == 0) {
+  /* pass 0 (one tile per wave): the rays at the block's place in the list */
+  if (TPW == 1 && active0 && pass == 0) {
     uint32_t before = block_base;
     for (uint32_t w = 0; w < wave; ++w) before += wave_n[w];
-    before += (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    rays[before] = ((uint64_t)(vi + view_base) << 32) | pix;
+    before += (uint32_t)__popcll(m0 & ((1ull << lane) - 1ull));
+    rays[before] = ((uint64_t)(vi[0] + view_base) << 32) | pix[0];
   }
 }
 
@@ -826,14 +843,16 @@ int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, 
   const uint32_t nb = vfr_compact_blocks(ntiles);
   hipStream_t st = (hipStream_t)stream;
   if (!block_off) {
-    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(1024), 0, st, views, n_views_total, tile0, ntiles, mode, owner,
-                       cnt, entries, kmax, estride, rgba, steps, entry_out, rays, n_rays, view_base, 0u,
+    hipLaunchKernelGGL(k_compact<1>, dim3(nb), dim3(1024), 0, st, views, n_views_total, tile0, ntiles, mode,
+                       owner, cnt, entries, kmax, estride, rgba, steps, entry_out, rays, n_rays, view_base, 0u,
                        (uint32_t*)nullptr, (uint64_t*)nullptr);
   } else {
     /* rays in tile order: count per block, scan, then write at the offsets */
-    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(1024), 0, st, views, n_views_total, tile0, ntiles, mode, owner,
-                       cnt, entries, kmax, estride, rgba, steps, entry_out, rays, n_rays, view_base, 1u, block_off,
-                       tile_mask);
+    /* the count pass: two tiles per wave, eight waves per block (the
+     * blocks cover the same 16 tiles as the emit pass's) */
+    hipLaunchKernelGGL(k_compact<2>, dim3(nb), dim3(COMPACT_WAVES / 2 * 64), 0, st, views, n_views_total, tile0,
+                       ntiles, mode, owner, cnt, entries, kmax, estride, rgba, steps, entry_out, rays, n_rays,
+                       view_base, 1u, block_off, tile_mask);
     hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, block_off, nb);
     hipLaunchKernelGGL(k_compact_emit, dim3(nb), dim3(1024), 0, st, views, n_views_total, tile0, ntiles, tile_mask,
                        block_off, rays, n_rays, view_base);
