@@ -779,8 +779,8 @@ __global__ __launch_bounds__(1024) void k_compact(const DevView* __restrict__ vi
   }
 }
 
-/* Second pass of the tile-ordered compaction: one wave per tile, as the
- * first pass, writing the rays of its tile's active mask after those of the
+/* Second pass of the tile-ordered compaction: two tiles per wave, as the
+ * first pass, writing the rays of each tile's active mask after those of the
  * tiles before it (the block's offset from the scan, then the masks of the
  * block's earlier tiles). */
 __global__ __launch_bounds__(1024) void k_compact_emit(const DevView* __restrict__ views, uint32_t n_views,
@@ -789,23 +789,38 @@ __global__ __launch_bounds__(1024) void k_compact_emit(const DevView* __restrict
                                                        const uint32_t* __restrict__ block_off,
                                                        uint64_t* __restrict__ rays, uint32_t* __restrict__ n_rays,
                                                        uint32_t view_base) {
+  /* two tiles per wave, eight waves per block: each wave's loads (the
+   * masks before its tiles, then its tiles' views) are one dependent round
+   * for both tiles */
   const uint32_t lane = threadIdx.x & 63u, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t t0 = blockIdx.x * COMPACT_WAVES, wt = t0 + wave;
-  if (wt >= ntiles) return;
+  const uint32_t t0 = blockIdx.x * COMPACT_WAVES, wa = t0 + 2u * wave;
+  if (wa >= ntiles) return;
+  const bool hb = wa + 1u < ntiles;
   uint32_t before = block_off[blockIdx.x];
-  for (uint32_t w = 0; w < wave; ++w) before += (uint32_t)__popcll(tile_mask[t0 + w]);
-  const uint64_t m = tile_mask[wt];
-  if (wt == ntiles - 1 && lane == 0) *n_rays = before + (uint32_t)__popcll(m);
-  if (!((m >> lane) & 1ull)) return;
-  const uint32_t gt = tile0 + wt;
-  const uint32_t lo = reinterpret_cast<const uint32_t*>(tile_mask + ntiles)[wt];
-  const uint32_t W = views[lo].W, H = views[lo].H;
-  uint32_t tx, ty;
-  tile_xy(gt - views[lo].tile_off, views[lo].tiles_x, (H + 7u) / 8u, &tx, &ty);
+  for (uint32_t u = t0; u < wa; ++u) before += (uint32_t)__popcll(tile_mask[u]);
+  const uint32_t* const tview = reinterpret_cast<const uint32_t*>(tile_mask + ntiles);
+  const uint64_t ma = tile_mask[wa], mb = hb ? tile_mask[wa + 1u] : 0ull;
+  const uint32_t la = tview[wa], lb = hb ? tview[wa + 1u] : la;
+  const uint32_t before_b = before + (uint32_t)__popcll(ma);
+  if (lane == 0) {
+    if (wa == ntiles - 1u) *n_rays = before_b;
+    else if (wa + 1u == ntiles - 1u) *n_rays = before_b + (uint32_t)__popcll(mb);
+  }
+  const uint32_t Wa = views[la].W, Ha = views[la].H, Wb = views[lb].W, Hb = views[lb].H;
   const uint32_t lx = (lane & 1u) | ((lane >> 1) & 2u) | ((lane >> 2) & 4u);
   const uint32_t ly = ((lane >> 1) & 1u) | ((lane >> 2) & 2u) | ((lane >> 3) & 4u);
-  const uint32_t pix = (ty * 8u + ly) * W + tx * 8u + lx;
-  rays[before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = ((uint64_t)(lo + view_base) << 32) | pix;
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t tx, ty;
+  if ((ma >> lane) & 1ull) {
+    tile_xy(tile0 + wa - views[la].tile_off, views[la].tiles_x, (Ha + 7u) / 8u, &tx, &ty);
+    const uint32_t pix = (ty * 8u + ly) * Wa + tx * 8u + lx;
+    rays[before + (uint32_t)__popcll(ma & below)] = ((uint64_t)(la + view_base) << 32) | pix;
+  }
+  if ((mb >> lane) & 1ull) {
+    tile_xy(tile0 + wa + 1u - views[lb].tile_off, views[lb].tiles_x, (Hb + 7u) / 8u, &tx, &ty);
+    const uint32_t pix = (ty * 8u + ly) * Wb + tx * 8u + lx;
+    rays[before_b + (uint32_t)__popcll(mb & below)] = ((uint64_t)(lb + view_base) << 32) | pix;
+  }
 }
 
 /* In-place exclusive prefix sum of v[0, n) by one block: each thread sums a
@@ -854,8 +869,8 @@ int vfr_k_compact(const DevView* views, uint32_t n_views_total, uint32_t tile0, 
                        ntiles, mode, owner, cnt, entries, kmax, estride, rgba, steps, entry_out, rays, n_rays,
                        view_base, 1u, block_off, tile_mask);
     hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, block_off, nb);
-    hipLaunchKernelGGL(k_compact_emit, dim3(nb), dim3(1024), 0, st, views, n_views_total, tile0, ntiles, tile_mask,
-                       block_off, rays, n_rays, view_base);
+    hipLaunchKernelGGL(k_compact_emit, dim3(nb), dim3(COMPACT_WAVES / 2 * 64), 0, st, views, n_views_total, tile0,
+                       ntiles, tile_mask, block_off, rays, n_rays, view_base);
   }
   CHECK_HIP(hipGetLastError());
   return VFR_OK;
